@@ -1,0 +1,585 @@
+// api.hip -- C ABI of libdnlg.so: contexts, jobs, batched solve, read-back.
+//
+// Host C++ replaces the reference's Model/Explicit objects for the explicit
+// solve: Model::initSolve (Model.C:360-521) becomes dnlg_job_create,
+// Solver::solve / Explicit::solve (Solver.h:70, Explicit.C:126-253) becomes
+// dnlg_job_solve / dnlg_batch_solve.  All per-cycle work runs on one HIP
+// stream per context; the host only checks progress between chunks of
+// cycles.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/dnlg.h"
+#include "batch.h"
+#include "kernels.h"
+
+using namespace dnlg;
+
+namespace {
+
+int g_tables_uploaded[64] = {0};
+std::mutex g_tables_mu;
+
+// Gauss-point shape tables: closed forms of ElQua4NAx.C:63-110 and
+// ElHex8N3D.C:93-231 (products of (1 +- 1/sqrt3) factors).
+void make_tables(double hexN[64], double hexD[192], double quaN[16], double quaD[32]) {
+  const int hs[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                        {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+  const double hN[4] = {(9.0 - 5.0 * std::sqrt(3.0)) / 36, (3 - std::sqrt(3.0)) / 36, (3 + std::sqrt(3.0)) / 36,
+                        (9.0 + 5.0 * std::sqrt(3.0)) / 36};
+  const double hD[3] = {(2.0 - std::sqrt(3.0)) / 12, 1.0 / 12, (2.0 + std::sqrt(3.0)) / 12};
+  for (int p = 0; p < 8; p++)
+    for (int a = 0; a < 8; a++) {
+      int k = 0;
+      for (int c = 0; c < 3; c++) k += (hs[a][c] == hs[p][c]);
+      hexN[p * 8 + a] = hN[k];
+      for (int c = 0; c < 3; c++) {
+        int kk = 0;
+        for (int o = 0; o < 3; o++)
+          if (o != c) kk += (hs[a][o] == hs[p][o]);
+        hexD[(p * 8 + a) * 3 + c] = hs[a][c] > 0 ? hD[kk] : -hD[kk];
+      }
+    }
+  const int qs[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+  const double qN[3] = {(2.0 - std::sqrt(3.0)) / 6, 1.0 / 6, (2.0 + std::sqrt(3.0)) / 6};
+  const double qD[2] = {(3.0 - std::sqrt(3.0)) / 12, (3.0 + std::sqrt(3.0)) / 12};
+  for (int p = 0; p < 4; p++)
+    for (int a = 0; a < 4; a++) {
+      int k = 0;
+      for (int c = 0; c < 2; c++) k += (qs[a][c] == qs[p][c]);
+      quaN[p * 4 + a] = qN[k];
+      for (int c = 0; c < 2; c++) {
+        const int o = 1 - c;
+        const double v = qD[qs[a][o] == qs[p][o] ? 1 : 0];
+        quaD[(p * 4 + a) * 2 + c] = qs[a][c] > 0 ? v : -v;
+      }
+    }
+}
+
+int fail(dnlg_ctx *ctx, int st, const std::string &msg) {
+  if (ctx) ctx->err = msg;
+  return st;
+}
+
+#define HIPCHK(ctx, expr)                                                                  \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess) {                                                                \
+      return fail((ctx), e_ == hipErrorOutOfMemory ? DNLG_E_OOM : DNLG_E_HIP,              \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));                      \
+    }                                                                                      \
+  } while (0)
+
+template <typename T>
+int dalloc(dnlg_job *job, T **p, int64_t count, const T *init = nullptr) {
+  *p = nullptr;
+  if (count <= 0) return DNLG_OK;
+  void *ptr = nullptr;
+  HIPCHK(job->ctx, hipMalloc(&ptr, sizeof(T) * count));
+  job->allocs.push_back(ptr);
+  *p = (T *)ptr;
+  if (init) {
+    HIPCHK(job->ctx, hipMemcpy(ptr, init, sizeof(T) * count, hipMemcpyHostToDevice));
+  } else {
+    HIPCHK(job->ctx, hipMemset(ptr, 0, sizeof(T) * count));
+  }
+  return DNLG_OK;
+}
+
+int64_t footprint(int type, int64_t N, int64_t NE, int prec, int track) {
+  const int dim = type == DNLG_EL_HEX8N_3D ? 3 : 2;
+  const int nn = type == DNLG_EL_HEX8N_3D ? 8 : 4;
+  const int64_t G = NE * nn;
+  int64_t b = 0;
+  b += N * dim * 8 * 8;            // x, disp, fu/fv/fa x2
+  b += N * 8;                      // mass
+  b += (N + 1) * 4 * 2 + NE * nn * 4 * 2;  // CSRs, connectivity
+  b += NE * nn * dim * 8;          // elemF
+  if (prec == DNLG_PREC_MIXED) {
+    b += NE * nn * dim * 4;        // elemF32
+    b += G * (6 * 4 + 4 * 4 + 4 * 8 + 8 + 8 + 4);
+  } else {
+    b += G * (6 * 8 + 8 * 8 + 8 + 8);
+  }
+  if (track) b += G * 12 * 8;
+  b += ((NE + 255) / 256) * 8 + sizeof(JobDev) + 4096;
+  return b;
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+int dnlg_abi_version(void) { return DNLG_ABI_VERSION; }
+
+int dnlg_ctx_create(int device, dnlg_ctx **out) {
+  if (!out) return DNLG_E_INVALID;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return DNLG_E_HIP;
+  if (device < 0 || device >= ndev) return DNLG_E_INVALID;
+  if (hipSetDevice(device) != hipSuccess) return DNLG_E_HIP;
+  dnlg_ctx *ctx = new dnlg_ctx();
+  ctx->device = device;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return DNLG_E_HIP;
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_tables_mu);
+    if (!g_tables_uploaded[device]) {
+      double hexN[64], hexD[192], quaN[16], quaD[32];
+      make_tables(hexN, hexD, quaN, quaD);
+      upload_tables(hexN, hexD, quaN, quaD);
+      if (hipDeviceSynchronize() != hipSuccess) {
+        delete ctx;
+        return DNLG_E_HIP;
+      }
+      g_tables_uploaded[device] = 1;
+    }
+  }
+  *out = ctx;
+  return DNLG_OK;
+}
+
+int dnlg_ctx_destroy(dnlg_ctx *ctx) {
+  if (!ctx) return DNLG_E_INVALID;
+  hipSetDevice(ctx->device);
+  batch_cache_clear(ctx);
+  hipStreamSynchronize(ctx->stream);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return DNLG_OK;
+}
+
+const char *dnlg_last_error(const dnlg_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int64_t dnlg_footprint_bytes(int elem_type, int64_t n_nodes, int64_t n_elems, int precision, int track_tensors) {
+  if (elem_type != DNLG_EL_HEX8N_3D && elem_type != DNLG_EL_QUA4N_AX) return -1;
+  return footprint(elem_type, n_nodes, n_elems, precision, track_tensors);
+}
+
+int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *mat, const dnlg_bcs *bcs,
+                    const dnlg_solver_params *prm, dnlg_job **out) {
+  if (!ctx || !mesh || !mat || !prm || !out) return fail(ctx, DNLG_E_INVALID, "null argument");
+  *out = nullptr;
+  if (mesh->elem_type != DNLG_EL_QUA4N_AX && mesh->elem_type != DNLG_EL_HEX8N_3D)
+    return fail(ctx, DNLG_E_INVALID, "unsupported element type");
+  if (mesh->n_nodes <= 0 || mesh->n_elems <= 0 || !mesh->coords || !mesh->conn)
+    return fail(ctx, DNLG_E_INVALID, "empty mesh");
+  if (mesh->n_nodes >= (int64_t)1 << 31 || mesh->n_elems * 8 >= (int64_t)1 << 31)
+    return fail(ctx, DNLG_E_CAPACITY, "mesh exceeds 32-bit indexing");
+  if (prm->dt_method != DNLG_DT_COURANT) return fail(ctx, DNLG_E_INVALID, "only the Courant time step is implemented");
+  if (prm->stress_method != DNLG_STRESS_NR) return fail(ctx, DNLG_E_INVALID, "only the Newton-Raphson stress integration is implemented");
+  if (prm->dt_freq <= 0 || !(prm->safety > 0) || prm->rho_b < 0.0 || prm->rho_b > 1.0)
+    return fail(ctx, DNLG_E_INVALID, "invalid solver parameters");
+  if (prm->precision != DNLG_PREC_FP64 && prm->precision != DNLG_PREC_MIXED)
+    return fail(ctx, DNLG_E_INVALID, "invalid precision");
+  if (mat->law < DNLG_LAW_JOHNSON_COOK || mat->law > DNLG_LAW_VOCE_LINEAR)
+    return fail(ctx, DNLG_E_INVALID, "invalid hardening law");
+  if (!(mat->density > 0) || !(mat->young > 0) || !(mat->heat_capacity > 0))
+    return fail(ctx, DNLG_E_INVALID, "material needs density, Young modulus and heat capacity > 0");
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, DNLG_E_HIP, "hipSetDevice");
+
+  dnlg_job *job = new dnlg_job();
+  job->ctx = ctx;
+  const int type = mesh->elem_type;
+  const int dim = type == DNLG_EL_HEX8N_3D ? 3 : 2;
+  const int nn = type == DNLG_EL_HEX8N_3D ? 8 : 4;
+  const int64_t N = mesh->n_nodes, NE = mesh->n_elems, G = NE * nn;
+  job->type = type;
+  job->dim = dim;
+  job->nn = nn;
+  job->N = N;
+  job->NE = NE;
+  job->G = G;
+  job->prec = prm->precision;
+  job->track = prm->track_tensors ? 1 : 0;
+  job->t_end = prm->t_end;
+  auto bail = [&](int st) {
+    dnlg_job_destroy(job);
+    return st;
+  };
+
+  // connectivity check and node -> element CSR in element order
+  std::vector<int32_t> conn(mesh->conn, mesh->conn + NE * nn);
+  std::vector<int32_t> n2e_ptr(N + 1, 0), n2e_idx(NE * nn);
+  for (int64_t k = 0; k < NE * nn; k++) {
+    if (conn[k] < 0 || conn[k] >= N) return bail(fail(ctx, DNLG_E_INVALID, "connectivity index out of range"));
+    n2e_ptr[conn[k] + 1]++;
+  }
+  for (int64_t n = 0; n < N; n++) n2e_ptr[n + 1] += n2e_ptr[n];
+  {
+    std::vector<int32_t> fill(n2e_ptr.begin(), n2e_ptr.end() - 1);
+    for (int64_t e = 0; e < NE; e++)
+      for (int a = 0; a < nn; a++) n2e_idx[fill[conn[e * nn + a]]++] = (int32_t)(e * nn + a);
+  }
+  // coordinates (ElementAxi::checkLevel2, ElementAxi.C:103-126)
+  std::vector<double> x(dim * N);
+  for (int d = 0; d < dim; d++)
+    for (int64_t n = 0; n < N; n++) x[d * N + n] = mesh->coords[d * N + n];
+  if (dim == 2)
+    for (int64_t n = 0; n < N; n++)
+      if (x[n] < 0.0) {
+        if (x[n] > -1e-8)
+          x[n] = 0.0;
+        else
+          return bail(fail(ctx, DNLG_E_INVALID, "negative radius in an axisymmetric model"));
+      }
+  // boundary conditions
+  std::vector<Boundary> bnd;
+  std::vector<int32_t> bc_ptr(N + 1, 0), bc_idx;
+  std::vector<double> v0(dim * N, 0.0);
+  if (bcs && bcs->n_boundaries > 0) {
+    if (!bcs->boundaries) return bail(fail(ctx, DNLG_E_INVALID, "boundaries missing"));
+    for (int i = 0; i < bcs->n_boundaries; i++) {
+      const dnlg_boundary &b = bcs->boundaries[i];
+      if (b.kind != DNLG_BC_RESTRAIN && b.kind != DNLG_BC_SPEED)
+        return bail(fail(ctx, DNLG_E_INVALID, "unsupported boundary kind"));
+      if (b.ramp < DNLG_RAMP_NONE || b.ramp > DNLG_RAMP_SOFT)
+        return bail(fail(ctx, DNLG_E_INVALID, "unknown ramp function"));
+      Boundary B;
+      B.kind = b.kind;
+      B.ramp = b.ramp;
+      for (int c = 0; c < 3; c++) B.v[c] = b.value[c];
+      B.rmin = b.ramp_min;
+      B.rmax = b.ramp_max;
+      bnd.push_back(B);
+    }
+    auto check_csr = [&](const int64_t *ptr, const int32_t *idx) -> bool {
+      if (!ptr) return true;
+      if (ptr[0] != 0) return false;
+      for (int64_t n = 0; n < N; n++)
+        if (ptr[n + 1] < ptr[n]) return false;
+      for (int64_t k = 0; k < ptr[N]; k++)
+        if (!idx || idx[k] < 0 || idx[k] >= bcs->n_boundaries) return false;
+      return true;
+    };
+    if (!check_csr(bcs->initial_ptr, bcs->initial_idx) || !check_csr(bcs->constant_ptr, bcs->constant_idx))
+      return bail(fail(ctx, DNLG_E_INVALID, "malformed boundary lists"));
+    if (bcs->constant_ptr) {
+      if (bcs->constant_ptr[N] >= ((int64_t)1 << 31)) return bail(fail(ctx, DNLG_E_CAPACITY, "too many BC entries"));
+      for (int64_t n = 0; n <= N; n++) bc_ptr[n] = (int32_t)bcs->constant_ptr[n];
+      bc_idx.assign(bcs->constant_idx, bcs->constant_idx + bcs->constant_ptr[N]);
+    }
+    // BoundaryCondition::applyInitial (BoundaryCondition.C:131-150): initial
+    // list then constant list; BoundarySpeed::applyInitial sets the speed
+    // vector and zero acceleration (BoundarySpeed.C:63-76).
+    for (int64_t n = 0; n < N; n++) {
+      for (int pass = 0; pass < 2; pass++) {
+        const int64_t *ptr = pass == 0 ? bcs->initial_ptr : bcs->constant_ptr;
+        const int32_t *idx = pass == 0 ? bcs->initial_idx : bcs->constant_idx;
+        if (!ptr) continue;
+        for (int64_t k = ptr[n]; k < ptr[n + 1]; k++) {
+          const dnlg_boundary &b = bcs->boundaries[idx[k]];
+          if (b.kind != DNLG_BC_SPEED) continue;
+          for (int d = 0; d < dim; d++) v0[d * N + n] = b.value[d];
+        }
+      }
+    }
+  }
+  if (bc_idx.empty()) bc_idx.push_back(0);
+  if (bnd.empty()) bnd.push_back(Boundary{});
+
+  JobDev &H = job->host;
+  std::memset(&H, 0, sizeof(JobDev));
+  H.type = type;
+  H.dim = dim;
+  H.nn = nn;
+  H.nip = nn;
+  H.N = N;
+  H.NE = NE;
+  H.G = G;
+  H.prec = prm->precision;
+  H.track = job->track;
+  H.law = mat->law;
+  H.dt_freq = prm->dt_freq;
+  H.rho0 = mat->density;
+  H.young = mat->young;
+  H.nu = mat->poisson;
+  H.twoG = mat->young / (1.0 + mat->poisson);                 // Material.h:110-114
+  H.K = mat->young / (3.0 * (1.0 - 2.0 * mat->poisson));        // Material.h:152-156
+  H.heatFrac = mat->taylor_quinney / (mat->density * mat->heat_capacity);  // Element.C:438
+  H.eta = mat->taylor_quinney;
+  H.cp = mat->heat_capacity;
+  H.T0 = mat->T0;
+  for (int k = 0; k < 8; k++) H.lp[k] = mat->p[k];
+  // Explicit::computeChungHulbertIntegrationParameters (Explicit.C:86-104)
+  const double rb = prm->rho_b;
+  H.alphaM = (2.0 * rb - 1.0) / (1.0 + rb);
+  H.beta = (5.0 - 3.0 * rb) / ((1.0 + rb) * (1.0 + rb) * (2.0 - rb));
+  H.gammaCH = 1.5 - H.alphaM;
+  H.omegaS = std::sqrt((12.0 * std::pow(1.0 + rb, 3) * (2.0 - rb)) /
+                       (10.0 + 15.0 * rb - rb * rb + std::pow(rb, 3) - std::pow(rb, 4)));
+  H.safety = prm->safety;
+  H.c.t = prm->t_start;
+  H.c.dt = 0.0;
+  H.c.up_to = prm->t_start;
+  H.c.inc = 0;
+  H.c.f0 = 0;
+  H.c.status = 0;
+  H.c.running = 1;
+  H.c.courant_due = 0;
+  H.c.density_due = 0;
+  H.c.preloop = 1;
+
+  int st;
+#define ALLOC(ptr, count, init)                           \
+  if ((st = dalloc(job, &(ptr), (count), (init))) != 0) return bail(st);
+  ALLOC(H.x, dim * N, x.data());
+  ALLOC(H.disp, dim * N, (const double *)nullptr);
+  ALLOC(H.mass, N, (const double *)nullptr);
+  for (int k = 0; k < 2; k++) {
+    ALLOC(H.fu[k], dim * N, (const double *)nullptr);
+    ALLOC(H.fv[k], dim * N, v0.data());
+    ALLOC(H.fa[k], dim * N, (const double *)nullptr);
+  }
+  int32_t *p32;
+  Boundary *pb;
+  ALLOC(p32, N + 1, bc_ptr.data());
+  H.bc_ptr = p32;
+  ALLOC(p32, (int64_t)bc_idx.size(), bc_idx.data());
+  H.bc_idx = p32;
+  ALLOC(pb, (int64_t)bnd.size(), bnd.data());
+  H.bnd = pb;
+  ALLOC(p32, N + 1, n2e_ptr.data());
+  H.n2e_ptr = p32;
+  ALLOC(p32, NE * nn, n2e_idx.data());
+  H.n2e_idx = p32;
+  ALLOC(p32, NE * nn, conn.data());
+  H.conn = p32;
+  ALLOC(H.elemF, NE * nn * dim, (const double *)nullptr);
+  job->n_cmin = (int32_t)((NE + 255) / 256);
+  H.n_cmin = job->n_cmin;
+  ALLOC(H.cmin, (int64_t)job->n_cmin, (const double *)nullptr);
+  std::vector<double> fillT(G, mat->T0), fillR(G, mat->density);
+  ALLOC(H.ep, G, (const double *)nullptr);
+  ALLOC(H.eint, G, (const double *)nullptr);
+  ALLOC(H.einel, G, (const double *)nullptr);
+  ALLOC(H.T, G, fillT.data());
+  ALLOC(H.rho, G, fillR.data());
+  ALLOC(H.detJ0, G, (const double *)nullptr);
+  if (prm->precision == DNLG_PREC_MIXED) {
+    ALLOC(H.elemF32, NE * nn * dim, (const float *)nullptr);
+    ALLOC(H.sig32, 6 * G, (const float *)nullptr);
+    ALLOC(H.dep32, G, (const float *)nullptr);
+    ALLOC(H.sy32, G, (const float *)nullptr);
+    ALLOC(H.gam32, G, (const float *)nullptr);
+    ALLOC(H.gamc32, G, (const float *)nullptr);
+    ALLOC(H.detJ0f, G, (const float *)nullptr);
+  } else {
+    ALLOC(H.sig, 6 * G, (const double *)nullptr);
+    ALLOC(H.dep, G, (const double *)nullptr);
+    ALLOC(H.sy, G, (const double *)nullptr);
+    ALLOC(H.gam, G, (const double *)nullptr);
+    ALLOC(H.gamc, G, (const double *)nullptr);
+  }
+  if (job->track) {
+    ALLOC(H.eps, 6 * G, (const double *)nullptr);
+    ALLOC(H.epsP, 6 * G, (const double *)nullptr);
+  }
+  JobDev *dev;
+  ALLOC(dev, 1, &H);
+  job->dev = dev;
+#undef ALLOC
+  *out = job;
+  return DNLG_OK;
+}
+
+int dnlg_job_destroy(dnlg_job *job) {
+  if (!job) return DNLG_E_INVALID;
+  dnlg_ctx *ctx = job->ctx;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  batch_cache_forget(ctx, job);
+  for (void *p : job->allocs) hipFree(p);
+  delete job;
+  return DNLG_OK;
+}
+
+int dnlg_job_status(const dnlg_job *job) { return job ? job->status : DNLG_E_INVALID; }
+
+int dnlg_job_solve(dnlg_job *job, double t_up_to) {
+  if (!job) return DNLG_E_INVALID;
+  dnlg_job *const jobs[1] = {job};
+  return dnlg_batch_solve(job->ctx, jobs, 1, &t_up_to);
+}
+
+int dnlg_batch_solve(dnlg_ctx *ctx, dnlg_job *const *jobs, int n_jobs, const double *t_up_to) {
+  if (!ctx || !jobs || n_jobs <= 0 || !t_up_to) return fail(ctx, DNLG_E_INVALID, "invalid batch");
+  for (int i = 0; i < n_jobs; i++) {
+    if (!jobs[i] || jobs[i]->ctx != ctx) return fail(ctx, DNLG_E_INVALID, "job of another context");
+    for (int k = 0; k < i; k++)
+      if (jobs[k] == jobs[i]) return fail(ctx, DNLG_E_INVALID, "job listed twice");
+  }
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, DNLG_E_HIP, "hipSetDevice");
+  std::vector<dnlg_job *> list(jobs, jobs + n_jobs);
+  std::vector<double> up(t_up_to, t_up_to + n_jobs);
+  return batch_run(ctx, list, up);
+}
+
+int dnlg_job_sizes(const dnlg_job *job, int64_t *n_nodes, int64_t *n_elems, int32_t *n_ip, int32_t *dim) {
+  if (!job) return DNLG_E_INVALID;
+  if (n_nodes) *n_nodes = job->N;
+  if (n_elems) *n_elems = job->NE;
+  if (n_ip) *n_ip = job->nn;
+  if (dim) *dim = job->dim;
+  return DNLG_OK;
+}
+
+static int read_control(dnlg_job *job, Control *c) {
+  HIPCHK(job->ctx, hipStreamSynchronize(job->ctx->stream));
+  HIPCHK(job->ctx, hipMemcpy(c, &job->dev->c, sizeof(Control), hipMemcpyDeviceToHost));
+  return DNLG_OK;
+}
+
+int dnlg_job_read_nodes(dnlg_job *job, double *coords, double *disp, double *speed) {
+  if (!job) return DNLG_E_INVALID;
+  hipSetDevice(job->ctx->device);
+  Control c;
+  int st = read_control(job, &c);
+  if (st) return st;
+  const int64_t N = job->N;
+  auto get = [&](double *dst, const double *src) -> int {
+    if (!dst) return DNLG_OK;
+    std::memset(dst, 0, 3 * N * sizeof(double));
+    HIPCHK(job->ctx, hipMemcpy(dst, src, job->dim * N * sizeof(double), hipMemcpyDeviceToHost));
+    return DNLG_OK;
+  };
+  if ((st = get(coords, job->host.x))) return st;
+  if ((st = get(disp, job->host.disp))) return st;
+  if ((st = get(speed, job->host.fv[c.f0]))) return st;
+  return DNLG_OK;
+}
+
+int dnlg_job_read_mass(dnlg_job *job, double *mass) {
+  if (!job || !mass) return DNLG_E_INVALID;
+  hipSetDevice(job->ctx->device);
+  HIPCHK(job->ctx, hipStreamSynchronize(job->ctx->stream));
+  HIPCHK(job->ctx, hipMemcpy(mass, job->host.mass, job->N * sizeof(double), hipMemcpyDeviceToHost));
+  return DNLG_OK;
+}
+
+int dnlg_job_read_ip(dnlg_job *job, int field, double *out) {
+  if (!job || !out) return DNLG_E_INVALID;
+  hipSetDevice(job->ctx->device);
+  HIPCHK(job->ctx, hipStreamSynchronize(job->ctx->stream));
+  const int64_t G = job->G;
+  const JobDev &H = job->host;
+  const bool mixed = job->prec == DNLG_PREC_MIXED;
+  auto d64 = [&](const double *src, int64_t count, double *dst) -> int {
+    HIPCHK(job->ctx, hipMemcpy(dst, src, count * sizeof(double), hipMemcpyDeviceToHost));
+    return DNLG_OK;
+  };
+  auto f32 = [&](const float *src, int64_t count, double *dst) -> int {
+    std::vector<float> tmp(count);
+    HIPCHK(job->ctx, hipMemcpy(tmp.data(), src, count * sizeof(float), hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < count; k++) dst[k] = tmp[k];
+    return DNLG_OK;
+  };
+  auto stress = [&](double *dst) -> int { return mixed ? f32(H.sig32, 6 * G, dst) : d64(H.sig, 6 * G, dst); };
+  switch (field) {
+    case DNLG_F_PLASTIC_STRAIN: return d64(H.ep, G, out);
+    case DNLG_F_PLASTIC_STRAIN_RATE: return mixed ? f32(H.dep32, G, out) : d64(H.dep, G, out);
+    case DNLG_F_GAMMA: return mixed ? f32(H.gam32, G, out) : d64(H.gam, G, out);
+    case DNLG_F_GAMMA_CUMULATE: return mixed ? f32(H.gamc32, G, out) : d64(H.gamc, G, out);
+    case DNLG_F_YIELD_STRESS: return mixed ? f32(H.sy32, G, out) : d64(H.sy, G, out);
+    case DNLG_F_TEMPERATURE: return d64(H.T, G, out);
+    case DNLG_F_DENSITY: return d64(H.rho, G, out);
+    case DNLG_F_INTERNAL_ENERGY: return d64(H.eint, G, out);
+    case DNLG_F_INELASTIC_ENERGY: return d64(H.einel, G, out);
+    case DNLG_F_STRESS: return stress(out);
+    case DNLG_F_PRESSURE:
+    case DNLG_F_VON_MISES: {
+      std::vector<double> s(6 * G);
+      int st = stress(s.data());
+      if (st) return st;
+      for (int64_t g = 0; g < G; g++) {
+        const double xx = s[g], xy = s[G + g], xz = s[2 * G + g], yy = s[3 * G + g], yz = s[4 * G + g],
+                     zz = s[5 * G + g];
+        if (field == DNLG_F_PRESSURE)
+          out[g] = (xx + yy + zz) / 3.0;
+        else  // dnlVonMises (Macros.h:71-72)
+          out[g] = std::sqrt((sqd(xx - yy) + sqd(xx - zz) + sqd(yy - zz) + 6 * (sqd(xy) + sqd(xz) + sqd(yz))) / 2);
+      }
+      return DNLG_OK;
+    }
+    case DNLG_F_STRAIN:
+      if (!job->track) return fail(job->ctx, DNLG_E_STATE, "strain tensors not tracked (track_tensors = 0)");
+      return d64(H.eps, 6 * G, out);
+    case DNLG_F_PLASTIC_STRAIN_TENSOR:
+      if (!job->track) return fail(job->ctx, DNLG_E_STATE, "strain tensors not tracked (track_tensors = 0)");
+      return d64(H.epsP, 6 * G, out);
+  }
+  return fail(job->ctx, DNLG_E_INVALID, "unknown field");
+}
+
+int dnlg_job_scalars(dnlg_job *job, double *time, double *time_step, int64_t *increment) {
+  if (!job) return DNLG_E_INVALID;
+  hipSetDevice(job->ctx->device);
+  Control c;
+  int st = read_control(job, &c);
+  if (st) return st;
+  if (time) *time = c.t;
+  if (time_step) *time_step = c.dt;
+  if (increment) *increment = c.inc;
+  return DNLG_OK;
+}
+
+int dnlg_job_energies(dnlg_job *job, double out[4]) {
+  if (!job || !out) return DNLG_E_INVALID;
+  const int64_t N = job->N, G = job->G;
+  std::vector<double> v(3 * N), m(N), eint(G), einel(G), dj0(G);
+  int st;
+  if ((st = dnlg_job_read_nodes(job, nullptr, nullptr, v.data()))) return st;
+  if ((st = dnlg_job_read_mass(job, m.data()))) return st;
+  if ((st = dnlg_job_read_ip(job, DNLG_F_INTERNAL_ENERGY, eint.data()))) return st;
+  if ((st = dnlg_job_read_ip(job, DNLG_F_INELASTIC_ENERGY, einel.data()))) return st;
+  HIPCHK(job->ctx, hipMemcpy(dj0.data(), job->host.detJ0, G * sizeof(double), hipMemcpyDeviceToHost));
+  double ke = 0.0;
+  for (int64_t n = 0; n < N; n++) ke += m[n] * (sqd(v[n]) + sqd(v[N + n]) + sqd(v[2 * N + n])) / 2.0;
+  double ie = 0.0, wp = 0.0;
+  const double scale = job->dim == 2 ? k2PI : 1.0;
+  for (int64_t g = 0; g < G; g++) {
+    const double vol = dj0[g] * scale;
+    ie += job->host.rho0 * vol * eint[g];
+    wp += job->host.rho0 * vol * einel[g];
+  }
+  out[0] = ke;
+  out[1] = ie;
+  out[2] = wp;
+  out[3] = ke + ie;
+  return DNLG_OK;
+}
+
+int dnlg_ctx_set_timing(dnlg_ctx *ctx, int enable) {
+  if (!ctx) return DNLG_E_INVALID;
+  ctx->timing = enable != 0;
+  return DNLG_OK;
+}
+
+int dnlg_ctx_kernel_time(dnlg_ctx *ctx, const char *name, double *total_ms, int64_t *launches) {
+  if (!ctx || !name) return DNLG_E_INVALID;
+  auto it = ctx->ktime.find(name);
+  if (total_ms) *total_ms = it == ctx->ktime.end() ? 0.0 : it->second.first;
+  if (launches) *launches = it == ctx->ktime.end() ? 0 : it->second.second;
+  return DNLG_OK;
+}
+
+int dnlg_ctx_reset_timing(dnlg_ctx *ctx) {
+  if (!ctx) return DNLG_E_INVALID;
+  ctx->ktime.clear();
+  return DNLG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,53 @@
+// batch.h -- host-side job / context records and the batched cycle loop.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+
+struct dnlg_ctx;
+
+struct dnlg_job {
+  dnlg_ctx *ctx = nullptr;
+  int type = 0, dim = 0, nn = 0, prec = 0, track = 0;
+  int64_t N = 0, NE = 0, G = 0;
+  double t_end = 0.0;
+  dnlg::JobDev host{};          // host mirror holding the device pointers
+  dnlg::JobDev *dev = nullptr;  // device record (scalars live here)
+  std::vector<void *> allocs;
+  bool started = false;
+  int32_t n_cmin = 0;
+  int status = 0;
+};
+
+namespace dnlg {
+struct Batch;
+struct BatchDeleter {
+  void operator()(Batch *b) const;
+};
+using BatchPtr = std::unique_ptr<Batch, BatchDeleter>;
+}  // namespace dnlg
+
+struct dnlg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool timing = false;
+  std::map<std::string, std::pair<double, int64_t>> ktime;
+  std::vector<hipEvent_t> events;  // pool for timed launches
+  std::vector<dnlg::BatchPtr> cache;
+  int graph_cycles = 8;  // cycles per captured HIP graph (0 = plain launches)
+};
+
+namespace dnlg {
+
+int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vector<double> &up_to);
+void batch_cache_clear(dnlg_ctx *ctx);
+void batch_cache_forget(dnlg_ctx *ctx, const dnlg_job *job);
+
+}  // namespace dnlg

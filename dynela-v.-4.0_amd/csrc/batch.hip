@@ -1,0 +1,343 @@
+// batch.hip -- block task lists, the pre-loop, and the cycle loop over a
+// batch of jobs (one launch per phase covers every job of the batch).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "batch.h"
+#include "kernels.h"
+
+namespace dnlg {
+
+struct TaskList {
+  Task *d = nullptr;
+  int n = 0;
+};
+
+struct Batch {
+  std::vector<dnlg_job *> jobs;
+  JobDev **d_jobs = nullptr;
+  TaskList elem[2][2];  // [hex?][mixed?]
+  TaskList prep[2];     // [hex?]
+  TaskList cour[2];     // [hex?]
+  TaskList node[2][2];  // [dim==3?][mixed?]
+  double *d_snap = nullptr;
+  std::vector<double> h_snap;
+  hipGraphExec_t graph = nullptr;
+  int graph_cycles = 0;
+  std::vector<void *> allocs;
+  ~Batch() {
+    if (graph) hipGraphExecDestroy(graph);
+    for (void *p : allocs) hipFree(p);
+  }
+};
+
+namespace {
+
+int upload_tasks(Batch &B, const std::vector<Task> &h, TaskList &out) {
+  out.n = (int)h.size();
+  out.d = nullptr;
+  if (h.empty()) return 0;
+  void *p = nullptr;
+  if (hipMalloc(&p, h.size() * sizeof(Task)) != hipSuccess) return DNLG_E_OOM;
+  B.allocs.push_back(p);
+  if (hipMemcpy(p, h.data(), h.size() * sizeof(Task), hipMemcpyHostToDevice) != hipSuccess) return DNLG_E_HIP;
+  out.d = (Task *)p;
+  return 0;
+}
+
+// Contiguous item ranges per job, `per` items per block.
+std::vector<Task> make_tasks(const std::vector<int> &idx, const std::vector<int64_t> &count, int64_t per) {
+  std::vector<Task> t;
+  for (size_t k = 0; k < idx.size(); k++) {
+    const int64_t n = count[k];
+    int32_t blk = 0;
+    for (int64_t f = 0; f < n; f += per, blk++) t.push_back(Task{idx[k], blk, f, std::min(f + per, n)});
+  }
+  return t;
+}
+
+// Workgroups are dealt round-robin over the 8 XCDs, each with its own L2.
+// Place logical block L at launch slot (L % per) * 8 + L / per so that each
+// XCD walks one contiguous range of elements and their shared nodes stay in
+// that XCD's L2.  Padding slots carry empty ranges.
+std::vector<Task> xcd_order(const std::vector<Task> &t) {
+  const int n = (int)t.size();
+  if (n < 16) return t;
+  const int per = (n + 7) / 8;
+  std::vector<Task> o((size_t)per * 8, Task{t[0].job, 0, 0, 0});
+  for (int L = 0; L < n; L++) o[(size_t)(L % per) * 8 + L / per] = t[L];
+  return o;
+}
+
+int build_batch(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, Batch &B) {
+  B.jobs = jobs;
+  const int n = (int)jobs.size();
+  void *p = nullptr;
+  if (hipMalloc(&p, n * sizeof(JobDev *)) != hipSuccess) return DNLG_E_OOM;
+  B.allocs.push_back(p);
+  B.d_jobs = (JobDev **)p;
+  std::vector<JobDev *> h(n);
+  for (int i = 0; i < n; i++) h[i] = jobs[i]->dev;
+  if (hipMemcpy(B.d_jobs, h.data(), n * sizeof(JobDev *), hipMemcpyHostToDevice) != hipSuccess) return DNLG_E_HIP;
+  if (hipMalloc(&p, 5 * n * sizeof(double)) != hipSuccess) return DNLG_E_OOM;
+  B.allocs.push_back(p);
+  B.d_snap = (double *)p;
+  B.h_snap.assign(5 * n, 0.0);
+  int st;
+  for (int hx = 0; hx < 2; hx++) {
+    const int type = hx ? DNLG_EL_HEX8N_3D : DNLG_EL_QUA4N_AX;
+    const int nn = hx ? 8 : 4;
+    for (int mx = 0; mx < 2; mx++) {
+      std::vector<int> idx;
+      std::vector<int64_t> cnt;
+      for (int i = 0; i < n; i++)
+        if (jobs[i]->type == type && (jobs[i]->prec == DNLG_PREC_MIXED) == (mx == 1)) {
+          idx.push_back(i);
+          cnt.push_back(jobs[i]->NE);
+        }
+      const int threads = mx ? elem_block_threads_mixed(type) : elem_block_threads(type);
+      if ((st = upload_tasks(B, xcd_order(make_tasks(idx, cnt, threads / nn)), B.elem[hx][mx]))) return st;
+    }
+    std::vector<int> idx;
+    std::vector<int64_t> cnt;
+    for (int i = 0; i < n; i++)
+      if (jobs[i]->type == type) {
+        idx.push_back(i);
+        cnt.push_back(jobs[i]->NE);
+      }
+    if ((st = upload_tasks(B, make_tasks(idx, cnt, 256), B.prep[hx]))) return st;
+    if ((st = upload_tasks(B, make_tasks(idx, cnt, 256), B.cour[hx]))) return st;
+  }
+  for (int d3 = 0; d3 < 2; d3++)
+    for (int mx = 0; mx < 2; mx++) {
+      std::vector<int> idx;
+      std::vector<int64_t> cnt;
+      for (int i = 0; i < n; i++)
+        if ((jobs[i]->dim == 3) == (d3 == 1) && (jobs[i]->prec == DNLG_PREC_MIXED) == (mx == 1)) {
+          idx.push_back(i);
+          cnt.push_back(jobs[i]->N);
+        }
+      if ((st = upload_tasks(B, make_tasks(idx, cnt, 256), B.node[d3][mx]))) return st;
+    }
+  (void)ctx;
+  return 0;
+}
+
+struct Timer {
+  dnlg_ctx *ctx;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> marks;
+  size_t used = 0;
+  explicit Timer(dnlg_ctx *c) : ctx(c) {}
+  template <typename F>
+  void run(const char *name, F &&f) {
+    if (!ctx->timing) {
+      f();
+      return;
+    }
+    while (ctx->events.size() < used + 2) {
+      hipEvent_t e;
+      hipEventCreate(&e);
+      ctx->events.push_back(e);
+    }
+    hipEvent_t a = ctx->events[used], b = ctx->events[used + 1];
+    used += 2;
+    hipEventRecord(a, ctx->stream);
+    f();
+    hipEventRecord(b, ctx->stream);
+    marks.push_back({name, {a, b}});
+  }
+  void collect() {
+    for (auto &m : marks) {
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, m.second.first, m.second.second);
+      auto &acc = ctx->ktime[m.first];
+      acc.first += ms;
+      acc.second += 1;
+    }
+    marks.clear();
+    used = 0;
+  }
+};
+
+void launch_cycle(dnlg_ctx *ctx, Batch &B, Timer *tm) {
+  hipStream_t s = ctx->stream;
+  const int n = (int)B.jobs.size();
+  auto timed = [&](const char *name, auto &&f) {
+    if (tm)
+      tm->run(name, f);
+    else
+      f();
+  };
+  timed("control", [&] { launch_cycle_begin(B.d_jobs, n, s); });
+  timed("predict", [&] {
+    for (int d3 = 0; d3 < 2; d3++)
+      for (int mx = 0; mx < 2; mx++) launch_predict(d3 ? 3 : 2, B.d_jobs, B.node[d3][mx].d, B.node[d3][mx].n, s);
+  });
+  timed("element", [&] {
+    for (int hx = 0; hx < 2; hx++) {
+      const int type = hx ? DNLG_EL_HEX8N_3D : DNLG_EL_QUA4N_AX;
+      launch_elem_fp64(type, B.d_jobs, B.elem[hx][0].d, B.elem[hx][0].n, s);
+      launch_elem_mixed(type, B.d_jobs, B.elem[hx][1].d, B.elem[hx][1].n, s);
+    }
+  });
+  timed("correct", [&] {
+    for (int d3 = 0; d3 < 2; d3++)
+      for (int mx = 0; mx < 2; mx++)
+        launch_correct(d3 ? 3 : 2, mx == 1, B.d_jobs, B.node[d3][mx].d, B.node[d3][mx].n, s);
+  });
+  timed("courant", [&] {
+    for (int hx = 0; hx < 2; hx++) launch_courant(hx ? 3 : 2, B.d_jobs, B.cour[hx].d, B.cour[hx].n, 0, s);
+  });
+  timed("control", [&] { launch_cycle_end(B.d_jobs, n, s); });
+}
+
+// Explicit.C:143-178: reference Jacobian (detJ0), lumped mass, forced time
+// step, then one element pass on the zero field0 increment.
+int run_preloop(dnlg_ctx *ctx, Batch &B) {
+  hipStream_t s = ctx->stream;
+  const int n = (int)B.jobs.size();
+  for (int hx = 0; hx < 2; hx++)
+    launch_prep(hx ? DNLG_EL_HEX8N_3D : DNLG_EL_QUA4N_AX, B.d_jobs, B.prep[hx].d, B.prep[hx].n, s);
+  for (int d3 = 0; d3 < 2; d3++)
+    for (int mx = 0; mx < 2; mx++) launch_mass_gather(d3 ? 3 : 2, B.d_jobs, B.node[d3][mx].d, B.node[d3][mx].n, s);
+  for (int hx = 0; hx < 2; hx++) launch_courant(hx ? 3 : 2, B.d_jobs, B.cour[hx].d, B.cour[hx].n, 1, s);
+  launch_force_dt(B.d_jobs, n, s);
+  for (int hx = 0; hx < 2; hx++) {
+    const int type = hx ? DNLG_EL_HEX8N_3D : DNLG_EL_QUA4N_AX;
+    launch_elem_fp64(type, B.d_jobs, B.elem[hx][0].d, B.elem[hx][0].n, s);
+    launch_elem_mixed(type, B.d_jobs, B.elem[hx][1].d, B.elem[hx][1].n, s);
+  }
+  if (hipGetLastError() != hipSuccess) return DNLG_E_HIP;
+  return 0;
+}
+
+Batch *find_or_build(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, int &st) {
+  for (auto &b : ctx->cache)
+    if (b->jobs == jobs) return b.get();
+  BatchPtr B(new Batch());
+  st = build_batch(ctx, jobs, *B);
+  if (st) return nullptr;
+  if (ctx->cache.size() >= 8) ctx->cache.erase(ctx->cache.begin());
+  ctx->cache.push_back(std::move(B));
+  return ctx->cache.back().get();
+}
+
+int capture_graph(dnlg_ctx *ctx, Batch &B, int cycles) {
+  hipGraph_t g = nullptr;
+  if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return DNLG_E_HIP;
+  for (int k = 0; k < cycles; k++) launch_cycle(ctx, B, nullptr);
+  if (hipStreamEndCapture(ctx->stream, &g) != hipSuccess) return DNLG_E_HIP;
+  hipGraphExec_t ex = nullptr;
+  if (hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) != hipSuccess) {
+    hipGraphDestroy(g);
+    return DNLG_E_HIP;
+  }
+  hipGraphDestroy(g);
+  B.graph = ex;
+  B.graph_cycles = cycles;
+  return 0;
+}
+
+}  // namespace
+
+int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vector<double> &up_to) {
+  const int n = (int)jobs.size();
+  hipStream_t s = ctx->stream;
+  // target times, clipped to each job's end time
+  for (int i = 0; i < n; i++) {
+    double u = std::min(up_to[i], jobs[i]->t_end);
+    if (hipMemcpyAsync(&jobs[i]->dev->c.up_to, &u, sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      ctx->err = "setting the target time failed";
+      return DNLG_E_HIP;
+    }
+  }
+  int st = 0;
+  std::vector<dnlg_job *> fresh;
+  for (auto *j : jobs)
+    if (!j->started && j->status == 0) fresh.push_back(j);
+  if (!fresh.empty()) {
+    Batch P;
+    if ((st = build_batch(ctx, fresh, P)) || (st = run_preloop(ctx, P))) {
+      ctx->err = "pre-loop launch failed";
+      return st;
+    }
+    if (hipStreamSynchronize(s) != hipSuccess) {
+      ctx->err = "pre-loop failed";
+      return DNLG_E_HIP;
+    }
+    for (auto *j : fresh) j->started = true;
+  }
+  Batch *B = find_or_build(ctx, jobs, st);
+  if (!B) {
+    ctx->err = "building the batch failed";
+    return st;
+  }
+  const bool use_graph = !ctx->timing && ctx->graph_cycles > 0;
+  if (use_graph && !B->graph) {
+    if ((st = capture_graph(ctx, *B, ctx->graph_cycles))) {
+      ctx->err = "graph capture failed";
+      return st;
+    }
+  }
+  Timer tm(ctx);
+  while (true) {
+    launch_snapshot(B->d_jobs, n, B->d_snap, s);
+    if (hipMemcpyAsync(B->h_snap.data(), B->d_snap, 5 * n * sizeof(double), hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      ctx->err = "device error during the cycle loop";
+      return DNLG_E_HIP;
+    }
+    tm.collect();
+    double need = 0.0;
+    for (int i = 0; i < n; i++) {
+      const double t = B->h_snap[5 * i], dt = B->h_snap[5 * i + 1], up = B->h_snap[5 * i + 2];
+      const int js = (int)B->h_snap[5 * i + 3];
+      jobs[i]->status = js;
+      if (js != 0 || !(t < up)) continue;
+      need = std::max(need, dt > 0 ? std::ceil((up - t) / dt) : 1.0);
+    }
+    if (need <= 0.0) break;
+    const int chunk = (int)std::min(need, 4096.0);
+    if (use_graph) {
+      const int reps = (chunk + B->graph_cycles - 1) / B->graph_cycles;
+      for (int r = 0; r < reps; r++)
+        if (hipGraphLaunch(B->graph, s) != hipSuccess) {
+          ctx->err = "graph launch failed";
+          return DNLG_E_HIP;
+        }
+    } else {
+      const int lim = ctx->timing ? std::min(chunk, 64) : chunk;
+      for (int k = 0; k < lim; k++) launch_cycle(ctx, *B, &tm);
+    }
+    if (hipGetLastError() != hipSuccess) {
+      ctx->err = "kernel launch failed";
+      return DNLG_E_HIP;
+    }
+  }
+  for (auto *j : jobs)
+    if (j->status != 0) {
+      ctx->err = "job failed with status " + std::to_string(j->status);
+      return j->status;
+    }
+  return 0;
+}
+
+void BatchDeleter::operator()(Batch *b) const { delete b; }
+
+void batch_cache_clear(dnlg_ctx *ctx) { ctx->cache.clear(); }
+
+void batch_cache_forget(dnlg_ctx *ctx, const dnlg_job *job) {
+  auto &c = ctx->cache;
+  c.erase(std::remove_if(c.begin(), c.end(),
+                         [&](const BatchPtr &b) {
+                           return std::find(b->jobs.begin(), b->jobs.end(), job) != b->jobs.end();
+                         }),
+          c.end());
+}
+
+}  // namespace dnlg

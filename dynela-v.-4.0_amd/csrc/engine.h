@@ -1,0 +1,99 @@
+// engine.h -- device-side data layout of the explicit-dynamics engine.
+//
+// One job = one mesh.  Every job owns its device buffers (structure of
+// arrays) and a device-resident JobDev record that also carries the solver
+// scalars (time, time step, increment, ping-pong field index, status), so a
+// cycle of the time loop needs no host round trip and can be captured in a
+// HIP graph.  A batch is a list of JobDev pointers plus per-kernel block task
+// lists, so one launch per phase advances any number of uneven jobs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/dnlg.h"
+
+namespace dnlg {
+
+// Constants of Sources/dnlKernel/Macros.h:37-56.
+constexpr double kPI = 3.14159265358979324;
+constexpr double k2PI = 6.28318530717958648;
+constexpr double kSqrt3 = 1.73205080756887729;
+constexpr double kSqrt23 = 0.816496580927726034;
+constexpr double kSqrt32 = 1.22474487139158894;
+constexpr double kEps = 2.2204460492503131e-16;
+
+// Reference NR settings (Element.C:407-430).
+constexpr double kGammaInitial = 1e-8;
+constexpr double kTolNR = 1e-8;
+constexpr int kItMax = 250;
+
+__host__ __device__ __forceinline__ double sqd(double a) { return a * a; }
+
+struct Boundary {
+  int32_t kind;  // DNLG_BC_*
+  int32_t ramp;  // DNLG_RAMP_*
+  double v[3];
+  double rmin, rmax;
+};
+
+// Solver scalars of one job (Solver.h:45-50 plus engine flags).
+struct Control {
+  double t;      // model->currentTime
+  double dt;     // Solver::timeStep (persists, clipped at the end)
+  double up_to;  // target of the current advance
+  int64_t inc;   // Solver::currentIncrement
+  int32_t f0;    // which ping-pong slot is the reference's field0
+  int32_t status;       // DNLG_OK or an error code (sticky)
+  int32_t running;      // this cycle advances the job
+  int32_t courant_due;  // Courant recompute after this cycle
+  int32_t density_due;  // element kernel writes IP densities this cycle
+  int32_t preloop;      // element pass of Explicit.C:158-178
+};
+
+struct JobDev {
+  Control c;
+  // sizes
+  int32_t type, dim, nn, nip;
+  int64_t N, NE, G;  // nodes, elements, integration points
+  int32_t prec, track, law, dt_freq;
+  // material (Material.h:110-169) and law parameters
+  double rho0, young, nu, twoG, K, heatFrac, eta, cp, T0;
+  double lp[8];
+  // Chung-Hulbert (Explicit.C:86-104) and time-step settings
+  double alphaM, beta, gammaCH, omegaS, safety;
+  // nodal state, [dim][N]
+  double *x, *disp, *mass;
+  double *fu[2], *fv[2], *fa[2];
+  const int32_t *bc_ptr;  // [N+1] constant-BC list per node
+  const int32_t *bc_idx;
+  const Boundary *bnd;
+  const int32_t *n2e_ptr;  // [N+1] node -> (element*nn + local node), element order
+  const int32_t *n2e_idx;
+  // elements
+  const int32_t *conn;  // [NE][nn]
+  double *elemF;        // [NE][nn][dim] per-element internal forces (FP64)
+  float *elemF32;       // mixed-precision variant of elemF
+  double *cmin;         // Courant partial minima, one per Courant block
+  int32_t n_cmin;
+  // integration-point state, index e*nip + p; tensors as [6][G]
+  double *sig;  // FP64 stress (FP64 mode)
+  float *sig32; // FP32 stress (mixed mode)
+  double *ep, *eint, *einel, *T;  // FP64 in both modes
+  double *dep, *sy, *gam, *gamc;  // FP64 mode
+  float *dep32, *sy32, *gam32, *gamc32;  // mixed mode
+  double *rho, *detJ0;
+  float *detJ0f;
+  double *eps, *epsP;  // optional tensors (track_tensors)
+};
+
+// One block of work: job index into the batch, local block index within the
+// job, and the half-open item range [first, end).
+struct Task {
+  int32_t job;
+  int32_t blk;
+  int64_t first;
+  int64_t end;
+};
+
+}  // namespace dnlg

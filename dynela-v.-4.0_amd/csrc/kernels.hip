@@ -1,0 +1,792 @@
+// kernels.hip -- CDNA4 kernels of one explicit cycle (FP64 element path,
+// nodal kernels, Courant minimum, control) and of the pre-loop set-up.
+//
+// Cycle = Explicit::solve loop body (Explicit.C:184-252):
+//   k_cycle_begin   updateTimes (Explicit.C:414-427)
+//   k_predict       computePredictions + BCs (Explicit.C:438-482)
+//   k_elem_*        computeStrains, computePressure, computeStress,
+//                   computeFinalRotation, computeInternalForces,
+//                   computeDensity (Model.C:800-899, Explicit.C:555-562)
+//   k_correct       explicitSolve + BCs (Explicit.C:489-536), gather-based
+//                   force assembly in element order (Model.C:864-899)
+//   k_courant       computeCourantTimeStep (Model.C:636-670), per-block min
+//   k_cycle_end     endStep + computeTimeStep (Explicit.C:539-552,
+//                   Solver.C:196-236)
+// The element Jacobian of Explicit.C:245-246 is recomputed inside the next
+// cycle's element kernel from the same coordinates, so nothing per-IP
+// geometric is stored between cycles.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "material.h"
+#include "polar_fp64.h"
+
+namespace dnlg {
+
+// Element shape tables at the Gauss points: the closed forms tabulated in
+// ElQua4NAx.C:63-110 and ElHex8N3D.C:93-231, generated on the host
+// (tables.cpp) and uploaded once.
+__constant__ double c_hexN[8][8];
+__constant__ double c_hexD[8][8][3];
+__constant__ double c_quaN[4][4];
+__constant__ double c_quaD[4][4][2];
+
+template <int NN>
+__device__ __forceinline__ double tabN(int p, int a) {
+  if constexpr (NN == 8) return c_hexN[p][a];
+  else return c_quaN[p][a];
+}
+template <int NN>
+__device__ __forceinline__ double tabD(int p, int a, int j) {
+  if constexpr (NN == 8) return c_hexD[p][a][j];
+  else return c_quaD[p][a][j];
+}
+
+__device__ __forceinline__ int sym_idx(int i, int j) { return (i == j) ? (i == 0 ? 0 : (i == 1 ? 3 : 5)) : (i + j == 1 ? 1 : (i + j == 2 ? 2 : 4)); }
+
+__device__ __forceinline__ void set_status(JobDev *J, int st) { atomicCAS(&J->c.status, 0, st); }
+
+// ---------------------------------------------------------------------------
+// Element kernel, FP64.  One lane per integration point (NIP lanes per
+// element, NIP == NN for both element types); the element's nodes are staged
+// in LDS by its own lanes, per-IP B-matrices, stresses and weights go back to
+// LDS so that lane a can sum node a's force over the IPs in the reference's
+// order (IP, then direction j).
+// ---------------------------------------------------------------------------
+template <int NN, int DIM, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_elem_fp64(JobDev *const *jobs, const Task *tasks) {
+  constexpr int NIP = NN;
+  constexpr int EPB = BLOCK / NIP;
+  __shared__ double s_x[EPB][NN][DIM];
+  __shared__ double s_u[EPB][NN][DIM];
+  __shared__ double s_dN[EPB][NIP][NN][DIM];
+  __shared__ double s_sw[EPB][NIP][8];
+  __shared__ double s_tr[EPB][NIP];
+
+  const Task tk = tasks[blockIdx.x];
+  JobDev *J = jobs[tk.job];
+  if (!J->c.running || J->c.status != 0) return;
+  const int le = threadIdx.x / NIP;
+  const int p = threadIdx.x % NIP;
+  const int64_t e = tk.first + le;
+  const bool valid = e < tk.end;
+  const int64_t N = J->N;
+  const int f0 = J->c.f0;
+
+  // stage the element's nodes: lane p brings node p
+  if (valid) {
+    const int nd = J->conn[e * NN + p];
+    const double *u0 = J->fu[f0];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      s_x[le][p][d] = J->x[d * N + nd];
+      s_u[le][p][d] = u0[d * N + nd];
+    }
+  }
+  __syncthreads();
+
+  const int64_t G = J->G;
+  const int64_t g = e * NIP + p;
+  double dN[NN][DIM];
+  double detJ = 1.0, radius = 1.0;
+  double dE[6] = {0, 0, 0, 0, 0, 0}, R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  if (valid) {
+    // Jacobian, inverse, B-matrix (Element3D.C:64-115, ElementPlane.C:46-102)
+    double Jm[DIM][DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; i++)
+#pragma unroll
+      for (int j = 0; j < DIM; j++) Jm[i][j] = 0.0;
+#pragma unroll
+    for (int a = 0; a < NN; a++)
+#pragma unroll
+      for (int i = 0; i < DIM; i++)
+#pragma unroll
+        for (int j = 0; j < DIM; j++) Jm[i][j] += tabD<NN>(p, a, j) * s_x[le][a][i];
+    double inv[DIM][DIM];
+    if constexpr (DIM == 3) {
+      const double *dd = &Jm[0][0];
+      detJ = dd[1] * dd[5] * dd[6] - dd[2] * dd[4] * dd[6] + dd[2] * dd[3] * dd[7] - dd[0] * dd[5] * dd[7] -
+             dd[1] * dd[3] * dd[8] + dd[0] * dd[4] * dd[8];
+      const double id = 1.0 / detJ;
+      inv[0][0] = (dd[4] * dd[8] - dd[5] * dd[7]) * id;
+      inv[0][1] = (dd[2] * dd[7] - dd[1] * dd[8]) * id;
+      inv[0][2] = (dd[1] * dd[5] - dd[2] * dd[4]) * id;
+      inv[1][0] = (dd[5] * dd[6] - dd[3] * dd[8]) * id;
+      inv[1][1] = (dd[0] * dd[8] - dd[2] * dd[6]) * id;
+      inv[1][2] = (dd[2] * dd[3] - dd[0] * dd[5]) * id;
+      inv[2][0] = (dd[3] * dd[7] - dd[4] * dd[6]) * id;
+      inv[2][1] = (dd[1] * dd[6] - dd[0] * dd[7]) * id;
+      inv[2][2] = (dd[0] * dd[4] - dd[1] * dd[3]) * id;
+    } else {
+      detJ = Jm[0][0] * Jm[1][1] - Jm[1][0] * Jm[0][1];
+      const double id = 1.0 / detJ;
+      inv[0][0] = Jm[1][1] * id;
+      inv[0][1] = -Jm[0][1] * id;
+      inv[1][0] = -Jm[1][0] * id;
+      inv[1][1] = Jm[0][0] * id;
+    }
+    if (detJ < 0.0) set_status(J, DNLG_E_NEG_JACOBIAN);
+#pragma unroll
+    for (int a = 0; a < NN; a++)
+#pragma unroll
+      for (int j = 0; j < DIM; j++) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < DIM; k++) s += tabD<NN>(p, a, k) * inv[k][j];
+        dN[a][j] = s;
+      }
+    // deformation gradient from the field0 increment (Element3D.C:172-195,
+    // ElementAxi.C:78-100) and polar / log strain (Tensor2.C:1442-1457)
+    double F[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+#pragma unroll
+    for (int a = 0; a < NN; a++)
+#pragma unroll
+      for (int i = 0; i < DIM; i++)
+#pragma unroll
+        for (int j = 0; j < DIM; j++) F[3 * i + j] += dN[a][j] * s_u[le][a][i];
+    if constexpr (DIM == 2) {
+      double r = 0.0, vr = 0.0;
+#pragma unroll
+      for (int a = 0; a < NN; a++) {
+        r += tabN<NN>(p, a) * s_x[le][a][0];
+        vr += tabN<NN>(p, a) * s_u[le][a][0];
+      }
+      radius = r;
+      F[8] += vr / radius;
+    }
+    polar_lnU_fp64(F, dE, R);
+    s_tr[le][p] = dE[0] + dE[3] + dE[5];
+  }
+  __syncthreads();
+
+  double sig[6], wdj = 0.0;
+  if (valid) {
+    // Element::computePressure (Element.C:1125-1143)
+    double inc = 0.0;
+#pragma unroll
+    for (int q = 0; q < NIP; q++) inc += s_tr[le][q];
+    inc /= NIP;
+#pragma unroll
+    for (int c = 0; c < 6; c++) sig[c] = J->sig[c * G + g];
+    const double pressure = (sig[0] + sig[3] + sig[5]) / 3.0 + J->K * inc;
+
+    // Element::computeStress (Element.C:407-622)
+    const double dt = J->c.dt;
+    const double TwoG = J->twoG;
+    const double TwoG32 = kSqrt32 * TwoG;
+    const double heatFrac = J->heatFrac;
+    double old[6], s[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++) old[c] = sig[c];
+    const double pm = (old[0] + old[3] + old[5]) / 3.0;
+#pragma unroll
+    for (int c = 0; c < 6; c++) s[c] = old[c];
+    s[0] -= pm;
+    s[3] -= pm;
+    s[5] -= pm;
+    const double Snorm0 = sqrt(sqd(s[0]) + sqd(s[3]) + sqd(s[5]) + 2.0 * (sqd(s[1]) + sqd(s[2]) + sqd(s[4])));
+    const double em = (dE[0] + dE[3] + dE[5]) / 3.0;
+    double de[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++) de[c] = dE[c];
+    de[0] -= em;
+    de[3] -= em;
+    de[5] -= em;
+#pragma unroll
+    for (int c = 0; c < 6; c++) s[c] += TwoG * de[c];
+    const double Snorm = sqrt(sqd(s[0]) + sqd(s[3]) + sqd(s[5]) + 2.0 * (sqd(s[1]) + sqd(s[2]) + sqd(s[4])));
+    const double Strial = kSqrt32 * Snorm;
+    const double Tcur = J->T[g];
+    const double ep0 = J->ep[g];
+    double yield = J->sy[g];
+    double gamma = 0.0;
+    const int law = J->law;
+    const double *lp = J->lp;
+    if (yield == 0.0) {
+      yield = law_yield(law, lp, kGammaInitial, kGammaInitial / dt, Tcur);
+      J->sy[g] = yield;
+    }
+    if (Strial > yield) {
+      double gmin = 0.0, gmax = (Strial - yield) / TwoG32;
+      gamma = J->gam[g];
+      if (ep0 == 0.0) gamma = kSqrt32 * kGammaInitial;
+      double depsp = kSqrt23 * gamma / dt;
+      double epsp = ep0 + kSqrt23 * gamma;
+      double T = Tcur + 0.5 * gamma * heatFrac * (kSqrt23 * yield + Snorm0);
+      const double eta_rcp = heatFrac;
+      int iterate = 0;
+      while (true) {
+        double hard;
+        law_yield_and_slope(law, lp, eta_rcp, epsp, depsp, T, dt, yield, hard);
+        const double fun = Strial - gamma * TwoG32 - yield;
+        if (fun < 0.0)
+          gmax = gamma;
+        else
+          gmin = gamma;
+        const double dfun = TwoG32 + kSqrt23 * hard;
+        double dgamma = fun / dfun;
+        gamma += dgamma;
+        if ((gmax - gamma) * (gamma - gmin) < 0.0) {
+          dgamma = 0.5 * (gmax - gmin);
+          gamma = gmin + dgamma;
+        }
+        if (fabs(dgamma) < kTolNR) break;
+        depsp = kSqrt23 * gamma / dt;
+        epsp = ep0 + kSqrt23 * gamma;
+        T = Tcur + 0.5 * gamma * heatFrac * (kSqrt23 * yield + Snorm0);
+        if (++iterate > kItMax) {
+          set_status(J, DNLG_E_NR_DIVERGED);
+          break;
+        }
+      }
+      if (J->track) {
+#pragma unroll
+        for (int c = 0; c < 6; c++) J->epsP[c * G + g] += gamma * s[c] / Snorm;
+      }
+      const double scale = 1.0 - TwoG * gamma / Snorm;
+#pragma unroll
+      for (int c = 0; c < 6; c++) s[c] *= scale;
+      J->ep[g] = ep0 + kSqrt23 * gamma;
+      J->dep[g] = kSqrt23 * gamma / dt;
+      J->gam[g] = gamma;
+      J->gamc[g] += gamma;
+      J->sy[g] = yield;
+    }
+#pragma unroll
+    for (int c = 0; c < 6; c++) sig[c] = s[c];
+    sig[0] += pressure;
+    sig[3] += pressure;
+    sig[5] += pressure;
+    // specific internal energy, plastic work, adiabatic heating
+    const double power = 0.5 * (dE[0] * (old[0] + sig[0]) + dE[3] * (old[3] + sig[3]) + dE[5] * (old[5] + sig[5]) +
+                                2.0 * (dE[1] * (old[1] + sig[1]) + dE[2] * (old[2] + sig[2]) +
+                                       dE[4] * (old[4] + sig[4])));
+    J->eint[g] += power / J->rho0;
+    if (gamma != 0.0) {
+      const double sn = sqrt(sqd(s[0]) + sqd(s[3]) + sqd(s[5]) + 2.0 * (sqd(s[1]) + sqd(s[2]) + sqd(s[4])));
+      const double w = 0.5 * gamma * (sn + Snorm0);
+      J->einel[g] += w / J->rho0;
+      J->T[g] = Tcur + heatFrac * w;
+    }
+    // Element::computeFinalRotation (Element.C:1146-1161)
+    rotate_sym(R, sig);
+#pragma unroll
+    for (int c = 0; c < 6; c++) J->sig[c * G + g] = sig[c];
+    if (J->track) {
+      double t6[6];
+#pragma unroll
+      for (int c = 0; c < 6; c++) t6[c] = J->eps[c * G + g] + dE[c];
+      rotate_sym(R, t6);
+#pragma unroll
+      for (int c = 0; c < 6; c++) J->eps[c * G + g] = t6[c];
+#pragma unroll
+      for (int c = 0; c < 6; c++) t6[c] = J->epsP[c * G + g];
+      rotate_sym(R, t6);
+#pragma unroll
+      for (int c = 0; c < 6; c++) J->epsP[c * G + g] = t6[c];
+    }
+    // Element::computeDensity (Element.C:1243-1256), on x_n
+    if (J->c.density_due) {
+      double rr = J->rho0 * J->detJ0[g] / detJ;
+      if constexpr (DIM == 2) rr /= radius;
+      J->rho[g] = rr;
+    }
+    wdj = detJ;  // Gauss weights are 1 (ElQua4NAx.C:63-110, ElHex8N3D.C:93-231)
+    if constexpr (DIM == 2) wdj *= k2PI * radius;
+#pragma unroll
+    for (int a = 0; a < NN; a++)
+#pragma unroll
+      for (int j = 0; j < DIM; j++) s_dN[le][p][a][j] = dN[a][j];
+#pragma unroll
+    for (int c = 0; c < 6; c++) s_sw[le][p][c] = sig[c];
+    s_sw[le][p][6] = wdj;
+    s_sw[le][p][7] = radius;
+  }
+  __syncthreads();
+
+  // Element::computeInternalForces (Element.C:362-404): lane a sums node a
+  if (valid) {
+    const int a = p;
+    double f[DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; i++) f[i] = 0.0;
+#pragma unroll
+    for (int q = 0; q < NIP; q++) {
+      const double w = s_sw[le][q][6];
+#pragma unroll
+      for (int i = 0; i < DIM; i++)
+#pragma unroll
+        for (int j = 0; j < DIM; j++) f[i] -= s_dN[le][q][a][j] * s_sw[le][q][sym_idx(j, i)] * w;
+      if constexpr (DIM == 2) f[0] -= tabN<NN>(q, a) * s_sw[le][q][5] / s_sw[le][q][7] * w;
+    }
+    double *out = J->elemF + (e * NN + a) * DIM;
+#pragma unroll
+    for (int i = 0; i < DIM; i++) out[i] = f[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Nodal kernels
+// ---------------------------------------------------------------------------
+// BoundaryRestrain / BoundarySpeed ::applyConstantOnNewFields
+// (BoundaryRestrain.C:98-150, BoundarySpeed.C:139-226) on both fields.
+template <int DIM>
+__device__ __forceinline__ void apply_bcs(const JobDev *J, int64_t n, double time, double h, double ua[DIM],
+                                          double va[DIM], double aa[DIM], double ub[DIM], double vb[DIM],
+                                          double ab[DIM]) {
+  const int k0 = J->bc_ptr[n], k1 = J->bc_ptr[n + 1];
+  for (int k = k0; k < k1; k++) {
+    const Boundary B = J->bnd[J->bc_idx[k]];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      if (B.kind == DNLG_BC_RESTRAIN) {
+        if (B.v[d] > 0) {
+          ua[d] = ub[d] = 0.0;
+          va[d] = vb[d] = 0.0;
+          aa[d] = ab[d] = 0.0;
+        }
+      } else if (B.v[d] != 0.0) {
+        if (B.ramp == DNLG_RAMP_NONE) {
+          aa[d] = ab[d] = 0.0;
+          va[d] = vb[d] = B.v[d];
+          ua[d] = va[d] * h;
+          ub[d] = vb[d] * h;
+        } else {
+          const double r1 = ramp_value(B.ramp, B.rmin, B.rmax, time + h);
+          const double r0 = ramp_value(B.ramp, B.rmin, B.rmax, time);
+          aa[d] = (r1 - r0) * B.v[d] / h;
+          ab[d] = aa[d];
+          va[d] = r0 * B.v[d];
+          vb[d] = r1 * B.v[d];
+          ua[d] = va[d] * h;
+          ub[d] = vb[d] * h;
+        }
+      }
+    }
+  }
+}
+
+// Explicit::computePredictions (Explicit.C:438-482)
+template <int DIM>
+__global__ __launch_bounds__(256) void k_predict(JobDev *const *jobs, const Task *tasks) {
+  const Task tk = tasks[blockIdx.x];
+  const JobDev *J = jobs[tk.job];
+  if (!J->c.running || J->c.status != 0) return;
+  const int64_t n = tk.first + threadIdx.x;
+  if (n >= tk.end) return;
+  const int64_t N = J->N;
+  const int a = J->c.f0, b = 1 - a;
+  const double h = J->c.dt;
+  const double beta = J->beta, gam = J->gammaCH;
+  double ua[DIM], va[DIM], aa[DIM], ub[DIM], vb[DIM], ab[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    va[d] = J->fv[a][d * N + n];
+    aa[d] = J->fa[a][d * N + n];
+    ub[d] = h * (va[d] + (0.5 - beta) * h * aa[d]);
+    vb[d] = va[d] + (1.0 - gam) * h * aa[d];
+    ab[d] = 0.0;
+  }
+  const bool has_bc = J->bc_ptr[n + 1] > J->bc_ptr[n];
+  if (has_bc) {
+#pragma unroll
+    for (int d = 0; d < DIM; d++) ua[d] = J->fu[a][d * N + n];
+    apply_bcs<DIM>(J, n, J->c.t, h, ua, va, aa, ub, vb, ab);
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      J->fu[a][d * N + n] = ua[d];
+      J->fv[a][d * N + n] = va[d];
+      J->fa[a][d * N + n] = aa[d];
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    J->fu[b][d * N + n] = ub[d];
+    J->fv[b][d * N + n] = vb[d];
+    J->fa[b][d * N + n] = ab[d];
+  }
+}
+
+// Explicit::explicitSolve (Explicit.C:489-536): gather the element forces of
+// the node in element order (same summation order as the reference's
+// scatter-add of Model.C:864-899), a = f/M, Chung-Hulbert correction, BCs,
+// displacement and coordinates.
+template <int DIM, typename FT>
+__global__ __launch_bounds__(256) void k_correct(JobDev *const *jobs, const Task *tasks) {
+  const Task tk = tasks[blockIdx.x];
+  JobDev *J = jobs[tk.job];
+  if (!J->c.running || J->c.status != 0) return;
+  const int64_t n = tk.first + threadIdx.x;
+  if (n >= tk.end) return;
+  const int64_t N = J->N;
+  const int a = J->c.f0, b = 1 - a;
+  const double h = J->c.dt;
+  const FT *ef;
+  if constexpr (sizeof(FT) == 8) ef = J->elemF; else ef = J->elemF32;
+  double f[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; d++) f[d] = 0.0;
+  const int k0 = J->n2e_ptr[n], k1 = J->n2e_ptr[n + 1];
+  for (int k = k0; k < k1; k++) {
+    const int64_t slot = J->n2e_idx[k];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) f[d] += (double)ef[slot * DIM + d];
+  }
+  const double m = J->mass[n];
+  const double alphaM = J->alphaM, gam = J->gammaCH, beta = J->beta;
+  double ua[DIM], va[DIM], aa[DIM], ub[DIM], vb[DIM], ab[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    aa[d] = J->fa[a][d * N + n];
+    ub[d] = J->fu[b][d * N + n];
+    vb[d] = J->fv[b][d * N + n];
+    ab[d] = f[d] / m;
+    ab[d] -= alphaM * aa[d];
+    ab[d] /= (1.0 - alphaM);
+    vb[d] += gam * h * ab[d];
+    ub[d] += beta * (h * h) * ab[d];
+  }
+  const bool has_bc = J->bc_ptr[n + 1] > J->bc_ptr[n];
+  if (has_bc) {
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      ua[d] = J->fu[a][d * N + n];
+      va[d] = J->fv[a][d * N + n];
+    }
+    apply_bcs<DIM>(J, n, J->c.t, h, ua, va, aa, ub, vb, ab);
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      J->fu[a][d * N + n] = ua[d];
+      J->fv[a][d * N + n] = va[d];
+      J->fa[a][d * N + n] = aa[d];
+    }
+  }
+  bool finite = true;
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    J->fu[b][d * N + n] = ub[d];
+    J->fv[b][d * N + n] = vb[d];
+    J->fa[b][d * N + n] = ab[d];
+    J->disp[d * N + n] += ub[d];
+    const double xn = J->x[d * N + n] + ub[d];
+    J->x[d * N + n] = xn;
+    finite = finite && isfinite(xn) && isfinite(vb[d]);
+  }
+  if (!finite) set_status(J, DNLG_E_NONFINITE);
+}
+
+// ---------------------------------------------------------------------------
+// Courant critical time step: per-element L/c then a block minimum (wave
+// shuffles + LDS), one partial per block; k_cycle_end reduces the partials.
+// ---------------------------------------------------------------------------
+template <int DIM>
+__device__ __forceinline__ double element_char_length(const JobDev *J, int64_t e) {
+  const int64_t N = J->N;
+  if constexpr (DIM == 2) {
+    const int32_t *cn = J->conn + e * 4;
+    const int n0 = cn[0], n1 = cn[1], n2 = cn[2], n3 = cn[3];
+    // ElQua4NAx::getCharacteristicLength (ElQua4NAx.C:312-321)
+    const double x31 = J->x[n2] - J->x[n0];
+    const double x24 = J->x[n1] - J->x[n3];
+    const double y42 = J->x[N + n3] - J->x[N + n1];
+    const double y31 = J->x[N + n2] - J->x[N + n0];
+    return ((x31 * y42 + x24 * y31) / sqrt(2.0 * (sqd(x24) + sqd(y42) + sqd(x31) + sqd(y31))));
+  } else {
+    // ElHex8N3D::getCharacteristicLength / getVolume (ElHex8N3D.C:614-684)
+    const int32_t *cn = J->conn + e * 8;
+    double X[8][3];
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+      const int nd = cn[a];
+#pragma unroll
+      for (int c = 0; c < 3; c++) X[a][c] = J->x[c * N + nd];
+    }
+    auto aert = [&](int p1, int p2, int p3, int p4) {
+      const double x1 = X[p1][0], x2 = X[p2][0], x3 = X[p3][0], x4 = X[p4][0];
+      const double y1 = X[p1][1], y2 = X[p2][1], y3 = X[p3][1], y4 = X[p4][1];
+      const double z1 = X[p1][2], z2 = X[p2][2], z3 = X[p3][2], z4 = X[p4][2];
+      return sqrt(4 * (sqd(x1 - x3) + sqd(y1 - y3) + sqd(z1 - z3)) * (sqd(x2 - x4) + sqd(y2 - y4) + sqd(z2 - z4)) -
+                  4 * sqd(-(x2 * x3) + x1 * (x2 - x4) + x3 * x4 + y1 * y2 - y2 * y3 - y1 * y4 + y3 * y4 + z1 * z2 -
+                          z2 * z3 - z1 * z4 + z3 * z4)) /
+             4;
+    };
+    double A1 = aert(0, 1, 2, 3), A2 = aert(0, 1, 5, 4), A3 = aert(1, 2, 6, 5);
+    double A4 = aert(3, 2, 6, 7), A5 = aert(0, 3, 7, 4), A6 = aert(4, 5, 6, 7);
+    A1 = fmax(fmax(A1, A2), A3);
+    A2 = fmax(fmax(A4, A5), A6);
+    A1 = fmax(A1, A2);
+    auto pyramid = [&](int o, int a, int b, int c, int apex) {
+      double va[3], vb[3], vc[3], h[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        va[k] = X[a][k] - X[o][k];
+        vb[k] = X[b][k] - X[o][k];
+        vc[k] = X[c][k] - X[o][k];
+        h[k] = X[apex][k] - X[o][k];
+      }
+      const double n1x = va[1] * vb[2] - va[2] * vb[1], n1y = va[2] * vb[0] - va[0] * vb[2],
+                   n1z = va[0] * vb[1] - va[1] * vb[0];
+      const double n2x = vc[1] * vb[2] - vc[2] * vb[1], n2y = vc[2] * vb[0] - vc[0] * vb[2],
+                   n2z = vc[0] * vb[1] - vc[1] * vb[0];
+      const double base = (sqrt(n1x * n1x + n1y * n1y + n1z * n1z) + sqrt(n2x * n2x + n2y * n2y + n2z * n2z)) / 2.0;
+      double nx = va[1] * vc[2] - va[2] * vc[1], ny = va[2] * vc[0] - va[0] * vc[2], nz = va[0] * vc[1] - va[1] * vc[0];
+      const double ln = sqrt(nx * nx + ny * ny + nz * nz);
+      nx /= ln;
+      ny /= ln;
+      nz /= ln;
+      const double ht = fabs(nx * h[0] + ny * h[1] + nz * h[2]);
+      return base * ht / 3.0;
+    };
+    double vol = 0.0;
+    vol += pyramid(0, 1, 2, 3, 7);
+    vol += pyramid(0, 1, 5, 4, 7);
+    vol += pyramid(1, 2, 6, 5, 7);
+    return vol / A1;
+  }
+}
+
+template <int DIM>
+__global__ __launch_bounds__(256) void k_courant(JobDev *const *jobs, const Task *tasks, int force) {
+  __shared__ double s_min[4];
+  const Task tk = tasks[blockIdx.x];
+  JobDev *J = jobs[tk.job];
+  if (!force) {
+    // Explicit.C:243-251: only while t_{n+1} < t_up_to, every dt_freq increments
+    if (!J->c.running || J->c.status != 0 || !J->c.courant_due) return;
+    if (!(J->c.t + J->c.dt < J->c.up_to)) return;
+  }
+  const int64_t e = tk.first + threadIdx.x;
+  double v = 1.0e300;
+  if (e < tk.end) {
+    const int nip = J->nip;
+    double rho = 0.0;
+    for (int q = 0; q < nip; q++) rho += J->rho[e * nip + q];
+    rho /= nip;
+    const double nu = J->nu;
+    // Element::getElongationWaveSpeed (Element.C:343-359)
+    const double c = sqrt((J->young * (1.0 - nu)) / (rho * (1.0 + nu) * (1.0 - 2.0 * nu)));
+    v = element_char_length<DIM>(J, e) / c;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off));
+  if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = s_min[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmin(m, s_min[w]);
+    J->cmin[tk.blk] = m;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Control kernels (one thread per job).
+// ---------------------------------------------------------------------------
+// Explicit::updateTimes (Explicit.C:414-427) and the per-cycle flags.
+__global__ void k_cycle_begin(JobDev *const *jobs, int n_jobs) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_jobs) return;
+  JobDev *J = jobs[j];
+  Control &c = J->c;
+  c.preloop = 0;
+  c.running = (c.status == 0 && c.t < c.up_to) ? 1 : 0;
+  if (!c.running) return;
+  if (c.t + c.dt > c.up_to) c.dt = c.up_to - c.t;
+  c.inc++;
+  c.courant_due = (c.inc % J->dt_freq == 0) ? 1 : 0;
+  // densities are consumed by the Courant step and read back at the end
+  c.density_due = (c.courant_due || !(c.t + c.dt < c.up_to)) ? 1 : 0;
+}
+
+// Explicit::endStep (Explicit.C:539-552) + Solver::computeTimeStep
+// (Solver.C:196-236, Courant branch: omega_max = 2 / dt_crit).
+__global__ void k_cycle_end(JobDev *const *jobs, int n_jobs) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_jobs) return;
+  JobDev *J = jobs[j];
+  Control &c = J->c;
+  if (!c.running || c.status != 0) return;
+  c.t += c.dt;
+  c.f0 = 1 - c.f0;
+  if (c.courant_due && c.t < c.up_to) {
+    double m = J->cmin[0];
+    for (int k = 1; k < J->n_cmin; k++) m = fmin(m, J->cmin[k]);
+    const double fmax = 2.0 / m;
+    c.dt = J->safety * J->omegaS / fmax;
+  }
+}
+
+// Host-loop snapshot: t, dt, up_to, status, increment of every job.
+__global__ void k_snapshot(JobDev *const *jobs, int n_jobs, double *out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_jobs) return;
+  const Control &c = jobs[j]->c;
+  out[5 * j + 0] = c.t;
+  out[5 * j + 1] = c.dt;
+  out[5 * j + 2] = c.up_to;
+  out[5 * j + 3] = (double)c.status;
+  out[5 * j + 4] = (double)c.inc;
+}
+
+// Forced time step of the pre-loop (Explicit.C:152-154).
+__global__ void k_force_dt(JobDev *const *jobs, int n_jobs) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_jobs) return;
+  JobDev *J = jobs[j];
+  double m = J->cmin[0];
+  for (int k = 1; k < J->n_cmin; k++) m = fmin(m, J->cmin[k]);
+  const double fmax = 2.0 / m;
+  J->c.dt = J->safety * J->omegaS / fmax;
+}
+
+// ---------------------------------------------------------------------------
+// Pre-loop: reference Jacobian -> detJ0 (Element3D.C:112-113,
+// ElementPlane.C:92-100) and the element lumped masses
+// (Element::computeMassMatrix, Element.C:304-339) into elemF's first slot.
+// ---------------------------------------------------------------------------
+template <int NN, int DIM>
+__global__ __launch_bounds__(256) void k_prep(JobDev *const *jobs, const Task *tasks) {
+  const Task tk = tasks[blockIdx.x];
+  JobDev *J = jobs[tk.job];
+  const int64_t e = tk.first + threadIdx.x;
+  if (e >= tk.end) return;
+  const int64_t N = J->N;
+  double X[NN][DIM];
+#pragma unroll
+  for (int a = 0; a < NN; a++) {
+    const int nd = J->conn[e * NN + a];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) X[a][d] = J->x[d * N + nd];
+  }
+  double me[NN];
+#pragma unroll
+  for (int a = 0; a < NN; a++) me[a] = 0.0;
+  bool neg = false;
+  for (int p = 0; p < NN; p++) {
+    double Jm[DIM][DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; i++)
+#pragma unroll
+      for (int jj = 0; jj < DIM; jj++) Jm[i][jj] = 0.0;
+#pragma unroll
+    for (int a = 0; a < NN; a++)
+#pragma unroll
+      for (int i = 0; i < DIM; i++)
+#pragma unroll
+        for (int jj = 0; jj < DIM; jj++) Jm[i][jj] += tabD<NN>(p, a, jj) * X[a][i];
+    double det;
+    if constexpr (DIM == 3) {
+      const double *dd = &Jm[0][0];
+      det = dd[1] * dd[5] * dd[6] - dd[2] * dd[4] * dd[6] + dd[2] * dd[3] * dd[7] - dd[0] * dd[5] * dd[7] -
+            dd[1] * dd[3] * dd[8] + dd[0] * dd[4] * dd[8];
+    } else {
+      det = Jm[0][0] * Jm[1][1] - Jm[1][0] * Jm[0][1];
+    }
+    neg = neg || (det < 0.0);
+    double wdj = det;
+    double det0 = det;
+    if constexpr (DIM == 2) {
+      double r = 0.0;
+#pragma unroll
+      for (int a = 0; a < NN; a++) r += tabN<NN>(p, a) * X[a][0];
+      det0 = det * r;
+      wdj *= k2PI * r;
+    }
+    J->detJ0[e * NN + p] = det0;
+    if (J->detJ0f) J->detJ0f[e * NN + p] = (float)det0;
+#pragma unroll
+    for (int a = 0; a < NN; a++) me[a] += J->rho0 * tabN<NN>(p, a) * wdj;
+  }
+  if (neg) set_status(J, DNLG_E_NEG_JACOBIAN);
+#pragma unroll
+  for (int a = 0; a < NN; a++) J->elemF[(e * NN + a) * DIM] = me[a];
+}
+
+// Model::computeMassMatrix assembly (Model.C:537-590), gathered per node in
+// element order.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_mass_gather(JobDev *const *jobs, const Task *tasks) {
+  const Task tk = tasks[blockIdx.x];
+  JobDev *J = jobs[tk.job];
+  const int64_t n = tk.first + threadIdx.x;
+  if (n >= tk.end) return;
+  double m = 0.0;
+  for (int k = J->n2e_ptr[n]; k < J->n2e_ptr[n + 1]; k++) m += J->elemF[(int64_t)J->n2e_idx[k] * DIM];
+  J->mass[n] = m;
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launchers
+// ---------------------------------------------------------------------------
+void upload_tables(const double *hexN, const double *hexD, const double *quaN, const double *quaD) {
+  hipMemcpyToSymbol(HIP_SYMBOL(c_hexN), hexN, sizeof(double) * 64);
+  hipMemcpyToSymbol(HIP_SYMBOL(c_hexD), hexD, sizeof(double) * 192);
+  hipMemcpyToSymbol(HIP_SYMBOL(c_quaN), quaN, sizeof(double) * 16);
+  hipMemcpyToSymbol(HIP_SYMBOL(c_quaD), quaD, sizeof(double) * 32);
+}
+
+int elem_block_threads(int type) { return type == DNLG_EL_HEX8N_3D ? 128 : 256; }
+
+void launch_elem_fp64(int type, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s) {
+  if (nblk <= 0) return;
+  if (type == DNLG_EL_HEX8N_3D)
+    hipLaunchKernelGGL((k_elem_fp64<8, 3, 128>), dim3(nblk), dim3(128), 0, s, jobs, tasks);
+  else
+    hipLaunchKernelGGL((k_elem_fp64<4, 2, 256>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
+}
+
+void launch_predict(int dim, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s) {
+  if (nblk <= 0) return;
+  if (dim == 3)
+    hipLaunchKernelGGL(k_predict<3>, dim3(nblk), dim3(256), 0, s, jobs, tasks);
+  else
+    hipLaunchKernelGGL(k_predict<2>, dim3(nblk), dim3(256), 0, s, jobs, tasks);
+}
+
+void launch_correct(int dim, bool f32, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s) {
+  if (nblk <= 0) return;
+  if (dim == 3) {
+    if (f32) hipLaunchKernelGGL((k_correct<3, float>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
+    else hipLaunchKernelGGL((k_correct<3, double>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
+  } else {
+    if (f32) hipLaunchKernelGGL((k_correct<2, float>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
+    else hipLaunchKernelGGL((k_correct<2, double>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
+  }
+}
+
+void launch_courant(int dim, JobDev *const *jobs, const Task *tasks, int nblk, int force, hipStream_t s) {
+  if (nblk <= 0) return;
+  if (dim == 3)
+    hipLaunchKernelGGL(k_courant<3>, dim3(nblk), dim3(256), 0, s, jobs, tasks, force);
+  else
+    hipLaunchKernelGGL(k_courant<2>, dim3(nblk), dim3(256), 0, s, jobs, tasks, force);
+}
+
+void launch_cycle_begin(JobDev *const *jobs, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_cycle_begin, dim3((n + 63) / 64), dim3(64), 0, s, jobs, n);
+}
+void launch_cycle_end(JobDev *const *jobs, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_cycle_end, dim3((n + 63) / 64), dim3(64), 0, s, jobs, n);
+}
+void launch_snapshot(JobDev *const *jobs, int n, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_snapshot, dim3((n + 63) / 64), dim3(64), 0, s, jobs, n, out);
+}
+void launch_force_dt(JobDev *const *jobs, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_force_dt, dim3((n + 63) / 64), dim3(64), 0, s, jobs, n);
+}
+void launch_prep(int type, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s) {
+  if (nblk <= 0) return;
+  if (type == DNLG_EL_HEX8N_3D)
+    hipLaunchKernelGGL((k_prep<8, 3>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
+  else
+    hipLaunchKernelGGL((k_prep<4, 2>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
+}
+void launch_mass_gather(int dim, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s) {
+  if (nblk <= 0) return;
+  if (dim == 3)
+    hipLaunchKernelGGL(k_mass_gather<3>, dim3(nblk), dim3(256), 0, s, jobs, tasks);
+  else
+    hipLaunchKernelGGL(k_mass_gather<2>, dim3(nblk), dim3(256), 0, s, jobs, tasks);
+}
+
+}  // namespace dnlg

@@ -1,0 +1,694 @@
+"""dnlPython -- drop-in model-building API of DynELA's explicit solve, run on
+the MI355X engine (libdnlg.so).
+
+Mirrors the class and method names of the reference's SWIG module
+(Sources/dnlPython/dnlPython.i:10-25, dnlFEM.inc:11-35): ``DynELA``,
+``NodeSet``, ``ElementSet``, ``Material``, ``JohnsonCookLaw``,
+``ElastoplasticLaw``, ``BoundaryRestrain``, ``BoundarySpeed``,
+``RampFunction``, ``Explicit``, ``Field``, ``Element``.  ``model.solve()``
+replaces ``Explicit::solve`` (Sources/dnlFEM/Explicit.C:126-253) with the
+GPU engine.  Unlike the reference (one global DynELA object,
+Sources/dnlFEM/DynELA.C:21,47-52) any number of models may coexist, and bulk
+numpy ingestion (``createNodes`` / ``createElements``) replaces the O(N^2)
+per-entity build path (Model.C:91,122) for large meshes.
+
+Errors never exit the interpreter: engine statuses become exceptions
+(``NegativeJacobianError``, ``NewtonRaphsonError``, ``NonFiniteError``,
+``DnlgError``).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import (DnlgError, NegativeJacobianError, NewtonRaphsonError,  # noqa: F401
+                   NonFiniteError)
+
+__all__ = ["DynELA", "NodeSet", "ElementSet", "Material", "JohnsonCookLaw", "ElastoplasticLaw",
+           "VoceLinearLaw", "BoundaryRestrain", "BoundarySpeed", "RampFunction", "Explicit", "Field",
+           "Element", "Vec3D", "Context", "DnlgError", "NegativeJacobianError", "NewtonRaphsonError",
+           "NonFiniteError", "batch_solve"]
+
+
+class Element:
+    """Element type names of the reference enum (Element.h:62-71)."""
+    ElGeneric = 0
+    ElQua4N2D = 1
+    ElTri3N2D = 2
+    ElQua4NAx = _lib.EL_QUA4N_AX
+    ElHex8N3D = _lib.EL_HEX8N_3D
+    ElTet4N3D = 5
+    ElTet10N3D = 6
+
+
+NODES_PER_ELEMENT = {Element.ElQua4NAx: 4, Element.ElHex8N3D: 8}
+
+
+class Field:
+    """Field names of Field.h:46-79 that the engine can return."""
+    plasticStrain = "plasticStrain"
+    plasticStrainRate = "plasticStrainRate"
+    gamma = "gamma"
+    gammaCumulate = "gammaCumulate"
+    yieldStress = "yieldStress"
+    T = "T"
+    density = "density"
+    pressure = "pressure"
+    internalEnergy = "internalEnergy"
+    inelasticEnergy = "inelasticEnergy"
+    vonMises = "vonMises"
+    Stress = "Stress"
+    Strain = "Strain"
+    PlasticStrain = "PlasticStrain"
+    coords = "coords"
+    coordsX = "coordsX"
+    coordsY = "coordsY"
+    coordsZ = "coordsZ"
+    disp = "disp"
+    speed = "speed"
+    kineticEnergy = "kineticEnergy"
+    timeStep = "timeStep"
+    mass = "mass"
+
+
+_IP_FIELDS = {
+    Field.plasticStrain: _lib.F_PLASTIC_STRAIN, Field.plasticStrainRate: _lib.F_PLASTIC_STRAIN_RATE,
+    Field.gamma: _lib.F_GAMMA, Field.gammaCumulate: _lib.F_GAMMA_CUMULATE,
+    Field.yieldStress: _lib.F_YIELD_STRESS, Field.T: _lib.F_TEMPERATURE, Field.density: _lib.F_DENSITY,
+    Field.pressure: _lib.F_PRESSURE, Field.internalEnergy: _lib.F_INTERNAL_ENERGY,
+    Field.inelasticEnergy: _lib.F_INELASTIC_ENERGY, Field.vonMises: _lib.F_VON_MISES,
+    Field.Stress: _lib.F_STRESS, Field.Strain: _lib.F_STRAIN,
+    Field.PlasticStrain: _lib.F_PLASTIC_STRAIN_TENSOR,
+}
+
+
+class Vec3D:
+    def __init__(self, x=0.0, y=0.0, z=0.0):
+        self._v = [float(x), float(y), float(z)]
+
+    def __call__(self, i):
+        return self._v[i]
+
+    def __getitem__(self, i):
+        return self._v[i]
+
+    def __repr__(self):
+        return f"Vec3D({self._v[0]}, {self._v[1]}, {self._v[2]})"
+
+
+class NodeSet:
+    def __init__(self, name=""):
+        self.name = name
+        self._numbers = []
+
+    def numbers(self):
+        return np.array(self._numbers, dtype=np.int64)
+
+
+class ElementSet:
+    def __init__(self, name=""):
+        self.name = name
+        self._numbers = []
+
+
+class JohnsonCookLaw:
+    """JohnsonCookLaw::setParameters(A, B, C, n, m, depsp0, Tm, T0)
+    (JohnsonCookLaw.h:77)."""
+    law = _lib.LAW_JOHNSON_COOK
+
+    def __init__(self, name=""):
+        self.name = name
+        self.p = [0.0] * 8
+
+    def setParameters(self, A, B, C, n, m, depsp0, Tm, T0):
+        self.p = [float(A), float(B), float(C), float(n), float(m), float(depsp0), float(Tm), float(T0)]
+
+
+class ElastoplasticLaw:
+    """ElastoplasticLaw::setParameters(A, B, n) (ElastoplasticLaw.h:46)."""
+    law = _lib.LAW_POWER
+
+    def __init__(self, name=""):
+        self.name = name
+        self.p = [0.0] * 8
+
+    def setParameters(self, A, B, n):
+        self.p = [float(A), float(B), float(n)] + [0.0] * 5
+
+
+class VoceLinearLaw:
+    """sigma_y = A + Q (1 - exp(-b ep)) + H ep; no reference counterpart."""
+    law = _lib.LAW_VOCE_LINEAR
+
+    def __init__(self, name=""):
+        self.name = name
+        self.p = [0.0] * 8
+
+    def setParameters(self, A, Q, b, H):
+        self.p = [float(A), float(Q), float(b), float(H)] + [0.0] * 4
+
+
+class Material:
+    """Material attributes of Material.h:49-56."""
+
+    def __init__(self, name=""):
+        self.name = name
+        self.youngModulus = 0.0
+        self.poissonRatio = 0.0
+        self.density = 0.0
+        self.heatCapacity = 0.0
+        self.taylorQuinney = 0.0
+        self.T0 = 0.0
+        self.dilatation = 0.0
+        self.thermalConductivity = 0.0
+        self._law = None
+
+    def setHardeningLaw(self, law):
+        self._law = law
+
+    def getHardeningLaw(self):
+        return self._law
+
+    def getBulkModulus(self):
+        return self.youngModulus / (3.0 * (1.0 - 2.0 * self.poissonRatio))
+
+    def getShearModulus(self):
+        return self.youngModulus / (2.0 * (1.0 + self.poissonRatio))
+
+    def descriptor(self):
+        if self._law is None:
+            raise DnlgError(_lib.E_INVALID, f"material {self.name!r} has no hardening law")
+        d = _lib.MaterialDesc()
+        d.young, d.poisson, d.density = self.youngModulus, self.poissonRatio, self.density
+        d.heat_capacity, d.taylor_quinney, d.T0 = self.heatCapacity, self.taylorQuinney, self.T0
+        d.law = self._law.law
+        for k in range(8):
+            d.p[k] = self._law.p[k]
+        return d
+
+
+class RampFunction:
+    """RampFunction (RampFunction.h:31-39): setFunction(type, min, max)."""
+    Unknown, Linear, Sinus, Constant, Erf, Soft = range(6)
+
+    def __init__(self, name=""):
+        self.name = name
+        self.type = self.Unknown
+        self.min = 0.0
+        self.max = 1.0
+
+    def setFunction(self, type_, mn, mx):
+        self.type, self.min, self.max = int(type_), float(mn), float(mx)
+
+    set = setFunction
+
+
+class _Boundary:
+    kind = None
+
+    def __init__(self, name=""):
+        self.name = name
+        self.value = [0.0, 0.0, 0.0]
+        self.function = None
+
+    def setValue(self, x, y=None, z=None):
+        if y is None:
+            x, y, z = x(0), x(1), x(2)
+        self.value = [float(x), float(y), float(z)]
+
+    def getValue(self):
+        return Vec3D(*self.value)
+
+    def setFunction(self, function):
+        self.function = function
+
+    def descriptor(self):
+        b = _lib.BoundaryDesc()
+        b.kind = self.kind
+        for k in range(3):
+            b.value[k] = self.value[k]
+        if self.function is not None and self.kind == _lib.BC_SPEED:
+            b.ramp, b.ramp_min, b.ramp_max = self.function.type, self.function.min, self.function.max
+        return b
+
+
+class BoundaryRestrain(_Boundary):
+    """BoundaryRestrain (BoundaryRestrain.h:49): setValue(rx, ry, rz)."""
+    kind = _lib.BC_RESTRAIN
+
+
+class BoundarySpeed(_Boundary):
+    """BoundarySpeed (BoundarySpeed.h:49): setValue(vx, vy, vz), setFunction."""
+    kind = _lib.BC_SPEED
+
+
+class Explicit:
+    """Explicit solver settings (Solver.h:73-78, Explicit.h:56).  Defaults are
+    the reference's: Courant step every 10 increments, safety 0.9,
+    rho_b = 0.8182 (Solver.h:30-33, configuration.dnl:9-11, Explicit.h:16).
+    ``setPrecision("mixed")`` selects the FP32-force / FP64-kinematics path."""
+    Courant, PowerIteration, PowerIterationUnder = 0, 1, 2
+
+    def __init__(self, name=""):
+        self.name = name
+        self.startTime = 0.0
+        self.endTime = 0.0
+        self.timeStepMethod = self.Courant
+        self.computeTimeStepFrequency = 10
+        self.timeStepSafetyFactor = 0.9
+        self.dissipation = 0.8182
+        self.precision = _lib.PREC_FP64
+        self.trackTensors = True
+        self.model = None
+
+    def setTimes(self, start, end):
+        self.startTime, self.endTime = float(start), float(end)
+
+    def setTimeStepMethod(self, method):
+        self.timeStepMethod = int(method)
+
+    def setComputeTimeStepFrequency(self, frequency):
+        self.computeTimeStepFrequency = int(frequency)
+
+    def setTimeStepSafetyFactor(self, factor):
+        self.timeStepSafetyFactor = float(factor)
+
+    def setDissipation(self, rho_b):
+        if rho_b < 0.0 or rho_b > 1.0:
+            raise DnlgError(_lib.E_INVALID, "dissipation value out of range [0.0, 1.0]")
+        self.dissipation = float(rho_b)
+
+    def setPrecision(self, precision):
+        p = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED}.get(str(precision).lower(), precision)
+        if p not in (_lib.PREC_FP64, _lib.PREC_MIXED):
+            raise DnlgError(_lib.E_INVALID, f"unknown precision {precision!r}")
+        self.precision = p
+
+    def setTrackTensors(self, flag):
+        self.trackTensors = bool(flag)
+
+    def getTimeStep(self):
+        return self.model.getTimeStep() if self.model else 0.0
+
+    def params(self):
+        p = _lib.SolverParams()
+        p.t_start, p.t_end = self.startTime, self.endTime
+        p.rho_b, p.safety = self.dissipation, self.timeStepSafetyFactor
+        p.dt_method, p.dt_freq = self.timeStepMethod, self.computeTimeStepFrequency
+        p.stress_method = _lib.STRESS_NR
+        p.precision = self.precision
+        p.track_tensors = 1 if self.trackTensors else 0
+        return p
+
+
+class Context:
+    """One engine context (device + stream).  ``Context.default()`` is shared
+    by models that do not pass their own."""
+    _default = {}
+
+    def __init__(self, device=0):
+        self.lib = _lib.load()
+        self.handle = ctypes.c_void_p()
+        _lib.check(self.lib.dnlg_ctx_create(int(device), ctypes.byref(self.handle)))
+        self.device = device
+
+    @classmethod
+    def default(cls, device=0):
+        if device not in cls._default:
+            cls._default[device] = cls(device)
+        return cls._default[device]
+
+    def check(self, status):
+        return _lib.check(status, self.handle)
+
+    def set_timing(self, on):
+        self.check(self.lib.dnlg_ctx_set_timing(self.handle, 1 if on else 0))
+
+    def reset_timing(self):
+        self.check(self.lib.dnlg_ctx_reset_timing(self.handle))
+
+    def kernel_time(self, name):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        self.check(self.lib.dnlg_ctx_kernel_time(self.handle, name.encode(), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def close(self):
+        if self.handle:
+            self.lib.dnlg_ctx_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+
+class _NodeView:
+    """Node proxy for ``getNodeByNum`` (coords/disp/speed as Vec3D)."""
+
+    def __init__(self, model, number):
+        self._m, self.number = model, number
+
+    def _row(self, what):
+        arr = self._m._node_array(what)
+        return Vec3D(*arr[:, self._m._node_index(self.number)])
+
+    @property
+    def coords(self):
+        return self._row("coords")
+
+    @property
+    def disp(self):
+        return self._row("disp")
+
+    @property
+    def speed(self):
+        return self._row("speed")
+
+    @property
+    def mass(self):
+        return float(self._m.getNodalMasses()[self._m._node_index(self.number)])
+
+
+class _Parallel:
+    """Parallel (Parallel.h:45-80): core count is irrelevant on the GPU."""
+
+    def setCores(self, n):
+        self.cores = int(n)
+
+
+class DynELA:
+    """Model facade (DynELA.h:40-111)."""
+
+    def __init__(self, name="", context=None):
+        self.name = name
+        self._ctx = context
+        self._node_num = []
+        self._node_xyz = []
+        self._elem_num = []
+        self._elem_conn = []
+        self._default_element = None
+        self._elem_type = None
+        self._material = None
+        self._solver = None
+        self._initial = []   # (boundary, node numbers) in attach order
+        self._constant = []
+        self._job = None
+        self._arrays = None
+        self.parallel = _Parallel()
+        self.saveTimes = None
+        self._stress_method = "NewtonRaphson"
+
+    # ---- nodes and elements -------------------------------------------------
+    def createNode(self, number, x, y=None, z=None):
+        if y is None:
+            x, y, z = x(0), x(1), x(2)
+        self._node_num.append(np.array([int(number)], dtype=np.int64))
+        self._node_xyz.append(np.array([[float(x), float(y), float(z)]], dtype=np.float64))
+        self._invalidate()
+        return True
+
+    def createNodes(self, numbers, xyz):
+        """Bulk node ingestion: numbers [N], xyz [N, 3] (or [N, 2])."""
+        xyz = np.asarray(xyz, dtype=np.float64)
+        if xyz.shape[1] == 2:
+            xyz = np.concatenate([xyz, np.zeros((xyz.shape[0], 1))], axis=1)
+        self._node_num.append(np.asarray(numbers, dtype=np.int64).ravel())
+        self._node_xyz.append(xyz)
+        self._invalidate()
+        return True
+
+    def setDefaultElement(self, type_):
+        self._default_element = int(type_)
+
+    def createElement(self, number, *nodes):
+        if self._default_element is None:
+            raise DnlgError(_lib.E_INVALID, "setDefaultElement must be called first")
+        self._check_type(self._default_element)
+        if len(nodes) == 1 and hasattr(nodes[0], "__len__"):
+            nodes = tuple(nodes[0])
+        if len(nodes) != NODES_PER_ELEMENT[self._default_element]:
+            raise DnlgError(_lib.E_INVALID, f"element {number} needs "
+                            f"{NODES_PER_ELEMENT[self._default_element]} nodes")
+        self._elem_num.append(np.array([int(number)], dtype=np.int64))
+        self._elem_conn.append(np.array([nodes], dtype=np.int64))
+        self._invalidate()
+        return True
+
+    def createElements(self, numbers, conn):
+        """Bulk element ingestion with node numbers: conn [NE, nodes]."""
+        if self._default_element is None:
+            raise DnlgError(_lib.E_INVALID, "setDefaultElement must be called first")
+        self._check_type(self._default_element)
+        conn = np.asarray(conn, dtype=np.int64)
+        if conn.ndim != 2 or conn.shape[1] != NODES_PER_ELEMENT[self._default_element]:
+            raise DnlgError(_lib.E_INVALID, "connectivity has the wrong shape")
+        self._elem_num.append(np.asarray(numbers, dtype=np.int64).ravel())
+        self._elem_conn.append(conn)
+        self._invalidate()
+        return True
+
+    def _check_type(self, t):
+        if t not in NODES_PER_ELEMENT:
+            raise DnlgError(_lib.E_INVALID, f"element type {t} is not supported by the engine")
+        if self._elem_type is not None and self._elem_type != t:
+            raise DnlgError(_lib.E_INVALID, "a model holds a single element family (Model.C:328-357)")
+        self._elem_type = t
+
+    def getNodesNumber(self):
+        return int(sum(len(a) for a in self._node_num))
+
+    def getElementsNumber(self):
+        return int(sum(len(a) for a in self._elem_num))
+
+    # ---- sets, materials, BCs, solver ---------------------------------------
+    def add(self, obj, *args):
+        if isinstance(obj, NodeSet):
+            obj._numbers.extend(self._range(*args))
+        elif isinstance(obj, ElementSet):
+            obj._numbers.extend(self._range(*args))
+        elif isinstance(obj, Material):
+            self._material = obj
+        elif isinstance(obj, Explicit):
+            self._solver = obj
+            obj.model = self
+        else:
+            # HistoryFile and other outputs are not part of the engine
+            pass
+        self._invalidate()
+
+    @staticmethod
+    def _range(start=-1, end=-1, increment=1):
+        """Model::add(set, start, end, increment) (Model.C:181-233)."""
+        if start == -1:
+            return []
+        if end == -1:
+            return [int(start)]
+        return list(range(int(start), int(end) + 1, int(increment)))
+
+    def addMaterial(self, material):
+        self._material = material
+
+    def attachInitialBC(self, boundary, nodeSet):
+        self._initial.append((boundary, nodeSet.numbers()))
+        self._invalidate()
+
+    def attachConstantBC(self, boundary, nodeSet):
+        self._constant.append((boundary, nodeSet.numbers()))
+        self._invalidate()
+
+    def setSaveTimes(self, start, end, increment):
+        self.saveTimes = (float(start), float(end), float(increment))
+
+    def setStressComputation(self, method):
+        if method != "NewtonRaphson":
+            raise DnlgError(_lib.E_INVALID, "only the NewtonRaphson stress integration is implemented")
+        self._stress_method = method
+
+    # ---- assembly of the engine descriptors -----------------------------------
+    def _invalidate(self):
+        if self._job is not None:
+            raise DnlgError(_lib.E_STATE, "the model was modified after initSolve/solve")
+        self._arrays = None
+
+    def _node_tables(self):
+        nums = np.concatenate(self._node_num) if self._node_num else np.zeros(0, np.int64)
+        xyz = np.concatenate(self._node_xyz) if self._node_xyz else np.zeros((0, 3))
+        order = np.argsort(nums, kind="stable")
+        nums, xyz = nums[order], xyz[order]
+        if len(nums) > 1 and np.any(nums[1:] == nums[:-1]):
+            raise DnlgError(_lib.E_INVALID, "duplicate node number")
+        return nums, xyz
+
+    def _index_of(self, sorted_nums, numbers, what):
+        idx = np.searchsorted(sorted_nums, numbers)
+        idx = np.clip(idx, 0, max(len(sorted_nums) - 1, 0))
+        if len(numbers) and (len(sorted_nums) == 0 or np.any(sorted_nums[idx] != numbers)):
+            raise DnlgError(_lib.E_INVALID, f"{what} does not exist in the model")
+        return idx
+
+    def descriptors(self):
+        """Flat engine descriptors of the model (ModelArrays)."""
+        if self._arrays is not None:
+            return self._arrays
+        if self._elem_type is None or not self._elem_num:
+            raise DnlgError(_lib.E_INVALID, "model has no elements")
+        if self._material is None:
+            raise DnlgError(_lib.E_INVALID, "no material attached to the elements")
+        if self._solver is None:
+            raise DnlgError(_lib.E_INVALID, "no solver added to the model")
+        nums, xyz = self._node_tables()
+        enums = np.concatenate(self._elem_num)
+        econn = np.concatenate(self._elem_conn)
+        order = np.argsort(enums, kind="stable")
+        enums, econn = enums[order], econn[order]
+        conn = self._index_of(nums, econn.ravel(), "element node").reshape(econn.shape).astype(np.int32)
+        n = len(nums)
+        bnds, lists = [], []
+        for which in (self._initial, self._constant):
+            rows, keys, vals = [], [], []
+            for k, (b, members) in enumerate(which):
+                bi = len(bnds)
+                bnds.append(b.descriptor())
+                idx = self._index_of(nums, members, "boundary node")
+                rows.append(idx)
+                keys.append(np.full(len(idx), k, dtype=np.int64))
+                vals.append(np.full(len(idx), bi, dtype=np.int32))
+            if rows:
+                r, kk, v = np.concatenate(rows), np.concatenate(keys), np.concatenate(vals)
+                o = np.lexsort((kk, r))
+                ptr = np.zeros(n + 1, dtype=np.int64)
+                np.add.at(ptr, r + 1, 1)
+                lists.append((np.cumsum(ptr), v[o]))
+            else:
+                lists.append(None)
+        self._arrays = _lib.ModelArrays(self._elem_type, xyz.T.copy(), conn, self._material.descriptor(), bnds,
+                                        lists[0], lists[1], self._solver.params())
+        self._node_sorted = nums
+        return self._arrays
+
+    # ---- engine -------------------------------------------------------------
+    @property
+    def context(self):
+        if self._ctx is None:
+            self._ctx = Context.default()
+        return self._ctx
+
+    def initSolve(self):
+        """Model::initSolve (Model.C:360-521): creates the engine job."""
+        if self._job is not None:
+            return True
+        a = self.descriptors()
+        ctx = self.context
+        job = ctypes.c_void_p()
+        ctx.check(ctx.lib.dnlg_job_create(ctx.handle, ctypes.byref(a.mesh), ctypes.byref(a.material),
+                                          ctypes.byref(a.bcs), ctypes.byref(a.params), ctypes.byref(job)))
+        self._job = job
+        return True
+
+    def solve(self, upTo=None):
+        """DynELA::solve (DynELA.C:727-772) / Model::solve(upTo)."""
+        self.initSolve()
+        t = self._solver.endTime if upTo is None or upTo < 0 else float(upTo)
+        ctx = self.context
+        ctx.check(ctx.lib.dnlg_job_solve(self._job, ctypes.c_double(t)))
+        return True
+
+    def close(self):
+        if self._job is not None:
+            self.context.lib.dnlg_job_destroy(self._job)
+            self._job = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- results --------------------------------------------------------------
+    def _require_job(self):
+        if self._job is None:
+            raise DnlgError(_lib.E_STATE, "call initSolve() or solve() first")
+
+    def _node_array(self, what):
+        self._require_job()
+        n = self.getNodesNumber()
+        out = np.zeros((3, n))
+        ptrs = {"coords": None, "disp": None, "speed": None}
+        ptrs[what] = _lib.as_dp(out)
+        self.context.check(self.context.lib.dnlg_job_read_nodes(self._job, ptrs["coords"], ptrs["disp"],
+                                                                ptrs["speed"]))
+        return out
+
+    def _node_index(self, number):
+        if self._arrays is None:
+            self.descriptors()
+        return int(self._index_of(self._node_sorted, np.array([number]), "node")[0])
+
+    def getNodalCoordinates(self):
+        """[3, N] current coordinates, nodes in increasing number order."""
+        return self._node_array("coords")
+
+    def getNodalDisplacements(self):
+        return self._node_array("disp")
+
+    def getNodalSpeeds(self):
+        return self._node_array("speed")
+
+    def getNodalMasses(self):
+        self._require_job()
+        out = np.zeros(self.getNodesNumber())
+        self.context.check(self.context.lib.dnlg_job_read_mass(self._job, _lib.as_dp(out)))
+        return out
+
+    def getIntPointField(self, field):
+        """[n_elems, n_ip] (scalars) or [6, n_elems, n_ip] (tensors)."""
+        self._require_job()
+        f = _IP_FIELDS[field] if isinstance(field, str) else int(field)
+        ne = self.getElementsNumber()
+        nip = NODES_PER_ELEMENT[self._elem_type]
+        ncomp = 6 if f in _lib.TENSOR_FIELDS else 1
+        out = np.zeros(ncomp * ne * nip)
+        self.context.check(self.context.lib.dnlg_job_read_ip(self._job, f, _lib.as_dp(out)))
+        return out.reshape((ncomp, ne, nip) if ncomp == 6 else (ne, nip))
+
+    def getScalars(self):
+        self._require_job()
+        t, dt, inc = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+        self.context.check(self.context.lib.dnlg_job_scalars(self._job, ctypes.byref(t), ctypes.byref(dt),
+                                                             ctypes.byref(inc)))
+        return t.value, dt.value, inc.value
+
+    def getCurrentTime(self):
+        return self.getScalars()[0]
+
+    def getTimeStep(self):
+        return self.getScalars()[1]
+
+    def getIncrement(self):
+        return self.getScalars()[2]
+
+    def getEnergies(self):
+        """[kinetic, internal, dissipated plastic work, kinetic + internal]."""
+        self._require_job()
+        out = np.zeros(4)
+        self.context.check(self.context.lib.dnlg_job_energies(self._job, _lib.as_dp(out)))
+        return out
+
+    def getTotalKineticEnergy(self):
+        return float(self.getEnergies()[0])
+
+    def getTotalMass(self):
+        return float(self.getNodalMasses().sum())
+
+    def getNodeByNum(self, number):
+        return _NodeView(self, number)
+
+
+def batch_solve(models, upTo=None, context=None):
+    """Advance several models together (one launch per phase and cycle)."""
+    ctx = context or models[0].context
+    for m in models:
+        if m.context is not ctx:
+            raise DnlgError(_lib.E_INVALID, "all models of a batch must share one context")
+        m.initSolve()
+    n = len(models)
+    handles = (ctypes.c_void_p * n)(*[m._job.value for m in models])
+    t = np.array([m._solver.endTime if upTo is None else float(upTo) for m in models], dtype=np.float64)
+    ctx.check(ctx.lib.dnlg_batch_solve(ctx.handle, handles, n, _lib.as_dp(t)))

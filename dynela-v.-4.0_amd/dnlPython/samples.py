@@ -1,0 +1,222 @@
+"""Model builders for the benchmark workloads (bulk numpy ingestion).
+
+Each builder returns a ready ``DynELA`` model (not yet solved).  Units are
+SI (m, kg, s, Pa) with temperatures in degrees Celsius for the Johnson-Cook
+law.  Node and element numbering follows the reference samples: nodes
+row-major from 1, elements from 1 (Samples/Taylor/Taylor-Axi/Taylor.py:43-67).
+
+Workloads
+  taylor_axi   2D axisymmetric Taylor impact, 4-node quads (10 x 50)
+  taylor_3d    3D Taylor impact, full cylinder of 8-node hexes
+               (n x n cross-section mapped to the disc, nz layers)
+  necking_3d   3D round bar in tension with a 2 % mid-length taper,
+               Voce + linear hardening, one end fixed, the other pulled
+  sweep_params seeded parameter sweep around the copper impact
+"""
+import numpy as np
+
+from . import (BoundaryRestrain, BoundarySpeed, DynELA, Element, ElementSet, Explicit, JohnsonCookLaw,
+               Material, NodeSet, VoceLinearLaw)
+
+# Copper-like Johnson-Cook set of the impact workloads; Tm, T0, Cp and eta
+# are not given with the impact configuration and are copper handbook values.
+COPPER = dict(young=124e9, poisson=0.34, density=8930.0, A=90e6, B=292e6, C=0.025, n=0.31, m=1.09,
+              depsp0=1.0, Tm=1083.0, T0=20.0, heatCapacity=383.0, taylorQuinney=0.9)
+
+# Necking bar: Voce + linear law sigma_y = 80 + 300 (1 - exp(-16.93 p)) + 129.24 p MPa;
+# elastic and thermal constants are steel-like values chosen for this workload.
+NECKING = dict(young=206e9, poisson=0.3, density=7830.0, A=80e6, Q=300e6, b=16.93, H=129.24e6,
+               T0=20.0, heatCapacity=460.0, taylorQuinney=0.9)
+
+
+def _jc_material(p):
+    law = JohnsonCookLaw("JC")
+    law.setParameters(p["A"], p["B"], p["C"], p["n"], p["m"], p["depsp0"], p["Tm"], p["T0"])
+    mat = Material("copper")
+    mat.setHardeningLaw(law)
+    mat.youngModulus = p["young"]
+    mat.poissonRatio = p["poisson"]
+    mat.density = p["density"]
+    mat.heatCapacity = p["heatCapacity"]
+    mat.taylorQuinney = p["taylorQuinney"]
+    mat.T0 = p["T0"]
+    return mat
+
+
+def _solver(t_end, precision, track_tensors, freq=10):
+    s = Explicit("Solver")
+    s.setTimes(0.0, t_end)
+    s.setComputeTimeStepFrequency(freq)
+    s.setPrecision(precision)
+    s.setTrackTensors(track_tensors)
+    return s
+
+
+def _set(numbers, name):
+    ns = NodeSet(name)
+    ns._numbers.extend(int(v) for v in numbers)
+    return ns
+
+
+def taylor_axi(nr=10, nz=50, radius=3.2e-3, length=32.4e-3, speed=227.0, t_end=80e-6, material=None,
+               precision="fp64", track_tensors=True, context=None, freq=10):
+    """Taylor-Axi impact (Samples/Taylor/Taylor-Axi/Taylor.py): r = x, axis = y,
+    bottom row restrained in y, axis nodes restrained in x (and z),
+    initial axial speed -speed on every node."""
+    p = dict(COPPER)
+    if material:
+        p.update(material)
+    model = DynELA("Taylor-Axi", context=context)
+    i, j = np.meshgrid(np.arange(nr + 1), np.arange(nz + 1))
+    nums = (j * (nr + 1) + i + 1).ravel()
+    xyz = np.stack([(i * (radius / nr)).ravel(), (j * (length / nz)).ravel(), np.zeros(nums.size)], axis=1)
+    model.createNodes(nums, xyz)
+    model.setDefaultElement(Element.ElQua4NAx)
+    ei, ej = np.meshgrid(np.arange(nr), np.arange(nz))
+    n1 = ei + ej * (nr + 1) + 1
+    conn = np.stack([n1, n1 + 1, ei + (ej + 1) * (nr + 1) + 2, ei + (ej + 1) * (nr + 1) + 1], axis=-1)
+    model.createElements(np.arange(1, nr * nz + 1), conn.reshape(-1, 4))
+    allES = ElementSet("ES_All")
+    model.add(allES, 1, nr * nz)
+    model.add(_jc_material(p), allES)
+    bottom = BoundaryRestrain("BC_bottom")
+    bottom.setValue(0, 1, 1)
+    model.attachConstantBC(bottom, _set(range(1, nr + 2), "NS_Bottom"))
+    axis = BoundaryRestrain("BC_axis")
+    axis.setValue(1, 0, 1)
+    model.attachConstantBC(axis, _set(range(1, nums.size + 1, nr + 1), "NS_Axis"))
+    v = BoundarySpeed("BC_speed")
+    v.setValue(0, -speed, 0)
+    model.attachInitialBC(v, _set(nums, "NS_All"))
+    model.add(_solver(t_end, precision, track_tensors, freq))
+    return model
+
+
+def _disc_grid(n, radius):
+    """(n+1)^2 points of a square mapped onto the disc (elliptical grid
+    mapping); returns X, Z of shape (n+1, n+1) indexed [a, b]."""
+    u = np.linspace(-1.0, 1.0, n + 1)
+    a, b = np.meshgrid(u, u, indexing="ij")
+    X = a * np.sqrt(1.0 - b * b / 2.0)
+    Z = b * np.sqrt(1.0 - a * a / 2.0)
+    return radius * X, radius * Z
+
+
+def _hex_cylinder(n, nz, radius, ys):
+    """Full-cylinder hex mesh, axis y.  ys: nz+1 axial levels; radius may be
+    an array over levels.  Returns numbers, xyz [N,3], conn [NE,8] (1-based
+    node numbers)."""
+    X, Z = _disc_grid(n, 1.0)
+    nl = (n + 1) * (n + 1)
+    r = np.broadcast_to(np.asarray(radius, dtype=np.float64), (nz + 1,))
+    xyz = np.zeros((nz + 1, n + 1, n + 1, 3))
+    xyz[..., 0] = r[:, None, None] * X[None]
+    xyz[..., 1] = np.asarray(ys)[:, None, None]
+    xyz[..., 2] = r[:, None, None] * Z[None]
+    nums = np.arange(1, (nz + 1) * nl + 1)
+
+    def nid(k, a, b):
+        return k * nl + a * (n + 1) + b + 1
+
+    k, a, b = np.meshgrid(np.arange(nz), np.arange(n), np.arange(n), indexing="ij")
+    # local xi -> b (z), eta -> a (x), zeta -> k (y): right-handed, detJ > 0
+    conn = np.stack([nid(k, a, b), nid(k, a, b + 1), nid(k, a + 1, b + 1), nid(k, a + 1, b),
+                     nid(k + 1, a, b), nid(k + 1, a, b + 1), nid(k + 1, a + 1, b + 1), nid(k + 1, a + 1, b)],
+                    axis=-1).reshape(-1, 8)
+    return nums, xyz.reshape(-1, 3), conn, nl
+
+
+def taylor_3d(n=40, nz=75, radius=3.2e-3, length=32.4e-3, speed=227.0, t_end=80e-6, material=None,
+              precision="fp64", track_tensors=True, context=None, freq=10):
+    """3D Taylor impact of a full cylinder (no symmetry planes): n x n
+    cross-section, nz layers (n=40, nz=75 -> 120 000 hexes); bottom face
+    restrained along the axis (y), initial speed -speed on every node."""
+    p = dict(COPPER)
+    if material:
+        p.update(material)
+    model = DynELA("Taylor-3D", context=context)
+    nums, xyz, conn, nl = _hex_cylinder(n, nz, radius, np.linspace(0.0, length, nz + 1))
+    model.createNodes(nums, xyz)
+    model.setDefaultElement(Element.ElHex8N3D)
+    ne = conn.shape[0]
+    model.createElements(np.arange(1, ne + 1), conn)
+    allES = ElementSet("ES_All")
+    model.add(allES, 1, ne)
+    model.add(_jc_material(p), allES)
+    bottom = BoundaryRestrain("BC_bottom")
+    bottom.setValue(0, 1, 0)
+    model.attachConstantBC(bottom, _set(range(1, nl + 1), "NS_Bottom"))
+    v = BoundarySpeed("BC_speed")
+    v.setValue(0, -speed, 0)
+    model.attachInitialBC(v, _set(nums, "NS_All"))
+    model.add(_solver(t_end, precision, track_tensors, freq))
+    return model
+
+
+def necking_3d(n=16, nz=60, radius=6.413e-3, length=53.334e-3, taper=0.02, pull=1.0, t_end=50e-6,
+               material=None, precision="fp64", track_tensors=True, context=None, freq=10):
+    """Round bar in tension: radius reduced by `taper` at mid-length with a
+    cosine profile, bottom face restrained along the axis (y), top face
+    pulled at constant speed `pull` (BoundarySpeed, constant)."""
+    p = dict(NECKING)
+    if material:
+        p.update(material)
+    model = DynELA("BarNecking-3D", context=context)
+    ys = np.linspace(0.0, length, nz + 1)
+    rad = radius * (1.0 - taper * 0.5 * (1.0 + np.cos(2.0 * np.pi * (ys / length - 0.5))))
+    nums, xyz, conn, nl = _hex_cylinder(n, nz, rad, ys)
+    model.createNodes(nums, xyz)
+    model.setDefaultElement(Element.ElHex8N3D)
+    ne = conn.shape[0]
+    model.createElements(np.arange(1, ne + 1), conn)
+    allES = ElementSet("ES_All")
+    model.add(allES, 1, ne)
+    law = VoceLinearLaw("Voce")
+    law.setParameters(p["A"], p["Q"], p["b"], p["H"])
+    mat = Material("bar")
+    mat.setHardeningLaw(law)
+    mat.youngModulus, mat.poissonRatio, mat.density = p["young"], p["poisson"], p["density"]
+    mat.heatCapacity, mat.taylorQuinney, mat.T0 = p["heatCapacity"], p["taylorQuinney"], p["T0"]
+    model.add(mat, allES)
+    bottom = BoundaryRestrain("BC_bottom")
+    bottom.setValue(0, 1, 0)
+    model.attachConstantBC(bottom, _set(range(1, nl + 1), "NS_Bottom"))
+    top = BoundarySpeed("BC_top")
+    top.setValue(0, pull, 0)
+    model.attachConstantBC(top, _set(range(nz * nl + 1, (nz + 1) * nl + 1), "NS_Top"))
+    model.add(_solver(t_end, precision, track_tensors, freq))
+    return model
+
+
+def sweep_params(n_jobs=256, seed=20261203, lo=2000, hi=60000):
+    """Seeded parameter sweep around the copper impact: kind (axi / 3D, p=0.5),
+    element count log-uniform in [lo, hi], Johnson-Cook A, B, n, C, m each
+    scaled by U(0.8, 1.2), impact speed U(150, 300) m/s."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for _ in range(n_jobs):
+        kind = "axi" if rng.random() < 0.5 else "3d"
+        ne = int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+        scale = rng.uniform(0.8, 1.2, size=5)
+        speed = float(rng.uniform(150.0, 300.0))
+        mat = {k: COPPER[k] * s for k, s in zip(("A", "B", "n", "C", "m"), scale)}
+        if kind == "axi":
+            nr = max(2, int(round(np.sqrt(ne / 10.0))))
+            nz = max(2, int(round(ne / nr)))
+            shape = (nr, nz)
+        else:
+            nc = max(2, int(round((ne / 1.875) ** (1.0 / 3.0))))
+            nz = max(2, int(round(ne / (nc * nc))))
+            shape = (nc, nz)
+        jobs.append(dict(kind=kind, shape=shape, material=mat, speed=speed))
+    return jobs
+
+
+def build_sweep_job(job, t_end=80e-6, precision="mixed", track_tensors=False, context=None):
+    if job["kind"] == "axi":
+        nr, nz = job["shape"]
+        return taylor_axi(nr, nz, speed=job["speed"], t_end=t_end, material=job["material"],
+                          precision=precision, track_tensors=track_tensors, context=context)
+    nc, nz = job["shape"]
+    return taylor_3d(nc, nz, speed=job["speed"], t_end=t_end, material=job["material"], precision=precision,
+                     track_tensors=track_tensors, context=context)

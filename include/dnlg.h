@@ -1,0 +1,226 @@
+/*
+ * dnlg.h -- C ABI of the MI355X explicit-dynamics engine (libdnlg.so).
+ *
+ * This is the drop-in boundary for DynELA's explicit time-step solve.  In
+ * the reference the solve is reached through the virtual
+ *   Solver::solve(double solveUpToTime)          Sources/dnlFEM/Solver.h:70
+ * called by Model::solve (Sources/dnlFEM/Model.C:696-730), which dispatches
+ * to Explicit::solve (Sources/dnlFEM/Explicit.C:126-253).  Here the model is
+ * plain data (mesh, material, boundary conditions, solver parameters) handed
+ * over once at job creation; the engine keeps the state resident in HBM and
+ * advances it on the GPU.
+ *
+ * Conventions
+ *  - every entry point returns an int status (0 = DNLG_OK, < 0 = error);
+ *    nothing ever calls exit(), unlike the reference's fatalError
+ *    (Sources/dnlKernel/Errors.C:157,196);
+ *  - host arrays passed in are copied at create time; read-back buffers are
+ *    caller-allocated;
+ *  - nodal arrays are structure-of-arrays: component d of node i sits at
+ *    [d * n_nodes + i];
+ *  - integration-point arrays are element-major: IP k of element e sits at
+ *    [e * n_ip + k] (tensor components add an outer stride of n_elems*n_ip);
+ *  - symmetric tensors use the reference's component order
+ *    [xx, xy, xz, yy, yz, zz] (Sources/dnlMaths/SymTensor2.C:139-148).
+ */
+#ifndef DNLG_H
+#define DNLG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DNLG_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+#define DNLG_OK 0
+#define DNLG_E_INVALID (-1)      /* bad argument or unsupported model       */
+#define DNLG_E_OOM (-2)          /* device memory exhausted                 */
+#define DNLG_E_NEG_JACOBIAN (-3) /* detJ < 0 (Element3D.C:98-102)           */
+#define DNLG_E_NR_DIVERGED (-4)  /* radial return > 250 its (Element.C:560) */
+#define DNLG_E_NONFINITE (-5)    /* NaN/Inf in the nodal state              */
+#define DNLG_E_HIP (-6)          /* HIP runtime error                       */
+#define DNLG_E_STATE (-7)        /* call not valid in the job's state       */
+#define DNLG_E_CAPACITY (-8)     /* request exceeds a fixed capacity        */
+
+/* ---- enumerations -------------------------------------------------------- */
+/* element types: values of the reference's element enum (Element.h:62-71) */
+#define DNLG_EL_QUA4N_AX 3 /* 4-node axisymmetric quad, 2x2 Gauss points */
+#define DNLG_EL_HEX8N_3D 4 /* 8-node hexahedron, 2x2x2 Gauss points     */
+
+/* hardening laws */
+#define DNLG_LAW_JOHNSON_COOK 0 /* JohnsonCookLaw.C:146-170,298-339; p = A,B,C,n,m,depsp0,Tm,T0 */
+#define DNLG_LAW_POWER 1        /* ElastoplasticLaw.C:110-121;          p = A,B,n            */
+#define DNLG_LAW_VOCE_LINEAR 2  /* sy = A + Q(1-exp(-b ep)) + H ep;     p = A,Q,b,H          */
+
+/* boundary kinds (Boundary.h:59-62) */
+#define DNLG_BC_RESTRAIN 0 /* BoundaryRestrain.C:98-150 */
+#define DNLG_BC_SPEED 1    /* BoundarySpeed.C:63-76,139-226 */
+
+/* ramp functions: values of RampFunction's enum (RampFunction.h:31-39) */
+#define DNLG_RAMP_NONE 0
+#define DNLG_RAMP_LINEAR 1
+#define DNLG_RAMP_SINUS 2
+#define DNLG_RAMP_CONSTANT 3
+#define DNLG_RAMP_ERF 4
+#define DNLG_RAMP_SOFT 5
+
+/* time-step methods (Solver.h:28-33); only Courant is implemented */
+#define DNLG_DT_COURANT 0
+#define DNLG_DT_POWER_ITERATION 1
+#define DNLG_DT_POWER_ITERATION_UNDER 2
+
+/* stress integration (Model.h StressIntNR / StressIntDirect) */
+#define DNLG_STRESS_NR 0
+
+/* arithmetic of the element phases */
+#define DNLG_PREC_FP64 0  /* everything in FP64                              */
+#define DNLG_PREC_MIXED 1 /* FP32 strain/stress/forces, FP64 kinematics, FP64
+                             radial-return Newton and energy accumulators     */
+
+/* integration-point fields for dnlg_job_read_ip (names of Field.h:46-79) */
+#define DNLG_F_PLASTIC_STRAIN 0      /* scalar */
+#define DNLG_F_PLASTIC_STRAIN_RATE 1 /* scalar */
+#define DNLG_F_GAMMA 2               /* scalar */
+#define DNLG_F_GAMMA_CUMULATE 3      /* scalar */
+#define DNLG_F_YIELD_STRESS 4        /* scalar */
+#define DNLG_F_TEMPERATURE 5         /* scalar */
+#define DNLG_F_DENSITY 6             /* scalar */
+#define DNLG_F_PRESSURE 7            /* scalar, tr(Stress)/3 */
+#define DNLG_F_INTERNAL_ENERGY 8     /* scalar, specific */
+#define DNLG_F_INELASTIC_ENERGY 9    /* scalar, specific */
+#define DNLG_F_VON_MISES 10          /* scalar */
+#define DNLG_F_STRESS 11             /* 6 components */
+#define DNLG_F_STRAIN 12             /* 6 components (needs track_tensors) */
+#define DNLG_F_PLASTIC_STRAIN_TENSOR 13 /* 6 components (needs track_tensors) */
+#define DNLG_F_COUNT 14
+
+/* ---- model descriptors --------------------------------------------------- */
+typedef struct dnlg_mesh {
+  int32_t elem_type; /* DNLG_EL_* */
+  int32_t reserved;
+  int64_t n_nodes;
+  int64_t n_elems;
+  const double *coords; /* [3 * n_nodes] SoA; z ignored for axisymmetric */
+  const int32_t *conn;  /* [n_elems * nodes_per_elem], 0-based node indices,
+                           node order as ElQua4NAx.C:30-60 / ElHex8N3D.C:35-90 */
+} dnlg_mesh;
+
+typedef struct dnlg_material {
+  double young;          /* Material::youngModulus   (Material.h:49) */
+  double poisson;        /* Material::poissonRatio   (Material.h:50) */
+  double density;        /* Material::density        (Material.h:51) */
+  double heat_capacity;  /* Material::heatCapacity   (Material.h:52) */
+  double taylor_quinney; /* Material::taylorQuinney  (Material.h:56) */
+  double T0;             /* Material::T0             (Material.h:54) */
+  int32_t law;           /* DNLG_LAW_* */
+  int32_t reserved;
+  double p[8];           /* law parameters, see DNLG_LAW_* */
+} dnlg_material;
+
+typedef struct dnlg_boundary {
+  int32_t kind;     /* DNLG_BC_* */
+  int32_t ramp;     /* DNLG_RAMP_* (speed only; NONE = constant speed) */
+  double value[3];  /* restrain flags (> 0 = restrained) or speed vector */
+  double ramp_min;  /* RampFunction::setFunction(type, min, max) */
+  double ramp_max;
+} dnlg_boundary;
+
+/* Per-node boundary lists in CSR form, applied in list order exactly as the
+ * reference applies a node's initial then constant lists
+ * (BoundaryCondition.C:131-190).  Either CSR may be NULL (no entries). */
+typedef struct dnlg_bcs {
+  int32_t n_boundaries;
+  int32_t reserved;
+  const dnlg_boundary *boundaries;
+  const int64_t *initial_ptr;  /* [n_nodes + 1] */
+  const int32_t *initial_idx;  /* indices into boundaries */
+  const int64_t *constant_ptr; /* [n_nodes + 1] */
+  const int32_t *constant_idx;
+} dnlg_bcs;
+
+typedef struct dnlg_solver_params {
+  double t_start;        /* Solver::setTimes (Solver.C) */
+  double t_end;
+  double rho_b;          /* Explicit::setDissipation, default 0.8182 (Explicit.h:16) */
+  double safety;         /* Solver::_timeStepSafetyFactor, default 0.9 (Solver.h:30) */
+  int32_t dt_method;     /* DNLG_DT_*, default Courant */
+  int32_t dt_freq;       /* Solver::_computeTimeStepFrequency, default 10 */
+  int32_t stress_method; /* DNLG_STRESS_NR */
+  int32_t precision;     /* DNLG_PREC_* */
+  int32_t track_tensors; /* 1: carry Strain and PlasticStrain tensors */
+  int32_t reserved;
+} dnlg_solver_params;
+
+/* ---- opaque handles ------------------------------------------------------- */
+typedef struct dnlg_ctx dnlg_ctx;
+typedef struct dnlg_job dnlg_job;
+
+/* ---- context --------------------------------------------------------------- */
+/* One context per device.  Several contexts (and many jobs per context) may
+ * live in one process: the reference allowed a single global DynELA object
+ * (DynELA.C:21,47-52); this ABI keeps no global mutable state. */
+int dnlg_abi_version(void);
+int dnlg_ctx_create(int device, dnlg_ctx **out);
+int dnlg_ctx_destroy(dnlg_ctx *ctx);
+const char *dnlg_last_error(const dnlg_ctx *ctx);
+
+/* ---- jobs (one job = one mesh) ---------------------------------------------- */
+/* Model::initSolve (Model.C:360-521): topology and axisymmetric radius
+ * checks, IP initialisation (T = T0, rho = rho0), initial boundary
+ * conditions.  Copies every host array. */
+int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh,
+                    const dnlg_material *mat, const dnlg_bcs *bcs,
+                    const dnlg_solver_params *params, dnlg_job **out);
+int dnlg_job_destroy(dnlg_job *job);
+
+/* Replaces Solver::solve(double) (Solver.h:70) -> Explicit::solve
+ * (Explicit.C:126-253): advances the job until its time reaches t_up_to
+ * (clipped to t_end).  The pre-loop block (reference Jacobian, lumped mass,
+ * forced time step, initial strain/stress/force pass) runs on the first
+ * call only.  Blocking. */
+int dnlg_job_solve(dnlg_job *job, double t_up_to);
+
+/* Advances several jobs of one context together (one launch per phase and
+ * cycle covers all of them); t_up_to[i] per job.  Blocking. */
+int dnlg_batch_solve(dnlg_ctx *ctx, dnlg_job *const *jobs, int n_jobs,
+                     const double *t_up_to);
+
+/* ---- read-back ------------------------------------------------------------ */
+int dnlg_job_sizes(const dnlg_job *job, int64_t *n_nodes, int64_t *n_elems,
+                   int32_t *n_ip, int32_t *dim);
+/* any pointer may be NULL; each is [3 * n_nodes] */
+int dnlg_job_read_nodes(dnlg_job *job, double *coords, double *disp,
+                        double *speed);
+/* out: [ncomp * n_elems * n_ip], ncomp = 6 for tensor fields, else 1 */
+int dnlg_job_read_ip(dnlg_job *job, int field, double *out);
+/* time, current time step, increment count, cycles run by the engine */
+int dnlg_job_scalars(dnlg_job *job, double *time, double *time_step,
+                     int64_t *increment);
+/* out[0] kinetic energy (Model.C:612-625), out[1] internal (strain)
+ * energy sum_ip rho0 w detJ0 e_int, out[2] dissipated plastic work
+ * sum_ip rho0 w detJ0 e_inel, out[3] out[0] + out[1] */
+int dnlg_job_energies(dnlg_job *job, double out[4]);
+/* lumped nodal masses, [n_nodes] (Model.C:537-590) */
+int dnlg_job_read_mass(dnlg_job *job, double *mass);
+/* device bytes a job of this size and options holds (admission control) */
+int64_t dnlg_footprint_bytes(int elem_type, int64_t n_nodes, int64_t n_elems,
+                             int precision, int track_tensors);
+int dnlg_job_status(const dnlg_job *job);
+
+/* ---- measurement hooks ------------------------------------------------------ */
+/* Per-kernel device time (ms) accumulated with HIP events on the engine's own
+ * stream since the last reset; names: "element", "predict", "correct",
+ * "courant", "control".  Timing is off unless enabled. */
+int dnlg_ctx_set_timing(dnlg_ctx *ctx, int enable);
+int dnlg_ctx_kernel_time(dnlg_ctx *ctx, const char *name, double *total_ms,
+                         int64_t *launches);
+int dnlg_ctx_reset_timing(dnlg_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DNLG_H */

@@ -1,0 +1,1470 @@
+/*
+ * dnl_oracle.cpp -- CPU FP64 oracle for the explicit solve.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker, never the product:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load liboracle.so.  The engine (libdnlg.so) never links or calls it.
+ *
+ * It restates, in plain serial C++ (optional OpenMP over independent
+ * elements), the time-step loop of DynELA's Explicit::solve and everything it
+ * calls for the ElQua4NAx / ElHex8N3D elements with the J2 radial return.
+ * Loop order and arithmetic order follow the reference function by
+ * function; each function cites the reference lines it restates.  It takes
+ * the same descriptors as the engine's C ABI (include/dnlg.h), so a test
+ * can feed both the same model.
+ *
+ * Pinning (tests/test_oracle_*.py):
+ *  - kopp_eig3 is checked bit-for-bit against the reference's own dsyevd3
+ *    compiled from Sources/dnlMaths/Eigen3x3.C into oracle/_ref;
+ *  - shape tables vs the closed-form shape functions, partition of unity;
+ *  - polar/log strain vs scipy polar/logm and invariants;
+ *  - Johnson-Cook yield and derivative vs closed forms and the reference's
+ *    finite-difference check; the radial return vs a brute-force bisection;
+ *  - Chung-Hulbert constants vs closed forms; lumped mass vs rho*V;
+ *    Courant lengths vs box/rectangle closed forms; free flight of a body
+ *    with f = 0; elastic simple shear vs the Green-Naghdi closed form.
+ *  Parity unpinned: the CBLAS dgemm summation order behind dN/dx
+ *  (Matrix.C:815) -- a plain k-ordered sum is used.
+ */
+#include "../include/dnlg.h"
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+/* constants of Sources/dnlKernel/Macros.h:37-56 */
+const double kPI = 3.14159265358979324;
+const double k2PI = 6.28318530717958648;
+const double kSqrt3 = 1.73205080756887729;
+const double kSqrt23 = 0.816496580927726034;
+const double kSqrt32 = 1.22474487139158894;
+const double kEps = 2.2204460492503131e-16;
+
+inline double sq(double a) { return a * a; }
+
+/* ---------------------------------------------------------------------------
+ * Symmetric 3x3 eigen-decomposition: Cuppen divide & conquer after a
+ * Householder tridiagonalisation, the published method of J. Kopp,
+ * "Efficient numerical diagonalization of hermitian 3x3 matrices",
+ * arXiv:physics/0610206, as used by the reference in
+ * Sources/dnlMaths/Eigen3x3.C:14-88 (tridiagonalisation), :194-383 (secular
+ * equation), :385-453 (2x2 block), :454-634 (driver).  The arithmetic order
+ * is kept identical so results agree bit-for-bit with the reference build.
+ * ------------------------------------------------------------------------- */
+
+/* A = Q T Q^T with T tridiagonal (diag d, off-diag e); reads upper part. */
+void kopp_tridiag(const double A[3][3], double Q[3][3], double d[3], double e[2]) {
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) Q[i][j] = (i == j) ? 1.0 : 0.0;
+  const double h = sq(A[0][1]) + sq(A[0][2]);
+  const double g = (A[0][1] > 0) ? -std::sqrt(h) : std::sqrt(h);
+  e[0] = g;
+  double f = g * A[0][1];
+  double u[3], q[3];
+  u[1] = A[0][1] - g;
+  u[2] = A[0][2];
+  double omega = h - f;
+  if (omega > 0.0) {
+    omega = 1.0 / omega;
+    double K = 0.0;
+    for (int i = 1; i < 3; i++) {
+      f = A[1][i] * u[1] + A[i][2] * u[2];
+      q[i] = omega * f;
+      K += u[i] * f;
+    }
+    K *= 0.5 * sq(omega);
+    for (int i = 1; i < 3; i++) q[i] = q[i] - K * u[i];
+    d[0] = A[0][0];
+    d[1] = A[1][1] - 2.0 * q[1] * u[1];
+    d[2] = A[2][2] - 2.0 * q[2] * u[2];
+    for (int j = 1; j < 3; j++) {
+      f = omega * u[j];
+      for (int i = 1; i < 3; i++) Q[i][j] = Q[i][j] - f * u[i];
+    }
+    e[1] = A[1][2] - q[1] * u[2] - u[1] * q[2];
+  } else {
+    for (int i = 0; i < 3; i++) d[i] = A[i][i];
+    e[1] = A[1][2];
+  }
+}
+
+/* eigen-system of [[a,b],[b,c]]: rt1 >= rt2, vector (cs, sn) of rt1 */
+void kopp_eig2(double a, double b, double c, double *rt1, double *rt2, double *cs, double *sn) {
+  const double sm = a + c;
+  const double df = a - c;
+  const double rt = std::sqrt(sq(df) + 4.0 * b * b);
+  double t;
+  if (sm > 0.0) {
+    *rt1 = 0.5 * (sm + rt);
+    t = 1.0 / (*rt1);
+    *rt2 = (a * t) * c - (b * t) * b;
+  } else if (sm < 0.0) {
+    *rt2 = 0.5 * (sm - rt);
+    t = 1.0 / (*rt2);
+    *rt1 = (a * t) * c - (b * t) * b;
+  } else {
+    *rt1 = 0.5 * rt;
+    *rt2 = -0.5 * rt;
+  }
+  *cs = (df > 0.0) ? df + rt : df - rt;
+  if (std::fabs(*cs) > 2.0 * std::fabs(b)) {
+    t = -2.0 * b / *cs;
+    *sn = 1.0 / std::sqrt(1.0 + sq(t));
+    *cs = t * (*sn);
+  } else if (std::fabs(b) == 0.0) {
+    *cs = 1.0;
+    *sn = 0.0;
+  } else {
+    t = -0.5 * (*cs) / b;
+    *cs = 1.0 / std::sqrt(1.0 + sq(t));
+    *sn = t * (*cs);
+  }
+  if (df > 0.0) {
+    t = *cs;
+    *cs = -(*sn);
+    *sn = t;
+  }
+}
+
+/* Roots w of 1 + sum z_i/(d_i - w) = 0 with d[i0] <= d[i1] <= d[i2];
+ * P[j][i] = d_j - w_i. */
+void kopp_secular(const double d[3], const double z[3], double w[3], double P[3][3], int i0, int i1,
+                  int i2) {
+  double a[4];
+  if (z[0] > 0) {
+    a[0] = d[i0];
+    a[1] = d[i1];
+    a[2] = d[i2];
+    a[3] = std::fabs(d[0] + 3.0 * z[0]) + std::fabs(d[1] + 3.0 * z[1]) + std::fabs(d[2] + 3.0 * z[2]);
+  } else {
+    a[0] = -std::fabs(d[0] + 3.0 * z[0]) - std::fabs(d[1] + 3.0 * z[1]) - std::fabs(d[2] + 3.0 * z[2]);
+    a[1] = d[i0];
+    a[2] = d[i1];
+    a[3] = d[i2];
+  }
+  /* closed-form cubic roots as starting values */
+  double t0 = d[1] * d[2], t1 = d[0] * d[2], t2 = d[0] * d[1];
+  const double cg = t0 * d[0] + (z[0] * t0 + z[1] * t1 + z[2] * t2);
+  const double cb = (z[0] * (d[1] + d[2]) + z[1] * (d[0] + d[2]) + z[2] * (d[0] + d[1])) + (t0 + t1 + t2);
+  const double ca = (z[0] + z[1] + z[2]) + (d[0] + d[1] + d[2]);
+  const double p = sq(ca) - 3.0 * cb;
+  const double q = ca * (p - (3.0 / 2.0) * cb) + (27.0 / 2.0) * cg;
+  const double sqrt_p = std::sqrt(std::fabs(p));
+  double phi = 27.0 * (0.25 * sq(cb) * (p - cb) - cg * (q - 27.0 / 4.0 * cg));
+  phi = (1.0 / 3.0) * std::atan2(std::sqrt(std::fabs(phi)), q);
+  const double c = sqrt_p * std::cos(phi);
+  const double s = (1.0 / kSqrt3) * sqrt_p * std::fabs(std::sin(phi));
+  double x0[3];
+  x0[0] = x0[1] = x0[2] = (1.0 / 3.0) * (ca - c);
+  if (c > s) {
+    x0[0] -= s;
+    x0[1] += s;
+    x0[2] += c;
+  } else if (c < -s) {
+    x0[0] += c;
+    x0[1] -= s;
+    x0[2] += s;
+  } else {
+    x0[0] -= s;
+    x0[1] += c;
+    x0[2] += s;
+  }
+  /* safeguarded Newton / bisection refinement of each root */
+  for (int i = 0; i < 3; i++) {
+    double xl = a[i], xh = a[i + 1];
+    double dx = 0.5 * (xh - xl), dxold = dx;
+    double dd[3], x, delta;
+    if (dx == 0.0) {
+      w[i] = xl;
+      for (int j = 0; j < 3; j++) P[j][i] = d[j] - xl;
+      continue;
+    }
+    if (x0[i] >= xh) {
+      delta = xh;
+      x = -dx;
+      for (int j = 0; j < 3; j++) {
+        dd[j] = d[j] - delta;
+        P[j][i] = dd[j] - x;
+      }
+    } else if (x0[i] <= xl) {
+      delta = xl;
+      x = dx;
+      for (int j = 0; j < 3; j++) {
+        dd[j] = d[j] - delta;
+        P[j][i] = dd[j] - x;
+      }
+    } else {
+      delta = x0[i];
+      x = 0.0;
+      for (int j = 0; j < 3; j++) P[j][i] = dd[j] = d[j] - delta;
+    }
+    xl -= delta;
+    xh -= delta;
+    if (z[0] < 0.0) {
+      const double tmp = xh;
+      xh = xl;
+      xl = tmp;
+    }
+    for (int it = 0; it < 500; it++) {
+      double F = 1.0, dF = 0.0, err = 1.0;
+      for (int j = 0; j < 3; j++) {
+        const double r = 1.0 / P[j][i];
+        const double zr = z[j] * r;
+        const double zrr = zr * r;
+        F += zr;
+        err += std::fabs(zr);
+        dF += zrr;
+      }
+      if (std::fabs(F) <= kEps * (8.0 * err + std::fabs(x * dF))) break;
+      if (F < 0.0)
+        xl = x;
+      else
+        xh = x;
+      if (std::fabs(2.0 * F) < std::fabs(dxold * dF)) {
+        dxold = dx;
+        dx = F / dF;
+        x = x - dx;
+        if ((x - xh) * (x - xl) >= 0.0) {
+          dx = 0.5 * (xh - xl);
+          x = xl + dx;
+        }
+      } else {
+        dx = 0.5 * (xh - xl);
+        x = xl + dx;
+      }
+      for (int j = 0; j < 3; j++) P[j][i] = dd[j] - x;
+    }
+    w[i] = x + delta;
+  }
+}
+
+/* eigenvalues w and column eigenvectors Q of the symmetric A */
+void kopp_eig3(const double A[3][3], double Q[3][3], double w[3]) {
+  double Hh[3][3], P[3][3], e[2], d[3], z[3], c, s, t;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) Q[i][j] = 0.0;
+  kopp_tridiag(A, Hh, w, e);
+  /* split when an off-diagonal vanishes */
+  for (int i = 0; i < 2; i++) {
+    t = std::fabs(w[i]) + std::fabs(w[i + 1]);
+    if (std::fabs(e[i]) <= 8.0 * kEps * t) {
+      if (i == 0) {
+        kopp_eig2(w[1], e[1], w[2], &d[1], &d[2], &c, &s);
+        w[1] = d[1];
+        w[2] = d[2];
+        Q[0][0] = 1.0;
+        for (int j = 1; j < 3; j++) {
+          Q[j][1] = s * Hh[j][2] + c * Hh[j][1];
+          Q[j][2] = c * Hh[j][2] - s * Hh[j][1];
+        }
+      } else {
+        kopp_eig2(w[0], e[0], w[1], &d[0], &d[1], &c, &s);
+        w[0] = d[0];
+        w[1] = d[1];
+        Q[0][0] = c;
+        Q[0][1] = -s;
+        Q[1][0] = Hh[1][1] * s;
+        Q[1][1] = Hh[1][1] * c;
+        Q[1][2] = Hh[1][2];
+        Q[2][0] = Hh[2][1] * s;
+        Q[2][1] = Hh[2][1] * c;
+        Q[2][2] = Hh[2][2];
+      }
+      return;
+    }
+  }
+  /* divide: 2x2 block of T - e0 v v^T */
+  kopp_eig2(w[1] - e[0], e[1], w[2], &d[1], &d[2], &c, &s);
+  d[0] = w[0] - e[0];
+  /* conquer: rank-one update, secular equation */
+  z[0] = e[0];
+  z[1] = e[0] * sq(c);
+  z[2] = e[0] * sq(s);
+  if (d[0] < d[2])
+    kopp_secular(d, z, w, P, 0, 2, 1);
+  else if (d[0] < d[1])
+    kopp_secular(d, z, w, P, 2, 0, 1);
+  else
+    kopp_secular(d, z, w, P, 2, 1, 0);
+  z[0] = std::sqrt(std::fabs(e[0]));
+  z[1] = c * z[0];
+  z[2] = -s * z[0];
+  t = 8.0 * kEps * (std::fabs(d[0]) + std::fabs(d[1]) + std::fabs(d[2]));
+  if (std::fabs(d[1] - d[0]) <= t) {
+    for (int j = 0; j < 3; j++) {
+      if (P[0][j] * P[1][j] <= 0.0) {
+        P[0][j] = z[1];
+        P[1][j] = -z[0];
+        P[2][j] = 0.0;
+      } else
+        for (int i = 0; i < 3; i++) P[i][j] = z[i] / P[i][j];
+    }
+  } else if (std::fabs(d[2] - d[0]) <= t) {
+    for (int j = 0; j < 3; j++) {
+      if (P[0][j] * P[2][j] <= 0.0) {
+        P[0][j] = z[2];
+        P[1][j] = 0.0;
+        P[2][j] = -z[0];
+      } else
+        for (int i = 0; i < 3; i++) P[i][j] = z[i] / P[i][j];
+    }
+  } else {
+    for (int j = 0; j < 3; j++)
+      for (int i = 0; i < 3; i++) {
+        if (P[i][j] == 0.0) {
+          P[i][j] = 1.0;
+          P[(i + 1) % 3][j] = 0.0;
+          P[(i + 2) % 3][j] = 0.0;
+          break;
+        } else
+          P[i][j] = z[i] / P[i][j];
+      }
+  }
+  for (int j = 0; j < 3; j++) {
+    t = sq(P[0][j]) + sq(P[1][j]) + sq(P[2][j]);
+    t = 1.0 / std::sqrt(t);
+    for (int i = 0; i < 3; i++) P[i][j] *= t;
+  }
+  for (int j = 0; j < 3; j++) {
+    t = P[1][j];
+    P[1][j] = c * t - s * P[2][j];
+    P[2][j] = s * t + c * P[2][j];
+  }
+  for (int j = 0; j < 3; j++)
+    for (int k = 0; k < 3; k++) {
+      t = P[k][j];
+      for (int i = 0; i < 3; i++) Q[i][j] += t * Hh[i][k];
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * Polar decomposition F = R U and logarithmic strain increment ln U:
+ * Tensor2::buildFTF (Tensor2.C:1315-1327), Tensor2::polarExtractLnU
+ * (Tensor2.C:1237-1312), Tensor2::polarCuppenLnU (Tensor2.C:1442-1457).
+ * F row-major [9]; dE symmetric [6]; R row-major [9].
+ * ------------------------------------------------------------------------- */
+void polar_lnU(const double F[9], double dE[6], double R[9]) {
+  double C[3][3];
+  C[0][0] = sq(F[0]) + sq(F[3]) + sq(F[6]);
+  C[0][1] = F[0] * F[1] + F[3] * F[4] + F[6] * F[7];
+  C[0][2] = F[0] * F[2] + F[3] * F[5] + F[6] * F[8];
+  C[1][0] = C[0][1];
+  C[1][1] = sq(F[1]) + sq(F[4]) + sq(F[7]);
+  C[1][2] = F[1] * F[2] + F[4] * F[5] + F[7] * F[8];
+  C[2][0] = C[0][2];
+  C[2][1] = C[1][2];
+  C[2][2] = sq(F[2]) + sq(F[5]) + sq(F[8]);
+  double V[3][3], lam[3];
+  kopp_eig3(C, V, lam);
+  /* dyads v_k (x) v_k, symmetric storage */
+  double D[3][6];
+  for (int k = 0; k < 3; k++) {
+    D[k][0] = sq(V[0][k]);
+    D[k][1] = V[0][k] * V[1][k];
+    D[k][2] = V[0][k] * V[2][k];
+    D[k][3] = sq(V[1][k]);
+    D[k][4] = V[1][k] * V[2][k];
+    D[k][5] = sq(V[2][k]);
+  }
+  double g[3] = {std::sqrt(lam[0]), std::sqrt(lam[1]), std::sqrt(lam[2])};
+  double U[6];
+  for (int c = 0; c < 6; c++) U[c] = g[0] * D[0][c] + g[1] * D[1][c] + g[2] * D[2][c];
+  /* closed-form inverse of the symmetric U */
+  const double t1 = U[3] * U[5];
+  const double t2 = U[2] * U[4];
+  const double t4 = U[4] * U[4];
+  const double t5 = U[1] * U[5];
+  const double t6 = U[2] * U[3];
+  const double det = U[0] * t1 + 2.0 * U[1] * t2 - U[0] * t4 - U[1] * t5 - U[2] * t6;
+  double Ui[6];
+  Ui[0] = (t1 - t4) / det;
+  Ui[1] = (t2 - t5) / det;
+  Ui[2] = (U[1] * U[4] - t6) / det;
+  Ui[3] = (U[0] * U[5] - U[2] * U[2]) / det;
+  Ui[4] = (U[2] * U[1] - U[0] * U[4]) / det;
+  Ui[5] = (U[0] * U[3] - U[1] * U[1]) / det;
+  for (int r = 0; r < 3; r++) {
+    const double a0 = F[3 * r], a1 = F[3 * r + 1], a2 = F[3 * r + 2];
+    R[3 * r + 0] = a0 * Ui[0] + a1 * Ui[1] + a2 * Ui[2];
+    R[3 * r + 1] = a0 * Ui[1] + a1 * Ui[3] + a2 * Ui[4];
+    R[3 * r + 2] = a0 * Ui[2] + a1 * Ui[4] + a2 * Ui[5];
+  }
+  g[0] = std::log(lam[0]) / 2;
+  g[1] = std::log(lam[1]) / 2;
+  g[2] = std::log(lam[2]) / 2;
+  for (int c = 0; c < 6; c++) dE[c] = g[0] * D[0][c] + g[1] * D[1][c] + g[2] * D[2][c];
+}
+
+/* R . S . R^T for symmetric S (SymTensor2::dotRxRT, SymTensor2.C:512-525) */
+void rotate_sym(const double R[9], double S[6]) {
+  const double *r = R;
+  double o[6];
+  o[0] = r[0] * r[0] * S[0] + 2 * r[0] * (r[1] * S[1] + r[2] * S[2]) + r[1] * r[1] * S[3] + 2 * r[1] * r[2] * S[4] +
+         r[2] * r[2] * S[5];
+  o[1] = r[0] * (r[3] * S[0] + r[4] * S[1] + r[5] * S[2]) + r[1] * (r[3] * S[1] + r[4] * S[3] + r[5] * S[4]) +
+         r[2] * (r[3] * S[2] + r[4] * S[4] + r[5] * S[5]);
+  o[2] = r[0] * (r[6] * S[0] + r[7] * S[1] + r[8] * S[2]) + r[1] * (r[6] * S[1] + r[7] * S[3] + r[8] * S[4]) +
+         r[2] * (r[6] * S[2] + r[7] * S[4] + r[8] * S[5]);
+  o[3] = r[3] * r[3] * S[0] + 2 * r[3] * (r[4] * S[1] + r[5] * S[2]) + r[4] * r[4] * S[3] + 2 * r[4] * r[5] * S[4] +
+         r[5] * r[5] * S[5];
+  o[4] = r[3] * (r[6] * S[0] + r[7] * S[1] + r[8] * S[2]) + r[4] * (r[6] * S[1] + r[7] * S[3] + r[8] * S[4]) +
+         r[5] * (r[6] * S[2] + r[7] * S[4] + r[8] * S[5]);
+  o[5] = r[6] * r[6] * S[0] + 2 * r[6] * (r[7] * S[1] + r[8] * S[2]) + r[7] * r[7] * S[3] + 2 * r[7] * r[8] * S[4] +
+         r[8] * r[8] * S[5];
+  for (int c = 0; c < 6; c++) S[c] = o[c];
+}
+
+/* SymTensor2 helpers: thirdTrace (SymTensor2.h:327-331), deviator
+ * (SymTensor2.C:688-699), norm (SymTensor2.h:362-367), doubleDot
+ * (SymTensor2.C:640-649). */
+inline double third_trace(const double S[6]) { return (S[0] + S[3] + S[5]) / 3.0; }
+inline void deviator(const double S[6], double out[6]) {
+  const double p = third_trace(S);
+  for (int c = 0; c < 6; c++) out[c] = S[c];
+  out[0] -= p;
+  out[3] -= p;
+  out[5] -= p;
+}
+inline double norm_sym(const double S[6]) {
+  return std::sqrt(sq(S[0]) + sq(S[3]) + sq(S[5]) + 2.0 * (sq(S[1]) + sq(S[2]) + sq(S[4])));
+}
+inline double ddot_sym(const double A[6], const double B[6]) {
+  return (A[0] * B[0] + A[3] * B[3] + A[5] * B[5] + 2.0 * (A[1] * B[1] + A[2] * B[2] + A[4] * B[4]));
+}
+/* index of (i,j) in [xx,xy,xz,yy,yz,zz] */
+inline int sym_idx(int i, int j) {
+  static const int m[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+  return m[i][j];
+}
+
+/* ---------------------------------------------------------------------------
+ * Hardening laws.
+ * ------------------------------------------------------------------------- */
+struct Law {
+  int kind;
+  double p[8];
+  double eta, rho, cp; /* Taylor-Quinney, density, heat capacity */
+};
+
+/* JohnsonCookLaw::getYieldStress (JohnsonCookLaw.C:146-170) and
+ * ElastoplasticLaw::getYieldStress (ElastoplasticLaw.C:110-114). */
+double law_yield(const Law &L, double epsp, double depsp, double T) {
+  if (L.kind == DNLG_LAW_JOHNSON_COOK) {
+    const double A = L.p[0], B = L.p[1], C = L.p[2], n = L.p[3], m = L.p[4], d0 = L.p[5], Tm = L.p[6],
+                 T0 = L.p[7];
+    const double hard = A + B * std::pow(epsp, n);
+    double visc;
+    if (depsp > d0)
+      visc = 1.0 + C * std::log(depsp / d0);
+    else
+      visc = 1.0;
+    double temp = 1.0;
+    if (T > T0) {
+      if (T < Tm)
+        temp = 1.0 - std::pow(((T - T0) / (Tm - T0)), m);
+      else
+        temp = 0.0;
+    }
+    return hard * visc * temp;
+  }
+  if (L.kind == DNLG_LAW_POWER) return (L.p[0] + L.p[1] * std::pow(epsp, L.p[2]));
+  /* Voce + linear: no reference counterpart; closed form */
+  return L.p[0] + L.p[1] * (1.0 - std::exp(-L.p[2] * epsp)) + L.p[3] * epsp;
+}
+
+/* JohnsonCookLaw::getDerivateYieldStress (JohnsonCookLaw.C:298-339) and
+ * ElastoplasticLaw::getDerivateYieldStress (ElastoplasticLaw.C:117-121). */
+double law_dyield(const Law &L, double epsp, double depsp, double T, double dtime) {
+  if (L.kind == DNLG_LAW_JOHNSON_COOK) {
+    const double A = L.p[0], B = L.p[1], C = L.p[2], n = L.p[3], m = L.p[4], d0 = L.p[5], Tm = L.p[6],
+                 T0 = L.p[7];
+    double tp1 = 0.0;
+    const double hard0 = A + B * std::pow(epsp, n);
+    double visc0 = 1.0;
+    if (depsp > d0) visc0 = 1.0 + C * std::log(depsp / d0);
+    double temp0 = 1.0;
+    if (T > T0) {
+      if (T < Tm) {
+        tp1 = std::pow(((T - T0) / (Tm - T0)), m);
+        temp0 = 1.0 - tp1;
+      } else
+        temp0 = 0.0;
+    }
+    double h = n * B * std::pow(epsp, (n - 1.0)) * visc0 * temp0;
+    if (depsp > d0) h += hard0 * C * temp0 / (depsp * dtime);
+    if ((T > T0) && (T < Tm))
+      h -= m * L.eta * hard0 * hard0 * visc0 * visc0 * temp0 * tp1 / (L.rho * L.cp * (T - T0));
+    return h;
+  }
+  if (L.kind == DNLG_LAW_POWER) return (L.p[2] * L.p[1] * std::pow(epsp, (L.p[2] - 1.0)));
+  return L.p[1] * L.p[2] * std::exp(-L.p[2] * epsp) + L.p[3];
+}
+
+/* ---------------------------------------------------------------------------
+ * Element tables.  Values at the Gauss points are the closed forms the
+ * reference tabulates (ElQua4NAx.C:63-110, ElHex8N3D.C:93-231): each shape
+ * function value / derivative is a product of (1 +- 1/sqrt3) factors and is
+ * selected here from the number of factors whose signs agree.
+ * ------------------------------------------------------------------------- */
+struct Tables {
+  int type, dim, nn, nip;
+  double w[8];
+  double N[8][8];      /* [ip][node] */
+  double dNdr[8][8][3]; /* [ip][node][local dir] */
+};
+
+Tables make_tables(int type) {
+  Tables T{};
+  T.type = type;
+  if (type == DNLG_EL_QUA4N_AX) {
+    T.dim = 2;
+    T.nn = 4;
+    T.nip = 4;
+    const int ns[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+    const int ps[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+    const double Nv[3] = {(2.0 - std::sqrt(3.0)) / 6, 1.0 / 6, (2.0 + std::sqrt(3.0)) / 6};
+    const double Dv[2] = {(3.0 - std::sqrt(3.0)) / 12, (3.0 + std::sqrt(3.0)) / 12};
+    for (int p = 0; p < 4; p++) {
+      T.w[p] = 1.0;
+      for (int a = 0; a < 4; a++) {
+        int k = 0;
+        for (int c = 0; c < 2; c++) k += (ns[a][c] == ps[p][c]);
+        T.N[p][a] = Nv[k];
+        for (int c = 0; c < 2; c++) {
+          const int o = 1 - c;
+          const double v = Dv[ns[a][o] == ps[p][o] ? 1 : 0];
+          T.dNdr[p][a][c] = ns[a][c] > 0 ? v : -v;
+        }
+        T.dNdr[p][a][2] = 0.0;
+      }
+    }
+  } else {
+    T.dim = 3;
+    T.nn = 8;
+    T.nip = 8;
+    const int ns[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                          {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+    const double Nv[4] = {(9.0 - 5.0 * std::sqrt(3.0)) / 36, (3 - std::sqrt(3.0)) / 36, (3 + std::sqrt(3.0)) / 36,
+                          (9.0 + 5.0 * std::sqrt(3.0)) / 36};
+    const double Dv[3] = {(2.0 - std::sqrt(3.0)) / 12, 1.0 / 12, (2.0 + std::sqrt(3.0)) / 12};
+    for (int p = 0; p < 8; p++) {
+      T.w[p] = 1.0;
+      for (int a = 0; a < 8; a++) {
+        int k = 0;
+        for (int c = 0; c < 3; c++) k += (ns[a][c] == ns[p][c]);
+        T.N[p][a] = Nv[k];
+        for (int c = 0; c < 3; c++) {
+          int kk = 0;
+          for (int o = 0; o < 3; o++)
+            if (o != c) kk += (ns[a][o] == ns[p][o]);
+          const double v = Dv[kk];
+          T.dNdr[p][a][c] = ns[a][c] > 0 ? v : -v;
+        }
+      }
+    }
+  }
+  return T;
+}
+
+/* RampFunction::setFunction / getValue (RampFunction.C:40-131) */
+double ramp_value(int type, double mn, double mx, double x) {
+  if (type == DNLG_RAMP_NONE) return 0;
+  const double mm = mx - mn;
+  double a0 = 0, a1 = 0;
+  switch (type) {
+    case DNLG_RAMP_LINEAR: a0 = -2.0 / mm; a1 = 2.0 / mm; break;
+    case DNLG_RAMP_SINUS: a1 = kPI / 2; a0 = 1.0 / (1.0 - 2.0 / kPI) / mm; break;
+    case DNLG_RAMP_CONSTANT: a0 = 1.0 / mm; break;
+    case DNLG_RAMP_ERF: a0 = 5; a1 = std::sqrt(kPI) * mm / 10; break;
+    case DNLG_RAMP_SOFT: a0 = 1 / mm; break;
+    default: return 0;
+  }
+  if ((x < mn) || (x > mx)) return 0;
+  const double pt = (x - mn) / (mx - mn);
+  switch (type) {
+    case DNLG_RAMP_LINEAR: return a0 * pt + a1;
+    case DNLG_RAMP_SINUS: return a0 * (1.0 - std::sin(a1 * pt));
+    case DNLG_RAMP_CONSTANT: return a0;
+    case DNLG_RAMP_ERF: return std::exp(-sq(a0 * pt)) / a1;
+    case DNLG_RAMP_SOFT: return a0 * (kPI * std::sin(kPI * pt) / 2);
+  }
+  return 0;
+}
+
+}  // namespace
+
+/* ===========================================================================
+ * Oracle job: SoA state, reference loop order.
+ * ========================================================================= */
+struct OracleJob {
+  Tables tab;
+  int dim = 0, nn = 0, nip = 0;
+  int64_t N = 0, NE = 0;
+  dnlg_material mat{};
+  dnlg_solver_params prm{};
+  Law law{};
+  double alphaM = 0, beta = 0, gammaCH = 0, omegaS = 0;
+  std::vector<dnlg_boundary> bnd;
+  std::vector<int64_t> cptr;
+  std::vector<int32_t> cidx;
+  std::vector<int32_t> conn;
+  /* nodes */
+  std::vector<double> x, disp, mass;
+  std::vector<double> fu[2], fv[2], fa[2];
+  int f0 = 0; /* index of field0 */
+  std::vector<double> fint;
+  /* IPs */
+  std::vector<double> detJ, detJ0, radius, dN; /* dN: [ip_global][nn][dim] */
+  std::vector<double> sig, eps, epsP, dE, Rr;  /* [ip_global][6 or 9] */
+  std::vector<double> pres, ep, dep, sy, gam, gamc, T, eint, einel, rho;
+  std::vector<double> elemF; /* [e][nn*dim] */
+  double t = 0, dt = 0;
+  int64_t inc = 0;
+  bool started = false;
+  int status = DNLG_OK;
+  int threads = 1;
+  std::string err;
+};
+
+namespace {
+
+void set_err(OracleJob *J, int st, const char *msg) {
+  if (J->status == DNLG_OK) {
+    J->status = st;
+    J->err = msg;
+  }
+}
+
+/* Element3D::computeJacobian (Element3D.C:64-115), ElementPlane::
+ * computeJacobian (ElementPlane.C:46-102) with the axisymmetric radius
+ * (ElementAxi.C:46-59).  Returns false on a negative determinant. */
+bool element_jacobian(OracleJob *J, int64_t e, bool reference) {
+  const Tables &T = J->tab;
+  const int32_t *cn = &J->conn[e * J->nn];
+  for (int p = 0; p < J->nip; p++) {
+    const int64_t g = e * J->nip + p;
+    double Jm[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int a = 0; a < J->nn; a++) {
+      const int64_t nd = cn[a];
+      for (int i = 0; i < J->dim; i++)
+        for (int j = 0; j < J->dim; j++) Jm[i][j] += T.dNdr[p][a][j] * J->x[i * J->N + nd];
+    }
+    double det, inv[3][3];
+    if (J->dim == 3) {
+      const double *d = &Jm[0][0];
+      det = d[1] * d[5] * d[6] - d[2] * d[4] * d[6] + d[2] * d[3] * d[7] - d[0] * d[5] * d[7] -
+            d[1] * d[3] * d[8] + d[0] * d[4] * d[8];
+      if (det < 0.0) return false;
+      double *o = &inv[0][0];
+      o[0] = (d[4] * d[8] - d[5] * d[7]) / det;
+      o[1] = (d[2] * d[7] - d[1] * d[8]) / det;
+      o[2] = (d[1] * d[5] - d[2] * d[4]) / det;
+      o[3] = (d[5] * d[6] - d[3] * d[8]) / det;
+      o[4] = (d[0] * d[8] - d[2] * d[6]) / det;
+      o[5] = (d[2] * d[3] - d[0] * d[5]) / det;
+      o[6] = (d[3] * d[7] - d[4] * d[6]) / det;
+      o[7] = (d[1] * d[6] - d[0] * d[7]) / det;
+      o[8] = (d[0] * d[4] - d[1] * d[3]) / det;
+    } else {
+      det = Jm[0][0] * Jm[1][1] - Jm[1][0] * Jm[0][1];
+      if (det < 0.0) return false;
+      inv[0][0] = Jm[1][1] / det;
+      inv[0][1] = -Jm[0][1] / det;
+      inv[1][0] = -Jm[1][0] / det;
+      inv[1][1] = Jm[0][0] / det;
+    }
+    J->detJ[g] = det;
+    double *dn = &J->dN[g * J->nn * J->dim];
+    for (int a = 0; a < J->nn; a++)
+      for (int j = 0; j < J->dim; j++) {
+        double s = 0.0;
+        for (int k = 0; k < J->dim; k++) s += T.dNdr[p][a][k] * inv[k][j];
+        dn[a * J->dim + j] = s;
+      }
+    if (J->dim == 2) {
+      double r = 0.0;
+      for (int a = 0; a < J->nn; a++) r += T.N[p][a] * J->x[cn[a]];
+      J->radius[g] = r;
+    }
+    if (reference) {
+      J->detJ0[g] = det;
+      if (J->dim == 2) J->detJ0[g] = det * J->radius[g];
+    }
+  }
+  return true;
+}
+
+bool compute_jacobian(OracleJob *J, bool reference) {
+  bool ok = true;
+#pragma omp parallel for schedule(static) num_threads(J->threads) reduction(&& : ok)
+  for (int64_t e = 0; e < J->NE; e++) ok = element_jacobian(J, e, reference) && ok;
+  if (!ok) set_err(J, DNLG_E_NEG_JACOBIAN, "negative detJ");
+  return ok;
+}
+
+/* Element::computeMassMatrix (Element.C:304-339) assembled as in
+ * Model::computeMassMatrix (Model.C:537-590). */
+void compute_mass(OracleJob *J) {
+  J->mass.assign(J->N, 0.0);
+  std::vector<double> me(J->nn);
+  for (int64_t e = 0; e < J->NE; e++) {
+    for (int a = 0; a < J->nn; a++) me[a] = 0.0;
+    for (int p = 0; p < J->nip; p++) {
+      const int64_t g = e * J->nip + p;
+      double wdj = J->tab.w[p] * J->detJ[g];
+      if (J->dim == 2) wdj *= k2PI * J->radius[g];
+      for (int a = 0; a < J->nn; a++) me[a] += J->mat.density * J->tab.N[p][a] * wdj;
+    }
+    for (int a = 0; a < J->nn; a++) J->mass[J->conn[e * J->nn + a]] += me[a];
+  }
+}
+
+/* Characteristic lengths: ElQua4NAx::getCharacteristicLength
+ * (ElQua4NAx.C:312-321); ElHex8N3D::getCharacteristicLength / getVolume
+ * (ElHex8N3D.C:614-684). */
+double char_length(const OracleJob *J, int64_t e) {
+  const int32_t *cn = &J->conn[e * J->nn];
+  auto X = [&](int a, int c) { return J->x[c * J->N + cn[a]]; };
+  if (J->dim == 2) {
+    const double x31 = X(2, 0) - X(0, 0);
+    const double x24 = X(1, 0) - X(3, 0);
+    const double y42 = X(3, 1) - X(1, 1);
+    const double y31 = X(2, 1) - X(0, 1);
+    return ((x31 * y42 + x24 * y31) / std::sqrt(2.0 * (sq(x24) + sq(y42) + sq(x31) + sq(y31))));
+  }
+  auto aert = [&](int p1, int p2, int p3, int p4) {
+    const double x1 = X(p1, 0), x2 = X(p2, 0), x3 = X(p3, 0), x4 = X(p4, 0);
+    const double y1 = X(p1, 1), y2 = X(p2, 1), y3 = X(p3, 1), y4 = X(p4, 1);
+    const double z1 = X(p1, 2), z2 = X(p2, 2), z3 = X(p3, 2), z4 = X(p4, 2);
+    return std::sqrt(4 * (sq(x1 - x3) + sq(y1 - y3) + sq(z1 - z3)) * (sq(x2 - x4) + sq(y2 - y4) + sq(z2 - z4)) -
+                     4 * sq(-(x2 * x3) + x1 * (x2 - x4) + x3 * x4 + y1 * y2 - y2 * y3 - y1 * y4 + y3 * y4 + z1 * z2 -
+                            z2 * z3 - z1 * z4 + z3 * z4)) /
+           4;
+  };
+  auto mx3 = [](double i, double j, double k) { return (i > j) ? ((i > k) ? i : k) : ((j > k) ? j : k); };
+  double A1 = aert(0, 1, 2, 3), A2 = aert(0, 1, 5, 4), A3 = aert(1, 2, 6, 5);
+  double A4 = aert(3, 2, 6, 7), A5 = aert(0, 3, 7, 4), A6 = aert(4, 5, 6, 7);
+  A1 = mx3(A1, A2, A3);
+  A2 = mx3(A4, A5, A6);
+  A1 = (A1 > A2) ? A1 : A2;
+  /* volume from three pyramids */
+  auto sub = [&](int a, int b, double o[3]) {
+    for (int c = 0; c < 3; c++) o[c] = X(a, c) - X(b, c);
+  };
+  auto cross = [](const double a[3], const double b[3], double o[3]) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+  };
+  auto nrm = [](const double a[3]) { return std::sqrt(sq(a[0]) + sq(a[1]) + sq(a[2])); };
+  auto pyramid = [&](int o, int a, int b, int c, int apex) {
+    double va[3], vb[3], vc[3], n1[3], n2[3], nn[3], h[3];
+    sub(a, o, va);
+    sub(b, o, vb);
+    sub(c, o, vc);
+    cross(va, vb, n1);
+    cross(vc, vb, n2);
+    const double base = (nrm(n1) + nrm(n2)) / 2.0;
+    cross(va, vc, nn);
+    const double ln = nrm(nn);
+    for (int k = 0; k < 3; k++) nn[k] /= ln;
+    sub(apex, o, h);
+    double ht = nn[0] * h[0] + nn[1] * h[1] + nn[2] * h[2];
+    ht = (ht < 0) ? -ht : ht;
+    return base * ht / 3.0;
+  };
+  double vol = 0.0;
+  vol += pyramid(0, 1, 2, 3, 7);
+  vol += pyramid(0, 1, 5, 4, 7);
+  vol += pyramid(1, 2, 6, 5, 7);
+  return vol / A1;
+}
+
+/* Element::getElongationWaveSpeed (Element.C:343-359) */
+double wave_speed(const OracleJob *J, int64_t e) {
+  double rho = 0.0;
+  for (int p = 0; p < J->nip; p++) rho += J->rho[e * J->nip + p];
+  rho /= J->nip;
+  const double nu = J->mat.poisson;
+  return std::sqrt((J->mat.young * (1.0 - nu)) / (rho * (1.0 + nu) * (1.0 - 2.0 * nu)));
+}
+
+/* Model::computeCourantTimeStep (Model.C:636-670) */
+double courant(OracleJob *J) {
+  double crit = char_length(J, 0) / wave_speed(J, 0);
+  for (int64_t e = 1; e < J->NE; e++) {
+    const double ts = char_length(J, e) / wave_speed(J, e);
+    if (ts < crit) crit = ts;
+  }
+  return crit;
+}
+
+/* Solver::computeTimeStep (Solver.C:196-236), Courant branch */
+void compute_time_step(OracleJob *J, bool force) {
+  if (force || (J->inc % J->prm.dt_freq == 0)) {
+    const double fmax = 2.0 / courant(J);
+    J->dt = J->prm.safety * J->omegaS / fmax;
+  }
+}
+
+/* Element::computeStrains (Element.C:1103-1122) with the deformation
+ * gradient of Element3D.C:172-195 / ElementAxi.C:78-100, built from the
+ * field0 displacement increment. */
+void element_strains(OracleJob *J, int64_t e) {
+  const int32_t *cn = &J->conn[e * J->nn];
+  const std::vector<double> &u = J->fu[J->f0];
+  for (int p = 0; p < J->nip; p++) {
+    const int64_t g = e * J->nip + p;
+    const double *dn = &J->dN[g * J->nn * J->dim];
+    double F[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    double vr = 0.0;
+    for (int a = 0; a < J->nn; a++) {
+      const int64_t nd = cn[a];
+      for (int i = 0; i < J->dim; i++)
+        for (int j = 0; j < J->dim; j++) F[3 * i + j] += dn[a * J->dim + j] * u[i * J->N + nd];
+      if (J->dim == 2) vr += J->tab.N[p][a] * u[nd];
+    }
+    if (J->dim == 2) F[8] += vr / J->radius[g];
+    polar_lnU(F, &J->dE[g * 6], &J->Rr[g * 9]);
+    if (J->prm.track_tensors)
+      for (int c = 0; c < 6; c++) J->eps[g * 6 + c] += J->dE[g * 6 + c];
+  }
+}
+
+/* Element::computePressure (Element.C:1125-1143) */
+void element_pressure(OracleJob *J, int64_t e) {
+  const double K = J->mat.young / (3.0 * (1.0 - 2.0 * J->mat.poisson));
+  double inc = 0.0;
+  for (int p = 0; p < J->nip; p++) {
+    const double *d = &J->dE[(e * J->nip + p) * 6];
+    inc += (d[0] + d[3] + d[5]);
+  }
+  inc /= J->nip;
+  for (int p = 0; p < J->nip; p++) {
+    const int64_t g = e * J->nip + p;
+    J->pres[g] = third_trace(&J->sig[g * 6]) + K * inc;
+  }
+}
+
+/* Element::computeStress (Element.C:407-622): J2 radial return with a
+ * safeguarded Newton-Raphson on gamma, adiabatic heating, energies. */
+bool element_stress(OracleJob *J, int64_t e, double timeStep) {
+  const double gammaInitial = 1e-8, tolNR = 1e-8;
+  const int itMax = 250;
+  const double TwoG = J->mat.young / (1.0 + J->mat.poisson);
+  const double TwoG32 = kSqrt32 * TwoG;
+  const double heatFrac = J->mat.taylor_quinney / (J->mat.density * J->mat.heat_capacity);
+  const Law &L = J->law;
+  for (int p = 0; p < J->nip; p++) {
+    const int64_t g = e * J->nip + p;
+    double old[6], s[6], dd[6];
+    for (int c = 0; c < 6; c++) old[c] = J->sig[g * 6 + c];
+    deviator(old, s);
+    const double Snorm0 = norm_sym(s);
+    deviator(&J->dE[g * 6], dd);
+    for (int c = 0; c < 6; c++) s[c] += TwoG * dd[c];
+    const double Snorm = norm_sym(s);
+    const double Strial = kSqrt32 * Snorm;
+    const double T0 = J->T[g];
+    double T = T0;
+    double epsp = J->ep[g];
+    double depsp = J->dep[g];
+    double gamma = 0.0;
+    double yield = J->sy[g];
+    if (yield == 0.0) {
+      yield = law_yield(L, gammaInitial, gammaInitial / timeStep, T);
+      J->sy[g] = yield;
+    }
+    int iterate = 0;
+    if (Strial > yield) {
+      double gmin = 0.0;
+      double gmax = (Strial - yield) / TwoG32;
+      gamma = J->gam[g];
+      if (epsp == 0.0) gamma = kSqrt32 * gammaInitial;
+      depsp = kSqrt23 * gamma / timeStep;
+      epsp = J->ep[g] + kSqrt23 * gamma;
+      T = T0 + 0.5 * gamma * heatFrac * (kSqrt23 * yield + Snorm0);
+      bool run = true;
+      while (run) {
+        yield = law_yield(L, epsp, depsp, T);
+        const double fun = Strial - gamma * TwoG32 - yield;
+        if (fun < 0.0)
+          gmax = gamma;
+        else
+          gmin = gamma;
+        const double hard = law_dyield(L, epsp, depsp, T, timeStep);
+        const double dfun = TwoG32 + kSqrt23 * hard;
+        double dgamma = fun / dfun;
+        gamma += dgamma;
+        if ((gmax - gamma) * (gamma - gmin) < 0.0) {
+          dgamma = 0.5 * (gmax - gmin);
+          gamma = gmin + dgamma;
+        }
+        if (std::fabs(dgamma) < tolNR) {
+          run = false;
+        } else {
+          depsp = kSqrt23 * gamma / timeStep;
+          epsp = J->ep[g] + kSqrt23 * gamma;
+          T = T0 + 0.5 * gamma * heatFrac * (kSqrt23 * yield + Snorm0);
+          iterate += 1;
+          if (iterate > itMax) return false;
+        }
+      }
+      if (J->prm.track_tensors)
+        for (int c = 0; c < 6; c++) J->epsP[g * 6 + c] += gamma * s[c] / Snorm;
+      const double scale = 1.0 - TwoG * gamma / Snorm;
+      for (int c = 0; c < 6; c++) s[c] *= scale;
+      J->ep[g] += kSqrt23 * gamma;
+      J->dep[g] = kSqrt23 * gamma / timeStep;
+      J->gam[g] = gamma;
+      J->gamc[g] += gamma;
+      J->sy[g] = yield;
+    }
+    double nw[6];
+    for (int c = 0; c < 6; c++) nw[c] = s[c];
+    nw[0] += J->pres[g];
+    nw[3] += J->pres[g];
+    nw[5] += J->pres[g];
+    for (int c = 0; c < 6; c++) J->sig[g * 6 + c] = nw[c];
+    double sum[6];
+    for (int c = 0; c < 6; c++) sum[c] = old[c] + nw[c];
+    const double power = 0.5 * ddot_sym(&J->dE[g * 6], sum);
+    J->eint[g] += power / J->mat.density;
+    if (gamma != 0.0) {
+      const double w = 0.5 * gamma * (norm_sym(s) + Snorm0);
+      J->einel[g] += w / J->mat.density;
+      J->T[g] += heatFrac * w;
+    }
+  }
+  return true;
+}
+
+/* Element::computeFinalRotation (Element.C:1146-1161) */
+void element_rotation(OracleJob *J, int64_t e) {
+  for (int p = 0; p < J->nip; p++) {
+    const int64_t g = e * J->nip + p;
+    rotate_sym(&J->Rr[g * 9], &J->sig[g * 6]);
+    if (J->prm.track_tensors) {
+      rotate_sym(&J->Rr[g * 9], &J->eps[g * 6]);
+      rotate_sym(&J->Rr[g * 9], &J->epsP[g * 6]);
+    }
+  }
+}
+
+/* Element::computeInternalForces (Element.C:362-404) */
+void element_forces(OracleJob *J, int64_t e) {
+  double *f = &J->elemF[e * J->nn * J->dim];
+  for (int k = 0; k < J->nn * J->dim; k++) f[k] = 0.0;
+  for (int p = 0; p < J->nip; p++) {
+    const int64_t g = e * J->nip + p;
+    const double *dn = &J->dN[g * J->nn * J->dim];
+    const double *S = &J->sig[g * 6];
+    double wdj = J->tab.w[p] * J->detJ[g];
+    if (J->dim == 2) wdj *= k2PI * J->radius[g];
+    for (int a = 0; a < J->nn; a++) {
+      for (int i = 0; i < J->dim; i++)
+        for (int j = 0; j < J->dim; j++) f[a * J->dim + i] -= dn[a * J->dim + j] * S[sym_idx(j, i)] * wdj;
+      if (J->dim == 2) f[a * J->dim] -= J->tab.N[p][a] * S[5] / J->radius[g] * wdj;
+    }
+  }
+}
+
+/* Element::computeDensity (Element.C:1243-1256) */
+void element_density(OracleJob *J, int64_t e) {
+  for (int p = 0; p < J->nip; p++) {
+    const int64_t g = e * J->nip + p;
+    J->rho[g] = J->mat.density * J->detJ0[g] / J->detJ[g];
+    if (J->dim == 2) J->rho[g] /= J->radius[g];
+  }
+}
+
+/* Model::computeInternalForces (Model.C:864-899): element-order assembly */
+void assemble_forces(OracleJob *J) {
+  J->fint.assign(J->dim * J->N, 0.0);
+  for (int64_t e = 0; e < J->NE; e++)
+    for (int a = 0; a < J->nn; a++) {
+      const int64_t gl = (int64_t)J->conn[e * J->nn + a] * J->dim;
+      for (int d = 0; d < J->dim; d++) J->fint[gl + d] += J->elemF[(e * J->nn + a) * J->dim + d];
+    }
+}
+
+/* the five element phases strains .. forces of one cycle */
+bool element_phases(OracleJob *J, double timeStep) {
+  bool ok = true;
+#pragma omp parallel for schedule(static) num_threads(J->threads) reduction(&& : ok)
+  for (int64_t e = 0; e < J->NE; e++) {
+    element_strains(J, e);
+    element_pressure(J, e);
+    ok = element_stress(J, e, timeStep) && ok;
+    element_rotation(J, e);
+    element_forces(J, e);
+  }
+  if (!ok) {
+    set_err(J, DNLG_E_NR_DIVERGED, "no convergence in Newton-Raphson");
+    return false;
+  }
+  assemble_forces(J);
+  return true;
+}
+
+/* BoundaryRestrain / BoundarySpeed ::applyConstantOnNewFields
+ * (BoundaryRestrain.C:98-150, BoundarySpeed.C:139-226), node's constant
+ * list in attach order (BoundaryCondition.C:174-190). */
+void apply_constant_bc(OracleJob *J, int64_t n, double time, double timeStep) {
+  if (J->cptr.empty()) return;
+  const int a = J->f0, b = 1 - J->f0;
+  for (int64_t k = J->cptr[n]; k < J->cptr[n + 1]; k++) {
+    const dnlg_boundary &B = J->bnd[J->cidx[k]];
+    for (int d = 0; d < 3; d++) {
+      const int64_t q = d * J->N + n;
+      if (B.kind == DNLG_BC_RESTRAIN) {
+        if (B.value[d] > 0) {
+          J->fu[a][q] = 0.0;
+          J->fu[b][q] = 0.0;
+          J->fv[a][q] = 0.0;
+          J->fv[b][q] = 0.0;
+          J->fa[a][q] = 0.0;
+          J->fa[b][q] = 0.0;
+        }
+      } else if (B.value[d] != 0.0) {
+        if (B.ramp == DNLG_RAMP_NONE) {
+          J->fa[a][q] = 0.0;
+          J->fa[b][q] = 0.0;
+          J->fv[a][q] = B.value[d];
+          J->fv[b][q] = B.value[d];
+          J->fu[a][q] = J->fv[a][q] * timeStep;
+          J->fu[b][q] = J->fv[b][q] * timeStep;
+        } else {
+          const double f1 = ramp_value(B.ramp, B.ramp_min, B.ramp_max, time + timeStep);
+          const double f0 = ramp_value(B.ramp, B.ramp_min, B.ramp_max, time);
+          J->fa[a][q] = (f1 - f0) * B.value[d] / timeStep;
+          J->fa[b][q] = J->fa[a][q];
+          J->fv[a][q] = f0 * B.value[d];
+          J->fv[b][q] = f1 * B.value[d];
+          J->fu[a][q] = J->fv[a][q] * timeStep;
+          J->fu[b][q] = J->fv[b][q] * timeStep;
+        }
+      }
+    }
+  }
+}
+
+/* Explicit::computePredictions (Explicit.C:438-482) */
+void predictor(OracleJob *J) {
+  const int a = J->f0, b = 1 - J->f0;
+  const double h = J->dt;
+  for (int64_t n = 0; n < J->N; n++) {
+    for (int d = 0; d < 3; d++) {
+      const int64_t q = d * J->N + n;
+      J->fu[b][q] = h * (J->fv[a][q] + (0.5 - J->beta) * h * J->fa[a][q]);
+      J->fv[b][q] = J->fv[a][q] + (1.0 - J->gammaCH) * h * J->fa[a][q];
+      J->fa[b][q] = 0.0;
+    }
+    apply_constant_bc(J, n, J->t, h);
+  }
+}
+
+/* Explicit::explicitSolve (Explicit.C:489-536) with MatrixDiag::solve
+ * (MatrixDiag.C:751-756) */
+void corrector(OracleJob *J) {
+  const int a = J->f0, b = 1 - J->f0;
+  const double h = J->dt;
+  for (int64_t n = 0; n < J->N; n++) {
+    for (int d = 0; d < J->dim; d++) J->fa[b][d * J->N + n] = J->fint[n * J->dim + d] / J->mass[n];
+    for (int d = 0; d < 3; d++) {
+      const int64_t q = d * J->N + n;
+      J->fa[b][q] -= J->alphaM * J->fa[a][q];
+      J->fa[b][q] /= (1.0 - J->alphaM);
+    }
+    for (int d = 0; d < 3; d++) {
+      const int64_t q = d * J->N + n;
+      J->fv[b][q] += J->gammaCH * h * J->fa[b][q];
+    }
+    for (int d = 0; d < 3; d++) {
+      const int64_t q = d * J->N + n;
+      J->fu[b][q] += J->beta * sq(h) * J->fa[b][q];
+    }
+    apply_constant_bc(J, n, J->t, h);
+    for (int d = 0; d < 3; d++) {
+      const int64_t q = d * J->N + n;
+      J->disp[q] += J->fu[b][q];
+      J->x[q] += J->fu[b][q];
+    }
+  }
+}
+
+void densities(OracleJob *J) {
+#pragma omp parallel for schedule(static) num_threads(J->threads)
+  for (int64_t e = 0; e < J->NE; e++) element_density(J, e);
+}
+
+bool nodes_finite(const OracleJob *J) {
+  for (int64_t q = 0; q < 3 * J->N; q++)
+    if (!std::isfinite(J->x[q]) || !std::isfinite(J->fv[J->f0][q])) return false;
+  return true;
+}
+
+}  // namespace
+
+/* ===========================================================================
+ * C API of the oracle (mirrors the engine's dnlg_* entry points).
+ * ========================================================================= */
+extern "C" {
+
+int oracle_job_create(const dnlg_mesh *mesh, const dnlg_material *mat, const dnlg_bcs *bcs,
+                      const dnlg_solver_params *prm, OracleJob **out) {
+  if (!mesh || !mat || !prm || !out) return DNLG_E_INVALID;
+  if (mesh->elem_type != DNLG_EL_QUA4N_AX && mesh->elem_type != DNLG_EL_HEX8N_3D) return DNLG_E_INVALID;
+  if (mesh->n_nodes <= 0 || mesh->n_elems <= 0 || !mesh->coords || !mesh->conn) return DNLG_E_INVALID;
+  if (prm->dt_method != DNLG_DT_COURANT || prm->stress_method != DNLG_STRESS_NR) return DNLG_E_INVALID;
+  if (prm->dt_freq <= 0) return DNLG_E_INVALID;
+  OracleJob *J = new OracleJob();
+  J->tab = make_tables(mesh->elem_type);
+  J->dim = J->tab.dim;
+  J->nn = J->tab.nn;
+  J->nip = J->tab.nip;
+  J->N = mesh->n_nodes;
+  J->NE = mesh->n_elems;
+  J->mat = *mat;
+  J->prm = *prm;
+  J->law.kind = mat->law;
+  for (int k = 0; k < 8; k++) J->law.p[k] = mat->p[k];
+  J->law.eta = mat->taylor_quinney;
+  J->law.rho = mat->density;
+  J->law.cp = mat->heat_capacity;
+  /* Explicit::computeChungHulbertIntegrationParameters (Explicit.C:86-104) */
+  const double rb = prm->rho_b;
+  J->alphaM = (2.0 * rb - 1.0) / (1.0 + rb);
+  J->beta = (5.0 - 3.0 * rb) / ((1.0 + rb) * (1.0 + rb) * (2.0 - rb));
+  J->gammaCH = 1.5 - J->alphaM;
+  J->omegaS = std::sqrt((12.0 * std::pow(1.0 + rb, 3) * (2.0 - rb)) /
+                        (10.0 + 15.0 * rb - rb * rb + std::pow(rb, 3) - std::pow(rb, 4)));
+  J->conn.assign(mesh->conn, mesh->conn + J->NE * J->nn);
+  for (int64_t k = 0; k < J->NE * J->nn; k++)
+    if (J->conn[k] < 0 || J->conn[k] >= J->N) {
+      delete J;
+      return DNLG_E_INVALID;
+    }
+  J->x.assign(3 * J->N, 0.0);
+  for (int d = 0; d < 3; d++)
+    if (d < J->dim)
+      for (int64_t n = 0; n < J->N; n++) J->x[d * J->N + n] = mesh->coords[d * J->N + n];
+  /* ElementAxi::checkLevel2 (ElementAxi.C:103-126) */
+  if (J->dim == 2)
+    for (int64_t n = 0; n < J->N; n++)
+      if (J->x[n] < 0.0) {
+        if (J->x[n] > -1e-8)
+          J->x[n] = 0.0;
+        else {
+          delete J;
+          return DNLG_E_INVALID;
+        }
+      }
+  J->disp.assign(3 * J->N, 0.0);
+  for (int k = 0; k < 2; k++) {
+    J->fu[k].assign(3 * J->N, 0.0);
+    J->fv[k].assign(3 * J->N, 0.0);
+    J->fa[k].assign(3 * J->N, 0.0);
+  }
+  const int64_t G = J->NE * J->nip;
+  J->detJ.assign(G, 0.0);
+  J->detJ0.assign(G, 0.0);
+  J->radius.assign(G, 1.0);
+  J->dN.assign(G * J->nn * J->dim, 0.0);
+  J->sig.assign(G * 6, 0.0);
+  J->dE.assign(G * 6, 0.0);
+  J->Rr.assign(G * 9, 0.0);
+  if (prm->track_tensors) {
+    J->eps.assign(G * 6, 0.0);
+    J->epsP.assign(G * 6, 0.0);
+  }
+  J->pres.assign(G, 0.0);
+  J->ep.assign(G, 0.0);
+  J->dep.assign(G, 0.0);
+  J->sy.assign(G, 0.0);
+  J->gam.assign(G, 0.0);
+  J->gamc.assign(G, 0.0);
+  J->T.assign(G, mat->T0);      /* Element::initializeData (Element.C:258-301) */
+  J->rho.assign(G, mat->density);
+  J->eint.assign(G, 0.0);
+  J->einel.assign(G, 0.0);
+  J->elemF.assign(J->NE * J->nn * J->dim, 0.0);
+  /* boundary conditions */
+  if (bcs && bcs->n_boundaries > 0) {
+    J->bnd.assign(bcs->boundaries, bcs->boundaries + bcs->n_boundaries);
+    if (bcs->constant_ptr) {
+      J->cptr.assign(bcs->constant_ptr, bcs->constant_ptr + J->N + 1);
+      J->cidx.assign(bcs->constant_idx, bcs->constant_idx + J->cptr[J->N]);
+    }
+    /* BoundaryCondition::applyInitial (BoundaryCondition.C:131-150):
+     * initial list then constant list; BoundarySpeed::applyInitial
+     * (BoundarySpeed.C:63-76) sets the whole speed vector; restrain is a
+     * no-op at this stage (BoundaryRestrain.C:60-64). */
+    for (int64_t n = 0; n < J->N; n++) {
+      auto apply = [&](const int64_t *ptr, const int32_t *idx) {
+        if (!ptr) return;
+        for (int64_t k = ptr[n]; k < ptr[n + 1]; k++) {
+          const dnlg_boundary &B = bcs->boundaries[idx[k]];
+          if (B.kind != DNLG_BC_SPEED) continue;
+          for (int d = 0; d < 3; d++) {
+            J->fv[0][d * J->N + n] = B.value[d];
+            J->fv[1][d * J->N + n] = B.value[d];
+            J->fa[0][d * J->N + n] = 0.0;
+            J->fa[1][d * J->N + n] = 0.0;
+          }
+        }
+      };
+      apply(bcs->initial_ptr, bcs->initial_idx);
+      apply(bcs->constant_ptr, bcs->constant_idx);
+    }
+  }
+  J->t = prm->t_start;
+  *out = J;
+  return DNLG_OK;
+}
+
+int oracle_job_destroy(OracleJob *J) {
+  delete J;
+  return DNLG_OK;
+}
+
+int oracle_job_set_threads(OracleJob *J, int threads) {
+  J->threads = threads < 1 ? 1 : threads;
+  return DNLG_OK;
+}
+
+/* Explicit::solve (Explicit.C:126-253).  The pre-loop block runs on the
+ * first call only. */
+int oracle_job_solve(OracleJob *J, double upTo) {
+  if (J->status != DNLG_OK) return J->status;
+  if (upTo > J->prm.t_end) upTo = J->prm.t_end;
+  if (!J->started) {
+    J->started = true;
+    if (!compute_jacobian(J, true)) return J->status;
+    compute_mass(J);
+    compute_time_step(J, true);
+    if (!element_phases(J, J->dt)) return J->status;
+  }
+  while (J->t < upTo) {
+    /* Explicit::updateTimes (Explicit.C:414-427) */
+    if (J->t + J->dt > upTo) J->dt = upTo - J->t;
+    J->inc++;
+    predictor(J);
+    if (!element_phases(J, J->dt)) return J->status;
+    corrector(J);
+    densities(J);
+    /* Explicit::endStep (Explicit.C:539-552): time, field swap */
+    J->t += J->dt;
+    J->f0 = 1 - J->f0;
+    if (!nodes_finite(J)) {
+      set_err(J, DNLG_E_NONFINITE, "non-finite nodal state");
+      return J->status;
+    }
+    if (J->t < upTo) {
+      if (!compute_jacobian(J, false)) return J->status;
+      compute_time_step(J, false);
+    }
+  }
+  return DNLG_OK;
+}
+
+int oracle_job_sizes(const OracleJob *J, int64_t *n_nodes, int64_t *n_elems, int32_t *n_ip, int32_t *dim) {
+  if (n_nodes) *n_nodes = J->N;
+  if (n_elems) *n_elems = J->NE;
+  if (n_ip) *n_ip = J->nip;
+  if (dim) *dim = J->dim;
+  return DNLG_OK;
+}
+
+int oracle_job_read_nodes(const OracleJob *J, double *coords, double *disp, double *speed) {
+  if (coords) std::memcpy(coords, J->x.data(), 3 * J->N * sizeof(double));
+  if (disp) std::memcpy(disp, J->disp.data(), 3 * J->N * sizeof(double));
+  if (speed) std::memcpy(speed, J->fv[J->f0].data(), 3 * J->N * sizeof(double));
+  return DNLG_OK;
+}
+
+int oracle_job_read_mass(const OracleJob *J, double *mass) {
+  std::memcpy(mass, J->mass.data(), J->N * sizeof(double));
+  return DNLG_OK;
+}
+
+int oracle_job_read_ip(const OracleJob *J, int field, double *out) {
+  const int64_t G = J->NE * J->nip;
+  auto scalar = [&](const std::vector<double> &v) {
+    std::memcpy(out, v.data(), G * sizeof(double));
+    return DNLG_OK;
+  };
+  auto tensor = [&](const std::vector<double> &v) {
+    if (v.empty()) return DNLG_E_STATE;
+    for (int c = 0; c < 6; c++)
+      for (int64_t g = 0; g < G; g++) out[c * G + g] = v[g * 6 + c];
+    return DNLG_OK;
+  };
+  switch (field) {
+    case DNLG_F_PLASTIC_STRAIN: return scalar(J->ep);
+    case DNLG_F_PLASTIC_STRAIN_RATE: return scalar(J->dep);
+    case DNLG_F_GAMMA: return scalar(J->gam);
+    case DNLG_F_GAMMA_CUMULATE: return scalar(J->gamc);
+    case DNLG_F_YIELD_STRESS: return scalar(J->sy);
+    case DNLG_F_TEMPERATURE: return scalar(J->T);
+    case DNLG_F_DENSITY: return scalar(J->rho);
+    case DNLG_F_PRESSURE:
+      for (int64_t g = 0; g < G; g++) out[g] = third_trace(&J->sig[g * 6]);
+      return DNLG_OK;
+    case DNLG_F_INTERNAL_ENERGY: return scalar(J->eint);
+    case DNLG_F_INELASTIC_ENERGY: return scalar(J->einel);
+    case DNLG_F_VON_MISES:
+      for (int64_t g = 0; g < G; g++) {
+        const double *s = &J->sig[g * 6];
+        out[g] = std::sqrt((sq(s[0] - s[3]) + sq(s[0] - s[5]) + sq(s[3] - s[5]) +
+                            6 * (sq(s[1]) + sq(s[2]) + sq(s[4]))) / 2);
+      }
+      return DNLG_OK;
+    case DNLG_F_STRESS: return tensor(J->sig);
+    case DNLG_F_STRAIN: return tensor(J->eps);
+    case DNLG_F_PLASTIC_STRAIN_TENSOR: return tensor(J->epsP);
+  }
+  return DNLG_E_INVALID;
+}
+
+int oracle_job_scalars(const OracleJob *J, double *time, double *time_step, int64_t *increment) {
+  if (time) *time = J->t;
+  if (time_step) *time_step = J->dt;
+  if (increment) *increment = J->inc;
+  return DNLG_OK;
+}
+
+/* KE as Model::getTotalKineticEnergy (Model.C:612-625); IE and plastic work
+ * integrated over the reference volume (no reference counterpart). */
+int oracle_job_energies(const OracleJob *J, double out[4]) {
+  double ke = 0.0;
+  for (int64_t n = 0; n < J->N; n++) {
+    double v2 = 0.0;
+    for (int d = 0; d < 3; d++) v2 += sq(J->fv[J->f0][d * J->N + n]);
+    ke += (J->mass.empty() ? 0.0 : J->mass[n]) * v2 / 2.0;
+  }
+  double ie = 0.0, wp = 0.0;
+  for (int64_t g = 0; g < J->NE * J->nip; g++) {
+    double vol = J->tab.w[g % J->nip] * J->detJ0[g];
+    if (J->dim == 2) vol *= k2PI;
+    ie += J->mat.density * vol * J->eint[g];
+    wp += J->mat.density * vol * J->einel[g];
+  }
+  out[0] = ke;
+  out[1] = ie;
+  out[2] = wp;
+  out[3] = ke + ie;
+  return DNLG_OK;
+}
+
+int oracle_job_status(const OracleJob *J) { return J->status; }
+
+/* ---- unit-level entry points for pinning tests ---- */
+void oracle_eig3(const double A[9], double Q[9], double w[3]) {
+  double a[3][3], q[3][3];
+  for (int i = 0; i < 9; i++) a[i / 3][i % 3] = A[i];
+  kopp_eig3(a, q, w);
+  for (int i = 0; i < 9; i++) Q[i] = q[i / 3][i % 3];
+}
+
+void oracle_polar_lnU(const double F[9], double dE[6], double R[9]) { polar_lnU(F, dE, R); }
+
+void oracle_rotate_sym(const double R[9], double S[6]) { rotate_sym(R, S); }
+
+double oracle_law_yield(const dnlg_material *m, double epsp, double depsp, double T) {
+  Law L{m->law, {0}, m->taylor_quinney, m->density, m->heat_capacity};
+  for (int k = 0; k < 8; k++) L.p[k] = m->p[k];
+  return law_yield(L, epsp, depsp, T);
+}
+
+double oracle_law_dyield(const dnlg_material *m, double epsp, double depsp, double T, double dtime) {
+  Law L{m->law, {0}, m->taylor_quinney, m->density, m->heat_capacity};
+  for (int k = 0; k < 8; k++) L.p[k] = m->p[k];
+  return law_dyield(L, epsp, depsp, T, dtime);
+}
+
+/* one-IP radial return for unit tests: state[] = {sig[6], ep, dep, sy, gam,
+ * gamc, T, eint, einel}; dE[6]; returns 0 or DNLG_E_NR_DIVERGED */
+int oracle_radial_return(const dnlg_material *m, const double dE[6], double pressure, double timeStep,
+                         double state[14]) {
+  OracleJob J;
+  J.nip = 1;
+  J.mat = *m;
+  J.law.kind = m->law;
+  for (int k = 0; k < 8; k++) J.law.p[k] = m->p[k];
+  J.law.eta = m->taylor_quinney;
+  J.law.rho = m->density;
+  J.law.cp = m->heat_capacity;
+  J.sig.assign(state, state + 6);
+  J.dE.assign(dE, dE + 6);
+  J.pres.assign(1, pressure);
+  J.ep.assign(1, state[6]);
+  J.dep.assign(1, state[7]);
+  J.sy.assign(1, state[8]);
+  J.gam.assign(1, state[9]);
+  J.gamc.assign(1, state[10]);
+  J.T.assign(1, state[11]);
+  J.eint.assign(1, state[12]);
+  J.einel.assign(1, state[13]);
+  if (!element_stress(&J, 0, timeStep)) return DNLG_E_NR_DIVERGED;
+  for (int c = 0; c < 6; c++) state[c] = J.sig[c];
+  state[6] = J.ep[0];
+  state[7] = J.dep[0];
+  state[8] = J.sy[0];
+  state[9] = J.gam[0];
+  state[10] = J.gamc[0];
+  state[11] = J.T[0];
+  state[12] = J.eint[0];
+  state[13] = J.einel[0];
+  return DNLG_OK;
+}
+
+/* shape tables: N [nip*nn], dNdr [nip*nn*3], w [nip]; returns nip */
+int oracle_tables(int type, double *N, double *dNdr, double *w) {
+  Tables T = make_tables(type);
+  for (int p = 0; p < T.nip; p++) {
+    w[p] = T.w[p];
+    for (int a = 0; a < T.nn; a++) {
+      N[p * T.nn + a] = T.N[p][a];
+      for (int c = 0; c < 3; c++) dNdr[(p * T.nn + a) * 3 + c] = T.dNdr[p][a][c];
+    }
+  }
+  return T.nip;
+}
+
+/* characteristic length of one element given its node coordinates
+ * xyz[nn*3] (node-major) */
+double oracle_char_length(int type, const double *xyz) {
+  OracleJob J;
+  J.tab = make_tables(type);
+  J.dim = J.tab.dim;
+  J.nn = J.tab.nn;
+  J.N = J.nn;
+  J.x.assign(3 * J.nn, 0.0);
+  for (int a = 0; a < J.nn; a++)
+    for (int c = 0; c < 3; c++) J.x[c * J.N + a] = xyz[a * 3 + c];
+  J.conn.resize(J.nn);
+  for (int a = 0; a < J.nn; a++) J.conn[a] = a;
+  return char_length(&J, 0);
+}
+
+void oracle_chung_hulbert(double rho_b, double out[4]) {
+  const double rb = rho_b;
+  out[0] = (2.0 * rb - 1.0) / (1.0 + rb);
+  out[1] = (5.0 - 3.0 * rb) / ((1.0 + rb) * (1.0 + rb) * (2.0 - rb));
+  out[2] = 1.5 - out[0];
+  out[3] = std::sqrt((12.0 * std::pow(1.0 + rb, 3) * (2.0 - rb)) /
+                     (10.0 + 15.0 * rb - rb * rb + std::pow(rb, 3) - std::pow(rb, 4)));
+}
+
+double oracle_ramp_value(int type, double mn, double mx, double x) { return ramp_value(type, mn, mx, x); }
+
+}  // extern "C"

@@ -1,0 +1,73 @@
+"""FP64 engine vs the FP64 CPU oracle on the same models (GPU).
+
+Tolerances (FP64 vs FP64): the engine contracts multiply-adds into FMAs and
+uses the device libm, so results differ from the oracle by rounding that the
+time loop accumulates.  Stated bounds: nodal displacement 1e-9 normwise
+relative, IP fields 1e-7 normwise relative, time and increment count exact.
+"""
+import numpy as np
+import pytest
+
+from helpers import disp_err, rel_norm_err
+
+pytestmark = pytest.mark.gpu
+
+DISP_TOL = 1e-9
+FIELD_TOL = 1e-7
+
+
+def run_both(model, t_end):
+    from oracle.bridge import OracleJob
+    a = model.descriptors()
+    o = OracleJob(a)
+    assert o.solve(t_end) == 0
+    model.solve(t_end)
+    return o
+
+
+def compare(model, o, fields=("plasticStrain", "T", "Stress", "density", "internalEnergy")):
+    import dnlPython as dnl
+    t, dt, inc = model.getScalars()
+    to, dto, inco = o.scalars()
+    assert inc == inco
+    assert t == pytest.approx(to, rel=1e-12)
+    _, d_ref, v_ref = o.nodes()
+    err = disp_err(model.getNodalDisplacements(), d_ref)
+    print(f"increments={inc} disp_err={err:.3e}")
+    assert err <= DISP_TOL
+    ids = {"plasticStrain": 0, "T": 5, "Stress": 11, "density": 6, "internalEnergy": 8}
+    for f in fields:
+        g = model.getIntPointField(getattr(dnl.Field, f))
+        r = o.ip(ids[f])
+        e = rel_norm_err(g, r)
+        print(f"  {f}: {e:.3e}")
+        assert e <= FIELD_TOL, f
+    np.testing.assert_allclose(model.getEnergies(), o.energies(), rtol=1e-7, atol=1e-12)
+
+
+def test_taylor_axi_short(ctx):
+    from dnlPython import samples
+    m = samples.taylor_axi(t_end=10e-6, context=ctx)
+    o = run_both(m, 10e-6)
+    compare(m, o)
+
+
+def test_taylor_axi_full(ctx):
+    from dnlPython import samples
+    m = samples.taylor_axi(t_end=80e-6, context=ctx)
+    o = run_both(m, 80e-6)
+    compare(m, o)
+
+
+def test_taylor_3d_small(ctx):
+    from dnlPython import samples
+    m = samples.taylor_3d(n=4, nz=10, t_end=20e-6, context=ctx)
+    o = run_both(m, 20e-6)
+    compare(m, o)
+
+
+def test_necking_small(ctx):
+    from dnlPython import samples
+    m = samples.necking_3d(n=4, nz=12, t_end=20e-6, pull=5.0, context=ctx)
+    o = run_both(m, 20e-6)
+    compare(m, o)
