@@ -1,0 +1,176 @@
+#!/usr/bin/env python3
+"""Benchmark of the explicit-solve engine on one MI355X.
+
+Workload: 3D Taylor impact of a full copper cylinder, 40 x 40 x 75 = 120 000
+8-node hexes (127 756 nodes), Johnson-Cook, 227 m/s, 80 us end time,
+Courant step every 10 increments.  One step = one complete solve of that
+model, from t = 0 to 80 us, on a job whose state is already resident in HBM
+(jobs are created before the timed region).
+
+Metric: elem-cycles/s = n_elem x cycles / wall seconds.  ``value`` is the
+mixed-precision path (FP32 strain / stress / forces, FP64 kinematics, FP64
+radial-return Newton and energies); the plain FP64 path runs on the same
+model for the speed-up and the error of the mixed result against it.
+
+The roofline object is measured on the dominant kernel (the hex element
+kernel) with HIP events recorded around each of its launches on the engine's
+stream during the timed region; its algorithmic bytes per element-cycle are
+the compulsory state traffic of the layout (see ALGO_BYTES below).
+
+cpu_baseline: the FP64 CPU oracle (the restatement of the reference loop,
+test infrastructure under oracle/) timed on this host on a bounded sample of
+the same model (the first 2 us of the impact), all element phases OpenMP
+parallel.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "dynela-v.-4.0_amd"))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algo_bytes_per_elem(prec, nodes_per_elem):
+    """Compulsory HBM bytes of one element-cycle in the element kernel.
+
+    Per IP: the persistent state read and written once (FP64 path: stress 6,
+    ep, dep, sy, gamma, gamma_cum, T, e_int, e_inel = 14 doubles each way;
+    mixed path: stress 6 + dep, sy, gamma, gamma_cum in FP32, ep, T, e_int,
+    e_inel in FP64 = 72 B each way) plus detJ0 read (8 B / 4 B).  Per
+    element: connectivity 8 x 4 B, the per-element force vector written
+    (24 x 8 B / 24 x 4 B).  Per node (x nodes/element): coordinates and the
+    field0 increment read once (6 x 8 B).
+    """
+    if prec == "fp64":
+        ip = 14 * 8 * 2 + 8
+        force = 24 * 8
+    else:
+        ip = 72 * 2 + 4
+        force = 24 * 4
+    return 8 * ip + 8 * 4 + force + nodes_per_elem * 6 * 8
+
+
+def make_model(prec, ctx, n=20, nz=100, t_end=80e-6):
+    from dnlPython import samples
+    return samples.taylor_3d(n=n, nz=nz, t_end=t_end, precision=prec, track_tensors=False, context=ctx)
+
+
+def run_precision(prec, ctx, steps, warmup, n, nz, t_end):
+    models = [make_model(prec, ctx, n, nz, t_end) for _ in range(steps + warmup)]
+    for m in models:
+        m.initSolve()
+    for m in models[:warmup]:
+        m.solve()
+    ctx.set_timing(True)
+    ctx.reset_timing()
+    t0 = time.perf_counter()
+    for m in models[warmup:]:
+        m.solve()
+    wall = time.perf_counter() - t0
+    ctx.set_timing(False)
+    el_ms, el_n = ctx.kernel_time("element")
+    ne = models[0].getElementsNumber()
+    nn = models[0].getNodesNumber()
+    cycles = [m.getIncrement() for m in models[warmup:]]
+    disp = models[-1].getNodalDisplacements()
+    out = dict(wall=wall, cycles=cycles, ne=ne, nn=nn, elem_ms=el_ms, elem_launches=el_n,
+               value=ne * sum(cycles) / wall, disp=disp)
+    for m in models:
+        m.close()
+    return out
+
+
+def cpu_baseline(n, nz, sample_t=1e-6):
+    from dnlPython import samples
+    from oracle.bridge import OracleJob
+    threads = min(16, os.cpu_count() or 1)
+    m = samples.taylor_3d(n=n, nz=nz, t_end=80e-6, precision="fp64", track_tensors=False)
+    job = OracleJob(m.descriptors(), threads=threads)
+    t0 = time.perf_counter()
+    job.solve(sample_t)
+    wall = time.perf_counter() - t0
+    _, _, inc = job.scalars()
+    ne = m.getElementsNumber()
+    return dict(value=ne * inc / wall, unit="elem-cycles/s", cores=threads, kind="port",
+                sample=f"first {sample_t * 1e6:g} us of the same model ({inc} cycles x {ne} elements), "
+                       f"FP64 oracle, OpenMP element phases", wall_s=wall)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", default="mixed,fp64")
+    ap.add_argument("--n", type=int, default=20)
+    ap.add_argument("--nz", type=int, default=100)
+    ap.add_argument("--t-end", type=float, default=80e-6)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(independent jobs would shard one per GPU, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import dnlPython as dnl
+    ctx = dnl.Context.default()
+    precs = [p.strip() for p in args.precision.split(",") if p.strip()]
+    res = {}
+    for p in precs:
+        print(f"[bench] precision={p} steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
+        res[p] = run_precision(p, ctx, args.steps, args.warmup, args.n, args.nz, args.t_end)
+        r = res[p]
+        print(f"[bench] {p}: {r['value']:.4g} elem-cycles/s, cycles={r['cycles']}, wall={r['wall']:.3f}s",
+              file=sys.stderr, flush=True)
+    head = precs[0]
+    r = res[head]
+    npe = r["nn"] / r["ne"]
+    bytes_el = algo_bytes_per_elem(head, npe)
+    avg_ms = r["elem_ms"] / max(r["elem_launches"], 1)
+    achieved = bytes_el * r["ne"] / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    line = {
+        "metric": "elem-cycles/s (mixed precision; FP64 path and error vs FP64 alongside)",
+        "value": r["value"],
+        "unit": "elem-cycles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["wall"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64" if head == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"Taylor impact 3D, {args.n}x{args.n}x{args.nz} hex8 "
+                               f"({r['ne']} elements, {r['nn']} nodes), Johnson-Cook copper, 227 m/s, "
+                               f"{args.t_end * 1e6:g} us",
+                   "precision": head, "cycles_per_step": r["cycles"][0]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_elem (hex8)", "kernel_avg_ms": avg_ms,
+                     "algo_bytes_per_elem_cycle": bytes_el},
+    }
+    for p in precs[1:]:
+        line[f"value_{p}"] = res[p]["value"]
+    if "mixed" in res and "fp64" in res:
+        line["speedup_mixed_vs_fp64"] = res["mixed"]["value"] / res["fp64"]["value"]
+        d64 = res["fp64"]["disp"]
+        dmx = res["mixed"]["disp"]
+        err = np.linalg.norm(dmx - d64, axis=0).max() / np.linalg.norm(d64, axis=0).max()
+        line["mixed_disp_err_vs_fp64"] = float(err)
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline (oracle)", file=sys.stderr, flush=True)
+        cb = cpu_baseline(args.n, args.nz)
+        line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

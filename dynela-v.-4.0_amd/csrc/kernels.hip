@@ -30,6 +30,10 @@ __constant__ double c_hexN[8][8];
 __constant__ double c_hexD[8][8][3];
 __constant__ double c_quaN[4][4];
 __constant__ double c_quaD[4][4][2];
+__constant__ float c_hexNf[8][8];
+__constant__ float c_hexDf[8][8][3];
+__constant__ float c_quaNf[4][4];
+__constant__ float c_quaDf[4][4][2];
 
 template <int NN>
 __device__ __forceinline__ double tabN(int p, int a) {
@@ -325,6 +329,12 @@ __global__ __launch_bounds__(BLOCK) void k_elem_fp64(JobDev *const *jobs, const 
     for (int i = 0; i < DIM; i++) out[i] = f[i];
   }
 }
+
+}  // namespace dnlg
+
+#include "elem_mixed.h"
+
+namespace dnlg {
 
 // ---------------------------------------------------------------------------
 // Nodal kernels
@@ -723,9 +733,27 @@ void upload_tables(const double *hexN, const double *hexD, const double *quaN, c
   hipMemcpyToSymbol(HIP_SYMBOL(c_hexD), hexD, sizeof(double) * 192);
   hipMemcpyToSymbol(HIP_SYMBOL(c_quaN), quaN, sizeof(double) * 16);
   hipMemcpyToSymbol(HIP_SYMBOL(c_quaD), quaD, sizeof(double) * 32);
+  float f[192];
+  for (int k = 0; k < 64; k++) f[k] = (float)hexN[k];
+  hipMemcpyToSymbol(HIP_SYMBOL(c_hexNf), f, sizeof(float) * 64);
+  for (int k = 0; k < 192; k++) f[k] = (float)hexD[k];
+  hipMemcpyToSymbol(HIP_SYMBOL(c_hexDf), f, sizeof(float) * 192);
+  for (int k = 0; k < 16; k++) f[k] = (float)quaN[k];
+  hipMemcpyToSymbol(HIP_SYMBOL(c_quaNf), f, sizeof(float) * 16);
+  for (int k = 0; k < 32; k++) f[k] = (float)quaD[k];
+  hipMemcpyToSymbol(HIP_SYMBOL(c_quaDf), f, sizeof(float) * 32);
 }
 
 int elem_block_threads(int type) { return type == DNLG_EL_HEX8N_3D ? 128 : 256; }
+int elem_block_threads_mixed(int type) { return type == DNLG_EL_HEX8N_3D ? 128 : 256; }
+
+void launch_elem_mixed(int type, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s) {
+  if (nblk <= 0) return;
+  if (type == DNLG_EL_HEX8N_3D)
+    hipLaunchKernelGGL((k_elem_mixed<8, 3, 128>), dim3(nblk), dim3(128), 0, s, jobs, tasks);
+  else
+    hipLaunchKernelGGL((k_elem_mixed<4, 2, 256>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
+}
 
 void launch_elem_fp64(int type, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s) {
   if (nblk <= 0) return;

@@ -7,8 +7,9 @@ row-major from 1, elements from 1 (Samples/Taylor/Taylor-Axi/Taylor.py:43-67).
 
 Workloads
   taylor_axi   2D axisymmetric Taylor impact, 4-node quads (10 x 50)
-  taylor_3d    3D Taylor impact, full cylinder of 8-node hexes
-               (n x n cross-section mapped to the disc, nz layers)
+  taylor_3d    3D Taylor impact, full cylinder of 8-node hexes (O-grid
+               cross-section of 3 n^2 cells, nz layers; n=20, nz=100 gives
+               120 000 hexes)
   necking_3d   3D round bar in tension with a 2 % mid-length taper,
                Voce + linear hardening, one end fixed, the other pulled
   sweep_params seeded parameter sweep around the copper impact
@@ -92,45 +93,83 @@ def taylor_axi(nr=10, nz=50, radius=3.2e-3, length=32.4e-3, speed=227.0, t_end=8
     return model
 
 
-def _disc_grid(n, radius):
-    """(n+1)^2 points of a square mapped onto the disc (elliptical grid
-    mapping); returns X, Z of shape (n+1, n+1) indexed [a, b]."""
-    u = np.linspace(-1.0, 1.0, n + 1)
+def ogrid_disc(m, k, s=0.42):
+    """O-grid ("butterfly") quad mesh of the unit disc: an m x m square core of
+    half-width s plus four blocks of m x k cells joining the core edges to the
+    circle.  Returns points [P, 2] (x, z) and counter-clockwise quads [Q, 4]
+    (indices into points); m*m + 4*m*k cells."""
+    pts, quads = [], []
+    u = np.linspace(-1.0, 1.0, m + 1)
+    # core
     a, b = np.meshgrid(u, u, indexing="ij")
-    X = a * np.sqrt(1.0 - b * b / 2.0)
-    Z = b * np.sqrt(1.0 - a * a / 2.0)
-    return radius * X, radius * Z
+    core = np.stack([s * a, s * b], axis=-1)
+    base = 0
+    pts.append(core.reshape(-1, 2))
+    ia, ib = np.meshgrid(np.arange(m), np.arange(m), indexing="ij")
+    q = lambda i, j: base + i * (m + 1) + j  # noqa: E731
+    quads.append(np.stack([q(ia, ib), q(ia + 1, ib), q(ia + 1, ib + 1), q(ia, ib + 1)], -1).reshape(-1, 4))
+    base += (m + 1) * (m + 1)
+    # four radial blocks: edge parameter t in [-1, 1], radial j in [0, k]
+    w = np.linspace(0.0, 1.0, k + 1)
+    for rot in range(4):
+        ang = rot * np.pi / 2
+        c, sn = np.cos(ang), np.sin(ang)
+        t, jj = np.meshgrid(u, w, indexing="ij")
+        pin = np.stack([np.full_like(t, s), s * t], -1)
+        th = (np.pi / 4) * t
+        pout = np.stack([np.cos(th), np.sin(th)], -1)
+        p = (1 - jj)[..., None] * pin + jj[..., None] * pout
+        p = np.stack([c * p[..., 0] - sn * p[..., 1], sn * p[..., 0] + c * p[..., 1]], -1)
+        pts.append(p.reshape(-1, 2))
+        it, ij = np.meshgrid(np.arange(m), np.arange(k), indexing="ij")
+        q = lambda i, j, b0=base: b0 + i * (k + 1) + j  # noqa: E731
+        quads.append(np.stack([q(it, ij), q(it, ij + 1), q(it + 1, ij + 1), q(it + 1, ij)], -1).reshape(-1, 4))
+        base += (m + 1) * (k + 1)
+    pts = np.concatenate(pts)
+    quads = np.concatenate(quads)
+    # merge coincident points (core edges and block diagonals)
+    key = np.round(pts / (1e-9 + 0.25 / max(m, k)) * 1e6).astype(np.int64)
+    _, first, inv = np.unique(key, axis=0, return_index=True, return_inverse=True)
+    order = np.argsort(first)
+    remap = np.empty_like(order)
+    remap[order] = np.arange(order.size)
+    pts = pts[first[order]]
+    quads = remap[inv.ravel()[quads]]
+    # counter-clockwise orientation in the (x, z) plane
+    x, z = pts[quads, 0], pts[quads, 1]
+    area = 0.5 * np.sum(x * np.roll(z, -1, 1) - np.roll(x, -1, 1) * z, axis=1)
+    quads[area < 0] = quads[area < 0][:, ::-1]
+    return pts, quads
 
 
-def _hex_cylinder(n, nz, radius, ys):
-    """Full-cylinder hex mesh, axis y.  ys: nz+1 axial levels; radius may be
-    an array over levels.  Returns numbers, xyz [N,3], conn [NE,8] (1-based
-    node numbers)."""
-    X, Z = _disc_grid(n, 1.0)
-    nl = (n + 1) * (n + 1)
+def _hex_cylinder(n, nz, radius, ys, k=None):
+    """Full-cylinder hex mesh, axis y, O-grid cross-section (n x n core cells
+    plus 4 x n x k ring cells).  ys: nz+1 axial levels; radius may be an
+    array over levels.  Returns numbers, xyz [N,3], conn [NE,8] (1-based node
+    numbers) and the node count of one level."""
+    k = max(1, n // 2) if k is None else k
+    pts, quads = ogrid_disc(n, k)
+    nl = pts.shape[0]
     r = np.broadcast_to(np.asarray(radius, dtype=np.float64), (nz + 1,))
-    xyz = np.zeros((nz + 1, n + 1, n + 1, 3))
-    xyz[..., 0] = r[:, None, None] * X[None]
-    xyz[..., 1] = np.asarray(ys)[:, None, None]
-    xyz[..., 2] = r[:, None, None] * Z[None]
+    xyz = np.zeros((nz + 1, nl, 3))
+    xyz[..., 0] = r[:, None] * pts[None, :, 0]
+    xyz[..., 1] = np.asarray(ys)[:, None]
+    xyz[..., 2] = r[:, None] * pts[None, :, 1]
     nums = np.arange(1, (nz + 1) * nl + 1)
-
-    def nid(k, a, b):
-        return k * nl + a * (n + 1) + b + 1
-
-    k, a, b = np.meshgrid(np.arange(nz), np.arange(n), np.arange(n), indexing="ij")
-    # local xi -> b (z), eta -> a (x), zeta -> k (y): right-handed, detJ > 0
-    conn = np.stack([nid(k, a, b), nid(k, a, b + 1), nid(k, a + 1, b + 1), nid(k, a + 1, b),
-                     nid(k + 1, a, b), nid(k + 1, a, b + 1), nid(k + 1, a + 1, b + 1), nid(k + 1, a + 1, b)],
-                    axis=-1).reshape(-1, 8)
+    lev = np.arange(nz)[:, None, None] * nl
+    # quads are counter-clockwise in (x, z); with the axis along y the
+    # right-handed hex order (detJ > 0) walks each face clockwise in (x, z)
+    bot = quads[:, ::-1][None] + lev + 1
+    conn = np.concatenate([bot, bot + nl], axis=-1).reshape(-1, 8)
     return nums, xyz.reshape(-1, 3), conn, nl
 
 
-def taylor_3d(n=40, nz=75, radius=3.2e-3, length=32.4e-3, speed=227.0, t_end=80e-6, material=None,
+def taylor_3d(n=20, nz=100, radius=3.2e-3, length=32.4e-3, speed=227.0, t_end=80e-6, material=None,
               precision="fp64", track_tensors=True, context=None, freq=10):
-    """3D Taylor impact of a full cylinder (no symmetry planes): n x n
-    cross-section, nz layers (n=40, nz=75 -> 120 000 hexes); bottom face
-    restrained along the axis (y), initial speed -speed on every node."""
+    """3D Taylor impact of a full cylinder (no symmetry planes): O-grid
+    cross-section of n*n + 4*n*(n/2) hexes, nz layers (n=20, nz=100 ->
+    120 000 hexes); bottom face restrained along the axis (y), initial speed
+    -speed on every node."""
     p = dict(COPPER)
     if material:
         p.update(material)
@@ -205,8 +244,8 @@ def sweep_params(n_jobs=256, seed=20261203, lo=2000, hi=60000):
             nz = max(2, int(round(ne / nr)))
             shape = (nr, nz)
         else:
-            nc = max(2, int(round((ne / 1.875) ** (1.0 / 3.0))))
-            nz = max(2, int(round(ne / (nc * nc))))
+            nc = max(2, int(round((ne / 15.0) ** (1.0 / 3.0))))
+            nz = max(2, int(round(ne / (3 * nc * nc))))
             shape = (nc, nz)
         jobs.append(dict(kind=kind, shape=shape, material=mat, speed=speed))
     return jobs

@@ -1,0 +1,45 @@
+"""Batched solve: uneven jobs advanced together give the same results as
+each job alone (GPU).  FP64 jobs use per-job fixed-order reductions, so the
+results are expected bit-identical."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _models(ctx, prec):
+    from dnlPython import samples
+    return [samples.taylor_axi(6, 30, speed=200.0, t_end=15e-6, precision=prec, context=ctx),
+            samples.taylor_3d(4, 12, speed=250.0, t_end=12e-6, precision=prec, context=ctx),
+            samples.taylor_axi(10, 50, speed=280.0, t_end=10e-6, precision=prec, context=ctx),
+            samples.necking_3d(4, 10, t_end=9e-6, pull=4.0, precision=prec, context=ctx)]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_batch_equals_alone(ctx, prec):
+    import dnlPython as dnl
+    alone = _models(ctx, prec)
+    for m in alone:
+        m.solve()
+    mixed = _models(ctx, prec)
+    dnl.batch_solve(mixed)
+    for a, b in zip(alone, mixed):
+        assert a.getScalars() == b.getScalars()
+        np.testing.assert_array_equal(a.getNodalDisplacements(), b.getNodalDisplacements())
+        np.testing.assert_array_equal(a.getIntPointField(dnl.Field.plasticStrain),
+                                      b.getIntPointField(dnl.Field.plasticStrain))
+        np.testing.assert_array_equal(a.getIntPointField(dnl.Field.Stress), b.getIntPointField(dnl.Field.Stress))
+
+
+def test_batch_in_segments(ctx):
+    """Advancing a batch in several solve calls equals one call."""
+    import dnlPython as dnl
+    one = _models(ctx, "fp64")
+    dnl.batch_solve(one)
+    seg = _models(ctx, "fp64")
+    for t in (3e-6, 7e-6, None):
+        dnl.batch_solve(seg, upTo=t)
+    for a, b in zip(one, seg):
+        ta, _, _ = a.getScalars()
+        tb, _, _ = b.getScalars()
+        assert ta == pytest.approx(tb, rel=1e-12)

@@ -71,3 +71,48 @@ def test_necking_small(ctx):
     m = samples.necking_3d(n=4, nz=12, t_end=20e-6, pull=5.0, context=ctx)
     o = run_both(m, 20e-6)
     compare(m, o)
+
+
+# ---- mixed precision: FP32 strain / stress / forces, FP64 kinematics ------
+# Bound stated for the mixed path: nodal displacement <= 1e-5 normwise
+# relative to the FP64 oracle; IP fields are reported (looser, FP32 state).
+MIXED_DISP_TOL = 1e-5
+
+
+def compare_mixed(model, o):
+    import dnlPython as dnl
+    _, d_ref, _ = o.nodes()
+    err = disp_err(model.getNodalDisplacements(), d_ref)
+    out = {"disp": err}
+    ids = {"plasticStrain": 0, "T": 5, "Stress": 11}
+    for f, k in ids.items():
+        out[f] = rel_norm_err(model.getIntPointField(getattr(dnl.Field, f)), o.ip(k))
+    e_g, e_o = model.getEnergies(), o.energies()
+    out["energy"] = float(abs(e_g[3] - e_o[3]) / abs(e_o[3]))
+    print("mixed errors:", {k: f"{v:.2e}" for k, v in out.items()})
+    return out
+
+
+def test_mixed_taylor_axi(ctx):
+    from dnlPython import samples
+    m = samples.taylor_axi(t_end=80e-6, precision="mixed", track_tensors=False, context=ctx)
+    o = run_both(m, 80e-6)
+    e = compare_mixed(m, o)
+    assert e["disp"] <= MIXED_DISP_TOL
+
+
+def test_mixed_taylor_3d(ctx):
+    from dnlPython import samples
+    m = samples.taylor_3d(n=6, nz=30, t_end=40e-6, precision="mixed", track_tensors=False, context=ctx)
+    o = run_both(m, 40e-6)
+    e = compare_mixed(m, o)
+    assert e["disp"] <= MIXED_DISP_TOL
+
+
+def test_mixed_necking(ctx):
+    from dnlPython import samples
+    m = samples.necking_3d(n=4, nz=12, t_end=40e-6, pull=5.0, precision="mixed", track_tensors=False,
+                           context=ctx)
+    o = run_both(m, 40e-6)
+    e = compare_mixed(m, o)
+    assert e["disp"] <= MIXED_DISP_TOL
