@@ -68,11 +68,13 @@ def run_precision(prec, ctx, steps, warmup, n, nz, t_end):
         m.initSolve()
     for m in models[:warmup]:
         m.solve()
+        print(f"[bench] {prec} warmup solve done", file=sys.stderr, flush=True)
     ctx.set_timing(True)
     ctx.reset_timing()
     t0 = time.perf_counter()
-    for m in models[warmup:]:
+    for k, m in enumerate(models[warmup:]):
         m.solve()
+        print(f"[bench] {prec} timed solve {k + 1}/{steps} done", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     ctx.set_timing(False)
     el_ms, el_n = ctx.kernel_time("element")
@@ -103,6 +105,19 @@ def cpu_baseline(n, nz, sample_t=1e-6):
                        f"FP64 oracle, OpenMP element phases", wall_s=wall)
 
 
+def copy_calibration(mib):
+    """Plain device copy of `mib` MiB (reads and writes mib MiB), five times:
+    the reference point that scales rocprofv3's FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_mib": mib, "bytes_read": n * 4, "bytes_written": n * 4}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,7 +128,12 @@ def main():
     ap.add_argument("--nz", type=int, default=100)
     ap.add_argument("--t-end", type=float, default=80e-6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run the plain-copy reference kernel for the traffic counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(independent jobs would shard one per GPU, see DESIGN.md)", file=sys.stderr)

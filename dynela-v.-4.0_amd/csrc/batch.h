@@ -41,7 +41,7 @@ struct dnlg_ctx {
   std::map<std::string, std::pair<double, int64_t>> ktime;
   std::vector<hipEvent_t> events;  // pool for timed launches
   std::vector<dnlg::BatchPtr> cache;
-  int graph_cycles = 8;  // cycles per captured HIP graph (0 = plain launches)
+  int graph_cycles = 16;  // cycles per captured HIP graph (0 = plain launches)
 };
 
 namespace dnlg {

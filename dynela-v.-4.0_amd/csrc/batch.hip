@@ -25,11 +25,15 @@ struct Batch {
   TaskList node[2][2];  // [dim==3?][mixed?]
   double *d_snap = nullptr;
   std::vector<double> h_snap;
-  hipGraphExec_t graph = nullptr;
+  hipGraphExec_t graph = nullptr;       // graph_cycles whole cycles
+  hipGraphExec_t graph_pre = nullptr;   // timed mode: pre phase
+  hipGraphExec_t graph_post = nullptr;  // timed mode: post phase
   int graph_cycles = 0;
   std::vector<void *> allocs;
   ~Batch() {
     if (graph) hipGraphExecDestroy(graph);
+    if (graph_pre) hipGraphExecDestroy(graph_pre);
+    if (graph_post) hipGraphExecDestroy(graph_post);
     for (void *p : allocs) hipFree(p);
   }
 };
@@ -126,72 +130,50 @@ int build_batch(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, Batch &B) {
   return 0;
 }
 
-struct Timer {
-  dnlg_ctx *ctx;
-  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> marks;
-  size_t used = 0;
-  explicit Timer(dnlg_ctx *c) : ctx(c) {}
-  template <typename F>
-  void run(const char *name, F &&f) {
-    if (!ctx->timing) {
-      f();
-      return;
-    }
-    while (ctx->events.size() < used + 2) {
-      hipEvent_t e;
-      hipEventCreate(&e);
-      ctx->events.push_back(e);
-    }
-    hipEvent_t a = ctx->events[used], b = ctx->events[used + 1];
-    used += 2;
-    hipEventRecord(a, ctx->stream);
-    f();
-    hipEventRecord(b, ctx->stream);
-    marks.push_back({name, {a, b}});
-  }
-  void collect() {
-    for (auto &m : marks) {
-      float ms = 0.f;
-      hipEventElapsedTime(&ms, m.second.first, m.second.second);
-      auto &acc = ctx->ktime[m.first];
-      acc.first += ms;
-      acc.second += 1;
-    }
-    marks.clear();
-    used = 0;
-  }
-};
-
-void launch_cycle(dnlg_ctx *ctx, Batch &B, Timer *tm) {
+// Phases of one cycle.  pre = updateTimes + predictor, element = the element
+// kernels, post = corrector + Courant + endStep.
+void launch_pre(dnlg_ctx *ctx, Batch &B) {
   hipStream_t s = ctx->stream;
-  const int n = (int)B.jobs.size();
-  auto timed = [&](const char *name, auto &&f) {
-    if (tm)
-      tm->run(name, f);
-    else
-      f();
-  };
-  timed("control", [&] { launch_cycle_begin(B.d_jobs, n, s); });
-  timed("predict", [&] {
-    for (int d3 = 0; d3 < 2; d3++)
-      for (int mx = 0; mx < 2; mx++) launch_predict(d3 ? 3 : 2, B.d_jobs, B.node[d3][mx].d, B.node[d3][mx].n, s);
-  });
-  timed("element", [&] {
-    for (int hx = 0; hx < 2; hx++) {
-      const int type = hx ? DNLG_EL_HEX8N_3D : DNLG_EL_QUA4N_AX;
-      launch_elem_fp64(type, B.d_jobs, B.elem[hx][0].d, B.elem[hx][0].n, s);
-      launch_elem_mixed(type, B.d_jobs, B.elem[hx][1].d, B.elem[hx][1].n, s);
+  launch_cycle_begin(B.d_jobs, (int)B.jobs.size(), s);
+  for (int d3 = 0; d3 < 2; d3++)
+    for (int mx = 0; mx < 2; mx++) launch_predict(d3 ? 3 : 2, B.d_jobs, B.node[d3][mx].d, B.node[d3][mx].n, s);
+}
+
+// ev: nullptr, or two events per element launch (up to 4 launches); each
+// launch is bracketed by event records on the engine's stream.
+int launch_element(dnlg_ctx *ctx, Batch &B, const hipEvent_t *ev) {
+  hipStream_t s = ctx->stream;
+  int k = 0;
+  for (int hx = 0; hx < 2; hx++) {
+    const int type = hx ? DNLG_EL_HEX8N_3D : DNLG_EL_QUA4N_AX;
+    for (int mx = 0; mx < 2; mx++) {
+      const TaskList &L = B.elem[hx][mx];
+      if (L.n <= 0) continue;
+      if (ev) hipEventRecord(ev[2 * k], s);
+      if (mx)
+        launch_elem_mixed(type, B.d_jobs, L.d, L.n, s);
+      else
+        launch_elem_fp64(type, B.d_jobs, L.d, L.n, s);
+      if (ev) hipEventRecord(ev[2 * k + 1], s);
+      k++;
     }
-  });
-  timed("correct", [&] {
-    for (int d3 = 0; d3 < 2; d3++)
-      for (int mx = 0; mx < 2; mx++)
-        launch_correct(d3 ? 3 : 2, mx == 1, B.d_jobs, B.node[d3][mx].d, B.node[d3][mx].n, s);
-  });
-  timed("courant", [&] {
-    for (int hx = 0; hx < 2; hx++) launch_courant(hx ? 3 : 2, B.d_jobs, B.cour[hx].d, B.cour[hx].n, 0, s);
-  });
-  timed("control", [&] { launch_cycle_end(B.d_jobs, n, s); });
+  }
+  return k;
+}
+
+void launch_post(dnlg_ctx *ctx, Batch &B) {
+  hipStream_t s = ctx->stream;
+  for (int d3 = 0; d3 < 2; d3++)
+    for (int mx = 0; mx < 2; mx++)
+      launch_correct(d3 ? 3 : 2, mx == 1, B.d_jobs, B.node[d3][mx].d, B.node[d3][mx].n, s);
+  for (int hx = 0; hx < 2; hx++) launch_courant(hx ? 3 : 2, B.d_jobs, B.cour[hx].d, B.cour[hx].n, 0, s);
+  launch_cycle_end(B.d_jobs, (int)B.jobs.size(), s);
+}
+
+void launch_cycle(dnlg_ctx *ctx, Batch &B) {
+  launch_pre(ctx, B);
+  launch_element(ctx, B, nullptr);
+  launch_post(ctx, B);
 }
 
 // Explicit.C:143-178: reference Jacobian (detJ0), lumped mass, forced time
@@ -225,18 +207,31 @@ Batch *find_or_build(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, int &st
   return ctx->cache.back().get();
 }
 
-int capture_graph(dnlg_ctx *ctx, Batch &B, int cycles) {
+template <typename F>
+int capture(dnlg_ctx *ctx, F &&f, hipGraphExec_t *out) {
   hipGraph_t g = nullptr;
   if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return DNLG_E_HIP;
-  for (int k = 0; k < cycles; k++) launch_cycle(ctx, B, nullptr);
+  f();
   if (hipStreamEndCapture(ctx->stream, &g) != hipSuccess) return DNLG_E_HIP;
   hipGraphExec_t ex = nullptr;
-  if (hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) != hipSuccess) {
-    hipGraphDestroy(g);
-    return DNLG_E_HIP;
-  }
+  const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
   hipGraphDestroy(g);
-  B.graph = ex;
+  if (e != hipSuccess) return DNLG_E_HIP;
+  *out = ex;
+  return 0;
+}
+
+// Untimed: `cycles` whole cycles in one graph.  Timed: the pre and post
+// phases as two graphs, the element kernels launched between them with
+// start/stop events.
+int capture_graphs(dnlg_ctx *ctx, Batch &B, int cycles, bool timed) {
+  int st;
+  if (timed) {
+    if ((st = capture(ctx, [&] { launch_pre(ctx, B); }, &B.graph_pre))) return st;
+    if ((st = capture(ctx, [&] { launch_post(ctx, B); }, &B.graph_post))) return st;
+    return 0;
+  }
+  if ((st = capture(ctx, [&] { for (int k = 0; k < cycles; k++) launch_cycle(ctx, B); }, &B.graph))) return st;
   B.graph_cycles = cycles;
   return 0;
 }
@@ -276,14 +271,35 @@ int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vec
     ctx->err = "building the batch failed";
     return st;
   }
-  const bool use_graph = !ctx->timing && ctx->graph_cycles > 0;
-  if (use_graph && !B->graph) {
-    if ((st = capture_graph(ctx, *B, ctx->graph_cycles))) {
+  const bool use_graph = ctx->graph_cycles > 0;
+  const bool timed = ctx->timing;
+  if (use_graph && !(timed ? B->graph_pre : B->graph)) {
+    if ((st = capture_graphs(ctx, *B, ctx->graph_cycles, timed))) {
       ctx->err = "graph capture failed";
       return st;
     }
   }
-  Timer tm(ctx);
+  constexpr int kTimedChunk = 64;  // cycles between event read-backs
+  std::vector<int> ev_used;        // events used per timed cycle
+  auto collect = [&]() -> int {
+    auto &acc = ctx->ktime["element"];
+    size_t base = 0;
+    for (int used : ev_used) {
+      for (int k = 0; k < used; k++) {
+        float ms = 0.f;
+        const hipError_t e = hipEventElapsedTime(&ms, ctx->events[base + 2 * k], ctx->events[base + 2 * k + 1]);
+        if (e != hipSuccess) {
+          ctx->err = std::string("kernel timing failed: ") + hipGetErrorString(e);
+          return DNLG_E_HIP;
+        }
+        acc.first += ms;
+        acc.second += 1;
+      }
+      base += 2 * used;
+    }
+    ev_used.clear();
+    return 0;
+  };
   while (true) {
     launch_snapshot(B->d_jobs, n, B->d_snap, s);
     if (hipMemcpyAsync(B->h_snap.data(), B->d_snap, 5 * n * sizeof(double), hipMemcpyDeviceToHost, s) !=
@@ -292,7 +308,7 @@ int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vec
       ctx->err = "device error during the cycle loop";
       return DNLG_E_HIP;
     }
-    tm.collect();
+    if ((st = collect())) return st;
     double need = 0.0;
     for (int i = 0; i < n; i++) {
       const double t = B->h_snap[5 * i], dt = B->h_snap[5 * i + 1], up = B->h_snap[5 * i + 2];
@@ -303,7 +319,30 @@ int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vec
     }
     if (need <= 0.0) break;
     const int chunk = (int)std::min(need, 4096.0);
-    if (use_graph) {
+    if (timed) {
+      const int cycles = std::min(chunk, kTimedChunk);
+      while (ctx->events.size() < (size_t)8 * cycles) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return DNLG_E_HIP;
+        ctx->events.push_back(e);
+      }
+      size_t base = 0;
+      for (int k = 0; k < cycles; k++) {
+        if (use_graph) {
+          if (hipGraphLaunch(B->graph_pre, s) != hipSuccess) return DNLG_E_HIP;
+        } else {
+          launch_pre(ctx, *B);
+        }
+        const int used = launch_element(ctx, *B, &ctx->events[base]);
+        ev_used.push_back(used);
+        base += 2 * used;
+        if (use_graph) {
+          if (hipGraphLaunch(B->graph_post, s) != hipSuccess) return DNLG_E_HIP;
+        } else {
+          launch_post(ctx, *B);
+        }
+      }
+    } else if (use_graph) {
       const int reps = (chunk + B->graph_cycles - 1) / B->graph_cycles;
       for (int r = 0; r < reps; r++)
         if (hipGraphLaunch(B->graph, s) != hipSuccess) {
@@ -311,11 +350,11 @@ int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vec
           return DNLG_E_HIP;
         }
     } else {
-      const int lim = ctx->timing ? std::min(chunk, 64) : chunk;
-      for (int k = 0; k < lim; k++) launch_cycle(ctx, *B, &tm);
+      for (int k = 0; k < chunk; k++) launch_cycle(ctx, *B);
     }
-    if (hipGetLastError() != hipSuccess) {
-      ctx->err = "kernel launch failed";
+    const hipError_t le = hipGetLastError();
+    if (le != hipSuccess) {
+      ctx->err = std::string("kernel launch failed: ") + hipGetErrorString(le);
       return DNLG_E_HIP;
     }
   }

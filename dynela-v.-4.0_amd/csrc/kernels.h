@@ -9,8 +9,12 @@ void upload_tables(const double *hexN, const double *hexD, const double *quaN, c
 int elem_block_threads(int type);
 int elem_block_threads_mixed(int type);
 
-void launch_elem_fp64(int type, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s);
-void launch_elem_mixed(int type, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s);
+// With ev != nullptr the launch records the kernel's start/stop in ev[0..1]
+// (hipExtLaunchKernel), so its duration is measured on the launch stream.
+void launch_elem_fp64(int type, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s,
+                      const hipEvent_t *ev = nullptr);
+void launch_elem_mixed(int type, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s,
+                       const hipEvent_t *ev = nullptr);
 void launch_predict(int dim, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s);
 void launch_correct(int dim, bool f32, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s);
 void launch_courant(int dim, JobDev *const *jobs, const Task *tasks, int nblk, int force, hipStream_t s);

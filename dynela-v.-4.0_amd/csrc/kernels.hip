@@ -15,6 +15,7 @@
 // The element Jacobian of Explicit.C:245-246 is recomputed inside the next
 // cycle's element kernel from the same coordinates, so nothing per-IP
 // geometric is stored between cycles.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -747,20 +748,44 @@ void upload_tables(const double *hexN, const double *hexD, const double *quaN, c
 int elem_block_threads(int type) { return type == DNLG_EL_HEX8N_3D ? 128 : 256; }
 int elem_block_threads_mixed(int type) { return type == DNLG_EL_HEX8N_3D ? 128 : 256; }
 
-void launch_elem_mixed(int type, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s) {
+void launch_elem_mixed(int type, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s,
+                       const hipEvent_t *ev) {
   if (nblk <= 0) return;
+  hipEvent_t a = ev ? ev[0] : nullptr, b = ev ? ev[1] : nullptr;
   if (type == DNLG_EL_HEX8N_3D)
-    hipLaunchKernelGGL((k_elem_mixed<8, 3, 128>), dim3(nblk), dim3(128), 0, s, jobs, tasks);
+  {
+    if (ev)
+      hipExtLaunchKernelGGL((k_elem_mixed<8, 3, 128>), dim3(nblk), dim3(128), 0, s, a, b, 0, jobs, tasks);
+    else
+      hipLaunchKernelGGL((k_elem_mixed<8, 3, 128>), dim3(nblk), dim3(128), 0, s, jobs, tasks);
+  }
   else
-    hipLaunchKernelGGL((k_elem_mixed<4, 2, 256>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
+  {
+    if (ev)
+      hipExtLaunchKernelGGL((k_elem_mixed<4, 2, 256>), dim3(nblk), dim3(256), 0, s, a, b, 0, jobs, tasks);
+    else
+      hipLaunchKernelGGL((k_elem_mixed<4, 2, 256>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
+  }
 }
 
-void launch_elem_fp64(int type, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s) {
+void launch_elem_fp64(int type, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s,
+                      const hipEvent_t *ev) {
   if (nblk <= 0) return;
+  hipEvent_t a = ev ? ev[0] : nullptr, b = ev ? ev[1] : nullptr;
   if (type == DNLG_EL_HEX8N_3D)
-    hipLaunchKernelGGL((k_elem_fp64<8, 3, 128>), dim3(nblk), dim3(128), 0, s, jobs, tasks);
+  {
+    if (ev)
+      hipExtLaunchKernelGGL((k_elem_fp64<8, 3, 128>), dim3(nblk), dim3(128), 0, s, a, b, 0, jobs, tasks);
+    else
+      hipLaunchKernelGGL((k_elem_fp64<8, 3, 128>), dim3(nblk), dim3(128), 0, s, jobs, tasks);
+  }
   else
-    hipLaunchKernelGGL((k_elem_fp64<4, 2, 256>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
+  {
+    if (ev)
+      hipExtLaunchKernelGGL((k_elem_fp64<4, 2, 256>), dim3(nblk), dim3(256), 0, s, a, b, 0, jobs, tasks);
+    else
+      hipLaunchKernelGGL((k_elem_fp64<4, 2, 256>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
+  }
 }
 
 void launch_predict(int dim, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s) {
