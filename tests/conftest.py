@@ -13,6 +13,7 @@ for p in (PKG, ROOT):
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs through libdnlg.so on the GPU)")
+    config.addinivalue_line("markers", "slow: CPU test of more than a few seconds")
 
 
 @pytest.fixture(scope="session")

@@ -116,3 +116,28 @@ def test_mixed_necking(ctx):
     o = run_both(m, 40e-6)
     e = compare_mixed(m, o)
     assert e["disp"] <= MIXED_DISP_TOL
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_taylor_axi_published_results_on_gpu(ctx, prec):
+    """The reference's Taylor-Axi sample (Samples/Taylor/Taylor-Axi/Taylor.py,
+    4340 steel, 5 x 50, 287 m/s, 80 us) through the engine: published final
+    length 26.52 mm and impact-face diameter 11.15 mm
+    (Documentation/Samples-Impacts.tex:106-107), centre element plastic
+    strain 1.78 and temperature 582.34 C."""
+    from dnlPython import samples
+    steel = dict(young=206000.0, poisson=0.3, density=7.83e-09, heatCapacity=4.6e+08, taylorQuinney=0.9,
+                 A=806.0, B=614.0, C=0.0089, n=0.168, m=1.1, depsp0=1.0, Tm=1540.0, T0=20.0)
+    m = samples.taylor_axi(nr=5, nz=50, radius=3.2, length=32.4, speed=287000.0, t_end=80e-6, material=steel,
+                           precision=prec, context=ctx)
+    m.solve()
+    import dnlPython as dnl
+    c = m.getNodalCoordinates()
+    L_f, D_f = c[1].max(), 2 * c[0][:6].max()
+    ep = m.getIntPointField(dnl.Field.plasticStrain)[0].mean()
+    T = m.getIntPointField(dnl.Field.T)[0].mean()
+    print(f"{prec}: L_f={L_f:.3f} D_f={D_f:.3f} ep={ep:.3f} T={T:.1f}")
+    assert L_f == pytest.approx(26.52, rel=1e-3)
+    assert D_f == pytest.approx(11.15, rel=1e-3)
+    assert ep == pytest.approx(1.78, rel=1e-2)
+    assert T == pytest.approx(582.34, rel=2e-3)
