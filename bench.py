@@ -89,6 +89,40 @@ def run_precision(prec, ctx, steps, warmup, n, nz, t_end):
     return out
 
 
+def run_sweep(prec, ctx, n_jobs, t_end, seed=20261203):
+    """Batched parameter sweep: n_jobs seeded Taylor impacts (2D axisymmetric
+    or 3D, 2k-60k elements log-uniform, Johnson-Cook A, B, n, C, m x U(0.8,
+    1.2), 150-300 m/s) advanced together by one batched solve (one launch per
+    phase and cycle for all jobs)."""
+    import dnlPython as dnl
+    from dnlPython import samples
+    jobs = samples.sweep_params(n_jobs, seed=seed)
+    models = [samples.build_sweep_job(j, t_end=t_end, precision=prec, context=ctx) for j in jobs]
+    for m in models:
+        m.initSolve()
+    print(f"[bench] sweep {prec}: {n_jobs} jobs, {sum(m.getElementsNumber() for m in models)} elements",
+          file=sys.stderr, flush=True)
+    ctx.set_timing(True)
+    ctx.reset_timing()
+    t0 = time.perf_counter()
+    statuses = dnl.batch_solve(models, raise_on_error=False)
+    wall = time.perf_counter() - t0
+    failed = [(i, s, models[i].getCurrentTime()) for i, s in enumerate(statuses) if s != 0]
+    if failed:
+        print(f"[bench] sweep {prec}: failed jobs (index, status, time): {failed}", file=sys.stderr, flush=True)
+    ctx.set_timing(False)
+    el_ms, el_n = ctx.kernel_time("element")
+    ec = sum(m.getElementsNumber() * m.getIncrement() for m in models)
+    cyc = [m.getIncrement() for m in models]
+    res = dict(wall=wall, value=ec / wall, elem_cycles=ec, jobs=n_jobs, failed=len(failed),
+               elements=sum(m.getElementsNumber() for m in models), max_cycles=max(cyc),
+               elem_ms=el_ms, elem_launches=el_n,
+               disp=[m.getNodalDisplacements() for m in models[:8]])
+    for m in models:
+        m.close()
+    return res
+
+
 def cpu_baseline(n, nz, sample_t=1e-6):
     from dnlPython import samples
     from oracle.bridge import OracleJob
@@ -128,6 +162,8 @@ def main():
     ap.add_argument("--nz", type=int, default=100)
     ap.add_argument("--t-end", type=float, default=80e-6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="taylor3d", choices=["taylor3d", "sweep"])
+    ap.add_argument("--jobs", type=int, default=256, help="sweep workload: number of jobs")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the plain-copy reference kernel for the traffic counters")
     args = ap.parse_args()
@@ -142,6 +178,30 @@ def main():
     import dnlPython as dnl
     ctx = dnl.Context.default()
     precs = [p.strip() for p in args.precision.split(",") if p.strip()]
+    if args.workload == "sweep":
+        res = {}
+        for p in precs:
+            res[p] = run_sweep(p, ctx, args.jobs, args.t_end)
+            print(f"[bench] sweep {p}: {res[p]['value']:.4g} elem-cycles/s, wall {res[p]['wall']:.2f}s, "
+                  f"max cycles {res[p]['max_cycles']}", file=sys.stderr, flush=True)
+        r = res[precs[0]]
+        line = {"metric": "batched elem-cycles/s (parameter sweep)", "value": r["value"], "unit": "elem-cycles/s",
+                "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": r["wall"] * 1e3, "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64" if precs[0] == "mixed" else "f64",
+                "data": "synthetic",
+                "config": {"workload": f"batched sweep of {r['jobs']} Taylor impacts (2D axi / 3D hex, "
+                                       f"{r['elements']} elements in total), {args.t_end * 1e6:g} us",
+                           "precision": precs[0], "max_cycles": r["max_cycles"], "failed_jobs": r["failed"]},
+                "element_kernel_ms_per_cycle": r["elem_ms"] / max(r["elem_launches"], 1)}
+        for p in precs[1:]:
+            line[f"value_{p}"] = res[p]["value"]
+        if "mixed" in res and "fp64" in res:
+            line["speedup_mixed_vs_fp64"] = res["mixed"]["value"] / res["fp64"]["value"]
+            errs = [float(np.linalg.norm(a - b, axis=0).max() / np.linalg.norm(b, axis=0).max())
+                    for a, b in zip(res["mixed"]["disp"], res["fp64"]["disp"])]
+            line["mixed_disp_err_vs_fp64_max"] = max(errs)
+        print(json.dumps(line), flush=True)
+        return
     res = {}
     for p in precs:
         print(f"[bench] precision={p} steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)

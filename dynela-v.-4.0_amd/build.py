@@ -16,7 +16,7 @@ FLAGS = [
     "-std=c++17",
     "-fPIC",
     "-shared",
-    # packed FP32 instructions stay off (GPU-box traps)
+    # no packed FP32 (v_pk_*_f32) code generation: scalar FP32 only
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
     "-Wno-unused-value",

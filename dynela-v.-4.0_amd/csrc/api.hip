@@ -103,6 +103,7 @@ int64_t footprint(int type, int64_t N, int64_t NE, int prec, int track) {
   b += N * dim * 8 * 8;            // x, disp, fu/fv/fa x2
   b += N * 8;                      // mass
   b += (N + 1) * 4 * 2 + NE * nn * 4 * 2;  // CSRs, connectivity
+  b += N * 4;                      // BC node list (upper bound)
   b += NE * nn * dim * 8;          // elemF
   if (prec == DNLG_PREC_MIXED) {
     b += NE * nn * dim * 4;        // elemF32
@@ -288,6 +289,11 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
       }
     }
   }
+  std::vector<int32_t> bcn;  // nodes with constant BCs
+  for (int64_t n = 0; n < N; n++)
+    if (bc_ptr[n + 1] > bc_ptr[n]) bcn.push_back((int32_t)n);
+  job->n_bcn = (int32_t)bcn.size();
+  if (bcn.empty()) bcn.push_back(0);
   if (bc_idx.empty()) bc_idx.push_back(0);
   if (bnd.empty()) bnd.push_back(Boundary{});
 
@@ -332,6 +338,8 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
   H.c.courant_due = 0;
   H.c.density_due = 0;
   H.c.preloop = 1;
+  H.c.pending = 0;
+  H.c.arrive = 0;
 
   int st;
 #define ALLOC(ptr, count, init)                           \
@@ -352,6 +360,9 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
   H.bc_idx = p32;
   ALLOC(pb, (int64_t)bnd.size(), bnd.data());
   H.bnd = pb;
+  ALLOC(p32, (int64_t)bcn.size(), bcn.data());
+  H.bcn = p32;
+  H.n_bcn = job->n_bcn;
   ALLOC(p32, N + 1, n2e_ptr.data());
   H.n2e_ptr = p32;
   ALLOC(p32, NE * nn, n2e_idx.data());

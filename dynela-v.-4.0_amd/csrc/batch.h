@@ -22,6 +22,7 @@ struct dnlg_job {
   std::vector<void *> allocs;
   bool started = false;
   int32_t n_cmin = 0;
+  int32_t n_bcn = 0;  // nodes with constant BCs
   int status = 0;
 };
 

@@ -3,7 +3,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "batch.h"
@@ -23,6 +26,7 @@ struct Batch {
   TaskList prep[2];     // [hex?]
   TaskList cour[2];     // [hex?]
   TaskList node[2][2];  // [dim==3?][mixed?]
+  TaskList step;        // cycle control + field0 BCs, over the BC node lists
   double *d_snap = nullptr;
   std::vector<double> h_snap;
   hipGraphExec_t graph = nullptr;       // graph_cycles whole cycles
@@ -115,6 +119,16 @@ int build_batch(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, Batch &B) {
     if ((st = upload_tasks(B, make_tasks(idx, cnt, 256), B.prep[hx]))) return st;
     if ((st = upload_tasks(B, make_tasks(idx, cnt, 256), B.cour[hx]))) return st;
   }
+  {
+    // at least one block per job (k_cycle_step counts max(1, ceil(n_bcn / 256)))
+    std::vector<Task> t;
+    for (int i = 0; i < n; i++) {
+      const int64_t nb = jobs[i]->n_bcn;
+      if (nb == 0) t.push_back(Task{i, 0, 0, 0});
+      for (int64_t f = 0, blk = 0; f < nb; f += 256, blk++) t.push_back(Task{i, (int32_t)blk, f, std::min(f + 256, nb)});
+    }
+    if ((st = upload_tasks(B, t, B.step))) return st;
+  }
   for (int d3 = 0; d3 < 2; d3++)
     for (int mx = 0; mx < 2; mx++) {
       std::vector<int> idx;
@@ -130,14 +144,13 @@ int build_batch(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, Batch &B) {
   return 0;
 }
 
-// Phases of one cycle.  pre = updateTimes + predictor, element = the element
-// kernels, post = corrector + Courant + endStep.
-void launch_pre(dnlg_ctx *ctx, Batch &B) {
-  hipStream_t s = ctx->stream;
-  launch_cycle_begin(B.d_jobs, (int)B.jobs.size(), s);
-  for (int d3 = 0; d3 < 2; d3++)
-    for (int mx = 0; mx < 2; mx++) launch_predict(d3 ? 3 : 2, B.d_jobs, B.node[d3][mx].d, B.node[d3][mx].n, s);
-}
+// Phases of one cycle.  pre = end of the previous cycle + updateTimes +
+// field0 BCs, element = the element kernels, post = predictor/corrector +
+// Courant.  The last cycle of a launch sequence stays open until
+// launch_close (end of step and time step only).
+void launch_pre(dnlg_ctx *ctx, Batch &B) { launch_cycle_step(B.d_jobs, B.step.d, B.step.n, 1, ctx->stream); }
+
+void launch_close(dnlg_ctx *ctx, Batch &B) { launch_cycle_step(B.d_jobs, B.step.d, B.step.n, 0, ctx->stream); }
 
 // ev: nullptr, or two events per element launch (up to 4 launches); each
 // launch is bracketed by event records on the engine's stream.
@@ -167,7 +180,6 @@ void launch_post(dnlg_ctx *ctx, Batch &B) {
     for (int mx = 0; mx < 2; mx++)
       launch_correct(d3 ? 3 : 2, mx == 1, B.d_jobs, B.node[d3][mx].d, B.node[d3][mx].n, s);
   for (int hx = 0; hx < 2; hx++) launch_courant(hx ? 3 : 2, B.d_jobs, B.cour[hx].d, B.cour[hx].n, 0, s);
-  launch_cycle_end(B.d_jobs, (int)B.jobs.size(), s);
 }
 
 void launch_cycle(dnlg_ctx *ctx, Batch &B) {
@@ -300,7 +312,24 @@ int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vec
     ev_used.clear();
     return 0;
   };
+  // DNLG_PROGRESS=<seconds>: a progress line on stderr at that period
+  const char *prog_env = std::getenv("DNLG_PROGRESS");
+  const double prog_period = prog_env ? std::atof(prog_env) : 0.0;
+  auto prog_last = std::chrono::steady_clock::now();
+  int64_t launched = 0;
   while (true) {
+    if (prog_period > 0.0) {
+      const auto now = std::chrono::steady_clock::now();
+      if (std::chrono::duration<double>(now - prog_last).count() >= prog_period) {
+        prog_last = now;
+        int active = 0;
+        for (int i = 0; i < n; i++)
+          if ((int)B->h_snap[5 * i + 3] == 0 && B->h_snap[5 * i] < B->h_snap[5 * i + 2]) active++;
+        std::fprintf(stderr, "[dnlg] batch of %d jobs: %lld cycles launched, %d active\n", n, (long long)launched,
+                     active);
+        std::fflush(stderr);
+      }
+    }
     launch_snapshot(B->d_jobs, n, B->d_snap, s);
     if (hipMemcpyAsync(B->h_snap.data(), B->d_snap, 5 * n * sizeof(double), hipMemcpyDeviceToHost, s) !=
             hipSuccess ||
@@ -319,6 +348,7 @@ int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vec
     }
     if (need <= 0.0) break;
     const int chunk = (int)std::min(need, 4096.0);
+    launched += chunk;
     if (timed) {
       const int cycles = std::min(chunk, kTimedChunk);
       while (ctx->events.size() < (size_t)8 * cycles) {
@@ -352,6 +382,7 @@ int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vec
     } else {
       for (int k = 0; k < chunk; k++) launch_cycle(ctx, *B);
     }
+    launch_close(ctx, *B);
     const hipError_t le = hipGetLastError();
     if (le != hipSuccess) {
       ctx->err = std::string("kernel launch failed: ") + hipGetErrorString(le);

@@ -10,10 +10,24 @@
 //   ln U = 1/2 ln(I + X)   = 1/2 sum_k (-1)^(k+1) X^k / k,
 //   U^-1 = (I + X)^(-1/2)  = sum_k binom(-1/2, k) X^k,
 //   R = F U^-1 = (I + H) U^-1,
-// evaluated with powers of X up to X^6; for ||X||_F <= 0.05 the truncation
-// is below 1e-9 relative, i.e. under FP32 rounding.  Larger increments take
-// the FP64 eigen-decomposition path (polar_lnU_fp64).  This avoids the
+// evaluated with powers of X up to X^6 (X^4 when every lane of the wave has
+// ||X||_F <= 1e-2, X^3 below 1e-3); the truncation stays under 2e-9
+// relative, i.e. under FP32 rounding.  Larger increments (> 0.05) take the
+// FP64 eigen-decomposition path (polar_lnU_fp64).  This avoids the
 // catastrophic cancellation of forming F^T F - I and ln(lambda ~ 1) in FP32.
+//
+// Element geometry uses the tensor-product structure of the 8-node hex
+// (4-node quad): with s_d(b) = +-1 the local coordinate signs of node b, the
+// Walsh modes M_q = sum_b prod_{d in q} s_d(b) x_b give
+//   x(xi) = 2^-D sum_q M_q prod_{d in q} xi_d,
+// so the Jacobian at a Gauss point needs 3 FMAs per entry.  The modes are a
+// D-stage butterfly over the element's lanes (lane b holds node b).  Forces
+// use the transpose: with T_p = w_p J_p^-1 sigma_p per Gauss point,
+//   f_a = -sum_p sum_k dN_a/dxi_k(p) T_p,k
+// and sum_p dN_a/dxi_k(p) T_p,k is a short combination of the Walsh
+// transform of T over the Gauss-point lanes.  Lane b <-> node / Gauss point
+// b ^ ((b >> 1) & 1) of the reference numbering (ElHex8N3D.C:30-231,
+// ElQua4NAx.C:30-110), whose local signs are the bits of b.
 
 namespace dnlg {
 
@@ -40,7 +54,7 @@ __device__ __forceinline__ void polar_lnU_large(const float *H, float *dE, float
 }
 
 // H (row-major 3x3, FP32) -> dE = ln U (sym), R = F U^-1 (row-major).
-__device__ __forceinline__ void polar_lnU_mixed(const float H[9], float dE[6], float R[9]) {
+__device__ __forceinline__ void polar_lnU_mixed(const float H[9], float dE[6], float R[9], int order) {
   float X[6];
   X[0] = 2.f * H[0] + (H[0] * H[0] + H[3] * H[3] + H[6] * H[6]);
   X[1] = H[1] + H[3] + (H[0] * H[1] + H[3] * H[4] + H[6] * H[7]);
@@ -53,12 +67,17 @@ __device__ __forceinline__ void polar_lnU_mixed(const float H[9], float dE[6], f
     polar_lnU_large(H, dE, R);
     return;
   }
+  // powers to X^order (order 3, 4 or 6, uniform over the wave)
   float X2[6], X3[6], X4[6], X5[6], X6[6];
   sym_mul(X, X, X2);
   sym_mul(X2, X, X3);
-  sym_mul(X2, X2, X4);
-  sym_mul(X4, X, X5);
-  sym_mul(X3, X3, X6);
+#pragma unroll
+  for (int c = 0; c < 6; c++) X4[c] = X5[c] = X6[c] = 0.f;
+  if (order >= 4) sym_mul(X2, X2, X4);
+  if (order >= 6) {
+    sym_mul(X4, X, X5);
+    sym_mul(X3, X3, X6);
+  }
   float Ui[6];
 #pragma unroll
   for (int c = 0; c < 6; c++) {
@@ -83,130 +102,170 @@ __device__ __forceinline__ void polar_lnU_mixed(const float H[9], float dE[6], f
   }
 }
 
-template <int NN>
-__device__ __forceinline__ float tabDf(int p, int a, int j) {
-  if constexpr (NN == 8) return c_hexDf[p][a][j];
-  else return c_quaDf[p][a][j];
+
+// Walsh-Hadamard transform over the NB-bit lane group of an element: lane q
+// ends with sum_b prod_{d in q} s_d(b) v_b.
+template <int NB>
+__device__ __forceinline__ float wht(float v, int b) {
+#pragma unroll
+  for (int d = 0; d < NB; d++) {
+    const float o = __shfl_xor(v, 1 << d);
+    v = ((b >> d) & 1) ? (v - o) : (v + o);
+  }
+  return v;
 }
-template <int NN>
-__device__ __forceinline__ float tabNf(int p, int a) {
-  if constexpr (NN == 8) return c_hexNf[p][a];
-  else return c_quaNf[p][a];
+
+template <int NB>
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+  for (int d = 0; d < NB; d++) v += __shfl_xor(v, 1 << d);
+  return v;
 }
+
+__device__ __forceinline__ float sgn(int b, int d) { return ((b >> d) & 1) ? 1.f : -1.f; }
 
 template <int NN, int DIM, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const Task *tasks) {
-  constexpr int NIP = NN;
-  constexpr int EPB = BLOCK / NIP;
-  __shared__ float s_x[EPB][NN][DIM];  // coordinates relative to node 0
-  __shared__ float s_u[EPB][NN][DIM];
-  __shared__ float s_dN[EPB][NIP][NN][DIM];
-  __shared__ float s_sw[EPB][NIP][8];
-  __shared__ float s_tr[EPB][NIP];
+  constexpr int NB = DIM;               // lanes per element = 2^DIM = NN
+  constexpr float inv2D = 1.f / NN;     // 2^-D
+  const float g = 0.577350269189625764f;  // Gauss abscissa 1/sqrt(3)
 
   const Task tk = tasks[blockIdx.x];
   JobDev *J = jobs[tk.job];
   if (!J->c.running || J->c.status != 0) return;
-  const int le = threadIdx.x / NIP;
-  const int p = threadIdx.x % NIP;
+  const int lane = threadIdx.x & 63;
+  const int b = lane & (NN - 1);            // lane within the element group
+  const int base = lane & ~(NN - 1);        // first lane of the group
+  const int le = threadIdx.x / NN;
   const int64_t e = tk.first + le;
   const bool valid = e < tk.end;
+  const int ref = b ^ ((b >> 1) & 1);       // node / IP index in the reference order
   const int64_t N = J->N;
   const int f0 = J->c.f0;
-  double x0r = 0.0;  // radius of node 0 (axisymmetric)
+
+  // ---- node b of the element: coordinates relative to node 0, increment
+  float xl[DIM], ul[DIM];
+  double x0r = 0.0;
   if (valid) {
-    const int nd = J->conn[e * NN + p];
+    const int nd = J->conn[e * NN + ref];
     const int n0 = J->conn[e * NN];
     const double *u0 = J->fu[f0];
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
-      s_x[le][p][d] = (float)(J->x[d * N + nd] - J->x[d * N + n0]);
-      s_u[le][p][d] = (float)u0[d * N + nd];
+      xl[d] = (float)(J->x[d * N + nd] - J->x[d * N + n0]);
+      ul[d] = (float)u0[d * N + nd];
     }
     if constexpr (DIM == 2) x0r = J->x[n0];
+  } else {
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      xl[d] = sgn(b, d);  // dummy unit element keeps the idle lanes finite
+      ul[d] = 0.f;
+    }
   }
-  __syncthreads();
+  // ---- Walsh modes of coordinates and increments
+  float Mx[DIM], Mu[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    Mx[d] = wht<NB>(xl[d], b);
+    Mu[d] = wht<NB>(ul[d], b);
+  }
+  // modes of the group, broadcast: mx[q][d]
+  float mx[NN][DIM], mu[NN][DIM];
+#pragma unroll
+  for (int q = 0; q < NN; q++)
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      mx[q][d] = __shfl(Mx[d], base + q);
+      mu[q][d] = __shfl(Mu[d], base + q);
+    }
+  // Gauss point of this lane
+  float xi[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; d++) xi[d] = g * sgn(b, d);
+
+  // ---- Jacobian J_ij = dx_i/dxi_j and G_ij = du_i/dxi_j
+  float Jm[DIM][DIM], Gm[DIM][DIM];
+#pragma unroll
+  for (int i = 0; i < DIM; i++) {
+    if constexpr (DIM == 3) {
+      Jm[i][0] = inv2D * (mx[1][i] + mx[3][i] * xi[1] + mx[5][i] * xi[2] + mx[7][i] * (xi[1] * xi[2]));
+      Jm[i][1] = inv2D * (mx[2][i] + mx[3][i] * xi[0] + mx[6][i] * xi[2] + mx[7][i] * (xi[0] * xi[2]));
+      Jm[i][2] = inv2D * (mx[4][i] + mx[5][i] * xi[0] + mx[6][i] * xi[1] + mx[7][i] * (xi[0] * xi[1]));
+      Gm[i][0] = inv2D * (mu[1][i] + mu[3][i] * xi[1] + mu[5][i] * xi[2] + mu[7][i] * (xi[1] * xi[2]));
+      Gm[i][1] = inv2D * (mu[2][i] + mu[3][i] * xi[0] + mu[6][i] * xi[2] + mu[7][i] * (xi[0] * xi[2]));
+      Gm[i][2] = inv2D * (mu[4][i] + mu[5][i] * xi[0] + mu[6][i] * xi[1] + mu[7][i] * (xi[0] * xi[1]));
+    } else {
+      Jm[i][0] = inv2D * (mx[1][i] + mx[3][i] * xi[1]);
+      Jm[i][1] = inv2D * (mx[2][i] + mx[3][i] * xi[0]);
+      Gm[i][0] = inv2D * (mu[1][i] + mu[3][i] * xi[1]);
+      Gm[i][1] = inv2D * (mu[2][i] + mu[3][i] * xi[0]);
+    }
+  }
+  float detJ, inv[DIM][DIM];
+  if constexpr (DIM == 3) {
+    const float *dd = &Jm[0][0];
+    detJ = dd[1] * dd[5] * dd[6] - dd[2] * dd[4] * dd[6] + dd[2] * dd[3] * dd[7] - dd[0] * dd[5] * dd[7] -
+           dd[1] * dd[3] * dd[8] + dd[0] * dd[4] * dd[8];
+    const float id = 1.f / detJ;
+    inv[0][0] = (dd[4] * dd[8] - dd[5] * dd[7]) * id;
+    inv[0][1] = (dd[2] * dd[7] - dd[1] * dd[8]) * id;
+    inv[0][2] = (dd[1] * dd[5] - dd[2] * dd[4]) * id;
+    inv[1][0] = (dd[5] * dd[6] - dd[3] * dd[8]) * id;
+    inv[1][1] = (dd[0] * dd[8] - dd[2] * dd[6]) * id;
+    inv[1][2] = (dd[2] * dd[3] - dd[0] * dd[5]) * id;
+    inv[2][0] = (dd[3] * dd[7] - dd[4] * dd[6]) * id;
+    inv[2][1] = (dd[1] * dd[6] - dd[0] * dd[7]) * id;
+    inv[2][2] = (dd[0] * dd[4] - dd[1] * dd[3]) * id;
+  } else {
+    detJ = Jm[0][0] * Jm[1][1] - Jm[1][0] * Jm[0][1];
+    const float id = 1.f / detJ;
+    inv[0][0] = Jm[1][1] * id;
+    inv[0][1] = -Jm[0][1] * id;
+    inv[1][0] = -Jm[1][0] * id;
+    inv[1][1] = Jm[0][0] * id;
+  }
+  if (valid && detJ < 0.f) set_status(J, DNLG_E_NEG_JACOBIAN);
+  // H = du/dx = G J^-1 (Element3D.C:172-195, ElementAxi.C:78-100)
+  float H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int i = 0; i < DIM; i++)
+#pragma unroll
+    for (int j = 0; j < DIM; j++) {
+      float h = 0.f;
+#pragma unroll
+      for (int k = 0; k < DIM; k++) h += Gm[i][k] * inv[k][j];
+      H[3 * i + j] = h;
+    }
+  float radius = 1.f;
+  if constexpr (DIM == 2) {
+    const float rl = inv2D * (mx[0][0] + mx[1][0] * xi[0] + mx[2][0] * xi[1] + mx[3][0] * (xi[0] * xi[1]));
+    const float ur = inv2D * (mu[0][0] + mu[1][0] * xi[0] + mu[2][0] * xi[1] + mu[3][0] * (xi[0] * xi[1]));
+    radius = (float)(x0r + (double)rl);
+    H[8] = ur / radius;
+  }
+  // series order, uniform over the wave, from the linear part of X
+  float Xn2;
+  {
+    const float a0 = 2.f * H[0], a3 = 2.f * H[4], a5 = 2.f * H[8];
+    const float a1 = H[1] + H[3], a2 = H[2] + H[6], a4 = H[5] + H[7];
+    Xn2 = a0 * a0 + a3 * a3 + a5 * a5 + 2.f * (a1 * a1 + a2 * a2 + a4 * a4);
+  }
+  const int order = __any(Xn2 > 1e-4f) ? 6 : (__any(Xn2 > 1e-6f) ? 4 : 3);
+  float dE[6], R[9];
+  polar_lnU_mixed(H, dE, R, order);
+
+  // ---- pressure: element mean of tr(dE) (Element.C:1125-1143)
+  const float inc = group_sum<NB>(dE[0] + dE[3] + dE[5]) * inv2D;
 
   const int64_t G = J->G;
-  const int64_t g = e * NIP + p;
-  float dN[NN][DIM];
-  float detJ = 1.f, radius = 1.f;
-  float dE[6] = {0, 0, 0, 0, 0, 0}, R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  const int64_t gi = e * NN + ref;
+  float sig[6] = {0, 0, 0, 0, 0, 0};
+  float T9[DIM][DIM];
+  float hoop = 0.f;
   if (valid) {
-    float Jm[DIM][DIM];
 #pragma unroll
-    for (int i = 0; i < DIM; i++)
-#pragma unroll
-      for (int j = 0; j < DIM; j++) Jm[i][j] = 0.f;
-#pragma unroll
-    for (int a = 0; a < NN; a++)
-#pragma unroll
-      for (int i = 0; i < DIM; i++)
-#pragma unroll
-        for (int j = 0; j < DIM; j++) Jm[i][j] += tabDf<NN>(p, a, j) * s_x[le][a][i];
-    float inv[DIM][DIM];
-    if constexpr (DIM == 3) {
-      const float *dd = &Jm[0][0];
-      detJ = dd[1] * dd[5] * dd[6] - dd[2] * dd[4] * dd[6] + dd[2] * dd[3] * dd[7] - dd[0] * dd[5] * dd[7] -
-             dd[1] * dd[3] * dd[8] + dd[0] * dd[4] * dd[8];
-      const float id = 1.f / detJ;
-      inv[0][0] = (dd[4] * dd[8] - dd[5] * dd[7]) * id;
-      inv[0][1] = (dd[2] * dd[7] - dd[1] * dd[8]) * id;
-      inv[0][2] = (dd[1] * dd[5] - dd[2] * dd[4]) * id;
-      inv[1][0] = (dd[5] * dd[6] - dd[3] * dd[8]) * id;
-      inv[1][1] = (dd[0] * dd[8] - dd[2] * dd[6]) * id;
-      inv[1][2] = (dd[2] * dd[3] - dd[0] * dd[5]) * id;
-      inv[2][0] = (dd[3] * dd[7] - dd[4] * dd[6]) * id;
-      inv[2][1] = (dd[1] * dd[6] - dd[0] * dd[7]) * id;
-      inv[2][2] = (dd[0] * dd[4] - dd[1] * dd[3]) * id;
-    } else {
-      detJ = Jm[0][0] * Jm[1][1] - Jm[1][0] * Jm[0][1];
-      const float id = 1.f / detJ;
-      inv[0][0] = Jm[1][1] * id;
-      inv[0][1] = -Jm[0][1] * id;
-      inv[1][0] = -Jm[1][0] * id;
-      inv[1][1] = Jm[0][0] * id;
-    }
-    if (detJ < 0.f) set_status(J, DNLG_E_NEG_JACOBIAN);
-#pragma unroll
-    for (int a = 0; a < NN; a++)
-#pragma unroll
-      for (int j = 0; j < DIM; j++) {
-        float s = 0.f;
-#pragma unroll
-        for (int k = 0; k < DIM; k++) s += tabDf<NN>(p, a, k) * inv[k][j];
-        dN[a][j] = s;
-      }
-    float H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-    for (int a = 0; a < NN; a++)
-#pragma unroll
-      for (int i = 0; i < DIM; i++)
-#pragma unroll
-        for (int j = 0; j < DIM; j++) H[3 * i + j] += dN[a][j] * s_u[le][a][i];
-    if constexpr (DIM == 2) {
-      float rl = 0.f, vr = 0.f;
-#pragma unroll
-      for (int a = 0; a < NN; a++) {
-        rl += tabNf<NN>(p, a) * s_x[le][a][0];
-        vr += tabNf<NN>(p, a) * s_u[le][a][0];
-      }
-      radius = (float)(x0r + (double)rl);
-      H[8] = vr / radius;
-    }
-    polar_lnU_mixed(H, dE, R);
-    s_tr[le][p] = dE[0] + dE[3] + dE[5];
-  }
-  __syncthreads();
-
-  float sig[6], wdj = 0.f;
-  if (valid) {
-    float inc = 0.f;
-#pragma unroll
-    for (int q = 0; q < NIP; q++) inc += s_tr[le][q];
-    inc *= (1.f / NIP);
-#pragma unroll
-    for (int c = 0; c < 6; c++) sig[c] = J->sig32[c * G + g];
+    for (int c = 0; c < 6; c++) sig[c] = J->sig32[c * G + gi];
     const float pressure = (sig[0] + sig[3] + sig[5]) * (1.f / 3.f) + (float)J->K * inc;
 
     const double dt = J->c.dt;
@@ -222,47 +281,52 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
     s[0] -= pm;
     s[3] -= pm;
     s[5] -= pm;
-    const float Snorm0 = sqrtf(s[0] * s[0] + s[3] * s[3] + s[5] * s[5] + 2.f * (s[1] * s[1] + s[2] * s[2] + s[4] * s[4]));
+    const float Snorm0 =
+        sqrtf(s[0] * s[0] + s[3] * s[3] + s[5] * s[5] + 2.f * (s[1] * s[1] + s[2] * s[2] + s[4] * s[4]));
     const float em = (dE[0] + dE[3] + dE[5]) * (1.f / 3.f);
-    float de[6];
-#pragma unroll
-    for (int c = 0; c < 6; c++) de[c] = dE[c];
-    de[0] -= em;
-    de[3] -= em;
-    de[5] -= em;
-#pragma unroll
-    for (int c = 0; c < 6; c++) s[c] += TwoGf * de[c];
-    const float Snorm = sqrtf(s[0] * s[0] + s[3] * s[3] + s[5] * s[5] + 2.f * (s[1] * s[1] + s[2] * s[2] + s[4] * s[4]));
+    s[0] += TwoGf * (dE[0] - em);
+    s[1] += TwoGf * dE[1];
+    s[2] += TwoGf * dE[2];
+    s[3] += TwoGf * (dE[3] - em);
+    s[4] += TwoGf * dE[4];
+    s[5] += TwoGf * (dE[5] - em);
+    const float Snorm =
+        sqrtf(s[0] * s[0] + s[3] * s[3] + s[5] * s[5] + 2.f * (s[1] * s[1] + s[2] * s[2] + s[4] * s[4]));
     const double Strial = kSqrt32 * (double)Snorm;
-    const double Tcur = J->T[g];
-    double yield = (double)J->sy32[g];
+    const double Tcur = J->T[gi];
+    double yield = (double)J->sy32[gi];
     double gamma = 0.0;
     const int law = J->law;
     const double *lp = J->lp;
     if (yield == 0.0) {
       yield = law_yield(law, lp, kGammaInitial, kGammaInitial / dt, Tcur);
-      J->sy32[g] = (float)yield;
+      J->sy32[gi] = (float)yield;
     }
     if (Strial > yield) {
       // radial return in FP64 (Element.C:485-601)
-      const double ep0 = J->ep[g];
+      const double ep0 = J->ep[gi];
       double gmin = 0.0, gmax = (Strial - yield) / TwoG32;
-      gamma = (double)J->gam32[g];
+      gamma = (double)J->gam32[gi];
       if (ep0 == 0.0) gamma = kSqrt32 * kGammaInitial;
       double depsp = kSqrt23 * gamma / dt;
       double epsp = ep0 + kSqrt23 * gamma;
       double T = Tcur + 0.5 * gamma * heatFrac * (kSqrt23 * yield + (double)Snorm0);
+      const bool jc = law == DNLG_LAW_JOHNSON_COOK;
+      const ReturnConsts rk = return_consts(lp, dt);
       int iterate = 0;
       while (true) {
         double hard;
-        law_yield_and_slope(law, lp, heatFrac, epsp, depsp, T, dt, yield, hard);
+        if (jc)
+          jc_yield_and_slope_fast(lp, rk, heatFrac, epsp, gamma, T, yield, hard);
+        else
+          law_yield_and_slope(law, lp, heatFrac, epsp, depsp, T, dt, yield, hard);
         const double fun = Strial - gamma * TwoG32 - yield;
         if (fun < 0.0)
           gmax = gamma;
         else
           gmin = gamma;
         const double dfun = TwoG32 + kSqrt23 * hard;
-        double dgamma = fun / dfun;
+        double dgamma = fun * __builtin_amdgcn_rcp(dfun);
         gamma += dgamma;
         if ((gmax - gamma) * (gamma - gmin) < 0.0) {
           dgamma = 0.5 * (gmax - gmin);
@@ -280,11 +344,11 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
       const float scale = (float)(1.0 - J->twoG * gamma / (double)Snorm);
 #pragma unroll
       for (int c = 0; c < 6; c++) s[c] *= scale;
-      J->ep[g] = ep0 + kSqrt23 * gamma;
-      J->dep32[g] = (float)(kSqrt23 * gamma / dt);
-      J->gam32[g] = (float)gamma;
-      J->gamc32[g] += (float)gamma;
-      J->sy32[g] = (float)yield;
+      J->ep[gi] = ep0 + kSqrt23 * gamma;
+      J->dep32[gi] = (float)(kSqrt23 * gamma / dt);
+      J->gam32[gi] = (float)gamma;
+      J->gamc32[gi] += (float)gamma;
+      J->sy32[gi] = (float)yield;
     }
 #pragma unroll
     for (int c = 0; c < 6; c++) sig[c] = s[c];
@@ -294,51 +358,87 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
     const float power = 0.5f * (dE[0] * (old[0] + sig[0]) + dE[3] * (old[3] + sig[3]) + dE[5] * (old[5] + sig[5]) +
                                 2.f * (dE[1] * (old[1] + sig[1]) + dE[2] * (old[2] + sig[2]) +
                                        dE[4] * (old[4] + sig[4])));
-    J->eint[g] += (double)power / J->rho0;
+    J->eint[gi] += (double)power / J->rho0;
     if (gamma != 0.0) {
-      const float sn = sqrtf(s[0] * s[0] + s[3] * s[3] + s[5] * s[5] + 2.f * (s[1] * s[1] + s[2] * s[2] + s[4] * s[4]));
+      const float sn =
+          sqrtf(s[0] * s[0] + s[3] * s[3] + s[5] * s[5] + 2.f * (s[1] * s[1] + s[2] * s[2] + s[4] * s[4]));
       const double w = 0.5 * gamma * ((double)sn + (double)Snorm0);
-      J->einel[g] += w / J->rho0;
-      J->T[g] = Tcur + heatFrac * w;
+      J->einel[gi] += w / J->rho0;
+      J->T[gi] = Tcur + heatFrac * w;
     }
+    // Element::computeFinalRotation (Element.C:1146-1161)
     rotate_sym(R, sig);
 #pragma unroll
-    for (int c = 0; c < 6; c++) J->sig32[c * G + g] = sig[c];
+    for (int c = 0; c < 6; c++) J->sig32[c * G + gi] = sig[c];
+    // Element::computeDensity (Element.C:1243-1256)
     if (J->c.density_due) {
-      double rr = J->rho0 * (double)J->detJ0f[g] / (double)detJ;
+      double rr = J->rho0 * (double)J->detJ0f[gi] / (double)detJ;
       if constexpr (DIM == 2) rr /= (double)radius;
-      J->rho[g] = rr;
+      J->rho[gi] = rr;
     }
-    wdj = detJ;
-    if constexpr (DIM == 2) wdj *= (float)k2PI * radius;
-#pragma unroll
-    for (int a = 0; a < NN; a++)
-#pragma unroll
-      for (int j = 0; j < DIM; j++) s_dN[le][p][a][j] = dN[a][j];
-#pragma unroll
-    for (int c = 0; c < 6; c++) s_sw[le][p][c] = sig[c];
-    s_sw[le][p][6] = wdj;
-    s_sw[le][p][7] = radius;
   }
-  __syncthreads();
-
-  if (valid) {
-    const int a = p;
-    float f[DIM];
+  // ---- internal forces (Element.C:362-404): T = w J^-1 sigma per Gauss point
+  float w = detJ;
+  if constexpr (DIM == 2) w *= (float)k2PI * radius;
 #pragma unroll
-    for (int i = 0; i < DIM; i++) f[i] = 0.f;
+  for (int k = 0; k < DIM; k++)
 #pragma unroll
-    for (int q = 0; q < NIP; q++) {
-      const float w = s_sw[le][q][6];
+    for (int i = 0; i < DIM; i++) {
+      float t = 0.f;
 #pragma unroll
-      for (int i = 0; i < DIM; i++)
-#pragma unroll
-        for (int j = 0; j < DIM; j++) f[i] -= s_dN[le][q][a][j] * s_sw[le][q][sym_idx(j, i)] * w;
-      if constexpr (DIM == 2) f[0] -= tabNf<NN>(q, a) * s_sw[le][q][5] / s_sw[le][q][7] * w;
+      for (int j = 0; j < DIM; j++) {
+        const int si = (i == j) ? (i == 0 ? 0 : (i == 1 ? 3 : 5)) : (i + j == 1 ? 1 : (i + j == 2 ? 2 : 4));
+        t += inv[k][j] * sig[si];
+      }
+      T9[k][i] = valid ? w * t : 0.f;
     }
-    float *out = J->elemF32 + (e * NN + a) * DIM;
+  if constexpr (DIM == 2) hoop = valid ? w * sig[5] / radius : 0.f;
+  // Walsh transform of T over the Gauss points, then node lane b combines
+  float WT[DIM][DIM];
 #pragma unroll
-    for (int i = 0; i < DIM; i++) out[i] = f[i];
+  for (int k = 0; k < DIM; k++)
+#pragma unroll
+    for (int i = 0; i < DIM; i++) WT[k][i] = wht<NB>(T9[k][i], b);
+  float f[DIM];
+#pragma unroll
+  for (int i = 0; i < DIM; i++) f[i] = 0.f;
+#pragma unroll
+  for (int k = 0; k < DIM; k++) {
+    const float sk = sgn(b, k);
+    if constexpr (DIM == 3) {
+      const int l = (k + 1) % 3, m = (k + 2) % 3;
+      const float cl = g * sgn(b, l), cm = g * sgn(b, m), clm = cl * cm;
+#pragma unroll
+      for (int i = 0; i < DIM; i++) {
+        const float w0 = __shfl(WT[k][i], base + 0);
+        const float wl = __shfl(WT[k][i], base + (1 << l));
+        const float wm = __shfl(WT[k][i], base + (1 << m));
+        const float wlm = __shfl(WT[k][i], base + (1 << l) + (1 << m));
+        f[i] -= sk * (w0 + cl * wl + cm * wm + clm * wlm);
+      }
+    } else {
+      const int l = 1 - k;
+      const float cl = g * sgn(b, l);
+#pragma unroll
+      for (int i = 0; i < DIM; i++) {
+        const float w0 = __shfl(WT[k][i], base + 0);
+        const float wl = __shfl(WT[k][i], base + (1 << l));
+        f[i] -= sk * (w0 + cl * wl);
+      }
+    }
+  }
+  if constexpr (DIM == 2) {
+    // hoop term: f_r -= sum_p N_a(p) w_p sigma_tt / r_p
+    const float Wh = wht<NB>(hoop, b);
+    const float h0 = __shfl(Wh, base + 0), h1 = __shfl(Wh, base + 1), h2 = __shfl(Wh, base + 2),
+                h3 = __shfl(Wh, base + 3);
+    const float c0 = g * sgn(b, 0), c1 = g * sgn(b, 1);
+    f[0] -= h0 + c0 * h1 + c1 * h2 + c0 * c1 * h3;
+  }
+  if (valid) {
+    float *out = J->elemF32 + (e * NN + ref) * DIM;
+#pragma unroll
+    for (int i = 0; i < DIM; i++) out[i] = f[i] * inv2D;
   }
 }
 

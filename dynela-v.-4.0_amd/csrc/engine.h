@@ -49,6 +49,8 @@ struct Control {
   int32_t courant_due;  // Courant recompute after this cycle
   int32_t density_due;  // element kernel writes IP densities this cycle
   int32_t preloop;      // element pass of Explicit.C:158-178
+  int32_t pending;      // the last cycle ran and awaits its end-of-step
+  uint32_t arrive;      // blocks of the step kernel done with this record
 };
 
 struct JobDev {
@@ -68,6 +70,8 @@ struct JobDev {
   const int32_t *bc_ptr;  // [N+1] constant-BC list per node
   const int32_t *bc_idx;
   const Boundary *bnd;
+  const int32_t *bcn;     // nodes that carry constant BCs, ascending
+  int32_t n_bcn;
   const int32_t *n2e_ptr;  // [N+1] node -> (element*nn + local node), element order
   const int32_t *n2e_idx;
   // elements

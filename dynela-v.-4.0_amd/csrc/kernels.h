@@ -15,11 +15,12 @@ void launch_elem_fp64(int type, JobDev *const *jobs, const Task *tasks, int nblk
                       const hipEvent_t *ev = nullptr);
 void launch_elem_mixed(int type, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s,
                        const hipEvent_t *ev = nullptr);
-void launch_predict(int dim, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s);
 void launch_correct(int dim, bool f32, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s);
 void launch_courant(int dim, JobDev *const *jobs, const Task *tasks, int nblk, int force, hipStream_t s);
-void launch_cycle_begin(JobDev *const *jobs, int n, hipStream_t s);
-void launch_cycle_end(JobDev *const *jobs, int n, hipStream_t s);
+// close the last cycle (end of step, Courant time step) and, with open != 0,
+// open the next one; tasks: max(1, ceil(n_bcn / 256)) blocks per job over
+// the job's BC node list
+void launch_cycle_step(JobDev *const *jobs, const Task *tasks, int nblk, int open, hipStream_t s);
 void launch_force_dt(JobDev *const *jobs, int n, hipStream_t s);
 void launch_prep(int type, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s);
 void launch_mass_gather(int dim, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s);

@@ -1,17 +1,18 @@
 // kernels.hip -- CDNA4 kernels of one explicit cycle (FP64 element path,
 // nodal kernels, Courant minimum, control) and of the pre-loop set-up.
 //
-// Cycle = Explicit::solve loop body (Explicit.C:184-252):
-//   k_cycle_begin   updateTimes (Explicit.C:414-427)
-//   k_predict       computePredictions + BCs (Explicit.C:438-482)
+// Cycle = Explicit::solve loop body (Explicit.C:184-252), four launches:
+//   k_cycle_step    endStep + computeTimeStep of the previous cycle
+//                   (Explicit.C:539-552, Solver.C:196-236), updateTimes
+//                   (Explicit.C:414-427), field0 BCs of computePredictions
 //   k_elem_*        computeStrains, computePressure, computeStress,
 //                   computeFinalRotation, computeInternalForces,
 //                   computeDensity (Model.C:800-899, Explicit.C:555-562)
-//   k_correct       explicitSolve + BCs (Explicit.C:489-536), gather-based
-//                   force assembly in element order (Model.C:864-899)
+//   k_correct       computePredictions + explicitSolve + BCs
+//                   (Explicit.C:438-536), gather-based force assembly in
+//                   element order (Model.C:864-899)
 //   k_courant       computeCourantTimeStep (Model.C:636-670), per-block min
-//   k_cycle_end     endStep + computeTimeStep (Explicit.C:539-552,
-//                   Solver.C:196-236)
+//                   (every dt_freq-th cycle; an early exit otherwise)
 // The element Jacobian of Explicit.C:245-246 is recomputed inside the next
 // cycle's element kernel from the same coordinates, so nothing per-IP
 // geometric is stored between cycles.
@@ -26,7 +27,7 @@ namespace dnlg {
 
 // Element shape tables at the Gauss points: the closed forms tabulated in
 // ElQua4NAx.C:63-110 and ElHex8N3D.C:93-231, generated on the host
-// (tables.cpp) and uploaded once.
+// (api.hip, make_tables) and uploaded once.
 __constant__ double c_hexN[8][8];
 __constant__ double c_hexD[8][8][3];
 __constant__ double c_quaN[4][4];
@@ -378,51 +379,13 @@ __device__ __forceinline__ void apply_bcs(const JobDev *J, int64_t n, double tim
   }
 }
 
-// Explicit::computePredictions (Explicit.C:438-482)
-template <int DIM>
-__global__ __launch_bounds__(256) void k_predict(JobDev *const *jobs, const Task *tasks) {
-  const Task tk = tasks[blockIdx.x];
-  const JobDev *J = jobs[tk.job];
-  if (!J->c.running || J->c.status != 0) return;
-  const int64_t n = tk.first + threadIdx.x;
-  if (n >= tk.end) return;
-  const int64_t N = J->N;
-  const int a = J->c.f0, b = 1 - a;
-  const double h = J->c.dt;
-  const double beta = J->beta, gam = J->gammaCH;
-  double ua[DIM], va[DIM], aa[DIM], ub[DIM], vb[DIM], ab[DIM];
-#pragma unroll
-  for (int d = 0; d < DIM; d++) {
-    va[d] = J->fv[a][d * N + n];
-    aa[d] = J->fa[a][d * N + n];
-    ub[d] = h * (va[d] + (0.5 - beta) * h * aa[d]);
-    vb[d] = va[d] + (1.0 - gam) * h * aa[d];
-    ab[d] = 0.0;
-  }
-  const bool has_bc = J->bc_ptr[n + 1] > J->bc_ptr[n];
-  if (has_bc) {
-#pragma unroll
-    for (int d = 0; d < DIM; d++) ua[d] = J->fu[a][d * N + n];
-    apply_bcs<DIM>(J, n, J->c.t, h, ua, va, aa, ub, vb, ab);
-#pragma unroll
-    for (int d = 0; d < DIM; d++) {
-      J->fu[a][d * N + n] = ua[d];
-      J->fv[a][d * N + n] = va[d];
-      J->fa[a][d * N + n] = aa[d];
-    }
-  }
-#pragma unroll
-  for (int d = 0; d < DIM; d++) {
-    J->fu[b][d * N + n] = ub[d];
-    J->fv[b][d * N + n] = vb[d];
-    J->fa[b][d * N + n] = ab[d];
-  }
-}
-
-// Explicit::explicitSolve (Explicit.C:489-536): gather the element forces of
-// the node in element order (same summation order as the reference's
+// Explicit::computePredictions + Explicit::explicitSolve (Explicit.C:438-536)
+// in one pass per node: the predicted field1 values are formed from field0
+// in registers (they are read nowhere else), the element forces of the node
+// are gathered in element order (same summation order as the reference's
 // scatter-add of Model.C:864-899), a = f/M, Chung-Hulbert correction, BCs,
-// displacement and coordinates.
+// displacement and coordinates.  The field0 BC values that the element pass
+// reads were applied by k_cycle_step.
 template <int DIM, typename FT>
 __global__ __launch_bounds__(256) void k_correct(JobDev *const *jobs, const Task *tasks) {
   const Task tk = tasks[blockIdx.x];
@@ -449,9 +412,12 @@ __global__ __launch_bounds__(256) void k_correct(JobDev *const *jobs, const Task
   double ua[DIM], va[DIM], aa[DIM], ub[DIM], vb[DIM], ab[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; d++) {
+    va[d] = J->fv[a][d * N + n];
     aa[d] = J->fa[a][d * N + n];
-    ub[d] = J->fu[b][d * N + n];
-    vb[d] = J->fv[b][d * N + n];
+    // predictor (Explicit.C:458-466)
+    ub[d] = h * (va[d] + (0.5 - beta) * h * aa[d]);
+    vb[d] = va[d] + (1.0 - gam) * h * aa[d];
+    // corrector (Explicit.C:508-521)
     ab[d] = f[d] / m;
     ab[d] -= alphaM * aa[d];
     ab[d] /= (1.0 - alphaM);
@@ -461,10 +427,7 @@ __global__ __launch_bounds__(256) void k_correct(JobDev *const *jobs, const Task
   const bool has_bc = J->bc_ptr[n + 1] > J->bc_ptr[n];
   if (has_bc) {
 #pragma unroll
-    for (int d = 0; d < DIM; d++) {
-      ua[d] = J->fu[a][d * N + n];
-      va[d] = J->fv[a][d * N + n];
-    }
+    for (int d = 0; d < DIM; d++) ua[d] = J->fu[a][d * N + n];
     apply_bcs<DIM>(J, n, J->c.t, h, ua, va, aa, ub, vb, ab);
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
@@ -591,39 +554,108 @@ __global__ __launch_bounds__(256) void k_courant(JobDev *const *jobs, const Task
 }
 
 // ---------------------------------------------------------------------------
-// Control kernels (one thread per job).
+// Cycle control.
 // ---------------------------------------------------------------------------
-// Explicit::updateTimes (Explicit.C:414-427) and the per-cycle flags.
-__global__ void k_cycle_begin(JobDev *const *jobs, int n_jobs) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n_jobs) return;
-  JobDev *J = jobs[j];
-  Control &c = J->c;
-  c.preloop = 0;
-  c.running = (c.status == 0 && c.t < c.up_to) ? 1 : 0;
-  if (!c.running) return;
-  if (c.t + c.dt > c.up_to) c.dt = c.up_to - c.t;
-  c.inc++;
-  c.courant_due = (c.inc % J->dt_freq == 0) ? 1 : 0;
-  // densities are consumed by the Courant step and read back at the end
-  c.density_due = (c.courant_due || !(c.t + c.dt < c.up_to)) ? 1 : 0;
+// BC predictor on field0 of one node (the field0 half of
+// Explicit::computePredictions, Explicit.C:468-480): constant BCs at the
+// cycle's time and step, read by the element pass.
+template <int DIM>
+__device__ __forceinline__ void bc_field0(JobDev *J, int64_t n, int f0, double t, double h) {
+  const int64_t N = J->N;
+  double ua[DIM], va[DIM], aa[DIM], ub[DIM], vb[DIM], ab[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    ua[d] = J->fu[f0][d * N + n];
+    va[d] = J->fv[f0][d * N + n];
+    aa[d] = J->fa[f0][d * N + n];
+    ub[d] = vb[d] = ab[d] = 0.0;
+  }
+  apply_bcs<DIM>(J, n, t, h, ua, va, aa, ub, vb, ab);
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    J->fu[f0][d * N + n] = ua[d];
+    J->fv[f0][d * N + n] = va[d];
+    J->fa[f0][d * N + n] = aa[d];
+  }
 }
 
-// Explicit::endStep (Explicit.C:539-552) + Solver::computeTimeStep
-// (Solver.C:196-236, Courant branch: omega_max = 2 / dt_crit).
-__global__ void k_cycle_end(JobDev *const *jobs, int n_jobs) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n_jobs) return;
-  JobDev *J = jobs[j];
-  Control &c = J->c;
-  if (!c.running || c.status != 0) return;
-  c.t += c.dt;
-  c.f0 = 1 - c.f0;
-  if (c.courant_due && c.t < c.up_to) {
-    double m = J->cmin[0];
-    for (int k = 1; k < J->n_cmin; k++) m = fmin(m, J->cmin[k]);
-    const double fmax = 2.0 / m;
-    c.dt = J->safety * J->omegaS / fmax;
+// One launch between two cycles, per job:
+//   close  Explicit::endStep (Explicit.C:539-552) and Solver::computeTimeStep
+//          (Solver.C:196-236, Courant branch: omega_max = 2 / dt_crit, the
+//          minimum over the per-block partials of k_courant) for the cycle
+//          that ran last, if any;
+//   open   (open != 0) Explicit::updateTimes (Explicit.C:414-427), the
+//          per-cycle flags and the field0 BCs of the job's BC nodes.
+// Every block of a job derives the new control record from the old one; the
+// last block to arrive (device-scope counter) stores it, after every block of
+// the job has read the old one.  Blocks per job: max(1, ceil(n_bcn / 256)).
+__global__ __launch_bounds__(256) void k_cycle_step(JobDev *const *jobs, const Task *tasks, int open) {
+  __shared__ double s_min[4];
+  const Task tk = tasks[blockIdx.x];
+  JobDev *J = jobs[tk.job];
+  double t = J->c.t, dt = J->c.dt;
+  const double up_to = J->c.up_to;
+  int64_t inc = J->c.inc;
+  int f0 = J->c.f0;
+  const int status = J->c.status;
+  int running = J->c.running, courant_due = J->c.courant_due, density_due = J->c.density_due;
+  int preloop = J->c.preloop, pending = J->c.pending;
+  if (pending && status == 0) {
+    t += dt;
+    f0 = 1 - f0;
+    if (courant_due && t < up_to) {
+      double v = 1.0e300;
+      for (int k = threadIdx.x; k < J->n_cmin; k += blockDim.x) v = fmin(v, J->cmin[k]);
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off));
+      if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = v;
+      __syncthreads();
+      double m = s_min[0];
+      for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmin(m, s_min[w]);
+      const double fmax = 2.0 / m;
+      dt = J->safety * J->omegaS / fmax;
+    }
+  }
+  pending = 0;
+  if (open) {
+    preloop = 0;
+    running = (status == 0 && t < up_to) ? 1 : 0;
+    if (running) {
+      if (t + dt > up_to) dt = up_to - t;
+      inc++;
+      courant_due = (inc % J->dt_freq == 0) ? 1 : 0;
+      // densities are consumed by the Courant step and read back at the end
+      density_due = (courant_due || !(t + dt < up_to)) ? 1 : 0;
+      pending = 1;
+      const int64_t k = tk.first + threadIdx.x;
+      if (k < tk.end) {
+        const int64_t n = J->bcn[k];
+        if (J->dim == 3)
+          bc_field0<3>(J, n, f0, t, dt);
+        else
+          bc_field0<2>(J, n, f0, t, dt);
+      }
+    }
+  } else {
+    running = 0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned nb = J->n_bcn > 0 ? (unsigned)((J->n_bcn + 255) / 256) : 1u;
+    __threadfence();
+    if (atomicAdd(&J->c.arrive, 1u) == nb - 1) {
+      Control &c = J->c;
+      c.t = t;
+      c.dt = dt;
+      c.inc = inc;
+      c.f0 = f0;
+      c.running = running;
+      c.courant_due = courant_due;
+      c.density_due = density_due;
+      c.preloop = preloop;
+      c.pending = pending;
+      c.arrive = 0;
+    }
   }
 }
 
@@ -788,14 +820,6 @@ void launch_elem_fp64(int type, JobDev *const *jobs, const Task *tasks, int nblk
   }
 }
 
-void launch_predict(int dim, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s) {
-  if (nblk <= 0) return;
-  if (dim == 3)
-    hipLaunchKernelGGL(k_predict<3>, dim3(nblk), dim3(256), 0, s, jobs, tasks);
-  else
-    hipLaunchKernelGGL(k_predict<2>, dim3(nblk), dim3(256), 0, s, jobs, tasks);
-}
-
 void launch_correct(int dim, bool f32, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s) {
   if (nblk <= 0) return;
   if (dim == 3) {
@@ -815,11 +839,9 @@ void launch_courant(int dim, JobDev *const *jobs, const Task *tasks, int nblk, i
     hipLaunchKernelGGL(k_courant<2>, dim3(nblk), dim3(256), 0, s, jobs, tasks, force);
 }
 
-void launch_cycle_begin(JobDev *const *jobs, int n, hipStream_t s) {
-  hipLaunchKernelGGL(k_cycle_begin, dim3((n + 63) / 64), dim3(64), 0, s, jobs, n);
-}
-void launch_cycle_end(JobDev *const *jobs, int n, hipStream_t s) {
-  hipLaunchKernelGGL(k_cycle_end, dim3((n + 63) / 64), dim3(64), 0, s, jobs, n);
+void launch_cycle_step(JobDev *const *jobs, const Task *tasks, int nblk, int open, hipStream_t s) {
+  if (nblk <= 0) return;
+  hipLaunchKernelGGL(k_cycle_step, dim3(nblk), dim3(256), 0, s, jobs, tasks, open);
 }
 void launch_snapshot(JobDev *const *jobs, int n, double *out, hipStream_t s) {
   hipLaunchKernelGGL(k_snapshot, dim3((n + 63) / 64), dim3(64), 0, s, jobs, n, out);

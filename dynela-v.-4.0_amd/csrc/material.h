@@ -60,6 +60,52 @@ __device__ __forceinline__ void law_yield_and_slope(int law, const double *p, do
   hard = p[1] * p[2] * ex + p[3];
 }
 
+// Per-IP constants of the radial-return iteration of the mixed path.
+struct ReturnConsts {
+  double lrate;     // ln(sqrt(2/3) / (dt * depsp0)): ln(depsp/depsp0) = ln(gamma) + lrate
+  double inv_dT;    // 1 / (Tm - T0)
+  double gamma_rate;  // depsp > depsp0 <=> gamma > depsp0 dt / sqrt(2/3)
+};
+
+__device__ __forceinline__ ReturnConsts return_consts(const double *p, double dt) {
+  ReturnConsts c;
+  c.lrate = log(kSqrt23 / (dt * p[5]));
+  c.inv_dT = 1.0 / (p[6] - p[7]);
+  c.gamma_rate = dt * p[5] / kSqrt23;
+  return c;
+}
+
+// Same yield stress and slope as law_yield_and_slope for the Johnson-Cook
+// law, in terms of gamma (depsp = sqrt(2/3) gamma / dt, so
+// depsp * dt = sqrt(2/3) gamma) with powers as exp/log pairs and the slope's
+// divisions as hardware reciprocals (the slope only sets the Newton step).
+__device__ __forceinline__ void jc_yield_and_slope_fast(const double *p, const ReturnConsts &k, double eta_rcp,
+                                                        double epsp, double gamma, double T, double &yield,
+                                                        double &hard) {
+  const double A = p[0], B = p[1], C = p[2], n = p[3], m = p[4], Tm = p[6], T0 = p[7];
+  const double lg = log(epsp);
+  const double pn = exp(n * lg);
+  const double hard0 = A + B * pn;
+  const bool rate = gamma > k.gamma_rate;  // depsp > depsp0
+  const double visc0 = rate ? 1.0 + C * (log(gamma) + k.lrate) : 1.0;
+  double tp1 = 0.0, temp0 = 1.0;
+  const bool warm = T > T0;
+  const bool soft = warm && T < Tm;
+  if (warm) {
+    if (T < Tm) {
+      tp1 = exp(m * log((T - T0) * k.inv_dT));
+      temp0 = 1.0 - tp1;
+    } else {
+      temp0 = 0.0;
+    }
+  }
+  yield = hard0 * visc0 * temp0;
+  double h = n * B * pn * __builtin_amdgcn_rcp(epsp) * visc0 * temp0;
+  if (rate) h += hard0 * C * temp0 * __builtin_amdgcn_rcp(kSqrt23 * gamma);
+  if (soft) h -= m * eta_rcp * hard0 * hard0 * visc0 * visc0 * temp0 * tp1 * __builtin_amdgcn_rcp(T - T0);
+  hard = h;
+}
+
 // RampFunction::getValue (RampFunction.C:87-131) with the coefficients of
 // RampFunction::setFunction (RampFunction.C:40-84).
 __device__ __forceinline__ double ramp_value(int type, double mn, double mx, double x) {

@@ -681,8 +681,13 @@ class DynELA:
         return _NodeView(self, number)
 
 
-def batch_solve(models, upTo=None, context=None):
-    """Advance several models together (one launch per phase and cycle)."""
+def batch_solve(models, upTo=None, context=None, raise_on_error=True):
+    """Advance several models together (one launch per phase and cycle).
+
+    A job that fails (negative Jacobian, Newton divergence, non-finite state)
+    stops; the others run to their target time.  With raise_on_error the
+    first failure raises after the batch; otherwise the per-model status
+    codes are returned (0 = OK)."""
     ctx = context or models[0].context
     for m in models:
         if m.context is not ctx:
@@ -691,4 +696,10 @@ def batch_solve(models, upTo=None, context=None):
     n = len(models)
     handles = (ctypes.c_void_p * n)(*[m._job.value for m in models])
     t = np.array([m._solver.endTime if upTo is None else float(upTo) for m in models], dtype=np.float64)
-    ctx.check(ctx.lib.dnlg_batch_solve(ctx.handle, handles, n, _lib.as_dp(t)))
+    st = ctx.lib.dnlg_batch_solve(ctx.handle, handles, n, _lib.as_dp(t))
+    statuses = [ctx.lib.dnlg_job_status(m._job) for m in models]
+    if raise_on_error:
+        ctx.check(st)
+    elif st != _lib.OK and all(s == _lib.OK for s in statuses):
+        ctx.check(st)  # a failure of the batch itself, not of a job
+    return statuses
