@@ -123,6 +123,84 @@ def run_sweep(prec, ctx, n_jobs, t_end, seed=20261203):
     return res
 
 
+def _wave_speed(mat):
+    E, nu, rho = mat.youngModulus, mat.poissonRatio, mat.density
+    return np.sqrt(E * (1.0 - nu) / (rho * (1.0 + nu) * (1.0 - 2.0 * nu)))
+
+
+def estimate_work(model):
+    """Rough elem-cycles of a job: n_elem x t_end / dt, dt from the Courant
+    formula with the shortest element edge as length (for choosing the
+    arrival rate only)."""
+    a = model.descriptors()
+    x = a.coords[:, a.conn]                       # [3, NE, nn]
+    edges = np.linalg.norm(x - np.roll(x, 1, axis=2), axis=0).min()
+    omega_s = 1.9818187003
+    dt = 0.9 * omega_s * edges / (2.0 * _wave_speed(model._material))
+    return a.conn.shape[0] * model._solver.endTime / dt
+
+
+def run_serve(prec, ctx, n_jobs, util, rate, cycles_per_step, seed=20261203):
+    """Serving loop: Poisson arrivals of a seeded job mix (axisymmetric / 3D x
+    impact / necking, 1k-500k elements log-uniform, t_end U(20, 100) us);
+    jobs are submitted at their arrival times, admitted by device-memory
+    footprint and advanced together (continuous batching).  Latency =
+    completion - scheduled arrival."""
+    import dnlPython as dnl
+    from dnlPython import samples
+    specs = samples.serve_params(n_jobs, seed=seed)
+    models = [samples.build_serve_job(j, precision=prec, context=ctx) for j in specs]
+    work = np.array([estimate_work(m) for m in models])
+    if rate is None:
+        # offered load = util x an assumed 1e9 elem-cycles/s service rate
+        rate = util * 1.0e9 / work.mean()
+    rng = np.random.default_rng(seed + 1)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_jobs))
+    print(f"[bench] serve {prec}: {n_jobs} jobs, {sum(m.getElementsNumber() for m in models)} elements, "
+          f"rate {rate:.3g} jobs/s, estimated work {work.sum():.3g} elem-cycles", file=sys.stderr, flush=True)
+    srv = dnl.Server(ctx)
+    tickets = {}
+    done = {}
+    busy = 0.0
+    nxt = 0
+    t0 = time.perf_counter()
+    last_print = t0
+    while len(done) < n_jobs:
+        now = time.perf_counter() - t0
+        while nxt < n_jobs and arrivals[nxt] <= now:
+            tickets[srv.submit(models[nxt])] = nxt
+            nxt += 1
+        if len(tickets) == len(done):
+            if nxt < n_jobs:
+                time.sleep(max(0.0, min(arrivals[nxt] - now, 0.05)))
+            continue
+        ts = time.perf_counter()
+        running, queued = srv.step(cycles_per_step)
+        busy += time.perf_counter() - ts
+        for t, i in list(tickets.items()):
+            if i in done:
+                continue
+            state, _ = srv.poll(t)
+            if state == dnl.Server.DONE:
+                m, status = srv.take(t)
+                done[i] = (time.perf_counter() - t0 - arrivals[i], status)
+        if time.perf_counter() - last_print > 20.0:
+            last_print = time.perf_counter()
+            print(f"[bench] serve {prec}: {len(done)}/{n_jobs} done, {running} running, {queued} queued",
+                  file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    lat = np.array([done[i][0] for i in range(n_jobs)])
+    failed = [i for i in range(n_jobs) if done[i][1] != 0]
+    ec = sum(m.getElementsNumber() * m.getIncrement() for m in models)
+    res = dict(wall=wall, value=ec / wall, elem_cycles=ec, p50=float(np.percentile(lat, 50)),
+               p99=float(np.percentile(lat, 99)), busy=busy / wall, rate=rate, failed=failed,
+               elements=sum(m.getElementsNumber() for m in models), span=float(arrivals[-1]))
+    srv.close()
+    for m in models:
+        m.close()
+    return res
+
+
 def cpu_baseline(n, nz, sample_t=1e-6):
     from dnlPython import samples
     from oracle.bridge import OracleJob
@@ -162,8 +240,11 @@ def main():
     ap.add_argument("--nz", type=int, default=100)
     ap.add_argument("--t-end", type=float, default=80e-6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="taylor3d", choices=["taylor3d", "sweep"])
-    ap.add_argument("--jobs", type=int, default=256, help="sweep workload: number of jobs")
+    ap.add_argument("--workload", default="taylor3d", choices=["taylor3d", "sweep", "serve"])
+    ap.add_argument("--jobs", type=int, default=256, help="sweep / serve workloads: number of jobs")
+    ap.add_argument("--util", type=float, default=0.7, help="serve: offered load for the arrival rate")
+    ap.add_argument("--rate", type=float, default=None, help="serve: arrival rate in jobs/s (overrides --util)")
+    ap.add_argument("--step-cycles", type=int, default=32, help="serve: cycles per server step")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the plain-copy reference kernel for the traffic counters")
     args = ap.parse_args()
@@ -172,12 +253,30 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(independent jobs would shard one per GPU, see DESIGN.md)", file=sys.stderr)
+              "(independent jobs would shard one job per GPU; one large mesh by element slabs)",
+              file=sys.stderr)
         sys.exit(2)
 
     import dnlPython as dnl
     ctx = dnl.Context.default()
     precs = [p.strip() for p in args.precision.split(",") if p.strip()]
+    if args.workload == "serve":
+        p = precs[0]
+        r = run_serve(p, ctx, args.jobs, args.util, args.rate, args.step_cycles)
+        print(f"[bench] serve {p}: {r['value']:.4g} elem-cycles/s, p50 {r['p50']:.3f}s, p99 {r['p99']:.3f}s, "
+              f"busy {r['busy']:.2f}, failed {r['failed']}", file=sys.stderr, flush=True)
+        line = {"metric": "served elem-cycles/s (continuous batching, Poisson arrivals)", "value": r["value"],
+                "unit": "elem-cycles/s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": r["wall"] * 1e3,
+                "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                "dtype": "fp32+fp64" if p == "mixed" else "f64", "data": "synthetic",
+                "config": {"workload": f"serving mix of {args.jobs} jobs (axi / 3D x impact / necking, "
+                                       f"{r['elements']} elements in total, 1k-500k each, t_end 20-100 us)",
+                           "precision": p, "arrival_rate_jobs_per_s": r["rate"],
+                           "cycles_per_step": args.step_cycles},
+                "latency_p50_s": r["p50"], "latency_p99_s": r["p99"], "busy_fraction": r["busy"],
+                "failed_jobs": r["failed"]}
+        print(json.dumps(line), flush=True)
+        return
     if args.workload == "sweep":
         res = {}
         for p in precs:

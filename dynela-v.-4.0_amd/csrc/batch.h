@@ -23,6 +23,8 @@ struct dnlg_job {
   bool started = false;
   int32_t n_cmin = 0;
   int32_t n_bcn = 0;  // nodes with constant BCs
+  double t_now = 0.0;  // time and target at the last progress snapshot
+  double up_to = 0.0;
   int status = 0;
 };
 
@@ -47,7 +49,12 @@ struct dnlg_ctx {
 
 namespace dnlg {
 
-int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vector<double> &up_to);
+// Advance the jobs together toward up_to[i] (clipped to each job's end
+// time); with max_cycles >= 0, return after about that many cycles (whole
+// graphs) even if some job has not arrived.  Returns the first failing job's
+// status, or a launch error.
+int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vector<double> &up_to,
+              int64_t max_cycles = -1);
 void batch_cache_clear(dnlg_ctx *ctx);
 void batch_cache_forget(dnlg_ctx *ctx, const dnlg_job *job);
 

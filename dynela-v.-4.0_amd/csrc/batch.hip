@@ -32,12 +32,14 @@ struct Batch {
   hipGraphExec_t graph = nullptr;       // graph_cycles whole cycles
   hipGraphExec_t graph_pre = nullptr;   // timed mode: pre phase
   hipGraphExec_t graph_post = nullptr;  // timed mode: post phase
+  hipGraphExec_t graph_rest = nullptr;  // timed mode: graph_cycles - 1 whole cycles
   int graph_cycles = 0;
   std::vector<void *> allocs;
   ~Batch() {
     if (graph) hipGraphExecDestroy(graph);
     if (graph_pre) hipGraphExecDestroy(graph_pre);
     if (graph_post) hipGraphExecDestroy(graph_post);
+    if (graph_rest) hipGraphExecDestroy(graph_rest);
     for (void *p : allocs) hipFree(p);
   }
 };
@@ -233,14 +235,19 @@ int capture(dnlg_ctx *ctx, F &&f, hipGraphExec_t *out) {
   return 0;
 }
 
-// Untimed: `cycles` whole cycles in one graph.  Timed: the pre and post
-// phases as two graphs, the element kernels launched between them with
-// start/stop events.
+// Untimed: `cycles` whole cycles in one graph.  Timed: one sampled cycle
+// (the pre and post phases as two graphs, the element kernels launched
+// between them with start/stop events) followed by `cycles` - 1 whole
+// cycles in one graph, so the timing costs one cycle in `cycles`.
 int capture_graphs(dnlg_ctx *ctx, Batch &B, int cycles, bool timed) {
   int st;
   if (timed) {
     if ((st = capture(ctx, [&] { launch_pre(ctx, B); }, &B.graph_pre))) return st;
     if ((st = capture(ctx, [&] { launch_post(ctx, B); }, &B.graph_post))) return st;
+    if (cycles > 1 &&
+        (st = capture(ctx, [&] { for (int k = 0; k < cycles - 1; k++) launch_cycle(ctx, B); }, &B.graph_rest)))
+      return st;
+    B.graph_cycles = cycles;
     return 0;
   }
   if ((st = capture(ctx, [&] { for (int k = 0; k < cycles; k++) launch_cycle(ctx, B); }, &B.graph))) return st;
@@ -250,18 +257,10 @@ int capture_graphs(dnlg_ctx *ctx, Batch &B, int cycles, bool timed) {
 
 }  // namespace
 
-int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vector<double> &up_to) {
+int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vector<double> &up_to,
+              int64_t max_cycles) {
   const int n = (int)jobs.size();
   hipStream_t s = ctx->stream;
-  // target times, clipped to each job's end time
-  for (int i = 0; i < n; i++) {
-    double u = std::min(up_to[i], jobs[i]->t_end);
-    if (hipMemcpyAsync(&jobs[i]->dev->c.up_to, &u, sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess) {
-      ctx->err = "setting the target time failed";
-      return DNLG_E_HIP;
-    }
-  }
   int st = 0;
   std::vector<dnlg_job *> fresh;
   for (auto *j : jobs)
@@ -283,6 +282,13 @@ int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vec
     ctx->err = "building the batch failed";
     return st;
   }
+  // target times, clipped to each job's end time
+  for (int i = 0; i < n; i++) B->h_snap[i] = std::min(up_to[i], jobs[i]->t_end);
+  if (hipMemcpyAsync(B->d_snap, B->h_snap.data(), n * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess) {
+    ctx->err = "setting the target times failed";
+    return DNLG_E_HIP;
+  }
+  launch_set_up_to(B->d_jobs, n, B->d_snap, s);
   const bool use_graph = ctx->graph_cycles > 0;
   const bool timed = ctx->timing;
   if (use_graph && !(timed ? B->graph_pre : B->graph)) {
@@ -291,7 +297,7 @@ int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vec
       return st;
     }
   }
-  constexpr int kTimedChunk = 64;  // cycles between event read-backs
+  constexpr int kTimedChunk = 64;  // sampled cycles between event read-backs
   std::vector<int> ev_used;        // events used per timed cycle
   auto collect = [&]() -> int {
     auto &acc = ctx->ktime["element"];
@@ -343,21 +349,30 @@ int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vec
       const double t = B->h_snap[5 * i], dt = B->h_snap[5 * i + 1], up = B->h_snap[5 * i + 2];
       const int js = (int)B->h_snap[5 * i + 3];
       jobs[i]->status = js;
+      jobs[i]->t_now = t;
+      jobs[i]->up_to = up;
       if (js != 0 || !(t < up)) continue;
       need = std::max(need, dt > 0 ? std::ceil((up - t) / dt) : 1.0);
     }
     if (need <= 0.0) break;
+    if (max_cycles >= 0) {
+      if (launched >= max_cycles) break;
+      need = std::min(need, (double)(max_cycles - launched));
+    }
     const int chunk = (int)std::min(need, 4096.0);
     launched += chunk;
     if (timed) {
-      const int cycles = std::min(chunk, kTimedChunk);
-      while (ctx->events.size() < (size_t)8 * cycles) {
+      // units of one sampled cycle + (graph_cycles - 1) graph cycles
+      const int per = use_graph ? std::max(1, B->graph_cycles) : 1;
+      const int units = std::min((chunk + per - 1) / per, kTimedChunk);
+      launched += (int64_t)units * per - chunk;
+      while (ctx->events.size() < (size_t)8 * units) {
         hipEvent_t e;
         if (hipEventCreate(&e) != hipSuccess) return DNLG_E_HIP;
         ctx->events.push_back(e);
       }
       size_t base = 0;
-      for (int k = 0; k < cycles; k++) {
+      for (int k = 0; k < units; k++) {
         if (use_graph) {
           if (hipGraphLaunch(B->graph_pre, s) != hipSuccess) return DNLG_E_HIP;
         } else {
@@ -368,6 +383,7 @@ int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vec
         base += 2 * used;
         if (use_graph) {
           if (hipGraphLaunch(B->graph_post, s) != hipSuccess) return DNLG_E_HIP;
+          if (B->graph_rest && hipGraphLaunch(B->graph_rest, s) != hipSuccess) return DNLG_E_HIP;
         } else {
           launch_post(ctx, *B);
         }

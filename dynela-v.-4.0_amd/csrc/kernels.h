@@ -22,6 +22,7 @@ void launch_courant(int dim, JobDev *const *jobs, const Task *tasks, int nblk, i
 // the job's BC node list
 void launch_cycle_step(JobDev *const *jobs, const Task *tasks, int nblk, int open, hipStream_t s);
 void launch_force_dt(JobDev *const *jobs, int n, hipStream_t s);
+void launch_set_up_to(JobDev *const *jobs, int n, const double *up_to, hipStream_t s);
 void launch_prep(int type, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s);
 void launch_mass_gather(int dim, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s);
 // per job: {t, dt, up_to, status, inc} snapshot for the host loop

@@ -671,6 +671,13 @@ __global__ void k_snapshot(JobDev *const *jobs, int n_jobs, double *out) {
   out[5 * j + 4] = (double)c.inc;
 }
 
+// Target time of the current advance, per job.
+__global__ void k_set_up_to(JobDev *const *jobs, int n_jobs, const double *up_to) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_jobs) return;
+  jobs[j]->c.up_to = up_to[j];
+}
+
 // Forced time step of the pre-loop (Explicit.C:152-154).
 __global__ void k_force_dt(JobDev *const *jobs, int n_jobs) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -845,6 +852,9 @@ void launch_cycle_step(JobDev *const *jobs, const Task *tasks, int nblk, int ope
 }
 void launch_snapshot(JobDev *const *jobs, int n, double *out, hipStream_t s) {
   hipLaunchKernelGGL(k_snapshot, dim3((n + 63) / 64), dim3(64), 0, s, jobs, n, out);
+}
+void launch_set_up_to(JobDev *const *jobs, int n, const double *up_to, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_up_to, dim3((n + 63) / 64), dim3(64), 0, s, jobs, n, up_to);
 }
 void launch_force_dt(JobDev *const *jobs, int n, hipStream_t s) {
   hipLaunchKernelGGL(k_force_dt, dim3((n + 63) / 64), dim3(64), 0, s, jobs, n);

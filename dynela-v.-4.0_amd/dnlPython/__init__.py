@@ -703,3 +703,70 @@ def batch_solve(models, upTo=None, context=None, raise_on_error=True):
     elif st != _lib.OK and all(s == _lib.OK for s in statuses):
         ctx.check(st)  # a failure of the batch itself, not of a job
     return statuses
+
+
+class Server:
+    """Continuous-batching serving loop over one context (dnlg_server_*).
+
+    ``submit(model)`` queues a model built with this API and returns a ticket;
+    ``step(cycles)`` admits queued jobs whose device footprint fits in the
+    memory budget, advances every running job by about ``cycles`` cycles and
+    retires finished ones; ``take(ticket)`` returns the finished model (its
+    getters read the engine job as after ``solve()``) together with its
+    status.  Results of a job do not depend on the jobs it ran beside."""
+
+    QUEUED, RUNNING, DONE = 0, 1, 2
+
+    def __init__(self, context=None, hbm_budget=0, max_jobs=0):
+        self.ctx = context or Context.default()
+        self.handle = ctypes.c_void_p()
+        self.ctx.check(self.ctx.lib.dnlg_server_create(self.ctx.handle, int(hbm_budget), int(max_jobs),
+                                                       ctypes.byref(self.handle)))
+        self._models = {}
+
+    def submit(self, model):
+        if model._job is not None:
+            raise DnlgError(_lib.E_STATE, "the model already has an engine job")
+        a = model.descriptors()
+        t = ctypes.c_int64()
+        self.ctx.check(self.ctx.lib.dnlg_server_submit(self.handle, ctypes.byref(a.mesh), ctypes.byref(a.material),
+                                                       ctypes.byref(a.bcs), ctypes.byref(a.params),
+                                                       ctypes.byref(t)))
+        self._models[t.value] = model
+        return t.value
+
+    def step(self, cycles=64):
+        """Returns (running, queued) after the step."""
+        r, q = ctypes.c_int32(), ctypes.c_int32()
+        self.ctx.check(self.ctx.lib.dnlg_server_step(self.handle, int(cycles), ctypes.byref(r), ctypes.byref(q)))
+        return r.value, q.value
+
+    def poll(self, ticket):
+        """(state, status): state QUEUED / RUNNING / DONE."""
+        st, code = ctypes.c_int32(), ctypes.c_int32()
+        self.ctx.check(self.ctx.lib.dnlg_server_poll(self.handle, int(ticket), ctypes.byref(st), ctypes.byref(code)))
+        return st.value, code.value
+
+    def take(self, ticket):
+        """(model, status) of a finished job; the model owns the job from now on."""
+        job = ctypes.c_void_p()
+        status = self.ctx.lib.dnlg_server_take(self.handle, int(ticket), ctypes.byref(job))
+        model = self._models.pop(int(ticket))
+        if job.value:
+            model._job = job
+            model._ctx = self.ctx
+        elif status != _lib.OK:
+            self.ctx.check(status)
+        return model, status
+
+    def close(self):
+        if self.handle:
+            self.ctx.lib.dnlg_server_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+            self._models.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

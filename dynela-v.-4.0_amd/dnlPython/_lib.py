@@ -94,6 +94,13 @@ SIGNATURES = {
     "dnlg_ctx_set_timing": (ctypes.c_int, [_VP, ctypes.c_int]),
     "dnlg_ctx_kernel_time": (ctypes.c_int, [_VP, ctypes.c_char_p, _dp, _i64p]),
     "dnlg_ctx_reset_timing": (ctypes.c_int, [_VP]),
+    "dnlg_server_create": (ctypes.c_int, [_VP, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(_VP)]),
+    "dnlg_server_destroy": (ctypes.c_int, [_VP]),
+    "dnlg_server_submit": (ctypes.c_int, [_VP, ctypes.POINTER(Mesh), ctypes.POINTER(MaterialDesc),
+                                          ctypes.POINTER(Bcs), ctypes.POINTER(SolverParams), _i64p]),
+    "dnlg_server_step": (ctypes.c_int, [_VP, ctypes.c_int, _i32p, _i32p]),
+    "dnlg_server_poll": (ctypes.c_int, [_VP, ctypes.c_int64, _i32p, _i32p]),
+    "dnlg_server_take": (ctypes.c_int, [_VP, ctypes.c_int64, ctypes.POINTER(_VP)]),
 }
 
 _lib = None

@@ -12,7 +12,9 @@ Workloads
                120 000 hexes)
   necking_3d   3D round bar in tension with a 2 % mid-length taper,
                Voce + linear hardening, one end fixed, the other pulled
+  necking_axi  the same bar with 4-node axisymmetric quads
   sweep_params seeded parameter sweep around the copper impact
+  serve_params seeded serving mix (axi / 3D x impact / necking)
 """
 import numpy as np
 
@@ -210,13 +212,7 @@ def necking_3d(n=16, nz=60, radius=6.413e-3, length=53.334e-3, taper=0.02, pull=
     model.createElements(np.arange(1, ne + 1), conn)
     allES = ElementSet("ES_All")
     model.add(allES, 1, ne)
-    law = VoceLinearLaw("Voce")
-    law.setParameters(p["A"], p["Q"], p["b"], p["H"])
-    mat = Material("bar")
-    mat.setHardeningLaw(law)
-    mat.youngModulus, mat.poissonRatio, mat.density = p["young"], p["poisson"], p["density"]
-    mat.heatCapacity, mat.taylorQuinney, mat.T0 = p["heatCapacity"], p["taylorQuinney"], p["T0"]
-    model.add(mat, allES)
+    model.add(_voce_material(p), allES)
     bottom = BoundaryRestrain("BC_bottom")
     bottom.setValue(0, 1, 0)
     model.attachConstantBC(bottom, _set(range(1, nl + 1), "NS_Bottom"))
@@ -225,6 +221,95 @@ def necking_3d(n=16, nz=60, radius=6.413e-3, length=53.334e-3, taper=0.02, pull=
     model.attachConstantBC(top, _set(range(nz * nl + 1, (nz + 1) * nl + 1), "NS_Top"))
     model.add(_solver(t_end, precision, track_tensors, freq))
     return model
+
+
+def _voce_material(p):
+    law = VoceLinearLaw("Voce")
+    law.setParameters(p["A"], p["Q"], p["b"], p["H"])
+    mat = Material("bar")
+    mat.setHardeningLaw(law)
+    mat.youngModulus, mat.poissonRatio, mat.density = p["young"], p["poisson"], p["density"]
+    mat.heatCapacity, mat.taylorQuinney, mat.T0 = p["heatCapacity"], p["taylorQuinney"], p["T0"]
+    return mat
+
+
+def necking_axi(nr=10, nz=80, radius=6.413e-3, length=53.334e-3, taper=0.02, pull=1.0, t_end=50e-6,
+                material=None, precision="fp64", track_tensors=True, context=None, freq=10):
+    """Axisymmetric round bar in tension (4-node axisymmetric quads): the
+    necking_3d geometry, loads and law on the (r, y) half-plane, nodes
+    restrained radially on the axis."""
+    p = dict(NECKING)
+    if material:
+        p.update(material)
+    model = DynELA("BarNecking-Axi", context=context)
+    ys = np.linspace(0.0, length, nz + 1)
+    rad = radius * (1.0 - taper * 0.5 * (1.0 + np.cos(2.0 * np.pi * (ys / length - 0.5))))
+    i, j = np.meshgrid(np.arange(nr + 1), np.arange(nz + 1))
+    nums = (j * (nr + 1) + i + 1).ravel()
+    xyz = np.stack([(i / nr * rad[j]).ravel(), ys[j].ravel(), np.zeros(nums.size)], axis=1)
+    model.createNodes(nums, xyz)
+    model.setDefaultElement(Element.ElQua4NAx)
+    ei, ej = np.meshgrid(np.arange(nr), np.arange(nz))
+    n1 = ei + ej * (nr + 1) + 1
+    conn = np.stack([n1, n1 + 1, ei + (ej + 1) * (nr + 1) + 2, ei + (ej + 1) * (nr + 1) + 1], axis=-1)
+    model.createElements(np.arange(1, nr * nz + 1), conn.reshape(-1, 4))
+    allES = ElementSet("ES_All")
+    model.add(allES, 1, nr * nz)
+    model.add(_voce_material(p), allES)
+    bottom = BoundaryRestrain("BC_bottom")
+    bottom.setValue(0, 1, 1)
+    model.attachConstantBC(bottom, _set(range(1, nr + 2), "NS_Bottom"))
+    axis = BoundaryRestrain("BC_axis")
+    axis.setValue(1, 0, 1)
+    model.attachConstantBC(axis, _set(range(1, nums.size + 1, nr + 1), "NS_Axis"))
+    top = BoundarySpeed("BC_top")
+    top.setValue(0, pull, 0)
+    model.attachConstantBC(top, _set(range(nz * (nr + 1) + 1, (nz + 1) * (nr + 1) + 1), "NS_Top"))
+    model.add(_solver(t_end, precision, track_tensors, freq))
+    return model
+
+
+def serve_params(n_jobs, seed=20261203, lo=1000, hi=500000):
+    """Seeded serving mix: kind {axisymmetric quads, 3D hexes} x {impact,
+    necking} uniformly, element count log-uniform in [lo, hi], end time
+    U(20, 100) us; impacts draw the sweep's copper Johnson-Cook scalings and
+    speed, necking bars scale the Voce A, Q, b, H by U(0.8, 1.2)."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for _ in range(n_jobs):
+        dim = "axi" if rng.random() < 0.5 else "3d"
+        load = "impact" if rng.random() < 0.5 else "necking"
+        ne = int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+        t_end = float(rng.uniform(20e-6, 100e-6))
+        scale = rng.uniform(0.8, 1.2, size=5)
+        speed = float(rng.uniform(150.0, 300.0))
+        if load == "impact":
+            mat = {k: COPPER[k] * s for k, s in zip(("A", "B", "n", "C", "m"), scale)}
+            aspect = 10.0
+        else:
+            mat = {k: NECKING[k] * s for k, s in zip(("A", "Q", "b", "H"), scale[:4])}
+            aspect = 8.0
+        if dim == "axi":
+            nr = max(2, int(round(np.sqrt(ne / aspect))))
+            shape = (nr, max(2, int(round(ne / nr))))
+        else:
+            nc = max(2, int(round((ne / (1.5 * aspect)) ** (1.0 / 3.0))))
+            shape = (nc, max(2, int(round(ne / (3 * nc * nc)))))
+        jobs.append(dict(kind=dim, load=load, shape=shape, material=mat, speed=speed, t_end=t_end))
+    return jobs
+
+
+def build_serve_job(job, precision="mixed", track_tensors=False, context=None):
+    a, b = job["shape"]
+    kw = dict(t_end=job["t_end"], material=job["material"], precision=precision, track_tensors=track_tensors,
+              context=context)
+    if job["load"] == "impact":
+        if job["kind"] == "axi":
+            return taylor_axi(a, b, speed=job["speed"], **kw)
+        return taylor_3d(a, b, speed=job["speed"], **kw)
+    if job["kind"] == "axi":
+        return necking_axi(a, b, **kw)
+    return necking_3d(a, b, **kw)
 
 
 def sweep_params(n_jobs=256, seed=20261203, lo=2000, hi=60000):

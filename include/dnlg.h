@@ -188,6 +188,38 @@ int dnlg_job_solve(dnlg_job *job, double t_up_to);
 int dnlg_batch_solve(dnlg_ctx *ctx, dnlg_job *const *jobs, int n_jobs,
                      const double *t_up_to);
 
+/* ---- serving loop ----------------------------------------------------------
+ * The reference solves one model per process to its end time (Model::solve,
+ * Model.C:696-730, through Solver::solve, Solver.h:70); it has no queue.
+ * The server is that same solve run as a service: submitted models wait in a
+ * queue, are created on the device when their footprint
+ * (dnlg_footprint_bytes) fits in the memory budget, and every admitted job
+ * advances toward its t_end together with the others, max_cycles cycles per
+ * dnlg_server_step (continuous batching: jobs join and leave between steps).
+ * A job's results do not depend on the other jobs it ran with. */
+typedef struct dnlg_server dnlg_server;
+/* hbm_budget <= 0: 85 % of the device memory free at creation;
+ * max_jobs <= 0: 1024 jobs in flight at most */
+int dnlg_server_create(dnlg_ctx *ctx, int64_t hbm_budget, int max_jobs,
+                       dnlg_server **out);
+/* destroys every job the server still holds */
+int dnlg_server_destroy(dnlg_server *srv);
+/* copies the descriptors (as dnlg_job_create would) and queues the job;
+ * DNLG_E_CAPACITY if its footprint exceeds the whole budget */
+int dnlg_server_submit(dnlg_server *srv, const dnlg_mesh *mesh,
+                       const dnlg_material *mat, const dnlg_bcs *bcs,
+                       const dnlg_solver_params *params, int64_t *ticket);
+/* admits what fits, advances the running jobs by about max_cycles cycles,
+ * retires finished and failed ones.  Blocking. */
+int dnlg_server_step(dnlg_server *srv, int max_cycles, int *n_running,
+                     int *n_queued);
+/* state: 0 queued, 1 running, 2 finished; status: the job's status code */
+int dnlg_server_poll(dnlg_server *srv, int64_t ticket, int *state, int *status);
+/* hands a finished job over to the caller (read it with dnlg_job_read_*, free
+ * it with dnlg_job_destroy) and forgets the ticket; returns the job's final
+ * status.  *job is NULL when the job could not be created. */
+int dnlg_server_take(dnlg_server *srv, int64_t ticket, dnlg_job **job);
+
 /* ---- read-back ------------------------------------------------------------ */
 int dnlg_job_sizes(const dnlg_job *job, int64_t *n_nodes, int64_t *n_elems,
                    int32_t *n_ip, int32_t *dim);
@@ -211,9 +243,10 @@ int64_t dnlg_footprint_bytes(int elem_type, int64_t n_nodes, int64_t n_elems,
 int dnlg_job_status(const dnlg_job *job);
 
 /* ---- measurement hooks ------------------------------------------------------ */
-/* Per-kernel device time (ms) accumulated with HIP events on the engine's own
- * stream since the last reset; names: "element", "predict", "correct",
- * "courant", "control".  Timing is off unless enabled. */
+/* Device time (ms) and launch count of the element kernels, accumulated with
+ * HIP events recorded around them on the engine's own stream since the last
+ * reset; name: "element".  With graphs on, one cycle in every graph's worth
+ * of cycles is sampled.  Timing is off unless enabled. */
 int dnlg_ctx_set_timing(dnlg_ctx *ctx, int enable);
 int dnlg_ctx_kernel_time(dnlg_ctx *ctx, const char *name, double *total_ms,
                          int64_t *launches);

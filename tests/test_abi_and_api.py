@@ -165,3 +165,33 @@ def test_sweep_parameters_are_seeded_and_in_range():
         assert 150.0 <= j["speed"] <= 300.0
         for k in ("A", "B", "n", "C", "m"):
             assert 0.8 * samples.COPPER[k] <= j["material"][k] <= 1.2 * samples.COPPER[k]
+
+
+def test_serve_mix_is_seeded_and_in_range():
+    a = samples.serve_params(48)
+    assert a == samples.serve_params(48)
+    kinds = {(j["kind"], j["load"]) for j in a}
+    assert kinds == {("axi", "impact"), ("axi", "necking"), ("3d", "impact"), ("3d", "necking")}
+    for j in a:
+        assert 20e-6 <= j["t_end"] <= 100e-6
+        m = samples.build_serve_job(j, precision="fp64")
+        ne = m.getElementsNumber()
+        assert 700 <= ne <= 700000
+
+
+def test_necking_axi_descriptors():
+    """Axisymmetric bar: CCW quads, bottom row axial restraint, axis radial
+    restraint, top row constant pull speed; mid-length radius reduced by the
+    taper."""
+    m = samples.necking_axi(nr=4, nz=8, pull=2.0)
+    a = m.descriptors()
+    x, y = a.coords[0][a.conn], a.coords[1][a.conn]
+    area = 0.5 * np.sum(x * np.roll(y, -1, 1) - np.roll(x, -1, 1) * y, axis=1)
+    assert np.all(area > 0)
+    r_outer = a.coords[0].reshape(9, 5)[:, -1]
+    assert r_outer[4] == pytest.approx(6.413e-3 * 0.98, rel=1e-12)
+    assert r_outer[0] == pytest.approx(6.413e-3, rel=1e-12)
+    kinds = [(b.kind, tuple(b.value)) for b in list(a.boundaries)[: a.bcs.n_boundaries]]
+    ptr, idx = a.const_ptr, a.const_idx
+    top = [kinds[i] for i in idx[ptr[8 * 5 + 2]:ptr[8 * 5 + 3]]]
+    assert top == [(_lib.BC_SPEED, (0.0, 2.0, 0.0))]

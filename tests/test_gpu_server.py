@@ -1,0 +1,110 @@
+"""Serving loop (GPU): jobs submitted at different times, admitted by memory
+footprint and advanced together in short steps finish with the same results
+as the same job solved alone.  Per-job task lists and reductions make the
+results bit-identical, in FP64 and in mixed precision."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _specs():
+    from dnlPython import samples
+    return [
+        lambda ctx, p: samples.taylor_axi(6, 30, speed=200.0, t_end=12e-6, precision=p, context=ctx),
+        lambda ctx, p: samples.taylor_3d(4, 12, speed=250.0, t_end=9e-6, precision=p, context=ctx),
+        lambda ctx, p: samples.necking_axi(5, 40, pull=4.0, t_end=10e-6, precision=p, context=ctx),
+        lambda ctx, p: samples.necking_3d(4, 10, pull=4.0, t_end=8e-6, precision=p, context=ctx),
+        lambda ctx, p: samples.taylor_axi(10, 50, speed=280.0, t_end=6e-6, precision=p, context=ctx),
+    ]
+
+
+def _same(a, b):
+    import dnlPython as dnl
+    assert a.getScalars() == b.getScalars()
+    np.testing.assert_array_equal(a.getNodalDisplacements(), b.getNodalDisplacements())
+    np.testing.assert_array_equal(a.getNodalSpeeds(), b.getNodalSpeeds())
+    np.testing.assert_array_equal(a.getIntPointField(dnl.Field.plasticStrain),
+                                  b.getIntPointField(dnl.Field.plasticStrain))
+    np.testing.assert_array_equal(a.getIntPointField(dnl.Field.Stress), b.getIntPointField(dnl.Field.Stress))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_server_results_equal_alone(ctx, prec):
+    import dnlPython as dnl
+    specs = _specs()
+    alone = [f(ctx, prec) for f in specs]
+    for m in alone:
+        m.solve()
+    srv = dnl.Server(ctx)
+    models = [f(ctx, prec) for f in specs]
+    tickets = [srv.submit(models[0]), srv.submit(models[1])]
+    srv.step(7)
+    tickets.append(srv.submit(models[2]))
+    srv.step(5)
+    tickets += [srv.submit(models[3]), srv.submit(models[4])]
+    out = {}
+    for _ in range(100000):
+        running, queued = srv.step(16)
+        for t in tickets:
+            if t not in out and srv.poll(t)[0] == dnl.Server.DONE:
+                out[t] = srv.take(t)
+        if len(out) == len(tickets):
+            break
+    assert len(out) == len(tickets)
+    for t, a in zip(tickets, alone):
+        m, status = out[t]
+        assert status == 0
+        _same(a, m)
+    srv.close()
+
+
+def _footprint(model):
+    from dnlPython import _lib
+    a = model.descriptors()
+    return _lib.load().dnlg_footprint_bytes(a.mesh.elem_type, a.mesh.n_nodes, a.mesh.n_elems,
+                                            a.params.precision, a.params.track_tensors)
+
+
+def test_server_admission_by_footprint(ctx):
+    """With a budget just above the largest job's footprint, the running jobs
+    never hold more than the budget (the largest job runs alone, smaller
+    ones may share), all finish and match the alone results; a job larger
+    than the whole budget is refused at submission."""
+    import dnlPython as dnl
+    from dnlPython import samples
+    specs = _specs()[:3]
+    alone = [f(ctx, "fp64") for f in specs]
+    for m in alone:
+        m.solve()
+    models = [f(ctx, "fp64") for f in specs]
+    fps = [_footprint(m) for m in models]
+    budget = max(fps) + 1
+    srv = dnl.Server(ctx, hbm_budget=budget)
+    tickets = [srv.submit(m) for m in models]
+    out = {}
+    largest_alone = False
+    for _ in range(100000):
+        srv.step(32)
+        states = [srv.poll(t)[0] if t not in out else None for t in tickets]
+        held = sum(f for f, s in zip(fps, states) if s in (dnl.Server.RUNNING, dnl.Server.DONE))
+        assert held <= budget
+        big = int(np.argmax(fps))
+        if states[big] == dnl.Server.RUNNING:
+            assert sum(s == dnl.Server.RUNNING for s in states) == 1
+            largest_alone = True
+        for t, s in zip(tickets, states):
+            if s == dnl.Server.DONE:
+                out[t] = srv.take(t)
+        if len(out) == len(tickets):
+            break
+    assert largest_alone
+    for t, a in zip(tickets, alone):
+        m, status = out[t]
+        assert status == 0
+        _same(a, m)
+    big = samples.taylor_3d(12, 40, t_end=1e-6, context=ctx)
+    assert _footprint(big) > budget
+    with pytest.raises(dnl.DnlgError):
+        srv.submit(big)
+    srv.close()
