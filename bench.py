@@ -163,6 +163,7 @@ def run_serve(prec, ctx, n_jobs, util, rate, cycles_per_step, seed=20261203):
     done = {}
     busy = 0.0
     nxt = 0
+    step = cycles_per_step
     t0 = time.perf_counter()
     last_print = t0
     while len(done) < n_jobs:
@@ -170,13 +171,21 @@ def run_serve(prec, ctx, n_jobs, util, rate, cycles_per_step, seed=20261203):
         while nxt < n_jobs and arrivals[nxt] <= now:
             tickets[srv.submit(models[nxt])] = nxt
             nxt += 1
+            step = cycles_per_step
         if len(tickets) == len(done):
             if nxt < n_jobs:
                 time.sleep(max(0.0, min(arrivals[nxt] - now, 0.05)))
             continue
         ts = time.perf_counter()
-        running, queued = srv.step(cycles_per_step)
-        busy += time.perf_counter() - ts
+        running, queued = srv.step(step)
+        el = time.perf_counter() - ts
+        busy += el
+        # steps of 10-40 ms: short enough to admit new arrivals promptly,
+        # long enough that the host loop costs little
+        if el < 0.010:
+            step = min(step * 2, 8192)
+        elif el > 0.040:
+            step = max(cycles_per_step, step // 2)
         for t, i in list(tickets.items()):
             if i in done:
                 continue
@@ -190,6 +199,11 @@ def run_serve(prec, ctx, n_jobs, util, rate, cycles_per_step, seed=20261203):
                   file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     lat = np.array([done[i][0] for i in range(n_jobs)])
+    for i in np.argsort(-lat)[:8]:
+        j = specs[i]
+        print(f"[bench] serve job {i}: {j['kind']} {j['load']} {models[i].getElementsNumber()} elements, "
+              f"t_end {j['t_end'] * 1e6:.1f} us, {models[i].getIncrement()} cycles, latency {lat[i]:.2f} s, "
+              f"status {done[i][1]}", file=sys.stderr, flush=True)
     failed = [i for i in range(n_jobs) if done[i][1] != 0]
     ec = sum(m.getElementsNumber() * m.getIncrement() for m in models)
     res = dict(wall=wall, value=ec / wall, elem_cycles=ec, p50=float(np.percentile(lat, 50)),

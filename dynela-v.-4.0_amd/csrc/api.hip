@@ -12,6 +12,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -132,6 +133,9 @@ int dnlg_ctx_create(int device, dnlg_ctx **out) {
   if (hipSetDevice(device) != hipSuccess) return DNLG_E_HIP;
   dnlg_ctx *ctx = new dnlg_ctx();
   ctx->device = device;
+  // DNLG_GRAPH_CYCLES=<n>: cycles per captured graph; 0 launches every
+  // kernel directly (for tools that do not follow graph launches)
+  if (const char *g = std::getenv("DNLG_GRAPH_CYCLES")) ctx->graph_cycles = std::max(0, std::atoi(g));
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
     delete ctx;
     return DNLG_E_HIP;

@@ -1,9 +1,9 @@
 // elem_mixed.h -- mixed-precision element kernel (included by kernels.hip).
 //
 // Same phases as k_elem_fp64 (Element.C:1103-1161, 362-404, 407-622,
-// 1243-1256) with FP32 geometry, strain, stress and forces, and FP64 for the
-// radial-return Newton iteration on gamma, the equivalent plastic strain, the
-// temperature and the energy accumulators.
+// 1243-1256) with FP32 geometry, strain, stress, radial return and forces,
+// and FP64 for the nodal kinematics and the accumulated equivalent plastic
+// strain, temperature and energies.
 //
 // Strain increment in increment form: with H = d(du)/dx the right
 // Cauchy-Green tensor is C = I + X, X = H + H^T + H^T H (small), and
@@ -303,52 +303,55 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
       J->sy32[gi] = (float)yield;
     }
     if (Strial > yield) {
-      // radial return in FP64 (Element.C:485-601)
+      // radial return (Element.C:485-601): Newton on gamma with bisection
+      // safeguard, FP32 arithmetic; ep, T and the energies accumulate in FP64
+      constexpr float s23 = (float)kSqrt23;
       const double ep0 = J->ep[gi];
-      double gmin = 0.0, gmax = (Strial - yield) / TwoG32;
-      gamma = (double)J->gam32[gi];
-      if (ep0 == 0.0) gamma = kSqrt32 * kGammaInitial;
-      double depsp = kSqrt23 * gamma / dt;
-      double epsp = ep0 + kSqrt23 * gamma;
-      double T = Tcur + 0.5 * gamma * heatFrac * (kSqrt23 * yield + (double)Snorm0);
-      const bool jc = law == DNLG_LAW_JOHNSON_COOK;
-      const ReturnConsts rk = return_consts(lp, dt);
+      const float TwoG32f = (float)TwoG32;
+      const float Strialf = (float)Strial;
+      const float heatf = (float)heatFrac;
+      const float Tc = (float)Tcur;
+      const float ep0f = (float)ep0;
+      const LawF L = law_f32_setup(law, lp, dt, heatFrac);
+      float yieldf = (float)yield;
+      float gmin = 0.f, gmax = (Strialf - yieldf) / TwoG32f;
+      float g = J->gam32[gi];
+      if (ep0 == 0.0) g = (float)(kSqrt32 * kGammaInitial);
+      float epsp = ep0f + s23 * g;
+      float T = Tc + 0.5f * g * heatf * (s23 * yieldf + Snorm0);
       int iterate = 0;
       while (true) {
-        double hard;
-        if (jc)
-          jc_yield_and_slope_fast(lp, rk, heatFrac, epsp, gamma, T, yield, hard);
+        float hard;
+        law_f32(L, epsp, g, T, yieldf, hard);
+        const float fun = Strialf - g * TwoG32f - yieldf;
+        if (fun < 0.f)
+          gmax = g;
         else
-          law_yield_and_slope(law, lp, heatFrac, epsp, depsp, T, dt, yield, hard);
-        const double fun = Strial - gamma * TwoG32 - yield;
-        if (fun < 0.0)
-          gmax = gamma;
-        else
-          gmin = gamma;
-        const double dfun = TwoG32 + kSqrt23 * hard;
-        double dgamma = fun * __builtin_amdgcn_rcp(dfun);
-        gamma += dgamma;
-        if ((gmax - gamma) * (gamma - gmin) < 0.0) {
-          dgamma = 0.5 * (gmax - gmin);
-          gamma = gmin + dgamma;
+          gmin = g;
+        const float dfun = TwoG32f + s23 * hard;
+        float dg = fun * __builtin_amdgcn_rcpf(dfun);
+        g += dg;
+        if ((gmax - g) * (g - gmin) < 0.f) {
+          dg = 0.5f * (gmax - gmin);
+          g = gmin + dg;
         }
-        if (fabs(dgamma) < kTolNR) break;
-        depsp = kSqrt23 * gamma / dt;
-        epsp = ep0 + kSqrt23 * gamma;
-        T = Tcur + 0.5 * gamma * heatFrac * (kSqrt23 * yield + (double)Snorm0);
+        if (fabsf(dg) < (float)kTolNR) break;
+        epsp = ep0f + s23 * g;
+        T = Tc + 0.5f * g * heatf * (s23 * yieldf + Snorm0);
         if (++iterate > kItMax) {
           set_status(J, DNLG_E_NR_DIVERGED);
           break;
         }
       }
-      const float scale = (float)(1.0 - J->twoG * gamma / (double)Snorm);
+      gamma = (double)g;
+      const float scale = 1.f - TwoGf * g / Snorm;
 #pragma unroll
       for (int c = 0; c < 6; c++) s[c] *= scale;
       J->ep[gi] = ep0 + kSqrt23 * gamma;
       J->dep32[gi] = (float)(kSqrt23 * gamma / dt);
-      J->gam32[gi] = (float)gamma;
-      J->gamc32[gi] += (float)gamma;
-      J->sy32[gi] = (float)yield;
+      J->gam32[gi] = g;
+      J->gamc32[gi] += g;
+      J->sy32[gi] = yieldf;
     }
 #pragma unroll
     for (int c = 0; c < 6; c++) sig[c] = s[c];

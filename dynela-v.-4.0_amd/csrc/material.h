@@ -60,50 +60,77 @@ __device__ __forceinline__ void law_yield_and_slope(int law, const double *p, do
   hard = p[1] * p[2] * ex + p[3];
 }
 
-// Per-IP constants of the radial-return iteration of the mixed path.
-struct ReturnConsts {
-  double lrate;     // ln(sqrt(2/3) / (dt * depsp0)): ln(depsp/depsp0) = ln(gamma) + lrate
-  double inv_dT;    // 1 / (Tm - T0)
-  double gamma_rate;  // depsp > depsp0 <=> gamma > depsp0 dt / sqrt(2/3)
+// FP32 law evaluation of the mixed path's radial return.  Same yield stress
+// and slope as law_yield_and_slope, written in terms of gamma (depsp =
+// sqrt(2/3) gamma / dt, so depsp * dt = sqrt(2/3) gamma), with powers and
+// logarithms from the hardware log2 / exp2 (about 1 ulp).  The FP32 residual
+// of the return equation carries an error of order 1e-7 x the trial stress,
+// i.e. a few 1e-10 in gamma, below the 1e-8 Newton tolerance.
+struct LawF {
+  int law;
+  float p[8];
+  float lrate;       // ln(sqrt(2/3) / (dt depsp0)): ln(depsp / depsp0) = ln(gamma) + lrate
+  float inv_dT;      // 1 / (Tm - T0)
+  float gamma_rate;  // depsp > depsp0  <=>  gamma > depsp0 dt / sqrt(2/3)
+  float eta_rcp;     // eta / (rho Cp)
 };
 
-__device__ __forceinline__ ReturnConsts return_consts(const double *p, double dt) {
-  ReturnConsts c;
-  c.lrate = log(kSqrt23 / (dt * p[5]));
-  c.inv_dT = 1.0 / (p[6] - p[7]);
-  c.gamma_rate = dt * p[5] / kSqrt23;
-  return c;
+__device__ __forceinline__ LawF law_f32_setup(int law, const double *p, double dt, double eta_rcp) {
+  LawF L;
+  L.law = law;
+#pragma unroll
+  for (int k = 0; k < 8; k++) L.p[k] = (float)p[k];
+  L.lrate = 0.f;
+  L.inv_dT = 0.f;
+  L.gamma_rate = 0.f;
+  if (law == DNLG_LAW_JOHNSON_COOK) {
+    L.lrate = (float)log(kSqrt23 / (dt * p[5]));
+    L.inv_dT = (float)(1.0 / (p[6] - p[7]));
+    L.gamma_rate = (float)(dt * p[5] / kSqrt23);
+  }
+  L.eta_rcp = (float)eta_rcp;
+  return L;
 }
 
-// Same yield stress and slope as law_yield_and_slope for the Johnson-Cook
-// law, in terms of gamma (depsp = sqrt(2/3) gamma / dt, so
-// depsp * dt = sqrt(2/3) gamma) with powers as exp/log pairs and the slope's
-// divisions as hardware reciprocals (the slope only sets the Newton step).
-__device__ __forceinline__ void jc_yield_and_slope_fast(const double *p, const ReturnConsts &k, double eta_rcp,
-                                                        double epsp, double gamma, double T, double &yield,
-                                                        double &hard) {
-  const double A = p[0], B = p[1], C = p[2], n = p[3], m = p[4], Tm = p[6], T0 = p[7];
-  const double lg = log(epsp);
-  const double pn = exp(n * lg);
-  const double hard0 = A + B * pn;
-  const bool rate = gamma > k.gamma_rate;  // depsp > depsp0
-  const double visc0 = rate ? 1.0 + C * (log(gamma) + k.lrate) : 1.0;
-  double tp1 = 0.0, temp0 = 1.0;
-  const bool warm = T > T0;
-  const bool soft = warm && T < Tm;
-  if (warm) {
-    if (T < Tm) {
-      tp1 = exp(m * log((T - T0) * k.inv_dT));
-      temp0 = 1.0 - tp1;
-    } else {
-      temp0 = 0.0;
+__device__ __forceinline__ float pow_f32(float x, float e) { return __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(x)); }
+
+__device__ __forceinline__ void law_f32(const LawF &L, float epsp, float gamma, float T, float &yield,
+                                        float &hard) {
+  constexpr float kLn2 = 0.693147180559945309f;
+  constexpr float s23 = (float)kSqrt23;
+  if (L.law == DNLG_LAW_JOHNSON_COOK) {
+    const float A = L.p[0], B = L.p[1], C = L.p[2], n = L.p[3], m = L.p[4], Tm = L.p[6], T0 = L.p[7];
+    const float pn = pow_f32(epsp, n);
+    const float hard0 = A + B * pn;
+    const bool rate = gamma > L.gamma_rate;
+    const float visc0 = rate ? 1.f + C * (__builtin_amdgcn_logf(gamma) * kLn2 + L.lrate) : 1.f;
+    float tp1 = 0.f, temp0 = 1.f;
+    const bool warm = T > T0;
+    const bool soft = warm && T < Tm;
+    if (warm) {
+      if (T < Tm) {
+        tp1 = pow_f32((T - T0) * L.inv_dT, m);
+        temp0 = 1.f - tp1;
+      } else {
+        temp0 = 0.f;
+      }
     }
+    yield = hard0 * visc0 * temp0;
+    float h = n * B * pn * __builtin_amdgcn_rcpf(epsp) * visc0 * temp0;
+    if (rate) h += hard0 * C * temp0 * __builtin_amdgcn_rcpf(s23 * gamma);
+    if (soft) h -= m * L.eta_rcp * hard0 * hard0 * visc0 * visc0 * temp0 * tp1 * __builtin_amdgcn_rcpf(T - T0);
+    hard = h;
+    return;
   }
-  yield = hard0 * visc0 * temp0;
-  double h = n * B * pn * __builtin_amdgcn_rcp(epsp) * visc0 * temp0;
-  if (rate) h += hard0 * C * temp0 * __builtin_amdgcn_rcp(kSqrt23 * gamma);
-  if (soft) h -= m * eta_rcp * hard0 * hard0 * visc0 * visc0 * temp0 * tp1 * __builtin_amdgcn_rcp(T - T0);
-  hard = h;
+  if (L.law == DNLG_LAW_POWER) {
+    const float pn = pow_f32(epsp, L.p[2]);
+    yield = L.p[0] + L.p[1] * pn;
+    hard = L.p[2] * L.p[1] * pn * __builtin_amdgcn_rcpf(epsp);
+    return;
+  }
+  const float ex = __builtin_amdgcn_exp2f(-L.p[2] * epsp * 1.44269504088896341f);
+  yield = L.p[0] + L.p[1] * (1.f - ex) + L.p[3] * epsp;
+  hard = L.p[1] * L.p[2] * ex + L.p[3];
 }
 
 // RampFunction::getValue (RampFunction.C:87-131) with the coefficients of
