@@ -101,7 +101,7 @@ int64_t footprint(int type, int64_t N, int64_t NE, int prec, int track) {
   const int nn = type == DNLG_EL_HEX8N_3D ? 8 : 4;
   const int64_t G = NE * nn;
   int64_t b = 0;
-  b += N * dim * 8 * 8;            // x, disp, fu/fv/fa x2
+  b += N * dim * 8 * 8;            // x, x0, fu/fv/fa x2
   b += N * 8;                      // mass
   b += (N + 1) * 4 * 2 + NE * nn * 4 * 2;  // CSRs, connectivity
   b += N * 4;                      // BC node list (upper bound)
@@ -293,6 +293,9 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
       }
     }
   }
+  int32_t bc_flags = 0;
+  for (int32_t k : bc_idx)
+    if (!bnd.empty() && bnd[k].kind == DNLG_BC_SPEED) bc_flags |= (bnd[k].ramp == DNLG_RAMP_NONE) ? 1 : 3;
   std::vector<int32_t> bcn;  // nodes with constant BCs
   for (int64_t n = 0; n < N; n++)
     if (bc_ptr[n + 1] > bc_ptr[n]) bcn.push_back((int32_t)n);
@@ -349,7 +352,7 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
 #define ALLOC(ptr, count, init)                           \
   if ((st = dalloc(job, &(ptr), (count), (init))) != 0) return bail(st);
   ALLOC(H.x, dim * N, x.data());
-  ALLOC(H.disp, dim * N, (const double *)nullptr);
+  ALLOC(H.x0, dim * N, x.data());
   ALLOC(H.mass, N, (const double *)nullptr);
   for (int k = 0; k < 2; k++) {
     ALLOC(H.fu[k], dim * N, (const double *)nullptr);
@@ -367,6 +370,7 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
   ALLOC(p32, (int64_t)bcn.size(), bcn.data());
   H.bcn = p32;
   H.n_bcn = job->n_bcn;
+  H.bc_flags = bc_flags;
   ALLOC(p32, N + 1, n2e_ptr.data());
   H.n2e_ptr = p32;
   ALLOC(p32, NE * nn, n2e_idx.data());
@@ -472,7 +476,12 @@ int dnlg_job_read_nodes(dnlg_job *job, double *coords, double *disp, double *spe
     return DNLG_OK;
   };
   if ((st = get(coords, job->host.x))) return st;
-  if ((st = get(disp, job->host.disp))) return st;
+  if (disp) {
+    // disp = sum of the increments u = x - x0 (Explicit.C:530-534)
+    std::vector<double> xs(3 * N), x0(3 * N);
+    if ((st = get(xs.data(), job->host.x)) || (st = get(x0.data(), job->host.x0))) return st;
+    for (int64_t k = 0; k < 3 * N; k++) disp[k] = xs[k] - x0[k];
+  }
   if ((st = get(speed, job->host.fv[c.f0]))) return st;
   return DNLG_OK;
 }

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 
 #include "batch.h"
 #include "kernels.h"
@@ -29,17 +30,11 @@ struct Batch {
   TaskList step;        // cycle control + field0 BCs, over the BC node lists
   double *d_snap = nullptr;
   std::vector<double> h_snap;
-  hipGraphExec_t graph = nullptr;       // graph_cycles whole cycles
-  hipGraphExec_t graph_pre = nullptr;   // timed mode: pre phase
-  hipGraphExec_t graph_post = nullptr;  // timed mode: post phase
-  hipGraphExec_t graph_rest = nullptr;  // timed mode: graph_cycles - 1 whole cycles
-  int graph_cycles = 0;
+  // captured graphs by (kind, increment phase), see graph_for
+  std::map<std::pair<int, int>, hipGraphExec_t> graphs;
   std::vector<void *> allocs;
   ~Batch() {
-    if (graph) hipGraphExecDestroy(graph);
-    if (graph_pre) hipGraphExecDestroy(graph_pre);
-    if (graph_post) hipGraphExecDestroy(graph_post);
-    if (graph_rest) hipGraphExecDestroy(graph_rest);
+    for (auto &g : graphs) hipGraphExecDestroy(g.second);
     for (void *p : allocs) hipFree(p);
   }
 };
@@ -176,18 +171,21 @@ int launch_element(dnlg_ctx *ctx, Batch &B, const hipEvent_t *ev) {
   return k;
 }
 
-void launch_post(dnlg_ctx *ctx, Batch &B) {
+// courant = false leaves out the Courant kernel: only for a cycle known not
+// to be a time-step cycle of any running job (see graph_for).
+void launch_post(dnlg_ctx *ctx, Batch &B, bool courant = true) {
   hipStream_t s = ctx->stream;
   for (int d3 = 0; d3 < 2; d3++)
     for (int mx = 0; mx < 2; mx++)
       launch_correct(d3 ? 3 : 2, mx == 1, B.d_jobs, B.node[d3][mx].d, B.node[d3][mx].n, s);
-  for (int hx = 0; hx < 2; hx++) launch_courant(hx ? 3 : 2, B.d_jobs, B.cour[hx].d, B.cour[hx].n, 0, s);
+  if (courant)
+    for (int hx = 0; hx < 2; hx++) launch_courant(hx ? 3 : 2, B.d_jobs, B.cour[hx].d, B.cour[hx].n, 0, s);
 }
 
-void launch_cycle(dnlg_ctx *ctx, Batch &B) {
+void launch_cycle(dnlg_ctx *ctx, Batch &B, bool courant = true) {
   launch_pre(ctx, B);
   launch_element(ctx, B, nullptr);
-  launch_post(ctx, B);
+  launch_post(ctx, B, courant);
 }
 
 // Explicit.C:143-178: reference Jacobian (detJ0), lumped mass, forced time
@@ -235,24 +233,48 @@ int capture(dnlg_ctx *ctx, F &&f, hipGraphExec_t *out) {
   return 0;
 }
 
-// Untimed: `cycles` whole cycles in one graph.  Timed: one sampled cycle
-// (the pre and post phases as two graphs, the element kernels launched
-// between them with start/stop events) followed by `cycles` - 1 whole
-// cycles in one graph, so the timing costs one cycle in `cycles`.
-int capture_graphs(dnlg_ctx *ctx, Batch &B, int cycles, bool timed) {
-  int st;
-  if (timed) {
-    if ((st = capture(ctx, [&] { launch_pre(ctx, B); }, &B.graph_pre))) return st;
-    if ((st = capture(ctx, [&] { launch_post(ctx, B); }, &B.graph_post))) return st;
-    if (cycles > 1 &&
-        (st = capture(ctx, [&] { for (int k = 0; k < cycles - 1; k++) launch_cycle(ctx, B); }, &B.graph_rest)))
-      return st;
-    B.graph_cycles = cycles;
-    return 0;
+// Graph kinds: whole cycles, and for the timed mode the pre and post phases
+// of one sampled cycle (the element kernels run between them, bracketed by
+// events) plus the remaining cycles of a unit.
+enum GraphKind { kWhole = 0, kPre = 1, kPost = 2, kRest = 3 };
+
+// Cycles per graph: with phase < 0 (running jobs at different increments
+// modulo the time-step frequency) `cycles` cycles with the Courant kernel in
+// every cycle (it returns at once on the other cycles); with a common phase
+// p = increment mod freq, a multiple of freq cycles with the Courant kernel
+// only in the cycles whose increment is a multiple of freq, so the phase is
+// the same at the end of the graph.
+int graph_len(int cycles, int phase, int freq) {
+  if (phase < 0) return cycles;
+  return freq * std::max(1, (cycles + freq - 1) / freq);
+}
+
+hipGraphExec_t graph_for(dnlg_ctx *ctx, Batch &B, int kind, int phase, int freq, int &st) {
+  const auto key = std::make_pair(kind * 65536 + (phase < 0 ? 0 : freq), phase);
+  auto it = B.graphs.find(key);
+  if (it != B.graphs.end()) return it->second;
+  const int len = graph_len(ctx->graph_cycles, phase, freq);
+  // cycle j (0-based) of a graph starting at phase p has increment
+  // p + j + 1 (mod freq)
+  auto due = [&](int j) { return phase < 0 || (phase + j + 1) % freq == 0; };
+  hipGraphExec_t g = nullptr;
+  switch (kind) {
+    case kWhole:
+      st = capture(ctx, [&] { for (int j = 0; j < len; j++) launch_cycle(ctx, B, due(j)); }, &g);
+      break;
+    case kPre:
+      st = capture(ctx, [&] { launch_pre(ctx, B); }, &g);
+      break;
+    case kPost:
+      st = capture(ctx, [&] { launch_post(ctx, B, true); }, &g);
+      break;
+    default:  // the len - 1 cycles after a sampled one
+      st = capture(ctx, [&] { for (int j = 1; j < len; j++) launch_cycle(ctx, B, due(j)); }, &g);
+      break;
   }
-  if ((st = capture(ctx, [&] { for (int k = 0; k < cycles; k++) launch_cycle(ctx, B); }, &B.graph))) return st;
-  B.graph_cycles = cycles;
-  return 0;
+  if (st) return nullptr;
+  B.graphs[key] = g;
+  return g;
 }
 
 }  // namespace
@@ -291,12 +313,6 @@ int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vec
   launch_set_up_to(B->d_jobs, n, B->d_snap, s);
   const bool use_graph = ctx->graph_cycles > 0;
   const bool timed = ctx->timing;
-  if (use_graph && !(timed ? B->graph_pre : B->graph)) {
-    if ((st = capture_graphs(ctx, *B, ctx->graph_cycles, timed))) {
-      ctx->err = "graph capture failed";
-      return st;
-    }
-  }
   constexpr int kTimedChunk = 64;  // sampled cycles between event read-backs
   std::vector<int> ev_used;        // events used per timed cycle
   auto collect = [&]() -> int {
@@ -345,6 +361,7 @@ int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vec
     }
     if ((st = collect())) return st;
     double need = 0.0;
+    int phase = -2, freq = 0;  // common increment phase of the running jobs, -1 if none
     for (int i = 0; i < n; i++) {
       const double t = B->h_snap[5 * i], dt = B->h_snap[5 * i + 1], up = B->h_snap[5 * i + 2];
       const int js = (int)B->h_snap[5 * i + 3];
@@ -353,17 +370,43 @@ int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vec
       jobs[i]->up_to = up;
       if (js != 0 || !(t < up)) continue;
       need = std::max(need, dt > 0 ? std::ceil((up - t) / dt) : 1.0);
+      const int f = jobs[i]->host.dt_freq;
+      const int p = (int)((int64_t)B->h_snap[5 * i + 4] % f);
+      if (phase == -2) {
+        phase = p;
+        freq = f;
+      } else if (phase != p || freq != f) {
+        phase = -1;
+      }
     }
     if (need <= 0.0) break;
+    if (phase < 0) {
+      phase = -1;
+      freq = 1;
+    }
     if (max_cycles >= 0) {
       if (launched >= max_cycles) break;
       need = std::min(need, (double)(max_cycles - launched));
     }
     const int chunk = (int)std::min(need, 4096.0);
     launched += chunk;
+    hipGraphExec_t g_whole = nullptr, g_pre = nullptr, g_post = nullptr, g_rest = nullptr;
+    if (use_graph) {
+      if (timed) {
+        g_pre = graph_for(ctx, *B, kPre, -1, 1, st);
+        if (!st) g_post = graph_for(ctx, *B, kPost, -1, 1, st);
+        if (!st && graph_len(ctx->graph_cycles, phase, freq) > 1) g_rest = graph_for(ctx, *B, kRest, phase, freq, st);
+      } else {
+        g_whole = graph_for(ctx, *B, kWhole, phase, freq, st);
+      }
+      if (st) {
+        ctx->err = "graph capture failed";
+        return st;
+      }
+    }
     if (timed) {
-      // units of one sampled cycle + (graph_cycles - 1) graph cycles
-      const int per = use_graph ? std::max(1, B->graph_cycles) : 1;
+      // units of one sampled cycle + the rest of a graph's cycles
+      const int per = use_graph ? graph_len(ctx->graph_cycles, phase, freq) : 1;
       const int units = std::min((chunk + per - 1) / per, kTimedChunk);
       launched += (int64_t)units * per - chunk;
       while (ctx->events.size() < (size_t)8 * units) {
@@ -374,7 +417,7 @@ int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vec
       size_t base = 0;
       for (int k = 0; k < units; k++) {
         if (use_graph) {
-          if (hipGraphLaunch(B->graph_pre, s) != hipSuccess) return DNLG_E_HIP;
+          if (hipGraphLaunch(g_pre, s) != hipSuccess) return DNLG_E_HIP;
         } else {
           launch_pre(ctx, *B);
         }
@@ -382,16 +425,18 @@ int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vec
         ev_used.push_back(used);
         base += 2 * used;
         if (use_graph) {
-          if (hipGraphLaunch(B->graph_post, s) != hipSuccess) return DNLG_E_HIP;
-          if (B->graph_rest && hipGraphLaunch(B->graph_rest, s) != hipSuccess) return DNLG_E_HIP;
+          if (hipGraphLaunch(g_post, s) != hipSuccess) return DNLG_E_HIP;
+          if (g_rest && hipGraphLaunch(g_rest, s) != hipSuccess) return DNLG_E_HIP;
         } else {
           launch_post(ctx, *B);
         }
       }
     } else if (use_graph) {
-      const int reps = (chunk + B->graph_cycles - 1) / B->graph_cycles;
+      const int len = graph_len(ctx->graph_cycles, phase, freq);
+      const int reps = (chunk + len - 1) / len;
+      launched += (int64_t)reps * len - chunk;
       for (int r = 0; r < reps; r++)
-        if (hipGraphLaunch(B->graph, s) != hipSuccess) {
+        if (hipGraphLaunch(g_whole, s) != hipSuccess) {
           ctx->err = "graph launch failed";
           return DNLG_E_HIP;
         }

@@ -65,13 +65,14 @@ struct JobDev {
   // Chung-Hulbert (Explicit.C:86-104) and time-step settings
   double alphaM, beta, gammaCH, omegaS, safety;
   // nodal state, [dim][N]
-  double *x, *disp, *mass;
+  double *x, *x0, *mass;  // current and initial coordinates, lumped mass
   double *fu[2], *fv[2], *fa[2];
   const int32_t *bc_ptr;  // [N+1] constant-BC list per node
   const int32_t *bc_idx;
   const Boundary *bnd;
   const int32_t *bcn;     // nodes that carry constant BCs, ascending
   int32_t n_bcn;
+  int32_t bc_flags;       // 1: a constant speed BC, 2: a ramped one
   const int32_t *n2e_ptr;  // [N+1] node -> (element*nn + local node), element order
   const int32_t *n2e_idx;
   // elements

@@ -384,8 +384,9 @@ __device__ __forceinline__ void apply_bcs(const JobDev *J, int64_t n, double tim
 // in registers (they are read nowhere else), the element forces of the node
 // are gathered in element order (same summation order as the reference's
 // scatter-add of Model.C:864-899), a = f/M, Chung-Hulbert correction, BCs,
-// displacement and coordinates.  The field0 BC values that the element pass
-// reads were applied by k_cycle_step.
+// coordinates (the node displacement of Explicit.C:530-531 is x - x0, formed
+// at read-back).  The field0 BC values that the element pass reads were
+// applied by k_cycle_step.
 template <int DIM, typename FT>
 __global__ __launch_bounds__(256) void k_correct(JobDev *const *jobs, const Task *tasks) {
   const Task tk = tasks[blockIdx.x];
@@ -442,7 +443,6 @@ __global__ __launch_bounds__(256) void k_correct(JobDev *const *jobs, const Task
     J->fu[b][d * N + n] = ub[d];
     J->fv[b][d * N + n] = vb[d];
     J->fa[b][d * N + n] = ab[d];
-    J->disp[d * N + n] += ub[d];
     const double xn = J->x[d * N + n] + ub[d];
     J->x[d * N + n] = xn;
     finite = finite && isfinite(xn) && isfinite(vb[d]);
@@ -594,6 +594,8 @@ __global__ __launch_bounds__(256) void k_cycle_step(JobDev *const *jobs, const T
   const Task tk = tasks[blockIdx.x];
   JobDev *J = jobs[tk.job];
   double t = J->c.t, dt = J->c.dt;
+  const double dt_last = dt;
+  const bool resumed = !J->c.pending;  // first cycle of an advance
   const double up_to = J->c.up_to;
   int64_t inc = J->c.inc;
   int f0 = J->c.f0;
@@ -627,8 +629,12 @@ __global__ __launch_bounds__(256) void k_cycle_step(JobDev *const *jobs, const T
       // densities are consumed by the Courant step and read back at the end
       density_due = (courant_due || !(t + dt < up_to)) ? 1 : 0;
       pending = 1;
+      // After the corrector, field0 already holds the BC values of this
+      // cycle unless a speed BC depends on a changed step or on time; a
+      // restraint zeroes the same components either way.
+      const bool bc_work = resumed || (J->bc_flags & 2) || ((J->bc_flags & 1) && dt != dt_last);
       const int64_t k = tk.first + threadIdx.x;
-      if (k < tk.end) {
+      if (bc_work && k < tk.end) {
         const int64_t n = J->bcn[k];
         if (J->dim == 3)
           bc_field0<3>(J, n, f0, t, dt);
@@ -643,7 +649,7 @@ __global__ __launch_bounds__(256) void k_cycle_step(JobDev *const *jobs, const T
   if (threadIdx.x == 0) {
     const unsigned nb = J->n_bcn > 0 ? (unsigned)((J->n_bcn + 255) / 256) : 1u;
     __threadfence();
-    if (atomicAdd(&J->c.arrive, 1u) == nb - 1) {
+    if (nb == 1 || atomicAdd(&J->c.arrive, 1u) == nb - 1) {
       Control &c = J->c;
       c.t = t;
       c.dt = dt;
