@@ -57,13 +57,17 @@ def algo_bytes_per_elem(prec, nodes_per_elem):
     return 8 * ip + 8 * 4 + force + nodes_per_elem * 6 * 8
 
 
-def make_model(prec, ctx, n=20, nz=100, t_end=80e-6):
+def make_model(prec, ctx, n=20, nz=100, t_end=80e-6, workload="taylor3d", pull=1.0):
     from dnlPython import samples
+    if workload == "necking":
+        return samples.necking_3d(n=n, nz=nz, t_end=t_end, pull=pull, precision=prec, track_tensors=False,
+                                  context=ctx)
     return samples.taylor_3d(n=n, nz=nz, t_end=t_end, precision=prec, track_tensors=False, context=ctx)
 
 
-def run_precision(prec, ctx, steps, warmup, n, nz, t_end):
-    models = [make_model(prec, ctx, n, nz, t_end) for _ in range(steps + warmup)]
+def run_precision(prec, ctx, steps, warmup, n, nz, t_end, workload="taylor3d", pull=1.0):
+    import dnlPython as dnl
+    models = [make_model(prec, ctx, n, nz, t_end, workload, pull) for _ in range(steps + warmup)]
     for m in models:
         m.initSolve()
     for m in models[:warmup]:
@@ -81,9 +85,13 @@ def run_precision(prec, ctx, steps, warmup, n, nz, t_end):
     ne = models[0].getElementsNumber()
     nn = models[0].getNodesNumber()
     cycles = [m.getIncrement() for m in models[warmup:]]
-    disp = models[-1].getNodalDisplacements()
+    last = models[-1]
+    disp = last.getNodalDisplacements()
+    ep = last.getIntPointField(dnl.Field.plasticStrain)
+    energies = last.getEnergies()       # kinetic, internal, plastic work, kinetic + internal
+    mass = last.getNodalMasses()
     out = dict(wall=wall, cycles=cycles, ne=ne, nn=nn, elem_ms=el_ms, elem_launches=el_n,
-               value=ne * sum(cycles) / wall, disp=disp)
+               value=ne * sum(cycles) / wall, disp=disp, ep=ep, energies=energies, mass_total=float(mass.sum()))
     for m in models:
         m.close()
     return out
@@ -215,11 +223,10 @@ def run_serve(prec, ctx, n_jobs, util, rate, cycles_per_step, seed=20261203):
     return res
 
 
-def cpu_baseline(n, nz, sample_t=1e-6):
-    from dnlPython import samples
+def cpu_baseline(n, nz, sample_t=1e-6, workload="taylor3d", pull=1.0):
     from oracle.bridge import OracleJob
     threads = min(16, os.cpu_count() or 1)
-    m = samples.taylor_3d(n=n, nz=nz, t_end=80e-6, precision="fp64", track_tensors=False)
+    m = make_model("fp64", None, n, nz, 80e-6, workload, pull)
     job = OracleJob(m.descriptors(), threads=threads)
     t0 = time.perf_counter()
     job.solve(sample_t)
@@ -254,7 +261,8 @@ def main():
     ap.add_argument("--nz", type=int, default=100)
     ap.add_argument("--t-end", type=float, default=80e-6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="taylor3d", choices=["taylor3d", "sweep", "serve"])
+    ap.add_argument("--workload", default="taylor3d", choices=["taylor3d", "necking", "sweep", "serve"])
+    ap.add_argument("--pull", type=float, default=1.0, help="necking: pulled-end speed (m/s)")
     ap.add_argument("--jobs", type=int, default=256, help="sweep / serve workloads: number of jobs")
     ap.add_argument("--util", type=float, default=0.7, help="serve: offered load for the arrival rate")
     ap.add_argument("--rate", type=float, default=None, help="serve: arrival rate in jobs/s (overrides --util)")
@@ -318,7 +326,8 @@ def main():
     res = {}
     for p in precs:
         print(f"[bench] precision={p} steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
-        res[p] = run_precision(p, ctx, args.steps, args.warmup, args.n, args.nz, args.t_end)
+        res[p] = run_precision(p, ctx, args.steps, args.warmup, args.n, args.nz, args.t_end, args.workload,
+                               args.pull)
         r = res[p]
         print(f"[bench] {p}: {r['value']:.4g} elem-cycles/s, cycles={r['cycles']}, wall={r['wall']:.3f}s",
               file=sys.stderr, flush=True)
@@ -341,9 +350,12 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32+fp64" if head == "mixed" else "f64",
         "data": "synthetic",
-        "config": {"workload": f"Taylor impact 3D, {args.n}x{args.n}x{args.nz} hex8 "
-                               f"({r['ne']} elements, {r['nn']} nodes), Johnson-Cook copper, 227 m/s, "
-                               f"{args.t_end * 1e6:g} us",
+        "config": {"workload": (f"Taylor impact 3D, {args.n}x{args.n}x{args.nz} hex8 "
+                                f"({r['ne']} elements, {r['nn']} nodes), Johnson-Cook copper, 227 m/s, "
+                                f"{args.t_end * 1e6:g} us") if args.workload == "taylor3d" else
+                               (f"3D round bar necking, O-grid {args.n} x {args.nz} layers hex8 "
+                                f"({r['ne']} elements, {r['nn']} nodes), Voce + linear law, 2 % taper, "
+                                f"pulled at {args.pull:g} m/s, {args.t_end * 1e6:g} us"),
                    "precision": head, "cycles_per_step": r["cycles"][0]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
@@ -358,9 +370,21 @@ def main():
         dmx = res["mixed"]["disp"]
         err = np.linalg.norm(dmx - d64, axis=0).max() / np.linalg.norm(d64, axis=0).max()
         line["mixed_disp_err_vs_fp64"] = float(err)
+        e64, emx = res["fp64"]["ep"], res["mixed"]["ep"]
+        line["mixed_ep_err_vs_fp64"] = float(np.abs(emx - e64).max() / max(np.abs(e64).max(), 1e-300))
+        line["max_plastic_strain"] = float(np.abs(e64).max())
+    if args.workload == "taylor3d":
+        # closed system (rigid wall does no work): KE + IE against the initial
+        # kinetic energy, 1/2 M v0^2; the difference is the Chung-Hulbert
+        # numerical dissipation plus arithmetic error
+        bal = {}
+        for p in precs:
+            ke0 = 0.5 * res[p]["mass_total"] * 227.0 ** 2
+            bal[p] = float(abs(res[p]["energies"][3] - ke0) / ke0)
+        line["energy_balance_err"] = bal
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (oracle)", file=sys.stderr, flush=True)
-        cb = cpu_baseline(args.n, args.nz)
+        cb = cpu_baseline(args.n, args.nz, workload=args.workload, pull=args.pull)
         line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
     print(json.dumps(line), flush=True)
 

@@ -12,8 +12,8 @@
 //   R = F U^-1 = (I + H) U^-1,
 // evaluated with powers of X up to X^6 (X^4 when every lane of the wave has
 // ||X||_F <= 1e-2, X^3 below 1e-3); the truncation stays under 2e-9
-// relative, i.e. under FP32 rounding.  Larger increments (> 0.05) take the
-// FP64 eigen-decomposition path (polar_lnU_fp64).  This avoids the
+// relative, i.e. under FP32 rounding.  Larger increments (> 0.05) take an
+// FP64 eigen-decomposition path (polar_lnU_jacobi).  This avoids the
 // catastrophic cancellation of forming F^T F - I and ln(lambda ~ 1) in FP32.
 //
 // Element geometry uses the tensor-product structure of the 8-node hex
@@ -41,12 +41,12 @@ __device__ __forceinline__ void sym_mul(const float A[6], const float B[6], floa
   C[5] = A[2] * B[2] + A[4] * B[4] + A[5] * B[5];
 }
 
-// Large increments: FP64 eigen path (rarely taken).
+// Large increments: FP64 Jacobi eigen path (rarely taken).
 __device__ __forceinline__ void polar_lnU_large(const float *H, float *dE, float *R) {
   double F[9], e64[6], r64[9];
 #pragma unroll
   for (int k = 0; k < 9; k++) F[k] = (double)H[k] + ((k % 4 == 0) ? 1.0 : 0.0);
-  polar_lnU_fp64(F, e64, r64);
+  polar_lnU_jacobi(F, e64, r64);
 #pragma unroll
   for (int c = 0; c < 6; c++) dE[c] = (float)e64[c];
 #pragma unroll

@@ -343,9 +343,58 @@ __device__ __forceinline__ void eig_sym3(const double A[3][3], double Q[3][3], d
     }
 }
 
-// F (row-major 9) -> dE = ln U (sym 6), R = F U^-1 (row-major 9).
-__device__ __forceinline__ void polar_lnU_fp64(const double F[9], double dE[6], double R[9]) {
-  double C[3][3];
+// Cyclic Jacobi eigen-decomposition (Golub & Van Loan, Matrix Computations,
+// 4th ed., Algorithms 8.5.1-8.5.3).  Not part of the reference: taken only
+// where eig_sym3 (like the reference's dsyevd3) returns non-finite or
+// non-orthonormal eigenvectors, which happens for nearly repeated
+// eigenvalue pairs (e.g. the equal lateral stretches of a uniaxial wave).
+// Columns of Q are eigenvectors.
+__device__ __forceinline__ void jacobi_eig3(const double A[3][3], double Q[3][3], double w[3]) {
+  double a[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      a[i][j] = A[i][j];
+      Q[i][j] = (i == j) ? 1.0 : 0.0;
+    }
+  for (int sweep = 0; sweep < 32; sweep++) {
+    const double off = sqd(a[0][1]) + sqd(a[0][2]) + sqd(a[1][2]);
+    const double nrm = sqd(a[0][0]) + sqd(a[1][1]) + sqd(a[2][2]) + 2.0 * off;
+    if (off <= 1e-32 * nrm) break;
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      const int p = (r == 2) ? 1 : 0, q = (r == 0) ? 1 : 2;
+      if (a[p][q] == 0.0) continue;
+      const double tau = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+      const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+      const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const double akp = a[k][p], akq = a[k][q];
+        a[k][p] = c * akp - s * akq;
+        a[k][q] = s * akp + c * akq;
+      }
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const double apk = a[p][k], aqk = a[q][k];
+        a[p][k] = c * apk - s * aqk;
+        a[q][k] = s * apk + c * aqk;
+      }
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const double vkp = Q[k][p], vkq = Q[k][q];
+        Q[k][p] = c * vkp - s * vkq;
+        Q[k][q] = s * vkp + c * vkq;
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 3; i++) w[i] = a[i][i];
+}
+
+// F^T F of F (row-major 9), as Tensor2::buildFTF (Tensor2.C:1315-1327).
+__device__ __forceinline__ void build_ftf(const double F[9], double C[3][3]) {
   C[0][0] = sqd(F[0]) + sqd(F[3]) + sqd(F[6]);
   C[0][1] = F[0] * F[1] + F[3] * F[4] + F[6] * F[7];
   C[0][2] = F[0] * F[2] + F[3] * F[5] + F[6] * F[8];
@@ -355,8 +404,12 @@ __device__ __forceinline__ void polar_lnU_fp64(const double F[9], double dE[6], 
   C[1][0] = C[0][1];
   C[2][0] = C[0][2];
   C[2][1] = C[1][2];
-  double V[3][3], lam[3];
-  eig_sym3(C, V, lam);
+}
+
+// Tensor2::polarExtractLnU (Tensor2.C:1237-1312): from the eigen pairs of
+// F^T F, dE = 1/2 sum ln(lam_k) v_k v_k^T and R = F U^-1.
+__device__ __forceinline__ void polar_extract(const double F[9], const double V[3][3], const double lam[3],
+                                              double dE[6], double R[9]) {
   double D[3][6];
 #pragma unroll
   for (int k = 0; k < 3; k++) {
@@ -391,6 +444,42 @@ __device__ __forceinline__ void polar_lnU_fp64(const double F[9], double dE[6], 
   const double l0 = log(lam[0]) * 0.5, l1 = log(lam[1]) * 0.5, l2 = log(lam[2]) * 0.5;
 #pragma unroll
   for (int c = 0; c < 6; c++) dE[c] = l0 * D[0][c] + l1 * D[1][c] + l2 * D[2][c];
+}
+
+// Tensor2::polarCuppenLnU (Tensor2.C:1442-1457) on F (row-major 9):
+// dE = ln U (sym 6), R = F U^-1 (row-major 9).  Where the Kopp solver
+// returns non-finite or non-orthonormal eigenvectors the Jacobi
+// decomposition replaces it.
+__device__ __forceinline__ void polar_lnU_fp64(const double F[9], double dE[6], double R[9]) {
+  double C[3][3];
+  build_ftf(F, C);
+  double V[3][3], lam[3];
+  eig_sym3(C, V, lam);
+  // usable: finite, eigenvector columns orthonormal to 64 eps (the Cuppen
+  // step loses orthogonality for nearly repeated eigenvalues, which makes
+  // U and R wrong even for tiny strains)
+  bool ok = true;
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    ok = ok && isfinite(lam[i]);
+#pragma unroll
+    for (int j = i; j < 3; j++) {
+      const double d = V[0][i] * V[0][j] + V[1][i] * V[1][j] + V[2][i] * V[2][j] - (i == j ? 1.0 : 0.0);
+      ok = ok && (fabs(d) <= 64.0 * kEps);
+    }
+  }
+  if (!ok) jacobi_eig3(C, V, lam);
+  polar_extract(F, V, lam, dE, R);
+}
+
+// FP64 polar decomposition through the Jacobi solver alone (no reference
+// arithmetic to follow): the large-increment branch of the mixed path.
+__device__ __forceinline__ void polar_lnU_jacobi(const double F[9], double dE[6], double R[9]) {
+  double C[3][3];
+  build_ftf(F, C);
+  double V[3][3], lam[3];
+  jacobi_eig3(C, V, lam);
+  polar_extract(F, V, lam, dE, R);
 }
 
 // R . S . R^T for symmetric S (SymTensor2::dotRxRT, SymTensor2.C:512-525).

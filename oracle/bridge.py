@@ -44,6 +44,7 @@ def load():
         lib.oracle_job_status.argtypes = [_vp]
         lib.oracle_eig3.argtypes = [_dp, _dp, _dp]
         lib.oracle_polar_lnU.argtypes = [_dp, _dp, _dp]
+        lib.oracle_jacobi_eig3.argtypes = [_dp, _dp, _dp]
         lib.oracle_rotate_sym.argtypes = [_dp, _dp]
         lib.oracle_law_yield.restype = ctypes.c_double
         lib.oracle_law_yield.argtypes = [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double]

@@ -25,6 +25,12 @@
  *    with f = 0; elastic simple shear vs the Green-Naghdi closed form.
  *  Parity unpinned: the CBLAS dgemm summation order behind dN/dx
  *  (Matrix.C:815) -- a plain k-ordered sum is used.
+ *
+ * Deliberate extension: where kopp_eig3 (like the reference's dsyevd3)
+ * returns non-finite eigenvectors, or eigenvectors that are not orthonormal
+ * to 64 eps -- nearly repeated eigenvalue pairs -- polar_lnU falls back to a
+ * cyclic Jacobi decomposition instead of propagating NaN or a wrong U and R;
+ * the engine does the same.
  */
 #include "../include/dnlg.h"
 
@@ -346,6 +352,73 @@ void kopp_eig3(const double A[3][3], double Q[3][3], double w[3]) {
 }
 
 /* ---------------------------------------------------------------------------
+ * Cyclic Jacobi eigen-decomposition of a symmetric 3x3 matrix (Golub & Van
+ * Loan, Matrix Computations, 4th ed., Algorithms 8.5.1-8.5.3): rotations in
+ * the (0,1), (0,2), (1,2) planes until the off-diagonal mass is below
+ * 1e-32 of the Frobenius mass.  Not part of the reference: used only where
+ * kopp_eig3 (like the reference's dsyevd3) returns non-finite or
+ * non-orthonormal eigenvectors, which happens for nearly repeated
+ * eigenvalue pairs (e.g. the equal lateral stretches of a uniaxial wave).
+ * Q holds the eigenvectors as columns.
+ * ------------------------------------------------------------------------- */
+void jacobi_eig3(const double A[3][3], double Q[3][3], double w[3]) {
+  double a[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      a[i][j] = A[i][j];
+      Q[i][j] = (i == j) ? 1.0 : 0.0;
+    }
+  const int P[3] = {0, 0, 1}, Qi[3] = {1, 2, 2};
+  for (int sweep = 0; sweep < 32; sweep++) {
+    const double off = sq(a[0][1]) + sq(a[0][2]) + sq(a[1][2]);
+    const double nrm = sq(a[0][0]) + sq(a[1][1]) + sq(a[2][2]) + 2.0 * off;
+    if (off <= 1e-32 * nrm) break;
+    for (int r = 0; r < 3; r++) {
+      const int p = P[r], q = Qi[r];
+      if (a[p][q] == 0.0) continue;
+      const double tau = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+      const double t = (tau >= 0.0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+      const double c = 1.0 / std::sqrt(1.0 + t * t), s = t * c;
+      for (int k = 0; k < 3; k++) {
+        const double akp = a[k][p], akq = a[k][q];
+        a[k][p] = c * akp - s * akq;
+        a[k][q] = s * akp + c * akq;
+      }
+      for (int k = 0; k < 3; k++) {
+        const double apk = a[p][k], aqk = a[q][k];
+        a[p][k] = c * apk - s * aqk;
+        a[q][k] = s * apk + c * aqk;
+      }
+      for (int k = 0; k < 3; k++) {
+        const double vkp = Q[k][p], vkq = Q[k][q];
+        Q[k][p] = c * vkp - s * vkq;
+        Q[k][q] = s * vkp + c * vkq;
+      }
+    }
+  }
+  for (int i = 0; i < 3; i++) w[i] = a[i][i];
+}
+
+/* Eigen pairs usable for the polar decomposition: finite, and eigenvector
+ * columns orthonormal to 64 eps (the Cuppen step loses orthogonality for
+ * nearly repeated eigenvalues, which turns U = sum g_k v_k v_k^T and
+ * R = F U^-1 into wrong tensors even for tiny strains). */
+bool usable_eigen(const double Q[3][3], const double w[3]) {
+  bool ok = true;
+  for (int i = 0; i < 3; i++) {
+    ok = ok && std::isfinite(w[i]);
+    for (int j = 0; j < 3; j++) ok = ok && std::isfinite(Q[i][j]);
+  }
+  if (!ok) return false;
+  for (int i = 0; i < 3; i++)
+    for (int j = i; j < 3; j++) {
+      const double d = Q[0][i] * Q[0][j] + Q[1][i] * Q[1][j] + Q[2][i] * Q[2][j] - (i == j ? 1.0 : 0.0);
+      if (!(std::fabs(d) <= 64.0 * kEps)) return false;
+    }
+  return true;
+}
+
+/* ---------------------------------------------------------------------------
  * Polar decomposition F = R U and logarithmic strain increment ln U:
  * Tensor2::buildFTF (Tensor2.C:1315-1327), Tensor2::polarExtractLnU
  * (Tensor2.C:1237-1312), Tensor2::polarCuppenLnU (Tensor2.C:1442-1457).
@@ -364,6 +437,7 @@ void polar_lnU(const double F[9], double dE[6], double R[9]) {
   C[2][2] = sq(F[2]) + sq(F[5]) + sq(F[8]);
   double V[3][3], lam[3];
   kopp_eig3(C, V, lam);
+  if (!usable_eigen(V, lam)) jacobi_eig3(C, V, lam);
   /* dyads v_k (x) v_k, symmetric storage */
   double D[3][6];
   for (int k = 0; k < 3; k++) {
@@ -1376,6 +1450,13 @@ void oracle_eig3(const double A[9], double Q[9], double w[3]) {
 }
 
 void oracle_polar_lnU(const double F[9], double dE[6], double R[9]) { polar_lnU(F, dE, R); }
+
+void oracle_jacobi_eig3(const double A[9], double Q[9], double w[3]) {
+  double a[3][3], q[3][3];
+  for (int i = 0; i < 9; i++) a[i / 3][i % 3] = A[i];
+  jacobi_eig3(a, q, w);
+  for (int i = 0; i < 9; i++) Q[i] = q[i / 3][i % 3];
+}
 
 void oracle_rotate_sym(const double R[9], double S[6]) { rotate_sym(R, S); }
 

@@ -120,6 +120,113 @@ def test_polar_against_scipy():
         np.testing.assert_allclose(sym(dE), logU, atol=tol)
 
 
+# Deformation gradients with a nearly repeated pair of principal stretches
+# (equal lateral stretches of a uniaxial wave, tiny shears) on which the
+# Kopp / Cuppen solver returns NaN eigenvectors; found by brute force over
+# near-identity F with random sparsity and exponents.
+DEGENERATE_F = [
+    [0.9999977554099807, -6.0866254721819887e-17, -2.5673284452683152e-11, -3.8754985562266079e-18,
+     0.99928435481611444, -2.0196019442639356e-21, 5.3641098223804799e-20, 2.0071721319477389e-10,
+     0.9999977554099807],
+    [1, -5.197663647861295e-15, 0, 1.5588983398066094e-08, 1.0000000000000004, -9.724847248502339e-14, 0, 0,
+     1.0000144463341012],
+    [1.0000000000000002, 0, 1.6394305225272209e-13, 0, 0.99999375465211049, -2.6574145181028933e-13,
+     3.0252562602304473e-12, 0, 1.0000000000000002],
+    [0.99999999998633027, 0, -1.8783223923328994e-16, -1.2823616560458548e-09, 0.99999999998633027, 0, 0,
+     -6.7279513642765257e-13, 0.99992631291167289],
+    [0.9999999996621316, 0, -6.1559412697832952e-08, 0, 0.99993721886807174, 1.3501324046147427e-12,
+     -2.9045539929949685e-12, 0, 0.9999999996621316],
+]
+
+
+# Near-uniaxial F on which the Kopp / Cuppen solver returns finite but
+# non-orthonormal eigenvectors (orthogonality error 1e-6 .. 8e-2).
+NONORTHO_F = [
+    [0.9999979082640562, 0.0, 0.0, -5.150127289795424e-14, 1.000006275207761, 0.0, 0.0, 7.600647450335941e-14,
+     0.9999979082640562],
+    [0.9999998171516018, 0.0, -5.793171832096426e-15, 1.3557360005053231e-15, 1.00000054854519,
+     2.482959666128738e-15, 0.0, 0.0, 0.999999817151603],
+    [0.9996046643630397, -4.0170422591200285e-09, 0.0, 0.0, 1.0011860025955537, 2.9862197451576665e-09, 0.0, 0.0,
+     0.9996046658014821],
+    [0.999999998796853, 1.875980360862417e-15, 2.3646520162924696e-15, 3.267901182264114e-15,
+     1.0000000036094319, 3.37880430528116e-15, 0.0, 0.0, 0.9999999987968561],
+]
+
+
+def _ftf(F):
+    """F^T F with the summation order of Tensor2::buildFTF (no fused ops)."""
+    C = np.zeros((3, 3))
+    for i in range(3):
+        for j in range(3):
+            C[i, j] = F[0, i] * F[0, j] + F[1, i] * F[1, j] + F[2, i] * F[2, j]
+    return C
+
+
+@pytest.mark.skipif(not os.path.exists(bridge.REF_EIGEN), reason="reference eigensolver not built (needs the reference tree)")
+def test_reference_dsyevd3_fails_on_nearly_repeated_pairs():
+    """The reference's own dsyevd3 (Eigen3x3.C) returns non-finite
+    eigenvectors on these inputs, bit-equal to the oracle's restatement."""
+    ref = ctypes.CDLL(bridge.REF_EIGEN)
+    ref.ref_dsyevd3.argtypes = [_dp, _dp, _dp]
+    for F in DEGENERATE_F:
+        a = np.ascontiguousarray(_ftf(np.array(F).reshape(3, 3))).ravel()
+        q1, w1 = eig_oracle(a)
+        q2, w2 = np.zeros(9), np.zeros(3)
+        ref.ref_dsyevd3(P(a), P(q2), P(w2))
+        assert not np.isfinite(q2).all()
+        assert np.array_equal(q1.ravel().view(np.int64), q2.view(np.int64))
+
+
+@pytest.mark.skipif(not os.path.exists(bridge.REF_EIGEN), reason="reference eigensolver not built (needs the reference tree)")
+def test_reference_dsyevd3_loses_orthogonality_on_near_uniaxial_stretch():
+    """The reference's dsyevd3 returns finite eigenvectors that are far from
+    orthonormal on these inputs, bit-equal to the oracle's restatement."""
+    ref = ctypes.CDLL(bridge.REF_EIGEN)
+    ref.ref_dsyevd3.argtypes = [_dp, _dp, _dp]
+    for F in NONORTHO_F:
+        a = np.ascontiguousarray(_ftf(np.array(F).reshape(3, 3))).ravel()
+        q1, w1 = eig_oracle(a)
+        q2, w2 = np.zeros(9), np.zeros(3)
+        ref.ref_dsyevd3(P(a), P(q2), P(w2))
+        assert np.array_equal(q1.ravel().view(np.int64), q2.view(np.int64))
+        Q = q2.reshape(3, 3)
+        assert np.isfinite(Q).all() and np.abs(Q.T @ Q - np.eye(3)).max() > 1e-7
+
+
+def jacobi(A):
+    Q, w = np.zeros(9), np.zeros(3)
+    lib.oracle_jacobi_eig3(P(np.ascontiguousarray(A, dtype=np.float64).ravel()), P(Q), P(w))
+    return Q.reshape(3, 3), w
+
+
+def test_jacobi_eig3():
+    """Cyclic Jacobi: A Q = Q diag(w), Q orthonormal, for random, repeated
+    and diagonal spectra (including the degenerate F^T F above)."""
+    mats = list(random_matrices(2000, seed=13))
+    mats += [_ftf(np.array(F).reshape(3, 3)) for F in DEGENERATE_F + NONORTHO_F]
+    mats += [np.eye(3), np.diag([2.0, 2.0, 1.0]), np.zeros((3, 3))]
+    for A in mats:
+        A = np.asarray(A, dtype=float)
+        Q, w = jacobi(A)
+        scale = max(1.0, np.abs(A).max())
+        np.testing.assert_allclose(A @ Q, Q * w, atol=4e-15 * scale)
+        np.testing.assert_allclose(Q.T @ Q, np.eye(3), atol=4e-15)
+        np.testing.assert_allclose(np.sort(w), np.linalg.eigvalsh(A), atol=4e-15 * scale)
+
+
+def test_polar_falls_back_on_nearly_repeated_pairs():
+    """Where the Kopp solver fails (non-finite or non-orthonormal vectors),
+    the oracle's polar decomposition (and the engine's) still returns R and
+    ln U, equal to scipy's polar / logm."""
+    for F in DEGENERATE_F + NONORTHO_F:
+        F = np.array(F).reshape(3, 3)
+        dE, R = polar(F)
+        assert np.isfinite(dE).all() and np.isfinite(R).all()
+        Rs, Us = sla.polar(F)
+        np.testing.assert_allclose(R, Rs, atol=1e-13)
+        np.testing.assert_allclose(sym(dE), sla.logm(Us).real, atol=1e-14)
+
+
 def test_polar_invariants():
     rng = np.random.default_rng(5)
     for k in range(300):
