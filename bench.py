@@ -97,7 +97,7 @@ def run_precision(prec, ctx, steps, warmup, n, nz, t_end, workload="taylor3d", p
     return out
 
 
-def run_sweep(prec, ctx, n_jobs, t_end, seed=20261203):
+def run_sweep(prec, ctx, n_jobs, t_end, seed=20261203, max_cycles=None):
     """Batched parameter sweep: n_jobs seeded Taylor impacts (2D axisymmetric
     or 3D, 2k-60k elements log-uniform, Johnson-Cook A, B, n, C, m x U(0.8,
     1.2), 150-300 m/s) advanced together by one batched solve (one launch per
@@ -113,16 +113,21 @@ def run_sweep(prec, ctx, n_jobs, t_end, seed=20261203):
     ctx.set_timing(True)
     ctx.reset_timing()
     t0 = time.perf_counter()
-    statuses = dnl.batch_solve(models, raise_on_error=False)
+    statuses = dnl.batch_solve(models, raise_on_error=False, max_cycles=max_cycles)
     wall = time.perf_counter() - t0
     failed = [(i, s, models[i].getCurrentTime()) for i, s in enumerate(statuses) if s != 0]
     if failed:
         print(f"[bench] sweep {prec}: failed jobs (index, status, time): {failed}", file=sys.stderr, flush=True)
+    capped = [(i, models[i].getCurrentTime(), models[i].getTimeStep()) for i, s in enumerate(statuses)
+              if s == 0 and models[i].getCurrentTime() < t_end]
+    if capped:
+        print(f"[bench] sweep {prec}: jobs stopped by the cycle cap (index, time, dt): {capped}", file=sys.stderr,
+              flush=True)
     ctx.set_timing(False)
     el_ms, el_n = ctx.kernel_time("element")
     ec = sum(m.getElementsNumber() * m.getIncrement() for m in models)
     cyc = [m.getIncrement() for m in models]
-    res = dict(wall=wall, value=ec / wall, elem_cycles=ec, jobs=n_jobs, failed=len(failed),
+    res = dict(wall=wall, value=ec / wall, elem_cycles=ec, jobs=n_jobs, failed=len(failed), capped=len(capped),
                elements=sum(m.getElementsNumber() for m in models), max_cycles=max(cyc),
                elem_ms=el_ms, elem_launches=el_n,
                disp=[m.getNodalDisplacements() for m in models[:8]])
@@ -264,6 +269,8 @@ def main():
     ap.add_argument("--workload", default="taylor3d", choices=["taylor3d", "necking", "sweep", "serve"])
     ap.add_argument("--pull", type=float, default=1.0, help="necking: pulled-end speed (m/s)")
     ap.add_argument("--jobs", type=int, default=256, help="sweep / serve workloads: number of jobs")
+    ap.add_argument("--max-cycles", type=int, default=None,
+                    help="sweep: stop the batch after about this many cycles (jobs whose time step collapsed)")
     ap.add_argument("--util", type=float, default=0.7, help="serve: offered load for the arrival rate")
     ap.add_argument("--rate", type=float, default=None, help="serve: arrival rate in jobs/s (overrides --util)")
     ap.add_argument("--step-cycles", type=int, default=32, help="serve: cycles per server step")
@@ -302,7 +309,7 @@ def main():
     if args.workload == "sweep":
         res = {}
         for p in precs:
-            res[p] = run_sweep(p, ctx, args.jobs, args.t_end)
+            res[p] = run_sweep(p, ctx, args.jobs, args.t_end, max_cycles=args.max_cycles)
             print(f"[bench] sweep {p}: {res[p]['value']:.4g} elem-cycles/s, wall {res[p]['wall']:.2f}s, "
                   f"max cycles {res[p]['max_cycles']}", file=sys.stderr, flush=True)
         r = res[precs[0]]
@@ -312,7 +319,8 @@ def main():
                 "data": "synthetic",
                 "config": {"workload": f"batched sweep of {r['jobs']} Taylor impacts (2D axi / 3D hex, "
                                        f"{r['elements']} elements in total), {args.t_end * 1e6:g} us",
-                           "precision": precs[0], "max_cycles": r["max_cycles"], "failed_jobs": r["failed"]},
+                           "precision": precs[0], "max_cycles": r["max_cycles"], "failed_jobs": r["failed"],
+                           "cycle_cap": args.max_cycles, "jobs_stopped_by_cap": r["capped"]},
                 "element_kernel_ms_per_cycle": r["elem_ms"] / max(r["elem_launches"], 1)}
         for p in precs[1:]:
             line[f"value_{p}"] = res[p]["value"]

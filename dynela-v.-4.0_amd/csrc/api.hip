@@ -435,6 +435,11 @@ int dnlg_job_solve(dnlg_job *job, double t_up_to) {
 }
 
 int dnlg_batch_solve(dnlg_ctx *ctx, dnlg_job *const *jobs, int n_jobs, const double *t_up_to) {
+  return dnlg_batch_advance(ctx, jobs, n_jobs, t_up_to, -1);
+}
+
+int dnlg_batch_advance(dnlg_ctx *ctx, dnlg_job *const *jobs, int n_jobs, const double *t_up_to,
+                       int64_t max_cycles) {
   if (!ctx || !jobs || n_jobs <= 0 || !t_up_to) return fail(ctx, DNLG_E_INVALID, "invalid batch");
   for (int i = 0; i < n_jobs; i++) {
     if (!jobs[i] || jobs[i]->ctx != ctx) return fail(ctx, DNLG_E_INVALID, "job of another context");
@@ -444,7 +449,7 @@ int dnlg_batch_solve(dnlg_ctx *ctx, dnlg_job *const *jobs, int n_jobs, const dou
   if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, DNLG_E_HIP, "hipSetDevice");
   std::vector<dnlg_job *> list(jobs, jobs + n_jobs);
   std::vector<double> up(t_up_to, t_up_to + n_jobs);
-  return batch_run(ctx, list, up);
+  return batch_run(ctx, list, up, max_cycles);
 }
 
 int dnlg_job_sizes(const dnlg_job *job, int64_t *n_nodes, int64_t *n_elems, int32_t *n_ip, int32_t *dim) {

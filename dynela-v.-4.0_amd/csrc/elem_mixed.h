@@ -335,7 +335,8 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
           dg = 0.5f * (gmax - gmin);
           g = gmin + dg;
         }
-        if (fabsf(dg) < (float)kTolNR) break;
+        // the reference tolerance, or 4 ulp of gamma where FP32 cannot resolve it
+        if (fabsf(dg) < fmaxf((float)kTolNR, 4.8e-7f * g)) break;
         epsp = ep0f + s23 * g;
         T = Tc + 0.5f * g * heatf * (s23 * yieldf + Snorm0);
         if (++iterate > kItMax) {

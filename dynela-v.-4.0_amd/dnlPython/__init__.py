@@ -681,13 +681,15 @@ class DynELA:
         return _NodeView(self, number)
 
 
-def batch_solve(models, upTo=None, context=None, raise_on_error=True):
+def batch_solve(models, upTo=None, context=None, raise_on_error=True, max_cycles=None):
     """Advance several models together (one launch per phase and cycle).
 
     A job that fails (negative Jacobian, Newton divergence, non-finite state)
     stops; the others run to their target time.  With raise_on_error the
     first failure raises after the batch; otherwise the per-model status
-    codes are returned (0 = OK)."""
+    codes are returned (0 = OK).  max_cycles: return after about that many
+    cycles even if some model has not reached its target (check
+    getCurrentTime)."""
     ctx = context or models[0].context
     for m in models:
         if m.context is not ctx:
@@ -696,7 +698,10 @@ def batch_solve(models, upTo=None, context=None, raise_on_error=True):
     n = len(models)
     handles = (ctypes.c_void_p * n)(*[m._job.value for m in models])
     t = np.array([m._solver.endTime if upTo is None else float(upTo) for m in models], dtype=np.float64)
-    st = ctx.lib.dnlg_batch_solve(ctx.handle, handles, n, _lib.as_dp(t))
+    if max_cycles is None:
+        st = ctx.lib.dnlg_batch_solve(ctx.handle, handles, n, _lib.as_dp(t))
+    else:
+        st = ctx.lib.dnlg_batch_advance(ctx.handle, handles, n, _lib.as_dp(t), int(max_cycles))
     statuses = [ctx.lib.dnlg_job_status(m._job) for m in models]
     if raise_on_error:
         ctx.check(st)

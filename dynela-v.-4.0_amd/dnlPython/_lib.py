@@ -82,6 +82,7 @@ SIGNATURES = {
     "dnlg_job_destroy": (ctypes.c_int, [_VP]),
     "dnlg_job_solve": (ctypes.c_int, [_VP, ctypes.c_double]),
     "dnlg_batch_solve": (ctypes.c_int, [_VP, ctypes.POINTER(_VP), ctypes.c_int, _dp]),
+    "dnlg_batch_advance": (ctypes.c_int, [_VP, ctypes.POINTER(_VP), ctypes.c_int, _dp, ctypes.c_int64]),
     "dnlg_job_sizes": (ctypes.c_int, [_VP, _i64p, _i64p, _i32p, _i32p]),
     "dnlg_job_read_nodes": (ctypes.c_int, [_VP, _dp, _dp, _dp]),
     "dnlg_job_read_ip": (ctypes.c_int, [_VP, ctypes.c_int, _dp]),

@@ -77,8 +77,9 @@ extern "C" {
 
 /* arithmetic of the element phases */
 #define DNLG_PREC_FP64 0  /* everything in FP64                              */
-#define DNLG_PREC_MIXED 1 /* FP32 strain/stress/forces, FP64 kinematics, FP64
-                             radial-return Newton and energy accumulators     */
+#define DNLG_PREC_MIXED 1 /* FP32 strain/stress/radial return/forces, FP64
+                             kinematics and accumulated plastic strain,
+                             temperature and energies                         */
 
 /* integration-point fields for dnlg_job_read_ip (names of Field.h:46-79) */
 #define DNLG_F_PLASTIC_STRAIN 0      /* scalar */
@@ -187,6 +188,13 @@ int dnlg_job_solve(dnlg_job *job, double t_up_to);
  * cycle covers all of them); t_up_to[i] per job.  Blocking. */
 int dnlg_batch_solve(dnlg_ctx *ctx, dnlg_job *const *jobs, int n_jobs,
                      const double *t_up_to);
+
+/* dnlg_batch_solve that returns after about max_cycles cycles (whole
+ * graphs) even if some job has not reached its target; read each job's time
+ * with dnlg_job_scalars.  A later call continues where it stopped.  No
+ * counterpart in the reference, whose loop runs to the target time. */
+int dnlg_batch_advance(dnlg_ctx *ctx, dnlg_job *const *jobs, int n_jobs,
+                       const double *t_up_to, int64_t max_cycles);
 
 /* ---- serving loop ----------------------------------------------------------
  * The reference solves one model per process to its end time (Model::solve,
