@@ -130,7 +130,8 @@ def run_sweep(prec, ctx, n_jobs, t_end, seed=20261203, max_cycles=None):
     res = dict(wall=wall, value=ec / wall, elem_cycles=ec, jobs=n_jobs, failed=len(failed), capped=len(capped),
                elements=sum(m.getElementsNumber() for m in models), max_cycles=max(cyc),
                elem_ms=el_ms, elem_launches=el_n,
-               disp=[m.getNodalDisplacements() for m in models[:8]])
+               done=[m.getCurrentTime() >= t_end for m in models],
+               disp=[m.getNodalDisplacements() for m in models])
     for m in models:
         m.close()
     return res
@@ -326,9 +327,13 @@ def main():
             line[f"value_{p}"] = res[p]["value"]
         if "mixed" in res and "fp64" in res:
             line["speedup_mixed_vs_fp64"] = res["mixed"]["value"] / res["fp64"]["value"]
+            # only jobs that reached t_end in both precisions are comparable
             errs = [float(np.linalg.norm(a - b, axis=0).max() / np.linalg.norm(b, axis=0).max())
-                    for a, b in zip(res["mixed"]["disp"], res["fp64"]["disp"])]
-            line["mixed_disp_err_vs_fp64_max"] = max(errs)
+                    for a, b, da, db in zip(res["mixed"]["disp"], res["fp64"]["disp"], res["mixed"]["done"],
+                                            res["fp64"]["done"]) if da and db]
+            if errs:
+                line["mixed_disp_err_vs_fp64"] = {"max": max(errs), "median": float(np.median(errs)),
+                                                  "jobs_compared": len(errs)}
         print(json.dumps(line), flush=True)
         return
     res = {}

@@ -23,8 +23,9 @@ struct dnlg_job {
   bool started = false;
   int32_t n_cmin = 0;
   int32_t n_bcn = 0;  // nodes with constant BCs
-  double t_now = 0.0;  // time and target at the last progress snapshot
+  double t_now = 0.0;  // time, target and increment at the last progress snapshot
   double up_to = 0.0;
+  int64_t inc_now = 0;
   int status = 0;
 };
 

@@ -368,6 +368,7 @@ int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vec
       jobs[i]->status = js;
       jobs[i]->t_now = t;
       jobs[i]->up_to = up;
+      jobs[i]->inc_now = (int64_t)B->h_snap[5 * i + 4];
       if (js != 0 || !(t < up)) continue;
       need = std::max(need, dt > 0 ? std::ceil((up - t) / dt) : 1.0);
       const int f = jobs[i]->host.dt_freq;

@@ -130,32 +130,34 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
   constexpr float inv2D = 1.f / NN;     // 2^-D
   const float g = 0.577350269189625764f;  // Gauss abscissa 1/sqrt(3)
 
-  const Task tk = tasks[blockIdx.x];
-  JobDev *J = jobs[tk.job];
-  if (!J->c.running || J->c.status != 0) return;
+  const Task tk = load_task(tasks);
+  JobDev *J = load_job(jobs, tk.job);
+  CJobDev *C = (CJobDev *)J;
+  if (!C->c.running || C->c.status != 0) return;
   const int lane = threadIdx.x & 63;
   const int b = lane & (NN - 1);            // lane within the element group
   const int base = lane & ~(NN - 1);        // first lane of the group
   const int le = threadIdx.x / NN;
   const int64_t e = tk.first + le;
   const bool valid = e < tk.end;
+
   const int ref = b ^ ((b >> 1) & 1);       // node / IP index in the reference order
-  const int64_t N = J->N;
-  const int f0 = J->c.f0;
+  const int64_t N = C->N;
+  const int f0 = C->c.f0;
 
   // ---- node b of the element: coordinates relative to node 0, increment
   float xl[DIM], ul[DIM];
   double x0r = 0.0;
   if (valid) {
-    const int nd = J->conn[e * NN + ref];
-    const int n0 = J->conn[e * NN];
-    const double *u0 = J->fu[f0];
+    const int nd = gbl(C->conn)[e * NN + ref];
+    const int n0 = gbl(C->conn)[e * NN];
+    const DNLG_GLOBAL double *u0 = gbl(C->fu[f0]);
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
-      xl[d] = (float)(J->x[d * N + nd] - J->x[d * N + n0]);
+      xl[d] = (float)(gbl(C->x)[d * N + nd] - gbl(C->x)[d * N + n0]);
       ul[d] = (float)u0[d * N + nd];
     }
-    if constexpr (DIM == 2) x0r = J->x[n0];
+    if constexpr (DIM == 2) x0r = gbl(C->x)[n0];
   } else {
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
@@ -163,6 +165,23 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
       ul[d] = 0.f;
     }
   }
+  // ---- integration-point state, issued after the node loads (vector loads
+  // retire in order) so that its latency overlaps the geometry below.  Idle
+  // lanes leave it unset: every use is under `valid`, and a zero written on
+  // the idle path would make the hardware wait for these loads right here.
+  const int64_t G = C->G;
+  const int64_t gi = e * NN + ref;
+  float sig[6], sy0, gam0;
+  double Tcur, ep0;
+  if (valid) {
+#pragma unroll
+    for (int c = 0; c < 6; c++) sig[c] = gbl(C->sig32)[c * G + gi];
+    sy0 = gbl(C->sy32)[gi];
+    gam0 = gbl(C->gam32)[gi];
+    Tcur = gbl(C->T)[gi];
+    ep0 = gbl(C->ep)[gi];
+  }
+
   // ---- Walsh modes of coordinates and increments
   float Mx[DIM], Mu[DIM];
 #pragma unroll
@@ -258,20 +277,15 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
   // ---- pressure: element mean of tr(dE) (Element.C:1125-1143)
   const float inc = group_sum<NB>(dE[0] + dE[3] + dE[5]) * inv2D;
 
-  const int64_t G = J->G;
-  const int64_t gi = e * NN + ref;
-  float sig[6] = {0, 0, 0, 0, 0, 0};
   float T9[DIM][DIM];
   float hoop = 0.f;
   if (valid) {
-#pragma unroll
-    for (int c = 0; c < 6; c++) sig[c] = J->sig32[c * G + gi];
-    const float pressure = (sig[0] + sig[3] + sig[5]) * (1.f / 3.f) + (float)J->K * inc;
+    const float pressure = (sig[0] + sig[3] + sig[5]) * (1.f / 3.f) + (float)C->K * inc;
 
-    const double dt = J->c.dt;
-    const float TwoGf = (float)J->twoG;
-    const double TwoG32 = kSqrt32 * J->twoG;
-    const double heatFrac = J->heatFrac;
+    const double dt = C->c.dt;
+    const float TwoGf = (float)C->twoG;
+    const double TwoG32 = kSqrt32 * C->twoG;
+    const double heatFrac = C->heatFrac;
     float old[6], s[6];
 #pragma unroll
     for (int c = 0; c < 6; c++) old[c] = sig[c];
@@ -292,30 +306,29 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
     s[5] += TwoGf * (dE[5] - em);
     const float Snorm =
         sqrtf(s[0] * s[0] + s[3] * s[3] + s[5] * s[5] + 2.f * (s[1] * s[1] + s[2] * s[2] + s[4] * s[4]));
-    const double Strial = kSqrt32 * (double)Snorm;
-    const double Tcur = J->T[gi];
-    double yield = (double)J->sy32[gi];
+    // trial test and radial return in FP32 (the yield stress is stored in FP32)
+    const float Strialf = (float)(kSqrt32 * (double)Snorm);
+    float yieldf = sy0;
     double gamma = 0.0;
-    const int law = J->law;
-    const double *lp = J->lp;
-    if (yield == 0.0) {
-      yield = law_yield(law, lp, kGammaInitial, kGammaInitial / dt, Tcur);
-      J->sy32[gi] = (float)yield;
+    const int law = C->law;
+    double lp[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) lp[k] = C->lp[k];
+    if (yieldf == 0.f) {
+      yieldf = (float)law_yield(law, lp, kGammaInitial, kGammaInitial / dt, Tcur);
+      gbl(C->sy32)[gi] = yieldf;
     }
-    if (Strial > yield) {
+    if (Strialf > yieldf) {
       // radial return (Element.C:485-601): Newton on gamma with bisection
       // safeguard, FP32 arithmetic; ep, T and the energies accumulate in FP64
       constexpr float s23 = (float)kSqrt23;
-      const double ep0 = J->ep[gi];
       const float TwoG32f = (float)TwoG32;
-      const float Strialf = (float)Strial;
       const float heatf = (float)heatFrac;
       const float Tc = (float)Tcur;
       const float ep0f = (float)ep0;
       const LawF L = law_f32_setup(law, lp, dt, heatFrac);
-      float yieldf = (float)yield;
       float gmin = 0.f, gmax = (Strialf - yieldf) / TwoG32f;
-      float g = J->gam32[gi];
+      float g = gam0;
       if (ep0 == 0.0) g = (float)(kSqrt32 * kGammaInitial);
       float epsp = ep0f + s23 * g;
       float T = Tc + 0.5f * g * heatf * (s23 * yieldf + Snorm0);
@@ -348,11 +361,11 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
       const float scale = 1.f - TwoGf * g / Snorm;
 #pragma unroll
       for (int c = 0; c < 6; c++) s[c] *= scale;
-      J->ep[gi] = ep0 + kSqrt23 * gamma;
-      J->dep32[gi] = (float)(kSqrt23 * gamma / dt);
-      J->gam32[gi] = g;
-      J->gamc32[gi] += g;
-      J->sy32[gi] = yieldf;
+      gbl(C->ep)[gi] = ep0 + kSqrt23 * gamma;
+      gbl(C->dep32)[gi] = (float)(kSqrt23 * gamma / dt);
+      gbl(C->gam32)[gi] = g;
+      gbl(C->gamc32)[gi] += g;
+      gbl(C->sy32)[gi] = yieldf;
     }
 #pragma unroll
     for (int c = 0; c < 6; c++) sig[c] = s[c];
@@ -362,23 +375,23 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
     const float power = 0.5f * (dE[0] * (old[0] + sig[0]) + dE[3] * (old[3] + sig[3]) + dE[5] * (old[5] + sig[5]) +
                                 2.f * (dE[1] * (old[1] + sig[1]) + dE[2] * (old[2] + sig[2]) +
                                        dE[4] * (old[4] + sig[4])));
-    J->eint[gi] += (double)power / J->rho0;
+    gbl(C->eint)[gi] += (double)power / C->rho0;
     if (gamma != 0.0) {
       const float sn =
           sqrtf(s[0] * s[0] + s[3] * s[3] + s[5] * s[5] + 2.f * (s[1] * s[1] + s[2] * s[2] + s[4] * s[4]));
       const double w = 0.5 * gamma * ((double)sn + (double)Snorm0);
-      J->einel[gi] += w / J->rho0;
-      J->T[gi] = Tcur + heatFrac * w;
+      gbl(C->einel)[gi] += w / C->rho0;
+      gbl(C->T)[gi] = Tcur + heatFrac * w;
     }
     // Element::computeFinalRotation (Element.C:1146-1161)
     rotate_sym(R, sig);
 #pragma unroll
-    for (int c = 0; c < 6; c++) J->sig32[c * G + gi] = sig[c];
+    for (int c = 0; c < 6; c++) gbl(C->sig32)[c * G + gi] = sig[c];
     // Element::computeDensity (Element.C:1243-1256)
-    if (J->c.density_due) {
-      double rr = J->rho0 * (double)J->detJ0f[gi] / (double)detJ;
+    if (C->c.density_due) {
+      double rr = C->rho0 * (double)gbl(C->detJ0f)[gi] / (double)detJ;
       if constexpr (DIM == 2) rr /= (double)radius;
-      J->rho[gi] = rr;
+      gbl(C->rho)[gi] = rr;
     }
   }
   // ---- internal forces (Element.C:362-404): T = w J^-1 sigma per Gauss point
@@ -440,7 +453,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
     f[0] -= h0 + c0 * h1 + c1 * h2 + c0 * c1 * h3;
   }
   if (valid) {
-    float *out = J->elemF32 + (e * NN + ref) * DIM;
+    DNLG_GLOBAL float *out = gbl(C->elemF32) + (e * NN + ref) * DIM;
 #pragma unroll
     for (int i = 0; i < DIM; i++) out[i] = f[i] * inv2D;
   }

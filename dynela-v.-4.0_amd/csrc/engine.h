@@ -92,6 +92,20 @@ struct JobDev {
   double *eps, *epsP;  // optional tensors (track_tensors)
 };
 
+// Address spaces for the kernels.  A job record is read-only while a kernel
+// runs (apart from the sticky status word, which only the atomic setter
+// touches), and uniform over a block: reading it through a constant-space
+// pointer lets the compiler use scalar loads, which do not wait behind the
+// vector loads of the element / node data.  Data arrays are accessed through
+// global-space pointers (global instead of flat instructions).
+#define DNLG_GLOBAL __attribute__((address_space(1)))
+#define DNLG_CONST __attribute__((address_space(4)))
+typedef const DNLG_CONST JobDev CJobDev;
+template <typename T>
+__device__ __forceinline__ DNLG_GLOBAL T *gbl(T *p) {
+  return (DNLG_GLOBAL T *)p;
+}
+
 // One block of work: job index into the batch, local block index within the
 // job, and the half-open item range [first, end).
 struct Task {
@@ -100,5 +114,20 @@ struct Task {
   int64_t first;
   int64_t end;
 };
+
+// The block's task and job record, read through the constant space.
+__device__ __forceinline__ Task load_task(const Task *tasks) {
+  const DNLG_CONST Task *t = (const DNLG_CONST Task *)tasks + blockIdx.x;
+  Task tk;
+  tk.job = t->job;
+  tk.blk = t->blk;
+  tk.first = t->first;
+  tk.end = t->end;
+  return tk;
+}
+
+__device__ __forceinline__ JobDev *load_job(JobDev *const *jobs, int j) {
+  return ((JobDev *const DNLG_CONST *)jobs)[j];
+}
 
 }  // namespace dnlg

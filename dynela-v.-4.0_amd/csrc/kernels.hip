@@ -344,12 +344,12 @@ namespace dnlg {
 // BoundaryRestrain / BoundarySpeed ::applyConstantOnNewFields
 // (BoundaryRestrain.C:98-150, BoundarySpeed.C:139-226) on both fields.
 template <int DIM>
-__device__ __forceinline__ void apply_bcs(const JobDev *J, int64_t n, double time, double h, double ua[DIM],
+__device__ __forceinline__ void apply_bcs(CJobDev *C, int64_t n, double time, double h, double ua[DIM],
                                           double va[DIM], double aa[DIM], double ub[DIM], double vb[DIM],
                                           double ab[DIM]) {
-  const int k0 = J->bc_ptr[n], k1 = J->bc_ptr[n + 1];
+  const int k0 = gbl(C->bc_ptr)[n], k1 = gbl(C->bc_ptr)[n + 1];
   for (int k = k0; k < k1; k++) {
-    const Boundary B = J->bnd[J->bc_idx[k]];
+    const DNLG_GLOBAL Boundary &B = gbl(C->bnd)[gbl(C->bc_idx)[k]];
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
       if (B.kind == DNLG_BC_RESTRAIN) {
@@ -389,32 +389,33 @@ __device__ __forceinline__ void apply_bcs(const JobDev *J, int64_t n, double tim
 // applied by k_cycle_step.
 template <int DIM, typename FT>
 __global__ __launch_bounds__(256) void k_correct(JobDev *const *jobs, const Task *tasks) {
-  const Task tk = tasks[blockIdx.x];
-  JobDev *J = jobs[tk.job];
-  if (!J->c.running || J->c.status != 0) return;
+  const Task tk = load_task(tasks);
+  JobDev *J = load_job(jobs, tk.job);
+  CJobDev *C = (CJobDev *)J;
+  if (!C->c.running || C->c.status != 0) return;
   const int64_t n = tk.first + threadIdx.x;
   if (n >= tk.end) return;
-  const int64_t N = J->N;
-  const int a = J->c.f0, b = 1 - a;
-  const double h = J->c.dt;
-  const FT *ef;
-  if constexpr (sizeof(FT) == 8) ef = J->elemF; else ef = J->elemF32;
+  const int64_t N = C->N;
+  const int a = C->c.f0, b = 1 - a;
+  const double h = C->c.dt;
+  const DNLG_GLOBAL FT *ef;
+  if constexpr (sizeof(FT) == 8) ef = gbl(C->elemF); else ef = gbl(C->elemF32);
   double f[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; d++) f[d] = 0.0;
-  const int k0 = J->n2e_ptr[n], k1 = J->n2e_ptr[n + 1];
+  const int k0 = gbl(C->n2e_ptr)[n], k1 = gbl(C->n2e_ptr)[n + 1];
   for (int k = k0; k < k1; k++) {
-    const int64_t slot = J->n2e_idx[k];
+    const int64_t slot = gbl(C->n2e_idx)[k];
 #pragma unroll
     for (int d = 0; d < DIM; d++) f[d] += (double)ef[slot * DIM + d];
   }
-  const double m = J->mass[n];
-  const double alphaM = J->alphaM, gam = J->gammaCH, beta = J->beta;
+  const double m = gbl(C->mass)[n];
+  const double alphaM = C->alphaM, gam = C->gammaCH, beta = C->beta;
   double ua[DIM], va[DIM], aa[DIM], ub[DIM], vb[DIM], ab[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; d++) {
-    va[d] = J->fv[a][d * N + n];
-    aa[d] = J->fa[a][d * N + n];
+    va[d] = gbl(C->fv[a])[d * N + n];
+    aa[d] = gbl(C->fa[a])[d * N + n];
     // predictor (Explicit.C:458-466)
     ub[d] = h * (va[d] + (0.5 - beta) * h * aa[d]);
     vb[d] = va[d] + (1.0 - gam) * h * aa[d];
@@ -425,26 +426,26 @@ __global__ __launch_bounds__(256) void k_correct(JobDev *const *jobs, const Task
     vb[d] += gam * h * ab[d];
     ub[d] += beta * (h * h) * ab[d];
   }
-  const bool has_bc = J->bc_ptr[n + 1] > J->bc_ptr[n];
+  const bool has_bc = gbl(C->bc_ptr)[n + 1] > gbl(C->bc_ptr)[n];
   if (has_bc) {
 #pragma unroll
-    for (int d = 0; d < DIM; d++) ua[d] = J->fu[a][d * N + n];
-    apply_bcs<DIM>(J, n, J->c.t, h, ua, va, aa, ub, vb, ab);
+    for (int d = 0; d < DIM; d++) ua[d] = gbl(C->fu[a])[d * N + n];
+    apply_bcs<DIM>(C, n, C->c.t, h, ua, va, aa, ub, vb, ab);
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
-      J->fu[a][d * N + n] = ua[d];
-      J->fv[a][d * N + n] = va[d];
-      J->fa[a][d * N + n] = aa[d];
+      gbl(C->fu[a])[d * N + n] = ua[d];
+      gbl(C->fv[a])[d * N + n] = va[d];
+      gbl(C->fa[a])[d * N + n] = aa[d];
     }
   }
   bool finite = true;
 #pragma unroll
   for (int d = 0; d < DIM; d++) {
-    J->fu[b][d * N + n] = ub[d];
-    J->fv[b][d * N + n] = vb[d];
-    J->fa[b][d * N + n] = ab[d];
-    const double xn = J->x[d * N + n] + ub[d];
-    J->x[d * N + n] = xn;
+    gbl(C->fu[b])[d * N + n] = ub[d];
+    gbl(C->fv[b])[d * N + n] = vb[d];
+    gbl(C->fa[b])[d * N + n] = ab[d];
+    const double xn = gbl(C->x)[d * N + n] + ub[d];
+    gbl(C->x)[d * N + n] = xn;
     finite = finite && isfinite(xn) && isfinite(vb[d]);
   }
   if (!finite) set_status(J, DNLG_E_NONFINITE);
@@ -455,26 +456,26 @@ __global__ __launch_bounds__(256) void k_correct(JobDev *const *jobs, const Task
 // shuffles + LDS), one partial per block; k_cycle_end reduces the partials.
 // ---------------------------------------------------------------------------
 template <int DIM>
-__device__ __forceinline__ double element_char_length(const JobDev *J, int64_t e) {
-  const int64_t N = J->N;
+__device__ __forceinline__ double element_char_length(CJobDev *C, int64_t e) {
+  const int64_t N = C->N;
   if constexpr (DIM == 2) {
-    const int32_t *cn = J->conn + e * 4;
+    const DNLG_GLOBAL int32_t *cn = gbl(C->conn) + e * 4;
     const int n0 = cn[0], n1 = cn[1], n2 = cn[2], n3 = cn[3];
     // ElQua4NAx::getCharacteristicLength (ElQua4NAx.C:312-321)
-    const double x31 = J->x[n2] - J->x[n0];
-    const double x24 = J->x[n1] - J->x[n3];
-    const double y42 = J->x[N + n3] - J->x[N + n1];
-    const double y31 = J->x[N + n2] - J->x[N + n0];
+    const double x31 = gbl(C->x)[n2] - gbl(C->x)[n0];
+    const double x24 = gbl(C->x)[n1] - gbl(C->x)[n3];
+    const double y42 = gbl(C->x)[N + n3] - gbl(C->x)[N + n1];
+    const double y31 = gbl(C->x)[N + n2] - gbl(C->x)[N + n0];
     return ((x31 * y42 + x24 * y31) / sqrt(2.0 * (sqd(x24) + sqd(y42) + sqd(x31) + sqd(y31))));
   } else {
     // ElHex8N3D::getCharacteristicLength / getVolume (ElHex8N3D.C:614-684)
-    const int32_t *cn = J->conn + e * 8;
+    const DNLG_GLOBAL int32_t *cn = gbl(C->conn) + e * 8;
     double X[8][3];
 #pragma unroll
     for (int a = 0; a < 8; a++) {
       const int nd = cn[a];
 #pragma unroll
-      for (int c = 0; c < 3; c++) X[a][c] = J->x[c * N + nd];
+      for (int c = 0; c < 3; c++) X[a][c] = gbl(C->x)[c * N + nd];
     }
     auto aert = [&](int p1, int p2, int p3, int p4) {
       const double x1 = X[p1][0], x2 = X[p2][0], x3 = X[p3][0], x4 = X[p4][0];
@@ -523,24 +524,25 @@ __device__ __forceinline__ double element_char_length(const JobDev *J, int64_t e
 template <int DIM>
 __global__ __launch_bounds__(256) void k_courant(JobDev *const *jobs, const Task *tasks, int force) {
   __shared__ double s_min[4];
-  const Task tk = tasks[blockIdx.x];
-  JobDev *J = jobs[tk.job];
+  const Task tk = load_task(tasks);
+  JobDev *J = load_job(jobs, tk.job);
+  CJobDev *C = (CJobDev *)J;
   if (!force) {
     // Explicit.C:243-251: only while t_{n+1} < t_up_to, every dt_freq increments
-    if (!J->c.running || J->c.status != 0 || !J->c.courant_due) return;
-    if (!(J->c.t + J->c.dt < J->c.up_to)) return;
+    if (!C->c.running || C->c.status != 0 || !C->c.courant_due) return;
+    if (!(C->c.t + C->c.dt < C->c.up_to)) return;
   }
   const int64_t e = tk.first + threadIdx.x;
   double v = 1.0e300;
   if (e < tk.end) {
-    const int nip = J->nip;
+    const int nip = C->nip;
     double rho = 0.0;
-    for (int q = 0; q < nip; q++) rho += J->rho[e * nip + q];
+    for (int q = 0; q < nip; q++) rho += gbl(C->rho)[e * nip + q];
     rho /= nip;
-    const double nu = J->nu;
+    const double nu = C->nu;
     // Element::getElongationWaveSpeed (Element.C:343-359)
-    const double c = sqrt((J->young * (1.0 - nu)) / (rho * (1.0 + nu) * (1.0 - 2.0 * nu)));
-    v = element_char_length<DIM>(J, e) / c;
+    const double c = sqrt((C->young * (1.0 - nu)) / (rho * (1.0 + nu) * (1.0 - 2.0 * nu)));
+    v = element_char_length<DIM>(C, e) / c;
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off));
@@ -549,7 +551,7 @@ __global__ __launch_bounds__(256) void k_courant(JobDev *const *jobs, const Task
   if (threadIdx.x == 0) {
     double m = s_min[0];
     for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmin(m, s_min[w]);
-    J->cmin[tk.blk] = m;
+    gbl(C->cmin)[tk.blk] = m;
   }
 }
 
@@ -560,22 +562,22 @@ __global__ __launch_bounds__(256) void k_courant(JobDev *const *jobs, const Task
 // Explicit::computePredictions, Explicit.C:468-480): constant BCs at the
 // cycle's time and step, read by the element pass.
 template <int DIM>
-__device__ __forceinline__ void bc_field0(JobDev *J, int64_t n, int f0, double t, double h) {
-  const int64_t N = J->N;
+__device__ __forceinline__ void bc_field0(CJobDev *C, int64_t n, int f0, double t, double h) {
+  const int64_t N = C->N;
   double ua[DIM], va[DIM], aa[DIM], ub[DIM], vb[DIM], ab[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; d++) {
-    ua[d] = J->fu[f0][d * N + n];
-    va[d] = J->fv[f0][d * N + n];
-    aa[d] = J->fa[f0][d * N + n];
+    ua[d] = gbl(C->fu[f0])[d * N + n];
+    va[d] = gbl(C->fv[f0])[d * N + n];
+    aa[d] = gbl(C->fa[f0])[d * N + n];
     ub[d] = vb[d] = ab[d] = 0.0;
   }
-  apply_bcs<DIM>(J, n, t, h, ua, va, aa, ub, vb, ab);
+  apply_bcs<DIM>(C, n, t, h, ua, va, aa, ub, vb, ab);
 #pragma unroll
   for (int d = 0; d < DIM; d++) {
-    J->fu[f0][d * N + n] = ua[d];
-    J->fv[f0][d * N + n] = va[d];
-    J->fa[f0][d * N + n] = aa[d];
+    gbl(C->fu[f0])[d * N + n] = ua[d];
+    gbl(C->fv[f0])[d * N + n] = va[d];
+    gbl(C->fa[f0])[d * N + n] = aa[d];
   }
 }
 
@@ -591,23 +593,24 @@ __device__ __forceinline__ void bc_field0(JobDev *J, int64_t n, int f0, double t
 // the job has read the old one.  Blocks per job: max(1, ceil(n_bcn / 256)).
 __global__ __launch_bounds__(256) void k_cycle_step(JobDev *const *jobs, const Task *tasks, int open) {
   __shared__ double s_min[4];
-  const Task tk = tasks[blockIdx.x];
-  JobDev *J = jobs[tk.job];
-  double t = J->c.t, dt = J->c.dt;
+  const Task tk = load_task(tasks);
+  JobDev *J = load_job(jobs, tk.job);
+  CJobDev *C = (CJobDev *)J;
+  double t = C->c.t, dt = C->c.dt;
   const double dt_last = dt;
-  const bool resumed = !J->c.pending;  // first cycle of an advance
-  const double up_to = J->c.up_to;
-  int64_t inc = J->c.inc;
-  int f0 = J->c.f0;
-  const int status = J->c.status;
-  int running = J->c.running, courant_due = J->c.courant_due, density_due = J->c.density_due;
-  int preloop = J->c.preloop, pending = J->c.pending;
+  const bool resumed = !C->c.pending;  // first cycle of an advance
+  const double up_to = C->c.up_to;
+  int64_t inc = C->c.inc;
+  int f0 = C->c.f0;
+  const int status = C->c.status;
+  int running = C->c.running, courant_due = C->c.courant_due, density_due = C->c.density_due;
+  int preloop = C->c.preloop, pending = C->c.pending;
   if (pending && status == 0) {
     t += dt;
     f0 = 1 - f0;
     if (courant_due && t < up_to) {
       double v = 1.0e300;
-      for (int k = threadIdx.x; k < J->n_cmin; k += blockDim.x) v = fmin(v, J->cmin[k]);
+      for (int k = threadIdx.x; k < C->n_cmin; k += blockDim.x) v = fmin(v, gbl(C->cmin)[k]);
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off));
       if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = v;
@@ -615,7 +618,7 @@ __global__ __launch_bounds__(256) void k_cycle_step(JobDev *const *jobs, const T
       double m = s_min[0];
       for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmin(m, s_min[w]);
       const double fmax = 2.0 / m;
-      dt = J->safety * J->omegaS / fmax;
+      dt = C->safety * C->omegaS / fmax;
     }
   }
   pending = 0;
@@ -625,21 +628,21 @@ __global__ __launch_bounds__(256) void k_cycle_step(JobDev *const *jobs, const T
     if (running) {
       if (t + dt > up_to) dt = up_to - t;
       inc++;
-      courant_due = (inc % J->dt_freq == 0) ? 1 : 0;
+      courant_due = (inc % C->dt_freq == 0) ? 1 : 0;
       // densities are consumed by the Courant step and read back at the end
       density_due = (courant_due || !(t + dt < up_to)) ? 1 : 0;
       pending = 1;
       // After the corrector, field0 already holds the BC values of this
       // cycle unless a speed BC depends on a changed step or on time; a
       // restraint zeroes the same components either way.
-      const bool bc_work = resumed || (J->bc_flags & 2) || ((J->bc_flags & 1) && dt != dt_last);
+      const bool bc_work = resumed || (C->bc_flags & 2) || ((C->bc_flags & 1) && dt != dt_last);
       const int64_t k = tk.first + threadIdx.x;
       if (bc_work && k < tk.end) {
-        const int64_t n = J->bcn[k];
-        if (J->dim == 3)
-          bc_field0<3>(J, n, f0, t, dt);
+        const int64_t n = gbl(C->bcn)[k];
+        if (C->dim == 3)
+          bc_field0<3>(C, n, f0, t, dt);
         else
-          bc_field0<2>(J, n, f0, t, dt);
+          bc_field0<2>(C, n, f0, t, dt);
       }
     }
   } else {
@@ -647,7 +650,7 @@ __global__ __launch_bounds__(256) void k_cycle_step(JobDev *const *jobs, const T
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    const unsigned nb = J->n_bcn > 0 ? (unsigned)((J->n_bcn + 255) / 256) : 1u;
+    const unsigned nb = C->n_bcn > 0 ? (unsigned)((C->n_bcn + 255) / 256) : 1u;
     __threadfence();
     if (nb == 1 || atomicAdd(&J->c.arrive, 1u) == nb - 1) {
       Control &c = J->c;

@@ -64,6 +64,7 @@ struct dnlg_server {
   int64_t budget = 0;
   int max_jobs = 0;
   int64_t in_use = 0;  // footprints of running and finished-but-not-taken jobs
+  int64_t cycle_budget = 0;  // > 0: a job still running after this many increments is stopped
   int64_t next_ticket = 1;
   std::deque<ServerJob *> queue;
   std::vector<ServerJob *> running;
@@ -211,6 +212,10 @@ int dnlg_server_step(dnlg_server *s, int max_cycles, int *n_running, int *n_queu
       if (j->job->status != 0 || !(j->job->t_now < j->job->up_to)) {
         j->state = kDone;
         j->status = j->job->status;
+      } else if (s->cycle_budget > 0 && j->job->inc_now >= s->cycle_budget) {
+        j->state = kDone;
+        j->status = DNLG_E_CAPACITY;
+        j->err = "cycle budget exhausted before t_end";
       } else {
         keep.push_back(j);
       }
@@ -220,6 +225,12 @@ int dnlg_server_step(dnlg_server *s, int max_cycles, int *n_running, int *n_queu
   if (n_running) *n_running = (int)s->running.size();
   if (n_queued) *n_queued = (int)s->queue.size();
   return st;
+}
+
+int dnlg_server_set_cycle_budget(dnlg_server *s, int64_t cycles) {
+  if (!s || cycles < 0) return DNLG_E_INVALID;
+  s->cycle_budget = cycles;
+  return DNLG_OK;
 }
 
 int dnlg_server_poll(dnlg_server *s, int64_t ticket, int *state, int *status) {
@@ -243,9 +254,8 @@ int dnlg_server_take(dnlg_server *s, int64_t ticket, dnlg_job **job) {
     s->in_use -= j->footprint;
     *job = j->job;
     j->job = nullptr;
-  } else if (!j->err.empty()) {
-    s->ctx->err = j->err;
   }
+  if (!j->err.empty()) s->ctx->err = j->err;
   s->all.erase(it);
   return status;
 }

@@ -718,15 +718,19 @@ class Server:
     memory budget, advances every running job by about ``cycles`` cycles and
     retires finished ones; ``take(ticket)`` returns the finished model (its
     getters read the engine job as after ``solve()``) together with its
-    status.  Results of a job do not depend on the jobs it ran beside."""
+    status.  Results of a job do not depend on the jobs it ran beside.
+    cycle_budget > 0 stops a job still short of its end time after that many
+    increments (status E_CAPACITY)."""
 
     QUEUED, RUNNING, DONE = 0, 1, 2
 
-    def __init__(self, context=None, hbm_budget=0, max_jobs=0):
+    def __init__(self, context=None, hbm_budget=0, max_jobs=0, cycle_budget=0):
         self.ctx = context or Context.default()
         self.handle = ctypes.c_void_p()
         self.ctx.check(self.ctx.lib.dnlg_server_create(self.ctx.handle, int(hbm_budget), int(max_jobs),
                                                        ctypes.byref(self.handle)))
+        if cycle_budget:
+            self.ctx.check(self.ctx.lib.dnlg_server_set_cycle_budget(self.handle, int(cycle_budget)))
         self._models = {}
 
     def submit(self, model):

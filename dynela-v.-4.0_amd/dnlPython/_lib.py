@@ -100,6 +100,7 @@ SIGNATURES = {
     "dnlg_server_submit": (ctypes.c_int, [_VP, ctypes.POINTER(Mesh), ctypes.POINTER(MaterialDesc),
                                           ctypes.POINTER(Bcs), ctypes.POINTER(SolverParams), _i64p]),
     "dnlg_server_step": (ctypes.c_int, [_VP, ctypes.c_int, _i32p, _i32p]),
+    "dnlg_server_set_cycle_budget": (ctypes.c_int, [_VP, ctypes.c_int64]),
     "dnlg_server_poll": (ctypes.c_int, [_VP, ctypes.c_int64, _i32p, _i32p]),
     "dnlg_server_take": (ctypes.c_int, [_VP, ctypes.c_int64, ctypes.POINTER(_VP)]),
 }

@@ -221,6 +221,10 @@ int dnlg_server_submit(dnlg_server *srv, const dnlg_mesh *mesh,
  * retires finished and failed ones.  Blocking. */
 int dnlg_server_step(dnlg_server *srv, int max_cycles, int *n_running,
                      int *n_queued);
+/* cycles > 0: a job still short of t_end after that many increments is
+ * stopped and finishes with DNLG_E_CAPACITY (its state can still be read
+ * after dnlg_server_take); 0: no limit (default) */
+int dnlg_server_set_cycle_budget(dnlg_server *srv, int64_t cycles);
 /* state: 0 queued, 1 running, 2 finished; status: the job's status code */
 int dnlg_server_poll(dnlg_server *srv, int64_t ticket, int *state, int *status);
 /* hands a finished job over to the caller (read it with dnlg_job_read_*, free
