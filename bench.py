@@ -1,26 +1,33 @@
 #!/usr/bin/env python3
 """Benchmark of the explicit-solve engine on one MI355X.
 
-Workload: 3D Taylor impact of a full copper cylinder, 40 x 40 x 75 = 120 000
-8-node hexes (127 756 nodes), Johnson-Cook, 227 m/s, 80 us end time,
-Courant step every 10 increments.  One step = one complete solve of that
-model, from t = 0 to 80 us, on a job whose state is already resident in HBM
-(jobs are created before the timed region).
+Default workload (--workload taylor3d): 3D Taylor impact of a full copper
+cylinder, O-grid cross-section of 20 x 20 + 4 x 20 x 10 cells x 100 layers =
+120 000 8-node hexes (125 341 nodes), Johnson-Cook, 227 m/s, 80 us, Courant
+step every 10 increments.  One step = one complete solve of that model, from
+t = 0 to 80 us, on a job whose state is already resident in HBM (jobs are
+created before the timed region).
 
 Metric: elem-cycles/s = n_elem x cycles / wall seconds.  ``value`` is the
-mixed-precision path (FP32 strain / stress / forces, FP64 kinematics, FP64
-radial-return Newton and energies); the plain FP64 path runs on the same
-model for the speed-up and the error of the mixed result against it.
+mixed-precision path (FP32 strain / stress / radial return / forces, FP64
+kinematics and accumulators); the plain FP64 path runs on the same model for
+the speed-up and the error of the mixed result against it.
 
 The roofline object is measured on the dominant kernel (the hex element
-kernel) with HIP events recorded around each of its launches on the engine's
-stream during the timed region; its algorithmic bytes per element-cycle are
-the compulsory state traffic of the layout (see ALGO_BYTES below).
+kernel) with HIP events recorded around its launches on the engine's stream
+in one sampled cycle per graph during the timed region; its algorithmic bytes
+per element-cycle are the compulsory state traffic of the layout (see
+algo_bytes_per_elem); ``traffic`` is the rocprofv3 measurement committed under
+profiles/ for this kernel and workload.
 
 cpu_baseline: the FP64 CPU oracle (the restatement of the reference loop,
 test infrastructure under oracle/) timed on this host on a bounded sample of
-the same model (the first 2 us of the impact), all element phases OpenMP
+the same model (the first 1 us of the impact), element phases OpenMP
 parallel.
+
+Other workloads: --workload necking (round bar in tension, e.g. --n 40
+--nz 208 for 998 400 hexes), sweep (batched seeded parameter sweep), serve
+(serving loop with Poisson arrivals, p50/p99 latency).
 """
 import argparse
 import json
@@ -154,7 +161,7 @@ def estimate_work(model):
     return a.conn.shape[0] * model._solver.endTime / dt
 
 
-def run_serve(prec, ctx, n_jobs, util, rate, cycles_per_step, seed=20261203):
+def run_serve(prec, ctx, n_jobs, util, rate, cycles_per_step, seed=20261203, cycle_budget=0):
     """Serving loop: Poisson arrivals of a seeded job mix (axisymmetric / 3D x
     impact / necking, 1k-500k elements log-uniform, t_end U(20, 100) us);
     jobs are submitted at their arrival times, admitted by device-memory
@@ -172,7 +179,7 @@ def run_serve(prec, ctx, n_jobs, util, rate, cycles_per_step, seed=20261203):
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_jobs))
     print(f"[bench] serve {prec}: {n_jobs} jobs, {sum(m.getElementsNumber() for m in models)} elements, "
           f"rate {rate:.3g} jobs/s, estimated work {work.sum():.3g} elem-cycles", file=sys.stderr, flush=True)
-    srv = dnl.Server(ctx)
+    srv = dnl.Server(ctx, cycle_budget=cycle_budget)
     tickets = {}
     done = {}
     busy = 0.0
@@ -218,7 +225,7 @@ def run_serve(prec, ctx, n_jobs, util, rate, cycles_per_step, seed=20261203):
         print(f"[bench] serve job {i}: {j['kind']} {j['load']} {models[i].getElementsNumber()} elements, "
               f"t_end {j['t_end'] * 1e6:.1f} us, {models[i].getIncrement()} cycles, latency {lat[i]:.2f} s, "
               f"status {done[i][1]}", file=sys.stderr, flush=True)
-    failed = [i for i in range(n_jobs) if done[i][1] != 0]
+    failed = [(i, done[i][1]) for i in range(n_jobs) if done[i][1] != 0]
     ec = sum(m.getElementsNumber() * m.getIncrement() for m in models)
     res = dict(wall=wall, value=ec / wall, elem_cycles=ec, p50=float(np.percentile(lat, 50)),
                p99=float(np.percentile(lat, 99)), busy=busy / wall, rate=rate, failed=failed,
@@ -227,6 +234,17 @@ def run_serve(prec, ctx, n_jobs, util, rate, cycles_per_step, seed=20261203):
     for m in models:
         m.close()
     return res
+
+
+def measured_traffic(workload, prec, n, nz):
+    """HBM bytes per element-kernel launch measured offline with rocprofv3
+    (FETCH_SIZE and WRITE_SIZE passes, scaled by a plain-copy calibration),
+    when a summary exists for exactly this kernel and workload."""
+    path = os.path.join(ROOT, "profiles", "k_elem_mixed_hex8_traffic.json")
+    if workload != "taylor3d" or prec != "mixed" or (n, nz) != (20, 100) or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return float(json.load(f)["traffic_bytes_per_launch"])
 
 
 def cpu_baseline(n, nz, sample_t=1e-6, workload="taylor3d", pull=1.0):
@@ -275,6 +293,8 @@ def main():
     ap.add_argument("--util", type=float, default=0.7, help="serve: offered load for the arrival rate")
     ap.add_argument("--rate", type=float, default=None, help="serve: arrival rate in jobs/s (overrides --util)")
     ap.add_argument("--step-cycles", type=int, default=32, help="serve: cycles per server step")
+    ap.add_argument("--cycle-budget", type=int, default=0,
+                    help="serve: stop a job still short of t_end after this many increments (0: none)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the plain-copy reference kernel for the traffic counters")
     args = ap.parse_args()
@@ -292,7 +312,7 @@ def main():
     precs = [p.strip() for p in args.precision.split(",") if p.strip()]
     if args.workload == "serve":
         p = precs[0]
-        r = run_serve(p, ctx, args.jobs, args.util, args.rate, args.step_cycles)
+        r = run_serve(p, ctx, args.jobs, args.util, args.rate, args.step_cycles, cycle_budget=args.cycle_budget)
         print(f"[bench] serve {p}: {r['value']:.4g} elem-cycles/s, p50 {r['p50']:.3f}s, p99 {r['p99']:.3f}s, "
               f"busy {r['busy']:.2f}, failed {r['failed']}", file=sys.stderr, flush=True)
         line = {"metric": "served elem-cycles/s (continuous batching, Poisson arrivals)", "value": r["value"],
@@ -302,7 +322,7 @@ def main():
                 "config": {"workload": f"serving mix of {args.jobs} jobs (axi / 3D x impact / necking, "
                                        f"{r['elements']} elements in total, 1k-500k each, t_end 20-100 us)",
                            "precision": p, "arrival_rate_jobs_per_s": r["rate"],
-                           "cycles_per_step": args.step_cycles},
+                           "cycles_per_step": args.step_cycles, "cycle_budget": args.cycle_budget},
                 "latency_p50_s": r["p50"], "latency_p99_s": r["p99"], "busy_fraction": r["busy"],
                 "failed_jobs": r["failed"]}
         print(json.dumps(line), flush=True)
@@ -371,7 +391,8 @@ def main():
                                 f"pulled at {args.pull:g} m/s, {args.t_end * 1e6:g} us"),
                    "precision": head, "cycles_per_step": r["cycles"][0]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(args.workload, head, args.n, args.nz),
                      "kernel": "k_elem (hex8)", "kernel_avg_ms": avg_ms,
                      "algo_bytes_per_elem_cycle": bytes_el},
     }

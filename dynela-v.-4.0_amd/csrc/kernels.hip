@@ -69,29 +69,30 @@ __global__ __launch_bounds__(BLOCK) void k_elem_fp64(JobDev *const *jobs, const 
   __shared__ double s_sw[EPB][NIP][8];
   __shared__ double s_tr[EPB][NIP];
 
-  const Task tk = tasks[blockIdx.x];
-  JobDev *J = jobs[tk.job];
-  if (!J->c.running || J->c.status != 0) return;
+  const Task tk = load_task(tasks);
+  JobDev *J = load_job(jobs, tk.job);
+  CJobDev *C = (CJobDev *)J;
+  if (!C->c.running || C->c.status != 0) return;
   const int le = threadIdx.x / NIP;
   const int p = threadIdx.x % NIP;
   const int64_t e = tk.first + le;
   const bool valid = e < tk.end;
-  const int64_t N = J->N;
-  const int f0 = J->c.f0;
+  const int64_t N = C->N;
+  const int f0 = C->c.f0;
 
   // stage the element's nodes: lane p brings node p
   if (valid) {
-    const int nd = J->conn[e * NN + p];
-    const double *u0 = J->fu[f0];
+    const int nd = gbl(C->conn)[e * NN + p];
+    const DNLG_GLOBAL double *u0 = gbl(C->fu[f0]);
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
-      s_x[le][p][d] = J->x[d * N + nd];
+      s_x[le][p][d] = gbl(C->x)[d * N + nd];
       s_u[le][p][d] = u0[d * N + nd];
     }
   }
   __syncthreads();
 
-  const int64_t G = J->G;
+  const int64_t G = C->G;
   const int64_t g = e * NIP + p;
   double dN[NN][DIM];
   double detJ = 1.0, radius = 1.0;
@@ -174,14 +175,14 @@ __global__ __launch_bounds__(BLOCK) void k_elem_fp64(JobDev *const *jobs, const 
     for (int q = 0; q < NIP; q++) inc += s_tr[le][q];
     inc /= NIP;
 #pragma unroll
-    for (int c = 0; c < 6; c++) sig[c] = J->sig[c * G + g];
-    const double pressure = (sig[0] + sig[3] + sig[5]) / 3.0 + J->K * inc;
+    for (int c = 0; c < 6; c++) sig[c] = gbl(C->sig)[c * G + g];
+    const double pressure = (sig[0] + sig[3] + sig[5]) / 3.0 + C->K * inc;
 
     // Element::computeStress (Element.C:407-622)
-    const double dt = J->c.dt;
-    const double TwoG = J->twoG;
+    const double dt = C->c.dt;
+    const double TwoG = C->twoG;
     const double TwoG32 = kSqrt32 * TwoG;
-    const double heatFrac = J->heatFrac;
+    const double heatFrac = C->heatFrac;
     double old[6], s[6];
 #pragma unroll
     for (int c = 0; c < 6; c++) old[c] = sig[c];
@@ -203,19 +204,21 @@ __global__ __launch_bounds__(BLOCK) void k_elem_fp64(JobDev *const *jobs, const 
     for (int c = 0; c < 6; c++) s[c] += TwoG * de[c];
     const double Snorm = sqrt(sqd(s[0]) + sqd(s[3]) + sqd(s[5]) + 2.0 * (sqd(s[1]) + sqd(s[2]) + sqd(s[4])));
     const double Strial = kSqrt32 * Snorm;
-    const double Tcur = J->T[g];
-    const double ep0 = J->ep[g];
-    double yield = J->sy[g];
+    const double Tcur = gbl(C->T)[g];
+    const double ep0 = gbl(C->ep)[g];
+    double yield = gbl(C->sy)[g];
     double gamma = 0.0;
-    const int law = J->law;
-    const double *lp = J->lp;
+    const int law = C->law;
+    double lp[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) lp[k] = C->lp[k];
     if (yield == 0.0) {
       yield = law_yield(law, lp, kGammaInitial, kGammaInitial / dt, Tcur);
-      J->sy[g] = yield;
+      gbl(C->sy)[g] = yield;
     }
     if (Strial > yield) {
       double gmin = 0.0, gmax = (Strial - yield) / TwoG32;
-      gamma = J->gam[g];
+      gamma = gbl(C->gam)[g];
       if (ep0 == 0.0) gamma = kSqrt32 * kGammaInitial;
       double depsp = kSqrt23 * gamma / dt;
       double epsp = ep0 + kSqrt23 * gamma;
@@ -246,18 +249,18 @@ __global__ __launch_bounds__(BLOCK) void k_elem_fp64(JobDev *const *jobs, const 
           break;
         }
       }
-      if (J->track) {
+      if (C->track) {
 #pragma unroll
-        for (int c = 0; c < 6; c++) J->epsP[c * G + g] += gamma * s[c] / Snorm;
+        for (int c = 0; c < 6; c++) gbl(C->epsP)[c * G + g] += gamma * s[c] / Snorm;
       }
       const double scale = 1.0 - TwoG * gamma / Snorm;
 #pragma unroll
       for (int c = 0; c < 6; c++) s[c] *= scale;
-      J->ep[g] = ep0 + kSqrt23 * gamma;
-      J->dep[g] = kSqrt23 * gamma / dt;
-      J->gam[g] = gamma;
-      J->gamc[g] += gamma;
-      J->sy[g] = yield;
+      gbl(C->ep)[g] = ep0 + kSqrt23 * gamma;
+      gbl(C->dep)[g] = kSqrt23 * gamma / dt;
+      gbl(C->gam)[g] = gamma;
+      gbl(C->gamc)[g] += gamma;
+      gbl(C->sy)[g] = yield;
     }
 #pragma unroll
     for (int c = 0; c < 6; c++) sig[c] = s[c];
@@ -268,35 +271,35 @@ __global__ __launch_bounds__(BLOCK) void k_elem_fp64(JobDev *const *jobs, const 
     const double power = 0.5 * (dE[0] * (old[0] + sig[0]) + dE[3] * (old[3] + sig[3]) + dE[5] * (old[5] + sig[5]) +
                                 2.0 * (dE[1] * (old[1] + sig[1]) + dE[2] * (old[2] + sig[2]) +
                                        dE[4] * (old[4] + sig[4])));
-    J->eint[g] += power / J->rho0;
+    gbl(C->eint)[g] += power / C->rho0;
     if (gamma != 0.0) {
       const double sn = sqrt(sqd(s[0]) + sqd(s[3]) + sqd(s[5]) + 2.0 * (sqd(s[1]) + sqd(s[2]) + sqd(s[4])));
       const double w = 0.5 * gamma * (sn + Snorm0);
-      J->einel[g] += w / J->rho0;
-      J->T[g] = Tcur + heatFrac * w;
+      gbl(C->einel)[g] += w / C->rho0;
+      gbl(C->T)[g] = Tcur + heatFrac * w;
     }
     // Element::computeFinalRotation (Element.C:1146-1161)
     rotate_sym(R, sig);
 #pragma unroll
-    for (int c = 0; c < 6; c++) J->sig[c * G + g] = sig[c];
-    if (J->track) {
+    for (int c = 0; c < 6; c++) gbl(C->sig)[c * G + g] = sig[c];
+    if (C->track) {
       double t6[6];
 #pragma unroll
-      for (int c = 0; c < 6; c++) t6[c] = J->eps[c * G + g] + dE[c];
+      for (int c = 0; c < 6; c++) t6[c] = gbl(C->eps)[c * G + g] + dE[c];
       rotate_sym(R, t6);
 #pragma unroll
-      for (int c = 0; c < 6; c++) J->eps[c * G + g] = t6[c];
+      for (int c = 0; c < 6; c++) gbl(C->eps)[c * G + g] = t6[c];
 #pragma unroll
-      for (int c = 0; c < 6; c++) t6[c] = J->epsP[c * G + g];
+      for (int c = 0; c < 6; c++) t6[c] = gbl(C->epsP)[c * G + g];
       rotate_sym(R, t6);
 #pragma unroll
-      for (int c = 0; c < 6; c++) J->epsP[c * G + g] = t6[c];
+      for (int c = 0; c < 6; c++) gbl(C->epsP)[c * G + g] = t6[c];
     }
     // Element::computeDensity (Element.C:1243-1256), on x_n
-    if (J->c.density_due) {
-      double rr = J->rho0 * J->detJ0[g] / detJ;
+    if (C->c.density_due) {
+      double rr = C->rho0 * gbl(C->detJ0)[g] / detJ;
       if constexpr (DIM == 2) rr /= radius;
-      J->rho[g] = rr;
+      gbl(C->rho)[g] = rr;
     }
     wdj = detJ;  // Gauss weights are 1 (ElQua4NAx.C:63-110, ElHex8N3D.C:93-231)
     if constexpr (DIM == 2) wdj *= k2PI * radius;
@@ -326,7 +329,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_fp64(JobDev *const *jobs, const 
         for (int j = 0; j < DIM; j++) f[i] -= s_dN[le][q][a][j] * s_sw[le][q][sym_idx(j, i)] * w;
       if constexpr (DIM == 2) f[0] -= tabN<NN>(q, a) * s_sw[le][q][5] / s_sw[le][q][7] * w;
     }
-    double *out = J->elemF + (e * NN + a) * DIM;
+    DNLG_GLOBAL double *out = gbl(C->elemF) + (e * NN + a) * DIM;
 #pragma unroll
     for (int i = 0; i < DIM; i++) out[i] = f[i];
   }

@@ -43,3 +43,22 @@ def test_batch_in_segments(ctx):
         ta, _, _ = a.getScalars()
         tb, _, _ = b.getScalars()
         assert ta == pytest.approx(tb, rel=1e-12)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_batch_advance_in_cycle_chunks(ctx, prec):
+    """dnlg_batch_advance: advancing by short cycle caps until every job
+    reaches its end time gives bit-identical results to one batch_solve
+    (each stop closes the last cycle; the next call resumes it)."""
+    import dnlPython as dnl
+    one = _models(ctx, prec)
+    dnl.batch_solve(one)
+    chunked = _models(ctx, prec)
+    for _ in range(100000):
+        dnl.batch_solve(chunked, max_cycles=37)
+        if all(m.getCurrentTime() >= m._solver.endTime for m in chunked):
+            break
+    for a, b in zip(one, chunked):
+        assert a.getScalars() == b.getScalars()
+        np.testing.assert_array_equal(a.getNodalDisplacements(), b.getNodalDisplacements())
+        np.testing.assert_array_equal(a.getIntPointField(dnl.Field.Stress), b.getIntPointField(dnl.Field.Stress))
