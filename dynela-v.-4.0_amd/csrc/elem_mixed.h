@@ -41,6 +41,23 @@ __device__ __forceinline__ void sym_mul(const float A[6], const float B[6], floa
   C[5] = A[2] * B[2] + A[4] * B[4] + A[5] * B[5];
 }
 
+// R S R^T for symmetric S as (R S) R^T: 45 FMAs (SymTensor2::dotRxRT,
+// SymTensor2.C:512-525, which the FP64 path follows term by term).
+__device__ __forceinline__ void rotate_sym_f32(const float r[9], float S[6]) {
+  const float s3[3][3] = {{S[0], S[1], S[2]}, {S[1], S[3], S[4]}, {S[2], S[4], S[5]}};
+  float m[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) m[i][j] = r[3 * i] * s3[0][j] + r[3 * i + 1] * s3[1][j] + r[3 * i + 2] * s3[2][j];
+  S[0] = m[0][0] * r[0] + m[0][1] * r[1] + m[0][2] * r[2];
+  S[1] = m[0][0] * r[3] + m[0][1] * r[4] + m[0][2] * r[5];
+  S[2] = m[0][0] * r[6] + m[0][1] * r[7] + m[0][2] * r[8];
+  S[3] = m[1][0] * r[3] + m[1][1] * r[4] + m[1][2] * r[5];
+  S[4] = m[1][0] * r[6] + m[1][1] * r[7] + m[1][2] * r[8];
+  S[5] = m[2][0] * r[6] + m[2][1] * r[7] + m[2][2] * r[8];
+}
+
 // Large increments: FP64 Jacobi eigen path (rarely taken).
 __device__ __forceinline__ void polar_lnU_large(const float *H, float *dE, float *R) {
   double F[9], e64[6], r64[9];
@@ -105,12 +122,15 @@ __device__ __forceinline__ void polar_lnU_mixed(const float H[9], float dE[6], f
 
 // Walsh-Hadamard transform over the NB-bit lane group of an element: lane q
 // ends with sum_b prod_{d in q} s_d(b) v_b.
+// Stage d pairs the lanes differing in bit d: the lane with the bit clear
+// keeps v_lo + v_hi, the one with the bit set v_hi - v_lo, i.e. one FMA
+// v + cs[d] * partner with cs[d] = -1 where bit d is set, +1 otherwise.
 template <int NB>
-__device__ __forceinline__ float wht(float v, int b) {
+__device__ __forceinline__ float wht(float v, const float (&cs)[3]) {
 #pragma unroll
   for (int d = 0; d < NB; d++) {
     const float o = __shfl_xor(v, 1 << d);
-    v = ((b >> d) & 1) ? (v - o) : (v + o);
+    v = fmaf(cs[d], o, v);
   }
   return v;
 }
@@ -182,12 +202,15 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
     ep0 = gbl(C->ep)[gi];
   }
 
-  // ---- Walsh modes of coordinates and increments
+  // ---- Walsh modes of coordinates and increments, scaled by 2^-D
+  float cs[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) cs[d] = ((b >> d) & 1) ? -1.f : 1.f;
   float Mx[DIM], Mu[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; d++) {
-    Mx[d] = wht<NB>(xl[d], b);
-    Mu[d] = wht<NB>(ul[d], b);
+    Mx[d] = inv2D * wht<NB>(xl[d], cs);
+    Mu[d] = inv2D * wht<NB>(ul[d], cs);
   }
   // modes of the group, broadcast: mx[q][d]
   float mx[NN][DIM], mu[NN][DIM];
@@ -208,17 +231,17 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
 #pragma unroll
   for (int i = 0; i < DIM; i++) {
     if constexpr (DIM == 3) {
-      Jm[i][0] = inv2D * (mx[1][i] + mx[3][i] * xi[1] + mx[5][i] * xi[2] + mx[7][i] * (xi[1] * xi[2]));
-      Jm[i][1] = inv2D * (mx[2][i] + mx[3][i] * xi[0] + mx[6][i] * xi[2] + mx[7][i] * (xi[0] * xi[2]));
-      Jm[i][2] = inv2D * (mx[4][i] + mx[5][i] * xi[0] + mx[6][i] * xi[1] + mx[7][i] * (xi[0] * xi[1]));
-      Gm[i][0] = inv2D * (mu[1][i] + mu[3][i] * xi[1] + mu[5][i] * xi[2] + mu[7][i] * (xi[1] * xi[2]));
-      Gm[i][1] = inv2D * (mu[2][i] + mu[3][i] * xi[0] + mu[6][i] * xi[2] + mu[7][i] * (xi[0] * xi[2]));
-      Gm[i][2] = inv2D * (mu[4][i] + mu[5][i] * xi[0] + mu[6][i] * xi[1] + mu[7][i] * (xi[0] * xi[1]));
+      Jm[i][0] = mx[1][i] + mx[3][i] * xi[1] + mx[5][i] * xi[2] + mx[7][i] * (xi[1] * xi[2]);
+      Jm[i][1] = mx[2][i] + mx[3][i] * xi[0] + mx[6][i] * xi[2] + mx[7][i] * (xi[0] * xi[2]);
+      Jm[i][2] = mx[4][i] + mx[5][i] * xi[0] + mx[6][i] * xi[1] + mx[7][i] * (xi[0] * xi[1]);
+      Gm[i][0] = mu[1][i] + mu[3][i] * xi[1] + mu[5][i] * xi[2] + mu[7][i] * (xi[1] * xi[2]);
+      Gm[i][1] = mu[2][i] + mu[3][i] * xi[0] + mu[6][i] * xi[2] + mu[7][i] * (xi[0] * xi[2]);
+      Gm[i][2] = mu[4][i] + mu[5][i] * xi[0] + mu[6][i] * xi[1] + mu[7][i] * (xi[0] * xi[1]);
     } else {
-      Jm[i][0] = inv2D * (mx[1][i] + mx[3][i] * xi[1]);
-      Jm[i][1] = inv2D * (mx[2][i] + mx[3][i] * xi[0]);
-      Gm[i][0] = inv2D * (mu[1][i] + mu[3][i] * xi[1]);
-      Gm[i][1] = inv2D * (mu[2][i] + mu[3][i] * xi[0]);
+      Jm[i][0] = mx[1][i] + mx[3][i] * xi[1];
+      Jm[i][1] = mx[2][i] + mx[3][i] * xi[0];
+      Gm[i][0] = mu[1][i] + mu[3][i] * xi[1];
+      Gm[i][1] = mu[2][i] + mu[3][i] * xi[0];
     }
   }
   float detJ, inv[DIM][DIM];
@@ -258,8 +281,8 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
     }
   float radius = 1.f;
   if constexpr (DIM == 2) {
-    const float rl = inv2D * (mx[0][0] + mx[1][0] * xi[0] + mx[2][0] * xi[1] + mx[3][0] * (xi[0] * xi[1]));
-    const float ur = inv2D * (mu[0][0] + mu[1][0] * xi[0] + mu[2][0] * xi[1] + mu[3][0] * (xi[0] * xi[1]));
+    const float rl = mx[0][0] + mx[1][0] * xi[0] + mx[2][0] * xi[1] + mx[3][0] * (xi[0] * xi[1]);
+    const float ur = mu[0][0] + mu[1][0] * xi[0] + mu[2][0] * xi[1] + mu[3][0] * (xi[0] * xi[1]);
     radius = (float)(x0r + (double)rl);
     H[8] = ur / radius;
   }
@@ -384,7 +407,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
       gbl(C->T)[gi] = Tcur + heatFrac * w;
     }
     // Element::computeFinalRotation (Element.C:1146-1161)
-    rotate_sym(R, sig);
+    rotate_sym_f32(R, sig);
 #pragma unroll
     for (int c = 0; c < 6; c++) gbl(C->sig32)[c * G + gi] = sig[c];
     // Element::computeDensity (Element.C:1243-1256)
@@ -415,7 +438,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
 #pragma unroll
   for (int k = 0; k < DIM; k++)
 #pragma unroll
-    for (int i = 0; i < DIM; i++) WT[k][i] = wht<NB>(T9[k][i], b);
+    for (int i = 0; i < DIM; i++) WT[k][i] = wht<NB>(T9[k][i], cs);
   float f[DIM];
 #pragma unroll
   for (int i = 0; i < DIM; i++) f[i] = 0.f;
@@ -446,7 +469,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
   }
   if constexpr (DIM == 2) {
     // hoop term: f_r -= sum_p N_a(p) w_p sigma_tt / r_p
-    const float Wh = wht<NB>(hoop, b);
+    const float Wh = wht<NB>(hoop, cs);
     const float h0 = __shfl(Wh, base + 0), h1 = __shfl(Wh, base + 1), h2 = __shfl(Wh, base + 2),
                 h3 = __shfl(Wh, base + 3);
     const float c0 = g * sgn(b, 0), c1 = g * sgn(b, 1);
