@@ -1326,6 +1326,13 @@ int oracle_job_solve(OracleJob *J, double upTo) {
     compute_mass(J);
     compute_time_step(J, true);
     if (!element_phases(J, J->dt)) return J->status;
+  } else if (J->t < upTo) {
+    /* A later call continues the loop (the engine's semantics; the
+     * reference re-runs its pre-loop block instead).  The last cycle of the
+     * previous call skipped the end-of-cycle Jacobian (Explicit.C:243-247),
+     * so it is computed here on the current coordinates, as a single call
+     * would have done. */
+    if (!compute_jacobian(J, false)) return J->status;
   }
   while (J->t < upTo) {
     /* Explicit::updateTimes (Explicit.C:414-427) */

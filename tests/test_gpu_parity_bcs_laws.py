@@ -1,0 +1,91 @@
+"""Engine vs oracle on the paths the cycle control treats specially (GPU):
+ramped prescribed speeds (the field0 BC pass runs every cycle), constant
+speeds across solve segments (the step is clipped at each segment end, so
+the speed BC values change with it), and the power hardening law.
+
+Tolerances as tests/test_gpu_parity.py: FP64 displacement 1e-9, IP fields
+1e-7; mixed displacement 1e-5 (normwise relative)."""
+import numpy as np
+import pytest
+
+from helpers import disp_err
+from test_gpu_parity import compare, run_both
+
+pytestmark = pytest.mark.gpu
+
+
+def _ramp(model, ramp, speed_scale, t0, t1):
+    import dnlPython as dnl
+    from dnlPython import _lib
+    n = 0
+    for b, _members in model._constant:
+        if b.kind == _lib.BC_SPEED:
+            rf = dnl.RampFunction("ramp")
+            rf.setFunction(ramp, t0, t1)
+            b.setValue(0.0, speed_scale, 0.0)
+            b.setFunction(rf)
+            n += 1
+    assert n == 1
+    model._invalidate()
+    return model
+
+
+@pytest.mark.parametrize("ramp", [1, 2, 5])  # Linear, Sinus, Soft (RampFunction.h:31-39)
+def test_ramped_pull_axi_fp64(ctx, ramp):
+    import dnlPython as dnl
+    from dnlPython import samples
+    # the ramp density integrates to 1 over [0, 20 us]: speed_scale is the
+    # imposed end displacement (20 um over 20 us, ~1 m/s average)
+    m = _ramp(samples.necking_axi(5, 40, t_end=12e-6, context=ctx), ramp, 20e-6, 0.0, 20e-6)
+    o = run_both(m, 12e-6)
+    compare(m, o)
+
+
+def test_ramped_pull_3d_mixed(ctx):
+    from dnlPython import samples
+    m = _ramp(samples.necking_3d(4, 10, t_end=10e-6, precision="mixed", context=ctx), 5, 30e-6, 0.0, 15e-6)
+    o = run_both(m, 10e-6)
+    _, d_ref, _ = o.nodes()
+    err = disp_err(m.getNodalDisplacements(), d_ref)
+    print(f"mixed ramped pull disp_err={err:.3e}")
+    assert err <= 1e-5
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_speed_bc_across_segments(ctx, prec):
+    """Three solve calls on the engine and the oracle; each call clips the
+    step at its end, which changes u = v dt on the pulled nodes."""
+    from dnlPython import samples
+    from oracle.bridge import OracleJob
+    m = samples.necking_axi(5, 40, pull=4.0, t_end=12e-6, precision=prec, context=ctx)
+    o = OracleJob(m.descriptors())
+    for t in (3e-6, 7.5e-6, 12e-6):
+        assert o.solve(t) == 0
+        m.solve(t)
+    if prec == "fp64":
+        compare(m, o)
+    else:
+        _, d_ref, _ = o.nodes()
+        assert disp_err(m.getNodalDisplacements(), d_ref) <= 1e-5
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_power_law_taylor_axi(ctx, prec):
+    """ElastoplasticLaw sigma_y = A + B ep^n (ElastoplasticLaw.C:110-121)."""
+    import dnlPython as dnl
+    from dnlPython import samples
+    m = samples.taylor_axi(6, 30, speed=200.0, t_end=10e-6, precision=prec, context=ctx)
+    law = dnl.ElastoplasticLaw("power")
+    law.setParameters(90e6, 292e6, 0.31)
+    mat = m._material
+    mat.setHardeningLaw(law)
+    m._invalidate()
+    o = run_both(m, 10e-6)
+    if prec == "fp64":
+        compare(m, o)
+    else:
+        _, d_ref, _ = o.nodes()
+        err = disp_err(m.getNodalDisplacements(), d_ref)
+        print(f"mixed power-law disp_err={err:.3e}")
+        assert err <= 1e-5
+    assert np.max(m.getIntPointField(dnl.Field.plasticStrain)) > 0.01
