@@ -216,9 +216,10 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
     return st;
   };
 
-  // connectivity check and node -> element CSR in element order
+  // connectivity check; node -> (element, local node) incidence ranges in
+  // element order, and the slot of each (element, local node) in them
   std::vector<int32_t> conn(mesh->conn, mesh->conn + NE * nn);
-  std::vector<int32_t> n2e_ptr(N + 1, 0), n2e_idx(NE * nn);
+  std::vector<int32_t> n2e_ptr(N + 1, 0), e2slot(NE * nn);
   for (int64_t k = 0; k < NE * nn; k++) {
     if (conn[k] < 0 || conn[k] >= N) return bail(fail(ctx, DNLG_E_INVALID, "connectivity index out of range"));
     n2e_ptr[conn[k] + 1]++;
@@ -227,7 +228,7 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
   {
     std::vector<int32_t> fill(n2e_ptr.begin(), n2e_ptr.end() - 1);
     for (int64_t e = 0; e < NE; e++)
-      for (int a = 0; a < nn; a++) n2e_idx[fill[conn[e * nn + a]]++] = (int32_t)(e * nn + a);
+      for (int a = 0; a < nn; a++) e2slot[e * nn + a] = fill[conn[e * nn + a]]++;
   }
   // coordinates (ElementAxi::checkLevel2, ElementAxi.C:103-126)
   std::vector<double> x(dim * N);
@@ -373,8 +374,8 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
   H.bc_flags = bc_flags;
   ALLOC(p32, N + 1, n2e_ptr.data());
   H.n2e_ptr = p32;
-  ALLOC(p32, NE * nn, n2e_idx.data());
-  H.n2e_idx = p32;
+  ALLOC(p32, NE * nn, e2slot.data());
+  H.e2slot = p32;
   ALLOC(p32, NE * nn, conn.data());
   H.conn = p32;
   ALLOC(H.elemF, NE * nn * dim, (const double *)nullptr);

@@ -193,7 +193,9 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
   const int64_t gi = e * NN + ref;
   float sig[6], sy0, gam0;
   double Tcur, ep0;
+  int slot;  // incidence slot of this lane's node, where its force goes
   if (valid) {
+    slot = gbl(C->e2slot)[gi];
 #pragma unroll
     for (int c = 0; c < 6; c++) sig[c] = gbl(C->sig32)[c * G + gi];
     sy0 = gbl(C->sy32)[gi];
@@ -476,7 +478,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
     f[0] -= h0 + c0 * h1 + c1 * h2 + c0 * c1 * h3;
   }
   if (valid) {
-    DNLG_GLOBAL float *out = gbl(C->elemF32) + (e * NN + ref) * DIM;
+    DNLG_GLOBAL float *out = gbl(C->elemF32) + (int64_t)slot * DIM;
 #pragma unroll
     for (int i = 0; i < DIM; i++) out[i] = f[i] * inv2D;
   }

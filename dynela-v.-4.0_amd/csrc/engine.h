@@ -73,11 +73,12 @@ struct JobDev {
   const int32_t *bcn;     // nodes that carry constant BCs, ascending
   int32_t n_bcn;
   int32_t bc_flags;       // 1: a constant speed BC, 2: a ramped one
-  const int32_t *n2e_ptr;  // [N+1] node -> (element*nn + local node), element order
-  const int32_t *n2e_idx;
+  const int32_t *n2e_ptr;  // [N+1] incidence range of each node, element order
+  const int32_t *e2slot;   // [NE*nn] incidence slot of (element, local node)
   // elements
   const int32_t *conn;  // [NE][nn]
-  double *elemF;        // [NE][nn][dim] per-element internal forces (FP64)
+  double *elemF;        // [slot][dim] element internal forces, node-major:
+                        // node n's incident forces are slots n2e_ptr[n]..n2e_ptr[n+1]
   float *elemF32;       // mixed-precision variant of elemF
   double *cmin;         // Courant partial minima, one per Courant block
   int32_t n_cmin;

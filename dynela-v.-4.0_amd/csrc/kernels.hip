@@ -329,7 +329,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_fp64(JobDev *const *jobs, const 
         for (int j = 0; j < DIM; j++) f[i] -= s_dN[le][q][a][j] * s_sw[le][q][sym_idx(j, i)] * w;
       if constexpr (DIM == 2) f[0] -= tabN<NN>(q, a) * s_sw[le][q][5] / s_sw[le][q][7] * w;
     }
-    DNLG_GLOBAL double *out = gbl(C->elemF) + (e * NN + a) * DIM;
+    DNLG_GLOBAL double *out = gbl(C->elemF) + (int64_t)gbl(C->e2slot)[e * NN + a] * DIM;
 #pragma unroll
     for (int i = 0; i < DIM; i++) out[i] = f[i];
   }
@@ -385,8 +385,8 @@ __device__ __forceinline__ void apply_bcs(CJobDev *C, int64_t n, double time, do
 // Explicit::computePredictions + Explicit::explicitSolve (Explicit.C:438-536)
 // in one pass per node: the predicted field1 values are formed from field0
 // in registers (they are read nowhere else), the element forces of the node
-// are gathered in element order (same summation order as the reference's
-// scatter-add of Model.C:864-899), a = f/M, Chung-Hulbert correction, BCs,
+// are read from its contiguous incidence slots in element order (same
+// summation order as the reference's scatter-add of Model.C:864-899), a = f/M, Chung-Hulbert correction, BCs,
 // coordinates (the node displacement of Explicit.C:530-531 is x - x0, formed
 // at read-back).  The field0 BC values that the element pass reads were
 // applied by k_cycle_step.
@@ -407,10 +407,9 @@ __global__ __launch_bounds__(256) void k_correct(JobDev *const *jobs, const Task
 #pragma unroll
   for (int d = 0; d < DIM; d++) f[d] = 0.0;
   const int k0 = gbl(C->n2e_ptr)[n], k1 = gbl(C->n2e_ptr)[n + 1];
-  for (int k = k0; k < k1; k++) {
-    const int64_t slot = gbl(C->n2e_idx)[k];
+  for (int64_t k = k0; k < k1; k++) {
 #pragma unroll
-    for (int d = 0; d < DIM; d++) f[d] += (double)ef[slot * DIM + d];
+    for (int d = 0; d < DIM; d++) f[d] += (double)ef[k * DIM + d];
   }
   const double m = gbl(C->mass)[n];
   const double alphaM = C->alphaM, gam = C->gammaCH, beta = C->beta;
@@ -761,7 +760,7 @@ __global__ __launch_bounds__(256) void k_prep(JobDev *const *jobs, const Task *t
   }
   if (neg) set_status(J, DNLG_E_NEG_JACOBIAN);
 #pragma unroll
-  for (int a = 0; a < NN; a++) J->elemF[(e * NN + a) * DIM] = me[a];
+  for (int a = 0; a < NN; a++) J->elemF[(int64_t)J->e2slot[e * NN + a] * DIM] = me[a];
 }
 
 // Model::computeMassMatrix assembly (Model.C:537-590), gathered per node in
@@ -773,7 +772,7 @@ __global__ __launch_bounds__(256) void k_mass_gather(JobDev *const *jobs, const 
   const int64_t n = tk.first + threadIdx.x;
   if (n >= tk.end) return;
   double m = 0.0;
-  for (int k = J->n2e_ptr[n]; k < J->n2e_ptr[n + 1]; k++) m += J->elemF[(int64_t)J->n2e_idx[k] * DIM];
+  for (int64_t k = J->n2e_ptr[n]; k < J->n2e_ptr[n + 1]; k++) m += J->elemF[k * DIM];
   J->mass[n] = m;
 }
 
