@@ -336,18 +336,19 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
   H.omegaS = std::sqrt((12.0 * std::pow(1.0 + rb, 3) * (2.0 - rb)) /
                        (10.0 + 15.0 * rb - rb * rb + std::pow(rb, 3) - std::pow(rb, 4)));
   H.safety = prm->safety;
-  H.c.t = prm->t_start;
-  H.c.dt = 0.0;
-  H.c.up_to = prm->t_start;
-  H.c.inc = 0;
-  H.c.f0 = 0;
-  H.c.status = 0;
-  H.c.running = 1;
-  H.c.courant_due = 0;
-  H.c.density_due = 0;
-  H.c.preloop = 1;
-  H.c.pending = 0;
-  H.c.arrive = 0;
+  for (Control &c : H.c) {
+    c.t = prm->t_start;
+    c.dt = 0.0;
+    c.up_to = prm->t_start;
+    c.inc = 0;
+    c.f0 = 0;
+    c.running = 1;
+    c.courant_due = 0;
+    c.density_due = 0;
+    c.preloop = 1;
+    c.pending = 0;
+  }
+  H.sync.st[0] = H.sync.st[1] = 0;
 
   int st;
 #define ALLOC(ptr, count, init)                           \
@@ -464,7 +465,7 @@ int dnlg_job_sizes(const dnlg_job *job, int64_t *n_nodes, int64_t *n_elems, int3
 
 static int read_control(dnlg_job *job, Control *c) {
   HIPCHK(job->ctx, hipStreamSynchronize(job->ctx->stream));
-  HIPCHK(job->ctx, hipMemcpy(c, &job->dev->c, sizeof(Control), hipMemcpyDeviceToHost));
+  HIPCHK(job->ctx, hipMemcpy(c, &job->dev->c[0], sizeof(Control), hipMemcpyDeviceToHost));
   return DNLG_OK;
 }
 

@@ -28,6 +28,7 @@ struct Batch {
   TaskList cour[2];     // [hex?]
   TaskList node[2][2];  // [dim==3?][mixed?]
   TaskList step;        // cycle control + field0 BCs, over the BC node lists
+  TaskList close;       // end of a launch sequence, one block per job
   double *d_snap = nullptr;
   std::vector<double> h_snap;
   // captured graphs by (kind, increment phase), see graph_for
@@ -125,6 +126,9 @@ int build_batch(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, Batch &B) {
       for (int64_t f = 0, blk = 0; f < nb; f += 256, blk++) t.push_back(Task{i, (int32_t)blk, f, std::min(f + 256, nb)});
     }
     if ((st = upload_tasks(B, t, B.step))) return st;
+    t.clear();
+    for (int i = 0; i < n; i++) t.push_back(Task{i, 0, 0, 0});
+    if ((st = upload_tasks(B, t, B.close))) return st;
   }
   for (int d3 = 0; d3 < 2; d3++)
     for (int mx = 0; mx < 2; mx++) {
@@ -135,23 +139,29 @@ int build_batch(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, Batch &B) {
           idx.push_back(i);
           cnt.push_back(jobs[i]->N);
         }
-      if ((st = upload_tasks(B, make_tasks(idx, cnt, 256), B.node[d3][mx]))) return st;
+      if ((st = upload_tasks(B, make_tasks(idx, cnt, kNodeBlock), B.node[d3][mx]))) return st;
     }
   (void)ctx;
   return 0;
 }
 
 // Phases of one cycle.  pre = end of the previous cycle + updateTimes +
-// field0 BCs, element = the element kernels, post = predictor/corrector +
-// Courant.  The last cycle of a launch sequence stays open until
-// launch_close (end of step and time step only).
-void launch_pre(dnlg_ctx *ctx, Batch &B) { launch_cycle_step(B.d_jobs, B.step.d, B.step.n, 1, ctx->stream); }
+// field0 BCs (k_cycle_step), element = the element kernels, post =
+// predictor/corrector + Courant.  cs: the control slot of the cycle (see
+// Control); the last cycle of a launch sequence stays open until
+// launch_close (end of step and time step only), which leaves the record in
+// slot 0.
+void launch_pre(dnlg_ctx *ctx, Batch &B, int cs) {
+  launch_cycle_step(B.d_jobs, B.step.d, B.step.n, cs, 1, 1 - cs, ctx->stream);
+}
 
-void launch_close(dnlg_ctx *ctx, Batch &B) { launch_cycle_step(B.d_jobs, B.step.d, B.step.n, 0, ctx->stream); }
+void launch_close(dnlg_ctx *ctx, Batch &B, int cs) {
+  launch_cycle_step(B.d_jobs, B.close.d, B.close.n, cs, 0, 0, ctx->stream);
+}
 
 // ev: nullptr, or two events per element launch (up to 4 launches); each
 // launch is bracketed by event records on the engine's stream.
-int launch_element(dnlg_ctx *ctx, Batch &B, const hipEvent_t *ev) {
+int launch_element(dnlg_ctx *ctx, Batch &B, const hipEvent_t *ev, int cs, int derive) {
   hipStream_t s = ctx->stream;
   int k = 0;
   for (int hx = 0; hx < 2; hx++) {
@@ -159,12 +169,11 @@ int launch_element(dnlg_ctx *ctx, Batch &B, const hipEvent_t *ev) {
     for (int mx = 0; mx < 2; mx++) {
       const TaskList &L = B.elem[hx][mx];
       if (L.n <= 0) continue;
-      if (ev) hipEventRecord(ev[2 * k], s);
+      const hipEvent_t *e = ev ? &ev[2 * k] : nullptr;
       if (mx)
-        launch_elem_mixed(type, B.d_jobs, L.d, L.n, s);
+        launch_elem_mixed(type, B.d_jobs, L.d, L.n, cs, derive, s, e);
       else
-        launch_elem_fp64(type, B.d_jobs, L.d, L.n, s);
-      if (ev) hipEventRecord(ev[2 * k + 1], s);
+        launch_elem_fp64(type, B.d_jobs, L.d, L.n, cs, derive, s, e);
       k++;
     }
   }
@@ -172,20 +181,43 @@ int launch_element(dnlg_ctx *ctx, Batch &B, const hipEvent_t *ev) {
 }
 
 // courant = false leaves out the Courant kernel: only for a cycle known not
-// to be a time-step cycle of any running job (see graph_for).
-void launch_post(dnlg_ctx *ctx, Batch &B, bool courant = true) {
+// to be a time-step cycle of any running job (see launch_cycles).  fold: the
+// corrector also writes the next cycle's field0 BCs, and the next cycle's
+// element launch derives its control record.
+void launch_post(dnlg_ctx *ctx, Batch &B, int cs, bool courant, bool fold = false) {
   hipStream_t s = ctx->stream;
   for (int d3 = 0; d3 < 2; d3++)
     for (int mx = 0; mx < 2; mx++)
-      launch_correct(d3 ? 3 : 2, mx == 1, B.d_jobs, B.node[d3][mx].d, B.node[d3][mx].n, s);
+      launch_correct(d3 ? 3 : 2, mx == 1, B.d_jobs, B.node[d3][mx].d, B.node[d3][mx].n, cs, fold ? 1 : 0, s);
   if (courant)
-    for (int hx = 0; hx < 2; hx++) launch_courant(hx ? 3 : 2, B.d_jobs, B.cour[hx].d, B.cour[hx].n, 0, s);
+    for (int hx = 0; hx < 2; hx++) launch_courant(hx ? 3 : 2, B.d_jobs, B.cour[hx].d, B.cour[hx].n, cs, 0, s);
 }
 
-void launch_cycle(dnlg_ctx *ctx, Batch &B, bool courant = true) {
-  launch_pre(ctx, B);
-  launch_element(ctx, B, nullptr);
-  launch_post(ctx, B, courant);
+// Cycles j0 .. j1-1 of a launch sequence whose running jobs share the
+// increment phase `phase` = increment mod freq (or phase < 0: no common
+// phase), starting from control slot cs; returns the slot of the last
+// cycle.  Cycle j has increment phase + j + 1 and updates the time step iff
+// that is a multiple of freq.  With a common phase, a cycle without a
+// time-step update that is not the sequence's last hands the opening of the
+// next cycle to the corrector (field0 BCs) and the next element launch
+// (control record), so k_cycle_step runs where the sequence starts and after
+// time-step cycles only.  Every cycle moves the record to the other slot.
+int launch_cycles(dnlg_ctx *ctx, Batch &B, int j0, int j1, int phase, int freq, int cs) {
+  auto due = [&](int j) { return phase < 0 || (phase + j + 1) % freq == 0; };
+  bool folded = false;
+  for (int j = j0; j < j1; j++) {
+    if (folded) {
+      launch_element(ctx, B, nullptr, cs, 1);
+      cs = 1 - cs;
+    } else {
+      launch_pre(ctx, B, cs);
+      cs = 1 - cs;
+      launch_element(ctx, B, nullptr, cs, 0);
+    }
+    folded = phase >= 0 && j + 1 < j1 && !due(j);
+    launch_post(ctx, B, cs, due(j), folded);
+  }
+  return cs;
 }
 
 // Explicit.C:143-178: reference Jacobian (detJ0), lumped mass, forced time
@@ -197,12 +229,12 @@ int run_preloop(dnlg_ctx *ctx, Batch &B) {
     launch_prep(hx ? DNLG_EL_HEX8N_3D : DNLG_EL_QUA4N_AX, B.d_jobs, B.prep[hx].d, B.prep[hx].n, s);
   for (int d3 = 0; d3 < 2; d3++)
     for (int mx = 0; mx < 2; mx++) launch_mass_gather(d3 ? 3 : 2, B.d_jobs, B.node[d3][mx].d, B.node[d3][mx].n, s);
-  for (int hx = 0; hx < 2; hx++) launch_courant(hx ? 3 : 2, B.d_jobs, B.cour[hx].d, B.cour[hx].n, 1, s);
+  for (int hx = 0; hx < 2; hx++) launch_courant(hx ? 3 : 2, B.d_jobs, B.cour[hx].d, B.cour[hx].n, 0, 1, s);
   launch_force_dt(B.d_jobs, n, s);
   for (int hx = 0; hx < 2; hx++) {
     const int type = hx ? DNLG_EL_HEX8N_3D : DNLG_EL_QUA4N_AX;
-    launch_elem_fp64(type, B.d_jobs, B.elem[hx][0].d, B.elem[hx][0].n, s);
-    launch_elem_mixed(type, B.d_jobs, B.elem[hx][1].d, B.elem[hx][1].n, s);
+    launch_elem_fp64(type, B.d_jobs, B.elem[hx][0].d, B.elem[hx][0].n, 0, 0, s);
+    launch_elem_mixed(type, B.d_jobs, B.elem[hx][1].d, B.elem[hx][1].n, 0, 0, s);
   }
   if (hipGetLastError() != hipSuccess) return DNLG_E_HIP;
   return 0;
@@ -245,8 +277,11 @@ enum GraphKind { kWhole = 0, kPre = 1, kPost = 2, kRest = 3 };
 // only in the cycles whose increment is a multiple of freq, so the phase is
 // the same at the end of the graph.
 int graph_len(int cycles, int phase, int freq) {
-  if (phase < 0) return cycles;
-  return freq * std::max(1, (cycles + freq - 1) / freq);
+  int len = phase < 0 ? cycles : freq * std::max(1, (cycles + freq - 1) / freq);
+  // an even count: each cycle moves the control record to the other slot,
+  // and every graph starts and ends with it in slot 0
+  if (len % 2) len += phase < 0 ? 1 : freq;
+  return len;
 }
 
 hipGraphExec_t graph_for(dnlg_ctx *ctx, Batch &B, int kind, int phase, int freq, int &st) {
@@ -254,22 +289,19 @@ hipGraphExec_t graph_for(dnlg_ctx *ctx, Batch &B, int kind, int phase, int freq,
   auto it = B.graphs.find(key);
   if (it != B.graphs.end()) return it->second;
   const int len = graph_len(ctx->graph_cycles, phase, freq);
-  // cycle j (0-based) of a graph starting at phase p has increment
-  // p + j + 1 (mod freq)
-  auto due = [&](int j) { return phase < 0 || (phase + j + 1) % freq == 0; };
   hipGraphExec_t g = nullptr;
   switch (kind) {
     case kWhole:
-      st = capture(ctx, [&] { for (int j = 0; j < len; j++) launch_cycle(ctx, B, due(j)); }, &g);
+      st = capture(ctx, [&] { launch_cycles(ctx, B, 0, len, phase, freq, 0); }, &g);
       break;
-    case kPre:
-      st = capture(ctx, [&] { launch_pre(ctx, B); }, &g);
+    case kPre:  // slot 0 -> 1
+      st = capture(ctx, [&] { launch_pre(ctx, B, 0); }, &g);
       break;
     case kPost:
-      st = capture(ctx, [&] { launch_post(ctx, B, true); }, &g);
+      st = capture(ctx, [&] { launch_post(ctx, B, 1, true); }, &g);
       break;
-    default:  // the len - 1 cycles after a sampled one
-      st = capture(ctx, [&] { for (int j = 1; j < len; j++) launch_cycle(ctx, B, due(j)); }, &g);
+    default:  // the len - 1 cycles after a sampled one, slot 1 -> 0
+      st = capture(ctx, [&] { launch_cycles(ctx, B, 1, len, phase, freq, 1); }, &g);
       break;
   }
   if (st) return nullptr;
@@ -405,6 +437,7 @@ int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vec
         return st;
       }
     }
+    int cs = 0;  // control slot of the cycle in flight (slot 0 between sequences)
     if (timed) {
       // units of one sampled cycle + the rest of a graph's cycles
       const int per = use_graph ? graph_len(ctx->graph_cycles, phase, freq) : 1;
@@ -420,16 +453,19 @@ int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vec
         if (use_graph) {
           if (hipGraphLaunch(g_pre, s) != hipSuccess) return DNLG_E_HIP;
         } else {
-          launch_pre(ctx, *B);
+          launch_pre(ctx, *B, cs);
         }
-        const int used = launch_element(ctx, *B, &ctx->events[base]);
+        cs = 1 - cs;
+        const int used = launch_element(ctx, *B, &ctx->events[base], cs, 0);
         ev_used.push_back(used);
         base += 2 * used;
         if (use_graph) {
+          // the sampled cycle runs in slot 1; the rest graph ends in slot 0
           if (hipGraphLaunch(g_post, s) != hipSuccess) return DNLG_E_HIP;
           if (g_rest && hipGraphLaunch(g_rest, s) != hipSuccess) return DNLG_E_HIP;
+          cs = 0;
         } else {
-          launch_post(ctx, *B);
+          launch_post(ctx, *B, cs, true);
         }
       }
     } else if (use_graph) {
@@ -442,9 +478,9 @@ int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vec
           return DNLG_E_HIP;
         }
     } else {
-      for (int k = 0; k < chunk; k++) launch_cycle(ctx, *B);
+      cs = launch_cycles(ctx, *B, 0, chunk, phase, freq, 0);
     }
-    launch_close(ctx, *B);
+    launch_close(ctx, *B, cs);
     const hipError_t le = hipGetLastError();
     if (le != hipSuccess) {
       ctx->err = std::string("kernel launch failed: ") + hipGetErrorString(le);

@@ -145,7 +145,7 @@ __device__ __forceinline__ float group_sum(float v) {
 __device__ __forceinline__ float sgn(int b, int d) { return ((b >> d) & 1) ? 1.f : -1.f; }
 
 template <int NN, int DIM, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const Task *tasks) {
+__global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const Task *tasks, int ctl, int derive) {
   constexpr int NB = DIM;               // lanes per element = 2^DIM = NN
   constexpr float inv2D = 1.f / NN;     // 2^-D
   const float g = 0.577350269189625764f;  // Gauss abscissa 1/sqrt(3)
@@ -153,7 +153,9 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
   const Task tk = load_task(tasks);
   JobDev *J = load_job(jobs, tk.job);
   CJobDev *C = (CJobDev *)J;
-  if (!C->c.running || C->c.status != 0) return;
+  const Control cc = cycle_control(J, C, ctl, derive, tk.blk);
+  if (!cc.running) return;
+  const int par = (int)(cc.inc & 1);
   const int lane = threadIdx.x & 63;
   const int b = lane & (NN - 1);            // lane within the element group
   const int base = lane & ~(NN - 1);        // first lane of the group
@@ -163,7 +165,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
 
   const int ref = b ^ ((b >> 1) & 1);       // node / IP index in the reference order
   const int64_t N = C->N;
-  const int f0 = C->c.f0;
+  const int f0 = cc.f0;
 
   // ---- node b of the element: coordinates relative to node 0, increment
   float xl[DIM], ul[DIM];
@@ -269,7 +271,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
     inv[1][0] = -Jm[1][0] * id;
     inv[1][1] = Jm[0][0] * id;
   }
-  if (valid && detJ < 0.f) set_status(J, DNLG_E_NEG_JACOBIAN);
+  if (valid && detJ < 0.f) set_status(J, DNLG_E_NEG_JACOBIAN, par);
   // H = du/dx = G J^-1 (Element3D.C:172-195, ElementAxi.C:78-100)
   float H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
@@ -307,7 +309,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
   if (valid) {
     const float pressure = (sig[0] + sig[3] + sig[5]) * (1.f / 3.f) + (float)C->K * inc;
 
-    const double dt = C->c.dt;
+    const double dt = cc.dt;
     const float TwoGf = (float)C->twoG;
     const double TwoG32 = kSqrt32 * C->twoG;
     const double heatFrac = C->heatFrac;
@@ -378,7 +380,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
         epsp = ep0f + s23 * g;
         T = Tc + 0.5f * g * heatf * (s23 * yieldf + Snorm0);
         if (++iterate > kItMax) {
-          set_status(J, DNLG_E_NR_DIVERGED);
+          set_status(J, DNLG_E_NR_DIVERGED, par);
           break;
         }
       }
@@ -413,7 +415,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
 #pragma unroll
     for (int c = 0; c < 6; c++) gbl(C->sig32)[c * G + gi] = sig[c];
     // Element::computeDensity (Element.C:1243-1256)
-    if (C->c.density_due) {
+    if (cc.density_due) {
       double rr = C->rho0 * (double)gbl(C->detJ0f)[gi] / (double)detJ;
       if constexpr (DIM == 2) rr /= (double)radius;
       gbl(C->rho)[gi] = rr;

@@ -37,24 +37,37 @@ struct Boundary {
   double rmin, rmax;
 };
 
-// Solver scalars of one job (Solver.h:45-50 plus engine flags).
+// Solver scalars of one cycle of a job (Solver.h:45-50 plus engine flags).
+// A job keeps two records: while a launch sequence runs, the host names for
+// each launch the slot that holds its cycle's record, and the kernel that
+// opens a cycle writes the other slot (so no launch rewrites a record that
+// its own blocks read).  Between sequences the record is in slot 0.
 struct Control {
   double t;      // model->currentTime
   double dt;     // Solver::timeStep (persists, clipped at the end)
   double up_to;  // target of the current advance
   int64_t inc;   // Solver::currentIncrement
   int32_t f0;    // which ping-pong slot is the reference's field0
-  int32_t status;       // DNLG_OK or an error code (sticky)
   int32_t running;      // this cycle advances the job
   int32_t courant_due;  // Courant recompute after this cycle
   int32_t density_due;  // element kernel writes IP densities this cycle
   int32_t preloop;      // element pass of Explicit.C:158-178
   int32_t pending;      // the last cycle ran and awaits its end-of-step
-  uint32_t arrive;      // blocks of the step kernel done with this record
+};
+
+// Error codes (DNLG_E_*, sticky, the first one wins), by the parity of the
+// increment of the cycle that raised them: a launch that opens cycle i + 1
+// reads cycle i's word, which no launch of cycle i + 1 writes.  A job stops
+// after its first failing cycle, so at most one word is ever set.  On a
+// cache line of its own: written by device atomics while other blocks store
+// control records.
+struct alignas(128) Sync {
+  int32_t st[2];
 };
 
 struct JobDev {
-  Control c;
+  Control c[2];
+  Sync sync;
   // sizes
   int32_t type, dim, nn, nip;
   int64_t N, NE, G;  // nodes, elements, integration points
@@ -106,6 +119,10 @@ template <typename T>
 __device__ __forceinline__ DNLG_GLOBAL T *gbl(T *p) {
   return (DNLG_GLOBAL T *)p;
 }
+
+// Threads per block of the node kernels (corrector, mass assembly): one node
+// per thread, contiguous node ranges per block.
+constexpr int kNodeBlock = 256;
 
 // One block of work: job index into the batch, local block index within the
 // job, and the half-open item range [first, end).

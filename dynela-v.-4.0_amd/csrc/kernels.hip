@@ -1,10 +1,13 @@
 // kernels.hip -- CDNA4 kernels of one explicit cycle (FP64 element path,
 // nodal kernels, Courant minimum, control) and of the pre-loop set-up.
 //
-// Cycle = Explicit::solve loop body (Explicit.C:184-252), four launches:
+// Cycle = Explicit::solve loop body (Explicit.C:184-252), up to four launches:
 //   k_cycle_step    endStep + computeTimeStep of the previous cycle
 //                   (Explicit.C:539-552, Solver.C:196-236), updateTimes
-//                   (Explicit.C:414-427), field0 BCs of computePredictions
+//                   (Explicit.C:414-427), field0 BCs of computePredictions;
+//                   inside a captured sequence of cycles that share their
+//                   time-step phase the corrector or the Courant kernel
+//                   does this at its end instead
 //   k_elem_*        computeStrains, computePressure, computeStress,
 //                   computeFinalRotation, computeInternalForces,
 //                   computeDensity (Model.C:800-899, Explicit.C:555-562)
@@ -12,7 +15,8 @@
 //                   (Explicit.C:438-536), gather-based force assembly in
 //                   element order (Model.C:864-899)
 //   k_courant       computeCourantTimeStep (Model.C:636-670), per-block min
-//                   (every dt_freq-th cycle; an early exit otherwise)
+//                   (every dt_freq-th cycle; an early exit otherwise, or
+//                   not launched when no running job is due)
 // The element Jacobian of Explicit.C:245-246 is recomputed inside the next
 // cycle's element kernel from the same coordinates, so nothing per-IP
 // geometric is stored between cycles.
@@ -50,7 +54,143 @@ __device__ __forceinline__ double tabD(int p, int a, int j) {
 
 __device__ __forceinline__ int sym_idx(int i, int j) { return (i == j) ? (i == 0 ? 0 : (i == 1 ? 3 : 5)) : (i + j == 1 ? 1 : (i + j == 2 ? 2 : 4)); }
 
-__device__ __forceinline__ void set_status(JobDev *J, int st) { atomicCAS(&J->c.status, 0, st); }
+// Sticky job status of the cycle with increment parity `par`.
+__device__ __forceinline__ void set_status(JobDev *J, int st, int par) { atomicCAS(&J->sync.st[par], 0, st); }
+
+// The job's status between launches of one cycle's predecessor and its own
+// (no launch that could raise an error runs beside the reader).
+__device__ __forceinline__ int job_status(CJobDev *C) {
+  const int a = C->sync.st[0];
+  return a != 0 ? a : C->sync.st[1];
+}
+
+// ---------------------------------------------------------------------------
+// Cycle control.  Between two cycles a job's control record goes through
+//   close  Explicit::endStep (Explicit.C:539-552): t += dt, field1 becomes
+//          field0; then Solver::computeTimeStep (Solver.C:196-236, Courant
+//          branch: omega_max = 2 / dt_crit) when the time step was due;
+//   open   Explicit::updateTimes (Explicit.C:414-427): last-step clip,
+//          increment, and the engine's per-cycle flags.
+// k_cycle_step does it where a launch sequence starts, after time-step
+// cycles, and (close only) where a sequence ends; between the other cycles of
+// a sequence the next cycle's element kernel derives the new record itself
+// (cycle_control) and the corrector has already written the new cycle's
+// field0 BC values.
+// ---------------------------------------------------------------------------
+// Read through the constant space (scalar loads); only for a record that no
+// block of the reading launch writes.
+template <typename R = const DNLG_CONST Control>
+__device__ __forceinline__ Control load_control(CJobDev *C, int slot) {
+  R &s = ((R *)C->c)[slot];
+  Control c;
+  c.t = s.t;
+  c.dt = s.dt;
+  c.up_to = s.up_to;
+  c.inc = s.inc;
+  c.f0 = s.f0;
+  c.running = s.running;
+  c.courant_due = s.courant_due;
+  c.density_due = s.density_due;
+  c.preloop = s.preloop;
+  c.pending = s.pending;
+  return c;
+}
+
+__device__ __forceinline__ void store_control(JobDev *J, int slot, const Control &c) {
+  DNLG_GLOBAL Control &d = *gbl(&J->c[slot]);
+  d.t = c.t;
+  d.dt = c.dt;
+  d.up_to = c.up_to;
+  d.inc = c.inc;
+  d.f0 = c.f0;
+  d.running = c.running;
+  d.courant_due = c.courant_due;
+  d.density_due = c.density_due;
+  d.preloop = c.preloop;
+  d.pending = c.pending;
+}
+
+// Close the cycle that ran last (if any) and, with `open`, open the next.
+// status: the job's status after that cycle; dt_crit > 0: the job's minimum
+// element L / c for the time-step update.  Returns whether the field0 BC
+// values of the new cycle differ from what the corrector left in field0
+// (first cycle of an advance, a ramped speed BC, or a speed BC under a
+// changed time step).
+__device__ __forceinline__ bool close_open(CJobDev *C, Control &c, int status, double dt_crit, bool open) {
+  const double dt_last = c.dt;
+  const bool resumed = !c.pending;
+  if (c.pending && status == 0) {
+    c.t += c.dt;
+    c.f0 = 1 - c.f0;
+    if (dt_crit > 0.0) {
+      const double fmax = 2.0 / dt_crit;
+      c.dt = C->safety * C->omegaS / fmax;
+    }
+  }
+  c.pending = 0;
+  bool bc_work = false;
+  if (open) {
+    c.preloop = 0;
+    c.running = (status == 0 && c.t < c.up_to) ? 1 : 0;
+    if (c.running) {
+      if (c.t + c.dt > c.up_to) c.dt = c.up_to - c.t;
+      c.inc++;
+      c.courant_due = (c.inc % C->dt_freq == 0) ? 1 : 0;
+      // densities are consumed by the Courant step and read back at the end
+      c.density_due = (c.courant_due || !(c.t + c.dt < c.up_to)) ? 1 : 0;
+      c.pending = 1;
+      bc_work = resumed || (C->bc_flags & 2) || ((C->bc_flags & 1) && c.dt != dt_last);
+    }
+  } else {
+    c.running = 0;
+  }
+  return bc_work;
+}
+
+// Whether the Courant kernel computed partial minima in the cycle that ran
+// last (Explicit.C:243-251: every dt_freq increments, while t_{n+1} < t_up_to).
+__device__ __forceinline__ bool courant_ran(const Control &c, int status) {
+  return c.pending && status == 0 && c.courant_due && c.t + c.dt < c.up_to;
+}
+
+// The record of the cycle that a launch works on.  derive == 0: slot cs
+// holds it.  derive != 0 (the first launch of a cycle that no k_cycle_step
+// opened, after a cycle without a time-step update): slot cs holds the
+// previous cycle; every block derives the new record from it and the job's
+// block 0 stores it in slot 1 - cs for the cycle's later launches.  The
+// previous cycle's failures are in its own parity word, which no launch of
+// the new cycle writes; a job that did not run in it stays stopped (it
+// finished or failed), so reading both words is then race-free too.
+// The record is uniform over the block; it is returned in scalar registers
+// (the FP64 arithmetic of the derivation runs on the vector unit).
+__device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int64_t uniform_l(int64_t v) {
+  const unsigned lo = (unsigned)uniform_i((int)(v & 0xffffffffLL));
+  const unsigned hi = (unsigned)uniform_i((int)(v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ double uniform_d(double v) { return __longlong_as_double(uniform_l(__double_as_longlong(v))); }
+
+__device__ __forceinline__ Control cycle_control(JobDev *J, CJobDev *C, int cs, int derive, int blk) {
+  Control c = load_control(C, cs);
+  if (derive) {
+    const int st0 = C->sync.st[0], st1 = C->sync.st[1];
+    const int status = c.pending ? ((c.inc & 1) ? st1 : st0) : (st0 != 0 ? st0 : st1);
+    close_open(C, c, status, 0.0, true);
+    if (blk == 0 && threadIdx.x == 0) store_control(J, 1 - cs, c);
+    c.t = uniform_d(c.t);
+    c.dt = uniform_d(c.dt);
+    c.inc = uniform_l(c.inc);
+    c.f0 = uniform_i(c.f0);
+    c.running = uniform_i(c.running);
+    c.courant_due = uniform_i(c.courant_due);
+    c.density_due = uniform_i(c.density_due);
+    c.preloop = uniform_i(c.preloop);
+    c.pending = uniform_i(c.pending);
+  }
+  return c;
+}
+
 
 // ---------------------------------------------------------------------------
 // Element kernel, FP64.  One lane per integration point (NIP lanes per
@@ -60,7 +200,7 @@ __device__ __forceinline__ void set_status(JobDev *J, int st) { atomicCAS(&J->c.
 // order (IP, then direction j).
 // ---------------------------------------------------------------------------
 template <int NN, int DIM, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_elem_fp64(JobDev *const *jobs, const Task *tasks) {
+__global__ __launch_bounds__(BLOCK) void k_elem_fp64(JobDev *const *jobs, const Task *tasks, int cs, int derive) {
   constexpr int NIP = NN;
   constexpr int EPB = BLOCK / NIP;
   __shared__ double s_x[EPB][NN][DIM];
@@ -72,13 +212,15 @@ __global__ __launch_bounds__(BLOCK) void k_elem_fp64(JobDev *const *jobs, const 
   const Task tk = load_task(tasks);
   JobDev *J = load_job(jobs, tk.job);
   CJobDev *C = (CJobDev *)J;
-  if (!C->c.running || C->c.status != 0) return;
+  const Control cc = cycle_control(J, C, cs, derive, tk.blk);
+  if (!cc.running) return;
+  const int par = (int)(cc.inc & 1);
   const int le = threadIdx.x / NIP;
   const int p = threadIdx.x % NIP;
   const int64_t e = tk.first + le;
   const bool valid = e < tk.end;
   const int64_t N = C->N;
-  const int f0 = C->c.f0;
+  const int f0 = cc.f0;
 
   // stage the element's nodes: lane p brings node p
   if (valid) {
@@ -133,7 +275,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_fp64(JobDev *const *jobs, const 
       inv[1][0] = -Jm[1][0] * id;
       inv[1][1] = Jm[0][0] * id;
     }
-    if (detJ < 0.0) set_status(J, DNLG_E_NEG_JACOBIAN);
+    if (detJ < 0.0) set_status(J, DNLG_E_NEG_JACOBIAN, par);
 #pragma unroll
     for (int a = 0; a < NN; a++)
 #pragma unroll
@@ -179,7 +321,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_fp64(JobDev *const *jobs, const 
     const double pressure = (sig[0] + sig[3] + sig[5]) / 3.0 + C->K * inc;
 
     // Element::computeStress (Element.C:407-622)
-    const double dt = C->c.dt;
+    const double dt = cc.dt;
     const double TwoG = C->twoG;
     const double TwoG32 = kSqrt32 * TwoG;
     const double heatFrac = C->heatFrac;
@@ -245,7 +387,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_fp64(JobDev *const *jobs, const 
         epsp = ep0 + kSqrt23 * gamma;
         T = Tcur + 0.5 * gamma * heatFrac * (kSqrt23 * yield + Snorm0);
         if (++iterate > kItMax) {
-          set_status(J, DNLG_E_NR_DIVERGED);
+          set_status(J, DNLG_E_NR_DIVERGED, par);
           break;
         }
       }
@@ -296,7 +438,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_fp64(JobDev *const *jobs, const 
       for (int c = 0; c < 6; c++) gbl(C->epsP)[c * G + g] = t6[c];
     }
     // Element::computeDensity (Element.C:1243-1256), on x_n
-    if (C->c.density_due) {
+    if (cc.density_due) {
       double rr = C->rho0 * gbl(C->detJ0)[g] / detJ;
       if constexpr (DIM == 2) rr /= radius;
       gbl(C->rho)[g] = rr;
@@ -386,38 +528,74 @@ __device__ __forceinline__ void apply_bcs(CJobDev *C, int64_t n, double time, do
 // in one pass per node: the predicted field1 values are formed from field0
 // in registers (they are read nowhere else), the element forces of the node
 // are read from its contiguous incidence slots in element order (same
-// summation order as the reference's scatter-add of Model.C:864-899), a = f/M, Chung-Hulbert correction, BCs,
-// coordinates (the node displacement of Explicit.C:530-531 is x - x0, formed
-// at read-back).  The field0 BC values that the element pass reads were
-// applied by k_cycle_step.
+// summation order as the reference's scatter-add of Model.C:864-899), a = f/M,
+// Chung-Hulbert correction, BCs, coordinates (the node displacement of
+// Explicit.C:530-531 is x - x0, formed at read-back).
+// fold != 0 (a cycle without a time-step update, followed by the next
+// cycle's element pass with derive != 0): a BC node's thread also writes the
+// next cycle's field0 BC values, which k_cycle_step would otherwise write.
+// Adds the forces of incidence slots k .. k+B-1 to f in slot order, with the
+// B slots' loads issued together.
+template <int B, int DIM, typename FT>
+__device__ __forceinline__ void add_slots(const DNLG_GLOBAL FT *ef, int64_t k, double f[DIM]) {
+  FT v[B][DIM];
+#pragma unroll
+  for (int i = 0; i < B; i++)
+#pragma unroll
+    for (int d = 0; d < DIM; d++) v[i][d] = ef[(k + i) * DIM + d];
+#pragma unroll
+  for (int i = 0; i < B; i++)
+#pragma unroll
+    for (int d = 0; d < DIM; d++) f[d] += (double)v[i][d];
+}
+
 template <int DIM, typename FT>
-__global__ __launch_bounds__(256) void k_correct(JobDev *const *jobs, const Task *tasks) {
+__global__ __launch_bounds__(kNodeBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_correct(
+    JobDev *const *jobs, const Task *tasks, int cs, int fold) {
   const Task tk = load_task(tasks);
   JobDev *J = load_job(jobs, tk.job);
   CJobDev *C = (CJobDev *)J;
-  if (!C->c.running || C->c.status != 0) return;
+  const Control c = load_control(C, cs);
+  const int st0 = C->sync.st[0], st1 = C->sync.st[1];
+  if (!c.running || ((c.inc & 1) ? st1 : st0) != 0) return;
   const int64_t n = tk.first + threadIdx.x;
   if (n >= tk.end) return;
   const int64_t N = C->N;
-  const int a = C->c.f0, b = 1 - a;
-  const double h = C->c.dt;
+  const int a = c.f0, b = 1 - a;
+  const double h = c.dt;
   const DNLG_GLOBAL FT *ef;
   if constexpr (sizeof(FT) == 8) ef = gbl(C->elemF); else ef = gbl(C->elemF32);
-  double f[DIM];
-#pragma unroll
-  for (int d = 0; d < DIM; d++) f[d] = 0.0;
   const int k0 = gbl(C->n2e_ptr)[n], k1 = gbl(C->n2e_ptr)[n + 1];
-  for (int64_t k = k0; k < k1; k++) {
-#pragma unroll
-    for (int d = 0; d < DIM; d++) f[d] += (double)ef[k * DIM + d];
-  }
+  // the node's own values first, then the incidence slots in batches
   const double m = gbl(C->mass)[n];
-  const double alphaM = C->alphaM, gam = C->gammaCH, beta = C->beta;
-  double ua[DIM], va[DIM], aa[DIM], ub[DIM], vb[DIM], ab[DIM];
+  double va[DIM], aa[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; d++) {
     va[d] = gbl(C->fv[a])[d * N + n];
     aa[d] = gbl(C->fa[a])[d * N + n];
+  }
+  double f[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; d++) f[d] = 0.0;
+  int64_t k = k0;
+  if constexpr (sizeof(FT) == 4) {
+    for (; k + 8 <= k1; k += 8) add_slots<8, DIM>(ef, k, f);
+    if (k + 4 <= k1) {
+      add_slots<4, DIM>(ef, k, f);
+      k += 4;
+    }
+  } else {  // FP64 slots: batches of 4 keep the kernel within its registers
+    for (; k + 4 <= k1; k += 4) add_slots<4, DIM>(ef, k, f);
+  }
+  if (k + 2 <= k1) {
+    add_slots<2, DIM>(ef, k, f);
+    k += 2;
+  }
+  if (k < k1) add_slots<1, DIM>(ef, k, f);
+  const double alphaM = C->alphaM, gam = C->gammaCH, beta = C->beta;
+  double ua[DIM], ub[DIM], vb[DIM], ab[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
     // predictor (Explicit.C:458-466)
     ub[d] = h * (va[d] + (0.5 - beta) * h * aa[d]);
     vb[d] = va[d] + (1.0 - gam) * h * aa[d];
@@ -432,7 +610,7 @@ __global__ __launch_bounds__(256) void k_correct(JobDev *const *jobs, const Task
   if (has_bc) {
 #pragma unroll
     for (int d = 0; d < DIM; d++) ua[d] = gbl(C->fu[a])[d * N + n];
-    apply_bcs<DIM>(C, n, C->c.t, h, ua, va, aa, ub, vb, ab);
+    apply_bcs<DIM>(C, n, c.t, h, ua, va, aa, ub, vb, ab);
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
       gbl(C->fu[a])[d * N + n] = ua[d];
@@ -443,14 +621,26 @@ __global__ __launch_bounds__(256) void k_correct(JobDev *const *jobs, const Task
   bool finite = true;
 #pragma unroll
   for (int d = 0; d < DIM; d++) {
-    gbl(C->fu[b])[d * N + n] = ub[d];
-    gbl(C->fv[b])[d * N + n] = vb[d];
-    gbl(C->fa[b])[d * N + n] = ab[d];
     const double xn = gbl(C->x)[d * N + n] + ub[d];
     gbl(C->x)[d * N + n] = xn;
     finite = finite && isfinite(xn) && isfinite(vb[d]);
   }
-  if (!finite) set_status(J, DNLG_E_NONFINITE);
+  if (has_bc && fold) {
+    // the next cycle's field0 BCs on field b, as k_cycle_step would write
+    // them if the job keeps running (if it stops, field b is not read)
+    Control nx = c;
+    if (close_open(C, nx, 0, 0.0, true)) {
+      double u2[DIM], v2[DIM], a2[DIM];
+      apply_bcs<DIM>(C, n, nx.t, nx.dt, ub, vb, ab, u2, v2, a2);
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    gbl(C->fu[b])[d * N + n] = ub[d];
+    gbl(C->fv[b])[d * N + n] = vb[d];
+    gbl(C->fa[b])[d * N + n] = ab[d];
+  }
+  if (!finite) set_status(J, DNLG_E_NONFINITE, (int)(c.inc & 1));
 }
 
 // ---------------------------------------------------------------------------
@@ -524,15 +714,17 @@ __device__ __forceinline__ double element_char_length(CJobDev *C, int64_t e) {
 }
 
 template <int DIM>
-__global__ __launch_bounds__(256) void k_courant(JobDev *const *jobs, const Task *tasks, int force) {
+__global__ __launch_bounds__(256) void k_courant(JobDev *const *jobs, const Task *tasks, int cs, int force) {
   __shared__ double s_min[4];
   const Task tk = load_task(tasks);
   JobDev *J = load_job(jobs, tk.job);
   CJobDev *C = (CJobDev *)J;
   if (!force) {
     // Explicit.C:243-251: only while t_{n+1} < t_up_to, every dt_freq increments
-    if (!C->c.running || C->c.status != 0 || !C->c.courant_due) return;
-    if (!(C->c.t + C->c.dt < C->c.up_to)) return;
+    const Control c = load_control(C, cs);
+    const int st0 = C->sync.st[0], st1 = C->sync.st[1];
+    if (!c.running || ((c.inc & 1) ? st1 : st0) != 0 || !c.courant_due) return;
+    if (!(c.t + c.dt < c.up_to)) return;
   }
   const int64_t e = tk.first + threadIdx.x;
   double v = 1.0e300;
@@ -558,7 +750,7 @@ __global__ __launch_bounds__(256) void k_courant(JobDev *const *jobs, const Task
 }
 
 // ---------------------------------------------------------------------------
-// Cycle control.
+// Cycle step kernel.
 // ---------------------------------------------------------------------------
 // BC predictor on field0 of one node (the field0 half of
 // Explicit::computePredictions, Explicit.C:468-480): constant BCs at the
@@ -583,102 +775,59 @@ __device__ __forceinline__ void bc_field0(CJobDev *C, int64_t n, int f0, double 
   }
 }
 
-// One launch between two cycles, per job:
-//   close  Explicit::endStep (Explicit.C:539-552) and Solver::computeTimeStep
-//          (Solver.C:196-236, Courant branch: omega_max = 2 / dt_crit, the
-//          minimum over the per-block partials of k_courant) for the cycle
-//          that ran last, if any;
-//   open   (open != 0) Explicit::updateTimes (Explicit.C:414-427), the
-//          per-cycle flags and the field0 BCs of the job's BC nodes.
-// Every block of a job derives the new control record from the old one; the
-// last block to arrive (device-scope counter) stores it, after every block of
-// the job has read the old one.  Blocks per job: max(1, ceil(n_bcn / 256)).
-__global__ __launch_bounds__(256) void k_cycle_step(JobDev *const *jobs, const Task *tasks, int open) {
-  __shared__ double s_min[4];
+// Minimum of the job's per-block Courant partials, over the whole block.
+__device__ __forceinline__ double block_min_partials(CJobDev *C) {
+  __shared__ double s_red[4];
+  double v = 1.0e300;
+  for (int k = threadIdx.x; k < C->n_cmin; k += blockDim.x) v = fmin(v, gbl(C->cmin)[k]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off));
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double m = s_red[0];
+  for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmin(m, s_red[w]);
+  return m;
+}
+
+// One launch between two cycles, per job: close the cycle that ran last, if
+// any (with the Courant time step when it was due: the minimum over the
+// per-block partials of k_courant), read from slot cs, and, with open != 0,
+// open the next one and write the field0 BCs of the job's BC nodes.  The
+// job's block 0 stores the new record in slot `out`.  Open: out = 1 - cs,
+// max(1, ceil(n_bcn / 256)) blocks per job.  Close only: out = 0, one block
+// per job (out may be cs: the block's waves have all read it first).
+__global__ __launch_bounds__(256) void k_cycle_step(JobDev *const *jobs, const Task *tasks, int cs, int open,
+                                                    int out) {
   const Task tk = load_task(tasks);
   JobDev *J = load_job(jobs, tk.job);
   CJobDev *C = (CJobDev *)J;
-  double t = C->c.t, dt = C->c.dt;
-  const double dt_last = dt;
-  const bool resumed = !C->c.pending;  // first cycle of an advance
-  const double up_to = C->c.up_to;
-  int64_t inc = C->c.inc;
-  int f0 = C->c.f0;
-  const int status = C->c.status;
-  int running = C->c.running, courant_due = C->c.courant_due, density_due = C->c.density_due;
-  int preloop = C->c.preloop, pending = C->c.pending;
-  if (pending && status == 0) {
-    t += dt;
-    f0 = 1 - f0;
-    if (courant_due && t < up_to) {
-      double v = 1.0e300;
-      for (int k = threadIdx.x; k < C->n_cmin; k += blockDim.x) v = fmin(v, gbl(C->cmin)[k]);
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off));
-      if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = v;
-      __syncthreads();
-      double m = s_min[0];
-      for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmin(m, s_min[w]);
-      const double fmax = 2.0 / m;
-      dt = C->safety * C->omegaS / fmax;
-    }
-  }
-  pending = 0;
-  if (open) {
-    preloop = 0;
-    running = (status == 0 && t < up_to) ? 1 : 0;
-    if (running) {
-      if (t + dt > up_to) dt = up_to - t;
-      inc++;
-      courant_due = (inc % C->dt_freq == 0) ? 1 : 0;
-      // densities are consumed by the Courant step and read back at the end
-      density_due = (courant_due || !(t + dt < up_to)) ? 1 : 0;
-      pending = 1;
-      // After the corrector, field0 already holds the BC values of this
-      // cycle unless a speed BC depends on a changed step or on time; a
-      // restraint zeroes the same components either way.
-      const bool bc_work = resumed || (C->bc_flags & 2) || ((C->bc_flags & 1) && dt != dt_last);
-      const int64_t k = tk.first + threadIdx.x;
-      if (bc_work && k < tk.end) {
-        const int64_t n = gbl(C->bcn)[k];
-        if (C->dim == 3)
-          bc_field0<3>(C, n, f0, t, dt);
-        else
-          bc_field0<2>(C, n, f0, t, dt);
-      }
-    }
-  } else {
-    running = 0;
+  // through the global space: a close launch may store into slot cs
+  Control c = load_control<const DNLG_GLOBAL Control>(C, cs);
+  const int status = job_status(C);
+  const double dt_crit = courant_ran(c, status) ? block_min_partials(C) : 0.0;
+  const bool bc_work = close_open(C, c, status, dt_crit, open != 0);
+  const int64_t k = tk.first + threadIdx.x;
+  if (c.running && bc_work && k < tk.end) {
+    const int64_t n = gbl(C->bcn)[k];
+    if (C->dim == 3)
+      bc_field0<3>(C, n, c.f0, c.t, c.dt);
+    else
+      bc_field0<2>(C, n, c.f0, c.t, c.dt);
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned nb = C->n_bcn > 0 ? (unsigned)((C->n_bcn + 255) / 256) : 1u;
-    __threadfence();
-    if (nb == 1 || atomicAdd(&J->c.arrive, 1u) == nb - 1) {
-      Control &c = J->c;
-      c.t = t;
-      c.dt = dt;
-      c.inc = inc;
-      c.f0 = f0;
-      c.running = running;
-      c.courant_due = courant_due;
-      c.density_due = density_due;
-      c.preloop = preloop;
-      c.pending = pending;
-      c.arrive = 0;
-    }
-  }
+  if (tk.blk == 0 && threadIdx.x == 0) store_control(J, out, c);
 }
 
 // Host-loop snapshot: t, dt, up_to, status, increment of every job.
 __global__ void k_snapshot(JobDev *const *jobs, int n_jobs, double *out) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n_jobs) return;
-  const Control &c = jobs[j]->c;
+  const JobDev *J = jobs[j];
+  const Control &c = J->c[0];
   out[5 * j + 0] = c.t;
   out[5 * j + 1] = c.dt;
   out[5 * j + 2] = c.up_to;
-  out[5 * j + 3] = (double)c.status;
+  out[5 * j + 3] = (double)(J->sync.st[0] != 0 ? J->sync.st[0] : J->sync.st[1]);
   out[5 * j + 4] = (double)c.inc;
 }
 
@@ -686,7 +835,7 @@ __global__ void k_snapshot(JobDev *const *jobs, int n_jobs, double *out) {
 __global__ void k_set_up_to(JobDev *const *jobs, int n_jobs, const double *up_to) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n_jobs) return;
-  jobs[j]->c.up_to = up_to[j];
+  jobs[j]->c[0].up_to = up_to[j];
 }
 
 // Forced time step of the pre-loop (Explicit.C:152-154).
@@ -697,7 +846,7 @@ __global__ void k_force_dt(JobDev *const *jobs, int n_jobs) {
   double m = J->cmin[0];
   for (int k = 1; k < J->n_cmin; k++) m = fmin(m, J->cmin[k]);
   const double fmax = 2.0 / m;
-  J->c.dt = J->safety * J->omegaS / fmax;
+  J->c[0].dt = J->safety * J->omegaS / fmax;
 }
 
 // ---------------------------------------------------------------------------
@@ -758,7 +907,7 @@ __global__ __launch_bounds__(256) void k_prep(JobDev *const *jobs, const Task *t
 #pragma unroll
     for (int a = 0; a < NN; a++) me[a] += J->rho0 * tabN<NN>(p, a) * wdj;
   }
-  if (neg) set_status(J, DNLG_E_NEG_JACOBIAN);
+  if (neg) set_status(J, DNLG_E_NEG_JACOBIAN, 0);
 #pragma unroll
   for (int a = 0; a < NN; a++) J->elemF[(int64_t)J->e2slot[e * NN + a] * DIM] = me[a];
 }
@@ -766,7 +915,7 @@ __global__ __launch_bounds__(256) void k_prep(JobDev *const *jobs, const Task *t
 // Model::computeMassMatrix assembly (Model.C:537-590), gathered per node in
 // element order.
 template <int DIM>
-__global__ __launch_bounds__(256) void k_mass_gather(JobDev *const *jobs, const Task *tasks) {
+__global__ __launch_bounds__(kNodeBlock) void k_mass_gather(JobDev *const *jobs, const Task *tasks) {
   const Task tk = tasks[blockIdx.x];
   JobDev *J = jobs[tk.job];
   const int64_t n = tk.first + threadIdx.x;
@@ -798,68 +947,67 @@ void upload_tables(const double *hexN, const double *hexD, const double *quaN, c
 int elem_block_threads(int type) { return type == DNLG_EL_HEX8N_3D ? 128 : 256; }
 int elem_block_threads_mixed(int type) { return type == DNLG_EL_HEX8N_3D ? 128 : 256; }
 
-void launch_elem_mixed(int type, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s,
-                       const hipEvent_t *ev) {
+void launch_elem_mixed(int type, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int derive,
+                       hipStream_t s, const hipEvent_t *ev) {
   if (nblk <= 0) return;
   hipEvent_t a = ev ? ev[0] : nullptr, b = ev ? ev[1] : nullptr;
-  if (type == DNLG_EL_HEX8N_3D)
-  {
+  if (type == DNLG_EL_HEX8N_3D) {
     if (ev)
-      hipExtLaunchKernelGGL((k_elem_mixed<8, 3, 128>), dim3(nblk), dim3(128), 0, s, a, b, 0, jobs, tasks);
+      hipExtLaunchKernelGGL((k_elem_mixed<8, 3, 128>), dim3(nblk), dim3(128), 0, s, a, b, 0, jobs, tasks, cs, derive);
     else
-      hipLaunchKernelGGL((k_elem_mixed<8, 3, 128>), dim3(nblk), dim3(128), 0, s, jobs, tasks);
-  }
-  else
-  {
+      hipLaunchKernelGGL((k_elem_mixed<8, 3, 128>), dim3(nblk), dim3(128), 0, s, jobs, tasks, cs, derive);
+  } else {
     if (ev)
-      hipExtLaunchKernelGGL((k_elem_mixed<4, 2, 256>), dim3(nblk), dim3(256), 0, s, a, b, 0, jobs, tasks);
+      hipExtLaunchKernelGGL((k_elem_mixed<4, 2, 256>), dim3(nblk), dim3(256), 0, s, a, b, 0, jobs, tasks, cs, derive);
     else
-      hipLaunchKernelGGL((k_elem_mixed<4, 2, 256>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
+      hipLaunchKernelGGL((k_elem_mixed<4, 2, 256>), dim3(nblk), dim3(256), 0, s, jobs, tasks, cs, derive);
   }
 }
 
-void launch_elem_fp64(int type, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s,
-                      const hipEvent_t *ev) {
+void launch_elem_fp64(int type, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int derive,
+                      hipStream_t s, const hipEvent_t *ev) {
   if (nblk <= 0) return;
   hipEvent_t a = ev ? ev[0] : nullptr, b = ev ? ev[1] : nullptr;
-  if (type == DNLG_EL_HEX8N_3D)
-  {
+  if (type == DNLG_EL_HEX8N_3D) {
     if (ev)
-      hipExtLaunchKernelGGL((k_elem_fp64<8, 3, 128>), dim3(nblk), dim3(128), 0, s, a, b, 0, jobs, tasks);
+      hipExtLaunchKernelGGL((k_elem_fp64<8, 3, 128>), dim3(nblk), dim3(128), 0, s, a, b, 0, jobs, tasks, cs, derive);
     else
-      hipLaunchKernelGGL((k_elem_fp64<8, 3, 128>), dim3(nblk), dim3(128), 0, s, jobs, tasks);
-  }
-  else
-  {
+      hipLaunchKernelGGL((k_elem_fp64<8, 3, 128>), dim3(nblk), dim3(128), 0, s, jobs, tasks, cs, derive);
+  } else {
     if (ev)
-      hipExtLaunchKernelGGL((k_elem_fp64<4, 2, 256>), dim3(nblk), dim3(256), 0, s, a, b, 0, jobs, tasks);
+      hipExtLaunchKernelGGL((k_elem_fp64<4, 2, 256>), dim3(nblk), dim3(256), 0, s, a, b, 0, jobs, tasks, cs, derive);
     else
-      hipLaunchKernelGGL((k_elem_fp64<4, 2, 256>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
+      hipLaunchKernelGGL((k_elem_fp64<4, 2, 256>), dim3(nblk), dim3(256), 0, s, jobs, tasks, cs, derive);
   }
 }
 
-void launch_correct(int dim, bool f32, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s) {
+void launch_correct(int dim, bool f32, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int fold,
+                    hipStream_t s) {
   if (nblk <= 0) return;
   if (dim == 3) {
-    if (f32) hipLaunchKernelGGL((k_correct<3, float>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
-    else hipLaunchKernelGGL((k_correct<3, double>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
+    if (f32)
+      hipLaunchKernelGGL((k_correct<3, float>), dim3(nblk), dim3(kNodeBlock), 0, s, jobs, tasks, cs, fold);
+    else
+      hipLaunchKernelGGL((k_correct<3, double>), dim3(nblk), dim3(kNodeBlock), 0, s, jobs, tasks, cs, fold);
   } else {
-    if (f32) hipLaunchKernelGGL((k_correct<2, float>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
-    else hipLaunchKernelGGL((k_correct<2, double>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
+    if (f32)
+      hipLaunchKernelGGL((k_correct<2, float>), dim3(nblk), dim3(kNodeBlock), 0, s, jobs, tasks, cs, fold);
+    else
+      hipLaunchKernelGGL((k_correct<2, double>), dim3(nblk), dim3(kNodeBlock), 0, s, jobs, tasks, cs, fold);
   }
 }
 
-void launch_courant(int dim, JobDev *const *jobs, const Task *tasks, int nblk, int force, hipStream_t s) {
+void launch_courant(int dim, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int force, hipStream_t s) {
   if (nblk <= 0) return;
   if (dim == 3)
-    hipLaunchKernelGGL(k_courant<3>, dim3(nblk), dim3(256), 0, s, jobs, tasks, force);
+    hipLaunchKernelGGL(k_courant<3>, dim3(nblk), dim3(256), 0, s, jobs, tasks, cs, force);
   else
-    hipLaunchKernelGGL(k_courant<2>, dim3(nblk), dim3(256), 0, s, jobs, tasks, force);
+    hipLaunchKernelGGL(k_courant<2>, dim3(nblk), dim3(256), 0, s, jobs, tasks, cs, force);
 }
 
-void launch_cycle_step(JobDev *const *jobs, const Task *tasks, int nblk, int open, hipStream_t s) {
+void launch_cycle_step(JobDev *const *jobs, const Task *tasks, int nblk, int cs, int open, int out, hipStream_t s) {
   if (nblk <= 0) return;
-  hipLaunchKernelGGL(k_cycle_step, dim3(nblk), dim3(256), 0, s, jobs, tasks, open);
+  hipLaunchKernelGGL(k_cycle_step, dim3(nblk), dim3(256), 0, s, jobs, tasks, cs, open, out);
 }
 void launch_snapshot(JobDev *const *jobs, int n, double *out, hipStream_t s) {
   hipLaunchKernelGGL(k_snapshot, dim3((n + 63) / 64), dim3(64), 0, s, jobs, n, out);
@@ -880,9 +1028,9 @@ void launch_prep(int type, JobDev *const *jobs, const Task *tasks, int nblk, hip
 void launch_mass_gather(int dim, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s) {
   if (nblk <= 0) return;
   if (dim == 3)
-    hipLaunchKernelGGL(k_mass_gather<3>, dim3(nblk), dim3(256), 0, s, jobs, tasks);
+    hipLaunchKernelGGL(k_mass_gather<3>, dim3(nblk), dim3(kNodeBlock), 0, s, jobs, tasks);
   else
-    hipLaunchKernelGGL(k_mass_gather<2>, dim3(nblk), dim3(256), 0, s, jobs, tasks);
+    hipLaunchKernelGGL(k_mass_gather<2>, dim3(nblk), dim3(kNodeBlock), 0, s, jobs, tasks);
 }
 
 }  // namespace dnlg
