@@ -108,7 +108,7 @@ int64_t footprint(int type, int64_t N, int64_t NE, int prec, int track) {
   b += NE * nn * dim * 8;          // elemF
   if (prec == DNLG_PREC_MIXED) {
     b += NE * nn * dim * 4;        // elemF32
-    b += G * (6 * 4 + 4 * 4 + 4 * 8 + 8 + 8 + 4);
+    b += G * (6 * 4 + 3 * 4 + 4 * 8 + 8 + 8 + 4);
   } else {
     b += G * (6 * 8 + 8 * 8 + 8 + 8);
   }
@@ -396,7 +396,6 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
     ALLOC(H.dep32, G, (const float *)nullptr);
     ALLOC(H.sy32, G, (const float *)nullptr);
     ALLOC(H.gam32, G, (const float *)nullptr);
-    ALLOC(H.gamc32, G, (const float *)nullptr);
     ALLOC(H.detJ0f, G, (const float *)nullptr);
   } else {
     ALLOC(H.sig, 6 * G, (const double *)nullptr);
@@ -523,7 +522,14 @@ int dnlg_job_read_ip(dnlg_job *job, int field, double *out) {
     case DNLG_F_PLASTIC_STRAIN: return d64(H.ep, G, out);
     case DNLG_F_PLASTIC_STRAIN_RATE: return mixed ? f32(H.dep32, G, out) : d64(H.dep, G, out);
     case DNLG_F_GAMMA: return mixed ? f32(H.gam32, G, out) : d64(H.gam, G, out);
-    case DNLG_F_GAMMA_CUMULATE: return mixed ? f32(H.gamc32, G, out) : d64(H.gamc, G, out);
+    case DNLG_F_GAMMA_CUMULATE: {
+      if (!mixed) return d64(H.gamc, G, out);
+      // gammaCumulate and the plastic strain grow together from 0 (Element.C:589-598):
+      // ep = sqrt(2/3) sum(gamma); the mixed path keeps ep only (FP64)
+      int st = d64(H.ep, G, out);
+      for (int64_t g = 0; g < G && st == DNLG_OK; g++) out[g] /= kSqrt23;
+      return st;
+    }
     case DNLG_F_YIELD_STRESS: return mixed ? f32(H.sy32, G, out) : d64(H.sy, G, out);
     case DNLG_F_TEMPERATURE: return d64(H.T, G, out);
     case DNLG_F_DENSITY: return d64(H.rho, G, out);

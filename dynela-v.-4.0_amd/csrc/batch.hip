@@ -42,10 +42,11 @@ struct Batch {
 
 namespace {
 
-int upload_tasks(Batch &B, const std::vector<Task> &h, TaskList &out) {
+int upload_tasks(Batch &B, std::vector<Task> h, TaskList &out) {
   out.n = (int)h.size();
   out.d = nullptr;
   if (h.empty()) return 0;
+  for (Task &t : h) t.jd = B.jobs[t.job]->dev;
   void *p = nullptr;
   if (hipMalloc(&p, h.size() * sizeof(Task)) != hipSuccess) return DNLG_E_OOM;
   B.allocs.push_back(p);

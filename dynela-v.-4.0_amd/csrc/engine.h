@@ -100,7 +100,7 @@ struct JobDev {
   float *sig32; // FP32 stress (mixed mode)
   double *ep, *eint, *einel, *T;  // FP64 in both modes
   double *dep, *sy, *gam, *gamc;  // FP64 mode
-  float *dep32, *sy32, *gam32, *gamc32;  // mixed mode
+  float *dep32, *sy32, *gam32;  // mixed mode (its cumulative gamma is ep / sqrt(2/3))
   double *rho, *detJ0;
   float *detJ0f;
   double *eps, *epsP;  // optional tensors (track_tensors)
@@ -125,15 +125,18 @@ __device__ __forceinline__ DNLG_GLOBAL T *gbl(T *p) {
 constexpr int kNodeBlock = 256;
 
 // One block of work: job index into the batch, local block index within the
-// job, and the half-open item range [first, end).
+// job, the half-open item range [first, end), and the job's record (filled
+// in when the task list is uploaded, so a block reaches its job with one
+// load less).
 struct Task {
   int32_t job;
   int32_t blk;
   int64_t first;
   int64_t end;
+  JobDev *jd = nullptr;
 };
 
-// The block's task and job record, read through the constant space.
+// The block's task, read through the constant space.
 __device__ __forceinline__ Task load_task(const Task *tasks) {
   const DNLG_CONST Task *t = (const DNLG_CONST Task *)tasks + blockIdx.x;
   Task tk;
@@ -141,11 +144,8 @@ __device__ __forceinline__ Task load_task(const Task *tasks) {
   tk.blk = t->blk;
   tk.first = t->first;
   tk.end = t->end;
+  tk.jd = t->jd;
   return tk;
-}
-
-__device__ __forceinline__ JobDev *load_job(JobDev *const *jobs, int j) {
-  return ((JobDev *const DNLG_CONST *)jobs)[j];
 }
 
 }  // namespace dnlg

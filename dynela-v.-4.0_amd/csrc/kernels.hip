@@ -210,7 +210,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_fp64(JobDev *const *jobs, const 
   __shared__ double s_tr[EPB][NIP];
 
   const Task tk = load_task(tasks);
-  JobDev *J = load_job(jobs, tk.job);
+  JobDev *J = tk.jd;
   CJobDev *C = (CJobDev *)J;
   const Control cc = cycle_control(J, C, cs, derive, tk.blk);
   if (!cc.running) return;
@@ -553,7 +553,7 @@ template <int DIM, typename FT>
 __global__ __launch_bounds__(kNodeBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_correct(
     JobDev *const *jobs, const Task *tasks, int cs, int fold) {
   const Task tk = load_task(tasks);
-  JobDev *J = load_job(jobs, tk.job);
+  JobDev *J = tk.jd;
   CJobDev *C = (CJobDev *)J;
   const Control c = load_control(C, cs);
   const int st0 = C->sync.st[0], st1 = C->sync.st[1];
@@ -717,7 +717,7 @@ template <int DIM>
 __global__ __launch_bounds__(256) void k_courant(JobDev *const *jobs, const Task *tasks, int cs, int force) {
   __shared__ double s_min[4];
   const Task tk = load_task(tasks);
-  JobDev *J = load_job(jobs, tk.job);
+  JobDev *J = tk.jd;
   CJobDev *C = (CJobDev *)J;
   if (!force) {
     // Explicit.C:243-251: only while t_{n+1} < t_up_to, every dt_freq increments
@@ -799,7 +799,7 @@ __device__ __forceinline__ double block_min_partials(CJobDev *C) {
 __global__ __launch_bounds__(256) void k_cycle_step(JobDev *const *jobs, const Task *tasks, int cs, int open,
                                                     int out) {
   const Task tk = load_task(tasks);
-  JobDev *J = load_job(jobs, tk.job);
+  JobDev *J = tk.jd;
   CJobDev *C = (CJobDev *)J;
   // through the global space: a close launch may store into slot cs
   Control c = load_control<const DNLG_GLOBAL Control>(C, cs);
@@ -856,8 +856,8 @@ __global__ void k_force_dt(JobDev *const *jobs, int n_jobs) {
 // ---------------------------------------------------------------------------
 template <int NN, int DIM>
 __global__ __launch_bounds__(256) void k_prep(JobDev *const *jobs, const Task *tasks) {
-  const Task tk = tasks[blockIdx.x];
-  JobDev *J = jobs[tk.job];
+  const Task tk = load_task(tasks);
+  JobDev *J = tk.jd;
   const int64_t e = tk.first + threadIdx.x;
   if (e >= tk.end) return;
   const int64_t N = J->N;
@@ -916,8 +916,8 @@ __global__ __launch_bounds__(256) void k_prep(JobDev *const *jobs, const Task *t
 // element order.
 template <int DIM>
 __global__ __launch_bounds__(kNodeBlock) void k_mass_gather(JobDev *const *jobs, const Task *tasks) {
-  const Task tk = tasks[blockIdx.x];
-  JobDev *J = jobs[tk.job];
+  const Task tk = load_task(tasks);
+  JobDev *J = tk.jd;
   const int64_t n = tk.first + threadIdx.x;
   if (n >= tk.end) return;
   double m = 0.0;

@@ -84,7 +84,7 @@ def compare_mixed(model, o):
     _, d_ref, _ = o.nodes()
     err = disp_err(model.getNodalDisplacements(), d_ref)
     out = {"disp": err}
-    ids = {"plasticStrain": 0, "T": 5, "Stress": 11}
+    ids = {"plasticStrain": 0, "gamma": 2, "gammaCumulate": 3, "yieldStress": 4, "T": 5, "Stress": 11}
     for f, k in ids.items():
         out[f] = rel_norm_err(model.getIntPointField(getattr(dnl.Field, f)), o.ip(k))
     e_g, e_o = model.getEnergies(), o.energies()
@@ -99,6 +99,9 @@ def test_mixed_taylor_axi(ctx):
     o = run_both(m, 80e-6)
     e = compare_mixed(m, o)
     assert e["disp"] <= MIXED_DISP_TOL
+    # the mixed path derives gammaCumulate from the FP64 plastic strain: it
+    # follows the plastic strain's error
+    assert e["gammaCumulate"] <= 10 * e["plasticStrain"] + 1e-6
 
 
 def test_mixed_taylor_3d(ctx):
