@@ -328,6 +328,9 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
   H.cp = mat->heat_capacity;
   H.T0 = mat->T0;
   for (int k = 0; k < 8; k++) H.lp[k] = mat->p[k];
+  for (int k = 0; k < 8; k++) H.lpf[k] = (float)mat->p[k];
+  H.inv_dTf = mat->law == DNLG_LAW_JOHNSON_COOK ? (float)(1.0 / (mat->p[6] - mat->p[7])) : 0.f;
+  H.inv_rho0 = 1.0 / mat->density;
   // Explicit::computeChungHulbertIntegrationParameters (Explicit.C:86-104)
   const double rb = prm->rho_b;
   H.alphaM = (2.0 * rb - 1.0) / (1.0 + rb);

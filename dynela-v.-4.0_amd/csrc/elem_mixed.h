@@ -365,7 +365,10 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
       const float heatf = (float)heatFrac;
       const float Tc = (float)Tcur;
       const float ep0f = (float)ep0;
-      const LawF L = law_f32_setup(law, lp, dt, heatFrac);
+      float pf[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) pf[k] = C->lpf[k];
+      const LawF L = law_f32_setup(law, pf, C->inv_dTf, (float)dt, (float)heatFrac);
       float gmin = 0.f, gmax = (Strialf - yieldf) / TwoG32f;
       float g = gam0;
       if (ep0 == 0.0) g = (float)(kSqrt32 * kGammaInitial);
@@ -406,7 +409,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
 #pragma unroll
       for (int c = 0; c < 6; c++) s[c] *= scale;
       gbl(C->ep)[gi] = ep0 + kSqrt23 * gamma;
-      gbl(C->dep32)[gi] = (float)(kSqrt23 * gamma / dt);
+      gbl(C->dep32)[gi] = s23 * g * __builtin_amdgcn_rcpf((float)dt);
       gbl(C->gam32)[gi] = g;
       gbl(C->sy32)[gi] = yieldf;
     }
@@ -418,12 +421,12 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
     const float power = 0.5f * (dE[0] * (old[0] + sig[0]) + dE[3] * (old[3] + sig[3]) + dE[5] * (old[5] + sig[5]) +
                                 2.f * (dE[1] * (old[1] + sig[1]) + dE[2] * (old[2] + sig[2]) +
                                        dE[4] * (old[4] + sig[4])));
-    gbl(C->eint)[gi] += (double)power / C->rho0;
+    gbl(C->eint)[gi] += (double)power * C->inv_rho0;
     if (gamma != 0.0) {
       const float sn =
           sqrtf(s[0] * s[0] + s[3] * s[3] + s[5] * s[5] + 2.f * (s[1] * s[1] + s[2] * s[2] + s[4] * s[4]));
       const double w = 0.5 * gamma * ((double)sn + (double)Snorm0);
-      gbl(C->einel)[gi] += w / C->rho0;
+      gbl(C->einel)[gi] += w * C->inv_rho0;
       gbl(C->T)[gi] = Tcur + heatFrac * w;
     }
     // Element::computeFinalRotation (Element.C:1146-1161)

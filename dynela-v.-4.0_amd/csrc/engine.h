@@ -75,6 +75,10 @@ struct JobDev {
   // material (Material.h:110-169) and law parameters
   double rho0, young, nu, twoG, K, heatFrac, eta, cp, T0;
   double lp[8];
+  // FP32 copies for the mixed path's radial return, and 1 / rho0
+  float lpf[8];
+  float inv_dTf;  // 1 / (Tm - T0) of Johnson-Cook
+  double inv_rho0;
   // Chung-Hulbert (Explicit.C:86-104) and time-step settings
   double alphaM, beta, gammaCH, omegaS, safety;
   // nodal state, [dim][N]

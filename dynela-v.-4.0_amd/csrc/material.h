@@ -75,20 +75,23 @@ struct LawF {
   float eta_rcp;     // eta / (rho Cp)
 };
 
-__device__ __forceinline__ LawF law_f32_setup(int law, const double *p, double dt, double eta_rcp) {
+// pf: the law parameters in FP32, inv_dT = 1 / (Tm - T0) (Johnson-Cook);
+// the rate terms from the cycle's step in FP32 (hardware log2).
+__device__ __forceinline__ LawF law_f32_setup(int law, const float (&pf)[8], float inv_dT, float dt, float eta_rcp) {
   LawF L;
   L.law = law;
 #pragma unroll
-  for (int k = 0; k < 8; k++) L.p[k] = (float)p[k];
+  for (int k = 0; k < 8; k++) L.p[k] = pf[k];
   L.lrate = 0.f;
   L.inv_dT = 0.f;
   L.gamma_rate = 0.f;
   if (law == DNLG_LAW_JOHNSON_COOK) {
-    L.lrate = (float)log(kSqrt23 / (dt * p[5]));
-    L.inv_dT = (float)(1.0 / (p[6] - p[7]));
-    L.gamma_rate = (float)(dt * p[5] / kSqrt23);
+    const float r0 = dt * pf[5] * (float)kSqrt32;  // depsp0 dt / sqrt(2/3)
+    L.gamma_rate = r0;
+    L.lrate = -__builtin_amdgcn_logf(r0) * 0.693147180559945309f;
+    L.inv_dT = inv_dT;
   }
-  L.eta_rcp = (float)eta_rcp;
+  L.eta_rcp = eta_rcp;
   return L;
 }
 
