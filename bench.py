@@ -49,8 +49,9 @@ def algo_bytes_per_elem(prec, nodes_per_elem):
 
     Per IP: the persistent state read and written once (FP64 path: stress 6,
     ep, dep, sy, gamma, gamma_cum, T, e_int, e_inel = 14 doubles each way;
-    mixed path: stress 6 + dep, sy, gamma, gamma_cum in FP32, ep, T, e_int,
-    e_inel in FP64 = 72 B each way) plus detJ0 read (8 B / 4 B).  Per
+    mixed path: stress 6 + dep, sy, gamma in FP32, ep, T, e_int, e_inel in
+    FP64 = 68 B each way; its gamma_cum is ep / sqrt(2/3) at read-back) plus
+    detJ0 read (8 B / 4 B).  Per
     element: connectivity 8 x 4 B, the per-element force vector written
     (24 x 8 B / 24 x 4 B).  Per node (x nodes/element): coordinates and the
     field0 increment read once (6 x 8 B).
@@ -59,7 +60,7 @@ def algo_bytes_per_elem(prec, nodes_per_elem):
         ip = 14 * 8 * 2 + 8
         force = 24 * 8
     else:
-        ip = 72 * 2 + 4
+        ip = 68 * 2 + 4
         force = 24 * 4
     return 8 * ip + 8 * 4 + force + nodes_per_elem * 6 * 8
 

@@ -156,6 +156,17 @@ __device__ __forceinline__ float group_sum(float v) {
 
 __device__ __forceinline__ float sgn(int b, int d) { return ((b >> d) & 1) ? 1.f : -1.f; }
 
+// Per-lane rows of R float4 in LDS, exchanged inside a wave (the NN lanes of
+// an element are in one wave; one wave's LDS accesses execute in order, and
+// wave_barrier keeps the compiler from moving them across the exchange).
+// Each element's rows are followed by one float4 of padding, so that the
+// element groups of a wave, which read one row each, fall on different banks.
+template <int NN, int R>
+struct LaneRows {
+  static constexpr int kGroup = NN * R + 1;  // float4 per element
+  __device__ static int at(int le, int lane, int v) { return le * kGroup + lane * R + v; }
+};
+
 template <int NN, int DIM, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const Task *tasks, int ctl, int derive) {
   constexpr int NB = DIM;               // lanes per element = 2^DIM = NN
@@ -228,15 +239,42 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
     Mx[d] = inv2D * wht<NB>(xl[d], cs);
     Mu[d] = inv2D * wht<NB>(ul[d], cs);
   }
-  // modes of the group, broadcast: mx[q][d]
-  float mx[NN][DIM], mu[NN][DIM];
+  // modes of the group to every lane of the element: mx[q][d], mu[q][d]
+  constexpr int MR = (2 * DIM + 3) / 4;  // float4 per lane row
+  typedef LaneRows<NN, MR> ModeRows;
+  __shared__ float4 s_mode[(BLOCK / NN) * ModeRows::kGroup];
+  {
+    float row[4 * MR];
 #pragma unroll
-  for (int q = 0; q < NN; q++)
+    for (int k = 0; k < 4 * MR; k++) row[k] = 0.f;
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
-      mx[q][d] = __shfl(Mx[d], base + q);
-      mu[q][d] = __shfl(Mu[d], base + q);
+      row[d] = Mx[d];
+      row[DIM + d] = Mu[d];
     }
+#pragma unroll
+    for (int v = 0; v < MR; v++)
+      s_mode[ModeRows::at(le, b, v)] = make_float4(row[4 * v], row[4 * v + 1], row[4 * v + 2], row[4 * v + 3]);
+  }
+  __builtin_amdgcn_wave_barrier();
+  float mx[NN][DIM], mu[NN][DIM];
+#pragma unroll
+  for (int q = 0; q < NN; q++) {
+    float row[4 * MR];
+#pragma unroll
+    for (int v = 0; v < MR; v++) {
+      const float4 t = s_mode[ModeRows::at(le, q, v)];
+      row[4 * v] = t.x;
+      row[4 * v + 1] = t.y;
+      row[4 * v + 2] = t.z;
+      row[4 * v + 3] = t.w;
+    }
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      mx[q][d] = row[d];
+      mu[q][d] = row[DIM + d];
+    }
+  }
   // Gauss point of this lane
   float xi[DIM];
 #pragma unroll
@@ -457,11 +495,23 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
     }
   if constexpr (DIM == 2) hoop = valid ? w * sig[5] / radius : 0.f;
   // Walsh transform of T over the Gauss points, then node lane b combines
+  // four (two) of them per direction, read from the element's lanes in LDS
   float WT[DIM][DIM];
 #pragma unroll
   for (int k = 0; k < DIM; k++)
 #pragma unroll
     for (int i = 0; i < DIM; i++) WT[k][i] = wht<NB>(T9[k][i], cs);
+  constexpr int FR = DIM == 3 ? 3 : 2;  // float4 per lane row
+  typedef LaneRows<NN, FR> ForceRows;
+  __shared__ float4 s_wt[(BLOCK / NN) * ForceRows::kGroup];
+  if constexpr (DIM == 3) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) s_wt[ForceRows::at(le, b, k)] = make_float4(WT[k][0], WT[k][1], WT[k][2], 0.f);
+  } else {
+    s_wt[ForceRows::at(le, b, 0)] = make_float4(WT[0][0], WT[0][1], WT[1][0], WT[1][1]);
+    s_wt[ForceRows::at(le, b, 1)] = make_float4(wht<NB>(hoop, cs), 0.f, 0.f, 0.f);
+  }
+  __builtin_amdgcn_wave_barrier();
   float f[DIM];
 #pragma unroll
   for (int i = 0; i < DIM; i++) f[i] = 0.f;
@@ -471,30 +521,31 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
     if constexpr (DIM == 3) {
       const int l = (k + 1) % 3, m = (k + 2) % 3;
       const float cl = g * sgn(b, l), cm = g * sgn(b, m), clm = cl * cm;
-#pragma unroll
-      for (int i = 0; i < DIM; i++) {
-        const float w0 = __shfl(WT[k][i], base + 0);
-        const float wl = __shfl(WT[k][i], base + (1 << l));
-        const float wm = __shfl(WT[k][i], base + (1 << m));
-        const float wlm = __shfl(WT[k][i], base + (1 << l) + (1 << m));
-        f[i] -= sk * (w0 + cl * wl + cm * wm + clm * wlm);
-      }
+      const float4 a0 = s_wt[ForceRows::at(le, 0, k)];
+      const float4 al = s_wt[ForceRows::at(le, 1 << l, k)];
+      const float4 am = s_wt[ForceRows::at(le, 1 << m, k)];
+      const float4 alm = s_wt[ForceRows::at(le, (1 << l) + (1 << m), k)];
+      f[0] -= sk * (a0.x + cl * al.x + cm * am.x + clm * alm.x);
+      f[1] -= sk * (a0.y + cl * al.y + cm * am.y + clm * alm.y);
+      f[2] -= sk * (a0.z + cl * al.z + cm * am.z + clm * alm.z);
     } else {
       const int l = 1 - k;
       const float cl = g * sgn(b, l);
-#pragma unroll
-      for (int i = 0; i < DIM; i++) {
-        const float w0 = __shfl(WT[k][i], base + 0);
-        const float wl = __shfl(WT[k][i], base + (1 << l));
-        f[i] -= sk * (w0 + cl * wl);
+      const float4 a0 = s_wt[ForceRows::at(le, 0, 0)];
+      const float4 al = s_wt[ForceRows::at(le, 1 << l, 0)];
+      if (k == 0) {
+        f[0] -= sk * (a0.x + cl * al.x);
+        f[1] -= sk * (a0.y + cl * al.y);
+      } else {
+        f[0] -= sk * (a0.z + cl * al.z);
+        f[1] -= sk * (a0.w + cl * al.w);
       }
     }
   }
   if constexpr (DIM == 2) {
     // hoop term: f_r -= sum_p N_a(p) w_p sigma_tt / r_p
-    const float Wh = wht<NB>(hoop, cs);
-    const float h0 = __shfl(Wh, base + 0), h1 = __shfl(Wh, base + 1), h2 = __shfl(Wh, base + 2),
-                h3 = __shfl(Wh, base + 3);
+    const float h0 = s_wt[ForceRows::at(le, 0, 1)].x, h1 = s_wt[ForceRows::at(le, 1, 1)].x;
+    const float h2 = s_wt[ForceRows::at(le, 2, 1)].x, h3 = s_wt[ForceRows::at(le, 3, 1)].x;
     const float c0 = g * sgn(b, 0), c1 = g * sgn(b, 1);
     f[0] -= h0 + c0 * h1 + c1 * h2 + c0 * c1 * h3;
   }
