@@ -36,7 +36,7 @@ __device__ __forceinline__ void law_yield_and_slope(int law, const double *p, do
     const bool warm = T > T0;
     if (warm) {
       if (T < Tm) {
-        tp1 = pow(((T - T0) / (Tm - T0)), m);
+        tp1 = exp(m * log((T - T0) / (Tm - T0)));  // ((T - T0) / (Tm - T0))^m, 0 < base < 1
         temp0 = 1.0 - tp1;
       } else {
         temp0 = 0.0;

@@ -23,8 +23,9 @@ def test_algorithmic_bytes_per_element_cycle():
     npe = 125341 / 120000  # nodes per element of the 120 000-hex cylinder
     # FP64: 8 IPs x (14 doubles read + written + detJ0) + conn + forces + 6 doubles per node
     assert bench.algo_bytes_per_elem("fp64", npe) == pytest.approx(8 * (14 * 8 * 2 + 8) + 32 + 192 + npe * 48)
-    assert bench.algo_bytes_per_elem("mixed", npe) == pytest.approx(8 * (72 * 2 + 4) + 32 + 96 + npe * 48)
-    assert round(bench.algo_bytes_per_elem("mixed", npe), 1) == 1362.1
+    # mixed: stress 6 + dep, sy, gamma in FP32 and ep, T, e_int, e_inel in FP64 = 68 B each way
+    assert bench.algo_bytes_per_elem("mixed", npe) == pytest.approx(8 * (68 * 2 + 4) + 32 + 96 + npe * 48)
+    assert round(bench.algo_bytes_per_elem("mixed", npe), 1) == 1298.1
 
 
 def test_measured_traffic_only_for_its_workload():
