@@ -3,7 +3,8 @@
 // Same phases as k_elem_fp64 (Element.C:1103-1161, 362-404, 407-622,
 // 1243-1256) with FP32 geometry, strain, stress, radial return and forces,
 // and FP64 for the nodal kinematics and the accumulated equivalent plastic
-// strain, temperature and energies.
+// strain, temperature and energies.  Divisions and square roots of the FP32
+// phases use the hardware reciprocal and square root (about 1 ulp).
 //
 // Strain increment in increment form: with H = d(du)/dx the right
 // Cauchy-Green tensor is C = I + X, X = H + H^T + H^T H (small), and
@@ -303,7 +304,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
     const float *dd = &Jm[0][0];
     detJ = dd[1] * dd[5] * dd[6] - dd[2] * dd[4] * dd[6] + dd[2] * dd[3] * dd[7] - dd[0] * dd[5] * dd[7] -
            dd[1] * dd[3] * dd[8] + dd[0] * dd[4] * dd[8];
-    const float id = 1.f / detJ;
+    const float id = __builtin_amdgcn_rcpf(detJ);
     inv[0][0] = (dd[4] * dd[8] - dd[5] * dd[7]) * id;
     inv[0][1] = (dd[2] * dd[7] - dd[1] * dd[8]) * id;
     inv[0][2] = (dd[1] * dd[5] - dd[2] * dd[4]) * id;
@@ -315,7 +316,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
     inv[2][2] = (dd[0] * dd[4] - dd[1] * dd[3]) * id;
   } else {
     detJ = Jm[0][0] * Jm[1][1] - Jm[1][0] * Jm[0][1];
-    const float id = 1.f / detJ;
+    const float id = __builtin_amdgcn_rcpf(detJ);
     inv[0][0] = Jm[1][1] * id;
     inv[0][1] = -Jm[0][1] * id;
     inv[1][0] = -Jm[1][0] * id;
@@ -338,7 +339,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
     const float rl = mx[0][0] + mx[1][0] * xi[0] + mx[2][0] * xi[1] + mx[3][0] * (xi[0] * xi[1]);
     const float ur = mu[0][0] + mu[1][0] * xi[0] + mu[2][0] * xi[1] + mu[3][0] * (xi[0] * xi[1]);
     radius = (float)(x0r + (double)rl);
-    H[8] = ur / radius;
+    H[8] = ur * __builtin_amdgcn_rcpf(radius);
   }
   // series order, uniform over the wave, from the linear part of X
   float Xn2;
@@ -373,7 +374,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
     s[3] -= pm;
     s[5] -= pm;
     const float Snorm0 =
-        sqrtf(s[0] * s[0] + s[3] * s[3] + s[5] * s[5] + 2.f * (s[1] * s[1] + s[2] * s[2] + s[4] * s[4]));
+        __builtin_amdgcn_sqrtf(s[0] * s[0] + s[3] * s[3] + s[5] * s[5] + 2.f * (s[1] * s[1] + s[2] * s[2] + s[4] * s[4]));
     const float em = (dE[0] + dE[3] + dE[5]) * (1.f / 3.f);
     s[0] += TwoGf * (dE[0] - em);
     s[1] += TwoGf * dE[1];
@@ -382,9 +383,9 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
     s[4] += TwoGf * dE[4];
     s[5] += TwoGf * (dE[5] - em);
     const float Snorm =
-        sqrtf(s[0] * s[0] + s[3] * s[3] + s[5] * s[5] + 2.f * (s[1] * s[1] + s[2] * s[2] + s[4] * s[4]));
+        __builtin_amdgcn_sqrtf(s[0] * s[0] + s[3] * s[3] + s[5] * s[5] + 2.f * (s[1] * s[1] + s[2] * s[2] + s[4] * s[4]));
     // trial test and radial return in FP32 (the yield stress is stored in FP32)
-    const float Strialf = (float)(kSqrt32 * (double)Snorm);
+    const float Strialf = (float)kSqrt32 * Snorm;
     float yieldf = sy0;
     double gamma = 0.0;
     const int law = C->law;
@@ -407,7 +408,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
 #pragma unroll
       for (int k = 0; k < 8; k++) pf[k] = C->lpf[k];
       const LawF L = law_f32_setup(law, pf, C->inv_dTf, (float)dt, (float)heatFrac);
-      float gmin = 0.f, gmax = (Strialf - yieldf) / TwoG32f;
+      float gmin = 0.f, gmax = (Strialf - yieldf) * __builtin_amdgcn_rcpf(TwoG32f);
       float g = gam0;
       if (ep0 == 0.0) g = (float)(kSqrt32 * kGammaInitial);
       float epsp = ep0f + s23 * g;
@@ -443,7 +444,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
         }
       }
       gamma = (double)g;
-      const float scale = 1.f - TwoGf * g / Snorm;
+      const float scale = 1.f - TwoGf * g * __builtin_amdgcn_rcpf(Snorm);
 #pragma unroll
       for (int c = 0; c < 6; c++) s[c] *= scale;
       gbl(C->ep)[gi] = ep0 + kSqrt23 * gamma;
@@ -462,7 +463,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
     gbl(C->eint)[gi] += (double)power * C->inv_rho0;
     if (gamma != 0.0) {
       const float sn =
-          sqrtf(s[0] * s[0] + s[3] * s[3] + s[5] * s[5] + 2.f * (s[1] * s[1] + s[2] * s[2] + s[4] * s[4]));
+          __builtin_amdgcn_sqrtf(s[0] * s[0] + s[3] * s[3] + s[5] * s[5] + 2.f * (s[1] * s[1] + s[2] * s[2] + s[4] * s[4]));
       const double w = 0.5 * gamma * ((double)sn + (double)Snorm0);
       gbl(C->einel)[gi] += w * C->inv_rho0;
       gbl(C->T)[gi] = Tcur + heatFrac * w;
@@ -493,7 +494,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
       }
       T9[k][i] = valid ? w * t : 0.f;
     }
-  if constexpr (DIM == 2) hoop = valid ? w * sig[5] / radius : 0.f;
+  if constexpr (DIM == 2) hoop = valid ? w * sig[5] * __builtin_amdgcn_rcpf(radius) : 0.f;
   // Walsh transform of T over the Gauss points, then node lane b combines
   // four (two) of them per direction, read from the element's lanes in LDS
   float WT[DIM][DIM];
