@@ -113,6 +113,7 @@ int64_t footprint(int type, int64_t N, int64_t NE, int prec, int track) {
     b += G * (6 * 8 + 8 * 8 + 8 + 8);
   }
   if (track) b += G * 12 * 8;
+  b += N * dim * 8 * 2 + ((N + 255) / 256) * 8;  // power-iteration iterates (upper bound: any method)
   b += ((NE + 255) / 256) * 8 + sizeof(JobDev) + 4096;
   return b;
 }
@@ -136,6 +137,9 @@ int dnlg_ctx_create(int device, dnlg_ctx **out) {
   // DNLG_GRAPH_CYCLES=<n>: cycles per captured graph; 0 launches every
   // kernel directly (for tools that do not follow graph launches)
   if (const char *g = std::getenv("DNLG_GRAPH_CYCLES")) ctx->graph_cycles = std::max(0, std::atoi(g));
+  // DNLG_PI_SLOTS=<n>: power iterations per time-step cycle inside a launch
+  // sequence (0..16); a job that needs more is finished by the host
+  if (const char *g = std::getenv("DNLG_PI_SLOTS")) ctx->pi_slots = std::min(16, std::max(0, std::atoi(g)));
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
     delete ctx;
     return DNLG_E_HIP;
@@ -184,7 +188,8 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
     return fail(ctx, DNLG_E_INVALID, "empty mesh");
   if (mesh->n_nodes >= (int64_t)1 << 31 || mesh->n_elems * 8 >= (int64_t)1 << 31)
     return fail(ctx, DNLG_E_CAPACITY, "mesh exceeds 32-bit indexing");
-  if (prm->dt_method != DNLG_DT_COURANT) return fail(ctx, DNLG_E_INVALID, "only the Courant time step is implemented");
+  if (prm->dt_method != DNLG_DT_COURANT && prm->dt_method != DNLG_DT_POWER_ITERATION)
+    return fail(ctx, DNLG_E_INVALID, "time-step method not implemented (Courant and PowerIteration are)");
   if (prm->stress_method != DNLG_STRESS_NR) return fail(ctx, DNLG_E_INVALID, "only the Newton-Raphson stress integration is implemented");
   if (prm->dt_freq <= 0 || !(prm->safety > 0) || prm->rho_b < 0.0 || prm->rho_b > 1.0)
     return fail(ctx, DNLG_E_INVALID, "invalid solver parameters");
@@ -318,6 +323,7 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
   H.track = job->track;
   H.law = mat->law;
   H.dt_freq = prm->dt_freq;
+  H.dt_method = prm->dt_method;
   H.rho0 = mat->density;
   H.young = mat->young;
   H.nu = mat->poisson;
@@ -386,6 +392,22 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
   job->n_cmin = (int32_t)((NE + 255) / 256);
   H.n_cmin = job->n_cmin;
   ALLOC(H.cmin, (int64_t)job->n_cmin, (const double *)nullptr);
+  H.pi = PiState{0.0, 1.0, 0, 0, 0, 0, 0, 0};
+  H.n_pibmax = 0;
+  if (prm->dt_method == DNLG_DT_POWER_ITERATION) {
+    // starting vector of Model.C:966-970 in the reference's DOF order
+    // (node * dim + d), stored [dim][N]
+    const int64_t ndof = (int64_t)dim * N;
+    std::vector<double> ev(ndof), evs(ndof);
+    ev[0] = 1.0;
+    for (int64_t i = 1; i < ndof; i++) ev[i] = ev[i - 1] - 2. / (ndof - 1);
+    for (int64_t n = 0; n < N; n++)
+      for (int d = 0; d < dim; d++) evs[d * N + n] = ev[n * dim + d];
+    ALLOC(H.pi_ev[0], ndof, evs.data());
+    ALLOC(H.pi_ev[1], ndof, (const double *)nullptr);
+    H.n_pibmax = (int32_t)((N + kNodeBlock - 1) / kNodeBlock);
+    ALLOC(H.pi_bmax, (int64_t)H.n_pibmax, (const double *)nullptr);
+  }
   std::vector<double> fillT(G, mat->T0), fillR(G, mat->density);
   ALLOC(H.ep, G, (const double *)nullptr);
   ALLOC(H.eint, G, (const double *)nullptr);

@@ -46,6 +46,7 @@ struct dnlg_ctx {
   std::vector<hipEvent_t> events;  // pool for timed launches
   std::vector<dnlg::BatchPtr> cache;
   int graph_cycles = 16;  // cycles per captured HIP graph (0 = plain launches)
+  int pi_slots = 2;       // power iterations per time-step cycle inside a sequence
 };
 
 namespace dnlg {

@@ -15,6 +15,8 @@
 
 namespace dnlg {
 
+constexpr int kSnap = 6;  // snapshot doubles per job (launch_snapshot)
+
 struct TaskList {
   Task *d = nullptr;
   int n = 0;
@@ -29,6 +31,9 @@ struct Batch {
   TaskList node[2][2];  // [dim==3?][mixed?]
   TaskList step;        // cycle control + field0 BCs, over the BC node lists
   TaskList close;       // end of a launch sequence, one block per job
+  bool has_courant = false, has_pi = false;  // time-step methods present
+  int *d_pend = nullptr;                     // power-iteration pending flags
+  std::vector<int> h_pend;
   double *d_snap = nullptr;
   std::vector<double> h_snap;
   // captured graphs by (kind, increment phase), see graph_for
@@ -89,10 +94,18 @@ int build_batch(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, Batch &B) {
   std::vector<JobDev *> h(n);
   for (int i = 0; i < n; i++) h[i] = jobs[i]->dev;
   if (hipMemcpy(B.d_jobs, h.data(), n * sizeof(JobDev *), hipMemcpyHostToDevice) != hipSuccess) return DNLG_E_HIP;
-  if (hipMalloc(&p, 5 * n * sizeof(double)) != hipSuccess) return DNLG_E_OOM;
+  if (hipMalloc(&p, kSnap * n * sizeof(double)) != hipSuccess) return DNLG_E_OOM;
   B.allocs.push_back(p);
   B.d_snap = (double *)p;
-  B.h_snap.assign(5 * n, 0.0);
+  B.h_snap.assign(kSnap * n, 0.0);
+  for (auto *j : jobs) {
+    B.has_pi = B.has_pi || j->host.dt_method == DNLG_DT_POWER_ITERATION;
+    B.has_courant = B.has_courant || j->host.dt_method == DNLG_DT_COURANT;
+  }
+  if (hipMalloc(&p, n * sizeof(int)) != hipSuccess) return DNLG_E_OOM;
+  B.allocs.push_back(p);
+  B.d_pend = (int *)p;
+  B.h_pend.assign(n, 0);
   int st;
   for (int hx = 0; hx < 2; hx++) {
     const int type = hx ? DNLG_EL_HEX8N_3D : DNLG_EL_QUA4N_AX;
@@ -185,13 +198,47 @@ int launch_element(dnlg_ctx *ctx, Batch &B, const hipEvent_t *ev, int cs, int de
 // to be a time-step cycle of any running job (see launch_cycles).  fold: the
 // corrector also writes the next cycle's field0 BCs, and the next cycle's
 // element launch derives its control record.
+// One power iteration for every job of the batch that is iterating.
+void launch_pi_iteration(dnlg_ctx *ctx, Batch &B, int cs) {
+  hipStream_t s = ctx->stream;
+  for (int hx = 0; hx < 2; hx++)
+    launch_pi_elem(hx ? DNLG_EL_HEX8N_3D : DNLG_EL_QUA4N_AX, B.d_jobs, B.cour[hx].d, B.cour[hx].n, s);
+  for (int d3 = 0; d3 < 2; d3++)
+    for (int mx = 0; mx < 2; mx++) launch_pi_node(d3 ? 3 : 2, B.d_jobs, B.node[d3][mx].d, B.node[d3][mx].n, s);
+  launch_pi_step(B.d_jobs, B.close.d, B.close.n, cs, s);
+}
+
+// Power iterations driven from the host until every job that iterates has
+// converged (or failed after 1000 iterations): 8 iterations between checks.
+int run_pi_host(dnlg_ctx *ctx, Batch &B, int cs) {
+  hipStream_t s = ctx->stream;
+  const int n = (int)B.jobs.size();
+  for (int round = 0; round < 200; round++) {
+    for (int k = 0; k < 8; k++) launch_pi_iteration(ctx, B, cs);
+    launch_pi_pending(B.d_jobs, n, B.d_pend, s);
+    if (hipMemcpyAsync(B.h_pend.data(), B.d_pend, n * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return DNLG_E_HIP;
+    bool pending = false;
+    for (int v : B.h_pend) pending = pending || v != 0;
+    if (!pending) return 0;
+  }
+  return DNLG_E_HIP;  // unreachable: k_pi_step stops a job after 1000 iterations
+}
+
 void launch_post(dnlg_ctx *ctx, Batch &B, int cs, bool courant, bool fold = false) {
   hipStream_t s = ctx->stream;
   for (int d3 = 0; d3 < 2; d3++)
     for (int mx = 0; mx < 2; mx++)
       launch_correct(d3 ? 3 : 2, mx == 1, B.d_jobs, B.node[d3][mx].d, B.node[d3][mx].n, cs, fold ? 1 : 0, s);
-  if (courant)
+  if (courant && B.has_courant)
     for (int hx = 0; hx < 2; hx++) launch_courant(hx ? 3 : 2, B.d_jobs, B.cour[hx].d, B.cour[hx].n, cs, 0, s);
+  if (courant && B.has_pi) {
+    // a few warm-started iterations in the sequence; a job that has not
+    // converged stalls at the next k_cycle_step and the host finishes it
+    launch_pi_begin(B.d_jobs, (int)B.jobs.size(), cs, 0, s);
+    for (int k = 0; k < ctx->pi_slots; k++) launch_pi_iteration(ctx, B, cs);
+  }
 }
 
 // Cycles j0 .. j1-1 of a launch sequence whose running jobs share the
@@ -232,6 +279,14 @@ int run_preloop(dnlg_ctx *ctx, Batch &B) {
     for (int mx = 0; mx < 2; mx++) launch_mass_gather(d3 ? 3 : 2, B.d_jobs, B.node[d3][mx].d, B.node[d3][mx].n, s);
   for (int hx = 0; hx < 2; hx++) launch_courant(hx ? 3 : 2, B.d_jobs, B.cour[hx].d, B.cour[hx].n, 0, 1, s);
   launch_force_dt(B.d_jobs, n, s);
+  if (B.has_pi) {
+    // cold start of the power iteration (Model.C:966-970) on the reference
+    // Jacobian and the lumped mass
+    launch_pi_begin(B.d_jobs, n, 0, 1, s);
+    int st = run_pi_host(ctx, B, 0);
+    if (st) return st;
+    launch_pi_finish(B.d_jobs, n, 1, s);
+  }
   for (int hx = 0; hx < 2; hx++) {
     const int type = hx ? DNLG_EL_HEX8N_3D : DNLG_EL_QUA4N_AX;
     launch_elem_fp64(type, B.d_jobs, B.elem[hx][0].d, B.elem[hx][0].n, 0, 0, s);
@@ -379,33 +434,44 @@ int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vec
         prog_last = now;
         int active = 0;
         for (int i = 0; i < n; i++)
-          if ((int)B->h_snap[5 * i + 3] == 0 && B->h_snap[5 * i] < B->h_snap[5 * i + 2]) active++;
+          if ((int)B->h_snap[kSnap * i + 3] == 0 && B->h_snap[kSnap * i] < B->h_snap[kSnap * i + 2]) active++;
         std::fprintf(stderr, "[dnlg] batch of %d jobs: %lld cycles launched, %d active\n", n, (long long)launched,
                      active);
         std::fflush(stderr);
       }
     }
     launch_snapshot(B->d_jobs, n, B->d_snap, s);
-    if (hipMemcpyAsync(B->h_snap.data(), B->d_snap, 5 * n * sizeof(double), hipMemcpyDeviceToHost, s) !=
+    if (hipMemcpyAsync(B->h_snap.data(), B->d_snap, kSnap * n * sizeof(double), hipMemcpyDeviceToHost, s) !=
             hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
       ctx->err = "device error during the cycle loop";
       return DNLG_E_HIP;
     }
     if ((st = collect())) return st;
+    bool stalled = false;
+    for (int i = 0; i < n; i++) stalled = stalled || B->h_snap[kSnap * i + 5] != 0.0;
+    if (stalled) {
+      // power iterations that the launch sequence's slots did not finish
+      if ((st = run_pi_host(ctx, *B, 0))) {
+        ctx->err = "power iteration failed";
+        return st;
+      }
+      launch_pi_finish(B->d_jobs, n, 0, s);
+      continue;
+    }
     double need = 0.0;
     int phase = -2, freq = 0;  // common increment phase of the running jobs, -1 if none
     for (int i = 0; i < n; i++) {
-      const double t = B->h_snap[5 * i], dt = B->h_snap[5 * i + 1], up = B->h_snap[5 * i + 2];
-      const int js = (int)B->h_snap[5 * i + 3];
+      const double t = B->h_snap[kSnap * i], dt = B->h_snap[kSnap * i + 1], up = B->h_snap[kSnap * i + 2];
+      const int js = (int)B->h_snap[kSnap * i + 3];
       jobs[i]->status = js;
       jobs[i]->t_now = t;
       jobs[i]->up_to = up;
-      jobs[i]->inc_now = (int64_t)B->h_snap[5 * i + 4];
+      jobs[i]->inc_now = (int64_t)B->h_snap[kSnap * i + 4];
       if (js != 0 || !(t < up)) continue;
       need = std::max(need, dt > 0 ? std::ceil((up - t) / dt) : 1.0);
       const int f = jobs[i]->host.dt_freq;
-      const int p = (int)((int64_t)B->h_snap[5 * i + 4] % f);
+      const int p = (int)((int64_t)B->h_snap[kSnap * i + 4] % f);
       if (phase == -2) {
         phase = p;
         freq = f;

@@ -65,13 +65,28 @@ struct alignas(128) Sync {
   int32_t st[2];
 };
 
+// Power-iteration time step (Model::computePowerIterationTimeStep,
+// Model.C:942-1009) of one job: the iterate lives in pi_ev[cur], normalised
+// as pi_ev[cur] / scale (the reference divides in place by the same value).
+struct PiState {
+  double fmax;    // last maximum (Model.h:46), 0 before the first call
+  double scale;   // divisor of the stored iterate (1 for the starting vector)
+  int32_t cur;    // which pi_ev buffer holds the iterate
+  int32_t iter;   // iterations of the current call
+  int32_t call;   // a time-step computation is in progress
+  int32_t done;   // ... and has converged (or failed)
+  int32_t stall;  // a cycle ended before its iteration converged
+  int32_t pad;
+};
+
 struct JobDev {
   Control c[2];
   Sync sync;
+  PiState pi;
   // sizes
   int32_t type, dim, nn, nip;
   int64_t N, NE, G;  // nodes, elements, integration points
-  int32_t prec, track, law, dt_freq;
+  int32_t prec, track, law, dt_freq, dt_method;
   // material (Material.h:110-169) and law parameters
   double rho0, young, nu, twoG, K, heatFrac, eta, cp, T0;
   double lp[8];
@@ -99,6 +114,11 @@ struct JobDev {
   float *elemF32;       // mixed-precision variant of elemF
   double *cmin;         // Courant partial minima, one per Courant block
   int32_t n_cmin;
+  // power iteration (DNLG_DT_POWER_ITERATION only): iterates [dim][N] x 2,
+  // per-node-block partial maxima
+  double *pi_ev[2];
+  double *pi_bmax;
+  int32_t n_pibmax;
   // integration-point state, index e*nip + p; tensors as [6][G]
   double *sig;  // FP64 stress (FP64 mode)
   float *sig32; // FP32 stress (mixed mode)

@@ -32,7 +32,19 @@ void launch_force_dt(JobDev *const *jobs, int n, hipStream_t s);
 void launch_set_up_to(JobDev *const *jobs, int n, const double *up_to, hipStream_t s);
 void launch_prep(int type, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s);
 void launch_mass_gather(int dim, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s);
-// per job: {t, dt, up_to, status, inc} snapshot for the host loop
+// per job: {t, dt, up_to, status, inc, power-iteration stall} snapshot for
+// the host loop (6 doubles per job)
 void launch_snapshot(JobDev *const *jobs, int n, double *out, hipStream_t s);
+// power-iteration time step: begin (force: the pre-loop's forced step,
+// otherwise the jobs whose cycle in slot cs is due), one iteration = elem
+// (per element type, over the Courant task lists) + node (per dim and
+// precision) + step (one block per job), finish (host-driven loops: the
+// converged step into slot 0), pending flags (1 = still iterating)
+void launch_pi_begin(JobDev *const *jobs, int n, int cs, int force, hipStream_t s);
+void launch_pi_elem(int type, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s);
+void launch_pi_node(int dim, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s);
+void launch_pi_step(JobDev *const *jobs, const Task *tasks, int nblk, int cs, hipStream_t s);
+void launch_pi_finish(JobDev *const *jobs, int n, int preloop, hipStream_t s);
+void launch_pi_pending(JobDev *const *jobs, int n, int *out, hipStream_t s);
 
 }  // namespace dnlg

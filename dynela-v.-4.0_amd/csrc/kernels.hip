@@ -111,27 +111,27 @@ __device__ __forceinline__ void store_control(JobDev *J, int slot, const Control
 }
 
 // Close the cycle that ran last (if any) and, with `open`, open the next.
-// status: the job's status after that cycle; dt_crit > 0: the job's minimum
-// element L / c for the time-step update.  Returns whether the field0 BC
+// status: the job's status after that cycle; freq > 0: the maximum
+// frequency of the time-step update (Courant: 2 / min L/c; power iteration:
+// sqrt(fmax)), dt = safety Omega_s / freq.  Returns whether the field0 BC
 // values of the new cycle differ from what the corrector left in field0
 // (first cycle of an advance, a ramped speed BC, or a speed BC under a
 // changed time step).
-__device__ __forceinline__ bool close_open(CJobDev *C, Control &c, int status, double dt_crit, bool open) {
+__device__ __forceinline__ bool close_open(CJobDev *C, Control &c, int status, double freq, bool open) {
   const double dt_last = c.dt;
   const bool resumed = !c.pending;
   if (c.pending && status == 0) {
     c.t += c.dt;
     c.f0 = 1 - c.f0;
-    if (dt_crit > 0.0) {
-      const double fmax = 2.0 / dt_crit;
-      c.dt = C->safety * C->omegaS / fmax;
-    }
+    if (freq > 0.0) c.dt = C->safety * C->omegaS / freq;
   }
   c.pending = 0;
   bool bc_work = false;
   if (open) {
     c.preloop = 0;
-    c.running = (status == 0 && c.t < c.up_to) ? 1 : 0;
+    // a job whose power iteration stalled waits for the host-driven
+    // iteration (k_pi_finish clears the flag)
+    c.running = (status == 0 && c.t < c.up_to && !C->pi.stall) ? 1 : 0;
     if (c.running) {
       if (c.t + c.dt > c.up_to) c.dt = c.up_to - c.t;
       c.inc++;
@@ -731,6 +731,7 @@ __global__ __launch_bounds__(256) void k_courant(JobDev *const *jobs, const Task
   const Task tk = load_task(tasks);
   JobDev *J = tk.jd;
   CJobDev *C = (CJobDev *)J;
+  if (C->dt_method != DNLG_DT_COURANT) return;
   if (!force) {
     // Explicit.C:243-251: only while t_{n+1} < t_up_to, every dt_freq increments
     const Control c = load_control(C, cs);
@@ -816,8 +817,19 @@ __global__ __launch_bounds__(256) void k_cycle_step(JobDev *const *jobs, const T
   // through the global space: a close launch may store into slot cs
   Control c = load_control<const DNLG_GLOBAL Control>(C, cs);
   const int status = job_status(C);
-  const double dt_crit = courant_ran(c, status) ? block_min_partials(C) : 0.0;
-  const bool bc_work = close_open(C, c, status, dt_crit, open != 0);
+  double freq = 0.0;
+  bool stall = false;
+  if (courant_ran(c, status)) {
+    if (C->dt_method == DNLG_DT_POWER_ITERATION) {
+      // the iteration slots of the launch sequence did not converge: close
+      // the cycle and stop the job until the host-driven iteration finishes
+      if (C->pi.done) freq = sqrt(C->pi.fmax);
+      else stall = true;
+    } else {
+      freq = 2.0 / block_min_partials(C);
+    }
+  }
+  const bool bc_work = close_open(C, c, status, freq, open != 0 && !stall);
   const int64_t k = tk.first + threadIdx.x;
   if (c.running && bc_work && k < tk.end) {
     const int64_t n = gbl(C->bcn)[k];
@@ -827,20 +839,289 @@ __global__ __launch_bounds__(256) void k_cycle_step(JobDev *const *jobs, const T
       bc_field0<2>(C, n, c.f0, c.t, c.dt);
   }
   __syncthreads();
-  if (tk.blk == 0 && threadIdx.x == 0) store_control(J, out, c);
+  if (tk.blk == 0 && threadIdx.x == 0) {
+    store_control(J, out, c);
+    if (stall) gbl(&J->pi)->stall = 1;
+  }
 }
 
-// Host-loop snapshot: t, dt, up_to, status, increment of every job.
+// ---------------------------------------------------------------------------
+// Power-iteration time step (Model::computePowerIterationTimeStep,
+// Model.C:942-1009).  One iteration = k_pi_elem (K_e v per element into its
+// incidence slots), k_pi_node (v <- M^-1 (v + sum_e K_e v_e), block maxima of
+// |v|), k_pi_step (fmax, convergence, buffer swap).  A job iterates while
+// its pi.call is set and pi.done is not.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool pi_active(CJobDev *C) {
+  return C->dt_method == DNLG_DT_POWER_ITERATION && C->pi.call && !C->pi.done;
+}
+
+// Starts a time-step computation: force (the pre-loop's forced step,
+// Explicit.C:152-154) or the cycle in slot cs ran and its step is due
+// (courant_ran).  A job whose previous computation stalled keeps its state
+// for the host loop.  One thread per job.
+__global__ void k_pi_begin(JobDev *const *jobs, int n_jobs, int cs, int force) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_jobs) return;
+  JobDev *J = jobs[j];
+  if (J->dt_method != DNLG_DT_POWER_ITERATION || J->pi.stall) return;
+  const Control &c = J->c[cs];
+  const int status = J->sync.st[0] != 0 ? J->sync.st[0] : J->sync.st[1];
+  const bool need = force ? status == 0 : courant_ran(c, status);
+  J->pi.call = need ? 1 : 0;
+  J->pi.done = 0;
+  J->pi.iter = 0;
+}
+
+// K_e v matrix-free: per IP the strain B v, the stress C (B v) with the
+// Hooke matrix of Material.C:98-152 (shear diagonal 1 - 2 nu, axisymmetric
+// hoop row), and B^T C B v w detJ (x 2 pi r), on the Jacobian of the current
+// coordinates (Explicit.C:245-249 computes it before the time step).  Equal
+// to the reference's assembled K_e times v up to rounding.
+template <int NN, int DIM>
+__global__ __launch_bounds__(256) void k_pi_elem(JobDev *const *jobs, const Task *tasks) {
+  const Task tk = load_task(tasks);
+  JobDev *J = tk.jd;
+  CJobDev *C = (CJobDev *)J;
+  if (!pi_active(C)) return;
+  const int64_t e = tk.first + threadIdx.x;
+  if (e >= tk.end) return;
+  const int64_t N = C->N;
+  const DNLG_GLOBAL double *ev = gbl(C->pi_ev[C->pi.cur]);
+  const double scale = C->pi.scale;
+  double X[NN][DIM], v[NN][DIM], f[NN][DIM];
+#pragma unroll
+  for (int a = 0; a < NN; a++) {
+    const int nd = gbl(C->conn)[e * NN + a];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      X[a][d] = gbl(C->x)[d * N + nd];
+      v[a][d] = ev[d * N + nd] / scale;
+      f[a][d] = 0.0;
+    }
+  }
+  const double nu = C->nu;
+  const double fct = C->young / ((1 + nu) * (1 - 2 * nu));
+  const double c11 = (1 - nu) * fct, c12 = nu * fct, c44 = (1 - 2 * nu) * fct;
+  for (int p = 0; p < NN; p++) {
+    double Jm[DIM][DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; i++)
+#pragma unroll
+      for (int j = 0; j < DIM; j++) Jm[i][j] = 0.0;
+#pragma unroll
+    for (int a = 0; a < NN; a++)
+#pragma unroll
+      for (int i = 0; i < DIM; i++)
+#pragma unroll
+        for (int j = 0; j < DIM; j++) Jm[i][j] += tabD<NN>(p, a, j) * X[a][i];
+    double det, inv[DIM][DIM];
+    if constexpr (DIM == 3) {
+      const double *dd = &Jm[0][0];
+      det = dd[1] * dd[5] * dd[6] - dd[2] * dd[4] * dd[6] + dd[2] * dd[3] * dd[7] - dd[0] * dd[5] * dd[7] -
+            dd[1] * dd[3] * dd[8] + dd[0] * dd[4] * dd[8];
+      const double id = 1.0 / det;
+      inv[0][0] = (dd[4] * dd[8] - dd[5] * dd[7]) * id;
+      inv[0][1] = (dd[2] * dd[7] - dd[1] * dd[8]) * id;
+      inv[0][2] = (dd[1] * dd[5] - dd[2] * dd[4]) * id;
+      inv[1][0] = (dd[5] * dd[6] - dd[3] * dd[8]) * id;
+      inv[1][1] = (dd[0] * dd[8] - dd[2] * dd[6]) * id;
+      inv[1][2] = (dd[2] * dd[3] - dd[0] * dd[5]) * id;
+      inv[2][0] = (dd[3] * dd[7] - dd[4] * dd[6]) * id;
+      inv[2][1] = (dd[1] * dd[6] - dd[0] * dd[7]) * id;
+      inv[2][2] = (dd[0] * dd[4] - dd[1] * dd[3]) * id;
+    } else {
+      det = Jm[0][0] * Jm[1][1] - Jm[1][0] * Jm[0][1];
+      const double id = 1.0 / det;
+      inv[0][0] = Jm[1][1] * id;
+      inv[0][1] = -Jm[0][1] * id;
+      inv[1][0] = -Jm[1][0] * id;
+      inv[1][1] = Jm[0][0] * id;
+    }
+    double dN[NN][DIM];
+#pragma unroll
+    for (int a = 0; a < NN; a++)
+#pragma unroll
+      for (int j = 0; j < DIM; j++) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < DIM; k++) s += tabD<NN>(p, a, k) * inv[k][j];
+        dN[a][j] = s;
+      }
+    if constexpr (DIM == 3) {
+      double exx = 0, eyy = 0, ezz = 0, gxy = 0, gyz = 0, gxz = 0;
+#pragma unroll
+      for (int a = 0; a < NN; a++) {
+        exx += dN[a][0] * v[a][0];
+        eyy += dN[a][1] * v[a][1];
+        ezz += dN[a][2] * v[a][2];
+        gxy += dN[a][1] * v[a][0] + dN[a][0] * v[a][1];
+        gyz += dN[a][2] * v[a][1] + dN[a][1] * v[a][2];
+        gxz += dN[a][2] * v[a][0] + dN[a][0] * v[a][2];
+      }
+      const double sxx = c11 * exx + c12 * (eyy + ezz), syy = c11 * eyy + c12 * (exx + ezz);
+      const double szz = c11 * ezz + c12 * (exx + eyy);
+      const double sxy = c44 * gxy, syz = c44 * gyz, sxz = c44 * gxz;
+      const double w = det;  // Gauss weights are 1
+#pragma unroll
+      for (int a = 0; a < NN; a++) {
+        f[a][0] += (dN[a][0] * sxx + dN[a][1] * sxy + dN[a][2] * sxz) * w;
+        f[a][1] += (dN[a][1] * syy + dN[a][0] * sxy + dN[a][2] * syz) * w;
+        f[a][2] += (dN[a][2] * szz + dN[a][1] * syz + dN[a][0] * sxz) * w;
+      }
+    } else {
+      double r = 0.0;
+#pragma unroll
+      for (int a = 0; a < NN; a++) r += tabN<NN>(p, a) * X[a][0];
+      double err = 0, ezz = 0, grz = 0, ett = 0;
+#pragma unroll
+      for (int a = 0; a < NN; a++) {
+        err += dN[a][0] * v[a][0];
+        ezz += dN[a][1] * v[a][1];
+        grz += dN[a][1] * v[a][0] + dN[a][0] * v[a][1];
+        ett += tabN<NN>(p, a) / r * v[a][0];
+      }
+      const double srr = c11 * err + c12 * (ezz + ett), szz = c11 * ezz + c12 * (err + ett);
+      const double stt = c11 * ett + c12 * (err + ezz), srz = c44 * grz;
+      const double w = det * k2PI * r;
+#pragma unroll
+      for (int a = 0; a < NN; a++) {
+        f[a][0] += (dN[a][0] * srr + dN[a][1] * srz + tabN<NN>(p, a) / r * stt) * w;
+        f[a][1] += (dN[a][1] * szz + dN[a][0] * srz) * w;
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < NN; a++) {
+    DNLG_GLOBAL double *out = gbl(C->elemF) + (int64_t)gbl(C->e2slot)[e * NN + a] * DIM;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) out[d] = f[a][d];
+  }
+}
+
+// v <- M^-1 (v + sum_e K_e v_e): the element products summed per node in
+// element order onto the normalised iterate (Vector::gatherFrom,
+// Vector.C:893-945), divided by the lumped mass (MatrixDiag::divideBy,
+// MatrixDiag.C:655-670); the block's maximum |v| to pi_bmax.
+template <int DIM>
+__global__ __launch_bounds__(kNodeBlock) void k_pi_node(JobDev *const *jobs, const Task *tasks) {
+  __shared__ double s_max[kNodeBlock / 64];
+  const Task tk = load_task(tasks);
+  JobDev *J = tk.jd;
+  CJobDev *C = (CJobDev *)J;
+  if (!pi_active(C)) return;
+  const int64_t n = tk.first + threadIdx.x;
+  double vmax = 0.0;
+  if (n < tk.end) {
+    const int64_t N = C->N;
+    const int cur = C->pi.cur;
+    const DNLG_GLOBAL double *src = gbl(C->pi_ev[cur]);
+    DNLG_GLOBAL double *dst = gbl(C->pi_ev[1 - cur]);
+    const double scale = C->pi.scale;
+    double acc[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) acc[d] = src[d * N + n] / scale;
+    const DNLG_GLOBAL double *ef = gbl(C->elemF);
+    const int64_t k1 = gbl(C->n2e_ptr)[n + 1];
+    int64_t k = gbl(C->n2e_ptr)[n];
+    for (; k + 4 <= k1; k += 4) add_slots<4, DIM>(ef, k, acc);
+    if (k + 2 <= k1) {
+      add_slots<2, DIM>(ef, k, acc);
+      k += 2;
+    }
+    if (k < k1) add_slots<1, DIM>(ef, k, acc);
+    const double m = gbl(C->mass)[n];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      acc[d] /= m;
+      dst[d * N + n] = acc[d];
+      vmax = fmax(vmax, fabs(acc[d]));
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) vmax = fmax(vmax, __shfl_xor(vmax, off));
+  if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = vmax;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = s_max[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmax(m, s_max[w]);
+    gbl(C->pi_bmax)[tk.blk] = m;
+  }
+}
+
+// fmax = max |v|, convergence |fmax - fmax_prev| / fmax < 1e-4, at most
+// 1000 iterations, checked in the reference's order (Model.C:990-1006);
+// the new iterate is normalised by fmax where it is read.  One block per
+// job; cs: the control slot whose increment parity takes a failure.
+__global__ __launch_bounds__(256) void k_pi_step(JobDev *const *jobs, const Task *tasks, int cs) {
+  __shared__ double s_max[4];
+  const Task tk = load_task(tasks);
+  JobDev *J = tk.jd;
+  CJobDev *C = (CJobDev *)J;
+  if (!pi_active(C)) return;
+  double v = 0.0;
+  for (int k = threadIdx.x; k < C->n_pibmax; k += blockDim.x) v = fmax(v, gbl(C->pi_bmax)[k]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+  if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double fmax_ = s_max[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) fmax_ = fmax(fmax_, s_max[w]);
+    DNLG_GLOBAL PiState &P = *gbl(&J->pi);
+    const double convergence = fabs(fmax_ - P.fmax) / fmax_;
+    P.fmax = fmax_;
+    P.scale = fmax_;
+    P.cur = 1 - P.cur;
+    P.iter = P.iter + 1;
+    if (P.iter > 1000) {
+      set_status(J, DNLG_E_PI_DIVERGED, (int)(C->c[cs].inc & 1));
+      P.done = 1;
+    } else if (convergence < 1e-4) {
+      P.done = 1;
+    }
+  }
+}
+
+// After a host-driven iteration: the jobs it ran for (preloop != 0: every
+// job that iterated in the pre-loop; otherwise the stalled jobs) take their
+// step, safety Omega_s / sqrt(fmax), in slot 0 when the iteration
+// converged; the stall and call flags are cleared.  One thread per job.
+__global__ void k_pi_finish(JobDev *const *jobs, int n_jobs, int preloop) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_jobs) return;
+  JobDev *J = jobs[j];
+  if (J->dt_method != DNLG_DT_POWER_ITERATION || !J->pi.call) return;
+  if (!preloop && !J->pi.stall) return;
+  if (J->pi.done && J->sync.st[0] == 0 && J->sync.st[1] == 0) {
+    const double freq = sqrt(J->pi.fmax);
+    J->c[0].dt = J->safety * J->omegaS / freq;
+  }
+  J->pi.call = 0;
+  J->pi.stall = 0;
+}
+
+// Jobs of the batch still iterating: out[j] = 1 (host-driven loops).
+__global__ void k_pi_pending(JobDev *const *jobs, int n_jobs, int *out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_jobs) return;
+  const JobDev *J = jobs[j];
+  out[j] = (J->dt_method == DNLG_DT_POWER_ITERATION && J->pi.call && !J->pi.done) ? 1 : 0;
+}
+
+// Host-loop snapshot: t, dt, up_to, status, increment, power-iteration stall
+// of every job.
 __global__ void k_snapshot(JobDev *const *jobs, int n_jobs, double *out) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n_jobs) return;
   const JobDev *J = jobs[j];
   const Control &c = J->c[0];
-  out[5 * j + 0] = c.t;
-  out[5 * j + 1] = c.dt;
-  out[5 * j + 2] = c.up_to;
-  out[5 * j + 3] = (double)(J->sync.st[0] != 0 ? J->sync.st[0] : J->sync.st[1]);
-  out[5 * j + 4] = (double)c.inc;
+  out[6 * j + 0] = c.t;
+  out[6 * j + 1] = c.dt;
+  out[6 * j + 2] = c.up_to;
+  out[6 * j + 3] = (double)(J->sync.st[0] != 0 ? J->sync.st[0] : J->sync.st[1]);
+  out[6 * j + 4] = (double)c.inc;
+  out[6 * j + 5] = (double)J->pi.stall;
 }
 
 // Target time of the current advance, per job.
@@ -855,6 +1136,7 @@ __global__ void k_force_dt(JobDev *const *jobs, int n_jobs) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n_jobs) return;
   JobDev *J = jobs[j];
+  if (J->dt_method != DNLG_DT_COURANT) return;  // k_pi_finish sets it
   double m = J->cmin[0];
   for (int k = 1; k < J->n_cmin; k++) m = fmin(m, J->cmin[k]);
   const double fmax = 2.0 / m;
@@ -1020,6 +1302,33 @@ void launch_courant(int dim, JobDev *const *jobs, const Task *tasks, int nblk, i
 void launch_cycle_step(JobDev *const *jobs, const Task *tasks, int nblk, int cs, int open, int out, hipStream_t s) {
   if (nblk <= 0) return;
   hipLaunchKernelGGL(k_cycle_step, dim3(nblk), dim3(256), 0, s, jobs, tasks, cs, open, out);
+}
+void launch_pi_begin(JobDev *const *jobs, int n, int cs, int force, hipStream_t s) {
+  hipLaunchKernelGGL(k_pi_begin, dim3((n + 63) / 64), dim3(64), 0, s, jobs, n, cs, force);
+}
+void launch_pi_elem(int type, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s) {
+  if (nblk <= 0) return;
+  if (type == DNLG_EL_HEX8N_3D)
+    hipLaunchKernelGGL((k_pi_elem<8, 3>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
+  else
+    hipLaunchKernelGGL((k_pi_elem<4, 2>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
+}
+void launch_pi_node(int dim, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s) {
+  if (nblk <= 0) return;
+  if (dim == 3)
+    hipLaunchKernelGGL(k_pi_node<3>, dim3(nblk), dim3(kNodeBlock), 0, s, jobs, tasks);
+  else
+    hipLaunchKernelGGL(k_pi_node<2>, dim3(nblk), dim3(kNodeBlock), 0, s, jobs, tasks);
+}
+void launch_pi_step(JobDev *const *jobs, const Task *tasks, int nblk, int cs, hipStream_t s) {
+  if (nblk <= 0) return;
+  hipLaunchKernelGGL(k_pi_step, dim3(nblk), dim3(256), 0, s, jobs, tasks, cs);
+}
+void launch_pi_finish(JobDev *const *jobs, int n, int preloop, hipStream_t s) {
+  hipLaunchKernelGGL(k_pi_finish, dim3((n + 63) / 64), dim3(64), 0, s, jobs, n, preloop);
+}
+void launch_pi_pending(JobDev *const *jobs, int n, int *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_pi_pending, dim3((n + 63) / 64), dim3(64), 0, s, jobs, n, out);
 }
 void launch_snapshot(JobDev *const *jobs, int n, double *out, hipStream_t s) {
   hipLaunchKernelGGL(k_snapshot, dim3((n + 63) / 64), dim3(64), 0, s, jobs, n, out);

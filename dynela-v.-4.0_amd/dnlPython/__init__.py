@@ -14,20 +14,20 @@ per-entity build path (Model.C:91,122) for large meshes.
 
 Errors never exit the interpreter: engine statuses become exceptions
 (``NegativeJacobianError``, ``NewtonRaphsonError``, ``NonFiniteError``,
-``DnlgError``).
+``PowerIterationError``, ``DnlgError``).
 """
 import ctypes
 
 import numpy as np
 
 from . import _lib
-from ._lib import (DnlgError, NegativeJacobianError, NewtonRaphsonError,  # noqa: F401
+from ._lib import (DnlgError, NegativeJacobianError, NewtonRaphsonError, PowerIterationError,  # noqa: F401
                    NonFiniteError)
 
 __all__ = ["DynELA", "NodeSet", "ElementSet", "Material", "JohnsonCookLaw", "ElastoplasticLaw",
            "VoceLinearLaw", "BoundaryRestrain", "BoundarySpeed", "RampFunction", "Explicit", "Field",
            "Element", "Vec3D", "Context", "DnlgError", "NegativeJacobianError", "NewtonRaphsonError",
-           "NonFiniteError", "batch_solve"]
+           "PowerIterationError", "NonFiniteError", "batch_solve"]
 
 
 class Element:

@@ -14,12 +14,12 @@ LIB_PATH = os.path.join(PKG_DIR, "libdnlg.so")
 
 # ---- constants of dnlg.h ---------------------------------------------------
 OK = 0
-E_INVALID, E_OOM, E_NEG_JACOBIAN, E_NR_DIVERGED, E_NONFINITE, E_HIP, E_STATE, E_CAPACITY = (
-    -1, -2, -3, -4, -5, -6, -7, -8)
+E_INVALID, E_OOM, E_NEG_JACOBIAN, E_NR_DIVERGED, E_NONFINITE, E_HIP, E_STATE, E_CAPACITY, E_PI_DIVERGED = (
+    -1, -2, -3, -4, -5, -6, -7, -8, -9)
 STATUS_NAMES = {
     OK: "OK", E_INVALID: "INVALID", E_OOM: "OUT_OF_MEMORY", E_NEG_JACOBIAN: "NEGATIVE_JACOBIAN",
     E_NR_DIVERGED: "NEWTON_RAPHSON_DIVERGED", E_NONFINITE: "NON_FINITE", E_HIP: "HIP_ERROR",
-    E_STATE: "INVALID_STATE", E_CAPACITY: "CAPACITY",
+    E_STATE: "INVALID_STATE", E_CAPACITY: "CAPACITY", E_PI_DIVERGED: "POWER_ITERATION_DIVERGED",
 }
 EL_QUA4N_AX = 3
 EL_HEX8N_3D = 4
@@ -149,8 +149,12 @@ class NonFiniteError(DnlgError):
     pass
 
 
+class PowerIterationError(DnlgError):
+    pass
+
+
 _ERRORS = {E_NEG_JACOBIAN: NegativeJacobianError, E_NR_DIVERGED: NewtonRaphsonError,
-           E_NONFINITE: NonFiniteError}
+           E_NONFINITE: NonFiniteError, E_PI_DIVERGED: PowerIterationError}
 
 
 def check(status, ctx=None):

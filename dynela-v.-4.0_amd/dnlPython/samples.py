@@ -46,10 +46,12 @@ def _jc_material(p):
     return mat
 
 
-def _solver(t_end, precision, track_tensors, freq=10):
+def _solver(t_end, precision, track_tensors, freq=10, dt_method=None):
     s = Explicit("Solver")
     s.setTimes(0.0, t_end)
     s.setComputeTimeStepFrequency(freq)
+    if dt_method is not None:
+        s.setTimeStepMethod(dt_method)
     s.setPrecision(precision)
     s.setTrackTensors(track_tensors)
     return s
@@ -62,7 +64,7 @@ def _set(numbers, name):
 
 
 def taylor_axi(nr=10, nz=50, radius=3.2e-3, length=32.4e-3, speed=227.0, t_end=80e-6, material=None,
-               precision="fp64", track_tensors=True, context=None, freq=10):
+               precision="fp64", track_tensors=True, context=None, freq=10, dt_method=None):
     """Taylor-Axi impact (Samples/Taylor/Taylor-Axi/Taylor.py): r = x, axis = y,
     bottom row restrained in y, axis nodes restrained in x (and z),
     initial axial speed -speed on every node."""
@@ -91,7 +93,7 @@ def taylor_axi(nr=10, nz=50, radius=3.2e-3, length=32.4e-3, speed=227.0, t_end=8
     v = BoundarySpeed("BC_speed")
     v.setValue(0, -speed, 0)
     model.attachInitialBC(v, _set(nums, "NS_All"))
-    model.add(_solver(t_end, precision, track_tensors, freq))
+    model.add(_solver(t_end, precision, track_tensors, freq, dt_method))
     return model
 
 
@@ -167,7 +169,7 @@ def _hex_cylinder(n, nz, radius, ys, k=None):
 
 
 def taylor_3d(n=20, nz=100, radius=3.2e-3, length=32.4e-3, speed=227.0, t_end=80e-6, material=None,
-              precision="fp64", track_tensors=True, context=None, freq=10):
+              precision="fp64", track_tensors=True, context=None, freq=10, dt_method=None):
     """3D Taylor impact of a full cylinder (no symmetry planes): O-grid
     cross-section of n*n + 4*n*(n/2) hexes, nz layers (n=20, nz=100 ->
     120 000 hexes); bottom face restrained along the axis (y), initial speed
@@ -190,12 +192,12 @@ def taylor_3d(n=20, nz=100, radius=3.2e-3, length=32.4e-3, speed=227.0, t_end=80
     v = BoundarySpeed("BC_speed")
     v.setValue(0, -speed, 0)
     model.attachInitialBC(v, _set(nums, "NS_All"))
-    model.add(_solver(t_end, precision, track_tensors, freq))
+    model.add(_solver(t_end, precision, track_tensors, freq, dt_method))
     return model
 
 
 def necking_3d(n=16, nz=60, radius=6.413e-3, length=53.334e-3, taper=0.02, pull=1.0, t_end=50e-6,
-               material=None, precision="fp64", track_tensors=True, context=None, freq=10):
+               material=None, precision="fp64", track_tensors=True, context=None, freq=10, dt_method=None):
     """Round bar in tension: radius reduced by `taper` at mid-length with a
     cosine profile, bottom face restrained along the axis (y), top face
     pulled at constant speed `pull` (BoundarySpeed, constant)."""
@@ -219,7 +221,7 @@ def necking_3d(n=16, nz=60, radius=6.413e-3, length=53.334e-3, taper=0.02, pull=
     top = BoundarySpeed("BC_top")
     top.setValue(0, pull, 0)
     model.attachConstantBC(top, _set(range(nz * nl + 1, (nz + 1) * nl + 1), "NS_Top"))
-    model.add(_solver(t_end, precision, track_tensors, freq))
+    model.add(_solver(t_end, precision, track_tensors, freq, dt_method))
     return model
 
 
@@ -234,7 +236,7 @@ def _voce_material(p):
 
 
 def necking_axi(nr=10, nz=80, radius=6.413e-3, length=53.334e-3, taper=0.02, pull=1.0, t_end=50e-6,
-                material=None, precision="fp64", track_tensors=True, context=None, freq=10):
+                material=None, precision="fp64", track_tensors=True, context=None, freq=10, dt_method=None):
     """Axisymmetric round bar in tension (4-node axisymmetric quads): the
     necking_3d geometry, loads and law on the (r, y) half-plane, nodes
     restrained radially on the axis."""
@@ -265,7 +267,7 @@ def necking_axi(nr=10, nz=80, radius=6.413e-3, length=53.334e-3, taper=0.02, pul
     top = BoundarySpeed("BC_top")
     top.setValue(0, pull, 0)
     model.attachConstantBC(top, _set(range(nz * (nr + 1) + 1, (nz + 1) * (nr + 1) + 1), "NS_Top"))
-    model.add(_solver(t_end, precision, track_tensors, freq))
+    model.add(_solver(t_end, precision, track_tensors, freq, dt_method))
     return model
 
 

@@ -44,6 +44,7 @@ extern "C" {
 #define DNLG_E_HIP (-6)          /* HIP runtime error                       */
 #define DNLG_E_STATE (-7)        /* call not valid in the job's state       */
 #define DNLG_E_CAPACITY (-8)     /* request exceeds a fixed capacity        */
+#define DNLG_E_PI_DIVERGED (-9)  /* power iteration > 1000 its (Model.C:1002) */
 
 /* ---- enumerations -------------------------------------------------------- */
 /* element types: values of the reference's element enum (Element.h:62-71) */

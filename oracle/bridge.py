@@ -57,6 +57,8 @@ def load():
         lib.oracle_chung_hulbert.argtypes = [ctypes.c_double, _dp]
         lib.oracle_ramp_value.restype = ctypes.c_double
         lib.oracle_ramp_value.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        lib.oracle_element_stiffness.argtypes = [ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double, _dp]
+        lib.oracle_job_pi_state.argtypes = [_vp, _dp, ctypes.POINTER(ctypes.c_int)]
         _lib = lib
     return _lib
 
@@ -119,9 +121,28 @@ class OracleJob:
     def status(self):
         return self.lib.oracle_job_status(self.h)
 
+    def pi_state(self):
+        """(last power-iteration maximum, iterations of the last call)"""
+        f, it = ctypes.c_double(), ctypes.c_int()
+        self.lib.oracle_job_pi_state(self.h, ctypes.byref(f), ctypes.byref(it))
+        return f.value, it.value
+
     def __del__(self):
         try:
             if self.h:
                 self.lib.oracle_job_destroy(self.h)
         except Exception:
             pass
+
+
+def element_stiffness(elem_type, xyz, young, poisson):
+    """Elastic stiffness matrix (nn dim x nn dim) of one element with node
+    coordinates xyz[a] = (x, y, z)."""
+    lib = load()
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    nd = xyz.shape[0] * (3 if elem_type == 4 else 2)  # DNLG_EL_HEX8N_3D = 4
+    K = np.zeros((nd, nd))
+    st = lib.oracle_element_stiffness(elem_type, p(xyz), young, poisson, K.ctypes.data_as(_dp))
+    if st != 0:
+        raise RuntimeError(f"oracle_element_stiffness failed: {st}")
+    return K

@@ -700,6 +700,12 @@ struct OracleJob {
   std::vector<double> sig, eps, epsP, dE, Rr;  /* [ip_global][6 or 9] */
   std::vector<double> pres, ep, dep, sy, gam, gamc, T, eint, einel, rho;
   std::vector<double> elemF; /* [e][nn*dim] */
+  /* power iteration state kept between time steps (Model.h:46, Model.C:966):
+   * the iterate in the reference's DOF order (node * dim + d) and the last
+   * maximum */
+  std::vector<double> pi_ev;
+  double pi_fmax = 0.0;
+  int pi_iterations = 0; /* iterations of the last call */
   double t = 0, dt = 0;
   int64_t inc = 0;
   bool started = false;
@@ -881,12 +887,171 @@ double courant(OracleJob *J) {
   return crit;
 }
 
-/* Solver::computeTimeStep (Solver.C:196-236), Courant branch */
-void compute_time_step(OracleJob *J, bool force) {
+/* Material::getHookeStiffnessMatrix (Material.C:98-152), threedimensional
+ * (6 x 6, rows xx yy zz xy yz xz) and axisymetric (4 x 4, rows rr zz rz tt)
+ * cases; the shear diagonal is (1 - 2 nu), as in the reference. */
+void hooke(const OracleJob *J, double C[6][6]) {
+  const double nu = J->mat.poisson, E = J->mat.young;
+  for (int i = 0; i < 6; i++)
+    for (int j = 0; j < 6; j++) C[i][j] = 0.0;
+  if (J->dim == 3) {
+    C[0][0] = C[1][1] = C[2][2] = 1 - nu;
+    C[3][3] = C[4][4] = C[5][5] = 1 - 2 * nu;
+    C[0][1] = C[1][0] = C[0][2] = C[2][0] = C[1][2] = C[2][1] = nu;
+  } else {
+    C[0][0] = C[1][1] = C[3][3] = 1 - nu;
+    C[2][2] = 1 - 2 * nu;
+    C[1][0] = C[0][1] = C[3][0] = C[0][3] = C[3][1] = C[1][3] = nu;
+  }
+  const double f = E / ((1 + nu) * (1 - 2 * nu));
+  for (int i = 0; i < 6; i++)
+    for (int j = 0; j < 6; j++) C[i][j] *= f;
+}
+
+/* Element3D::computeElasticStiffnessMatrix (Element3D.C:303-399) and
+ * ElementPlane::computeElasticStiffnessMatrix (ElementPlane.C:182-292),
+ * axisymmetric branch: K_e = sum_ip B^T C B w detJ (x 2 pi r), built as C B
+ * then B^T (C B) per IP in the reference's term order, on the Jacobian of the
+ * last compute_jacobian.  K is (nn dim) x (nn dim), row-major. */
+void element_stiffness(const OracleJob *J, int64_t e, double *K) {
+  const int nn = J->nn, dim = J->dim, nd = nn * dim;
+  double C[6][6];
+  hooke(J, C);
+  for (int k = 0; k < nd * nd; k++) K[k] = 0.0;
+  std::vector<double> CB((size_t)6 * nd);
+  auto cb = [&](int r, int c) -> double & { return CB[(size_t)r * nd + c]; };
+  auto kk = [&](int r, int c) -> double & { return K[(size_t)r * nd + c]; };
+  for (int p = 0; p < J->nip; p++) {
+    const int64_t g = e * J->nip + p;
+    const double *dn = &J->dN[g * nn * dim];
+    double WxdJ = J->tab.w[p] * J->detJ[g];
+    if (dim == 3) {
+      for (int i = 0; i < nn; i++) {
+        const int I = 3 * i;
+        const double dNx = dn[i * 3], dNy = dn[i * 3 + 1], dNz = dn[i * 3 + 2];
+        for (int r = 0; r < 6; r++) {
+          cb(r, I) = (C[r][0] * dNx + C[r][3] * dNy + C[r][5] * dNz);
+          cb(r, I + 1) = (C[r][1] * dNy + C[r][3] * dNx + C[r][4] * dNz);
+          cb(r, I + 2) = (C[r][2] * dNz + C[r][4] * dNy + C[r][5] * dNx);
+        }
+      }
+      for (int i = 0; i < nn; i++) {
+        const double dNx = dn[i * 3], dNy = dn[i * 3 + 1], dNz = dn[i * 3 + 2];
+        for (int j = 0; j < nn; j++) {
+          const int I = 3 * i, Jc = 3 * j;
+          for (int b = 0; b < 3; b++) {
+            kk(I, Jc + b) += (dNx * cb(0, Jc + b) + dNy * cb(3, Jc + b) + dNz * cb(5, Jc + b)) * WxdJ;
+            kk(I + 1, Jc + b) += (dNy * cb(1, Jc + b) + dNx * cb(3, Jc + b) + dNz * cb(4, Jc + b)) * WxdJ;
+            kk(I + 2, Jc + b) += (dNz * cb(2, Jc + b) + dNy * cb(4, Jc + b) + dNx * cb(5, Jc + b)) * WxdJ;
+          }
+        }
+      }
+    } else {
+      const double R = J->radius[g];
+      WxdJ *= k2PI * R;
+      for (int i = 0; i < nn; i++) {
+        const int I = 2 * i;
+        const double dNx = dn[i * 2], dNy = dn[i * 2 + 1];
+        const double NoverR = J->tab.N[p][i] / R;
+        for (int r = 0; r < 3; r++) {
+          cb(r, I) = (C[r][0] * dNx + C[r][2] * dNy);
+          cb(r, I + 1) = (C[r][1] * dNy + C[r][2] * dNx);
+        }
+        cb(3, I) = (C[3][0] * dNx + C[3][2] * dNy);
+        cb(3, I + 1) = (C[3][1] * dNy + C[3][2] * dNx);
+        for (int r = 0; r < 4; r++) cb(r, I) += C[r][3] * NoverR;
+      }
+      for (int i = 0; i < nn; i++) {
+        const double dNx = dn[i * 2], dNy = dn[i * 2 + 1];
+        const double NoverR = J->tab.N[p][i] / R;
+        for (int j = 0; j < nn; j++) {
+          const int I = 2 * i, Jc = 2 * j;
+          for (int b = 0; b < 2; b++) {
+            kk(I, Jc + b) += (dNx * cb(0, Jc + b) + dNy * cb(2, Jc + b)) * WxdJ;
+            kk(I + 1, Jc + b) += (dNy * cb(1, Jc + b) + dNx * cb(2, Jc + b)) * WxdJ;
+          }
+          kk(I, Jc) += cb(3, Jc) * NoverR * WxdJ;
+          kk(I, Jc + 1) += cb(3, Jc + 1) * NoverR * WxdJ;
+        }
+      }
+    }
+  }
+}
+
+/* Model::computePowerIterationTimeStep (Model.C:942-1009), full
+ * integration: element stiffness matrices, then iterations of
+ * v <- M^-1 (v + sum_e K_e v_e) (Vector::gatherFrom accumulates into the
+ * iterate, Vector.C:893-945; MatrixDiag::divideBy, MatrixDiag.C:655-670),
+ * fmax = max |v|, v <- v / fmax, until |fmax - fmax_prev| / fmax < 1e-4
+ * (Model.h:46-48).  The iterate and fmax_prev persist between calls; the
+ * first call starts from v_0 = 1, v_i = v_{i-1} - 2 / (n - 1).  Returns the
+ * maximum frequency sqrt(fmax), or a negative value after 1000 iterations. */
+double power_iteration(OracleJob *J) {
+  const int nd = J->nn * J->dim;
+  const int64_t ndof = (int64_t)J->dim * J->N;
+  std::vector<double> Ke((size_t)J->NE * nd * nd);
+#pragma omp parallel for schedule(static) num_threads(J->threads)
+  for (int64_t e = 0; e < J->NE; e++) element_stiffness(J, e, &Ke[(size_t)e * nd * nd]);
+  if (J->pi_fmax == 0.0 || (int64_t)J->pi_ev.size() != ndof) {
+    J->pi_ev.assign(ndof, 0.0);
+    J->pi_ev[0] = 1.0;
+    for (int64_t i = 1; i < ndof; i++) J->pi_ev[i] = J->pi_ev[i - 1] - 2. / (ndof - 1);
+  }
+  std::vector<double> &ev = J->pi_ev;
+  std::vector<double> ev0, loc(nd), y(nd);
+  double fmax = 0.0;
+  int iteration = 0;
+  while (true) {
+    iteration++;
+    ev0 = ev;
+    for (int64_t e = 0; e < J->NE; e++) {
+      const int32_t *cn = &J->conn[e * J->nn];
+      for (int a = 0; a < J->nn; a++)
+        for (int d = 0; d < J->dim; d++) loc[a * J->dim + d] = ev0[(int64_t)cn[a] * J->dim + d];
+      const double *K = &Ke[(size_t)e * nd * nd];
+      for (int r = 0; r < nd; r++) {
+        double sum = 0.0;
+        for (int c = 0; c < nd; c++) sum += K[(size_t)r * nd + c] * loc[c];
+        y[r] = sum;
+      }
+      for (int a = 0; a < J->nn; a++)
+        for (int d = 0; d < J->dim; d++) ev[(int64_t)cn[a] * J->dim + d] += y[a * J->dim + d];
+    }
+    for (int64_t i = 0; i < ndof; i++) ev[i] /= J->mass[i / J->dim];
+    fmax = std::fabs(ev[0]);
+    for (int64_t i = 1; i < ndof; i++)
+      if (std::fabs(ev[i]) > fmax) fmax = std::fabs(ev[i]);
+    for (int64_t i = 0; i < ndof; i++) ev[i] /= fmax;
+    const double convergence = std::fabs(fmax - J->pi_fmax) / fmax;
+    const bool ok = convergence < 1e-4;
+    J->pi_fmax = fmax;
+    if (iteration > 1000) { /* checked before the exit, as Model.C:1002 */
+      J->pi_iterations = iteration;
+      return -1.0;
+    }
+    if (ok) break;
+  }
+  J->pi_iterations = iteration;
+  return std::sqrt(fmax);
+}
+
+/* Solver::computeTimeStep (Solver.C:196-236): Courant (omega_max =
+ * 2 / dt_crit) or power-iteration branch */
+bool compute_time_step(OracleJob *J, bool force) {
   if (force || (J->inc % J->prm.dt_freq == 0)) {
-    const double fmax = 2.0 / courant(J);
+    double fmax;
+    if (J->prm.dt_method == DNLG_DT_POWER_ITERATION) {
+      fmax = power_iteration(J);
+      if (!(fmax > 0.0)) {
+        set_err(J, DNLG_E_PI_DIVERGED, "power iteration not converged in 1000 iterations");
+        return false;
+      }
+    } else {
+      fmax = 2.0 / courant(J);
+    }
     J->dt = J->prm.safety * J->omegaS / fmax;
   }
+  return true;
 }
 
 /* Element::computeStrains (Element.C:1103-1122) with the deformation
@@ -1198,7 +1363,8 @@ int oracle_job_create(const dnlg_mesh *mesh, const dnlg_material *mat, const dnl
   if (!mesh || !mat || !prm || !out) return DNLG_E_INVALID;
   if (mesh->elem_type != DNLG_EL_QUA4N_AX && mesh->elem_type != DNLG_EL_HEX8N_3D) return DNLG_E_INVALID;
   if (mesh->n_nodes <= 0 || mesh->n_elems <= 0 || !mesh->coords || !mesh->conn) return DNLG_E_INVALID;
-  if (prm->dt_method != DNLG_DT_COURANT || prm->stress_method != DNLG_STRESS_NR) return DNLG_E_INVALID;
+  if (prm->dt_method != DNLG_DT_COURANT && prm->dt_method != DNLG_DT_POWER_ITERATION) return DNLG_E_INVALID;
+  if (prm->stress_method != DNLG_STRESS_NR) return DNLG_E_INVALID;
   if (prm->dt_freq <= 0) return DNLG_E_INVALID;
   OracleJob *J = new OracleJob();
   J->tab = make_tables(mesh->elem_type);
@@ -1324,7 +1490,7 @@ int oracle_job_solve(OracleJob *J, double upTo) {
     J->started = true;
     if (!compute_jacobian(J, true)) return J->status;
     compute_mass(J);
-    compute_time_step(J, true);
+    if (!compute_time_step(J, true)) return J->status;
     if (!element_phases(J, J->dt)) return J->status;
   } else if (J->t < upTo) {
     /* A later call continues the loop (the engine's semantics; the
@@ -1351,7 +1517,7 @@ int oracle_job_solve(OracleJob *J, double upTo) {
     }
     if (J->t < upTo) {
       if (!compute_jacobian(J, false)) return J->status;
-      compute_time_step(J, false);
+      if (!compute_time_step(J, false)) return J->status;
     }
   }
   return DNLG_OK;
@@ -1554,5 +1720,39 @@ void oracle_chung_hulbert(double rho_b, double out[4]) {
 }
 
 double oracle_ramp_value(int type, double mn, double mx, double x) { return ramp_value(type, mn, mx, x); }
+
+/* Elastic stiffness matrix of one element (node coordinates xyz[a * 3 + c]),
+ * row-major (nn dim) x (nn dim), on its Jacobian; 0, or DNLG_E_NEG_JACOBIAN. */
+int oracle_element_stiffness(int type, const double *xyz, double young, double poisson, double *K) {
+  OracleJob J;
+  J.tab = make_tables(type);
+  J.dim = J.tab.dim;
+  J.nn = J.tab.nn;
+  J.nip = J.tab.nip;
+  J.N = J.nn;
+  J.NE = 1;
+  J.mat.young = young;
+  J.mat.poisson = poisson;
+  J.x.assign(3 * J.nn, 0.0);
+  for (int a = 0; a < J.nn; a++)
+    for (int c = 0; c < 3; c++) J.x[c * J.N + a] = xyz[a * 3 + c];
+  J.conn.resize(J.nn);
+  for (int a = 0; a < J.nn; a++) J.conn[a] = a;
+  J.detJ.assign(J.nip, 0.0);
+  J.detJ0.assign(J.nip, 0.0);
+  J.radius.assign(J.nip, 0.0);
+  J.dN.assign((size_t)J.nip * J.nn * J.dim, 0.0);
+  if (!element_jacobian(&J, 0, false)) return DNLG_E_NEG_JACOBIAN;
+  element_stiffness(&J, 0, K);
+  return DNLG_OK;
+}
+
+/* Power-iteration state of a job: the last maximum (omega_max^2) and the
+ * iterations of the last time-step computation. */
+int oracle_job_pi_state(const OracleJob *J, double *fmax, int *iterations) {
+  if (fmax) *fmax = J->pi_fmax;
+  if (iterations) *iterations = J->pi_iterations;
+  return DNLG_OK;
+}
 
 }  // extern "C"
