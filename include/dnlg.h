@@ -68,7 +68,8 @@ extern "C" {
 #define DNLG_RAMP_ERF 4
 #define DNLG_RAMP_SOFT 5
 
-/* time-step methods (Solver.h:28-33); only Courant is implemented */
+/* time-step methods (Solver.h:28-33); Courant and PowerIteration are
+ * implemented, PowerIterationUnder is not (DNLG_E_INVALID) */
 #define DNLG_DT_COURANT 0
 #define DNLG_DT_POWER_ITERATION 1
 #define DNLG_DT_POWER_ITERATION_UNDER 2
