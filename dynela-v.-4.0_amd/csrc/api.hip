@@ -12,6 +12,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstddef>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -120,6 +121,47 @@ int64_t footprint(int type, int64_t N, int64_t NE, int prec, int track) {
 
 }  // namespace
 
+
+namespace dnlg {
+
+// HistoryFile::save's row output (HistoryFile.C:301-312): "%10.7E " per
+// value, time first, a newline per row, flushed.
+int history_drain(dnlg_job *job) {
+  const int nf = (int)job->hist.size();
+  if (nf == 0) return DNLG_OK;
+  dnlg_ctx *ctx = job->ctx;
+  std::vector<HistFile> f(nf);
+  HIPCHK(ctx, hipMemcpy(f.data(), job->host.hf, nf * sizeof(HistFile), hipMemcpyDeviceToHost));
+  for (int i = 0; i < nf; i++) {
+    dnlg_history &h = job->hist[i];
+    const int64_t n_new = f[i].rows - h.drained;
+    if (n_new <= 0) continue;
+    if (n_new > h.cap) return fail(ctx, DNLG_E_CAPACITY, "history rows overran the device ring");
+    const int64_t width = 1 + h.n_items;
+    const size_t old = h.rows.size();
+    h.rows.resize(old + n_new * width);
+    const int64_t r0 = h.drained % h.cap;
+    const int64_t first = std::min(n_new, (int64_t)h.cap - r0);
+    HIPCHK(ctx, hipMemcpy(h.rows.data() + old, h.ring + r0 * width, first * width * sizeof(double),
+                          hipMemcpyDeviceToHost));
+    if (n_new > first)
+      HIPCHK(ctx, hipMemcpy(h.rows.data() + old + first * width, h.ring, (n_new - first) * width * sizeof(double),
+                            hipMemcpyDeviceToHost));
+    if (h.fp) {
+      for (int64_t r = 0; r < n_new; r++) {
+        const double *row = h.rows.data() + old + r * width;
+        for (int64_t k = 0; k < width; k++) std::fprintf(h.fp, "%10.7E ", row[k]);
+        std::fprintf(h.fp, "\n");
+      }
+      std::fflush(h.fp);
+    }
+    h.drained = f[i].rows;
+  }
+  return DNLG_OK;
+}
+
+}  // namespace dnlg
+
 // ===========================================================================
 extern "C" {
 
@@ -150,6 +192,7 @@ int dnlg_ctx_create(int device, dnlg_ctx **out) {
       double hexN[64], hexD[192], quaN[16], quaD[32];
       make_tables(hexN, hexD, quaN, quaD);
       upload_tables(hexN, hexD, quaN, quaD);
+      upload_extrapolation();
       if (hipDeviceSynchronize() != hipSuccess) {
         delete ctx;
         return DNLG_E_HIP;
@@ -448,6 +491,7 @@ int dnlg_job_destroy(dnlg_job *job) {
   hipStreamSynchronize(ctx->stream);
   batch_cache_forget(ctx, job);
   for (void *p : job->allocs) hipFree(p);
+  for (void *p : job->hist_tables) hipFree(p);
   delete job;
   return DNLG_OK;
 }
@@ -621,6 +665,229 @@ int dnlg_job_energies(dnlg_job *job, double out[4]) {
   out[1] = ie;
   out[2] = wp;
   out[3] = ke + ie;
+  return DNLG_OK;
+}
+
+// ---- history files (HistoryFile.C) ------------------------------------------
+int dnlg_job_add_history(dnlg_job *job, const char *path, double start_time, double save_time,
+                         const dnlg_history_item *items, int32_t n_items, const char *const *names,
+                         int32_t *file_id) {
+  if (!job) return DNLG_E_INVALID;
+  dnlg_ctx *ctx = job->ctx;
+  if (job->started) return fail(ctx, DNLG_E_STATE, "history files must be added before the first solve");
+  if (n_items < 0 || (n_items > 0 && !items)) return fail(ctx, DNLG_E_INVALID, "invalid history items");
+  if (!(save_time >= 0.0) || !std::isfinite(start_time))
+    return fail(ctx, DNLG_E_INVALID, "invalid history save times");
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, DNLG_E_HIP, "hipSetDevice");
+  const int fid = (int)job->hist.size();
+  const int nip = job->nn;
+  std::vector<HistItem> add;
+  std::vector<int64_t> ip_nodes;  // nodes of node-from-IP items, per item (-1: none)
+  bool need_ke = false, need_rho = false;
+  for (int32_t k = 0; k < n_items; k++) {
+    const dnlg_history_item &in = items[k];
+    HistItem it{};
+    it.file = fid;
+    it.col = 1 + k;
+    it.op = kHZero;
+    it.ip = -1;
+    it.index = in.index;
+    const int f = in.field;
+    if (f < 0 || f >= DNLG_FIELD_COUNT) return fail(ctx, DNLG_E_INVALID, "unknown history field");
+    // tensor blocks of Field.h:71-75: norm, XX, XY, XZ, YX, YY, YZ, ZX, ZY, ZZ
+    static const int tcomp[10] = {kHNorm, 0, 1, 2, 1, 3, 4, 2, 4, 5};
+    auto tensor = [&](int base, int src) {
+      it.src = src;
+      it.comp = tcomp[f - base];
+    };
+    auto ip_field = [&]() -> int {  // IP fields shared by node and element items
+      const int scalars[][2] = {{DNLG_FIELD_PLASTIC_STRAIN, kHEp}, {DNLG_FIELD_PLASTIC_STRAIN_RATE, kHDep},
+                                {DNLG_FIELD_GAMMA, kHGam},          {DNLG_FIELD_GAMMA_CUMULATE, kHGamc},
+                                {DNLG_FIELD_YIELD_STRESS, kHSy},    {DNLG_FIELD_T, kHT},
+                                {DNLG_FIELD_DENSITY, kHRho},        {DNLG_FIELD_PRESSURE, kHPres},
+                                {DNLG_FIELD_INTERNAL_ENERGY, kHEint}, {DNLG_FIELD_VON_MISES, kHVm}};
+      for (auto &sc : scalars)
+        if (f == sc[0]) {
+          it.src = sc[1];
+          it.comp = 0;
+          return 1;
+        }
+      if (f >= DNLG_FIELD_STRESS && f < DNLG_FIELD_STRESS + 10) {
+        tensor(DNLG_FIELD_STRESS, kHSig);
+        return 1;
+      }
+      if ((f >= DNLG_FIELD_STRAIN && f < DNLG_FIELD_STRAIN + 10) ||
+          (f >= DNLG_FIELD_PLASTIC_STRAIN_TENSOR && f < DNLG_FIELD_PLASTIC_STRAIN_TENSOR + 10)) {
+        if (!job->track) return -2;
+        if (f >= DNLG_FIELD_STRAIN && f < DNLG_FIELD_STRAIN + 10)
+          tensor(DNLG_FIELD_STRAIN, kHEps);
+        else
+          tensor(DNLG_FIELD_PLASTIC_STRAIN_TENSOR, kHEpsP);
+        return 1;
+      }
+      if ((f >= DNLG_FIELD_STRAIN_INC && f < DNLG_FIELD_STRAIN_INC + 10) ||
+          (f >= DNLG_FIELD_PLASTIC_STRAIN_INC && f < DNLG_FIELD_PLASTIC_STRAIN_INC + 10))
+        return -1;
+      return 0;
+    };
+    int64_t ipn = -1;
+    if (in.kind == DNLG_H_NODE) {
+      if (in.index < 0 || in.index >= job->N) return fail(ctx, DNLG_E_INVALID, "history node out of range");
+      if (f == DNLG_FIELD_MASS) {
+        it.op = kHNodal;
+        it.src = kHMass;
+        it.comp = 0;
+      } else if (f < DNLG_FIELD_DENSITY) {  // the five nodal vectors: norm, X, Y, Z
+        static const int vsrc[5] = {kHCoords, kHDisp, kHDispInc, kHSpeed, kHSpeedInc};
+        it.op = kHNodal;
+        it.src = vsrc[f / 4];
+        it.comp = f % 4 == 0 ? 3 : f % 4 - 1;
+      } else {
+        const int r = ip_field();
+        if (r == -1) return fail(ctx, DNLG_E_INVALID, "StrainInc and PlasticStrainInc are not kept by the engine");
+        if (r == -2) return fail(ctx, DNLG_E_STATE, "strain tensors not tracked (track_tensors = 0)");
+        if (r == 1) {
+          it.op = kHNodeIP;
+          ipn = in.index;
+        }
+      }
+    } else if (in.kind == DNLG_H_ELEMENT) {
+      if (in.index < 0 || in.index >= job->NE) return fail(ctx, DNLG_E_INVALID, "history element out of range");
+      if (in.int_pt > nip) return fail(ctx, DNLG_E_INVALID, "history integration point out of range");
+      const int r = ip_field();
+      if (r == -1) return fail(ctx, DNLG_E_INVALID, "StrainInc and PlasticStrainInc are not kept by the engine");
+      if (r == -2) return fail(ctx, DNLG_E_STATE, "strain tensors not tracked (track_tensors = 0)");
+      if (r == 1) {
+        it.op = kHElemIP;
+        // HistoryFile.C:247: point intPt - 1; Element.C:1229: a negative one is the mean
+        it.ip = in.int_pt > 0 ? in.int_pt - 1 : -1;
+      }
+    } else if (in.kind == DNLG_H_GLOBAL) {
+      if (f == DNLG_FIELD_TIME_STEP || f == DNLG_FIELD_REAL_TIME_STEP) it.op = kHDt;
+      if (f == DNLG_FIELD_KINETIC_ENERGY) it.op = kHKe;
+    } else {
+      return fail(ctx, DNLG_E_INVALID, "unknown history item kind");
+    }
+    need_ke = need_ke || it.op == kHKe;
+    need_rho = need_rho || ((it.op == kHNodeIP || it.op == kHElemIP) && it.src == kHRho);
+    add.push_back(it);
+    ip_nodes.push_back(ipn);
+  }
+  // incidence lists of the nodes of node-from-IP items, in element order
+  std::vector<int2> inc = job->hist_inc;
+  bool any_ipn = false;
+  for (int64_t n : ip_nodes) any_ipn = any_ipn || n >= 0;
+  if (any_ipn) {
+    const int nn = job->nn;
+    std::vector<int32_t> conn((size_t)job->NE * nn);
+    HIPCHK(ctx, hipMemcpy(conn.data(), job->host.conn, conn.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::map<int64_t, std::vector<int2>> lists;
+    for (int64_t n : ip_nodes)
+      if (n >= 0) lists[n];
+    for (int64_t e = 0; e < job->NE; e++)
+      for (int a = 0; a < nn; a++) {
+        auto itl = lists.find(conn[e * nn + a]);
+        if (itl != lists.end()) itl->second.push_back(make_int2((int)e, a));
+      }
+    for (size_t k = 0; k < add.size(); k++) {
+      if (ip_nodes[k] < 0) continue;
+      const auto &l = lists[ip_nodes[k]];
+      if (inc.size() + l.size() >= ((size_t)1 << 31)) return fail(ctx, DNLG_E_CAPACITY, "history lists too long");
+      add[k].r0 = (int32_t)inc.size();
+      inc.insert(inc.end(), l.begin(), l.end());
+      add[k].r1 = (int32_t)inc.size();
+    }
+  }
+  if (job->hist_items.size() + add.size() >= ((size_t)1 << 31)) return fail(ctx, DNLG_E_CAPACITY, "too many items");
+  // ring: room for the rows of the cycles between two host drains (a few
+  // captured sequences), 256 rows unless they are large
+  dnlg_history h;
+  h.n_items = n_items;
+  const int64_t row_bytes = 8 * (1 + (int64_t)n_items);
+  const int64_t base = 4 * (int64_t)std::max({ctx->graph_cycles, job->host.dt_freq, 1}) + 4;
+  h.cap = (int32_t)std::max(base, std::min<int64_t>(256, std::max<int64_t>(1, ((int64_t)64 << 20) / row_bytes)));
+  int st;
+  if ((st = dalloc(job, &h.ring, (int64_t)h.cap * (1 + n_items), (const double *)nullptr))) return st;
+  if (need_ke && !job->host.hke)
+    if ((st = dalloc(job, &job->host.hke, (job->N + kNodeBlock - 1) / kNodeBlock, (const double *)nullptr))) return st;
+  HistFile hf{};
+  // HistoryFile::setSaveTime (HistoryFile.C:131-148): next starts at start
+  hf.start = start_time;
+  hf.next = start_time;
+  hf.save = save_time;
+  hf.rows = 0;
+  hf.ring = h.ring;
+  hf.cap = h.cap;
+  hf.n_items = n_items;
+  hf.ke0 = (int32_t)job->hist_ke_cols.size();
+  std::vector<int32_t> ke_cols = job->hist_ke_cols;
+  for (const HistItem &it : add)
+    if (it.op == kHKe) ke_cols.push_back(it.col);
+  hf.n_ke = (int32_t)ke_cols.size() - hf.ke0;
+  if (path) {
+    // HistoryFile::headerWrite (HistoryFile.C:340-358) with the program name
+    // and version of Sources/dnlSettings/configuration.dnl
+    h.path = path;
+    h.fp = std::fopen(path, "w");
+    if (!h.fp) return fail(ctx, DNLG_E_INVALID, std::string("cannot open history file ") + path);
+    std::fprintf(h.fp, "#%s v. %s history file\n", "DynELA FEM Code", "4.0");
+    std::fprintf(h.fp, "#plotted :");
+    for (int32_t k = 0; k < n_items; k++) std::fprintf(h.fp, "%s ", names && names[k] ? names[k] : "");
+    std::fprintf(h.fp, "\n");
+    std::fflush(h.fp);
+  }
+  // new device tables: files, items, incidence pairs, kinetic-energy columns
+  std::vector<HistFile> files = job->hist_files;
+  files.push_back(hf);
+  std::vector<HistItem> all = job->hist_items;
+  all.insert(all.end(), add.begin(), add.end());
+  std::vector<void *> tables;
+  auto upload = [&](const void *src, size_t bytes, void **dst) -> int {
+    *dst = nullptr;
+    if (bytes == 0) return DNLG_OK;
+    HIPCHK(ctx, hipMalloc(dst, bytes));
+    tables.push_back(*dst);
+    HIPCHK(ctx, hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    return DNLG_OK;
+  };
+  void *d_files, *d_items, *d_inc, *d_ke;
+  if ((st = upload(files.data(), files.size() * sizeof(HistFile), &d_files)) ||
+      (st = upload(all.data(), all.size() * sizeof(HistItem), &d_items)) ||
+      (st = upload(inc.data(), inc.size() * sizeof(int2), &d_inc)) ||
+      (st = upload(ke_cols.data(), ke_cols.size() * sizeof(int32_t), &d_ke))) {
+    for (void *p : tables) hipFree(p);
+    if (h.fp) std::fclose(h.fp);
+    return st;
+  }
+  JobDev &H = job->host;
+  H.hf = (HistFile *)d_files;
+  H.hi = (const HistItem *)d_items;
+  H.hinc = (const int2 *)d_inc;
+  H.hke_col = (const int32_t *)d_ke;
+  H.n_hf = (int32_t)files.size();
+  H.n_hi = (int32_t)all.size();
+  H.density_always = H.density_always || need_rho;
+  // the history tail of the device record (no kernel has run on the job)
+  const size_t off = offsetof(JobDev, hf);
+  HIPCHK(ctx, hipMemcpy((char *)job->dev + off, (const char *)&H + off, sizeof(JobDev) - off, hipMemcpyHostToDevice));
+  for (void *p : job->hist_tables) hipFree(p);
+  job->hist_tables = tables;
+  job->hist_files = files;
+  job->hist_items = all;
+  job->hist_inc = inc;
+  job->hist_ke_cols = ke_cols;
+  job->hist.push_back(std::move(h));
+  if (file_id) *file_id = fid;
+  return DNLG_OK;
+}
+
+int dnlg_job_history_rows(dnlg_job *job, int32_t file_id, double *rows, int64_t max_rows, int64_t *n_rows) {
+  if (!job || file_id < 0 || file_id >= (int32_t)job->hist.size()) return DNLG_E_INVALID;
+  const dnlg_history &h = job->hist[file_id];
+  const int64_t width = 1 + h.n_items;
+  const int64_t n = (int64_t)h.rows.size() / width;
+  if (n_rows) *n_rows = n;
+  if (rows && max_rows > 0) std::memcpy(rows, h.rows.data(), std::min(n, max_rows) * width * sizeof(double));
   return DNLG_OK;
 }
 

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
 #include <map>
 #include <memory>
 #include <string>
@@ -11,6 +12,17 @@
 #include "engine.h"
 
 struct dnlg_ctx;
+
+// Host side of one history file: where its rows go and how far the device
+// ring has been drained.
+struct dnlg_history {
+  std::string path;   // empty: rows kept in memory only
+  FILE *fp = nullptr;
+  int32_t n_items = 0, cap = 0;
+  double *ring = nullptr;  // device ring (HistFile::ring)
+  int64_t drained = 0;
+  std::vector<double> rows;  // every row taken, 1 + n_items doubles each
+};
 
 struct dnlg_job {
   dnlg_ctx *ctx = nullptr;
@@ -27,6 +39,18 @@ struct dnlg_job {
   double up_to = 0.0;
   int64_t inc_now = 0;
   int status = 0;
+  // history files: host records, the device tables' host copies, and the
+  // device tables themselves (replaced whenever a file is added)
+  std::vector<dnlg_history> hist;
+  std::vector<dnlg::HistFile> hist_files;
+  std::vector<dnlg::HistItem> hist_items;
+  std::vector<int2> hist_inc;
+  std::vector<int32_t> hist_ke_cols;
+  std::vector<void *> hist_tables;
+  ~dnlg_job() {
+    for (auto &h : hist)
+      if (h.fp) std::fclose(h.fp);
+  }
 };
 
 namespace dnlg {
@@ -58,6 +82,9 @@ namespace dnlg {
 int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vector<double> &up_to,
               int64_t max_cycles = -1);
 void batch_cache_clear(dnlg_ctx *ctx);
+// Copies the rows that the job's history files took since the last drain
+// from the device rings to the host records and files (api.hip).
+int history_drain(dnlg_job *job);
 void batch_cache_forget(dnlg_ctx *ctx, const dnlg_job *job);
 
 }  // namespace dnlg

@@ -31,6 +31,10 @@ struct Batch {
   TaskList node[2][2];  // [dim==3?][mixed?]
   TaskList step;        // cycle control + field0 BCs, over the BC node lists
   TaskList close;       // end of a launch sequence, one block per job
+  // history files: items (256 per block), kinetic-energy node blocks, one
+  // block per job; hist_cap: the smallest ring (rows) of the batch's files
+  TaskList hist_items, hist_ke, hist_jobs;
+  int hist_cap = 0;
   bool has_courant = false, has_pi = false;  // time-step methods present
   int *d_pend = nullptr;                     // power-iteration pending flags
   std::vector<int> h_pend;
@@ -155,6 +159,26 @@ int build_batch(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, Batch &B) {
         }
       if ((st = upload_tasks(B, make_tasks(idx, cnt, kNodeBlock), B.node[d3][mx]))) return st;
     }
+  {
+    std::vector<int> hj, kj;
+    std::vector<int64_t> hn, kn;
+    std::vector<Task> per_job;
+    for (int i = 0; i < n; i++) {
+      if (jobs[i]->hist.empty()) continue;
+      per_job.push_back(Task{i, 0, 0, 0});
+      hj.push_back(i);
+      hn.push_back((int64_t)jobs[i]->hist_items.size());
+      if (!jobs[i]->hist_ke_cols.empty()) {
+        kj.push_back(i);
+        kn.push_back(jobs[i]->N);
+      }
+      for (const auto &h : jobs[i]->hist) B.hist_cap = B.hist_cap ? std::min(B.hist_cap, h.cap) : h.cap;
+    }
+    if ((st = upload_tasks(B, make_tasks(hj, hn, 256), B.hist_items)) ||
+        (st = upload_tasks(B, make_tasks(kj, kn, kNodeBlock), B.hist_ke)) ||
+        (st = upload_tasks(B, per_job, B.hist_jobs)))
+      return st;
+  }
   (void)ctx;
   return 0;
 }
@@ -226,11 +250,18 @@ int run_pi_host(dnlg_ctx *ctx, Batch &B, int cs) {
   return DNLG_E_HIP;  // unreachable: k_pi_step stops a job after 1000 iterations
 }
 
+void launch_history(dnlg_ctx *ctx, Batch &B, int cs, int preloop) {
+  launch_history(B.d_jobs, B.hist_items.d, B.hist_items.n, B.hist_ke.d, B.hist_ke.n, B.hist_jobs.d, B.hist_jobs.n, cs,
+                 preloop, ctx->stream);
+}
+
 void launch_post(dnlg_ctx *ctx, Batch &B, int cs, bool courant, bool fold = false) {
   hipStream_t s = ctx->stream;
   for (int d3 = 0; d3 < 2; d3++)
     for (int mx = 0; mx < 2; mx++)
       launch_correct(d3 ? 3 : 2, mx == 1, B.d_jobs, B.node[d3][mx].d, B.node[d3][mx].n, cs, fold ? 1 : 0, s);
+  // Explicit::endStep's history rows, before the time-step computation
+  launch_history(ctx, B, cs, 0);
   if (courant && B.has_courant)
     for (int hx = 0; hx < 2; hx++) launch_courant(hx ? 3 : 2, B.d_jobs, B.cour[hx].d, B.cour[hx].n, cs, 0, s);
   if (courant && B.has_pi) {
@@ -292,6 +323,8 @@ int run_preloop(dnlg_ctx *ctx, Batch &B) {
     launch_elem_fp64(type, B.d_jobs, B.elem[hx][0].d, B.elem[hx][0].n, 0, 0, s);
     launch_elem_mixed(type, B.d_jobs, B.elem[hx][1].d, B.elem[hx][1].n, 0, 0, s);
   }
+  // the pre-loop's history rows (Explicit.C:182)
+  launch_history(ctx, B, 0, 1);
   if (hipGetLastError() != hipSuccess) return DNLG_E_HIP;
   return 0;
 }
@@ -448,6 +481,8 @@ int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vec
       return DNLG_E_HIP;
     }
     if ((st = collect())) return st;
+    for (auto *j : jobs)
+      if ((st = history_drain(j))) return st;
     bool stalled = false;
     for (int i = 0; i < n; i++) stalled = stalled || B->h_snap[kSnap * i + 5] != 0.0;
     if (stalled) {
@@ -488,7 +523,12 @@ int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vec
       if (launched >= max_cycles) break;
       need = std::min(need, (double)(max_cycles - launched));
     }
-    const int chunk = (int)std::min(need, 4096.0);
+    int chunk = (int)std::min(need, 4096.0);
+    if (B->hist_cap > 0) {
+      // at most one history row per cycle and file between two drains
+      const int per = use_graph ? graph_len(ctx->graph_cycles, phase, freq) : 1;
+      chunk = std::min(chunk, std::max(1, B->hist_cap / per) * per);
+    }
     launched += chunk;
     hipGraphExec_t g_whole = nullptr, g_pre = nullptr, g_post = nullptr, g_rest = nullptr;
     if (use_graph) {

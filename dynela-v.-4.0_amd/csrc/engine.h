@@ -79,6 +79,46 @@ struct PiState {
   int32_t pad;
 };
 
+// History files (HistoryFile::save, HistoryFile.C:283-315).  A file takes a
+// row at time t unless start < t < next; taking one advances next by one
+// save interval.  Rows go to a device ring of `cap` rows (row r at r % cap,
+// 1 + n_items doubles: time, then the items in file order) that the host
+// drains between launch sequences; next and rows change only in
+// k_hist_commit.
+struct HistFile {
+  double start, next, save;
+  int64_t rows;       // rows taken so far
+  double *ring;
+  int32_t cap, n_items;
+  int32_t ke0, n_ke;  // kinetic-energy columns: hke_col[ke0 .. ke0 + n_ke)
+};
+
+// Item evaluations (HistoryFileNodeItem / ElementItem / GlobalItem::getValue,
+// HistoryFile.C:46-99).
+enum HistOp : int32_t {
+  kHZero = 0,    // a field the reference's getValue does not know: 0
+  kHNodal = 1,   // Node::fieldScalar nodal part (Node.C:311-330), src HistNodal
+  kHNodeIP = 2,  // IP values extrapolated to the node (Node.C:244-300, 350-364)
+  kHElemIP = 3,  // Element::getIntPointValue (Element.C:1195-1238)
+  kHDt = 4,      // Solver::timeStep (realTimeStep, timeStep)
+  kHKe = 5,      // Model::getTotalKineticEnergy, written by k_hist_commit
+};
+enum HistNodal : int32_t { kHCoords = 0, kHDisp = 1, kHDispInc = 2, kHSpeed = 3, kHSpeedInc = 4, kHMass = 5 };
+enum HistIP : int32_t {
+  kHEp = 0, kHDep = 1, kHGam = 2, kHGamc = 3, kHSy = 4, kHT = 5, kHRho = 6, kHPres = 7,
+  kHEint = 8, kHVm = 9, kHSig = 10, kHEps = 11, kHEpsP = 12,
+};
+constexpr int kHNorm = 6;  // comp of a tensor item: its norm (vectors: 3)
+
+struct HistItem {
+  int32_t file, col;  // owning file and column in its row (1 + position)
+  int32_t op, src;    // HistOp, HistNodal / HistIP
+  int32_t comp;       // vector 0..2 or 3 (norm); tensor [xx,xy,xz,yy,yz,zz] or 6
+  int32_t ip;         // element items: IP index, -1 = mean over the IPs
+  int64_t index;      // node or element
+  int32_t r0, r1;     // node-from-IP items: incidence pairs hinc[r0 .. r1)
+};
+
 struct JobDev {
   Control c[2];
   Sync sync;
@@ -128,6 +168,14 @@ struct JobDev {
   double *rho, *detJ0;
   float *detJ0f;
   double *eps, *epsP;  // optional tensors (track_tensors)
+  // history files (none: n_hf == 0)
+  HistFile *hf;
+  const HistItem *hi;
+  const int2 *hinc;       // (element, local node) pairs of node-from-IP items
+  const int32_t *hke_col;  // kinetic-energy columns, per file (HistFile::ke0)
+  double *hke;            // kinetic-energy partial sums, one per node block
+  int32_t n_hf, n_hi;
+  int32_t density_always;  // IP densities every cycle (a history item reads them)
 };
 
 // Address spaces for the kernels.  A job record is read-only while a kernel

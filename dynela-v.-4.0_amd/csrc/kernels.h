@@ -46,5 +46,14 @@ void launch_pi_node(int dim, JobDev *const *jobs, const Task *tasks, int nblk, h
 void launch_pi_step(JobDev *const *jobs, const Task *tasks, int nblk, int cs, hipStream_t s);
 void launch_pi_finish(JobDev *const *jobs, int n, int preloop, hipStream_t s);
 void launch_pi_pending(JobDev *const *jobs, int n, int *out, hipStream_t s);
+// history files: the IP-to-node extrapolation tables ([node][ip]) and their
+// upload; the launches of one row-taking moment (preloop != 0: the pre-loop
+// pass, slot 0; otherwise the end of the cycle in slot cs): items (256 per
+// block), kinetic-energy partials (node blocks of the jobs with a
+// kineticEnergy item), commit (one block per job with history files)
+void extrapolation_tables(double hexE[64], double quaE[16]);
+void upload_extrapolation();
+void launch_history(JobDev *const *jobs, const Task *items, int n_items, const Task *ke, int n_ke, const Task *per_job,
+                    int n_jobs, int cs, int preloop, hipStream_t s);
 
 }  // namespace dnlg

@@ -26,6 +26,7 @@
 #include "kernels.h"
 #include "material.h"
 #include "polar_fp64.h"
+#include "control.h"
 
 namespace dnlg {
 
@@ -57,13 +58,6 @@ __device__ __forceinline__ int sym_idx(int i, int j) { return (i == j) ? (i == 0
 // Sticky job status of the cycle with increment parity `par`.
 __device__ __forceinline__ void set_status(JobDev *J, int st, int par) { atomicCAS(&J->sync.st[par], 0, st); }
 
-// The job's status between launches of one cycle's predecessor and its own
-// (no launch that could raise an error runs beside the reader).
-__device__ __forceinline__ int job_status(CJobDev *C) {
-  const int a = C->sync.st[0];
-  return a != 0 ? a : C->sync.st[1];
-}
-
 // ---------------------------------------------------------------------------
 // Cycle control.  Between two cycles a job's control record goes through
 //   close  Explicit::endStep (Explicit.C:539-552): t += dt, field1 becomes
@@ -77,25 +71,6 @@ __device__ __forceinline__ int job_status(CJobDev *C) {
 // (cycle_control) and the corrector has already written the new cycle's
 // field0 BC values.
 // ---------------------------------------------------------------------------
-// Read through the constant space (scalar loads); only for a record that no
-// block of the reading launch writes.
-template <typename R = const DNLG_CONST Control>
-__device__ __forceinline__ Control load_control(CJobDev *C, int slot) {
-  R &s = ((R *)C->c)[slot];
-  Control c;
-  c.t = s.t;
-  c.dt = s.dt;
-  c.up_to = s.up_to;
-  c.inc = s.inc;
-  c.f0 = s.f0;
-  c.running = s.running;
-  c.courant_due = s.courant_due;
-  c.density_due = s.density_due;
-  c.preloop = s.preloop;
-  c.pending = s.pending;
-  return c;
-}
-
 __device__ __forceinline__ void store_control(JobDev *J, int slot, const Control &c) {
   DNLG_GLOBAL Control &d = *gbl(&J->c[slot]);
   d.t = c.t;
@@ -136,8 +111,9 @@ __device__ __forceinline__ bool close_open(CJobDev *C, Control &c, int status, d
       if (c.t + c.dt > c.up_to) c.dt = c.up_to - c.t;
       c.inc++;
       c.courant_due = (c.inc % C->dt_freq == 0) ? 1 : 0;
-      // densities are consumed by the Courant step and read back at the end
-      c.density_due = (c.courant_due || !(c.t + c.dt < c.up_to)) ? 1 : 0;
+      // densities are consumed by the Courant step, read back at the end
+      // and, when a history item reads them, after every cycle
+      c.density_due = (C->density_always || c.courant_due || !(c.t + c.dt < c.up_to)) ? 1 : 0;
       c.pending = 1;
       bc_work = resumed || (C->bc_flags & 2) || ((C->bc_flags & 1) && c.dt != dt_last);
     }

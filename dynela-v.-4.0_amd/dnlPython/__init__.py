@@ -24,7 +24,7 @@ from . import _lib
 from ._lib import (DnlgError, NegativeJacobianError, NewtonRaphsonError, PowerIterationError,  # noqa: F401
                    NonFiniteError)
 
-__all__ = ["DynELA", "NodeSet", "ElementSet", "Material", "JohnsonCookLaw", "ElastoplasticLaw",
+__all__ = ["DynELA", "NodeSet", "ElementSet", "Material", "JohnsonCookLaw", "ElastoplasticLaw", "HistoryFile",
            "VoceLinearLaw", "BoundaryRestrain", "BoundarySpeed", "RampFunction", "Explicit", "Field",
            "Element", "Vec3D", "Context", "DnlgError", "NegativeJacobianError", "NewtonRaphsonError",
            "PowerIterationError", "NonFiniteError", "batch_solve"]
@@ -44,31 +44,45 @@ class Element:
 NODES_PER_ELEMENT = {Element.ElQua4NAx: 4, Element.ElHex8N3D: 8}
 
 
+def _field_names():
+    """The identifiers of Field::FieldLabel in enum order (Field.h:46-79):
+    the values are the enum's, as the reference's Python module exposes them."""
+    names = []
+    for v in ("coords", "disp", "dispInc", "speed", "speedInc"):
+        names += [v, v + "X", v + "Y", v + "Z"]
+    names += ["density", "gamma", "gammaCumulate", "internalEnergy", "inelasticEnergy", "kineticEnergy", "mass",
+              "plasticStrain", "plasticStrainRate", "pressure", "realTimeStep", "T", "timeStep", "vonMises",
+              "yieldStress"]
+    for t in ("PlasticStrain", "PlasticStrainInc", "Strain", "StrainInc", "Stress"):
+        names += [t] + [t + c for c in ("XX", "XY", "XZ", "YX", "YY", "YZ", "ZX", "ZY", "ZZ")]
+    return names
+
+
+_FIELD_NAMES = _field_names()
+
+
 class Field:
-    """Field names of Field.h:46-79 that the engine can return."""
-    plasticStrain = "plasticStrain"
-    plasticStrainRate = "plasticStrainRate"
-    gamma = "gamma"
-    gammaCumulate = "gammaCumulate"
-    yieldStress = "yieldStress"
-    T = "T"
-    density = "density"
-    pressure = "pressure"
-    internalEnergy = "internalEnergy"
-    inelasticEnergy = "inelasticEnergy"
-    vonMises = "vonMises"
-    Stress = "Stress"
-    Strain = "Strain"
-    PlasticStrain = "PlasticStrain"
-    coords = "coords"
-    coordsX = "coordsX"
-    coordsY = "coordsY"
-    coordsZ = "coordsZ"
-    disp = "disp"
-    speed = "speed"
-    kineticEnergy = "kineticEnergy"
-    timeStep = "timeStep"
-    mass = "mass"
+    """Field labels of Field.h:46-79 with the reference's enum values."""
+
+    @staticmethod
+    def getVtklabel(field):
+        """Field::getVtklabel (Field.C): the identifier, except yieldStress,
+        whose label is "yield"."""
+        name = _FIELD_NAMES[int(field)]
+        return "yield" if name == "yieldStress" else name
+
+    @staticmethod
+    def getField(name):
+        """Field::getField: the value of a label, -1 if unknown."""
+        for f in range(len(_FIELD_NAMES)):
+            if Field.getVtklabel(f) == name:
+                return f
+        return -1
+
+
+for _i, _n in enumerate(_FIELD_NAMES):
+    setattr(Field, _n, _i)
+Field.ENDFIELDS = len(_FIELD_NAMES)
 
 
 _IP_FIELDS = {
@@ -373,6 +387,109 @@ class _Parallel:
         self.cores = int(n)
 
 
+class HistoryFile:
+    """HistoryFile (HistoryFile.h, HistoryFile.C): time series of node,
+    element and global values, one row per save time, written by the engine
+    while it solves (rows taken on the GPU after each cycle, written to the
+    file between launch sequences).  Items are bound when added, like the
+    reference's (the set's members at that moment)."""
+
+    def __init__(self, name=""):
+        self.name = name
+        self._file_name = ""
+        self._start, self._stop, self._save = 0.0, 0.0, 0.0
+        self._items = []  # (kind, field, node/element number, intPt, header name)
+        self._model = None
+        self._id = None
+
+    def setFileName(self, fileName):
+        self._file_name = str(fileName)
+
+    def getFileName(self):
+        return self._file_name
+
+    def setSaveTime(self, *args):
+        """setSaveTime(saveTime) or setSaveTime(startTime, stopTime, saveTime)
+        (HistoryFile.C:131-148)."""
+        if len(args) == 1:
+            self._start, self._stop, self._save = 0.0, -1.0, float(args[0])
+        elif len(args) == 3:
+            self._start, self._stop, self._save = float(args[0]), float(args[1]), float(args[2])
+        else:
+            raise TypeError("setSaveTime(saveTime) or setSaveTime(startTime, stopTime, saveTime)")
+
+    def getSaveTime(self):
+        return self._save
+
+    def getStartTime(self):
+        return self._start
+
+    def getStopTime(self):
+        return self._stop
+
+    def add(self, *args):
+        """add(nodeSet, field), add(elementSet, intPt, field) or add(field)
+        (HistoryFile.C:186-280): one item per set member, named
+        label_n<number> / label[_intPt]_e<number> when the set has more than
+        one member."""
+        if self._model is not None and self._model._job is not None:
+            raise DnlgError(_lib.E_STATE, "items must be added before the history file is attached to a solved model")
+        if len(args) == 1:
+            f = int(args[0])
+            self._items.append((_lib.H_GLOBAL, f, 0, 0, Field.getVtklabel(f)))
+        elif len(args) == 2:
+            nodes, f = args[0].numbers(), int(args[1])
+            base = Field.getVtklabel(f)
+            for n in nodes:
+                self._items.append((_lib.H_NODE, f, int(n), 0, f"{base}_n{n}" if len(nodes) > 1 else base))
+        elif len(args) == 3:
+            elems, ip, f = list(args[0]._numbers), int(args[1]), int(args[2])
+            base = Field.getVtklabel(f) + (f"_{ip}" if ip > 0 else "")
+            for e in elems:
+                self._items.append((_lib.H_ELEMENT, f, int(e), ip, f"{base}_e{e}" if len(elems) > 1 else base))
+        else:
+            raise TypeError("add(nodeSet, field), add(elementSet, intPt, field) or add(field)")
+
+    def _attach(self, model):
+        """Registers the file with the model's engine job (before its first
+        solve)."""
+        if not self._file_name:
+            raise DnlgError(_lib.E_INVALID, f"No file name specified for object : {self.name}")
+        n = len(self._items)
+        items = (_lib.HistoryItem * max(1, n))()
+        names = (ctypes.c_char_p * max(1, n))()
+        for k, (kind, f, num, ip, label) in enumerate(self._items):
+            if kind == _lib.H_NODE:
+                idx = int(model._index_of(model._node_sorted, np.array([num]), "history node")[0])
+            elif kind == _lib.H_ELEMENT:
+                idx = int(model._index_of(model._elem_sorted, np.array([num]), "history element")[0])
+            else:
+                idx = 0
+            items[k] = _lib.HistoryItem(kind, f, idx, ip, 0)
+            names[k] = label.encode()
+        fid = ctypes.c_int32()
+        ctx = model.context
+        ctx.check(ctx.lib.dnlg_job_add_history(model._job, self._file_name.encode(), self._start, self._save, items,
+                                               n, names, ctypes.byref(fid)))
+        self._id = fid.value
+
+    def getRows(self):
+        """The rows written so far, [rows, 1 + items] (time first)."""
+        if self._model is None or self._id is None:
+            return np.zeros((0, 1 + len(self._items)))
+        m = self._model
+        lib = m.context.lib
+        n = ctypes.c_int64()
+        m.context.check(lib.dnlg_job_history_rows(m._job, self._id, None, 0, ctypes.byref(n)))
+        out = np.zeros((n.value, 1 + len(self._items)))
+        if n.value:
+            m.context.check(lib.dnlg_job_history_rows(m._job, self._id, _lib.as_dp(out), n.value, ctypes.byref(n)))
+        return out
+
+    def getItemNames(self):
+        return [it[4] for it in self._items]
+
+
 class DynELA:
     """Model facade (DynELA.h:40-111)."""
 
@@ -394,6 +511,7 @@ class DynELA:
         self.parallel = _Parallel()
         self.saveTimes = None
         self._stress_method = "NewtonRaphson"
+        self._histories = []
 
     # ---- nodes and elements -------------------------------------------------
     def createNode(self, number, x, y=None, z=None):
@@ -468,9 +586,17 @@ class DynELA:
         elif isinstance(obj, Explicit):
             self._solver = obj
             obj.model = self
+        elif isinstance(obj, HistoryFile):
+            # Model::add(HistoryFile *) (Model.C:927-939)
+            if obj._model is not None and obj._model is not self:
+                raise DnlgError(_lib.E_INVALID, "the history file belongs to another model")
+            obj._model = self
+            self._histories.append(obj)
+            if self._job is not None:
+                obj._attach(self)
+            return
         else:
-            # HistoryFile and other outputs are not part of the engine
-            pass
+            raise DnlgError(_lib.E_INVALID, f"cannot add a {type(obj).__name__} to the model")
         self._invalidate()
 
     @staticmethod
@@ -561,6 +687,7 @@ class DynELA:
         self._arrays = _lib.ModelArrays(self._elem_type, xyz.T.copy(), conn, self._material.descriptor(), bnds,
                                         lists[0], lists[1], self._solver.params())
         self._node_sorted = nums
+        self._elem_sorted = enums
         return self._arrays
 
     # ---- engine -------------------------------------------------------------
@@ -580,6 +707,8 @@ class DynELA:
         ctx.check(ctx.lib.dnlg_job_create(ctx.handle, ctypes.byref(a.mesh), ctypes.byref(a.material),
                                           ctypes.byref(a.bcs), ctypes.byref(a.params), ctypes.byref(job)))
         self._job = job
+        for h in self._histories:
+            h._attach(self)
         return True
 
     def solve(self, upTo=None):
@@ -640,7 +769,10 @@ class DynELA:
     def getIntPointField(self, field):
         """[n_elems, n_ip] (scalars) or [6, n_elems, n_ip] (tensors)."""
         self._require_job()
-        f = _IP_FIELDS[field] if isinstance(field, str) else int(field)
+        key = getattr(Field, field, None) if isinstance(field, str) else field
+        if key not in _IP_FIELDS:
+            raise DnlgError(_lib.E_INVALID, f"no integration-point field {field!r}")
+        f = _IP_FIELDS[key]
         ne = self.getElementsNumber()
         nip = NODES_PER_ELEMENT[self._elem_type]
         ncomp = 6 if f in _lib.TENSOR_FIELDS else 1

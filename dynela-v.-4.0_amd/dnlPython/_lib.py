@@ -33,6 +33,7 @@ PREC_FP64, PREC_MIXED = 0, 1
  F_DENSITY, F_PRESSURE, F_INTERNAL_ENERGY, F_INELASTIC_ENERGY, F_VON_MISES, F_STRESS, F_STRAIN,
  F_PLASTIC_STRAIN_TENSOR) = range(14)
 TENSOR_FIELDS = (F_STRESS, F_STRAIN, F_PLASTIC_STRAIN_TENSOR)
+H_NODE, H_ELEMENT, H_GLOBAL = 0, 1, 2
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _i32p = ctypes.POINTER(ctypes.c_int32)
@@ -69,6 +70,11 @@ class SolverParams(ctypes.Structure):
                 ("track_tensors", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
+class HistoryItem(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("field", ctypes.c_int32), ("index", ctypes.c_int64),
+                ("int_pt", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
 # every entry point of include/dnlg.h: name -> (restype, argtypes)
 _VP = ctypes.c_void_p
 SIGNATURES = {
@@ -103,6 +109,10 @@ SIGNATURES = {
     "dnlg_server_set_cycle_budget": (ctypes.c_int, [_VP, ctypes.c_int64]),
     "dnlg_server_poll": (ctypes.c_int, [_VP, ctypes.c_int64, _i32p, _i32p]),
     "dnlg_server_take": (ctypes.c_int, [_VP, ctypes.c_int64, ctypes.POINTER(_VP)]),
+    "dnlg_job_add_history": (ctypes.c_int, [_VP, ctypes.c_char_p, ctypes.c_double, ctypes.c_double,
+                                            ctypes.POINTER(HistoryItem), ctypes.c_int32,
+                                            ctypes.POINTER(ctypes.c_char_p), _i32p]),
+    "dnlg_job_history_rows": (ctypes.c_int, [_VP, ctypes.c_int32, _dp, ctypes.c_int64, _i64p]),
 }
 
 _lib = None

@@ -256,6 +256,78 @@ int64_t dnlg_footprint_bytes(int elem_type, int64_t n_nodes, int64_t n_elems,
                              int precision, int track_tensors);
 int dnlg_job_status(const dnlg_job *job);
 
+/* ---- history files --------------------------------------------------------
+ * HistoryFile (Sources/dnlFEM/HistoryFile.C) attached to a model with
+ * Model::add(HistoryFile *) (Model.C:927-939).  A file takes a row -- the
+ * time, then every item in the order added, each printed "%10.7E " -- after
+ * the pre-loop pass and at the end of every cycle whose new time t has
+ * t <= start or t >= next; a row moves next one save interval on
+ * (HistoryFile::save, HistoryFile.C:283-315).  setSaveTime(save) is
+ * start = 0, setSaveTime(start, stop, save) is start (the reference does not
+ * use the stop time either).  Item fields are the values of the reference's
+ * Field::FieldLabel enum (Sources/dnlKernel/Field.h:46-79); a vector block is
+ * norm, X, Y, Z and a tensor block norm, XX, XY, XZ, YX, YY, YZ, ZX, ZY, ZZ. */
+#define DNLG_FIELD_COORDS 0
+#define DNLG_FIELD_DISP 4
+#define DNLG_FIELD_DISP_INC 8
+#define DNLG_FIELD_SPEED 12
+#define DNLG_FIELD_SPEED_INC 16
+#define DNLG_FIELD_DENSITY 20
+#define DNLG_FIELD_GAMMA 21
+#define DNLG_FIELD_GAMMA_CUMULATE 22
+#define DNLG_FIELD_INTERNAL_ENERGY 23
+#define DNLG_FIELD_INELASTIC_ENERGY 24
+#define DNLG_FIELD_KINETIC_ENERGY 25
+#define DNLG_FIELD_MASS 26
+#define DNLG_FIELD_PLASTIC_STRAIN 27
+#define DNLG_FIELD_PLASTIC_STRAIN_RATE 28
+#define DNLG_FIELD_PRESSURE 29
+#define DNLG_FIELD_REAL_TIME_STEP 30
+#define DNLG_FIELD_T 31
+#define DNLG_FIELD_TIME_STEP 32
+#define DNLG_FIELD_VON_MISES 33
+#define DNLG_FIELD_YIELD_STRESS 34
+#define DNLG_FIELD_PLASTIC_STRAIN_TENSOR 35 /* PlasticStrain */
+#define DNLG_FIELD_PLASTIC_STRAIN_INC 45
+#define DNLG_FIELD_STRAIN 55
+#define DNLG_FIELD_STRAIN_INC 65
+#define DNLG_FIELD_STRESS 75
+#define DNLG_FIELD_COUNT 85
+/* item kinds */
+#define DNLG_H_NODE 0    /* HistoryFile::add(NodeSet *, field), one item per node:
+                            Node::fieldScalar (Node.C:311-370), IP fields
+                            extrapolated to the node and averaged over its
+                            elements */
+#define DNLG_H_ELEMENT 1 /* HistoryFile::add(ElementSet *, intPt, field), one
+                            item per element: Element::getIntPointValue
+                            (Element.C:1195-1238) */
+#define DNLG_H_GLOBAL 2  /* HistoryFile::add(field): timeStep, realTimeStep,
+                            kineticEnergy (HistoryFile.C:80-99) */
+typedef struct dnlg_history_item {
+  int32_t kind;    /* DNLG_H_* */
+  int32_t field;   /* DNLG_FIELD_* (+ component offset) */
+  int64_t index;   /* node or element index (0-based, mesh order) */
+  int32_t int_pt;  /* element items: 1..n_ip, 0 = mean over the IPs */
+  int32_t reserved;
+} dnlg_history_item;
+/* Replaces HistoryFile::setFileName / setSaveTime / add + Model::add
+ * (HistoryFile.C:131-280, Model.C:927-939) for a job that has not been
+ * solved yet.  path: the file, created now with the reference's header
+ * (HistoryFile::headerWrite, HistoryFile.C:340-358); NULL keeps the rows in
+ * memory only.  names: the n_items header names (may be NULL).  A field
+ * that the reference's getValue does not know for the item's kind gives 0,
+ * as there (e.g. inelasticEnergy); StrainInc and PlasticStrainInc are not
+ * kept by the engine (DNLG_E_INVALID); Strain and PlasticStrain need
+ * track_tensors (DNLG_E_STATE).  *file_id: the file's index. */
+int dnlg_job_add_history(dnlg_job *job, const char *path, double start_time,
+                         double save_time, const dnlg_history_item *items,
+                         int32_t n_items, const char *const *names,
+                         int32_t *file_id);
+/* The rows the file has taken so far (*n_rows); rows != NULL: the first
+ * min(*n_rows, max_rows) rows of 1 + n_items doubles are copied out. */
+int dnlg_job_history_rows(dnlg_job *job, int32_t file_id, double *rows,
+                          int64_t max_rows, int64_t *n_rows);
+
 /* ---- measurement hooks ------------------------------------------------------ */
 /* Device time (ms) and launch count of the element kernels, accumulated with
  * HIP events recorded around them on the engine's own stream since the last

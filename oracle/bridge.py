@@ -59,12 +59,31 @@ def load():
         lib.oracle_ramp_value.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double]
         lib.oracle_element_stiffness.argtypes = [ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double, _dp]
         lib.oracle_job_pi_state.argtypes = [_vp, _dp, ctypes.POINTER(ctypes.c_int)]
+        lib.oracle_job_add_history.argtypes = [_vp, ctypes.c_double, ctypes.c_double, ctypes.POINTER(HistoryItem),
+                                               ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
+        lib.oracle_job_history_rows.argtypes = [_vp, ctypes.c_int32, _dp, ctypes.c_int64,
+                                                ctypes.POINTER(ctypes.c_int64)]
+        lib.oracle_extrapolation.argtypes = [ctypes.c_int, _dp]
         _lib = lib
     return _lib
 
 
 def p(a):
     return np.ascontiguousarray(a).ctypes.data_as(_dp)
+
+
+class HistoryItem(ctypes.Structure):
+    """dnlg_history_item of include/dnlg.h."""
+    _fields_ = [("kind", ctypes.c_int32), ("field", ctypes.c_int32), ("index", ctypes.c_int64),
+                ("int_pt", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+def extrapolation(elem_type):
+    """The oracle's IP-to-node extrapolation table, [node][ip]."""
+    nn = 8 if elem_type == 4 else 4
+    E = np.zeros((nn, nn))
+    load().oracle_extrapolation(elem_type, E.ctypes.data_as(_dp))
+    return E
 
 
 class OracleJob:
@@ -86,6 +105,7 @@ class OracleJob:
         lib.oracle_job_sizes(self.h, ctypes.byref(n_nodes), ctypes.byref(n_elems), ctypes.byref(nip),
                              ctypes.byref(dim))
         self.N, self.NE, self.nip, self.dim = n_nodes.value, n_elems.value, nip.value, dim.value
+        self._hist_width = {}
 
     def solve(self, up_to):
         return self.lib.oracle_job_solve(self.h, ctypes.c_double(up_to))
@@ -120,6 +140,41 @@ class OracleJob:
 
     def status(self):
         return self.lib.oracle_job_status(self.h)
+
+    def add_history(self, start, save, items):
+        """A history file with items (kind, field, index, int_pt); returns its id."""
+        n = len(items)
+        arr = (HistoryItem * max(1, n))(*[HistoryItem(k, f, i, ip, 0) for (k, f, i, ip) in items])
+        fid = ctypes.c_int32()
+        st = self.lib.oracle_job_add_history(self.h, start, save, arr, n, ctypes.byref(fid))
+        if st != 0:
+            raise RuntimeError(f"oracle_job_add_history failed: {st}")
+        self._hist_width[fid.value] = 1 + n
+        return fid.value
+
+    def add_history_file(self, model, hist):
+        """The items of a dnlPython HistoryFile attached to `model`, numbers
+        resolved to the model's node / element indices."""
+        model.descriptors()
+        items = []
+        for kind, f, num, ip, _ in hist._items:
+            if kind == 0:
+                idx = int(np.searchsorted(model._node_sorted, num))
+            elif kind == 1:
+                idx = int(np.searchsorted(model._elem_sorted, num))
+            else:
+                idx = 0
+            items.append((kind, f, idx, ip))
+        return self.add_history(hist.getStartTime(), hist.getSaveTime(), items)
+
+    def history_rows(self, fid):
+        """Rows taken so far, [rows, 1 + items] (time first)."""
+        n = ctypes.c_int64()
+        self.lib.oracle_job_history_rows(self.h, fid, None, 0, ctypes.byref(n))
+        out = np.zeros((n.value, self._hist_width[fid]))
+        if n.value:
+            self.lib.oracle_job_history_rows(self.h, fid, p(out), n.value, ctypes.byref(n))
+        return out
 
     def pi_state(self):
         """(last power-iteration maximum, iterations of the last call)"""

@@ -706,6 +706,14 @@ struct OracleJob {
   std::vector<double> pi_ev;
   double pi_fmax = 0.0;
   int pi_iterations = 0; /* iterations of the last call */
+  /* history files (HistoryFile.h): save times, items, rows taken */
+  struct History {
+    double start = 0, next = 0, save = 0;
+    std::vector<dnlg_history_item> items;
+    std::vector<double> rows; /* 1 + items.size() per row */
+  };
+  std::vector<History> hist;
+  std::vector<std::vector<std::pair<int64_t, int>>> node_elems; /* Node::elements, (element, local node) */
   double t = 0, dt = 0;
   int64_t inc = 0;
   bool started = false;
@@ -1351,6 +1359,158 @@ bool nodes_finite(const OracleJob *J) {
   return true;
 }
 
+/* The integrationPointsToNode vectors of ElQua4NAx.C:30-60 and
+ * ElHex8N3D.C:35-90, [node][ip]: the entry depends only on k, the number of
+ * directions in which IP and node lie on the same side of the element
+ * centre (hex: k = 3 -> (5 + 3 sqrt3)/4, 2 -> -(1 + sqrt3)/4,
+ * 1 -> (sqrt3 - 1)/4, 0 -> (5 - 3 sqrt3)/4; quad: 2 -> 1 + sqrt3/2,
+ * 1 -> -1/2, 0 -> 1 - sqrt3/2). */
+void extrapolation(int type, double E[8][8]) {
+  const double r3 = std::sqrt(3.0);
+  if (type == DNLG_EL_HEX8N_3D) {
+    const int ns[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                          {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+    const double v[4] = {(5.0 - 3.0 * r3) / 4, (-1.0 + r3) / 4, (-1.0 - r3) / 4, (5.0 + 3.0 * r3) / 4};
+    for (int a = 0; a < 8; a++)
+      for (int p = 0; p < 8; p++) {
+        int k = 0;
+        for (int c = 0; c < 3; c++) k += ns[a][c] == ns[p][c];
+        E[a][p] = v[k];
+      }
+  } else {
+    const int ns[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+    const double v[3] = {1.0 - r3 / 2, -0.5, 1.0 + r3 / 2};
+    for (int a = 0; a < 4; a++)
+      for (int p = 0; p < 4; p++) {
+        int k = 0;
+        for (int c = 0; c < 2; c++) k += ns[a][c] == ns[p][c];
+        E[a][p] = v[k];
+      }
+  }
+}
+
+/* Element::getIntPointValueExtract (Element.C:1195-1221) at IP g: the
+ * field's value, a tensor's component or norm (SymTensor2::norm), 0 for a
+ * field the function does not know.  Tensor blocks are norm, XX, XY, XZ,
+ * YX, YY, YZ, ZX, ZY, ZZ. */
+double ip_field(const OracleJob *J, int f, int64_t g) {
+  switch (f) {
+    case DNLG_FIELD_PLASTIC_STRAIN: return J->ep[g];
+    case DNLG_FIELD_PLASTIC_STRAIN_RATE: return J->dep[g];
+    case DNLG_FIELD_GAMMA: return J->gam[g];
+    case DNLG_FIELD_GAMMA_CUMULATE: return J->gamc[g];
+    case DNLG_FIELD_YIELD_STRESS: return J->sy[g];
+    case DNLG_FIELD_T: return J->T[g];
+    case DNLG_FIELD_DENSITY: return J->rho[g];
+    case DNLG_FIELD_PRESSURE: return J->pres[g];
+    case DNLG_FIELD_INTERNAL_ENERGY: return J->eint[g];
+    case DNLG_FIELD_VON_MISES: {
+      const double *s = &J->sig[g * 6];
+      return std::sqrt((sq(s[0] - s[3]) + sq(s[0] - s[5]) + sq(s[3] - s[5]) + 6 * (sq(s[1]) + sq(s[2]) + sq(s[4]))) /
+                       2);
+    }
+  }
+  const std::vector<double> *t = nullptr;
+  int base = 0;
+  if (f >= DNLG_FIELD_STRESS && f < DNLG_FIELD_STRESS + 10) {
+    t = &J->sig;
+    base = DNLG_FIELD_STRESS;
+  } else if (f >= DNLG_FIELD_STRAIN && f < DNLG_FIELD_STRAIN + 10) {
+    t = &J->eps;
+    base = DNLG_FIELD_STRAIN;
+  } else if (f >= DNLG_FIELD_PLASTIC_STRAIN_TENSOR && f < DNLG_FIELD_PLASTIC_STRAIN_TENSOR + 10) {
+    t = &J->epsP;
+    base = DNLG_FIELD_PLASTIC_STRAIN_TENSOR;
+  }
+  if (!t || t->empty()) return 0.0;
+  const double *s = &(*t)[g * 6];
+  const int c = f - base;
+  if (c == 0) return norm_sym(s);
+  const int rows[9] = {0, 0, 0, 1, 1, 1, 2, 2, 2}, cols[9] = {0, 1, 2, 0, 1, 2, 0, 1, 2};
+  return s[sym_idx(rows[c - 1], cols[c - 1])];
+}
+
+bool is_ip_field(int f) {
+  switch (f) {
+    case DNLG_FIELD_PLASTIC_STRAIN: case DNLG_FIELD_PLASTIC_STRAIN_RATE: case DNLG_FIELD_GAMMA:
+    case DNLG_FIELD_GAMMA_CUMULATE: case DNLG_FIELD_YIELD_STRESS: case DNLG_FIELD_T: case DNLG_FIELD_DENSITY:
+    case DNLG_FIELD_PRESSURE: case DNLG_FIELD_INTERNAL_ENERGY: case DNLG_FIELD_VON_MISES:
+      return true;
+  }
+  return (f >= DNLG_FIELD_STRESS && f < DNLG_FIELD_STRESS + 10) ||
+         (f >= DNLG_FIELD_STRAIN && f < DNLG_FIELD_STRAIN + 10) ||
+         (f >= DNLG_FIELD_PLASTIC_STRAIN_TENSOR && f < DNLG_FIELD_PLASTIC_STRAIN_TENSOR + 10);
+}
+
+/* Node::fieldScalar (Node.C:311-370) */
+double node_value(const OracleJob *J, int f, int64_t n) {
+  if (f == DNLG_FIELD_MASS) return J->mass[n];
+  if (f < DNLG_FIELD_DENSITY) {
+    const std::vector<double> *v[5] = {&J->x, &J->disp, &J->fu[J->f0], &J->fv[J->f0], &J->fa[J->f0]};
+    const std::vector<double> &a = *v[f / 4];
+    if (f % 4 == 0) return std::sqrt(sq(a[n]) + sq(a[J->N + n]) + sq(a[2 * J->N + n]));
+    return a[(f % 4 - 1) * J->N + n];
+  }
+  if (!is_ip_field(f)) return 0.0;
+  double E[8][8];
+  extrapolation(J->tab.type, E);
+  const auto &els = J->node_elems[n];
+  const bool tensor = f == DNLG_FIELD_STRESS || f == DNLG_FIELD_STRAIN || f == DNLG_FIELD_PLASTIC_STRAIN_TENSOR;
+  if (tensor) {
+    /* the norm of the extrapolated tensor */
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    const std::vector<double> &t = f == DNLG_FIELD_STRESS ? J->sig : (f == DNLG_FIELD_STRAIN ? J->eps : J->epsP);
+    for (const auto &el : els)
+      for (int p = 0; p < J->nip; p++)
+        for (int c = 0; c < 6; c++) acc[c] += E[el.second][p] * t[(el.first * J->nip + p) * 6 + c];
+    for (int c = 0; c < 6; c++) acc[c] = acc[c] / (double)els.size();
+    return norm_sym(acc);
+  }
+  double v = 0.0;
+  for (const auto &el : els)
+    for (int p = 0; p < J->nip; p++) v += E[el.second][p] * ip_field(J, f, el.first * J->nip + p);
+  return v / (double)els.size();
+}
+
+/* HistoryFile*Item::getValue (HistoryFile.C:46-99) */
+double item_value(const OracleJob *J, const dnlg_history_item &it) {
+  if (it.kind == DNLG_H_NODE) return node_value(J, it.field, it.index);
+  if (it.kind == DNLG_H_ELEMENT) {
+    if (!is_ip_field(it.field)) return 0.0;
+    /* Element::getIntPointValue (Element.C:1224-1238) */
+    if (it.int_pt > 0) return ip_field(J, it.field, it.index * J->nip + it.int_pt - 1);
+    double v = 0.0;
+    for (int p = 0; p < J->nip; p++) v += ip_field(J, it.field, it.index * J->nip + p);
+    return v / J->nip;
+  }
+  switch (it.field) {
+    case DNLG_FIELD_REAL_TIME_STEP:
+    case DNLG_FIELD_TIME_STEP: return J->dt;
+    case DNLG_FIELD_KINETIC_ENERGY: {
+      /* Model::getTotalKineticEnergy (Model.C:612-625) */
+      double ke = 0.0;
+      for (int64_t n = 0; n < J->N; n++) {
+        double v2 = 0.0;
+        for (int d = 0; d < 3; d++) v2 += sq(J->fv[J->f0][d * J->N + n]);
+        ke += (J->mass[n] * v2) / 2.0;
+      }
+      return ke;
+    }
+  }
+  return 0.0;
+}
+
+/* Model::writeHistoryFiles -> HistoryFile::save (Model.C:917-924,
+ * HistoryFile.C:283-315) */
+void write_history(OracleJob *J) {
+  for (auto &h : J->hist) {
+    if ((J->t > h.start) && (J->t < h.next)) continue;
+    h.next += h.save;
+    h.rows.push_back(J->t);
+    for (const auto &it : h.items) h.rows.push_back(item_value(J, it));
+  }
+}
+
 }  // namespace
 
 /* ===========================================================================
@@ -1492,6 +1652,7 @@ int oracle_job_solve(OracleJob *J, double upTo) {
     compute_mass(J);
     if (!compute_time_step(J, true)) return J->status;
     if (!element_phases(J, J->dt)) return J->status;
+    write_history(J);
   } else if (J->t < upTo) {
     /* A later call continues the loop (the engine's semantics; the
      * reference re-runs its pre-loop block instead).  The last cycle of the
@@ -1508,8 +1669,9 @@ int oracle_job_solve(OracleJob *J, double upTo) {
     if (!element_phases(J, J->dt)) return J->status;
     corrector(J);
     densities(J);
-    /* Explicit::endStep (Explicit.C:539-552): time, field swap */
+    /* Explicit::endStep (Explicit.C:539-552): time, history rows, field swap */
     J->t += J->dt;
+    write_history(J);
     J->f0 = 1 - J->f0;
     if (!nodes_finite(J)) {
       set_err(J, DNLG_E_NONFINITE, "non-finite nodal state");
@@ -1613,6 +1775,57 @@ int oracle_job_energies(const OracleJob *J, double out[4]) {
 }
 
 int oracle_job_status(const OracleJob *J) { return J->status; }
+
+/* HistoryFile::setSaveTime + add + Model::add (HistoryFile.C:131-280,
+ * Model.C:927-939), rows only (no file); the engine's item conventions
+ * (dnlg_history_item) */
+int oracle_job_add_history(OracleJob *J, double start_time, double save_time, const dnlg_history_item *items,
+                           int32_t n_items, int32_t *file_id) {
+  if (J->started) return DNLG_E_STATE;
+  for (int32_t k = 0; k < n_items; k++) {
+    const dnlg_history_item &it = items[k];
+    if (it.field < 0 || it.field >= DNLG_FIELD_COUNT) return DNLG_E_INVALID;
+    if ((it.field >= DNLG_FIELD_STRAIN_INC && it.field < DNLG_FIELD_STRAIN_INC + 10) ||
+        (it.field >= DNLG_FIELD_PLASTIC_STRAIN_INC && it.field < DNLG_FIELD_PLASTIC_STRAIN_INC + 10))
+      return DNLG_E_INVALID;
+    if (it.kind == DNLG_H_NODE && (it.index < 0 || it.index >= J->N)) return DNLG_E_INVALID;
+    if (it.kind == DNLG_H_ELEMENT && (it.index < 0 || it.index >= J->NE || it.int_pt > J->nip)) return DNLG_E_INVALID;
+  }
+  if (J->node_elems.empty()) {
+    /* Node::elements: the elements in creation (index) order */
+    J->node_elems.resize(J->N);
+    for (int64_t e = 0; e < J->NE; e++)
+      for (int a = 0; a < J->nn; a++) J->node_elems[J->conn[e * J->nn + a]].push_back({e, a});
+  }
+  OracleJob::History h;
+  h.start = start_time;
+  h.next = start_time;
+  h.save = save_time;
+  h.items.assign(items, items + n_items);
+  J->hist.push_back(h);
+  if (file_id) *file_id = (int32_t)J->hist.size() - 1;
+  return DNLG_OK;
+}
+
+int oracle_job_history_rows(const OracleJob *J, int32_t file_id, double *rows, int64_t max_rows, int64_t *n_rows) {
+  if (file_id < 0 || file_id >= (int32_t)J->hist.size()) return DNLG_E_INVALID;
+  const auto &h = J->hist[file_id];
+  const int64_t width = 1 + (int64_t)h.items.size();
+  const int64_t n = (int64_t)h.rows.size() / width;
+  if (n_rows) *n_rows = n;
+  if (rows && max_rows > 0) std::memcpy(rows, h.rows.data(), std::min(n, max_rows) * width * sizeof(double));
+  return DNLG_OK;
+}
+
+/* the integrationPointsToNode table, [node][ip] */
+int oracle_extrapolation(int type, double *E) {
+  double e[8][8];
+  extrapolation(type, e);
+  const int nn = type == DNLG_EL_HEX8N_3D ? 8 : 4;
+  for (int a = 0; a < nn; a++)
+    for (int p = 0; p < nn; p++) E[a * nn + p] = e[a][p];
+  return DNLG_OK;
+}
 
 /* ---- unit-level entry points for pinning tests ---- */
 void oracle_eig3(const double A[9], double Q[9], double w[3]) {
