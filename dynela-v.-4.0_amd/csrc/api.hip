@@ -119,8 +119,228 @@ int64_t footprint(int type, int64_t N, int64_t NE, int prec, int track) {
   return b;
 }
 
-}  // namespace
 
+// Labels of Field::FieldLabel (Field.C vtkNames): the enum identifiers,
+// yieldStress labelled "yield".
+std::string vtk_label(int f) {
+  static const char *vec[5] = {"coords", "disp", "dispInc", "speed", "speedInc"};
+  static const char *sca[15] = {"density", "gamma", "gammaCumulate", "internalEnergy", "inelasticEnergy",
+                                "kineticEnergy", "mass", "plasticStrain", "plasticStrainRate", "pressure",
+                                "realTimeStep", "T", "timeStep", "vonMises", "yield"};
+  static const char *ten[5] = {"PlasticStrain", "PlasticStrainInc", "Strain", "StrainInc", "Stress"};
+  static const char *vc[4] = {"", "X", "Y", "Z"};
+  static const char *tc[10] = {"", "XX", "XY", "XZ", "YX", "YY", "YZ", "ZX", "ZY", "ZZ"};
+  if (f < 0 || f >= DNLG_FIELD_COUNT) return "";
+  if (f < DNLG_FIELD_DENSITY) return std::string(vec[f / 4]) + vc[f % 4];
+  if (f < DNLG_FIELD_PLASTIC_STRAIN_TENSOR) return sca[f - DNLG_FIELD_DENSITY];
+  const int k = f - DNLG_FIELD_PLASTIC_STRAIN_TENSOR;
+  return std::string(ten[k / 10]) + tc[k % 10];
+}
+
+// How Node::fieldScalar (Node.C:311-370; kind DNLG_H_NODE) or
+// Element::getIntPointValueExtract (Element.C:1195-1221; DNLG_H_ELEMENT)
+// evaluates a field: it.op (kHZero for a field the function does not know),
+// it.src, it.comp.  Tensor blocks of Field.h:71-75 are norm, XX, XY, XZ, YX,
+// YY, YZ, ZX, ZY, ZZ.  Returns 0, DNLG_E_INVALID (the increments StrainInc,
+// PlasticStrainInc, which the engine does not keep) or DNLG_E_STATE (strain
+// tensors not tracked).
+int field_spec(const dnlg_job *job, int kind, int f, HistItem &it) {
+  static const int tcomp[10] = {kHNorm, 0, 1, 2, 1, 3, 4, 2, 4, 5};
+  it.op = kHZero;
+  it.src = 0;
+  it.comp = 0;
+  if (kind == DNLG_H_NODE) {
+    if (f == DNLG_FIELD_MASS) {
+      it.op = kHNodal;
+      it.src = kHMass;
+      return 0;
+    }
+    if (f < DNLG_FIELD_DENSITY) {  // the five nodal vectors: norm, X, Y, Z
+      static const int vsrc[5] = {kHCoords, kHDisp, kHDispInc, kHSpeed, kHSpeedInc};
+      it.op = kHNodal;
+      it.src = vsrc[f / 4];
+      it.comp = f % 4 == 0 ? 3 : f % 4 - 1;
+      return 0;
+    }
+  }
+  const int ipop = kind == DNLG_H_NODE ? kHNodeIP : kHElemIP;
+  static const int scalars[][2] = {{DNLG_FIELD_PLASTIC_STRAIN, kHEp},    {DNLG_FIELD_PLASTIC_STRAIN_RATE, kHDep},
+                                   {DNLG_FIELD_GAMMA, kHGam},            {DNLG_FIELD_GAMMA_CUMULATE, kHGamc},
+                                   {DNLG_FIELD_YIELD_STRESS, kHSy},      {DNLG_FIELD_T, kHT},
+                                   {DNLG_FIELD_DENSITY, kHRho},          {DNLG_FIELD_PRESSURE, kHPres},
+                                   {DNLG_FIELD_INTERNAL_ENERGY, kHEint}, {DNLG_FIELD_VON_MISES, kHVm}};
+  for (auto &sc : scalars)
+    if (f == sc[0]) {
+      it.op = ipop;
+      it.src = sc[1];
+      return 0;
+    }
+  auto in_block = [&](int base) { return f >= base && f < base + 10; };
+  if (in_block(DNLG_FIELD_STRAIN_INC) || in_block(DNLG_FIELD_PLASTIC_STRAIN_INC)) return DNLG_E_INVALID;
+  int base = -1, src = 0;
+  if (in_block(DNLG_FIELD_STRESS)) {
+    base = DNLG_FIELD_STRESS;
+    src = kHSig;
+  } else if (in_block(DNLG_FIELD_STRAIN)) {
+    base = DNLG_FIELD_STRAIN;
+    src = kHEps;
+  } else if (in_block(DNLG_FIELD_PLASTIC_STRAIN_TENSOR)) {
+    base = DNLG_FIELD_PLASTIC_STRAIN_TENSOR;
+    src = kHEpsP;
+  }
+  if (base < 0) return 0;
+  if (src != kHSig && !job->track) return DNLG_E_STATE;
+  it.op = ipop;
+  it.src = src;
+  it.comp = tcomp[f - base];
+  return 0;
+}
+
+// Node -> (element, local node) lists in element order (Node::elements,
+// filled as elements are created) for the node-field kernel; built once.
+int ensure_node_lists(dnlg_job *job) {
+  JobDev &H = job->host;
+  if (H.nf_ptr) return DNLG_OK;
+  dnlg_ctx *ctx = job->ctx;
+  const int nn = job->nn;
+  std::vector<int32_t> conn((size_t)job->NE * nn);
+  HIPCHK(ctx, hipMemcpy(conn.data(), H.conn, conn.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+  std::vector<int32_t> ptr(job->N + 1, 0);
+  for (int32_t c : conn) ptr[c + 1]++;
+  for (int64_t n = 0; n < job->N; n++) ptr[n + 1] += ptr[n];
+  std::vector<int2> pairs(conn.size());
+  std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
+  for (int64_t e = 0; e < job->NE; e++)
+    for (int a = 0; a < nn; a++) pairs[fill[conn[e * nn + a]]++] = make_int2((int)e, a);
+  int32_t *d_ptr;
+  int2 *d_pairs;
+  int st;
+  if ((st = dalloc(job, &d_ptr, (int64_t)ptr.size(), ptr.data())) ||
+      (st = dalloc(job, &d_pairs, (int64_t)pairs.size(), pairs.data())))
+    return st;
+  H.nf_ptr = d_ptr;
+  H.nf_pairs = d_pairs;
+  const size_t off = offsetof(JobDev, nf_ptr), end = offsetof(JobDev, nf_pairs) + sizeof(H.nf_pairs);
+  HIPCHK(ctx, hipMemcpy((char *)job->dev + off, (const char *)&H + off, end - off, hipMemcpyHostToDevice));
+  return DNLG_OK;
+}
+
+// Node::fieldScalar of every node (out [N]) with field0 in slot f0;
+// tensor = true: the six components of Node::fieldSymTensor2 (out [6][N]).
+int node_field(dnlg_job *job, int field, int f0, bool tensor, double *out) {
+  dnlg_ctx *ctx = job->ctx;
+  const int64_t N = job->N;
+  const JobDev &H = job->host;
+  HistItem it{};
+  int st = field_spec(job, DNLG_H_NODE, field, it);
+  if (st) return fail(ctx, st, "field not available at the nodes");
+  if (it.op == kHZero) {
+    std::memset(out, 0, (tensor ? 6 : 1) * N * sizeof(double));
+    return DNLG_OK;
+  }
+  if (it.op == kHNodal) {
+    if (it.src == kHMass) {
+      HIPCHK(ctx, hipMemcpy(out, H.mass, N * sizeof(double), hipMemcpyDeviceToHost));
+      return DNLG_OK;
+    }
+    const double *src = it.src == kHCoords || it.src == kHDisp ? H.x
+                        : it.src == kHDispInc                  ? H.fu[f0]
+                        : it.src == kHSpeed                    ? H.fv[f0]
+                                                               : H.fa[f0];
+    std::vector<double> v(3 * N, 0.0), x0;
+    HIPCHK(ctx, hipMemcpy(v.data(), src, job->dim * N * sizeof(double), hipMemcpyDeviceToHost));
+    if (it.src == kHDisp) {
+      // the node displacement sum of Explicit.C:530-531 is x - x0
+      x0.assign(3 * N, 0.0);
+      HIPCHK(ctx, hipMemcpy(x0.data(), H.x0, job->dim * N * sizeof(double), hipMemcpyDeviceToHost));
+      for (int64_t k = 0; k < 3 * N; k++) v[k] -= x0[k];
+    }
+    for (int64_t n = 0; n < N; n++)
+      out[n] = it.comp == 3 ? std::sqrt(sqd(v[n]) + sqd(v[N + n]) + sqd(v[2 * N + n])) : v[it.comp * N + n];
+    return DNLG_OK;
+  }
+  if ((st = ensure_node_lists(job))) return st;
+  double *d_out = nullptr;
+  const int64_t count = (tensor ? 6 : 1) * N;
+  HIPCHK(ctx, hipMalloc(&d_out, count * sizeof(double)));
+  launch_node_field(job->dev, N, it.src, tensor ? -1 : it.comp, d_out, ctx->stream);
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) e = hipMemcpy(out, d_out, count * sizeof(double), hipMemcpyDeviceToHost);
+  hipFree(d_out);
+  if (e != hipSuccess) return fail(ctx, DNLG_E_HIP, std::string("node field: ") + hipGetErrorString(e));
+  return DNLG_OK;
+}
+
+// VtkInterface::write (VtkInterface.C:78-278) of the job's state, field0 in
+// slot f0, to <base><index, 3 digits>.vtk (DynELA::writeVTKFile,
+// DynELA.C:688-710).  Numbers as C++ streams print doubles (%g).
+int write_vtk(dnlg_job *job, double time, int f0) {
+  dnlg_ctx *ctx = job->ctx;
+  char num[32];
+  std::snprintf(num, sizeof(num), "%03d", job->vtk_index);
+  const std::string path = job->vtk_base + num + ".vtk";
+  const int64_t N = job->N, NE = job->NE;
+  const int nn = job->nn;
+  std::vector<double> x(3 * N, 0.0);
+  HIPCHK(ctx, hipMemcpy(x.data(), job->host.x, job->dim * N * sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<int32_t> conn((size_t)NE * nn);
+  HIPCHK(ctx, hipMemcpy(conn.data(), job->host.conn, conn.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+  FILE *fp = std::fopen(path.c_str(), "w");
+  if (!fp) return fail(ctx, DNLG_E_INVALID, "cannot open result file " + path);
+  // VtkInterface::headerWrite with the name and version of
+  // Sources/dnlSettings/configuration.dnl
+  std::fprintf(fp, "# vtk DataFile Version 4.0\n%s v.%s\nASCII\nDATASET UNSTRUCTURED_GRID\n", "DynELA FEM Code", "4.0");
+  std::fprintf(fp, "FIELD FieldData 1\nTIME 1 1 double\n%g\n", time);
+  std::fprintf(fp, "POINTS %lld float\n", (long long)N);
+  for (int64_t n = 0; n < N; n++) std::fprintf(fp, "%g %g %g\n", x[n], x[N + n], x[2 * N + n]);
+  std::fprintf(fp, "\nCELLS %lld %lld\n", (long long)NE, (long long)(NE * (nn + 1)));
+  for (int64_t e = 0; e < NE; e++) {
+    std::fprintf(fp, "%d ", nn);
+    for (int a = 0; a < nn; a++) std::fprintf(fp, "%d ", conn[e * nn + a]);
+    std::fprintf(fp, "\n");
+  }
+  // vtk cell types of ElQua4NAx (9, quad) and ElHex8N3D (12, hexahedron)
+  std::fprintf(fp, "\nCELL_TYPES %lld\n", (long long)NE);
+  for (int64_t e = 0; e < NE; e++) std::fprintf(fp, "%d\n", job->type == DNLG_EL_HEX8N_3D ? 12 : 9);
+  std::fprintf(fp, "\nPOINT_DATA %lld\n", (long long)N);
+  std::vector<double> v(6 * N);
+  int st = DNLG_OK;
+  for (int32_t f : job->vtk_fields) {
+    // Field::getType (Field.C): the five nodal vectors and the five tensors
+    // by their block's first value, everything else a scalar
+    const bool vec = f == DNLG_FIELD_COORDS || f == DNLG_FIELD_DISP || f == DNLG_FIELD_DISP_INC ||
+                     f == DNLG_FIELD_SPEED || f == DNLG_FIELD_SPEED_INC;
+    const bool ten = f == DNLG_FIELD_STRESS || f == DNLG_FIELD_STRAIN || f == DNLG_FIELD_PLASTIC_STRAIN_TENSOR ||
+                     f == DNLG_FIELD_STRAIN_INC || f == DNLG_FIELD_PLASTIC_STRAIN_INC;
+    // fields the engine does not keep (increments, untracked strains) are left out
+    HistItem probe{};
+    if (field_spec(job, DNLG_H_NODE, f, probe)) continue;
+    const std::string label = vtk_label(f);
+    if (vec) {
+      for (int d = 0; d < 3 && !st; d++) st = node_field(job, f + 1 + d, f0, false, v.data() + d * N);
+      if (st) break;
+      std::fprintf(fp, "VECTORS %s float\n", label.c_str());
+      for (int64_t n = 0; n < N; n++) std::fprintf(fp, "%g %g %g\n", v[n], v[N + n], v[2 * N + n]);
+    } else if (ten) {
+      if ((st = node_field(job, f, f0, true, v.data()))) break;
+      std::fprintf(fp, "TENSORS %s float\n", label.c_str());
+      static const int k9[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+      for (int64_t n = 0; n < N; n++)
+        for (int k = 0; k < 9; k++) std::fprintf(fp, k < 8 ? "%g " : "%g\n", v[k9[k] * N + n]);
+    } else {
+      if ((st = node_field(job, f, f0, false, v.data()))) break;
+      std::fprintf(fp, "SCALARS %s float\nLOOKUP_TABLE default\n", label.c_str());
+      for (int64_t n = 0; n < N; n++) std::fprintf(fp, "%g\n", v[n]);
+    }
+  }
+  std::fprintf(fp, "\n");
+  std::fclose(fp);
+  if (st) return st;
+  job->vtk_index++;
+  return DNLG_OK;
+}
+
+}  // namespace
 
 namespace dnlg {
 
@@ -157,6 +377,21 @@ int history_drain(dnlg_job *job) {
     }
     h.drained = f[i].rows;
   }
+  return DNLG_OK;
+}
+
+// A job held at the end of a cycle: that cycle is closed (the record in
+// slot 0 has its new time and the swapped field0), the file takes the
+// written cycle's field0, the other slot, as the reference's endStep writes
+// before its field swap.
+int result_file_release(dnlg_job *job) {
+  dnlg_ctx *ctx = job->ctx;
+  Control c;
+  HIPCHK(ctx, hipMemcpy(&c, &job->dev->c[0], sizeof(Control), hipMemcpyDeviceToHost));
+  int st = write_vtk(job, c.t, 1 - c.f0);
+  if (st) return st;
+  const int32_t zero = 0;
+  HIPCHK(ctx, hipMemcpy(&job->dev->res.hold, &zero, sizeof(zero), hipMemcpyHostToDevice));
   return DNLG_OK;
 }
 
@@ -668,6 +903,55 @@ int dnlg_job_energies(dnlg_job *job, double out[4]) {
   return DNLG_OK;
 }
 
+// ---- result (VTK) files and node fields ---------------------------------------
+int dnlg_job_set_result_files(dnlg_job *job, const char *base_name, double start_time, double end_time,
+                              double increment, const int32_t *fields, int32_t n_fields) {
+  if (!job) return DNLG_E_INVALID;
+  dnlg_ctx *ctx = job->ctx;
+  if (!base_name || n_fields < 0 || (n_fields > 0 && !fields)) return fail(ctx, DNLG_E_INVALID, "invalid result files");
+  if (job->started) return fail(ctx, DNLG_E_STATE, "result files must be set before the first solve");
+  for (int32_t k = 0; k < n_fields; k++)
+    if (fields[k] < 0 || fields[k] >= DNLG_FIELD_COUNT) return fail(ctx, DNLG_E_INVALID, "unknown result field");
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, DNLG_E_HIP, "hipSetDevice");
+  job->vtk_base = base_name;
+  job->vtk_fields.assign(fields, fields + n_fields);
+  job->vtk_index = 0;
+  // DynELA::setSaveTimes (DynELA.C:479-486): next starts at start
+  job->host.res = ResultSched{start_time, end_time, increment, 1, 0};
+  HIPCHK(ctx, hipMemcpy(&job->dev->res, &job->host.res, sizeof(ResultSched), hipMemcpyHostToDevice));
+  return DNLG_OK;
+}
+
+int dnlg_job_write_result_file(dnlg_job *job, int32_t force) {
+  if (!job) return DNLG_E_INVALID;
+  dnlg_ctx *ctx = job->ctx;
+  if (job->vtk_base.empty()) return fail(ctx, DNLG_E_STATE, "no result files set");
+  hipSetDevice(ctx->device);
+  Control c;
+  int st = read_control(job, &c);
+  if (st) return st;
+  if (!force) {
+    // DynELA::writeResultFile (DynELA.C:489-503)
+    ResultSched r;
+    HIPCHK(ctx, hipMemcpy(&r, &job->dev->res, sizeof(r), hipMemcpyDeviceToHost));
+    if (!(c.t >= r.next && c.t <= r.end)) return DNLG_OK;
+    r.next += r.inc;
+    HIPCHK(ctx, hipMemcpy(&job->dev->res, &r, sizeof(r), hipMemcpyHostToDevice));
+  }
+  return write_vtk(job, c.t, c.f0);
+}
+
+int dnlg_job_read_node_field(dnlg_job *job, int32_t field, double *out) {
+  if (!job || !out) return DNLG_E_INVALID;
+  dnlg_ctx *ctx = job->ctx;
+  if (field < 0 || field >= DNLG_FIELD_COUNT) return fail(ctx, DNLG_E_INVALID, "unknown field");
+  hipSetDevice(ctx->device);
+  Control c;
+  int st = read_control(job, &c);
+  if (st) return st;
+  return node_field(job, field, c.f0, false, out);
+}
+
 // ---- history files (HistoryFile.C) ------------------------------------------
 int dnlg_job_add_history(dnlg_job *job, const char *path, double start_time, double save_time,
                          const dnlg_history_item *items, int32_t n_items, const char *const *names,
@@ -694,74 +978,20 @@ int dnlg_job_add_history(dnlg_job *job, const char *path, double start_time, dou
     it.index = in.index;
     const int f = in.field;
     if (f < 0 || f >= DNLG_FIELD_COUNT) return fail(ctx, DNLG_E_INVALID, "unknown history field");
-    // tensor blocks of Field.h:71-75: norm, XX, XY, XZ, YX, YY, YZ, ZX, ZY, ZZ
-    static const int tcomp[10] = {kHNorm, 0, 1, 2, 1, 3, 4, 2, 4, 5};
-    auto tensor = [&](int base, int src) {
-      it.src = src;
-      it.comp = tcomp[f - base];
-    };
-    auto ip_field = [&]() -> int {  // IP fields shared by node and element items
-      const int scalars[][2] = {{DNLG_FIELD_PLASTIC_STRAIN, kHEp}, {DNLG_FIELD_PLASTIC_STRAIN_RATE, kHDep},
-                                {DNLG_FIELD_GAMMA, kHGam},          {DNLG_FIELD_GAMMA_CUMULATE, kHGamc},
-                                {DNLG_FIELD_YIELD_STRESS, kHSy},    {DNLG_FIELD_T, kHT},
-                                {DNLG_FIELD_DENSITY, kHRho},        {DNLG_FIELD_PRESSURE, kHPres},
-                                {DNLG_FIELD_INTERNAL_ENERGY, kHEint}, {DNLG_FIELD_VON_MISES, kHVm}};
-      for (auto &sc : scalars)
-        if (f == sc[0]) {
-          it.src = sc[1];
-          it.comp = 0;
-          return 1;
-        }
-      if (f >= DNLG_FIELD_STRESS && f < DNLG_FIELD_STRESS + 10) {
-        tensor(DNLG_FIELD_STRESS, kHSig);
-        return 1;
-      }
-      if ((f >= DNLG_FIELD_STRAIN && f < DNLG_FIELD_STRAIN + 10) ||
-          (f >= DNLG_FIELD_PLASTIC_STRAIN_TENSOR && f < DNLG_FIELD_PLASTIC_STRAIN_TENSOR + 10)) {
-        if (!job->track) return -2;
-        if (f >= DNLG_FIELD_STRAIN && f < DNLG_FIELD_STRAIN + 10)
-          tensor(DNLG_FIELD_STRAIN, kHEps);
-        else
-          tensor(DNLG_FIELD_PLASTIC_STRAIN_TENSOR, kHEpsP);
-        return 1;
-      }
-      if ((f >= DNLG_FIELD_STRAIN_INC && f < DNLG_FIELD_STRAIN_INC + 10) ||
-          (f >= DNLG_FIELD_PLASTIC_STRAIN_INC && f < DNLG_FIELD_PLASTIC_STRAIN_INC + 10))
-        return -1;
-      return 0;
-    };
     int64_t ipn = -1;
     if (in.kind == DNLG_H_NODE) {
       if (in.index < 0 || in.index >= job->N) return fail(ctx, DNLG_E_INVALID, "history node out of range");
-      if (f == DNLG_FIELD_MASS) {
-        it.op = kHNodal;
-        it.src = kHMass;
-        it.comp = 0;
-      } else if (f < DNLG_FIELD_DENSITY) {  // the five nodal vectors: norm, X, Y, Z
-        static const int vsrc[5] = {kHCoords, kHDisp, kHDispInc, kHSpeed, kHSpeedInc};
-        it.op = kHNodal;
-        it.src = vsrc[f / 4];
-        it.comp = f % 4 == 0 ? 3 : f % 4 - 1;
-      } else {
-        const int r = ip_field();
-        if (r == -1) return fail(ctx, DNLG_E_INVALID, "StrainInc and PlasticStrainInc are not kept by the engine");
-        if (r == -2) return fail(ctx, DNLG_E_STATE, "strain tensors not tracked (track_tensors = 0)");
-        if (r == 1) {
-          it.op = kHNodeIP;
-          ipn = in.index;
-        }
-      }
     } else if (in.kind == DNLG_H_ELEMENT) {
       if (in.index < 0 || in.index >= job->NE) return fail(ctx, DNLG_E_INVALID, "history element out of range");
       if (in.int_pt > nip) return fail(ctx, DNLG_E_INVALID, "history integration point out of range");
-      const int r = ip_field();
-      if (r == -1) return fail(ctx, DNLG_E_INVALID, "StrainInc and PlasticStrainInc are not kept by the engine");
-      if (r == -2) return fail(ctx, DNLG_E_STATE, "strain tensors not tracked (track_tensors = 0)");
-      if (r == 1) {
-        it.op = kHElemIP;
-        // HistoryFile.C:247: point intPt - 1; Element.C:1229: a negative one is the mean
-        it.ip = in.int_pt > 0 ? in.int_pt - 1 : -1;
-      }
+      // HistoryFile.C:247: point intPt - 1; Element.C:1229: a negative one is the mean
+      it.ip = in.int_pt > 0 ? in.int_pt - 1 : -1;
+    }
+    if (in.kind == DNLG_H_NODE || in.kind == DNLG_H_ELEMENT) {
+      const int r = field_spec(job, in.kind, f, it);
+      if (r) return fail(ctx, r, r == DNLG_E_STATE ? "strain tensors not tracked (track_tensors = 0)"
+                                                   : "StrainInc and PlasticStrainInc are not kept by the engine");
+      if (it.op == kHNodeIP) ipn = in.index;
     } else if (in.kind == DNLG_H_GLOBAL) {
       if (f == DNLG_FIELD_TIME_STEP || f == DNLG_FIELD_REAL_TIME_STEP) it.op = kHDt;
       if (f == DNLG_FIELD_KINETIC_ENERGY) it.op = kHKe;

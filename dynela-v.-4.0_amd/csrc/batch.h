@@ -47,6 +47,10 @@ struct dnlg_job {
   std::vector<int2> hist_inc;
   std::vector<int32_t> hist_ke_cols;
   std::vector<void *> hist_tables;
+  // result (VTK) files: base name, next file index, fields written
+  std::string vtk_base;
+  int vtk_index = 0;
+  std::vector<int32_t> vtk_fields;
   ~dnlg_job() {
     for (auto &h : hist)
       if (h.fp) std::fclose(h.fp);
@@ -85,6 +89,9 @@ void batch_cache_clear(dnlg_ctx *ctx);
 // Copies the rows that the job's history files took since the last drain
 // from the device rings to the host records and files (api.hip).
 int history_drain(dnlg_job *job);
+// Writes the result file of a job held at the end of a cycle (the written
+// cycle's field0 is the other ping-pong slot) and releases it (api.hip).
+int result_file_release(dnlg_job *job);
 void batch_cache_forget(dnlg_ctx *ctx, const dnlg_job *job);
 
 }  // namespace dnlg

@@ -15,7 +15,7 @@
 
 namespace dnlg {
 
-constexpr int kSnap = 6;  // snapshot doubles per job (launch_snapshot)
+constexpr int kSnap = kSnapDoubles;  // snapshot doubles per job (launch_snapshot)
 
 struct TaskList {
   Task *d = nullptr;
@@ -32,7 +32,8 @@ struct Batch {
   TaskList step;        // cycle control + field0 BCs, over the BC node lists
   TaskList close;       // end of a launch sequence, one block per job
   // history files: items (256 per block), kinetic-energy node blocks, one
-  // block per job; hist_cap: the smallest ring (rows) of the batch's files
+  // block per job with history or result files (commit); hist_cap: the
+  // smallest ring (rows) of the batch's files
   TaskList hist_items, hist_ke, hist_jobs;
   int hist_cap = 0;
   bool has_courant = false, has_pi = false;  // time-step methods present
@@ -164,8 +165,9 @@ int build_batch(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, Batch &B) {
     std::vector<int64_t> hn, kn;
     std::vector<Task> per_job;
     for (int i = 0; i < n; i++) {
-      if (jobs[i]->hist.empty()) continue;
+      if (jobs[i]->hist.empty() && !jobs[i]->host.res.on) continue;
       per_job.push_back(Task{i, 0, 0, 0});
+      if (jobs[i]->hist.empty()) continue;
       hj.push_back(i);
       hn.push_back((int64_t)jobs[i]->hist_items.size());
       if (!jobs[i]->hist_ke_cols.empty()) {
@@ -483,6 +485,13 @@ int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vec
     if ((st = collect())) return st;
     for (auto *j : jobs)
       if ((st = history_drain(j))) return st;
+    bool held = false;
+    for (int i = 0; i < n; i++)
+      if (B->h_snap[kSnap * i + 6] != 0.0) {
+        held = true;
+        if ((st = result_file_release(jobs[i]))) return st;
+      }
+    if (held) continue;
     bool stalled = false;
     for (int i = 0; i < n; i++) stalled = stalled || B->h_snap[kSnap * i + 5] != 0.0;
     if (stalled) {

@@ -79,6 +79,18 @@ struct PiState {
   int32_t pad;
 };
 
+// Result (VTK) files of DynELA::writeResultFile (DynELA.C:489-503), called
+// from Explicit::endStep: after a cycle whose new time t has
+// next <= t <= end a file is written and next advances one increment.  On
+// the device the cycle's commit sets `hold`; a held job opens no further
+// cycle until the host has written the file from its state (the cycle is
+// closed by then: field0 of the written cycle is the other ping-pong slot)
+// and cleared the flag.
+struct ResultSched {
+  double next, end, inc;
+  int32_t on, hold;
+};
+
 // History files (HistoryFile::save, HistoryFile.C:283-315).  A file takes a
 // row at time t unless start < t < next; taking one advances next by one
 // save interval.  Rows go to a device ring of `cap` rows (row r at r % cap,
@@ -123,6 +135,7 @@ struct JobDev {
   Control c[2];
   Sync sync;
   PiState pi;
+  ResultSched res;
   // sizes
   int32_t type, dim, nn, nip;
   int64_t N, NE, G;  // nodes, elements, integration points
@@ -176,6 +189,10 @@ struct JobDev {
   double *hke;            // kinetic-energy partial sums, one per node block
   int32_t n_hf, n_hi;
   int32_t density_always;  // IP densities every cycle (a history item reads them)
+  // node -> (element, local node) lists in element order (Node::elements),
+  // for IP fields extrapolated to every node; built on first use
+  const int32_t *nf_ptr;
+  const int2 *nf_pairs;
 };
 
 // Address spaces for the kernels.  A job record is read-only while a kernel

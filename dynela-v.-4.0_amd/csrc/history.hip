@@ -11,7 +11,9 @@
 //   k_hist_ke      kinetic-energy partial sums, one per node block, when a
 //                  file with a kineticEnergy item takes a row
 //   k_hist_commit  one block per job: time column, kinetic energy, next save
-//                  time and row count of the files that take a row
+//                  time and row count of the files that take a row; the
+//                  result-file (VTK) schedule, which holds the job when a
+//                  file is due (DynELA::writeResultFile, DynELA.C:489-503)
 // At that point the cycle's field0 (the reference's field0 before the swap)
 // is fu/fv/fa[c.f0], coordinates and IP values are the new ones.  The host
 // drains the rings between launch sequences and writes the files.
@@ -284,6 +286,60 @@ __global__ __launch_bounds__(64) void k_hist_commit(JobDev *const *jobs, const T
     f.next += f.save;
     f.rows += 1;
   }
+  // DynELA::writeResultFile: after the history rows, in the cycle's endStep
+  if (!preloop && threadIdx.x == 0 && C->res.on && t >= C->res.next && t <= C->res.end) {
+    DNLG_GLOBAL ResultSched &r = *gbl(&J->res);
+    r.hold = 1;
+    r.next = C->res.next + C->res.inc;
+  }
+}
+
+// Node::fieldScalar / fieldSymTensor2 of an IP field at every node
+// (Node.C:244-300, 350-364, 411-425): the node's elements in element order,
+// per element the extrapolation weights of the node's local position times
+// the IP values, over the element count.
+__global__ __launch_bounds__(256) void k_node_field(JobDev *J, int src, int comp, double *out) {
+  CJobDev *C = (CJobDev *)J;
+  const int64_t N = C->N;
+  const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (n >= N) return;
+  const int nip = C->nip;
+  const int r0 = C->nf_ptr[n], r1 = C->nf_ptr[n + 1];
+  const bool tensor = comp < 0 || (comp == kHNorm && (src == kHSig || src == kHEps || src == kHEpsP));
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int r = r0; r < r1; r++) {
+    const int2 pr = C->nf_pairs[r];
+    for (int p = 0; p < nip; p++) {
+      const double w = extrap(nip, pr.y, p);
+      const int64_t g = (int64_t)pr.x * nip + p;
+      if (tensor) {
+        double s[6];
+        load_tensor(C, src, g, s);
+#pragma unroll
+        for (int k = 0; k < 6; k++) acc[k] += w * s[k];
+      } else {
+        acc[0] += w * ip_value(C, src, comp, g);
+      }
+    }
+  }
+  const double cnt = (double)(r1 - r0);
+  if (!tensor) {
+    gbl(out)[n] = acc[0] / cnt;
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < 6; k++) acc[k] = acc[k] / cnt;
+  if (comp < 0) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) gbl(out)[k * N + n] = acc[k];
+  } else {
+    gbl(out)[n] = sym_norm(acc);
+  }
+}
+
+void launch_node_field(JobDev *job, int64_t N, int src, int comp, double *out, hipStream_t s) {
+  if (N <= 0) return;
+  hipLaunchKernelGGL(k_node_field, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, job, src, comp, out);
 }
 
 void launch_history(JobDev *const *jobs, const Task *items, int n_items, const Task *ke, int n_ke, const Task *per_job,

@@ -32,8 +32,9 @@ void launch_force_dt(JobDev *const *jobs, int n, hipStream_t s);
 void launch_set_up_to(JobDev *const *jobs, int n, const double *up_to, hipStream_t s);
 void launch_prep(int type, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s);
 void launch_mass_gather(int dim, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s);
-// per job: {t, dt, up_to, status, inc, power-iteration stall} snapshot for
-// the host loop (6 doubles per job)
+// per job: {t, dt, up_to, status, inc, power-iteration stall, result-file
+// hold} snapshot for the host loop
+constexpr int kSnapDoubles = 7;
 void launch_snapshot(JobDev *const *jobs, int n, double *out, hipStream_t s);
 // power-iteration time step: begin (force: the pre-loop's forced step,
 // otherwise the jobs whose cycle in slot cs is due), one iteration = elem
@@ -55,5 +56,9 @@ void extrapolation_tables(double hexE[64], double quaE[16]);
 void upload_extrapolation();
 void launch_history(JobDev *const *jobs, const Task *items, int n_items, const Task *ke, int n_ke, const Task *per_job,
                     int n_jobs, int cs, int preloop, hipStream_t s);
+// Node::fieldScalar of an IP field at every node (src, comp as HistItem;
+// comp < 0 with a tensor source: all six components, out [6][N]); needs the
+// job's nf_ptr / nf_pairs lists
+void launch_node_field(JobDev *job, int64_t N, int src, int comp, double *out, hipStream_t s);
 
 }  // namespace dnlg

@@ -105,8 +105,9 @@ __device__ __forceinline__ bool close_open(CJobDev *C, Control &c, int status, d
   if (open) {
     c.preloop = 0;
     // a job whose power iteration stalled waits for the host-driven
-    // iteration (k_pi_finish clears the flag)
-    c.running = (status == 0 && c.t < c.up_to && !C->pi.stall) ? 1 : 0;
+    // iteration (k_pi_finish clears the flag), a held one for its result
+    // file
+    c.running = (status == 0 && c.t < c.up_to && !C->pi.stall && !C->res.hold) ? 1 : 0;
     if (c.running) {
       if (c.t + c.dt > c.up_to) c.dt = c.up_to - c.t;
       c.inc++;
@@ -1092,12 +1093,14 @@ __global__ void k_snapshot(JobDev *const *jobs, int n_jobs, double *out) {
   if (j >= n_jobs) return;
   const JobDev *J = jobs[j];
   const Control &c = J->c[0];
-  out[6 * j + 0] = c.t;
-  out[6 * j + 1] = c.dt;
-  out[6 * j + 2] = c.up_to;
-  out[6 * j + 3] = (double)(J->sync.st[0] != 0 ? J->sync.st[0] : J->sync.st[1]);
-  out[6 * j + 4] = (double)c.inc;
-  out[6 * j + 5] = (double)J->pi.stall;
+  double *o = out + kSnapDoubles * j;
+  o[0] = c.t;
+  o[1] = c.dt;
+  o[2] = c.up_to;
+  o[3] = (double)(J->sync.st[0] != 0 ? J->sync.st[0] : J->sync.st[1]);
+  o[4] = (double)c.inc;
+  o[5] = (double)J->pi.stall;
+  o[6] = (double)J->res.hold;
 }
 
 // Target time of the current advance, per job.

@@ -25,6 +25,7 @@ from ._lib import (DnlgError, NegativeJacobianError, NewtonRaphsonError, PowerIt
                    NonFiniteError)
 
 __all__ = ["DynELA", "NodeSet", "ElementSet", "Material", "JohnsonCookLaw", "ElastoplasticLaw", "HistoryFile",
+           "VtkInterface",
            "VoceLinearLaw", "BoundaryRestrain", "BoundarySpeed", "RampFunction", "Explicit", "Field",
            "Element", "Vec3D", "Context", "DnlgError", "NegativeJacobianError", "NewtonRaphsonError",
            "PowerIterationError", "NonFiniteError", "batch_solve"]
@@ -387,6 +388,36 @@ class _Parallel:
         self.cores = int(n)
 
 
+class VtkInterface:
+    """VtkInterface (VtkInterface.C): the point-data fields of the result
+    files.  The default list is the VtkFields setting of
+    Sources/dnlSettings/configuration.dnl, labels that name no field left out
+    (VtkInterface::initFields)."""
+
+    DEFAULT_LABELS = ("Stress", "Strain", "PlasticStrain", "vonMises", "yield", "pressure", "plasticStrain",
+                      "plasticStrainRate", "gamma", "gammaCumulate", "temperature", "speed", "displacement",
+                      "displacementIncrement")
+
+    def __init__(self, name=""):
+        self.name = name
+        self._fields = [f for f in (Field.getField(n) for n in self.DEFAULT_LABELS) if f != -1]
+
+    def existField(self, field):
+        return self._fields.index(int(field)) if int(field) in self._fields else -1
+
+    def addField(self, field):
+        if self.existField(field) == -1:
+            self._fields.append(int(field))
+
+    def removeField(self, field):
+        k = self.existField(field)
+        if k != -1:
+            del self._fields[k]
+
+    def getNumberOfFields(self):
+        return len(self._fields)
+
+
 class HistoryFile:
     """HistoryFile (HistoryFile.h, HistoryFile.C): time series of node,
     element and global values, one row per save time, written by the engine
@@ -512,6 +543,8 @@ class DynELA:
         self.saveTimes = None
         self._stress_method = "NewtonRaphson"
         self._histories = []
+        self.dataFile = VtkInterface()
+        self._results_started = False
 
     # ---- nodes and elements -------------------------------------------------
     def createNode(self, number, x, y=None, z=None):
@@ -709,15 +742,40 @@ class DynELA:
         self._job = job
         for h in self._histories:
             h._attach(self)
+        if self.saveTimes is not None:
+            # DynELA::setSaveTimes, result files <name><index>.vtk (DynELA.C:59, 688-710)
+            start, end, inc = self.saveTimes
+            fields = np.array(self.dataFile._fields, dtype=np.int32)
+            ctx.check(ctx.lib.dnlg_job_set_result_files(job, self.name.encode(), start, end, inc,
+                                                        _lib.as_i32p(fields), len(fields)))
         return True
 
     def solve(self, upTo=None):
-        """DynELA::solve (DynELA.C:727-772) / Model::solve(upTo)."""
+        """DynELA::solve (DynELA.C:727-772) / Model::solve(upTo).  With save
+        times set, the result files of the reference: the initial
+        configuration before the first solve (writeResultFile), one after
+        every cycle that reaches the next save time, and the final state
+        once the end time is reached (writeVTKFile)."""
         self.initSolve()
         t = self._solver.endTime if upTo is None or upTo < 0 else float(upTo)
         ctx = self.context
+        if self.saveTimes is not None and not self._results_started:
+            self._results_started = True
+            self.writeResultFile()
         ctx.check(ctx.lib.dnlg_job_solve(self._job, ctypes.c_double(t)))
+        if self.saveTimes is not None and self.getCurrentTime() >= self._solver.endTime:
+            self.writeVTKFile()
         return True
+
+    def writeResultFile(self):
+        """DynELA::writeResultFile (DynELA.C:489-503)."""
+        self.initSolve()
+        self.context.check(self.context.lib.dnlg_job_write_result_file(self._job, 0))
+
+    def writeVTKFile(self):
+        """DynELA::writeVTKFile (DynELA.C:688-710)."""
+        self.initSolve()
+        self.context.check(self.context.lib.dnlg_job_write_result_file(self._job, 1))
 
     def close(self):
         if self._job is not None:
@@ -811,6 +869,68 @@ class DynELA:
 
     def getNodeByNum(self, number):
         return _NodeView(self, number)
+
+    def getNodalField(self, field):
+        """Node::fieldScalar (Node.C:311-370) of every node in node-number
+        order: nodal values, or IP values extrapolated to the nodes and
+        averaged over each node's elements."""
+        self._require_job()
+        out = np.zeros(self.getNodesNumber())
+        self.context.check(self.context.lib.dnlg_job_read_node_field(self._job, int(field), _lib.as_dp(out)))
+        return out
+
+    def getNodalValuesRange(self, field):
+        """DynELA::getNodalValuesRange (DynELA.C:775-796): (min, max) of a
+        scalar field over the nodes."""
+        if int(field) in (Field.coords, Field.disp, Field.dispInc, Field.speed, Field.speedInc, Field.Strain,
+                          Field.StrainInc, Field.PlasticStrain, Field.PlasticStrainInc, Field.Stress):
+            raise DnlgError(_lib.E_INVALID, "field must be scalar")
+        v = self.getNodalField(field)
+        return float(v.min()), float(v.max())
+
+    def getGlobalBox(self):
+        """DynELA::getGlobalBox (DynELA.C:663-685): (min, max) corners of the
+        nodes' bounding box, current coordinates once solved."""
+        c = self.getNodalCoordinates() if self._job is not None else self._node_tables()[1].T
+        return Vec3D(*c.min(axis=1)), Vec3D(*c.max(axis=1))
+
+    # ---- geometry transforms of the mesh (before the first solve) ---------------
+    def _transform(self, fn, nodeSet):
+        nums, xyz = self._node_tables()
+        sel = np.ones(len(nums), bool) if nodeSet is None else np.isin(nums, nodeSet.numbers())
+        xyz = xyz.copy()
+        xyz[sel] = fn(xyz[sel])
+        self._invalidate()
+        self._node_num, self._node_xyz = [nums], [xyz]
+
+    def translate(self, vector, nodeSet=None):
+        """DynELA::translate (DynELA.C:542-552)."""
+        v = np.array([vector(0), vector(1), vector(2)])
+        self._transform(lambda x: x + v, nodeSet)
+
+    def scale(self, value, nodeSet=None):
+        """DynELA::scale (DynELA.C:554-578): a number or a Vec3D per axis."""
+        v = np.array([value(0), value(1), value(2)]) if isinstance(value, Vec3D) else float(value)
+        self._transform(lambda x: x * v, nodeSet)
+
+    def rotate(self, axis, angle, nodeSet=None):
+        """DynELA::rotate (DynELA.C:580-661), angle in degrees: about a Vec3D
+        axis (right-handed, Rodrigues' formula), or about "X" / "Y" / "Z";
+        the reference's "X" and "Y" matrices turn by -angle, its "Z" matrix
+        by +angle, and this keeps that."""
+        if isinstance(axis, str):
+            a = axis.upper()
+            if a not in ("X", "Y", "Z"):
+                return
+            k = np.eye(3)["XYZ".index(a)]
+            th = np.radians(angle) * (1.0 if a == "Z" else -1.0)
+        else:
+            k = np.array([axis(0), axis(1), axis(2)], dtype=np.float64)
+            k = k / np.linalg.norm(k)
+            th = np.radians(angle)
+        K = np.array([[0.0, -k[2], k[1]], [k[2], 0.0, -k[0]], [-k[1], k[0], 0.0]])
+        R = np.cos(th) * np.eye(3) + np.sin(th) * K + (1.0 - np.cos(th)) * np.outer(k, k)
+        self._transform(lambda x: x @ R.T, nodeSet)
 
 
 def batch_solve(models, upTo=None, context=None, raise_on_error=True, max_cycles=None):

@@ -113,6 +113,10 @@ SIGNATURES = {
                                             ctypes.POINTER(HistoryItem), ctypes.c_int32,
                                             ctypes.POINTER(ctypes.c_char_p), _i32p]),
     "dnlg_job_history_rows": (ctypes.c_int, [_VP, ctypes.c_int32, _dp, ctypes.c_int64, _i64p]),
+    "dnlg_job_set_result_files": (ctypes.c_int, [_VP, ctypes.c_char_p, ctypes.c_double, ctypes.c_double,
+                                                 ctypes.c_double, _i32p, ctypes.c_int32]),
+    "dnlg_job_write_result_file": (ctypes.c_int, [_VP, ctypes.c_int32]),
+    "dnlg_job_read_node_field": (ctypes.c_int, [_VP, ctypes.c_int32, _dp]),
 }
 
 _lib = None

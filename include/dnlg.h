@@ -328,6 +328,31 @@ int dnlg_job_add_history(dnlg_job *job, const char *path, double start_time,
 int dnlg_job_history_rows(dnlg_job *job, int32_t file_id, double *rows,
                           int64_t max_rows, int64_t *n_rows);
 
+/* ---- result (VTK) files and node fields -------------------------------------
+ * DynELA::setSaveTimes / writeResultFile / writeVTKFile (DynELA.C:479-503,
+ * 688-710) with VtkInterface::write (VtkInterface.C:78-278).  Set before the
+ * first solve: base name (files <base><index, 3 digits>.vtk), save times and
+ * the point-data fields (DNLG_FIELD_* values: a vector or tensor block's
+ * first value writes VECTORS / TENSORS, anything else SCALARS; fields the
+ * engine does not keep -- the increments, strains without track_tensors --
+ * are left out).  During a solve a file is written after every cycle whose
+ * new time t has next <= t <= end (next += increment): the job pauses at
+ * that cycle until the host has written the file from the device state, the
+ * others go on. */
+int dnlg_job_set_result_files(dnlg_job *job, const char *base_name,
+                              double start_time, double end_time,
+                              double increment, const int32_t *fields,
+                              int32_t n_fields);
+/* force = 0: DynELA::writeResultFile's rule at the job's current time (the
+ * reference calls it before its first solve); force = 1: DynELA::writeVTKFile
+ * (after the solve), a file whatever the time. */
+int dnlg_job_write_result_file(dnlg_job *job, int32_t force);
+/* Node::fieldScalar (Node.C:311-370) of every node, out [n_nodes]: nodal
+ * values, or IP values extrapolated to the nodes (integrationPointsToNode)
+ * and averaged over each node's elements; 0 for a field the reference's
+ * function does not know. */
+int dnlg_job_read_node_field(dnlg_job *job, int32_t field, double *out);
+
 /* ---- measurement hooks ------------------------------------------------------ */
 /* Device time (ms) and launch count of the element kernels, accumulated with
  * HIP events recorded around them on the engine's own stream since the last

@@ -64,6 +64,7 @@ def load():
         lib.oracle_job_history_rows.argtypes = [_vp, ctypes.c_int32, _dp, ctypes.c_int64,
                                                 ctypes.POINTER(ctypes.c_int64)]
         lib.oracle_extrapolation.argtypes = [ctypes.c_int, _dp]
+        lib.oracle_job_read_node_field.argtypes = [_vp, ctypes.c_int, _dp]
         _lib = lib
     return _lib
 
@@ -166,6 +167,14 @@ class OracleJob:
                 idx = 0
             items.append((kind, f, idx, ip))
         return self.add_history(hist.getStartTime(), hist.getSaveTime(), items)
+
+    def node_field(self, field):
+        """Node::fieldScalar of every node (Field enum value)."""
+        out = np.zeros(self.N)
+        st = self.lib.oracle_job_read_node_field(self.h, int(field), p(out))
+        if st != 0:
+            raise RuntimeError(f"oracle_job_read_node_field failed: {st}")
+        return out
 
     def history_rows(self, fid):
         """Rows taken so far, [rows, 1 + items] (time first)."""

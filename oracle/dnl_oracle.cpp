@@ -1817,6 +1817,18 @@ int oracle_job_history_rows(const OracleJob *J, int32_t file_id, double *rows, i
   return DNLG_OK;
 }
 
+/* Node::fieldScalar (Node.C:311-370) of every node, out [n_nodes] */
+int oracle_job_read_node_field(OracleJob *J, int field, double *out) {
+  if (field < 0 || field >= DNLG_FIELD_COUNT) return DNLG_E_INVALID;
+  if (J->node_elems.empty()) {
+    J->node_elems.resize(J->N);
+    for (int64_t e = 0; e < J->NE; e++)
+      for (int a = 0; a < J->nn; a++) J->node_elems[J->conn[e * J->nn + a]].push_back({e, a});
+  }
+  for (int64_t n = 0; n < J->N; n++) out[n] = node_value(J, field, n);
+  return DNLG_OK;
+}
+
 /* the integrationPointsToNode table, [node][ip] */
 int oracle_extrapolation(int type, double *E) {
   double e[8][8];

@@ -2,7 +2,9 @@
 HistoryFile::save (HistoryFile.C:283-315) through its consequences, item
 values against closed forms (free flight, uniform fields) and against the
 field read-back, the IP-to-node extrapolation table against the shape
-functions, and the header names of the dnlPython HistoryFile."""
+functions, the header names of the dnlPython HistoryFile; the node-field
+read-back, the result files' default fields and the mesh transforms of
+DynELA."""
 import ctypes
 
 import numpy as np
@@ -205,3 +207,68 @@ def test_history_item_names():
                                 "StressXY_e7", "StressXY_e9", "T", "kineticEnergy"]
     assert F.coordsX == 1 and F.density == 20 and F.yieldStress == 34 and F.Stress == 75 and F.ENDFIELDS == 85
     assert F.getField("yield") == F.yieldStress
+
+
+def test_vtk_default_fields():
+    """VtkInterface::initFields with the configuration's VtkFields: labels
+    that name no field (temperature, displacement, displacementIncrement) are
+    left out."""
+    v = dnl.VtkInterface()
+    assert v._fields == [F.Stress, F.Strain, F.PlasticStrain, F.vonMises, F.yieldStress, F.pressure, F.plasticStrain,
+                         F.plasticStrainRate, F.gamma, F.gammaCumulate, F.speed]
+    v.addField(F.T)
+    v.removeField(F.Strain)
+    assert v.getNumberOfFields() == 11 and v.existField(F.T) == 10 and v.existField(F.Strain) == -1
+
+
+def test_geometry_transforms():
+    """DynELA::translate / scale / rotate (DynELA.C:542-661) on the node
+    coordinates: a quarter turn about Z takes x to y; the reference's "X" and
+    "Y" matrices turn the other way (y -> -z, z -> -x); about an axis the
+    right-handed rotation; on a node set only its nodes move."""
+    def model():
+        m = dnl.DynELA("t")
+        m.createNodes([1, 2, 3], [[1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
+        return m
+
+    def xyz(m):
+        return m._node_tables()[1]
+
+    m = model()
+    m.rotate("Z", 90.0)
+    np.testing.assert_allclose(xyz(m), [[0, 1, 0], [-1, 0, 0], [0, 0, 1]], atol=1e-15)
+    m = model()
+    m.rotate("X", 90.0)
+    np.testing.assert_allclose(xyz(m), [[1, 0, 0], [0, 0, -1], [0, 1, 0]], atol=1e-15)
+    m = model()
+    m.rotate("Y", 90.0)
+    np.testing.assert_allclose(xyz(m), [[0, 0, 1], [0, 1, 0], [-1, 0, 0]], atol=1e-15)
+    m = model()
+    m.rotate(dnl.Vec3D(1, 1, 1), 120.0)  # the cyclic permutation x -> y -> z
+    np.testing.assert_allclose(xyz(m), [[0, 1, 0], [0, 0, 1], [1, 0, 0]], atol=1e-15)
+    m = model()
+    ns = dnl.NodeSet("2")
+    m.add(ns, 2)
+    m.translate(dnl.Vec3D(1, 2, 3), ns)
+    m.scale(2.0)
+    m.scale(dnl.Vec3D(1, 0.5, 1), ns)
+    np.testing.assert_allclose(xyz(m), [[2, 0, 0], [2, 3, 6], [0, 0, 2]], atol=1e-15)
+    lo, hi = m.getGlobalBox()
+    assert (lo(0), lo(1), lo(2), hi(0), hi(1), hi(2)) == (0, 0, 0, 2, 3, 6)
+
+
+def test_node_field_read_back_pins():
+    """Node::fieldScalar of every node: uniform IP fields extrapolate to the
+    same value at every node (weights sum to 1, divided by the element
+    count), nodal fields are the node arrays, unknown fields are 0."""
+    m = samples.taylor_axi(nr=3, nz=8, t_end=1e-6)
+    o = OracleJob(m.descriptors())
+    np.testing.assert_allclose(o.node_field(F.T), samples.COPPER["T0"], rtol=1e-14)
+    np.testing.assert_allclose(o.node_field(F.density), samples.COPPER["density"], rtol=1e-14)
+    assert o.solve(1e-6) == 0
+    c, d, v = o.nodes()
+    np.testing.assert_array_equal(o.node_field(F.coordsY), c[1])
+    np.testing.assert_array_equal(o.node_field(F.speedY), v[1])
+    np.testing.assert_array_equal(o.node_field(F.mass), o.mass())
+    np.testing.assert_allclose(o.node_field(F.disp), np.linalg.norm(d, axis=0), rtol=1e-15)
+    assert np.all(o.node_field(F.inelasticEnergy) == 0.0) and np.all(o.node_field(F.timeStep) == 0.0)
