@@ -23,6 +23,7 @@
 #include "../../include/dnlg.h"
 #include "batch.h"
 #include "kernels.h"
+#include "textfmt.h"
 
 using namespace dnlg;
 
@@ -290,18 +291,39 @@ int write_vtk(dnlg_job *job, double time, int f0) {
   // VtkInterface::headerWrite with the name and version of
   // Sources/dnlSettings/configuration.dnl
   std::fprintf(fp, "# vtk DataFile Version 4.0\n%s v.%s\nASCII\nDATASET UNSTRUCTURED_GRID\n", "DynELA FEM Code", "4.0");
-  std::fprintf(fp, "FIELD FieldData 1\nTIME 1 1 double\n%g\n", time);
-  std::fprintf(fp, "POINTS %lld float\n", (long long)N);
-  for (int64_t n = 0; n < N; n++) std::fprintf(fp, "%g %g %g\n", x[n], x[N + n], x[2 * N + n]);
-  std::fprintf(fp, "\nCELLS %lld %lld\n", (long long)NE, (long long)(NE * (nn + 1)));
-  for (int64_t e = 0; e < NE; e++) {
-    std::fprintf(fp, "%d ", nn);
-    for (int a = 0; a < nn; a++) std::fprintf(fp, "%d ", conn[e * nn + a]);
-    std::fprintf(fp, "\n");
+  {
+    char b[40];
+    *put_g(b, time) = 0;
+    std::fprintf(fp, "FIELD FieldData 1\nTIME 1 1 double\n%s\n", b);
   }
-  // vtk cell types of ElQua4NAx (9, quad) and ElHex8N3D (12, hexahedron)
-  std::fprintf(fp, "\nCELL_TYPES %lld\n", (long long)NE);
-  for (int64_t e = 0; e < NE; e++) std::fprintf(fp, "%d\n", job->type == DNLG_EL_HEX8N_3D ? 12 : 9);
+  // columns -> rows of `width` values, then "%g" text
+  std::vector<double> aos;
+  auto put_rows = [&](const double *const *cols, int width) {
+    aos.resize((size_t)N * width);
+    for (int64_t n = 0; n < N; n++)
+      for (int c = 0; c < width; c++) aos[n * width + c] = cols[c][n];
+    const std::string txt = format_rows(aos.data(), N, width, false, false);
+    std::fwrite(txt.data(), 1, txt.size(), fp);
+  };
+  std::fprintf(fp, "POINTS %lld float\n", (long long)N);
+  const double *xc[3] = {x.data(), x.data() + N, x.data() + 2 * N};
+  put_rows(xc, 3);
+  std::fprintf(fp, "\nCELLS %lld %lld\n", (long long)NE, (long long)(NE * (nn + 1)));
+  {
+    std::string txt;
+    txt.reserve((size_t)NE * (nn + 1) * 8);
+    char b[16];
+    for (int64_t e = 0; e < NE; e++) {
+      txt.append(b, std::to_chars(b, b + 16, nn).ptr - b) += ' ';
+      for (int a = 0; a < nn; a++) txt.append(b, std::to_chars(b, b + 16, conn[e * nn + a]).ptr - b) += ' ';
+      txt += '\n';
+    }
+    // vtk cell types of ElQua4NAx (9, quad) and ElHex8N3D (12, hexahedron)
+    txt += "\nCELL_TYPES " + std::to_string(NE) + "\n";
+    const char *ct = job->type == DNLG_EL_HEX8N_3D ? "12\n" : "9\n";
+    for (int64_t e = 0; e < NE; e++) txt += ct;
+    std::fwrite(txt.data(), 1, txt.size(), fp);
+  }
   std::fprintf(fp, "\nPOINT_DATA %lld\n", (long long)N);
   std::vector<double> v(6 * N);
   int st = DNLG_OK;
@@ -320,17 +342,21 @@ int write_vtk(dnlg_job *job, double time, int f0) {
       for (int d = 0; d < 3 && !st; d++) st = node_field(job, f + 1 + d, f0, false, v.data() + d * N);
       if (st) break;
       std::fprintf(fp, "VECTORS %s float\n", label.c_str());
-      for (int64_t n = 0; n < N; n++) std::fprintf(fp, "%g %g %g\n", v[n], v[N + n], v[2 * N + n]);
+      const double *vc[3] = {v.data(), v.data() + N, v.data() + 2 * N};
+      put_rows(vc, 3);
     } else if (ten) {
       if ((st = node_field(job, f, f0, true, v.data()))) break;
       std::fprintf(fp, "TENSORS %s float\n", label.c_str());
+      // t(0,0) t(0,1) t(0,2) t(1,0) t(1,1) t(1,2) t(2,0) t(2,1) t(2,2)
       static const int k9[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
-      for (int64_t n = 0; n < N; n++)
-        for (int k = 0; k < 9; k++) std::fprintf(fp, k < 8 ? "%g " : "%g\n", v[k9[k] * N + n]);
+      const double *tc[9];
+      for (int k = 0; k < 9; k++) tc[k] = v.data() + k9[k] * N;
+      put_rows(tc, 9);
     } else {
       if ((st = node_field(job, f, f0, false, v.data()))) break;
       std::fprintf(fp, "SCALARS %s float\nLOOKUP_TABLE default\n", label.c_str());
-      for (int64_t n = 0; n < N; n++) std::fprintf(fp, "%g\n", v[n]);
+      const double *sc[1] = {v.data()};
+      put_rows(sc, 1);
     }
   }
   std::fprintf(fp, "\n");
@@ -368,11 +394,8 @@ int history_drain(dnlg_job *job) {
       HIPCHK(ctx, hipMemcpy(h.rows.data() + old + first * width, h.ring, (n_new - first) * width * sizeof(double),
                             hipMemcpyDeviceToHost));
     if (h.fp) {
-      for (int64_t r = 0; r < n_new; r++) {
-        const double *row = h.rows.data() + old + r * width;
-        for (int64_t k = 0; k < width; k++) std::fprintf(h.fp, "%10.7E ", row[k]);
-        std::fprintf(h.fp, "\n");
-      }
+      const std::string txt = format_rows(h.rows.data() + old, n_new, width, true, true);
+      std::fwrite(txt.data(), 1, txt.size(), h.fp);
       std::fflush(h.fp);
     }
     h.drained = f[i].rows;
@@ -1039,7 +1062,7 @@ int dnlg_job_add_history(dnlg_job *job, const char *path, double start_time, dou
   int st;
   if ((st = dalloc(job, &h.ring, (int64_t)h.cap * (1 + n_items), (const double *)nullptr))) return st;
   if (need_ke && !job->host.hke)
-    if ((st = dalloc(job, &job->host.hke, (job->N + kNodeBlock - 1) / kNodeBlock, (const double *)nullptr))) return st;
+    if ((st = dalloc(job, &job->host.hke, (int64_t)hist_blocks(job->N), (const double *)nullptr))) return st;
   HistFile hf{};
   // HistoryFile::setSaveTime (HistoryFile.C:131-148): next starts at start
   hf.start = start_time;

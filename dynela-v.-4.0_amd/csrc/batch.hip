@@ -31,10 +31,11 @@ struct Batch {
   TaskList node[2][2];  // [dim==3?][mixed?]
   TaskList step;        // cycle control + field0 BCs, over the BC node lists
   TaskList close;       // end of a launch sequence, one block per job
-  // history files: items (256 per block), kinetic-energy node blocks, one
-  // block per job with history or result files (commit); hist_cap: the
-  // smallest ring (rows) of the batch's files
-  TaskList hist_items, hist_ke, hist_jobs;
+  // history files: evaluation (item ranges and kinetic-energy node ranges,
+  // hist_blocks() of each per job), one block per job with history or
+  // result files (commit); hist_cap: the smallest ring (rows) of the
+  // batch's files
+  TaskList hist_eval, hist_jobs;
   int hist_cap = 0;
   bool has_courant = false, has_pi = false;  // time-step methods present
   int *d_pend = nullptr;                     // power-iteration pending flags
@@ -161,25 +162,25 @@ int build_batch(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, Batch &B) {
       if ((st = upload_tasks(B, make_tasks(idx, cnt, kNodeBlock), B.node[d3][mx]))) return st;
     }
   {
-    std::vector<int> hj, kj;
-    std::vector<int64_t> hn, kn;
-    std::vector<Task> per_job;
+    // ranges of about equal size, hist_blocks(count) of them
+    auto ranges = [](std::vector<Task> &t, int job, int64_t count, bool ke) {
+      const int nb = hist_blocks(count);
+      const int64_t per = (count + nb - 1) / nb;
+      for (int b = 0; b < nb; b++) {
+        const int64_t f = b * per, e = std::min(count, f + per);
+        if (f < e || (ke && f == 0)) t.push_back(Task{job, ke ? -1 - b : b, f, std::max(f, e)});
+      }
+    };
+    std::vector<Task> eval, per_job;
     for (int i = 0; i < n; i++) {
       if (jobs[i]->hist.empty() && !jobs[i]->host.res.on) continue;
       per_job.push_back(Task{i, 0, 0, 0});
       if (jobs[i]->hist.empty()) continue;
-      hj.push_back(i);
-      hn.push_back((int64_t)jobs[i]->hist_items.size());
-      if (!jobs[i]->hist_ke_cols.empty()) {
-        kj.push_back(i);
-        kn.push_back(jobs[i]->N);
-      }
+      ranges(eval, i, (int64_t)jobs[i]->hist_items.size(), false);
+      if (!jobs[i]->hist_ke_cols.empty()) ranges(eval, i, jobs[i]->N, true);
       for (const auto &h : jobs[i]->hist) B.hist_cap = B.hist_cap ? std::min(B.hist_cap, h.cap) : h.cap;
     }
-    if ((st = upload_tasks(B, make_tasks(hj, hn, 256), B.hist_items)) ||
-        (st = upload_tasks(B, make_tasks(kj, kn, kNodeBlock), B.hist_ke)) ||
-        (st = upload_tasks(B, per_job, B.hist_jobs)))
-      return st;
+    if ((st = upload_tasks(B, eval, B.hist_eval)) || (st = upload_tasks(B, per_job, B.hist_jobs))) return st;
   }
   (void)ctx;
   return 0;
@@ -253,8 +254,7 @@ int run_pi_host(dnlg_ctx *ctx, Batch &B, int cs) {
 }
 
 void launch_history(dnlg_ctx *ctx, Batch &B, int cs, int preloop) {
-  launch_history(B.d_jobs, B.hist_items.d, B.hist_items.n, B.hist_ke.d, B.hist_ke.n, B.hist_jobs.d, B.hist_jobs.n, cs,
-                 preloop, ctx->stream);
+  launch_history(B.d_jobs, B.hist_eval.d, B.hist_eval.n, B.hist_jobs.d, B.hist_jobs.n, cs, preloop, ctx->stream);
 }
 
 void launch_post(dnlg_ctx *ctx, Batch &B, int cs, bool courant, bool fold = false) {

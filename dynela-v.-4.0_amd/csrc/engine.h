@@ -122,6 +122,15 @@ enum HistIP : int32_t {
 };
 constexpr int kHNorm = 6;  // comp of a tensor item: its norm (vectors: 3)
 
+// Blocks per job of the history evaluation launch: items and kinetic-energy
+// node ranges are split over at most this many blocks each (grid-stride
+// loops inside), so a cycle that takes no row costs a small launch.
+constexpr int kHistMaxBlocks = 256;
+__host__ __device__ inline int hist_blocks(int64_t n) {
+  const int64_t b = (n + 255) / 256;
+  return (int)(b < 1 ? 1 : (b > kHistMaxBlocks ? kHistMaxBlocks : b));
+}
+
 struct HistItem {
   int32_t file, col;  // owning file and column in its row (1 + position)
   int32_t op, src;    // HistOp, HistNodal / HistIP
@@ -186,7 +195,7 @@ struct JobDev {
   const HistItem *hi;
   const int2 *hinc;       // (element, local node) pairs of node-from-IP items
   const int32_t *hke_col;  // kinetic-energy columns, per file (HistFile::ke0)
-  double *hke;            // kinetic-energy partial sums, one per node block
+  double *hke;            // kinetic-energy partial sums, hist_blocks(N) of them
   int32_t n_hf, n_hi;
   int32_t density_always;  // IP densities every cycle (a history item reads them)
   // node -> (element, local node) lists in element order (Node::elements),

@@ -4,12 +4,12 @@
 // time update and before the field swap (Explicit.C:539-552 calling
 // Model::writeHistoryFiles, Model.C:917-924), and once after the pre-loop
 // pass (Explicit.C:182).  Here, for the jobs of a batch that have history
-// files, three launches follow the corrector of every cycle (and the
+// files, two launches follow the corrector of every cycle (and the
 // pre-loop pass):
-//   k_hist_items   one lane per item: the value, into the file's ring row,
-//                  when its file takes a row at t_{n+1}
-//   k_hist_ke      kinetic-energy partial sums, one per node block, when a
-//                  file with a kineticEnergy item takes a row
+//   k_hist_eval    the item values, into the rows of the files that take a
+//                  row at t_{n+1}, and kinetic-energy partial sums when such
+//                  a file has a kineticEnergy item (at most kHistMaxBlocks
+//                  blocks per job for each)
 //   k_hist_commit  one block per job: time column, kinetic energy, next save
 //                  time and row count of the files that take a row; the
 //                  result-file (VTK) schedule, which holds the job when a
@@ -211,47 +211,45 @@ __device__ __forceinline__ T block_sum(T v, T *lds) {
 
 }  // namespace
 
-__global__ __launch_bounds__(256) void k_hist_items(JobDev *const *jobs, const Task *tasks, int cs, int preloop) {
+// Items (task blk >= 0: a range of the job's items) and kinetic-energy
+// partial sums (blk < 0: node range -blk - 1 of the job's hist_blocks(N)),
+// both as grid-stride loops over the task's range.  Model::
+// getTotalKineticEnergy (Model.C:612-625) on the cycle's field0 speeds:
+// m (v . v) / 2 per node.
+__global__ __launch_bounds__(256) void k_hist_eval(JobDev *const *jobs, const Task *tasks, int cs, int preloop) {
+  __shared__ double lds[256 / 64];
   const Task tk = load_task(tasks);
   CJobDev *C = (CJobDev *)tk.jd;
   double t, dt;
   int f0;
   if (!hist_moment(C, cs, preloop, t, dt, f0)) return;
-  const int64_t k = tk.first + threadIdx.x;
-  if (k >= tk.end) return;
-  const HistItem it = C->hi[k];
-  if (it.op == kHKe) return;
-  const DNLG_GLOBAL HistFile &f = gbl(C->hf)[it.file];
-  if (!hist_due(f, t)) return;
-  const double v = item_value(C, it, f0, dt);
-  gbl(f.ring)[(f.rows % f.cap) * (1 + f.n_items) + it.col] = v;
-}
-
-// Model::getTotalKineticEnergy (Model.C:612-625) on the cycle's field0
-// speeds: m (v . v) / 2 per node, one partial sum per node block.
-__global__ __launch_bounds__(kNodeBlock) void k_hist_ke(JobDev *const *jobs, const Task *tasks, int cs, int preloop) {
-  __shared__ double lds[kNodeBlock / 64];
-  const Task tk = load_task(tasks);
-  CJobDev *C = (CJobDev *)tk.jd;
-  double t, dt;
-  int f0;
-  if (!hist_moment(C, cs, preloop, t, dt, f0)) return;
+  if (tk.blk >= 0) {
+    for (int64_t k = tk.first + threadIdx.x; k < tk.end; k += blockDim.x) {
+      const HistItem it = C->hi[k];
+      if (it.op == kHKe) continue;
+      const DNLG_GLOBAL HistFile &f = gbl(C->hf)[it.file];
+      if (!hist_due(f, t)) continue;
+      const double v = item_value(C, it, f0, dt);
+      gbl(f.ring)[(f.rows % f.cap) * (1 + f.n_items) + it.col] = v;
+    }
+    return;
+  }
   bool any = false;
   for (int i = 0; i < C->n_hf; i++) {
     const DNLG_GLOBAL HistFile &f = gbl(C->hf)[i];
     any = any || (f.n_ke > 0 && hist_due(f, t));
   }
   if (!any) return;
-  const int64_t n = tk.first + threadIdx.x, N = C->N;
+  const int64_t N = C->N;
+  const DNLG_GLOBAL double *v = gbl(C->fv[f0]);
   double e = 0.0;
-  if (n < tk.end) {
-    const DNLG_GLOBAL double *v = gbl(C->fv[f0]);
+  for (int64_t n = tk.first + threadIdx.x; n < tk.end; n += blockDim.x) {
     double v2 = 0.0;
     for (int d = 0; d < C->dim; d++) v2 += sqd(v[d * N + n]);
-    e = (gbl(C->mass)[n] * v2) / 2.0;
+    e += (gbl(C->mass)[n] * v2) / 2.0;
   }
   const double s = block_sum(e, lds);
-  if (threadIdx.x == 0) gbl(C->hke)[tk.blk] = s;
+  if (threadIdx.x == 0) gbl(C->hke)[-tk.blk - 1] = s;
 }
 
 __global__ __launch_bounds__(64) void k_hist_commit(JobDev *const *jobs, const Task *tasks, int cs, int preloop) {
@@ -270,7 +268,7 @@ __global__ __launch_bounds__(64) void k_hist_commit(JobDev *const *jobs, const T
     }
     double ke = 0.0;
     if (any) {
-      const int nb = (int)((C->N + kNodeBlock - 1) / kNodeBlock);
+      const int nb = hist_blocks(C->N);
       for (int b = 0; b < nb; b++) ke += gbl(C->hke)[b];
     }
     s_ke = ke;
@@ -342,11 +340,10 @@ void launch_node_field(JobDev *job, int64_t N, int src, int comp, double *out, h
   hipLaunchKernelGGL(k_node_field, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, job, src, comp, out);
 }
 
-void launch_history(JobDev *const *jobs, const Task *items, int n_items, const Task *ke, int n_ke, const Task *per_job,
-                    int n_jobs, int cs, int preloop, hipStream_t s) {
+void launch_history(JobDev *const *jobs, const Task *eval, int n_eval, const Task *per_job, int n_jobs, int cs,
+                    int preloop, hipStream_t s) {
   if (n_jobs <= 0) return;
-  if (n_items > 0) hipLaunchKernelGGL(k_hist_items, dim3(n_items), dim3(256), 0, s, jobs, items, cs, preloop);
-  if (n_ke > 0) hipLaunchKernelGGL(k_hist_ke, dim3(n_ke), dim3(kNodeBlock), 0, s, jobs, ke, cs, preloop);
+  if (n_eval > 0) hipLaunchKernelGGL(k_hist_eval, dim3(n_eval), dim3(256), 0, s, jobs, eval, cs, preloop);
   hipLaunchKernelGGL(k_hist_commit, dim3(n_jobs), dim3(64), 0, s, jobs, per_job, cs, preloop);
 }
 

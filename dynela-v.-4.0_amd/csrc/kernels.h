@@ -49,13 +49,13 @@ void launch_pi_finish(JobDev *const *jobs, int n, int preloop, hipStream_t s);
 void launch_pi_pending(JobDev *const *jobs, int n, int *out, hipStream_t s);
 // history files: the IP-to-node extrapolation tables ([node][ip]) and their
 // upload; the launches of one row-taking moment (preloop != 0: the pre-loop
-// pass, slot 0; otherwise the end of the cycle in slot cs): items (256 per
-// block), kinetic-energy partials (node blocks of the jobs with a
-// kineticEnergy item), commit (one block per job with history files)
+// pass, slot 0; otherwise the end of the cycle in slot cs): evaluation (item
+// ranges, blk >= 0, and kinetic-energy node ranges, blk = -1 - partial), then
+// commit (one block per job with history or result files)
 void extrapolation_tables(double hexE[64], double quaE[16]);
 void upload_extrapolation();
-void launch_history(JobDev *const *jobs, const Task *items, int n_items, const Task *ke, int n_ke, const Task *per_job,
-                    int n_jobs, int cs, int preloop, hipStream_t s);
+void launch_history(JobDev *const *jobs, const Task *eval, int n_eval, const Task *per_job, int n_jobs, int cs,
+                    int preloop, hipStream_t s);
 // Node::fieldScalar of an IP field at every node (src, comp as HistItem;
 // comp < 0 with a tensor source: all six components, out [6][N]); needs the
 // job's nf_ptr / nf_pairs lists

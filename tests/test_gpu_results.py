@@ -116,6 +116,11 @@ def test_result_files(ctx, tmp_path, monkeypatch):
         r = rows[np.argmin(np.abs(rows[:, 0] - v["time"]))]
         assert abs(r[0] - v["time"]) <= 1e-5 * v["time"]
         cols = r[1:].reshape(6, 150)
+        # the same doubles print as C++ streams print them ("%g")
+        with open(tmp_path / name) as fh:
+            text = fh.read().splitlines()
+        assert text[6] == "%g" % r[0]
+        assert text[8:158] == ["%g %g %g" % (x, y, 0.0) for x, y in zip(cols[0], cols[1])]
         for got, want in ((v["points"][:, 0], cols[0]), (v["points"][:, 1], cols[1]),
                           (v["fields"]["speed"][1][:, 1], cols[2]), (v["fields"]["vonMises"][1][:, 0], cols[3]),
                           (v["fields"]["plasticStrain"][1][:, 0], cols[4]), (v["fields"]["Stress"][1][:, 4], cols[5])):
