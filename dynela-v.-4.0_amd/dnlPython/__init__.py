@@ -17,6 +17,7 @@ Errors never exit the interpreter: engine statuses become exceptions
 ``PowerIterationError``, ``DnlgError``).
 """
 import ctypes
+import warnings
 
 import numpy as np
 
@@ -25,7 +26,7 @@ from ._lib import (DnlgError, NegativeJacobianError, NewtonRaphsonError, PowerIt
                    NonFiniteError)
 
 __all__ = ["DynELA", "NodeSet", "ElementSet", "Material", "JohnsonCookLaw", "ElastoplasticLaw", "HistoryFile",
-           "VtkInterface",
+           "VtkInterface", "SvgInterface",
            "VoceLinearLaw", "BoundaryRestrain", "BoundarySpeed", "RampFunction", "Explicit", "Field",
            "Element", "Vec3D", "Context", "DnlgError", "NegativeJacobianError", "NewtonRaphsonError",
            "PowerIterationError", "NonFiniteError", "batch_solve"]
@@ -416,6 +417,24 @@ class VtkInterface:
 
     def getNumberOfFields(self):
         return len(self._fields)
+
+
+class SvgInterface:
+    """SvgInterface (SvgInterface.h): SVG contour plots of a field.  Not
+    provided: the view and legend settings are accepted so that a sample
+    script runs its solve, and write() warns that no SVG file was written."""
+
+    def __init__(self, name=""):
+        self.name = name
+
+    def __getattr__(self, attr):
+        if attr.startswith(("set", "rotate", "reset")):
+            return lambda *args, **kwargs: None
+        raise AttributeError(attr)
+
+    def write(self, fileName, field=None):
+        warnings.warn(f"SvgInterface: SVG output is not provided, {fileName} was not written", RuntimeWarning,
+                      stacklevel=2)
 
 
 class HistoryFile:

@@ -272,3 +272,16 @@ def test_node_field_read_back_pins():
     np.testing.assert_array_equal(o.node_field(F.mass), o.mass())
     np.testing.assert_allclose(o.node_field(F.disp), np.linalg.norm(d, axis=0), rtol=1e-15)
     assert np.all(o.node_field(F.inelasticEnergy) == 0.0) and np.all(o.node_field(F.timeStep) == 0.0)
+
+
+def test_svg_interface_is_accepted_but_writes_nothing():
+    """SvgInterface (not provided): the samples' view settings are accepted,
+    write() warns instead of writing."""
+    svg = dnl.SvgInterface("SVG")
+    svg.setTitleDisplay(False)
+    svg.setLegendPosition(350, 150)
+    svg.rotate(dnl.Vec3D(1, 0, 0), -90)
+    with pytest.warns(RuntimeWarning, match="not provided"):
+        svg.write("temperature.svg", F.T)
+    with pytest.raises(AttributeError):
+        svg.drawSomething()
