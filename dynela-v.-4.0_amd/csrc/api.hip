@@ -475,7 +475,7 @@ int dnlg_ctx_destroy(dnlg_ctx *ctx) {
 const char *dnlg_last_error(const dnlg_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 int64_t dnlg_footprint_bytes(int elem_type, int64_t n_nodes, int64_t n_elems, int precision, int track_tensors) {
-  if (elem_type != DNLG_EL_HEX8N_3D && elem_type != DNLG_EL_QUA4N_AX) return -1;
+  if (elem_type != DNLG_EL_HEX8N_3D && elem_type != DNLG_EL_QUA4N_AX && elem_type != DNLG_EL_QUA4N_2D) return -1;
   return footprint(elem_type, n_nodes, n_elems, precision, track_tensors);
 }
 
@@ -483,7 +483,8 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
                     const dnlg_solver_params *prm, dnlg_job **out) {
   if (!ctx || !mesh || !mat || !prm || !out) return fail(ctx, DNLG_E_INVALID, "null argument");
   *out = nullptr;
-  if (mesh->elem_type != DNLG_EL_QUA4N_AX && mesh->elem_type != DNLG_EL_HEX8N_3D)
+  if (mesh->elem_type != DNLG_EL_QUA4N_AX && mesh->elem_type != DNLG_EL_HEX8N_3D &&
+      mesh->elem_type != DNLG_EL_QUA4N_2D)
     return fail(ctx, DNLG_E_INVALID, "unsupported element type");
   if (mesh->n_nodes <= 0 || mesh->n_elems <= 0 || !mesh->coords || !mesh->conn)
     return fail(ctx, DNLG_E_INVALID, "empty mesh");
@@ -536,11 +537,12 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
     for (int64_t e = 0; e < NE; e++)
       for (int a = 0; a < nn; a++) e2slot[e * nn + a] = fill[conn[e * nn + a]]++;
   }
-  // coordinates (ElementAxi::checkLevel2, ElementAxi.C:103-126)
+  // coordinates (ElementAxi::checkLevel2, ElementAxi.C:103-126; the plane
+  // element checks nothing, Element2D.C)
   std::vector<double> x(dim * N);
   for (int d = 0; d < dim; d++)
     for (int64_t n = 0; n < N; n++) x[d * N + n] = mesh->coords[d * N + n];
-  if (dim == 2)
+  if (type == DNLG_EL_QUA4N_AX)
     for (int64_t n = 0; n < N; n++)
       if (x[n] < 0.0) {
         if (x[n] > -1e-8)
@@ -913,7 +915,7 @@ int dnlg_job_energies(dnlg_job *job, double out[4]) {
   double ke = 0.0;
   for (int64_t n = 0; n < N; n++) ke += m[n] * (sqd(v[n]) + sqd(v[N + n]) + sqd(v[2 * N + n])) / 2.0;
   double ie = 0.0, wp = 0.0;
-  const double scale = job->dim == 2 ? k2PI : 1.0;
+  const double scale = job->type == DNLG_EL_QUA4N_AX ? k2PI : 1.0;
   for (int64_t g = 0; g < G; g++) {
     const double vol = dj0[g] * scale;
     ie += job->host.rho0 * vol * eint[g];

@@ -22,12 +22,18 @@ struct TaskList {
   int n = 0;
 };
 
+// Element types, indexed as the per-type task lists below.
+constexpr int kNumTypes = 3;
+constexpr int kTypes[kNumTypes] = {DNLG_EL_QUA4N_AX, DNLG_EL_HEX8N_3D, DNLG_EL_QUA4N_2D};
+inline int type_dim(int type) { return type == DNLG_EL_HEX8N_3D ? 3 : 2; }
+inline int type_nodes(int type) { return type == DNLG_EL_HEX8N_3D ? 8 : 4; }
+
 struct Batch {
   std::vector<dnlg_job *> jobs;
   JobDev **d_jobs = nullptr;
-  TaskList elem[2][2];  // [hex?][mixed?]
-  TaskList prep[2];     // [hex?]
-  TaskList cour[2];     // [hex?]
+  TaskList elem[kNumTypes][2];  // [type][mixed?]
+  TaskList prep[kNumTypes];     // [type]
+  TaskList cour[kNumTypes];     // [type]: Courant lengths and power-iteration products
   TaskList node[2][2];  // [dim==3?][mixed?]
   TaskList step;        // cycle control + field0 BCs, over the BC node lists
   TaskList close;       // end of a launch sequence, one block per job
@@ -113,9 +119,9 @@ int build_batch(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, Batch &B) {
   B.d_pend = (int *)p;
   B.h_pend.assign(n, 0);
   int st;
-  for (int hx = 0; hx < 2; hx++) {
-    const int type = hx ? DNLG_EL_HEX8N_3D : DNLG_EL_QUA4N_AX;
-    const int nn = hx ? 8 : 4;
+  for (int hx = 0; hx < kNumTypes; hx++) {
+    const int type = kTypes[hx];
+    const int nn = type_nodes(type);
     for (int mx = 0; mx < 2; mx++) {
       std::vector<int> idx;
       std::vector<int64_t> cnt;
@@ -200,13 +206,13 @@ void launch_close(dnlg_ctx *ctx, Batch &B, int cs) {
   launch_cycle_step(B.d_jobs, B.close.d, B.close.n, cs, 0, 0, ctx->stream);
 }
 
-// ev: nullptr, or two events per element launch (up to 4 launches); each
+// ev: nullptr, or two events per element launch (up to 2 per element type); each
 // launch is bracketed by event records on the engine's stream.
 int launch_element(dnlg_ctx *ctx, Batch &B, const hipEvent_t *ev, int cs, int derive) {
   hipStream_t s = ctx->stream;
   int k = 0;
-  for (int hx = 0; hx < 2; hx++) {
-    const int type = hx ? DNLG_EL_HEX8N_3D : DNLG_EL_QUA4N_AX;
+  for (int hx = 0; hx < kNumTypes; hx++) {
+    const int type = kTypes[hx];
     for (int mx = 0; mx < 2; mx++) {
       const TaskList &L = B.elem[hx][mx];
       if (L.n <= 0) continue;
@@ -228,8 +234,7 @@ int launch_element(dnlg_ctx *ctx, Batch &B, const hipEvent_t *ev, int cs, int de
 // One power iteration for every job of the batch that is iterating.
 void launch_pi_iteration(dnlg_ctx *ctx, Batch &B, int cs) {
   hipStream_t s = ctx->stream;
-  for (int hx = 0; hx < 2; hx++)
-    launch_pi_elem(hx ? DNLG_EL_HEX8N_3D : DNLG_EL_QUA4N_AX, B.d_jobs, B.cour[hx].d, B.cour[hx].n, s);
+  for (int hx = 0; hx < kNumTypes; hx++) launch_pi_elem(kTypes[hx], B.d_jobs, B.cour[hx].d, B.cour[hx].n, s);
   for (int d3 = 0; d3 < 2; d3++)
     for (int mx = 0; mx < 2; mx++) launch_pi_node(d3 ? 3 : 2, B.d_jobs, B.node[d3][mx].d, B.node[d3][mx].n, s);
   launch_pi_step(B.d_jobs, B.close.d, B.close.n, cs, s);
@@ -265,7 +270,8 @@ void launch_post(dnlg_ctx *ctx, Batch &B, int cs, bool courant, bool fold = fals
   // Explicit::endStep's history rows, before the time-step computation
   launch_history(ctx, B, cs, 0);
   if (courant && B.has_courant)
-    for (int hx = 0; hx < 2; hx++) launch_courant(hx ? 3 : 2, B.d_jobs, B.cour[hx].d, B.cour[hx].n, cs, 0, s);
+    for (int hx = 0; hx < kNumTypes; hx++)
+      launch_courant(type_dim(kTypes[hx]), B.d_jobs, B.cour[hx].d, B.cour[hx].n, cs, 0, s);
   if (courant && B.has_pi) {
     // a few warm-started iterations in the sequence; a job that has not
     // converged stalls at the next k_cycle_step and the host finishes it
@@ -306,11 +312,11 @@ int launch_cycles(dnlg_ctx *ctx, Batch &B, int j0, int j1, int phase, int freq, 
 int run_preloop(dnlg_ctx *ctx, Batch &B) {
   hipStream_t s = ctx->stream;
   const int n = (int)B.jobs.size();
-  for (int hx = 0; hx < 2; hx++)
-    launch_prep(hx ? DNLG_EL_HEX8N_3D : DNLG_EL_QUA4N_AX, B.d_jobs, B.prep[hx].d, B.prep[hx].n, s);
+  for (int hx = 0; hx < kNumTypes; hx++) launch_prep(kTypes[hx], B.d_jobs, B.prep[hx].d, B.prep[hx].n, s);
   for (int d3 = 0; d3 < 2; d3++)
     for (int mx = 0; mx < 2; mx++) launch_mass_gather(d3 ? 3 : 2, B.d_jobs, B.node[d3][mx].d, B.node[d3][mx].n, s);
-  for (int hx = 0; hx < 2; hx++) launch_courant(hx ? 3 : 2, B.d_jobs, B.cour[hx].d, B.cour[hx].n, 0, 1, s);
+  for (int hx = 0; hx < kNumTypes; hx++)
+    launch_courant(type_dim(kTypes[hx]), B.d_jobs, B.cour[hx].d, B.cour[hx].n, 0, 1, s);
   launch_force_dt(B.d_jobs, n, s);
   if (B.has_pi) {
     // cold start of the power iteration (Model.C:966-970) on the reference
@@ -320,8 +326,8 @@ int run_preloop(dnlg_ctx *ctx, Batch &B) {
     if (st) return st;
     launch_pi_finish(B.d_jobs, n, 1, s);
   }
-  for (int hx = 0; hx < 2; hx++) {
-    const int type = hx ? DNLG_EL_HEX8N_3D : DNLG_EL_QUA4N_AX;
+  for (int hx = 0; hx < kNumTypes; hx++) {
+    const int type = kTypes[hx];
     launch_elem_fp64(type, B.d_jobs, B.elem[hx][0].d, B.elem[hx][0].n, 0, 0, s);
     launch_elem_mixed(type, B.d_jobs, B.elem[hx][1].d, B.elem[hx][1].n, 0, 0, s);
   }
@@ -559,7 +565,7 @@ int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vec
       const int per = use_graph ? graph_len(ctx->graph_cycles, phase, freq) : 1;
       const int units = std::min((chunk + per - 1) / per, kTimedChunk);
       launched += (int64_t)units * per - chunk;
-      while (ctx->events.size() < (size_t)8 * units) {
+      while (ctx->events.size() < (size_t)4 * kNumTypes * units) {
         hipEvent_t e;
         if (hipEventCreate(&e) != hipSuccess) return DNLG_E_HIP;
         ctx->events.push_back(e);

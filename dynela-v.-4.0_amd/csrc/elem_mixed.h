@@ -168,7 +168,7 @@ struct LaneRows {
   __device__ static int at(int le, int lane, int v) { return le * kGroup + lane * R + v; }
 };
 
-template <int NN, int DIM, int BLOCK>
+template <int NN, int DIM, int BLOCK, bool AXI>
 __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const Task *tasks, int ctl, int derive) {
   constexpr int NB = DIM;               // lanes per element = 2^DIM = NN
   constexpr float inv2D = 1.f / NN;     // 2^-D
@@ -203,7 +203,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
       xl[d] = (float)(gbl(C->x)[d * N + nd] - gbl(C->x)[d * N + n0]);
       ul[d] = (float)u0[d * N + nd];
     }
-    if constexpr (DIM == 2) x0r = gbl(C->x)[n0];
+    if constexpr (AXI) x0r = gbl(C->x)[n0];
   } else {
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
       H[3 * i + j] = h;
     }
   float radius = 1.f;
-  if constexpr (DIM == 2) {
+  if constexpr (AXI) {
     const float rl = mx[0][0] + mx[1][0] * xi[0] + mx[2][0] * xi[1] + mx[3][0] * (xi[0] * xi[1]);
     const float ur = mu[0][0] + mu[1][0] * xi[0] + mu[2][0] * xi[1] + mu[3][0] * (xi[0] * xi[1]);
     radius = (float)(x0r + (double)rl);
@@ -475,13 +475,13 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
     // Element::computeDensity (Element.C:1243-1256)
     if (cc.density_due) {
       double rr = C->rho0 * (double)gbl(C->detJ0f)[gi] / (double)detJ;
-      if constexpr (DIM == 2) rr /= (double)radius;
+      if constexpr (AXI) rr /= (double)radius;
       gbl(C->rho)[gi] = rr;
     }
   }
   // ---- internal forces (Element.C:362-404): T = w J^-1 sigma per Gauss point
   float w = detJ;
-  if constexpr (DIM == 2) w *= (float)k2PI * radius;
+  if constexpr (AXI) w *= (float)k2PI * radius;
 #pragma unroll
   for (int k = 0; k < DIM; k++)
 #pragma unroll
@@ -494,7 +494,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
       }
       T9[k][i] = valid ? w * t : 0.f;
     }
-  if constexpr (DIM == 2) hoop = valid ? w * sig[5] * __builtin_amdgcn_rcpf(radius) : 0.f;
+  if constexpr (AXI) hoop = valid ? w * sig[5] * __builtin_amdgcn_rcpf(radius) : 0.f;
   // Walsh transform of T over the Gauss points, then node lane b combines
   // four (two) of them per direction, read from the element's lanes in LDS
   float WT[DIM][DIM];
@@ -510,7 +510,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
     for (int k = 0; k < 3; k++) s_wt[ForceRows::at(le, b, k)] = make_float4(WT[k][0], WT[k][1], WT[k][2], 0.f);
   } else {
     s_wt[ForceRows::at(le, b, 0)] = make_float4(WT[0][0], WT[0][1], WT[1][0], WT[1][1]);
-    s_wt[ForceRows::at(le, b, 1)] = make_float4(wht<NB>(hoop, cs), 0.f, 0.f, 0.f);
+    if constexpr (AXI) s_wt[ForceRows::at(le, b, 1)] = make_float4(wht<NB>(hoop, cs), 0.f, 0.f, 0.f);
   }
   __builtin_amdgcn_wave_barrier();
   float f[DIM];
@@ -543,7 +543,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
       }
     }
   }
-  if constexpr (DIM == 2) {
+  if constexpr (AXI) {
     // hoop term: f_r -= sum_p N_a(p) w_p sigma_tt / r_p
     const float h0 = s_wt[ForceRows::at(le, 0, 1)].x, h1 = s_wt[ForceRows::at(le, 1, 1)].x;
     const float h2 = s_wt[ForceRows::at(le, 2, 1)].x, h3 = s_wt[ForceRows::at(le, 3, 1)].x;

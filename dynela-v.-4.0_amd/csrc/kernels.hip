@@ -178,7 +178,7 @@ __device__ __forceinline__ Control cycle_control(JobDev *J, CJobDev *C, int cs, 
 // ---------------------------------------------------------------------------
 // Three waves per SIMD (168 VGPRs, a short spill); the LDS of a 16-hex block
 // (24 KB) allows three.  Measured 11 % faster than two waves without spill.
-template <int NN, int DIM, int BLOCK>
+template <int NN, int DIM, int BLOCK, bool AXI>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void k_elem_fp64(
     JobDev *const *jobs, const Task *tasks, int cs, int derive) {
   constexpr int NIP = NN;
@@ -278,7 +278,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
       for (int i = 0; i < DIM; i++)
 #pragma unroll
         for (int j = 0; j < DIM; j++) F[3 * i + j] += dN[a][j] * s_u[le][a][i];
-    if constexpr (DIM == 2) {
+    if constexpr (AXI) {
       double r = 0.0, vr = 0.0;
 #pragma unroll
       for (int a = 0; a < NN; a++) {
@@ -424,11 +424,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
     // Element::computeDensity (Element.C:1243-1256), on x_n
     if (cc.density_due) {
       double rr = C->rho0 * gbl(C->detJ0)[g] / detJ;
-      if constexpr (DIM == 2) rr /= radius;
+      if constexpr (AXI) rr /= radius;
       gbl(C->rho)[g] = rr;
     }
     wdj = detJ;  // Gauss weights are 1 (ElQua4NAx.C:63-110, ElHex8N3D.C:93-231)
-    if constexpr (DIM == 2) wdj *= k2PI * radius;
+    if constexpr (AXI) wdj *= k2PI * radius;
 #pragma unroll
     for (int c = 0; c < 6; c++) s_sw[le][p][c] = sig[c];
     s_sw[le][p][6] = wdj;
@@ -458,7 +458,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
       for (int i = 0; i < DIM; i++)
 #pragma unroll
         for (int j = 0; j < DIM; j++) f[i] -= dNq[j] * s_sw[le][q][sym_idx(j, i)] * w;
-      if constexpr (DIM == 2) f[0] -= tabN<NN>(q, a) * s_sw[le][q][5] / s_sw[le][q][7] * w;
+      if constexpr (AXI) f[0] -= tabN<NN>(q, a) * s_sw[le][q][5] / s_sw[le][q][7] * w;
     }
     DNLG_GLOBAL double *out = gbl(C->elemF) + (int64_t)gbl(C->e2slot)[e * NN + a] * DIM;
 #pragma unroll
@@ -855,7 +855,7 @@ __global__ void k_pi_begin(JobDev *const *jobs, int n_jobs, int cs, int force) {
 // hoop row), and B^T C B v w detJ (x 2 pi r), on the Jacobian of the current
 // coordinates (Explicit.C:245-249 computes it before the time step).  Equal
 // to the reference's assembled K_e times v up to rounding.
-template <int NN, int DIM>
+template <int NN, int DIM, bool AXI>
 __global__ __launch_bounds__(256) void k_pi_elem(JobDev *const *jobs, const Task *tasks) {
   const Task tk = load_task(tasks);
   JobDev *J = tk.jd;
@@ -945,6 +945,22 @@ __global__ __launch_bounds__(256) void k_pi_elem(JobDev *const *jobs, const Task
         f[a][0] += (dN[a][0] * sxx + dN[a][1] * sxy + dN[a][2] * sxz) * w;
         f[a][1] += (dN[a][1] * syy + dN[a][0] * sxy + dN[a][2] * syz) * w;
         f[a][2] += (dN[a][2] * szz + dN[a][1] * syz + dN[a][0] * sxz) * w;
+      }
+    } else if constexpr (!AXI) {
+      // plane strain (Material.C:104-111): the leading 3 x 3 block, unit thickness
+      double exx = 0, eyy = 0, gxy = 0;
+#pragma unroll
+      for (int a = 0; a < NN; a++) {
+        exx += dN[a][0] * v[a][0];
+        eyy += dN[a][1] * v[a][1];
+        gxy += dN[a][1] * v[a][0] + dN[a][0] * v[a][1];
+      }
+      const double sxx = c11 * exx + c12 * eyy, syy = c11 * eyy + c12 * exx, sxy = c44 * gxy;
+      const double w = det;
+#pragma unroll
+      for (int a = 0; a < NN; a++) {
+        f[a][0] += (dN[a][0] * sxx + dN[a][1] * sxy) * w;
+        f[a][1] += (dN[a][1] * syy + dN[a][0] * sxy) * w;
       }
     } else {
       double r = 0.0;
@@ -1127,7 +1143,7 @@ __global__ void k_force_dt(JobDev *const *jobs, int n_jobs) {
 // ElementPlane.C:92-100) and the element lumped masses
 // (Element::computeMassMatrix, Element.C:304-339) into elemF's first slot.
 // ---------------------------------------------------------------------------
-template <int NN, int DIM>
+template <int NN, int DIM, bool AXI>
 __global__ __launch_bounds__(256) void k_prep(JobDev *const *jobs, const Task *tasks) {
   const Task tk = load_task(tasks);
   JobDev *J = tk.jd;
@@ -1168,7 +1184,7 @@ __global__ __launch_bounds__(256) void k_prep(JobDev *const *jobs, const Task *t
     neg = neg || (det < 0.0);
     double wdj = det;
     double det0 = det;
-    if constexpr (DIM == 2) {
+    if constexpr (AXI) {
       double r = 0.0;
 #pragma unroll
       for (int a = 0; a < NN; a++) r += tabN<NN>(p, a) * X[a][0];
@@ -1220,38 +1236,37 @@ void upload_tables(const double *hexN, const double *hexD, const double *quaN, c
 int elem_block_threads(int type) { return type == DNLG_EL_HEX8N_3D ? 128 : 256; }
 int elem_block_threads_mixed(int type) { return type == DNLG_EL_HEX8N_3D ? 128 : 256; }
 
+// One launch of an element kernel template: with events (hipExtLaunchKernelGGL
+// records start / stop on the stream) or without.
+template <typename K>
+void launch_elem(K kern, int block, int nblk, hipStream_t s, const hipEvent_t *ev, JobDev *const *jobs,
+                 const Task *tasks, int cs, int derive) {
+  if (ev)
+    hipExtLaunchKernelGGL(kern, dim3(nblk), dim3(block), 0, s, ev[0], ev[1], 0, jobs, tasks, cs, derive);
+  else
+    hipLaunchKernelGGL(kern, dim3(nblk), dim3(block), 0, s, jobs, tasks, cs, derive);
+}
+
 void launch_elem_mixed(int type, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int derive,
                        hipStream_t s, const hipEvent_t *ev) {
   if (nblk <= 0) return;
-  hipEvent_t a = ev ? ev[0] : nullptr, b = ev ? ev[1] : nullptr;
-  if (type == DNLG_EL_HEX8N_3D) {
-    if (ev)
-      hipExtLaunchKernelGGL((k_elem_mixed<8, 3, 128>), dim3(nblk), dim3(128), 0, s, a, b, 0, jobs, tasks, cs, derive);
-    else
-      hipLaunchKernelGGL((k_elem_mixed<8, 3, 128>), dim3(nblk), dim3(128), 0, s, jobs, tasks, cs, derive);
-  } else {
-    if (ev)
-      hipExtLaunchKernelGGL((k_elem_mixed<4, 2, 256>), dim3(nblk), dim3(256), 0, s, a, b, 0, jobs, tasks, cs, derive);
-    else
-      hipLaunchKernelGGL((k_elem_mixed<4, 2, 256>), dim3(nblk), dim3(256), 0, s, jobs, tasks, cs, derive);
-  }
+  if (type == DNLG_EL_HEX8N_3D)
+    launch_elem(k_elem_mixed<8, 3, 128, false>, 128, nblk, s, ev, jobs, tasks, cs, derive);
+  else if (type == DNLG_EL_QUA4N_AX)
+    launch_elem(k_elem_mixed<4, 2, 256, true>, 256, nblk, s, ev, jobs, tasks, cs, derive);
+  else
+    launch_elem(k_elem_mixed<4, 2, 256, false>, 256, nblk, s, ev, jobs, tasks, cs, derive);
 }
 
 void launch_elem_fp64(int type, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int derive,
                       hipStream_t s, const hipEvent_t *ev) {
   if (nblk <= 0) return;
-  hipEvent_t a = ev ? ev[0] : nullptr, b = ev ? ev[1] : nullptr;
-  if (type == DNLG_EL_HEX8N_3D) {
-    if (ev)
-      hipExtLaunchKernelGGL((k_elem_fp64<8, 3, 128>), dim3(nblk), dim3(128), 0, s, a, b, 0, jobs, tasks, cs, derive);
-    else
-      hipLaunchKernelGGL((k_elem_fp64<8, 3, 128>), dim3(nblk), dim3(128), 0, s, jobs, tasks, cs, derive);
-  } else {
-    if (ev)
-      hipExtLaunchKernelGGL((k_elem_fp64<4, 2, 256>), dim3(nblk), dim3(256), 0, s, a, b, 0, jobs, tasks, cs, derive);
-    else
-      hipLaunchKernelGGL((k_elem_fp64<4, 2, 256>), dim3(nblk), dim3(256), 0, s, jobs, tasks, cs, derive);
-  }
+  if (type == DNLG_EL_HEX8N_3D)
+    launch_elem(k_elem_fp64<8, 3, 128, false>, 128, nblk, s, ev, jobs, tasks, cs, derive);
+  else if (type == DNLG_EL_QUA4N_AX)
+    launch_elem(k_elem_fp64<4, 2, 256, true>, 256, nblk, s, ev, jobs, tasks, cs, derive);
+  else
+    launch_elem(k_elem_fp64<4, 2, 256, false>, 256, nblk, s, ev, jobs, tasks, cs, derive);
 }
 
 void launch_correct(int dim, bool f32, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int fold,
@@ -1288,9 +1303,11 @@ void launch_pi_begin(JobDev *const *jobs, int n, int cs, int force, hipStream_t 
 void launch_pi_elem(int type, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s) {
   if (nblk <= 0) return;
   if (type == DNLG_EL_HEX8N_3D)
-    hipLaunchKernelGGL((k_pi_elem<8, 3>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
+    hipLaunchKernelGGL((k_pi_elem<8, 3, false>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
+  else if (type == DNLG_EL_QUA4N_AX)
+    hipLaunchKernelGGL((k_pi_elem<4, 2, true>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
   else
-    hipLaunchKernelGGL((k_pi_elem<4, 2>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
+    hipLaunchKernelGGL((k_pi_elem<4, 2, false>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
 }
 void launch_pi_node(int dim, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s) {
   if (nblk <= 0) return;
@@ -1321,9 +1338,11 @@ void launch_force_dt(JobDev *const *jobs, int n, hipStream_t s) {
 void launch_prep(int type, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s) {
   if (nblk <= 0) return;
   if (type == DNLG_EL_HEX8N_3D)
-    hipLaunchKernelGGL((k_prep<8, 3>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
+    hipLaunchKernelGGL((k_prep<8, 3, false>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
+  else if (type == DNLG_EL_QUA4N_AX)
+    hipLaunchKernelGGL((k_prep<4, 2, true>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
   else
-    hipLaunchKernelGGL((k_prep<4, 2>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
+    hipLaunchKernelGGL((k_prep<4, 2, false>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
 }
 void launch_mass_gather(int dim, JobDev *const *jobs, const Task *tasks, int nblk, hipStream_t s) {
   if (nblk <= 0) return;

@@ -43,7 +43,7 @@ class Element:
     ElTet10N3D = 6
 
 
-NODES_PER_ELEMENT = {Element.ElQua4NAx: 4, Element.ElHex8N3D: 8}
+NODES_PER_ELEMENT = {Element.ElQua4N2D: 4, Element.ElQua4NAx: 4, Element.ElHex8N3D: 8}
 
 
 def _field_names():

@@ -21,6 +21,7 @@ STATUS_NAMES = {
     E_NR_DIVERGED: "NEWTON_RAPHSON_DIVERGED", E_NONFINITE: "NON_FINITE", E_HIP: "HIP_ERROR",
     E_STATE: "INVALID_STATE", E_CAPACITY: "CAPACITY", E_PI_DIVERGED: "POWER_ITERATION_DIVERGED",
 }
+EL_QUA4N_2D = 1
 EL_QUA4N_AX = 3
 EL_HEX8N_3D = 4
 LAW_JOHNSON_COOK, LAW_POWER, LAW_VOCE_LINEAR = 0, 1, 2

@@ -13,6 +13,7 @@ Workloads
   necking_3d   3D round bar in tension with a 2 % mid-length taper,
                Voce + linear hardening, one end fixed, the other pulled
   necking_axi  the same bar with 4-node axisymmetric quads
+  necking_plane the same half-plane as a plane-strain plate (ElQua4N2D)
   sweep_params seeded parameter sweep around the copper impact
   serve_params seeded serving mix (axi / 3D x impact / necking)
 """
@@ -236,21 +237,24 @@ def _voce_material(p):
 
 
 def necking_axi(nr=10, nz=80, radius=6.413e-3, length=53.334e-3, taper=0.02, pull=1.0, t_end=50e-6,
-                material=None, precision="fp64", track_tensors=True, context=None, freq=10, dt_method=None):
+                material=None, precision="fp64", track_tensors=True, context=None, freq=10, dt_method=None,
+                plane=False):
     """Axisymmetric round bar in tension (4-node axisymmetric quads): the
     necking_3d geometry, loads and law on the (r, y) half-plane, nodes
-    restrained radially on the axis."""
+    restrained radially on the axis.  plane=True: the same half-plane as a
+    plane-strain plate of 4-node plane quads (Samples/PlateNecking), its
+    symmetry line restrained in x."""
     p = dict(NECKING)
     if material:
         p.update(material)
-    model = DynELA("BarNecking-Axi", context=context)
+    model = DynELA("PlateNecking" if plane else "BarNecking-Axi", context=context)
     ys = np.linspace(0.0, length, nz + 1)
     rad = radius * (1.0 - taper * 0.5 * (1.0 + np.cos(2.0 * np.pi * (ys / length - 0.5))))
     i, j = np.meshgrid(np.arange(nr + 1), np.arange(nz + 1))
     nums = (j * (nr + 1) + i + 1).ravel()
     xyz = np.stack([(i / nr * rad[j]).ravel(), ys[j].ravel(), np.zeros(nums.size)], axis=1)
     model.createNodes(nums, xyz)
-    model.setDefaultElement(Element.ElQua4NAx)
+    model.setDefaultElement(Element.ElQua4N2D if plane else Element.ElQua4NAx)
     ei, ej = np.meshgrid(np.arange(nr), np.arange(nz))
     n1 = ei + ej * (nr + 1) + 1
     conn = np.stack([n1, n1 + 1, ei + (ej + 1) * (nr + 1) + 2, ei + (ej + 1) * (nr + 1) + 1], axis=-1)
@@ -269,6 +273,11 @@ def necking_axi(nr=10, nz=80, radius=6.413e-3, length=53.334e-3, taper=0.02, pul
     model.attachConstantBC(top, _set(range(nz * (nr + 1) + 1, (nz + 1) * (nr + 1) + 1), "NS_Top"))
     model.add(_solver(t_end, precision, track_tensors, freq, dt_method))
     return model
+
+
+def necking_plane(nx=10, ny=80, **kw):
+    """Plane-strain plate in tension (ElQua4N2D): necking_axi(plane=True)."""
+    return necking_axi(nr=nx, nz=ny, plane=True, **kw)
 
 
 def serve_params(n_jobs, seed=20261203, lo=1000, hi=500000):

@@ -48,6 +48,7 @@ extern "C" {
 
 /* ---- enumerations -------------------------------------------------------- */
 /* element types: values of the reference's element enum (Element.h:62-71) */
+#define DNLG_EL_QUA4N_2D 1 /* 4-node plane-strain quad, 2x2 Gauss points (ElQua4N2D.C) */
 #define DNLG_EL_QUA4N_AX 3 /* 4-node axisymmetric quad, 2x2 Gauss points */
 #define DNLG_EL_HEX8N_3D 4 /* 8-node hexahedron, 2x2x2 Gauss points     */
 

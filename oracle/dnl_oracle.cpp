@@ -590,6 +590,7 @@ double law_dyield(const Law &L, double epsp, double depsp, double T, double dtim
  * ------------------------------------------------------------------------- */
 struct Tables {
   int type, dim, nn, nip;
+  bool axi; /* ElQua4NAx (Element::Axisymetric); ElQua4N2D is plane strain */
   double w[8];
   double N[8][8];      /* [ip][node] */
   double dNdr[8][8][3]; /* [ip][node][local dir] */
@@ -598,7 +599,8 @@ struct Tables {
 Tables make_tables(int type) {
   Tables T{};
   T.type = type;
-  if (type == DNLG_EL_QUA4N_AX) {
+  T.axi = type == DNLG_EL_QUA4N_AX;
+  if (type == DNLG_EL_QUA4N_AX || type == DNLG_EL_QUA4N_2D) {
     T.dim = 2;
     T.nn = 4;
     T.nip = 4;
@@ -777,14 +779,14 @@ bool element_jacobian(OracleJob *J, int64_t e, bool reference) {
         for (int k = 0; k < J->dim; k++) s += T.dNdr[p][a][k] * inv[k][j];
         dn[a * J->dim + j] = s;
       }
-    if (J->dim == 2) {
+    if (T.axi) {
       double r = 0.0;
       for (int a = 0; a < J->nn; a++) r += T.N[p][a] * J->x[cn[a]];
       J->radius[g] = r;
     }
     if (reference) {
       J->detJ0[g] = det;
-      if (J->dim == 2) J->detJ0[g] = det * J->radius[g];
+      if (T.axi) J->detJ0[g] = det * J->radius[g];
     }
   }
   return true;
@@ -808,7 +810,7 @@ void compute_mass(OracleJob *J) {
     for (int p = 0; p < J->nip; p++) {
       const int64_t g = e * J->nip + p;
       double wdj = J->tab.w[p] * J->detJ[g];
-      if (J->dim == 2) wdj *= k2PI * J->radius[g];
+      if (J->tab.axi) wdj *= k2PI * J->radius[g];
       for (int a = 0; a < J->nn; a++) me[a] += J->mat.density * J->tab.N[p][a] * wdj;
     }
     for (int a = 0; a < J->nn; a++) J->mass[J->conn[e * J->nn + a]] += me[a];
@@ -897,7 +899,8 @@ double courant(OracleJob *J) {
 
 /* Material::getHookeStiffnessMatrix (Material.C:98-152), threedimensional
  * (6 x 6, rows xx yy zz xy yz xz) and axisymetric (4 x 4, rows rr zz rz tt)
- * cases; the shear diagonal is (1 - 2 nu), as in the reference. */
+ * cases; the shear diagonal is (1 - 2 nu), as in the reference.  The
+ * planeStrain case is the leading 3 x 3 block of the axisymetric one. */
 void hooke(const OracleJob *J, double C[6][6]) {
   const double nu = J->mat.poisson, E = J->mat.young;
   for (int i = 0; i < 6; i++)
@@ -918,7 +921,7 @@ void hooke(const OracleJob *J, double C[6][6]) {
 
 /* Element3D::computeElasticStiffnessMatrix (Element3D.C:303-399) and
  * ElementPlane::computeElasticStiffnessMatrix (ElementPlane.C:182-292),
- * axisymmetric branch: K_e = sum_ip B^T C B w detJ (x 2 pi r), built as C B
+ * axisymmetric and plane-strain branches: K_e = sum_ip B^T C B w detJ (x 2 pi r), built as C B
  * then B^T (C B) per IP in the reference's term order, on the Jacobian of the
  * last compute_jacobian.  K is (nn dim) x (nn dim), row-major. */
 void element_stiffness(const OracleJob *J, int64_t e, double *K) {
@@ -955,8 +958,9 @@ void element_stiffness(const OracleJob *J, int64_t e, double *K) {
         }
       }
     } else {
-      const double R = J->radius[g];
-      WxdJ *= k2PI * R;
+      const bool axi = J->tab.axi;
+      const double R = axi ? J->radius[g] : 1.0;
+      if (axi) WxdJ *= k2PI * R;
       for (int i = 0; i < nn; i++) {
         const int I = 2 * i;
         const double dNx = dn[i * 2], dNy = dn[i * 2 + 1];
@@ -965,9 +969,11 @@ void element_stiffness(const OracleJob *J, int64_t e, double *K) {
           cb(r, I) = (C[r][0] * dNx + C[r][2] * dNy);
           cb(r, I + 1) = (C[r][1] * dNy + C[r][2] * dNx);
         }
-        cb(3, I) = (C[3][0] * dNx + C[3][2] * dNy);
-        cb(3, I + 1) = (C[3][1] * dNy + C[3][2] * dNx);
-        for (int r = 0; r < 4; r++) cb(r, I) += C[r][3] * NoverR;
+        if (axi) {
+          cb(3, I) = (C[3][0] * dNx + C[3][2] * dNy);
+          cb(3, I + 1) = (C[3][1] * dNy + C[3][2] * dNx);
+          for (int r = 0; r < 4; r++) cb(r, I) += C[r][3] * NoverR;
+        }
       }
       for (int i = 0; i < nn; i++) {
         const double dNx = dn[i * 2], dNy = dn[i * 2 + 1];
@@ -978,8 +984,10 @@ void element_stiffness(const OracleJob *J, int64_t e, double *K) {
             kk(I, Jc + b) += (dNx * cb(0, Jc + b) + dNy * cb(2, Jc + b)) * WxdJ;
             kk(I + 1, Jc + b) += (dNy * cb(1, Jc + b) + dNx * cb(2, Jc + b)) * WxdJ;
           }
-          kk(I, Jc) += cb(3, Jc) * NoverR * WxdJ;
-          kk(I, Jc + 1) += cb(3, Jc + 1) * NoverR * WxdJ;
+          if (axi) {
+            kk(I, Jc) += cb(3, Jc) * NoverR * WxdJ;
+            kk(I, Jc + 1) += cb(3, Jc + 1) * NoverR * WxdJ;
+          }
         }
       }
     }
@@ -1077,9 +1085,9 @@ void element_strains(OracleJob *J, int64_t e) {
       const int64_t nd = cn[a];
       for (int i = 0; i < J->dim; i++)
         for (int j = 0; j < J->dim; j++) F[3 * i + j] += dn[a * J->dim + j] * u[i * J->N + nd];
-      if (J->dim == 2) vr += J->tab.N[p][a] * u[nd];
+      if (J->tab.axi) vr += J->tab.N[p][a] * u[nd];
     }
-    if (J->dim == 2) F[8] += vr / J->radius[g];
+    if (J->tab.axi) F[8] += vr / J->radius[g];
     polar_lnU(F, &J->dE[g * 6], &J->Rr[g * 9]);
     if (J->prm.track_tensors)
       for (int c = 0; c < 6; c++) J->eps[g * 6 + c] += J->dE[g * 6 + c];
@@ -1215,11 +1223,11 @@ void element_forces(OracleJob *J, int64_t e) {
     const double *dn = &J->dN[g * J->nn * J->dim];
     const double *S = &J->sig[g * 6];
     double wdj = J->tab.w[p] * J->detJ[g];
-    if (J->dim == 2) wdj *= k2PI * J->radius[g];
+    if (J->tab.axi) wdj *= k2PI * J->radius[g];
     for (int a = 0; a < J->nn; a++) {
       for (int i = 0; i < J->dim; i++)
         for (int j = 0; j < J->dim; j++) f[a * J->dim + i] -= dn[a * J->dim + j] * S[sym_idx(j, i)] * wdj;
-      if (J->dim == 2) f[a * J->dim] -= J->tab.N[p][a] * S[5] / J->radius[g] * wdj;
+      if (J->tab.axi) f[a * J->dim] -= J->tab.N[p][a] * S[5] / J->radius[g] * wdj;
     }
   }
 }
@@ -1229,7 +1237,7 @@ void element_density(OracleJob *J, int64_t e) {
   for (int p = 0; p < J->nip; p++) {
     const int64_t g = e * J->nip + p;
     J->rho[g] = J->mat.density * J->detJ0[g] / J->detJ[g];
-    if (J->dim == 2) J->rho[g] /= J->radius[g];
+    if (J->tab.axi) J->rho[g] /= J->radius[g];
   }
 }
 
@@ -1521,7 +1529,9 @@ extern "C" {
 int oracle_job_create(const dnlg_mesh *mesh, const dnlg_material *mat, const dnlg_bcs *bcs,
                       const dnlg_solver_params *prm, OracleJob **out) {
   if (!mesh || !mat || !prm || !out) return DNLG_E_INVALID;
-  if (mesh->elem_type != DNLG_EL_QUA4N_AX && mesh->elem_type != DNLG_EL_HEX8N_3D) return DNLG_E_INVALID;
+  if (mesh->elem_type != DNLG_EL_QUA4N_AX && mesh->elem_type != DNLG_EL_HEX8N_3D &&
+      mesh->elem_type != DNLG_EL_QUA4N_2D)
+    return DNLG_E_INVALID;
   if (mesh->n_nodes <= 0 || mesh->n_elems <= 0 || !mesh->coords || !mesh->conn) return DNLG_E_INVALID;
   if (prm->dt_method != DNLG_DT_COURANT && prm->dt_method != DNLG_DT_POWER_ITERATION) return DNLG_E_INVALID;
   if (prm->stress_method != DNLG_STRESS_NR) return DNLG_E_INVALID;
@@ -1557,8 +1567,8 @@ int oracle_job_create(const dnlg_mesh *mesh, const dnlg_material *mat, const dnl
   for (int d = 0; d < 3; d++)
     if (d < J->dim)
       for (int64_t n = 0; n < J->N; n++) J->x[d * J->N + n] = mesh->coords[d * J->N + n];
-  /* ElementAxi::checkLevel2 (ElementAxi.C:103-126) */
-  if (J->dim == 2)
+  /* ElementAxi::checkLevel2 (ElementAxi.C:103-126); ElQua4N2D checks nothing */
+  if (J->tab.axi)
     for (int64_t n = 0; n < J->N; n++)
       if (J->x[n] < 0.0) {
         if (J->x[n] > -1e-8)
@@ -1763,7 +1773,7 @@ int oracle_job_energies(const OracleJob *J, double out[4]) {
   double ie = 0.0, wp = 0.0;
   for (int64_t g = 0; g < J->NE * J->nip; g++) {
     double vol = J->tab.w[g % J->nip] * J->detJ0[g];
-    if (J->dim == 2) vol *= k2PI;
+    if (J->tab.axi) vol *= k2PI;
     ie += J->mat.density * vol * J->eint[g];
     wp += J->mat.density * vol * J->einel[g];
   }
