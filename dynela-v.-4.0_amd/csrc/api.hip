@@ -492,7 +492,10 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
     return fail(ctx, DNLG_E_CAPACITY, "mesh exceeds 32-bit indexing");
   if (prm->dt_method != DNLG_DT_COURANT && prm->dt_method != DNLG_DT_POWER_ITERATION)
     return fail(ctx, DNLG_E_INVALID, "time-step method not implemented (Courant and PowerIteration are)");
-  if (prm->stress_method != DNLG_STRESS_NR) return fail(ctx, DNLG_E_INVALID, "only the Newton-Raphson stress integration is implemented");
+  if (prm->stress_method != DNLG_STRESS_NR && prm->stress_method != DNLG_STRESS_DIRECT)
+    return fail(ctx, DNLG_E_INVALID, "unknown stress integration method");
+  if (prm->stress_method == DNLG_STRESS_DIRECT && prm->precision != DNLG_PREC_FP64)
+    return fail(ctx, DNLG_E_INVALID, "the Direct stress integration runs in FP64 precision only");
   if (prm->dt_freq <= 0 || !(prm->safety > 0) || prm->rho_b < 0.0 || prm->rho_b > 1.0)
     return fail(ctx, DNLG_E_INVALID, "invalid solver parameters");
   if (prm->precision != DNLG_PREC_FP64 && prm->precision != DNLG_PREC_MIXED)
@@ -627,6 +630,7 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
   H.law = mat->law;
   H.dt_freq = prm->dt_freq;
   H.dt_method = prm->dt_method;
+  H.stress_method = prm->stress_method;
   H.rho0 = mat->density;
   H.young = mat->young;
   H.nu = mat->poisson;

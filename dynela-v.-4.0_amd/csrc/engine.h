@@ -148,7 +148,7 @@ struct JobDev {
   // sizes
   int32_t type, dim, nn, nip;
   int64_t N, NE, G;  // nodes, elements, integration points
-  int32_t prec, track, law, dt_freq, dt_method;
+  int32_t prec, track, law, dt_freq, dt_method, stress_method;
   // material (Material.h:110-169) and law parameters
   double rho0, young, nu, twoG, K, heatFrac, eta, cp, T0;
   double lp[8];

@@ -342,7 +342,17 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
       yield = law_yield(law, lp, kGammaInitial, kGammaInitial / dt, Tcur);
       gbl(C->sy)[g] = yield;
     }
-    if (Strial > yield) {
+    const bool plastic = Strial > yield;
+    if (plastic && C->stress_method == DNLG_STRESS_DIRECT) {
+      // Element::computeStressDirect (Element.C:625-768): one linearised
+      // return from the stored yield stress and the slope at the stored
+      // state (plastic strain, or gammaInitial before the first yield)
+      double y0, hard;
+      law_yield_and_slope(law, lp, heatFrac, ep0 > 0.0 ? ep0 : kGammaInitial, gbl(C->dep)[g], Tcur, dt, y0, hard);
+      gamma = (Snorm - kSqrt23 * yield) / (TwoG * (1.0 + hard / (1.5 * TwoG)));
+      const double T = Tcur + 0.5 * gamma * heatFrac * (kSqrt23 * yield + Snorm0);
+      yield = law_yield(law, lp, ep0 + kSqrt23 * gamma, kSqrt23 * gamma / dt, T);
+    } else if (plastic) {
       double gmin = 0.0, gmax = (Strial - yield) / TwoG32;
       gamma = gbl(C->gam)[g];
       if (ep0 == 0.0) gamma = kSqrt32 * kGammaInitial;
@@ -375,6 +385,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
           break;
         }
       }
+    }
+    if (plastic) {
       if (C->track) {
 #pragma unroll
         for (int c = 0; c < 6; c++) gbl(C->epsP)[c * G + g] += gamma * s[c] / Snorm;

@@ -675,8 +675,11 @@ class DynELA:
         self.saveTimes = (float(start), float(end), float(increment))
 
     def setStressComputation(self, method):
-        if method != "NewtonRaphson":
-            raise DnlgError(_lib.E_INVALID, "only the NewtonRaphson stress integration is implemented")
+        """DynELA::setStressComputation (DynELA.C:1406-1430): "NewtonRaphson"
+        (default) or "Direct" (one linearised return, FP64 precision)."""
+        if method not in ("NewtonRaphson", "Direct"):
+            raise DnlgError(_lib.E_INVALID, f"unknown stress computation method {method!r}")
+        self._invalidate()
         self._stress_method = method
 
     # ---- assembly of the engine descriptors -----------------------------------
@@ -736,8 +739,10 @@ class DynELA:
                 lists.append((np.cumsum(ptr), v[o]))
             else:
                 lists.append(None)
+        params = self._solver.params()
+        params.stress_method = _lib.STRESS_DIRECT if self._stress_method == "Direct" else _lib.STRESS_NR
         self._arrays = _lib.ModelArrays(self._elem_type, xyz.T.copy(), conn, self._material.descriptor(), bnds,
-                                        lists[0], lists[1], self._solver.params())
+                                        lists[0], lists[1], params)
         self._node_sorted = nums
         self._elem_sorted = enums
         return self._arrays

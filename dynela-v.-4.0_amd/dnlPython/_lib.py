@@ -28,7 +28,7 @@ LAW_JOHNSON_COOK, LAW_POWER, LAW_VOCE_LINEAR = 0, 1, 2
 BC_RESTRAIN, BC_SPEED = 0, 1
 RAMP_NONE, RAMP_LINEAR, RAMP_SINUS, RAMP_CONSTANT, RAMP_ERF, RAMP_SOFT = range(6)
 DT_COURANT, DT_POWER_ITERATION, DT_POWER_ITERATION_UNDER = 0, 1, 2
-STRESS_NR = 0
+STRESS_NR, STRESS_DIRECT = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 (F_PLASTIC_STRAIN, F_PLASTIC_STRAIN_RATE, F_GAMMA, F_GAMMA_CUMULATE, F_YIELD_STRESS, F_TEMPERATURE,
  F_DENSITY, F_PRESSURE, F_INTERNAL_ENERGY, F_INELASTIC_ENERGY, F_VON_MISES, F_STRESS, F_STRAIN,

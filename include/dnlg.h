@@ -75,8 +75,9 @@ extern "C" {
 #define DNLG_DT_POWER_ITERATION 1
 #define DNLG_DT_POWER_ITERATION_UNDER 2
 
-/* stress integration (Model.h StressIntNR / StressIntDirect) */
-#define DNLG_STRESS_NR 0
+/* stress integration (Model.h StressIntNR / StressIntDirect, Model.C:840-851) */
+#define DNLG_STRESS_NR 0     /* Newton-Raphson radial return, Element.C:407-622 */
+#define DNLG_STRESS_DIRECT 1 /* one linearised return, Element.C:625-768 (FP64 only) */
 
 /* arithmetic of the element phases */
 #define DNLG_PREC_FP64 0  /* everything in FP64                              */

@@ -1139,7 +1139,26 @@ bool element_stress(OracleJob *J, int64_t e, double timeStep) {
       J->sy[g] = yield;
     }
     int iterate = 0;
-    if (Strial > yield) {
+    if (Strial > yield && J->prm.stress_method == DNLG_STRESS_DIRECT) {
+      /* Element::computeStressDirect (Element.C:625-768): one linearised
+       * return from the stored yield stress and the slope at the stored
+       * state (plastic strain, or gammaInitial before the first yield) */
+      const double hard = law_dyield(L, epsp > 0.0 ? epsp : gammaInitial, depsp, T, timeStep);
+      gamma = (Snorm - kSqrt23 * yield) / (TwoG * (1.0 + hard / (1.5 * TwoG)));
+      depsp = kSqrt23 * gamma / timeStep;
+      epsp = J->ep[g] + kSqrt23 * gamma;
+      T = T0 + 0.5 * gamma * heatFrac * (kSqrt23 * yield + Snorm0);
+      yield = law_yield(L, epsp, depsp, T);
+      if (J->prm.track_tensors)
+        for (int c = 0; c < 6; c++) J->epsP[g * 6 + c] += gamma * s[c] / Snorm;
+      const double scale = 1.0 - TwoG * gamma / Snorm;
+      for (int c = 0; c < 6; c++) s[c] *= scale;
+      J->ep[g] += kSqrt23 * gamma;
+      J->dep[g] = kSqrt23 * gamma / timeStep;
+      J->gam[g] = gamma;
+      J->gamc[g] += gamma;
+      J->sy[g] = yield;
+    } else if (Strial > yield) {
       double gmin = 0.0;
       double gmax = (Strial - yield) / TwoG32;
       gamma = J->gam[g];
@@ -1534,7 +1553,7 @@ int oracle_job_create(const dnlg_mesh *mesh, const dnlg_material *mat, const dnl
     return DNLG_E_INVALID;
   if (mesh->n_nodes <= 0 || mesh->n_elems <= 0 || !mesh->coords || !mesh->conn) return DNLG_E_INVALID;
   if (prm->dt_method != DNLG_DT_COURANT && prm->dt_method != DNLG_DT_POWER_ITERATION) return DNLG_E_INVALID;
-  if (prm->stress_method != DNLG_STRESS_NR) return DNLG_E_INVALID;
+  if (prm->stress_method != DNLG_STRESS_NR && prm->stress_method != DNLG_STRESS_DIRECT) return DNLG_E_INVALID;
   if (prm->dt_freq <= 0) return DNLG_E_INVALID;
   OracleJob *J = new OracleJob();
   J->tab = make_tables(mesh->elem_type);

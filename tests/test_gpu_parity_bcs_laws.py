@@ -89,3 +89,25 @@ def test_power_law_taylor_axi(ctx, prec):
         print(f"mixed power-law disp_err={err:.3e}")
         assert err <= 1e-5
     assert np.max(m.getIntPointField(dnl.Field.plasticStrain)) > 0.01
+
+
+@pytest.mark.parametrize("kind", ["axi", "3d"])
+def test_direct_stress_integration(ctx, kind):
+    """setStressComputation("Direct") (Element.C:625-768) against the oracle."""
+    from dnlPython import samples
+    if kind == "axi":
+        m = samples.taylor_axi(nr=5, nz=25, t_end=10e-6, context=ctx)
+    else:
+        m = samples.taylor_3d(n=4, nz=12, t_end=10e-6, context=ctx)
+    m.setStressComputation("Direct")
+    o = run_both(m, 10e-6)
+    compare(m, o)
+
+
+def test_direct_stress_integration_needs_fp64(ctx):
+    import dnlPython as dnl
+    from dnlPython import samples
+    m = samples.taylor_axi(nr=3, nz=8, precision="mixed", context=ctx)
+    m.setStressComputation("Direct")
+    with pytest.raises(dnl.DnlgError):
+        m.solve()
