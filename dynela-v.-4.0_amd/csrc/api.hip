@@ -500,8 +500,19 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
     return fail(ctx, DNLG_E_INVALID, "invalid solver parameters");
   if (prm->precision != DNLG_PREC_FP64 && prm->precision != DNLG_PREC_MIXED)
     return fail(ctx, DNLG_E_INVALID, "invalid precision");
-  if (mat->law < DNLG_LAW_JOHNSON_COOK || mat->law > DNLG_LAW_VOCE_LINEAR)
+  if (mat->law < DNLG_LAW_JOHNSON_COOK || mat->law > DNLG_LAW_TABULAR)
     return fail(ctx, DNLG_E_INVALID, "invalid hardening law");
+  if (mat->law == DNLG_LAW_TABULAR) {
+    // DiscreteFunction with at least one segment, strains strictly ascending
+    if (!mat->table || mat->n_table < 2 || mat->n_table > (1 << 20))
+      return fail(ctx, DNLG_E_INVALID, "a tabular law needs 2 to 2^20 points");
+    for (int64_t k = 0; k < mat->n_table; k++)
+      if (!std::isfinite(mat->table[2 * k]) || !std::isfinite(mat->table[2 * k + 1]) ||
+          (k > 0 && !(mat->table[2 * k] > mat->table[2 * k - 2])))
+        return fail(ctx, DNLG_E_INVALID, "tabular law points must be finite with ascending strains");
+    if (prm->precision != DNLG_PREC_FP64)
+      return fail(ctx, DNLG_E_INVALID, "the tabular law runs in FP64 precision only");
+  }
   if (!(mat->density > 0) || !(mat->young > 0) || !(mat->heat_capacity > 0))
     return fail(ctx, DNLG_E_INVALID, "material needs density, Young modulus and heat capacity > 0");
   if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, DNLG_E_HIP, "hipSetDevice");
@@ -722,6 +733,17 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
   ALLOC(H.T, G, fillT.data());
   ALLOC(H.rho, G, fillR.data());
   ALLOC(H.detJ0, G, (const double *)nullptr);
+  if (mat->law == DNLG_LAW_TABULAR) {
+    std::vector<double> t(2 * mat->n_table);
+    for (int64_t k = 0; k < mat->n_table; k++) {
+      t[k] = mat->table[2 * k];
+      t[mat->n_table + k] = mat->table[2 * k + 1];
+    }
+    double *pt;
+    ALLOC(pt, (int64_t)t.size(), t.data());
+    H.ltab = pt;
+    H.n_ltab = (int32_t)mat->n_table;
+  }
   if (prm->precision == DNLG_PREC_MIXED) {
     ALLOC(H.elemF32, NE * nn * dim, (const float *)nullptr);
     ALLOC(H.sig32, 6 * G, (const float *)nullptr);

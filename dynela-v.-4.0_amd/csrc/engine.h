@@ -152,6 +152,8 @@ struct JobDev {
   // material (Material.h:110-169) and law parameters
   double rho0, young, nu, twoG, K, heatFrac, eta, cp, T0;
   double lp[8];
+  const double *ltab;  // DNLG_LAW_TABULAR: strains [0, n_ltab), yield stresses [n_ltab, 2 n_ltab)
+  int32_t n_ltab;
   // FP32 copies for the mixed path's radial return, and 1 / rho0
   float lpf[8];
   float inv_dTf;  // 1 / (Tm - T0) of Johnson-Cook

@@ -169,6 +169,28 @@ __device__ __forceinline__ Control cycle_control(JobDev *J, CJobDev *C, int cs, 
 }
 
 
+// The hardening law's yield stress (and slope) at (epsp, depsp, T) for the
+// FP64 radial returns: a tabular law through its table (out of it: `ok`
+// cleared), the closed-form laws through material.h.
+__device__ __forceinline__ double law_yield_of(CJobDev *C, int law, const double *lp, double epsp, double depsp,
+                                               double T, bool &ok) {
+  if (law == DNLG_LAW_TABULAR) {
+    double y = 0.0, h;
+    ok = law_tabular(C->ltab, C->n_ltab, epsp, y, h) && ok;
+    return y;
+  }
+  return law_yield(law, lp, epsp, depsp, T);
+}
+__device__ __forceinline__ void law_eval(CJobDev *C, int law, const double *lp, double eta_rcp, double epsp,
+                                         double depsp, double T, double dt, double &yield, double &hard, bool &ok) {
+  if (law == DNLG_LAW_TABULAR) {
+    yield = hard = 0.0;
+    ok = law_tabular(C->ltab, C->n_ltab, epsp, yield, hard) && ok;
+    return;
+  }
+  law_yield_and_slope(law, lp, eta_rcp, epsp, depsp, T, dt, yield, hard);
+}
+
 // ---------------------------------------------------------------------------
 // Element kernel, FP64.  One lane per integration point (NIP lanes per
 // element, NIP == NN for both element types); the element's nodes are staged
@@ -338,8 +360,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
     double lp[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) lp[k] = C->lp[k];
+    bool in_table = true;
     if (yield == 0.0) {
-      yield = law_yield(law, lp, kGammaInitial, kGammaInitial / dt, Tcur);
+      yield = law_yield_of(C, law, lp, kGammaInitial, kGammaInitial / dt, Tcur, in_table);
       gbl(C->sy)[g] = yield;
     }
     const bool plastic = Strial > yield;
@@ -348,10 +371,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
       // return from the stored yield stress and the slope at the stored
       // state (plastic strain, or gammaInitial before the first yield)
       double y0, hard;
-      law_yield_and_slope(law, lp, heatFrac, ep0 > 0.0 ? ep0 : kGammaInitial, gbl(C->dep)[g], Tcur, dt, y0, hard);
+      law_eval(C, law, lp, heatFrac, ep0 > 0.0 ? ep0 : kGammaInitial, gbl(C->dep)[g], Tcur, dt, y0, hard, in_table);
       gamma = (Snorm - kSqrt23 * yield) / (TwoG * (1.0 + hard / (1.5 * TwoG)));
       const double T = Tcur + 0.5 * gamma * heatFrac * (kSqrt23 * yield + Snorm0);
-      yield = law_yield(law, lp, ep0 + kSqrt23 * gamma, kSqrt23 * gamma / dt, T);
+      yield = law_yield_of(C, law, lp, ep0 + kSqrt23 * gamma, kSqrt23 * gamma / dt, T, in_table);
     } else if (plastic) {
       double gmin = 0.0, gmax = (Strial - yield) / TwoG32;
       gamma = gbl(C->gam)[g];
@@ -363,7 +386,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
       int iterate = 0;
       while (true) {
         double hard;
-        law_yield_and_slope(law, lp, eta_rcp, epsp, depsp, T, dt, yield, hard);
+        law_eval(C, law, lp, eta_rcp, epsp, depsp, T, dt, yield, hard, in_table);
+        if (!in_table) break;
         const double fun = Strial - gamma * TwoG32 - yield;
         if (fun < 0.0)
           gmax = gamma;
@@ -386,6 +410,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
         }
       }
     }
+    if (!in_table) set_status(J, DNLG_E_TABLE_RANGE, par);
     if (plastic) {
       if (C->track) {
 #pragma unroll

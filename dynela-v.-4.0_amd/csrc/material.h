@@ -60,6 +60,28 @@ __device__ __forceinline__ void law_yield_and_slope(int law, const double *p, do
   hard = p[1] * p[2] * ex + p[3];
 }
 
+// TabularLaw::getYieldStress / getDerivateYieldStress (TabularLaw.C): the
+// table's piecewise-linear interpolation and the slope of the segment that
+// ends at the first point at or past epsp (DiscreteFunction::getValue /
+// getSlope, DiscreteFunction.C:188-216, 284-308).  Returns false outside
+// the table (the reference stops there).
+__device__ __forceinline__ bool law_tabular(const double *t, int n, double epsp, double &yield, double &hard) {
+  const DNLG_GLOBAL double *x = gbl(t), *y = gbl(t) + n;
+  if (!(epsp >= x[0] && epsp <= x[n - 1])) return false;
+  double x0 = x[0], y0 = y[0];
+  for (int i = 1; i < n; i++) {
+    const double xi = x[i], yi = y[i];
+    if (xi >= epsp) {
+      yield = (xi == epsp) ? yi : ((epsp - x0) / (xi - x0)) * (yi - y0) + y0;
+      hard = (yi - y0) / (xi - x0);
+      return true;
+    }
+    x0 = xi;
+    y0 = yi;
+  }
+  return true;
+}
+
 // FP32 law evaluation of the mixed path's radial return.  Same yield stress
 // and slope as law_yield_and_slope, written in terms of gamma (depsp =
 // sqrt(2/3) gamma / dt, so depsp * dt = sqrt(2/3) gamma), with powers and

@@ -40,6 +40,7 @@ struct ServerJob {
   std::vector<double> coords;
   std::vector<int32_t> conn;
   dnlg_material mat{};
+  std::vector<double> table;  // tabular law points
   std::vector<dnlg_boundary> bnd;
   std::vector<int64_t> iptr, cptr;
   std::vector<int32_t> iidx, cidx;
@@ -50,6 +51,7 @@ struct ServerJob {
     std::vector<double>().swap(coords);
     std::vector<int32_t>().swap(conn);
     std::vector<dnlg_boundary>().swap(bnd);
+    std::vector<double>().swap(table);
     std::vector<int64_t>().swap(iptr);
     std::vector<int64_t>().swap(cptr);
     std::vector<int32_t>().swap(iidx);
@@ -153,6 +155,10 @@ int dnlg_server_submit(dnlg_server *s, const dnlg_mesh *mesh, const dnlg_materia
   j->coords.assign(mesh->coords, mesh->coords + (size_t)dim * mesh->n_nodes);
   j->conn.assign(mesh->conn, mesh->conn + (size_t)nn * mesh->n_elems);
   j->mat = *mat;
+  if (mat->law == DNLG_LAW_TABULAR && mat->table && mat->n_table > 0) {
+    j->table.assign(mat->table, mat->table + 2 * mat->n_table);
+    j->mat.table = j->table.data();
+  }
   j->prm = *prm;
   if (bcs && bcs->n_boundaries > 0) {
     if (!bcs->boundaries) return srv_fail(s, DNLG_E_INVALID, "boundaries missing");

@@ -26,6 +26,7 @@ from ._lib import (DnlgError, NegativeJacobianError, NewtonRaphsonError, PowerIt
                    NonFiniteError)
 
 __all__ = ["DynELA", "NodeSet", "ElementSet", "Material", "JohnsonCookLaw", "ElastoplasticLaw", "HistoryFile",
+           "TabularLaw", "DiscreteFunction",
            "VtkInterface", "SvgInterface",
            "VoceLinearLaw", "BoundaryRestrain", "BoundarySpeed", "RampFunction", "Explicit", "Field",
            "Element", "Vec3D", "Context", "DnlgError", "NegativeJacobianError", "NewtonRaphsonError",
@@ -164,6 +165,69 @@ class VoceLinearLaw:
         self.p = [float(A), float(Q), float(b), float(H)] + [0.0] * 4
 
 
+class DiscreteFunction:
+    """DiscreteFunction (DiscreteFunction.h): points (x, y) kept in
+    ascending x; getValue / getSlope as DiscreteFunction.C:188-216,
+    284-308 (a value outside [x_first, x_last] raises, where the reference
+    stops)."""
+
+    def __init__(self, name=""):
+        self.name = name
+        self._pts = []
+
+    def add(self, x, y):
+        self._pts.append((float(x), float(y)))
+        self._pts.sort(key=lambda p: p[0])
+
+    def getNumberOfPoints(self):
+        return len(self._pts)
+
+    def getXPoint(self, i):
+        return self._pts[i][0]
+
+    def getYPoint(self, i):
+        return self._pts[i][1]
+
+    def _segment(self, x):
+        if not self._pts or x < self._pts[0][0] or x > self._pts[-1][0]:
+            raise DnlgError(_lib.E_TABLE_RANGE, f"value {x} is out of bounds")
+        for i in range(1, len(self._pts)):
+            if self._pts[i][0] >= x:
+                return self._pts[i - 1], self._pts[i]
+        return None
+
+    def getValue(self, x):
+        seg = self._segment(x)
+        if seg is None:
+            return 0.0
+        (x0, y0), (x1, y1) = seg
+        return y1 if x1 == x else ((x - x0) / (x1 - x0)) * (y1 - y0) + y0
+
+    def getSlope(self, x):
+        seg = self._segment(x)
+        if seg is None:
+            return 0.0
+        (x0, y0), (x1, y1) = seg
+        return (y1 - y0) / (x1 - x0)
+
+
+class TabularLaw:
+    """TabularLaw (TabularLaw.C): yield stress as a DiscreteFunction of
+    the equivalent plastic strain (FP64 precision)."""
+    law = _lib.LAW_TABULAR
+
+    def __init__(self, name=""):
+        self.name = name
+        self.p = [0.0] * 8
+        self._function = None
+
+    def setFunction(self, function):
+        self._function = function
+
+    def getFunction(self):
+        return self._function
+
+
 class Material:
     """Material attributes of Material.h:49-56."""
 
@@ -200,6 +264,13 @@ class Material:
         d.law = self._law.law
         for k in range(8):
             d.p[k] = self._law.p[k]
+        if isinstance(self._law, TabularLaw):
+            f = self._law.getFunction()
+            if f is None:
+                raise DnlgError(_lib.E_INVALID, f"tabular law {self._law.name!r} has no function")
+            d._table = np.ascontiguousarray(np.array(f._pts, dtype=np.float64).reshape(-1))
+            d.table = _lib.as_dp(d._table)
+            d.n_table = len(f._pts)
         return d
 
 

@@ -16,15 +16,17 @@ LIB_PATH = os.path.join(PKG_DIR, "libdnlg.so")
 OK = 0
 E_INVALID, E_OOM, E_NEG_JACOBIAN, E_NR_DIVERGED, E_NONFINITE, E_HIP, E_STATE, E_CAPACITY, E_PI_DIVERGED = (
     -1, -2, -3, -4, -5, -6, -7, -8, -9)
+E_TABLE_RANGE = -10
 STATUS_NAMES = {
     OK: "OK", E_INVALID: "INVALID", E_OOM: "OUT_OF_MEMORY", E_NEG_JACOBIAN: "NEGATIVE_JACOBIAN",
     E_NR_DIVERGED: "NEWTON_RAPHSON_DIVERGED", E_NONFINITE: "NON_FINITE", E_HIP: "HIP_ERROR",
     E_STATE: "INVALID_STATE", E_CAPACITY: "CAPACITY", E_PI_DIVERGED: "POWER_ITERATION_DIVERGED",
+    E_TABLE_RANGE: "TABLE_RANGE",
 }
 EL_QUA4N_2D = 1
 EL_QUA4N_AX = 3
 EL_HEX8N_3D = 4
-LAW_JOHNSON_COOK, LAW_POWER, LAW_VOCE_LINEAR = 0, 1, 2
+LAW_JOHNSON_COOK, LAW_POWER, LAW_VOCE_LINEAR, LAW_TABULAR = 0, 1, 2, 3
 BC_RESTRAIN, BC_SPEED = 0, 1
 RAMP_NONE, RAMP_LINEAR, RAMP_SINUS, RAMP_CONSTANT, RAMP_ERF, RAMP_SOFT = range(6)
 DT_COURANT, DT_POWER_ITERATION, DT_POWER_ITERATION_UNDER = 0, 1, 2
@@ -50,7 +52,7 @@ class MaterialDesc(ctypes.Structure):
     _fields_ = [("young", ctypes.c_double), ("poisson", ctypes.c_double), ("density", ctypes.c_double),
                 ("heat_capacity", ctypes.c_double), ("taylor_quinney", ctypes.c_double),
                 ("T0", ctypes.c_double), ("law", ctypes.c_int32), ("reserved", ctypes.c_int32),
-                ("p", ctypes.c_double * 8)]
+                ("p", ctypes.c_double * 8), ("table", ctypes.POINTER(ctypes.c_double)), ("n_table", ctypes.c_int64)]
 
 
 class BoundaryDesc(ctypes.Structure):
@@ -137,7 +139,7 @@ def load(path=None):
         f = getattr(lib, name)
         f.restype = res
         f.argtypes = args
-    if lib.dnlg_abi_version() != 1:
+    if lib.dnlg_abi_version() != 2:
         raise RuntimeError("libdnlg.so ABI version mismatch")
     if path is None:
         _lib = lib

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define DNLG_ABI_VERSION 1
+#define DNLG_ABI_VERSION 2
 
 /* ---- status codes ------------------------------------------------------ */
 #define DNLG_OK 0
@@ -45,6 +45,8 @@ extern "C" {
 #define DNLG_E_STATE (-7)        /* call not valid in the job's state       */
 #define DNLG_E_CAPACITY (-8)     /* request exceeds a fixed capacity        */
 #define DNLG_E_PI_DIVERGED (-9)  /* power iteration > 1000 its (Model.C:1002) */
+#define DNLG_E_TABLE_RANGE (-10) /* plastic strain outside a tabular law's table
+                                    (DiscreteFunction.C:195-196, 291-292) */
 
 /* ---- enumerations -------------------------------------------------------- */
 /* element types: values of the reference's element enum (Element.h:62-71) */
@@ -56,6 +58,9 @@ extern "C" {
 #define DNLG_LAW_JOHNSON_COOK 0 /* JohnsonCookLaw.C:146-170,298-339; p = A,B,C,n,m,depsp0,Tm,T0 */
 #define DNLG_LAW_POWER 1        /* ElastoplasticLaw.C:110-121;          p = A,B,n            */
 #define DNLG_LAW_VOCE_LINEAR 2  /* sy = A + Q(1-exp(-b ep)) + H ep;     p = A,Q,b,H          */
+#define DNLG_LAW_TABULAR 3      /* TabularLaw.C: sy(ep) piecewise linear through the
+                                   material's table (DiscreteFunction::getValue /
+                                   getSlope, DiscreteFunction.C:188-308); FP64 only */
 
 /* boundary kinds (Boundary.h:59-62) */
 #define DNLG_BC_RESTRAIN 0 /* BoundaryRestrain.C:98-150 */
@@ -123,6 +128,9 @@ typedef struct dnlg_material {
   int32_t law;           /* DNLG_LAW_* */
   int32_t reserved;
   double p[8];           /* law parameters, see DNLG_LAW_* */
+  const double *table;   /* DNLG_LAW_TABULAR: n_table (plastic strain, yield
+                            stress) pairs, strains ascending; else NULL */
+  int64_t n_table;
 } dnlg_material;
 
 typedef struct dnlg_boundary {

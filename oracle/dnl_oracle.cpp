@@ -37,6 +37,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <atomic>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -526,11 +527,39 @@ struct Law {
   int kind;
   double p[8];
   double eta, rho, cp; /* Taylor-Quinney, density, heat capacity */
+  std::vector<double> tx, ty; /* DNLG_LAW_TABULAR points */
 };
+
+Law law_of(const dnlg_material *m) {
+  Law L{m->law, {0}, m->taylor_quinney, m->density, m->heat_capacity};
+  for (int k = 0; k < 8; k++) L.p[k] = m->p[k];
+  if (m->law == DNLG_LAW_TABULAR && m->table)
+    for (int64_t k = 0; k < m->n_table; k++) {
+      L.tx.push_back(m->table[2 * k]);
+      L.ty.push_back(m->table[2 * k + 1]);
+    }
+  return L;
+}
+
+/* DiscreteFunction::getValue / getSlope (DiscreteFunction.C:188-216,
+ * 284-308) of a TabularLaw; NaN outside the table, where the reference
+ * stops with a fatal error. */
+double tabular(const Law &L, double x, bool slope) {
+  const size_t n = L.tx.size();
+  if (n < 2 || x < L.tx[0] || x > L.tx[n - 1]) return std::nan("");
+  for (size_t i = 1; i < n; i++)
+    if (L.tx[i] >= x) {
+      if (slope) return (L.ty[i] - L.ty[i - 1]) / (L.tx[i] - L.tx[i - 1]);
+      if (L.tx[i] == x) return L.ty[i];
+      return ((x - L.tx[i - 1]) / (L.tx[i] - L.tx[i - 1])) * (L.ty[i] - L.ty[i - 1]) + L.ty[i - 1];
+    }
+  return 0.0;
+}
 
 /* JohnsonCookLaw::getYieldStress (JohnsonCookLaw.C:146-170) and
  * ElastoplasticLaw::getYieldStress (ElastoplasticLaw.C:110-114). */
 double law_yield(const Law &L, double epsp, double depsp, double T) {
+  if (L.kind == DNLG_LAW_TABULAR) return tabular(L, epsp, false);
   if (L.kind == DNLG_LAW_JOHNSON_COOK) {
     const double A = L.p[0], B = L.p[1], C = L.p[2], n = L.p[3], m = L.p[4], d0 = L.p[5], Tm = L.p[6],
                  T0 = L.p[7];
@@ -557,6 +586,7 @@ double law_yield(const Law &L, double epsp, double depsp, double T) {
 /* JohnsonCookLaw::getDerivateYieldStress (JohnsonCookLaw.C:298-339) and
  * ElastoplasticLaw::getDerivateYieldStress (ElastoplasticLaw.C:117-121). */
 double law_dyield(const Law &L, double epsp, double depsp, double T, double dtime) {
+  if (L.kind == DNLG_LAW_TABULAR) return tabular(L, epsp, true);
   if (L.kind == DNLG_LAW_JOHNSON_COOK) {
     const double A = L.p[0], B = L.p[1], C = L.p[2], n = L.p[3], m = L.p[4], d0 = L.p[5], Tm = L.p[6],
                  T0 = L.p[7];
@@ -722,6 +752,7 @@ struct OracleJob {
   int status = DNLG_OK;
   int threads = 1;
   std::string err;
+  std::atomic<bool> table_range{false}; /* a tabular law was evaluated outside its table */
 };
 
 namespace {
@@ -1138,6 +1169,10 @@ bool element_stress(OracleJob *J, int64_t e, double timeStep) {
       yield = law_yield(L, gammaInitial, gammaInitial / timeStep, T);
       J->sy[g] = yield;
     }
+    if (std::isnan(yield)) {
+      J->table_range = true;
+      return false;
+    }
     int iterate = 0;
     if (Strial > yield && J->prm.stress_method == DNLG_STRESS_DIRECT) {
       /* Element::computeStressDirect (Element.C:625-768): one linearised
@@ -1149,6 +1184,10 @@ bool element_stress(OracleJob *J, int64_t e, double timeStep) {
       epsp = J->ep[g] + kSqrt23 * gamma;
       T = T0 + 0.5 * gamma * heatFrac * (kSqrt23 * yield + Snorm0);
       yield = law_yield(L, epsp, depsp, T);
+      if (std::isnan(yield) || std::isnan(hard)) {
+        J->table_range = true;
+        return false;
+      }
       if (J->prm.track_tensors)
         for (int c = 0; c < 6; c++) J->epsP[g * 6 + c] += gamma * s[c] / Snorm;
       const double scale = 1.0 - TwoG * gamma / Snorm;
@@ -1169,6 +1208,10 @@ bool element_stress(OracleJob *J, int64_t e, double timeStep) {
       bool run = true;
       while (run) {
         yield = law_yield(L, epsp, depsp, T);
+        if (std::isnan(yield)) {
+          J->table_range = true;
+          return false;
+        }
         const double fun = Strial - gamma * TwoG32 - yield;
         if (fun < 0.0)
           gmax = gamma;
@@ -1282,7 +1325,10 @@ bool element_phases(OracleJob *J, double timeStep) {
     element_forces(J, e);
   }
   if (!ok) {
-    set_err(J, DNLG_E_NR_DIVERGED, "no convergence in Newton-Raphson");
+    if (J->table_range)
+      set_err(J, DNLG_E_TABLE_RANGE, "plastic strain outside the tabular law");
+    else
+      set_err(J, DNLG_E_NR_DIVERGED, "no convergence in Newton-Raphson");
     return false;
   }
   assemble_forces(J);
@@ -1564,11 +1610,7 @@ int oracle_job_create(const dnlg_mesh *mesh, const dnlg_material *mat, const dnl
   J->NE = mesh->n_elems;
   J->mat = *mat;
   J->prm = *prm;
-  J->law.kind = mat->law;
-  for (int k = 0; k < 8; k++) J->law.p[k] = mat->p[k];
-  J->law.eta = mat->taylor_quinney;
-  J->law.rho = mat->density;
-  J->law.cp = mat->heat_capacity;
+  J->law = law_of(mat);
   /* Explicit::computeChungHulbertIntegrationParameters (Explicit.C:86-104) */
   const double rb = prm->rho_b;
   J->alphaM = (2.0 * rb - 1.0) / (1.0 + rb);
@@ -1888,15 +1930,11 @@ void oracle_jacobi_eig3(const double A[9], double Q[9], double w[3]) {
 void oracle_rotate_sym(const double R[9], double S[6]) { rotate_sym(R, S); }
 
 double oracle_law_yield(const dnlg_material *m, double epsp, double depsp, double T) {
-  Law L{m->law, {0}, m->taylor_quinney, m->density, m->heat_capacity};
-  for (int k = 0; k < 8; k++) L.p[k] = m->p[k];
-  return law_yield(L, epsp, depsp, T);
+  return law_yield(law_of(m), epsp, depsp, T);
 }
 
 double oracle_law_dyield(const dnlg_material *m, double epsp, double depsp, double T, double dtime) {
-  Law L{m->law, {0}, m->taylor_quinney, m->density, m->heat_capacity};
-  for (int k = 0; k < 8; k++) L.p[k] = m->p[k];
-  return law_dyield(L, epsp, depsp, T, dtime);
+  return law_dyield(law_of(m), epsp, depsp, T, dtime);
 }
 
 /* one-IP radial return for unit tests: state[] = {sig[6], ep, dep, sy, gam,
@@ -1906,11 +1944,7 @@ int oracle_radial_return(const dnlg_material *m, const double dE[6], double pres
   OracleJob J;
   J.nip = 1;
   J.mat = *m;
-  J.law.kind = m->law;
-  for (int k = 0; k < 8; k++) J.law.p[k] = m->p[k];
-  J.law.eta = m->taylor_quinney;
-  J.law.rho = m->density;
-  J.law.cp = m->heat_capacity;
+  J.law = law_of(m);
   J.sig.assign(state, state + 6);
   J.dE.assign(dE, dE + 6);
   J.pres.assign(1, pressure);

@@ -36,7 +36,7 @@ def test_bindings_cover_the_header():
 
 def test_abi_version_and_footprint_without_gpu():
     lib = _lib.load()
-    assert lib.dnlg_abi_version() == 1
+    assert lib.dnlg_abi_version() == 2
     b64 = lib.dnlg_footprint_bytes(_lib.EL_HEX8N_3D, 125341, 120000, _lib.PREC_FP64, 0)
     bmx = lib.dnlg_footprint_bytes(_lib.EL_HEX8N_3D, 125341, 120000, _lib.PREC_MIXED, 0)
     assert b64 > 0 and bmx > 0 and bmx < b64 + 120000 * 8 * 4 * 24
@@ -47,7 +47,8 @@ def test_struct_layouts_match_header():
     # offsets follow from the C declarations (natural alignment, LP64)
     assert ctypes.sizeof(_lib.Mesh) == 40
     assert _lib.Mesh.coords.offset == 24
-    assert ctypes.sizeof(_lib.MaterialDesc) == 6 * 8 + 8 + 64
+    assert ctypes.sizeof(_lib.MaterialDesc) == 6 * 8 + 8 + 64 + 8 + 8
+    assert _lib.MaterialDesc.table.offset == 6 * 8 + 8 + 64
     assert ctypes.sizeof(_lib.BoundaryDesc) == 8 + 24 + 16
     assert ctypes.sizeof(_lib.Bcs) == 8 + 5 * 8
     assert ctypes.sizeof(_lib.SolverParams) == 4 * 8 + 6 * 4

@@ -111,3 +111,44 @@ def test_direct_stress_integration_needs_fp64(ctx):
     m.setStressComputation("Direct")
     with pytest.raises(dnl.DnlgError):
         m.solve()
+
+
+def _tabular(points):
+    import dnlPython as dnl
+    f = dnl.DiscreteFunction("table")
+    for x, y in points:
+        f.add(x, y)
+    law = dnl.TabularLaw("tab")
+    law.setFunction(f)
+    return law
+
+
+def test_tabular_law(ctx):
+    """TabularLaw (TabularLaw.C) through a table of the copper impact's
+    Johnson-Cook curve at room temperature, against the oracle."""
+    from dnlPython import samples
+    xs = np.concatenate([[0.0], np.geomspace(1e-9, 5.0, 300)])
+    law = _tabular(zip(xs, 90e6 + 292e6 * xs ** 0.31))
+    m = samples.taylor_axi(nr=5, nz=25, t_end=10e-6, context=ctx)
+    m._material.setHardeningLaw(law)
+    m._invalidate()
+    o = run_both(m, 10e-6)
+    compare(m, o)
+
+
+def test_tabular_law_errors(ctx):
+    """Out of the table: the job stops with DNLG_E_TABLE_RANGE (the reference
+    exits, DiscreteFunction.C:195-196); mixed precision is refused."""
+    import dnlPython as dnl
+    from dnlPython import _lib, samples
+    m = samples.taylor_axi(nr=4, nz=20, t_end=8e-6, context=ctx)
+    m._material.setHardeningLaw(_tabular([(0.0, 120e6), (0.01, 125e6)]))
+    m._invalidate()
+    with pytest.raises(dnl.DnlgError) as e:
+        m.solve()
+    assert e.value.status == _lib.E_TABLE_RANGE
+    m = samples.taylor_axi(nr=4, nz=20, precision="mixed", context=ctx)
+    m._material.setHardeningLaw(_tabular([(0.0, 120e6), (1.0, 125e6)]))
+    m._invalidate()
+    with pytest.raises(dnl.DnlgError):
+        m.solve()
