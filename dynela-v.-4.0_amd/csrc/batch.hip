@@ -538,6 +538,23 @@ int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vec
       if (launched >= max_cycles) break;
       need = std::min(need, (double)(max_cycles - launched));
     }
+    if (use_graph && phase > 0) {
+      // A captured sequence is specific to the increment phase it starts
+      // at, and capturing one costs as much as a few hundred cycles: bring
+      // the jobs to phase 0 with at most freq - 1 directly launched cycles,
+      // so that each batch captures the phase-0 sequence once and reuses it
+      // across solve calls.
+      const int k = (int)std::min(need, (double)(freq - phase));
+      const int cs = launch_cycles(ctx, *B, 0, k, phase, freq, 0);
+      launch_close(ctx, *B, cs);
+      launched += k;
+      const hipError_t le = hipGetLastError();
+      if (le != hipSuccess) {
+        ctx->err = std::string("kernel launch failed: ") + hipGetErrorString(le);
+        return DNLG_E_HIP;
+      }
+      continue;
+    }
     int chunk = (int)std::min(need, 4096.0);
     if (B->hist_cap > 0) {
       // at most one history row per cycle and file between two drains
