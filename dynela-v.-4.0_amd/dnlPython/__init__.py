@@ -18,6 +18,7 @@ Errors never exit the interpreter: engine statuses become exceptions
 """
 import ctypes
 import warnings
+import weakref
 
 import numpy as np
 
@@ -398,6 +399,7 @@ class Context:
         self.handle = ctypes.c_void_p()
         _lib.check(self.lib.dnlg_ctx_create(int(device), ctypes.byref(self.handle)))
         self.device = device
+        self._models = weakref.WeakSet()  # models whose engine job lives in this context
 
     @classmethod
     def default(cls, device=0):
@@ -421,7 +423,11 @@ class Context:
         return ms.value, n.value
 
     def close(self):
+        """Destroys the context; the engine jobs of its models are released
+        first (a job may not outlive its context)."""
         if self.handle:
+            for m in list(self._models):
+                m.close()
             self.lib.dnlg_ctx_destroy(self.handle)
             self.handle = ctypes.c_void_p()
 
@@ -835,6 +841,7 @@ class DynELA:
         ctx.check(ctx.lib.dnlg_job_create(ctx.handle, ctypes.byref(a.mesh), ctypes.byref(a.material),
                                           ctypes.byref(a.bcs), ctypes.byref(a.params), ctypes.byref(job)))
         self._job = job
+        ctx._models.add(self)
         for h in self._histories:
             h._attach(self)
         if self.saveTimes is not None:
@@ -874,7 +881,8 @@ class DynELA:
 
     def close(self):
         if self._job is not None:
-            self.context.lib.dnlg_job_destroy(self._job)
+            if self.context.handle:
+                self.context.lib.dnlg_job_destroy(self._job)
             self._job = None
 
     def __del__(self):
@@ -1111,6 +1119,7 @@ class Server:
         if job.value:
             model._job = job
             model._ctx = self.ctx
+            self.ctx._models.add(model)
         elif status != _lib.OK:
             self.ctx.check(status)
         return model, status

@@ -196,3 +196,31 @@ def test_necking_axi_descriptors():
     ptr, idx = a.const_ptr, a.const_idx
     top = [kinds[i] for i in idx[ptr[8 * 5 + 2]:ptr[8 * 5 + 3]]]
     assert top == [(_lib.BC_SPEED, (0.0, 2.0, 0.0))]
+
+
+def test_context_close_releases_its_models_jobs():
+    """Closing a context first releases the engine jobs of its models (a job
+    may not outlive its context); the models' later close is a no-op."""
+
+    class FakeLib:
+        def __init__(self):
+            self.calls = []
+
+        def dnlg_job_destroy(self, job):
+            self.calls.append(("job", job))
+
+        def dnlg_ctx_destroy(self, h):
+            self.calls.append(("ctx", h))
+
+    ctx = object.__new__(dnl.Context)
+    ctx.lib, ctx.handle = FakeLib(), ctypes.c_void_p(1)
+    import weakref
+    ctx._models = weakref.WeakSet()
+    m = dnl.DynELA("m", context=ctx)
+    m._job = ctypes.c_void_p(7)
+    ctx._models.add(m)
+    ctx.close()
+    assert [c[0] for c in ctx.lib.calls] == ["job", "ctx"]
+    assert m._job is None
+    m.close()
+    assert len(ctx.lib.calls) == 2
