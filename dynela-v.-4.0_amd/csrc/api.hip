@@ -500,8 +500,27 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
     return fail(ctx, DNLG_E_INVALID, "invalid solver parameters");
   if (prm->precision != DNLG_PREC_FP64 && prm->precision != DNLG_PREC_MIXED)
     return fail(ctx, DNLG_E_INVALID, "invalid precision");
-  if (mat->law < DNLG_LAW_JOHNSON_COOK || mat->law > DNLG_LAW_TABULAR)
+  if (mat->law < DNLG_LAW_JOHNSON_COOK || mat->law > DNLG_LAW_ANN2SIG)
     return fail(ctx, DNLG_E_INVALID, "invalid hardening law");
+  if (mat->law == DNLG_LAW_ANN2SIG) {
+    // the packed network: sizes, finite weights, non-empty scaling ranges
+    const double *t = mat->table;
+    if (!t || mat->n_table < 2 || !(t[0] >= 1 && t[0] <= DNLG_ANN_MAX_HIDDEN && t[0] == (int)t[0]) ||
+        !(t[1] >= 1 && t[1] <= DNLG_ANN_MAX_HIDDEN && t[1] == (int)t[1]))
+      return fail(ctx, DNLG_E_INVALID, "a network law needs 1 to 64 neurons per hidden layer");
+    const int h1 = (int)t[0], h2 = (int)t[1];
+    if (mat->n_table != DNLG_ANN_TABLE_SIZE(h1, h2))
+      return fail(ctx, DNLG_E_INVALID, "network law table size does not match its layer sizes");
+    for (int64_t k = 0; k < mat->n_table; k++)
+      if (!std::isfinite(t[k])) return fail(ctx, DNLG_E_INVALID, "network law weights must be finite");
+    const double *tail = t + mat->n_table - 10;  // logBase, min[4], max[4] follow b3
+    for (int k = 0; k < 4; k++)
+      if (!(tail[5 + k] != tail[1 + k]))
+        return fail(ctx, DNLG_E_INVALID, "network law scaling ranges must not be empty");
+    if (!(tail[0] > 0)) return fail(ctx, DNLG_E_INVALID, "network law logBase must be > 0");
+    if (prm->precision != DNLG_PREC_FP64)
+      return fail(ctx, DNLG_E_INVALID, "the network law runs in FP64 precision only");
+  }
   if (mat->law == DNLG_LAW_TABULAR) {
     // DiscreteFunction with at least one segment, strains strictly ascending
     if (!mat->table || mat->n_table < 2 || mat->n_table > (1 << 20))
@@ -741,6 +760,12 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
     }
     double *pt;
     ALLOC(pt, (int64_t)t.size(), t.data());
+    H.ltab = pt;
+    H.n_ltab = (int32_t)mat->n_table;
+  }
+  if (mat->law == DNLG_LAW_ANN2SIG) {
+    double *pt;
+    ALLOC(pt, mat->n_table, mat->table);
     H.ltab = pt;
     H.n_ltab = (int32_t)mat->n_table;
   }

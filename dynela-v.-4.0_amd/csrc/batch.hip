@@ -28,10 +28,17 @@ constexpr int kTypes[kNumTypes] = {DNLG_EL_QUA4N_AX, DNLG_EL_HEX8N_3D, DNLG_EL_Q
 inline int type_dim(int type) { return type == DNLG_EL_HEX8N_3D ? 3 : 2; }
 inline int type_nodes(int type) { return type == DNLG_EL_HEX8N_3D ? 8 : 4; }
 
+// Element kernel variants of a type: FP64, mixed, FP64 with a network law.
+constexpr int kElemVariants = 3;
+inline int elem_variant(const dnlg_job *j) {
+  if (j->prec == DNLG_PREC_MIXED) return 1;
+  return j->host.law == DNLG_LAW_ANN2SIG ? 2 : 0;
+}
+
 struct Batch {
   std::vector<dnlg_job *> jobs;
   JobDev **d_jobs = nullptr;
-  TaskList elem[kNumTypes][2];  // [type][mixed?]
+  TaskList elem[kNumTypes][kElemVariants];  // [type][elem_variant]
   TaskList prep[kNumTypes];     // [type]
   TaskList cour[kNumTypes];     // [type]: Courant lengths and power-iteration products
   TaskList node[2][2];  // [dim==3?][mixed?]
@@ -122,15 +129,15 @@ int build_batch(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, Batch &B) {
   for (int hx = 0; hx < kNumTypes; hx++) {
     const int type = kTypes[hx];
     const int nn = type_nodes(type);
-    for (int mx = 0; mx < 2; mx++) {
+    for (int mx = 0; mx < kElemVariants; mx++) {
       std::vector<int> idx;
       std::vector<int64_t> cnt;
       for (int i = 0; i < n; i++)
-        if (jobs[i]->type == type && (jobs[i]->prec == DNLG_PREC_MIXED) == (mx == 1)) {
+        if (jobs[i]->type == type && elem_variant(jobs[i]) == mx) {
           idx.push_back(i);
           cnt.push_back(jobs[i]->NE);
         }
-      const int threads = mx ? elem_block_threads_mixed(type) : elem_block_threads(type);
+      const int threads = mx == 1 ? elem_block_threads_mixed(type) : elem_block_threads(type);
       if ((st = upload_tasks(B, xcd_order(make_tasks(idx, cnt, threads / nn)), B.elem[hx][mx]))) return st;
     }
     std::vector<int> idx;
@@ -206,21 +213,21 @@ void launch_close(dnlg_ctx *ctx, Batch &B, int cs) {
   launch_cycle_step(B.d_jobs, B.close.d, B.close.n, cs, 0, 0, ctx->stream);
 }
 
-// ev: nullptr, or two events per element launch (up to 2 per element type); each
+// ev: nullptr, or two events per element launch (up to kElemVariants per element type); each
 // launch is bracketed by event records on the engine's stream.
 int launch_element(dnlg_ctx *ctx, Batch &B, const hipEvent_t *ev, int cs, int derive) {
   hipStream_t s = ctx->stream;
   int k = 0;
   for (int hx = 0; hx < kNumTypes; hx++) {
     const int type = kTypes[hx];
-    for (int mx = 0; mx < 2; mx++) {
+    for (int mx = 0; mx < kElemVariants; mx++) {
       const TaskList &L = B.elem[hx][mx];
       if (L.n <= 0) continue;
       const hipEvent_t *e = ev ? &ev[2 * k] : nullptr;
-      if (mx)
+      if (mx == 1)
         launch_elem_mixed(type, B.d_jobs, L.d, L.n, cs, derive, s, e);
       else
-        launch_elem_fp64(type, B.d_jobs, L.d, L.n, cs, derive, s, e);
+        launch_elem_fp64(type, B.d_jobs, L.d, L.n, cs, derive, s, e, mx == 2);
       k++;
     }
   }
@@ -330,6 +337,7 @@ int run_preloop(dnlg_ctx *ctx, Batch &B) {
     const int type = kTypes[hx];
     launch_elem_fp64(type, B.d_jobs, B.elem[hx][0].d, B.elem[hx][0].n, 0, 0, s);
     launch_elem_mixed(type, B.d_jobs, B.elem[hx][1].d, B.elem[hx][1].n, 0, 0, s);
+    launch_elem_fp64(type, B.d_jobs, B.elem[hx][2].d, B.elem[hx][2].n, 0, 0, s, nullptr, true);
   }
   // the pre-loop's history rows (Explicit.C:182)
   launch_history(ctx, B, 0, 1);
@@ -582,7 +590,7 @@ int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vec
       const int per = use_graph ? graph_len(ctx->graph_cycles, phase, freq) : 1;
       const int units = std::min((chunk + per - 1) / per, kTimedChunk);
       launched += (int64_t)units * per - chunk;
-      while (ctx->events.size() < (size_t)4 * kNumTypes * units) {
+      while (ctx->events.size() < (size_t)2 * kElemVariants * kNumTypes * units) {
         hipEvent_t e;
         if (hipEventCreate(&e) != hipSuccess) return DNLG_E_HIP;
         ctx->events.push_back(e);

@@ -14,8 +14,9 @@ int elem_block_threads_mixed(int type);
 // and stores it in slot 1 - cs).  With ev != nullptr the launch records the
 // kernel's start/stop in ev[0..1] (hipExtLaunchKernel), so its duration is
 // measured on the launch stream.
+// ann: the variant for jobs with a network law (DNLG_LAW_ANN2SIG).
 void launch_elem_fp64(int type, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int derive,
-                      hipStream_t s, const hipEvent_t *ev = nullptr);
+                      hipStream_t s, const hipEvent_t *ev = nullptr, bool ann = false);
 void launch_elem_mixed(int type, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int derive,
                        hipStream_t s, const hipEvent_t *ev = nullptr);
 // fold != 0: BC nodes also get the next cycle's field0 BC values (the next

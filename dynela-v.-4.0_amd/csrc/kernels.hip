@@ -171,9 +171,16 @@ __device__ __forceinline__ Control cycle_control(JobDev *J, CJobDev *C, int cs, 
 
 // The hardening law's yield stress (and slope) at (epsp, depsp, T) for the
 // FP64 radial returns: a tabular law through its table (out of it: `ok`
-// cleared), the closed-form laws through material.h.
+// cleared), the closed-form laws through material.h; ANN: the kernel
+// variant for network laws.
+template <bool ANN>
 __device__ __forceinline__ double law_yield_of(CJobDev *C, int law, const double *lp, double epsp, double depsp,
-                                               double T, bool &ok) {
+                                               double T, double dt, bool &ok) {
+  if (ANN) {
+    double y, h;
+    law_ann(C->ltab, epsp, depsp, T, dt, 0.0, false, y, h);
+    return y;
+  }
   if (law == DNLG_LAW_TABULAR) {
     double y = 0.0, h;
     ok = law_tabular(C->ltab, C->n_ltab, epsp, y, h) && ok;
@@ -181,8 +188,13 @@ __device__ __forceinline__ double law_yield_of(CJobDev *C, int law, const double
   }
   return law_yield(law, lp, epsp, depsp, T);
 }
+template <bool ANN>
 __device__ __forceinline__ void law_eval(CJobDev *C, int law, const double *lp, double eta_rcp, double epsp,
                                          double depsp, double T, double dt, double &yield, double &hard, bool &ok) {
+  if (ANN) {
+    law_ann(C->ltab, epsp, depsp, T, dt, eta_rcp, true, yield, hard);
+    return;
+  }
   if (law == DNLG_LAW_TABULAR) {
     yield = hard = 0.0;
     ok = law_tabular(C->ltab, C->n_ltab, epsp, yield, hard) && ok;
@@ -200,7 +212,9 @@ __device__ __forceinline__ void law_eval(CJobDev *C, int law, const double *lp, 
 // ---------------------------------------------------------------------------
 // Three waves per SIMD (168 VGPRs, a short spill); the LDS of a 16-hex block
 // (24 KB) allows three.  Measured 11 % faster than two waves without spill.
-template <int NN, int DIM, int BLOCK, bool AXI>
+// ANN: the variant for jobs whose law is DNLG_LAW_ANN2SIG (its network
+// evaluation keeps per-lane arrays that the other laws' variants do without).
+template <int NN, int DIM, int BLOCK, bool AXI, bool ANN>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void k_elem_fp64(
     JobDev *const *jobs, const Task *tasks, int cs, int derive) {
   constexpr int NIP = NN;
@@ -362,7 +376,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
     for (int k = 0; k < 8; k++) lp[k] = C->lp[k];
     bool in_table = true;
     if (yield == 0.0) {
-      yield = law_yield_of(C, law, lp, kGammaInitial, kGammaInitial / dt, Tcur, in_table);
+      yield = law_yield_of<ANN>(C, law, lp, kGammaInitial, kGammaInitial / dt, Tcur, dt, in_table);
       gbl(C->sy)[g] = yield;
     }
     const bool plastic = Strial > yield;
@@ -371,10 +385,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
       // return from the stored yield stress and the slope at the stored
       // state (plastic strain, or gammaInitial before the first yield)
       double y0, hard;
-      law_eval(C, law, lp, heatFrac, ep0 > 0.0 ? ep0 : kGammaInitial, gbl(C->dep)[g], Tcur, dt, y0, hard, in_table);
+      law_eval<ANN>(C, law, lp, heatFrac, ep0 > 0.0 ? ep0 : kGammaInitial, gbl(C->dep)[g], Tcur, dt, y0, hard,
+                    in_table);
       gamma = (Snorm - kSqrt23 * yield) / (TwoG * (1.0 + hard / (1.5 * TwoG)));
       const double T = Tcur + 0.5 * gamma * heatFrac * (kSqrt23 * yield + Snorm0);
-      yield = law_yield_of(C, law, lp, ep0 + kSqrt23 * gamma, kSqrt23 * gamma / dt, T, in_table);
+      yield = law_yield_of<ANN>(C, law, lp, ep0 + kSqrt23 * gamma, kSqrt23 * gamma / dt, T, dt, in_table);
     } else if (plastic) {
       double gmin = 0.0, gmax = (Strial - yield) / TwoG32;
       gamma = gbl(C->gam)[g];
@@ -386,7 +401,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
       int iterate = 0;
       while (true) {
         double hard;
-        law_eval(C, law, lp, eta_rcp, epsp, depsp, T, dt, yield, hard, in_table);
+        law_eval<ANN>(C, law, lp, eta_rcp, epsp, depsp, T, dt, yield, hard, in_table);
         if (!in_table) break;
         const double fun = Strial - gamma * TwoG32 - yield;
         if (fun < 0.0)
@@ -1295,15 +1310,24 @@ void launch_elem_mixed(int type, JobDev *const *jobs, const Task *tasks, int nbl
     launch_elem(k_elem_mixed<4, 2, 256, false>, 256, nblk, s, ev, jobs, tasks, cs, derive);
 }
 
-void launch_elem_fp64(int type, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int derive,
-                      hipStream_t s, const hipEvent_t *ev) {
-  if (nblk <= 0) return;
+template <bool ANN>
+void launch_elem_fp64_law(int type, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int derive,
+                          hipStream_t s, const hipEvent_t *ev) {
   if (type == DNLG_EL_HEX8N_3D)
-    launch_elem(k_elem_fp64<8, 3, 128, false>, 128, nblk, s, ev, jobs, tasks, cs, derive);
+    launch_elem(k_elem_fp64<8, 3, 128, false, ANN>, 128, nblk, s, ev, jobs, tasks, cs, derive);
   else if (type == DNLG_EL_QUA4N_AX)
-    launch_elem(k_elem_fp64<4, 2, 256, true>, 256, nblk, s, ev, jobs, tasks, cs, derive);
+    launch_elem(k_elem_fp64<4, 2, 256, true, ANN>, 256, nblk, s, ev, jobs, tasks, cs, derive);
   else
-    launch_elem(k_elem_fp64<4, 2, 256, false>, 256, nblk, s, ev, jobs, tasks, cs, derive);
+    launch_elem(k_elem_fp64<4, 2, 256, false, ANN>, 256, nblk, s, ev, jobs, tasks, cs, derive);
+}
+
+void launch_elem_fp64(int type, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int derive,
+                      hipStream_t s, const hipEvent_t *ev, bool ann) {
+  if (nblk <= 0) return;
+  if (ann)
+    launch_elem_fp64_law<true>(type, jobs, tasks, nblk, cs, derive, s, ev);
+  else
+    launch_elem_fp64_law<false>(type, jobs, tasks, nblk, cs, derive, s, ev);
 }
 
 void launch_correct(int dim, bool f32, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int fold,

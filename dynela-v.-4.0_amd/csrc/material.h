@@ -82,6 +82,60 @@ __device__ __forceinline__ bool law_tabular(const double *t, int n, double epsp,
   return true;
 }
 
+// ANN2SigLaw::getYieldStress / getDerivateYieldStress (ANN2SigLaw.C:173-245)
+// on the packed network t (layout of DNLG_LAW_ANN2SIG).  Inputs scaled to
+// the training ranges, the rate as log(depsp / logBase) (0 at or below
+// logBase, where the slope's rate term divides by logBase instead of
+// depsp, as in the reference); output scaled back.  The slope is the
+// network's gradient by back-propagation: d/d(ep) + d/d(dep) / dtime +
+// eta / (rho Cp) * yield * d/dT.  The first hidden layer's exp(-z) and
+// the second's go to per-lane arrays (private memory) of
+// DNLG_ANN_MAX_HIDDEN entries.
+__device__ __noinline__ void law_ann(const double *t, double epsp, double depsp, double T, double dtime,
+                                     double eta_rcp, bool slope, double &yield, double &hard) {
+  const DNLG_GLOBAL double *p = gbl(t);
+  const int h1 = (int)p[0], h2 = (int)p[1];
+  const DNLG_GLOBAL double *w1 = p + 2, *b1 = w1 + 3 * h1, *w2 = b1 + h1, *b2 = w2 + h2 * h1, *w3 = b2 + h2;
+  const DNLG_GLOBAL double *tail = w3 + h2;  // b3, logBase, min[4], max[4]
+  const double b3 = tail[0], logBase = tail[1];
+  const DNLG_GLOBAL double *mn = tail + 2, *mx = tail + 6;
+  const double r0 = mx[0] - mn[0], r1 = mx[1] - mn[1], r2 = mx[2] - mn[2], r3 = mx[3] - mn[3];
+  double x0 = (epsp - mn[0]) / r0, x1 = 0.0;
+  if (depsp > logBase)
+    x1 = (log(depsp / logBase) - mn[1]) / r1;
+  else
+    depsp = logBase;
+  const double x2 = (T - mn[2]) / r2;
+  double za[DNLG_ANN_MAX_HIDDEN], zc[DNLG_ANN_MAX_HIDDEN];
+  for (int i = 0; i < h1; i++) za[i] = exp(-(w1[3 * i] * x0 + w1[3 * i + 1] * x1 + w1[3 * i + 2] * x2 + b1[i]));
+  double y = 0.0;
+  for (int j = 0; j < h2; j++) {
+    double z = 0.0;
+    for (int i = 0; i < h1; i++) z += w2[j * h1 + i] * (1.0 / (za[i] + 1.0));
+    zc[j] = exp(-(z + b2[j]));
+    y += w3[j] * (1.0 / (zc[j] + 1.0));
+  }
+  y += b3;
+  yield = r3 * y + mn[3];
+  if (!slope) return;
+  // back-propagation: zc becomes dy/d(layer-2 input), zf is dy/d(layer-1 input)
+  for (int j = 0; j < h2; j++) {
+    const double c1 = zc[j] + 1.0;
+    zc[j] = (1.0 / (c1 * c1)) * zc[j] * w3[j];
+  }
+  double yd0 = 0.0, yd1 = 0.0, yd2 = 0.0;
+  for (int i = 0; i < h1; i++) {
+    double g = 0.0;
+    for (int j = 0; j < h2; j++) g += w2[j * h1 + i] * zc[j];
+    const double b = za[i] + 1.0;
+    const double zf = g * ((1.0 / (b * b)) * za[i]);
+    yd0 += w1[3 * i] * zf;
+    yd1 += w1[3 * i + 1] * zf;
+    yd2 += w1[3 * i + 2] * zf;
+  }
+  hard = r3 / r0 * yd0 + (r3 / r1 * yd1 / depsp) / dtime + eta_rcp * yield * (r3 / r2 * yd2);
+}
+
 // FP32 law evaluation of the mixed path's radial return.  Same yield stress
 // and slope as law_yield_and_slope, written in terms of gamma (depsp =
 // sqrt(2/3) gamma / dt, so depsp * dt = sqrt(2/3) gamma), with powers and

@@ -159,6 +159,15 @@ int dnlg_server_submit(dnlg_server *s, const dnlg_mesh *mesh, const dnlg_materia
     j->table.assign(mat->table, mat->table + 2 * mat->n_table);
     j->mat.table = j->table.data();
   }
+  if (mat->law == DNLG_LAW_ANN2SIG) {
+    // oversized tables are not kept; job creation then rejects the law
+    if (mat->table && mat->n_table > 0 && mat->n_table <= (1 << 20)) {
+      j->table.assign(mat->table, mat->table + mat->n_table);
+      j->mat.table = j->table.data();
+    } else {
+      j->mat.table = nullptr;
+    }
+  }
   j->prm = *prm;
   if (bcs && bcs->n_boundaries > 0) {
     if (!bcs->boundaries) return srv_fail(s, DNLG_E_INVALID, "boundaries missing");

@@ -27,7 +27,7 @@ from ._lib import (DnlgError, NegativeJacobianError, NewtonRaphsonError, PowerIt
                    NonFiniteError)
 
 __all__ = ["DynELA", "NodeSet", "ElementSet", "Material", "JohnsonCookLaw", "ElastoplasticLaw", "HistoryFile",
-           "TabularLaw", "DiscreteFunction",
+           "TabularLaw", "DiscreteFunction", "ANN2SigLaw",
            "VtkInterface", "SvgInterface",
            "VoceLinearLaw", "BoundaryRestrain", "BoundarySpeed", "RampFunction", "Explicit", "Field",
            "Element", "Vec3D", "Context", "DnlgError", "NegativeJacobianError", "NewtonRaphsonError",
@@ -229,6 +229,60 @@ class TabularLaw:
         return self._function
 
 
+class ANN2SigLaw:
+    """ANN2SigLaw (ANN2SigLaw.C): yield stress from a network of 3 inputs
+    (plastic strain, log plastic strain rate, temperature), two sigmoid
+    hidden layers and one linear output, read by setParameters(filename)
+    from a NumPy .npz file with the arrays w1 (3, h1), b1 (h1, 1),
+    w2 (h1, h2), b2 (h2, 1), w3 (h2, 1), b3 (1, 1), logBase,
+    minEntries (4) and maxEntries (4) (ANN2SigLaw.C:137-158: the weight
+    matrices are transposed on reading, the biases are their first
+    column).  FP64 precision only."""
+    law = _lib.LAW_ANN2SIG
+    _NAMES = ("w1", "b1", "w2", "b2", "w3", "b3", "logBase", "minEntries", "maxEntries")
+
+    def __init__(self, name=""):
+        self.name = name
+        self.p = [0.0] * 8
+        self._table = None
+
+    def getNumberOfParameters(self):
+        return len(self._NAMES)
+
+    def getParameterName(self, parameter):
+        return self._NAMES[parameter]
+
+    def setParameters(self, filename):
+        with np.load(str(filename), allow_pickle=False) as z:
+            missing = [k for k in self._NAMES if k not in z.files]
+            if missing:
+                raise DnlgError(_lib.E_INVALID, f"{filename}: no array {missing[0]!r}")
+            arrays = {k: np.asarray(z[k], dtype=np.float64) for k in self._NAMES}
+
+        def matrix(k):
+            a = arrays[k]
+            if a.ndim != 2:
+                raise DnlgError(_lib.E_INVALID, f"{filename}: {k} must be a 2-D array")
+            return a.T
+
+        def column(k):
+            a = arrays[k]
+            return a[:, 0] if a.ndim == 2 else a.reshape(-1)
+
+        w1, w2, w3 = matrix("w1"), matrix("w2"), matrix("w3")
+        b1, b2, b3 = column("b1"), column("b2"), column("b3")
+        h1, h2 = w1.shape[0], w2.shape[0]
+        if (w1.shape[1] != 3 or w2.shape[1] != h1 or w3.shape[1] != h2 or len(b1) != h1 or len(b2) != h2
+                or len(b3) < 1 or not 1 <= h1 <= _lib.ANN_MAX_HIDDEN or not 1 <= h2 <= _lib.ANN_MAX_HIDDEN):
+            raise DnlgError(_lib.E_INVALID, f"{filename}: inconsistent network shapes")
+        base = arrays["logBase"].reshape(-1)
+        mn, mx = arrays["minEntries"].reshape(-1), arrays["maxEntries"].reshape(-1)
+        if len(base) < 1 or len(mn) < 4 or len(mx) < 4:
+            raise DnlgError(_lib.E_INVALID, f"{filename}: logBase, minEntries or maxEntries too short")
+        self._table = np.concatenate([[h1, h2], w1.ravel(), b1, w2.ravel(), b2, w3[0], b3[:1], base[:1], mn[:4],
+                                      mx[:4]]).astype(np.float64)
+
+
 class Material:
     """Material attributes of Material.h:49-56."""
 
@@ -272,6 +326,12 @@ class Material:
             d._table = np.ascontiguousarray(np.array(f._pts, dtype=np.float64).reshape(-1))
             d.table = _lib.as_dp(d._table)
             d.n_table = len(f._pts)
+        if isinstance(self._law, ANN2SigLaw):
+            if self._law._table is None:
+                raise DnlgError(_lib.E_INVALID, f"network law {self._law.name!r} has no parameters")
+            d._table = np.ascontiguousarray(self._law._table)
+            d.table = _lib.as_dp(d._table)
+            d.n_table = len(d._table)
         return d
 
 

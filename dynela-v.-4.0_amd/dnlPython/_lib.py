@@ -26,7 +26,8 @@ STATUS_NAMES = {
 EL_QUA4N_2D = 1
 EL_QUA4N_AX = 3
 EL_HEX8N_3D = 4
-LAW_JOHNSON_COOK, LAW_POWER, LAW_VOCE_LINEAR, LAW_TABULAR = 0, 1, 2, 3
+LAW_JOHNSON_COOK, LAW_POWER, LAW_VOCE_LINEAR, LAW_TABULAR, LAW_ANN2SIG = 0, 1, 2, 3, 4
+ANN_MAX_HIDDEN = 64
 BC_RESTRAIN, BC_SPEED = 0, 1
 RAMP_NONE, RAMP_LINEAR, RAMP_SINUS, RAMP_CONSTANT, RAMP_ERF, RAMP_SOFT = range(6)
 DT_COURANT, DT_POWER_ITERATION, DT_POWER_ITERATION_UNDER = 0, 1, 2

@@ -61,6 +61,16 @@ extern "C" {
 #define DNLG_LAW_TABULAR 3      /* TabularLaw.C: sy(ep) piecewise linear through the
                                    material's table (DiscreteFunction::getValue /
                                    getSlope, DiscreteFunction.C:188-308); FP64 only */
+#define DNLG_LAW_ANN2SIG 4      /* ANN2SigLaw.C:173-245: sy(ep, dep, T) from a network of
+                                   3 inputs, two sigmoid hidden layers of h1 and h2
+                                   neurons and one linear output; FP64 only.  The
+                                   material's table holds the network, packed as
+                                   h1, h2, w1[h1][3], b1[h1], w2[h2][h1], b2[h2],
+                                   w3[h2], b3, logBase, minEntries[4], maxEntries[4]
+                                   (the reference's matrices after its transposes,
+                                   ANN2SigLaw.C:137-158, row-major) */
+#define DNLG_ANN_MAX_HIDDEN 64  /* neurons per hidden layer, at most */
+#define DNLG_ANN_TABLE_SIZE(h1, h2) (12 + 4 * (h1) + (h1) * (h2) + 2 * (h2))
 
 /* boundary kinds (Boundary.h:59-62) */
 #define DNLG_BC_RESTRAIN 0 /* BoundaryRestrain.C:98-150 */
@@ -129,7 +139,8 @@ typedef struct dnlg_material {
   int32_t reserved;
   double p[8];           /* law parameters, see DNLG_LAW_* */
   const double *table;   /* DNLG_LAW_TABULAR: n_table (plastic strain, yield
-                            stress) pairs, strains ascending; else NULL */
+                            stress) pairs, strains ascending; DNLG_LAW_ANN2SIG:
+                            the packed network, n_table doubles; else NULL */
   int64_t n_table;
 } dnlg_material;
 

@@ -22,7 +22,11 @@
  *    finite-difference check; the radial return vs a brute-force bisection;
  *  - Chung-Hulbert constants vs closed forms; lumped mass vs rho*V;
  *    Courant lengths vs box/rectangle closed forms; free flight of a body
- *    with f = 0; elastic simple shear vs the Green-Naghdi closed form.
+ *    with f = 0; elastic simple shear vs the Green-Naghdi closed form;
+ *  - the ANN2Sig network law: its yield stress vs an independent forward
+ *    pass on the weights as stored in the law's .npz file, its slope vs
+ *    central differences of the yield stress, and a constant network's
+ *    solve vs the same solve with a constant power law.
  *  Parity unpinned: the CBLAS dgemm summation order behind dN/dx
  *  (Matrix.C:815) -- a plain k-ordered sum is used.
  *
@@ -528,6 +532,7 @@ struct Law {
   double p[8];
   double eta, rho, cp; /* Taylor-Quinney, density, heat capacity */
   std::vector<double> tx, ty; /* DNLG_LAW_TABULAR points */
+  std::vector<double> ann;    /* DNLG_LAW_ANN2SIG packed network */
 };
 
 Law law_of(const dnlg_material *m) {
@@ -538,7 +543,66 @@ Law law_of(const dnlg_material *m) {
       L.tx.push_back(m->table[2 * k]);
       L.ty.push_back(m->table[2 * k + 1]);
     }
+  if (m->law == DNLG_LAW_ANN2SIG && m->table) L.ann.assign(m->table, m->table + m->n_table);
   return L;
+}
+
+/* ANN2SigLaw::getYieldStress / getDerivateYieldStress (ANN2SigLaw.C:173-245),
+ * layer by layer as the reference's vector expressions: inputs scaled by
+ * minEntries / rangeEntries (rate: log(depsp / logBase), 0 at or below
+ * logBase, where depsp becomes logBase), a = exp(-(w1 x + b1)),
+ * s1 = 1 / (1 + a), c = exp(-(w2 s1 + b2)), y = w3 / (1 + c) + b3,
+ * yield = range3 y + min3; the slope back-propagates
+ * d = w3 c / (1 + c)^2, f = (w2^T d) a / (1 + a)^2, yd = w1^T f. */
+double ann(const Law &L, double epsp, double depsp, double T, double dtime, bool slope) {
+  const double *t = L.ann.data();
+  const int h1 = (int)t[0], h2 = (int)t[1];
+  const double *w1 = t + 2, *b1 = w1 + 3 * h1, *w2 = b1 + h1, *b2 = w2 + h2 * h1, *w3 = b2 + h2;
+  const double b3 = w3[h2], logBase = w3[h2 + 1];
+  const double *mn = w3 + h2 + 2, *mx = mn + 4;
+  double range[4];
+  for (int k = 0; k < 4; k++) range[k] = mx[k] - mn[k];
+  double in[3];
+  in[0] = (epsp - mn[0]) / range[0];
+  if (depsp > logBase) {
+    in[1] = (std::log(depsp / logBase) - mn[1]) / range[1];
+  } else {
+    in[1] = 0.0;
+    depsp = logBase;
+  }
+  in[2] = (T - mn[2]) / range[2];
+  std::vector<double> a(h1), s1(h1), c(h2), s2(h2);
+  for (int i = 0; i < h1; i++) {
+    double z = 0.0;
+    for (int k = 0; k < 3; k++) z += w1[3 * i + k] * in[k];
+    a[i] = std::exp(-(z + b1[i]));
+    s1[i] = 1.0 / (a[i] + 1.0);
+  }
+  double y = 0.0;
+  for (int j = 0; j < h2; j++) {
+    double z = 0.0;
+    for (int i = 0; i < h1; i++) z += w2[j * h1 + i] * s1[i];
+    c[j] = std::exp(-(z + b2[j]));
+    s2[j] = 1.0 / (c[j] + 1.0);
+    y += w3[j] * s2[j];
+  }
+  y += b3;
+  const double yield = range[3] * y + mn[3];
+  if (!slope) return yield;
+  std::vector<double> d(h2), f(h1);
+  for (int j = 0; j < h2; j++) d[j] = 1.0 / ((c[j] + 1.0) * (c[j] + 1.0)) * c[j] * w3[j];
+  for (int i = 0; i < h1; i++) {
+    double g = 0.0;
+    for (int j = 0; j < h2; j++) g += w2[j * h1 + i] * d[j];
+    f[i] = g * (1.0 / ((a[i] + 1.0) * (a[i] + 1.0)) * a[i]);
+  }
+  double yd[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < 3; k++)
+    for (int i = 0; i < h1; i++) yd[k] += w1[3 * i + k] * f[i];
+  const double dep = range[3] / range[0] * yd[0];
+  const double drate = range[3] / range[1] * yd[1] / depsp;
+  const double dT = range[3] / range[2] * yd[2];
+  return dep + drate / dtime + L.eta / (L.rho * L.cp) * yield * dT;
 }
 
 /* DiscreteFunction::getValue / getSlope (DiscreteFunction.C:188-216,
@@ -560,6 +624,7 @@ double tabular(const Law &L, double x, bool slope) {
  * ElastoplasticLaw::getYieldStress (ElastoplasticLaw.C:110-114). */
 double law_yield(const Law &L, double epsp, double depsp, double T) {
   if (L.kind == DNLG_LAW_TABULAR) return tabular(L, epsp, false);
+  if (L.kind == DNLG_LAW_ANN2SIG) return ann(L, epsp, depsp, T, 0.0, false); /* dtime: slope only */
   if (L.kind == DNLG_LAW_JOHNSON_COOK) {
     const double A = L.p[0], B = L.p[1], C = L.p[2], n = L.p[3], m = L.p[4], d0 = L.p[5], Tm = L.p[6],
                  T0 = L.p[7];
@@ -587,6 +652,7 @@ double law_yield(const Law &L, double epsp, double depsp, double T) {
  * ElastoplasticLaw::getDerivateYieldStress (ElastoplasticLaw.C:117-121). */
 double law_dyield(const Law &L, double epsp, double depsp, double T, double dtime) {
   if (L.kind == DNLG_LAW_TABULAR) return tabular(L, epsp, true);
+  if (L.kind == DNLG_LAW_ANN2SIG) return ann(L, epsp, depsp, T, dtime, true);
   if (L.kind == DNLG_LAW_JOHNSON_COOK) {
     const double A = L.p[0], B = L.p[1], C = L.p[2], n = L.p[3], m = L.p[4], d0 = L.p[5], Tm = L.p[6],
                  T0 = L.p[7];

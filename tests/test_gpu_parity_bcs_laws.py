@@ -152,3 +152,54 @@ def test_tabular_law_errors(ctx):
     m._invalidate()
     with pytest.raises(dnl.DnlgError):
         m.solve()
+
+
+@pytest.mark.parametrize("kind", ["axi", "3d"])
+def test_ann_law(ctx, tmp_path, kind):
+    """ANN2SigLaw (ANN2SigLaw.C:173-245): a rate- and temperature-dependent
+    network of 7 and 5 neurons, against the oracle."""
+    import dnlPython as dnl
+    from dnlPython import samples
+    from helpers import ann_law, ann_network
+    if kind == "axi":
+        m = samples.taylor_axi(nr=5, nz=25, t_end=10e-6, context=ctx)
+    else:
+        m = samples.taylor_3d(n=4, nz=12, t_end=10e-6, context=ctx)
+    m._material.setHardeningLaw(ann_law(tmp_path / "net.npz", ann_network(seed=5)))
+    m._invalidate()
+    o = run_both(m, 10e-6)
+    compare(m, o)
+    assert np.max(m.getIntPointField(dnl.Field.plasticStrain)) > 0.05
+
+
+def test_ann_law_in_a_batch_with_another_law(ctx, tmp_path):
+    """One batch, one network-law job (its own element kernel variant) and
+    one Johnson-Cook job of the same element type: each matches its oracle
+    run, the Johnson-Cook job bit for bit its own solo run."""
+    import dnlPython as dnl
+    from dnlPython import samples
+    from helpers import ann_law, ann_network
+    from oracle.bridge import OracleJob
+    a = samples.taylor_axi(nr=5, nz=25, t_end=10e-6, context=ctx)
+    a._material.setHardeningLaw(ann_law(tmp_path / "net.npz", ann_network(seed=9, h1=15, h2=7)))
+    a._invalidate()
+    b = samples.taylor_axi(nr=4, nz=30, t_end=10e-6, context=ctx)
+    solo = samples.taylor_axi(nr=4, nz=30, t_end=10e-6, context=ctx)
+    oa, ob = OracleJob(a.descriptors()), OracleJob(b.descriptors())
+    assert oa.solve(10e-6) == 0 and ob.solve(10e-6) == 0
+    dnl.batch_solve([a, b])
+    solo.solve()
+    compare(a, oa)
+    compare(b, ob)
+    np.testing.assert_array_equal(b.getNodalDisplacements(), solo.getNodalDisplacements())
+
+
+def test_ann_law_needs_fp64(ctx, tmp_path):
+    import dnlPython as dnl
+    from dnlPython import samples
+    from helpers import ann_law, ann_network
+    m = samples.taylor_axi(nr=3, nz=8, precision="mixed", context=ctx)
+    m._material.setHardeningLaw(ann_law(tmp_path / "net.npz", ann_network()))
+    m._invalidate()
+    with pytest.raises(dnl.DnlgError):
+        m.solve()
