@@ -27,7 +27,7 @@ from ._lib import (DnlgError, NegativeJacobianError, NewtonRaphsonError, PowerIt
                    NonFiniteError)
 
 __all__ = ["DynELA", "NodeSet", "ElementSet", "Material", "JohnsonCookLaw", "ElastoplasticLaw", "HistoryFile",
-           "TabularLaw", "DiscreteFunction", "ANN2SigLaw",
+           "TabularLaw", "DiscreteFunction", "ANN2SigLaw", "ANNFortranLaw",
            "VtkInterface", "SvgInterface",
            "VoceLinearLaw", "BoundaryRestrain", "BoundarySpeed", "RampFunction", "Explicit", "Field",
            "Element", "Vec3D", "Context", "DnlgError", "NegativeJacobianError", "NewtonRaphsonError",
@@ -281,6 +281,48 @@ class ANN2SigLaw:
             raise DnlgError(_lib.E_INVALID, f"{filename}: logBase, minEntries or maxEntries too short")
         self._table = np.concatenate([[h1, h2], w1.ravel(), b1, w2.ravel(), b2, w3[0], b3[:1], base[:1], mn[:4],
                                       mx[:4]]).astype(np.float64)
+
+
+class ANNFortranLaw(ANN2SigLaw):
+    """ANNFortranLaw (ANNFortranLaw.C:76-114, 180-266): the fixed 3-7-4-1
+    sigmoid network written out in the reference's source (yield stress in
+    MPa).  Its formulas are ANN2SigLaw's for this network, so it is that law
+    with these weights.  The constants below keep the reference's form:
+    layer 1 is 1 + exp(A x + a), layer 2 exp(C / zb + c), the output
+    scale * (E / (1 + zc) + e) + offset, inputs (ep, log(rate) / 10.8198,
+    (T - 20) / 480); ANN2SigLaw's exp(-(w x + b)) takes w = -A, b = -a
+    and w = -C, b = -c.  setParameters ignores its argument, as the
+    reference's does."""
+    _A = ((-0.119231261313, -0.173828095198, 1.827572464943), (-0.428523629904, -0.572129368782, 1.296182870865),
+          (132.502944946289, -0.074804469943, 0.372308045626), (-0.238687828183, -0.654297590256, -1.221466302872),
+          (-0.216847836971, -0.142032980919, 3.807989597321), (16.061351776123, -0.11558868736, 0.41205266118),
+          (3.10570526123, -0.138235300779, 0.773996531963))
+    _a = (-1.533527970314, 1.823030114174, 1.611178278923, -0.61216711998, -0.042427442968, 1.614393234253,
+          1.072145581245)
+    _C = ((1.296875834465, 1.162090778351, 10.487031936646, 0.098825834692, 0.730871379375, -4.092911243439,
+           1.342621445656),
+          (-0.416251301765, 1.612317681313, -1.245551109314, 0.976023316383, 0.005203731358, -1.697751402855,
+           -0.834678351879),
+          (-1.942471027374, -0.364910364151, 10.365681648254, -0.165790036321, -0.691674530506, -5.159326076508,
+           2.959228277206),
+          (-1.463814258575, -0.175867184997, 0.67038166523, -0.306437820196, -0.191420003772, 2.523961305618,
+           0.034361626953))
+    _c = (0.492944657803, -0.692925989628, -0.345911204815, 0.274670749903)
+    _E = (-1.342184662819, -1.402877211571, 1.547661781311, 0.745113432407)
+    _e = -0.319332510233
+    _scale, _offset = 977.555715042962, 579.184642915415
+    _rate_div, _T0, _T_div = 10.819778284410, 20.0, 480.0
+
+    def __init__(self, name=""):
+        super().__init__(name)
+        mn = [0.0, 0.0, self._T0, self._offset]
+        mx = [1.0, self._rate_div, self._T0 + self._T_div, self._offset + self._scale]
+        self._table = np.concatenate([[7, 4], -np.array(self._A).ravel(), -np.array(self._a),
+                                      -np.array(self._C).ravel(), -np.array(self._c), self._E, [self._e, 1.0],
+                                      mn, mx]).astype(np.float64)
+
+    def setParameters(self, filename=None):
+        pass
 
 
 class Material:

@@ -132,3 +132,45 @@ def test_hardening_network_solve(tmp_path):
     assert len(live) > 10
     want = np.array([ann_forward(net, ep[g], dep[g], T[g]) for g in live])
     np.testing.assert_allclose(sy[live], want, rtol=1e-3)
+
+
+def _fortran_form(law, ep, rate, T, dtime):
+    """ANNFortranLaw's own expressions (ANNFortranLaw.C:76-114, 180-266) on
+    the class's constants, which keep the reference's signs: layer 1 as
+    1 + exp(A x + a), layer 2 as exp(C / zb + c), and the slope's rounded
+    constants 90.348959964501 = scale / 10.8198 and 2.036574406340 =
+    scale / 480."""
+    x = np.array([ep, np.log(rate) / law._rate_div if rate > 1 else 0.0, (T - law._T0) / law._T_div])
+    if rate <= 1:
+        rate = 1.0
+    A, a, C, c, E = (np.array(v) for v in (law._A, law._a, law._C, law._c, law._E))
+    za = np.exp(A @ x + a)
+    zb = 1 + za
+    zc = np.exp(C @ (1 / zb) + c)
+    y = E @ (1 / (1 + zc)) + law._e
+    zd = E * (zc / (1 + zc) ** 2)
+    ze = za / zb ** 2
+    zf = (-(C.T @ zd)) * ze
+    yd = -(A.T @ zf)
+    yld = law._scale * y + law._offset
+    hard = (law._scale * yd[0] + 90.348959964501 * yd[1] / rate / dtime
+            + ETA / (RHO * CP) * yld * 2.036574406340 * yd[2])
+    return yld, hard
+
+
+def test_fortran_network_is_the_ann2sig_law_with_its_weights():
+    """ANNFortranLaw packed as an ANN2SigLaw network gives the reference's
+    closed-form yield stress and slope (the sign flips of the packing, the
+    input scalings and the output scale)."""
+    law = dnl.ANNFortranLaw("f")
+    law.setParameters("ignored")
+    d = _material(law)
+    lib = load()
+    rng = np.random.default_rng(7)
+    for _ in range(200):
+        ep, T = rng.uniform(0.0, 1.0), rng.uniform(20.0, 500.0)
+        rate = float(np.exp(rng.uniform(-2.0, 10.0)))
+        dtime = float(10 ** rng.uniform(-9, -6))
+        y, h = _fortran_form(law, ep, rate, T, dtime)
+        assert lib.oracle_law_yield(ctypes.byref(d), ep, rate, T) == pytest.approx(y, rel=1e-13)
+        assert lib.oracle_law_dyield(ctypes.byref(d), ep, rate, T, dtime) == pytest.approx(h, rel=1e-9, abs=1e-9)
