@@ -155,18 +155,19 @@ int dnlg_server_submit(dnlg_server *s, const dnlg_mesh *mesh, const dnlg_materia
   j->coords.assign(mesh->coords, mesh->coords + (size_t)dim * mesh->n_nodes);
   j->conn.assign(mesh->conn, mesh->conn + (size_t)nn * mesh->n_elems);
   j->mat = *mat;
-  if (mat->law == DNLG_LAW_TABULAR && mat->table && mat->n_table > 0) {
-    j->table.assign(mat->table, mat->table + 2 * mat->n_table);
-    j->mat.table = j->table.data();
-  }
-  if (mat->law == DNLG_LAW_ANN2SIG) {
-    // oversized tables are not kept; job creation then rejects the law
-    if (mat->table && mat->n_table > 0 && mat->n_table <= (1 << 20)) {
-      j->table.assign(mat->table, mat->table + mat->n_table);
+  if (mat->law == DNLG_LAW_TABULAR || mat->law == DNLG_LAW_ANN2SIG) {
+    // the job keeps its own copy of the table (point pairs or the packed
+    // network); an oversized one is not kept, and job creation then
+    // rejects the law
+    const int64_t n = mat->law == DNLG_LAW_TABULAR ? 2 * mat->n_table : mat->n_table;
+    if (mat->table && n > 0 && n <= (int64_t)2 << 20) {
+      j->table.assign(mat->table, mat->table + n);
       j->mat.table = j->table.data();
     } else {
       j->mat.table = nullptr;
     }
+  } else {
+    j->mat.table = nullptr;
   }
   j->prm = *prm;
   if (bcs && bcs->n_boundaries > 0) {

@@ -108,3 +108,49 @@ def test_server_admission_by_footprint(ctx):
     with pytest.raises(dnl.DnlgError):
         srv.submit(big)
     srv.close()
+
+
+def test_server_table_laws(ctx, tmp_path):
+    """Jobs whose law carries a table (tabular points, a packed network)
+    keep the server's own copy of it: after submission the caller's
+    descriptors are dropped, and the results equal the jobs solved alone."""
+    import gc
+
+    import dnlPython as dnl
+    from dnlPython import samples
+    from helpers import ann_law, ann_network
+
+    def tabular():
+        f = dnl.DiscreteFunction("t")
+        for x in np.concatenate([[0.0], np.geomspace(1e-9, 5.0, 200)]):
+            f.add(x, 90e6 + 292e6 * x ** 0.31)
+        law = dnl.TabularLaw("tab")
+        law.setFunction(f)
+        return law
+
+    def make(kind):
+        m = samples.taylor_axi(5, 25, speed=220.0, t_end=8e-6, context=ctx)
+        m._material.setHardeningLaw(tabular() if kind == "tab" else ann_law(tmp_path / "n.npz", ann_network(seed=5)))
+        m._invalidate()
+        return m
+
+    alone = [make("tab"), make("ann")]
+    for m in alone:
+        m.solve()
+    srv = dnl.Server(ctx)
+    tickets = [srv.submit(make("tab")), srv.submit(make("ann"))]
+    gc.collect()
+    out = {}
+    for _ in range(100000):
+        srv.step(16)
+        for t in tickets:
+            if t not in out and srv.poll(t)[0] == dnl.Server.DONE:
+                out[t] = srv.take(t)
+        if len(out) == len(tickets):
+            break
+    assert len(out) == len(tickets)
+    for t, a in zip(tickets, alone):
+        m, status = out[t]
+        assert status == 0
+        _same(a, m)
+    srv.close()
