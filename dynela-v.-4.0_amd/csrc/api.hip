@@ -513,7 +513,7 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
       return fail(ctx, DNLG_E_INVALID, "network law table size does not match its layer sizes");
     for (int64_t k = 0; k < mat->n_table; k++)
       if (!std::isfinite(t[k])) return fail(ctx, DNLG_E_INVALID, "network law weights must be finite");
-    const double *tail = t + mat->n_table - 10;  // logBase, min[4], max[4] follow b3
+    const double *tail = t + mat->n_table - 9;  // logBase, min[4], max[4] (the last 9 entries)
     for (int k = 0; k < 4; k++)
       if (!(tail[5 + k] != tail[1 + k]))
         return fail(ctx, DNLG_E_INVALID, "network law scaling ranges must not be empty");

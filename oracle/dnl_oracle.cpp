@@ -555,8 +555,11 @@ Law law_of(const dnlg_material *m) {
  * yield = range3 y + min3; the slope back-propagates
  * d = w3 c / (1 + c)^2, f = (w2^T d) a / (1 + a)^2, yd = w1^T f. */
 double ann(const Law &L, double epsp, double depsp, double T, double dtime, bool slope) {
+  if (L.ann.size() < 2) return std::nan("");
   const double *t = L.ann.data();
   const int h1 = (int)t[0], h2 = (int)t[1];
+  if (h1 < 1 || h2 < 1 || (int64_t)L.ann.size() != DNLG_ANN_TABLE_SIZE((int64_t)h1, (int64_t)h2))
+    return std::nan("");
   const double *w1 = t + 2, *b1 = w1 + 3 * h1, *w2 = b1 + h1, *b2 = w2 + h2 * h1, *w3 = b2 + h2;
   const double b3 = w3[h2], logBase = w3[h2 + 1];
   const double *mn = w3 + h2 + 2, *mx = mn + 4;

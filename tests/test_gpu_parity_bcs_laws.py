@@ -194,12 +194,55 @@ def test_ann_law_in_a_batch_with_another_law(ctx, tmp_path):
     np.testing.assert_array_equal(b.getNodalDisplacements(), solo.getNodalDisplacements())
 
 
-def test_ann_law_needs_fp64(ctx, tmp_path):
+def test_ann_law_errors(ctx, tmp_path):
+    """Job creation refuses mixed precision for a network law, and packed
+    networks whose sizes, table length or scaling ranges are invalid."""
+    import dnlPython as dnl
+    from dnlPython import _lib, samples
+    from helpers import ann_law, ann_network
+
+    def attempt(precision="fp64", edit=None):
+        m = samples.taylor_axi(nr=3, nz=8, precision=precision, context=ctx)
+        law = ann_law(tmp_path / "net.npz", ann_network())
+        if edit is not None:
+            law._table = edit(law._table.copy())
+        m._material.setHardeningLaw(law)
+        m._invalidate()
+        with pytest.raises(dnl.DnlgError) as e:
+            m.solve()
+        assert e.value.status == _lib.E_INVALID
+
+    attempt("mixed")
+
+    def too_wide(t):
+        t[0] = 65
+        return t
+
+    def short(t):
+        return t[:-1]
+
+    def empty_range(t):
+        t[-1] = t[-5]  # maxEntries[3] = minEntries[3]
+        return t
+
+    def bad_base(t):
+        t[-9] = 0.0  # logBase
+        return t
+
+    for edit in (too_wide, short, empty_range, bad_base):
+        attempt(edit=edit)
+
+
+def test_ann_fortran_law(ctx):
+    """ANNFortranLaw (the fixed 3-7-4-1 network, yield stress in MPa; the
+    SI copper bar at 0.05 m/s is then plastic from the first cycles)
+    against the oracle: its output bias is negative, which the packed
+    network's validation must accept."""
     import dnlPython as dnl
     from dnlPython import samples
-    from helpers import ann_law, ann_network
-    m = samples.taylor_axi(nr=3, nz=8, precision="mixed", context=ctx)
-    m._material.setHardeningLaw(ann_law(tmp_path / "net.npz", ann_network()))
+    m = samples.taylor_axi(nr=5, nz=25, speed=0.05, t_end=2e-6, context=ctx)
+    m._material.setHardeningLaw(dnl.ANNFortranLaw("f"))
     m._invalidate()
-    with pytest.raises(dnl.DnlgError):
-        m.solve()
+    o = run_both(m, 2e-6)
+    compare(m, o)
+    assert np.max(m.getIntPointField(dnl.Field.plasticStrain)) > 1e-5
