@@ -17,6 +17,7 @@ Errors never exit the interpreter: engine statuses become exceptions
 ``PowerIterationError``, ``DnlgError``).
 """
 import ctypes
+import time
 import warnings
 import weakref
 
@@ -27,7 +28,7 @@ from ._lib import (DnlgError, NegativeJacobianError, NewtonRaphsonError, PowerIt
                    NonFiniteError)
 
 __all__ = ["DynELA", "NodeSet", "ElementSet", "Material", "JohnsonCookLaw", "ElastoplasticLaw", "HistoryFile",
-           "TabularLaw", "DiscreteFunction", "ANN2SigLaw", "ANNFortranLaw",
+           "TabularLaw", "DiscreteFunction", "ANN2SigLaw", "ANNFortranLaw", "ElasticLaw",
            "VtkInterface", "SvgInterface",
            "VoceLinearLaw", "BoundaryRestrain", "BoundarySpeed", "RampFunction", "Explicit", "Field",
            "Element", "Vec3D", "Context", "DnlgError", "NegativeJacobianError", "NewtonRaphsonError",
@@ -152,6 +153,21 @@ class ElastoplasticLaw:
 
     def setParameters(self, A, B, n):
         self.p = [float(A), float(B), float(n)] + [0.0] * 5
+
+
+class ElasticLaw:
+    """ElasticLaw (ElasticLaw.C:67-78): yield stress and slope 0.  The
+    explicit solve's return (Element.C:470-480) does not distinguish it
+    from the other laws, so a material with this law flows at zero
+    deviatoric stress: the power law with A = B = 0."""
+    law = _lib.LAW_POWER
+
+    def __init__(self, name=""):
+        self.name = name
+        self.p = [0.0, 0.0, 1.0] + [0.0] * 5
+
+    def setParameters(self, *args):
+        pass
 
 
 class VoceLinearLaw:
@@ -561,6 +577,34 @@ class _NodeView:
         return float(self._m.getNodalMasses()[self._m._node_index(self.number)])
 
 
+class _ElementView:
+    """Element proxy for ``getElementByNum``: its number, node numbers in
+    connectivity order, and integration-point values
+    (Element::getIntPointValue, Element.C:1195-1238)."""
+
+    def __init__(self, model, number):
+        self._m, self.number = model, number
+        self._index = model._element_index(number)
+
+    @property
+    def nodes(self):
+        m = self._m
+        return [int(m._node_sorted[k]) for k in m._arrays.conn[self._index]]
+
+    def getNumberOfNodes(self):
+        return NODES_PER_ELEMENT[self._m._elem_type]
+
+    def getNumberOfIntegrationPoints(self):
+        return NODES_PER_ELEMENT[self._m._elem_type]
+
+    def getIntPointValue(self, field, ip):
+        """A scalar integration-point field at IP `ip` of this element."""
+        v = self._m.getIntPointField(field)
+        if v.ndim != 2:
+            raise DnlgError(_lib.E_INVALID, "field must be scalar")
+        return float(v[self._index, ip])
+
+
 class _Parallel:
     """Parallel (Parallel.h:45-80): core count is irrelevant on the GPU."""
 
@@ -731,6 +775,8 @@ class DynELA:
         self._elem_conn = []
         self._default_element = None
         self._elem_type = None
+        self._solve_seconds = 0.0  # wall time inside solve calls (displayEstimatedEnd)
+        self._last_display = (0.0, 0.0)
         self._material = None
         self._solver = None
         self._initial = []   # (boundary, node numbers) in attach order
@@ -966,7 +1012,11 @@ class DynELA:
         if self.saveTimes is not None and not self._results_started:
             self._results_started = True
             self.writeResultFile()
-        ctx.check(ctx.lib.dnlg_job_solve(self._job, ctypes.c_double(t)))
+        t0 = time.perf_counter()
+        try:
+            ctx.check(ctx.lib.dnlg_job_solve(self._job, ctypes.c_double(t)))
+        finally:
+            self._solve_seconds += time.perf_counter() - t0
         if self.saveTimes is not None and self.getCurrentTime() >= self._solver.endTime:
             self.writeVTKFile()
         return True
@@ -1074,6 +1124,33 @@ class DynELA:
 
     def getNodeByNum(self, number):
         return _NodeView(self, number)
+
+    def _element_index(self, number):
+        if self._arrays is None:
+            self.descriptors()
+        return int(self._index_of(self._elem_sorted, np.array([number]), "element")[0])
+
+    def getElementByNum(self, number):
+        return _ElementView(self, number)
+
+    def displayEstimatedEnd(self):
+        """DynELA::displayEstimatedEnd (DynELA.C:506-538): the solve calls'
+        wall time so far and the remaining time estimated from the rate
+        since the previous display, in the reference's format."""
+        t = self.getCurrentTime() if self._job is not None else 0.0
+        if t <= 0.0:
+            return
+        elapsed = self._solve_seconds
+        last_wall, last_t = self._last_display
+        remaining = int((elapsed - last_wall) / (t - last_t) * (self._solver.endTime - t)) if t > last_t else 0
+
+        def hms(sec):
+            sec = int(sec)
+            return f"{sec // 3600:02d}:{sec % 3600 // 60:02d}:{sec % 60:02d}"
+
+        print(f"Current computational time {hms(elapsed)}")
+        print(f"Estimated end of computation in {hms(remaining)}")
+        self._last_display = (elapsed, t)
 
     def getNodalField(self, field):
         """Node::fieldScalar (Node.C:311-370) of every node in node-number

@@ -134,3 +134,22 @@ def test_result_files(ctx, tmp_path, monkeypatch):
     m2.solve()
     np.testing.assert_array_equal(m.getNodalDisplacements(), m2.getNodalDisplacements())
     np.testing.assert_array_equal(rows, h2.getRows())
+
+
+def test_element_view_values_and_estimated_end(ctx, capsys):
+    """getElementByNum(n).getIntPointValue(field, ip) is the model's
+    integration-point field at that element and IP; displayEstimatedEnd
+    prints the reference's two lines once time has passed."""
+    import re
+    m = samples.taylor_axi(nr=4, nz=12, t_end=6e-6, context=ctx)
+    m.solve(3e-6)
+    ep = m.getIntPointField(dnl.Field.plasticStrain)
+    nums = np.sort(np.concatenate(m._elem_num))
+    for k in (0, len(nums) // 2, len(nums) - 1):
+        e = m.getElementByNum(int(nums[k]))
+        for ip in range(4):
+            assert e.getIntPointValue(dnl.Field.plasticStrain, ip) == ep[k, ip]
+    m.displayEstimatedEnd()
+    out = capsys.readouterr().out.splitlines()
+    assert re.fullmatch(r"Current computational time \d\d:\d\d:\d\d", out[0])
+    assert re.fullmatch(r"Estimated end of computation in \d\d:\d\d:\d\d", out[1])

@@ -174,3 +174,26 @@ def test_fortran_network_is_the_ann2sig_law_with_its_weights():
         y, h = _fortran_form(law, ep, rate, T, dtime)
         assert lib.oracle_law_yield(ctypes.byref(d), ep, rate, T) == pytest.approx(y, rel=1e-13)
         assert lib.oracle_law_dyield(ctypes.byref(d), ep, rate, T, dtime) == pytest.approx(h, rel=1e-9, abs=1e-9)
+
+
+def test_elastic_law_is_the_zero_yield_return():
+    """ElasticLaw's yield stress and slope are 0 (ElasticLaw.C:67-78) and
+    the return treats it like any law (Element.C:470-480): after the solve
+    every integration point's stress is a pure pressure, to the deviatoric
+    stress 2G x 1e-8 that the return's tolerance on gamma leaves."""
+    m = samples.taylor_axi(nr=4, nz=20, t_end=2e-6)
+    m._material.setHardeningLaw(dnl.ElasticLaw("e"))
+    m._invalidate()
+    d = m._material.descriptor()
+    lib = load()
+    assert lib.oracle_law_yield(ctypes.byref(d), 0.3, 1e3, 300.0) == 0.0
+    assert lib.oracle_law_dyield(ctypes.byref(d), 0.3, 1e3, 300.0, 1e-8) == 0.0
+    o = OracleJob(m.descriptors(), threads=4)
+    assert o.solve(2e-6) == 0
+    sig = o.ip(F_STRESS)  # [xx, xy, xz, yy, yz, zz]
+    p = (sig[0] + sig[3] + sig[5]) / 3
+    scale = np.abs(p).max()
+    assert scale > 0
+    two_g = m._material.getShearModulus() * 2
+    for c, iso in ((0, True), (1, False), (2, False), (3, True), (4, False), (5, True)):
+        np.testing.assert_allclose(sig[c], p if iso else 0.0, rtol=0, atol=2 * two_g * 1e-8)

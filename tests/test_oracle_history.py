@@ -285,3 +285,32 @@ def test_svg_interface_is_accepted_but_writes_nothing():
         svg.write("temperature.svg", F.T)
     with pytest.raises(AttributeError):
         svg.drawSomething()
+
+
+def test_element_view_and_estimated_end_without_a_job(capsys):
+    """getElementByNum (DynELA.C:144-156): the element's node numbers in
+    connectivity order, found by element number whatever the creation
+    order; displayEstimatedEnd prints nothing before any time has passed."""
+    m = dnl.DynELA("v")
+    m.createNodes(np.array([10, 20, 30, 40, 50, 60]),
+                  np.array([[0, 0, 0], [1, 0, 0], [2, 0, 0], [0, 1, 0], [1, 1, 0], [2, 1, 0]], float))
+    m.setDefaultElement(dnl.Element.ElQua4N2D)
+    m.createElements(np.array([7, 3]), np.array([[20, 30, 60, 50], [10, 20, 50, 40]]))
+    es = dnl.ElementSet("all")
+    m.add(es, 3, 7, 4)
+    mat = dnl.Material("m")
+    mat.youngModulus, mat.poissonRatio, mat.density, mat.heatCapacity = 1e9, 0.3, 1000.0, 400.0
+    law = dnl.ElastoplasticLaw("p")
+    law.setParameters(1e6, 0.0, 1.0)
+    mat.setHardeningLaw(law)
+    m.add(mat, es)
+    s = dnl.Explicit("s")
+    s.setTimes(0.0, 1e-6)
+    m.add(s)
+    e = m.getElementByNum(7)
+    assert e.nodes == [20, 30, 60, 50] and e.getNumberOfNodes() == 4 and e.getNumberOfIntegrationPoints() == 4
+    assert m.getElementByNum(3).nodes == [10, 20, 50, 40]
+    with pytest.raises(dnl.DnlgError):
+        m.getElementByNum(5)
+    m.displayEstimatedEnd()
+    assert capsys.readouterr().out == ""
