@@ -479,6 +479,12 @@ class Explicit:
     def setTimeStepSafetyFactor(self, factor):
         self.timeStepSafetyFactor = float(factor)
 
+    def setIncrements(self, start, stop):
+        """Solver::setIncrements (Solver.C:175-180): kept for scripts; the
+        explicit loop runs on time only (Explicit.C:184) and never consults
+        the increment bounds, so neither does this engine."""
+        self.startIncrement, self.endIncrement = int(start), int(stop)
+
     def setDissipation(self, rho_b):
         if rho_b < 0.0 or rho_b > 1.0:
             raise DnlgError(_lib.E_INVALID, "dissipation value out of range [0.0, 1.0]")

@@ -314,3 +314,13 @@ def test_element_view_and_estimated_end_without_a_job(capsys):
         m.getElementByNum(5)
     m.displayEstimatedEnd()
     assert capsys.readouterr().out == ""
+
+
+def test_set_increments_does_not_bound_the_explicit_loop():
+    """Solver::setIncrements is accepted; the explicit loop runs on time
+    only (Explicit.C:184), so the solve's descriptors do not change."""
+    m1 = samples.taylor_axi(nr=3, nz=8, t_end=2e-6)
+    m2 = samples.taylor_axi(nr=3, nz=8, t_end=2e-6)
+    m2._solver.setIncrements(0, 3)
+    a, b = m1.descriptors().params, m2.descriptors().params
+    assert bytes(a) == bytes(b)
