@@ -115,19 +115,44 @@ class Vec3D:
         return f"Vec3D({self._v[0]}, {self._v[1]}, {self._v[2]})"
 
 
-class NodeSet:
+class _MemberSet:
+    """Members of a NodeSet / ElementSet by number, in insertion order, with
+    the reference's set operations (NodeSet.C:87-160, ElementSet.C)."""
+
     def __init__(self, name=""):
         self.name = name
         self._numbers = []
 
+    def setName(self, name):
+        self.name = name
+
+    def size(self):
+        return len(self._numbers)
+
+    def getNumber(self, item):
+        return self._numbers[item]
+
+    def clear(self):
+        self._numbers = []
+
+    def intersect(self, other):
+        """Keeps the members that are also in `other`."""
+        keep = set(other._numbers)
+        self._numbers = [n for n in self._numbers if n in keep]
+
+    def substract(self, other):
+        """Removes the members that are in `other`."""
+        drop = set(other._numbers)
+        self._numbers = [n for n in self._numbers if n not in drop]
+
+
+class NodeSet(_MemberSet):
     def numbers(self):
         return np.array(self._numbers, dtype=np.int64)
 
 
-class ElementSet:
-    def __init__(self, name=""):
-        self.name = name
-        self._numbers = []
+class ElementSet(_MemberSet):
+    pass
 
 
 class JohnsonCookLaw:
@@ -367,6 +392,27 @@ class Material:
 
     def getShearModulus(self):
         return self.youngModulus / (2.0 * (1.0 + self.poissonRatio))
+
+    def getTwoShearModulus(self):
+        """Material.h:113."""
+        return self.youngModulus / (1.0 + self.poissonRatio)
+
+    def getLambda(self):
+        """Material.h:141."""
+        return self.poissonRatio * self.youngModulus / ((1.0 + self.poissonRatio) * (1.0 - 2.0 * self.poissonRatio))
+
+    def getHeatFraction(self):
+        """Material.h:168: Taylor-Quinney / (rho Cp)."""
+        return self.taylorQuinney / (self.density * self.heatCapacity)
+
+    def isYield(self):
+        """Material::isYield (Material.C:170-175): the law's flag, false for
+        ElasticLaw."""
+        return self._law is not None and not isinstance(self._law, ElasticLaw)
+
+    def setColor(self, *rgb):
+        """Display colour (Material::setColor); not used by the solve."""
+        self.color = tuple(float(c) for c in rgb)
 
     def descriptor(self):
         if self._law is None:

@@ -324,3 +324,30 @@ def test_set_increments_does_not_bound_the_explicit_loop():
     m2._solver.setIncrements(0, 3)
     a, b = m1.descriptors().params, m2.descriptors().params
     assert bytes(a) == bytes(b)
+
+
+def test_set_operations_and_material_constants():
+    """NodeSet / ElementSet size, getNumber, intersect, substract
+    (NodeSet.C:87-160) and the Material constants of Material.h."""
+    a, b = dnl.NodeSet("a"), dnl.NodeSet("b")
+    a._numbers.extend([5, 1, 9, 3])
+    b._numbers.extend([9, 5, 7])
+    a.intersect(b)
+    assert a.size() == 2 and [a.getNumber(i) for i in range(2)] == [5, 9]
+    a._numbers.extend([4, 7])
+    a.substract(b)
+    assert a._numbers == [4]
+    a.clear()
+    assert a.size() == 0
+    mat = dnl.Material("m")
+    mat.youngModulus, mat.poissonRatio, mat.density, mat.heatCapacity, mat.taylorQuinney = 200e9, 0.3, 7800.0, 460.0, 0.9
+    G = mat.getShearModulus()
+    assert mat.getTwoShearModulus() == pytest.approx(2 * G, rel=1e-15)
+    lam = mat.getLambda()
+    assert mat.getBulkModulus() == pytest.approx(lam + 2 * G / 3, rel=1e-14)
+    assert mat.getHeatFraction() == pytest.approx(0.9 / (7800.0 * 460.0), rel=1e-15)
+    assert not mat.isYield()
+    mat.setHardeningLaw(dnl.ElasticLaw("e"))
+    assert not mat.isYield()
+    mat.setHardeningLaw(dnl.JohnsonCookLaw("j"))
+    assert mat.isYield()
