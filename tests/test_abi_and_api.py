@@ -224,3 +224,17 @@ def test_context_close_releases_its_models_jobs():
     assert m._job is None
     m.close()
     assert len(ctx.lib.calls) == 2
+
+
+def test_header_constants_match_the_binding():
+    """Every integer #define DNLG_X of include/dnlg.h that the binding names
+    X has the same value there."""
+    src = open(HEADER).read()
+    defines = dict((k, int(v)) for k, v in re.findall(r"#define DNLG_([A-Z0-9_]+)\s+\(?(-?\d+)\)?", src))
+    checked = 0
+    for name, value in defines.items():
+        if hasattr(_lib, name):
+            assert getattr(_lib, name) == value, (name, getattr(_lib, name), value)
+            checked += 1
+    assert checked >= 25
+    assert defines["LAW_ANN2SIG"] == _lib.LAW_ANN2SIG and defines["ANN_MAX_HIDDEN"] == _lib.ANN_MAX_HIDDEN
