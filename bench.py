@@ -248,19 +248,62 @@ def measured_traffic(workload, prec, n, nz):
         return float(json.load(f)["traffic_bytes_per_launch"])
 
 
-def cpu_baseline(n, nz, sample_t=1e-6, workload="taylor3d", pull=1.0):
+def host_cpu():
+    """lscpu model, sockets, cores and nproc of this host."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                info[keys[k.strip()]] = v.strip()
+    except Exception as e:  # noqa: BLE001 -- lscpu missing: report what os gives
+        info["lscpu_error"] = str(e)
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    return info
+
+
+def physical_cores(info):
+    """Physical cores this process may use (affinity / threads per core)."""
+    usable = info.get("affinity") or info.get("nproc") or 1
+    try:
+        tpc = int(info.get("threads_per_core", 1))
+    except ValueError:
+        tpc = 1
+    return max(1, usable // max(tpc, 1))
+
+
+def cpu_baseline(n, nz, sample_t=1e-6, workload="taylor3d", pull=1.0, threads=(1, None)):
+    """The FP64 CPU oracle (restatement of the reference loop, test
+    infrastructure) on the same model, timed on this host for the first
+    sample_t of the run, once per thread count (None: all physical cores)."""
     from oracle.bridge import OracleJob
-    threads = min(16, os.cpu_count() or 1)
-    m = make_model("fp64", None, n, nz, 80e-6, workload, pull)
-    job = OracleJob(m.descriptors(), threads=threads)
-    t0 = time.perf_counter()
-    job.solve(sample_t)
-    wall = time.perf_counter() - t0
-    _, _, inc = job.scalars()
-    ne = m.getElementsNumber()
-    return dict(value=ne * inc / wall, unit="elem-cycles/s", cores=threads, kind="port",
-                sample=f"first {sample_t * 1e6:g} us of the same model ({inc} cycles x {ne} elements), "
-                       f"FP64 oracle, OpenMP element phases", wall_s=wall)
+    info = host_cpu()
+    res = []
+    for th in threads:
+        th = physical_cores(info) if th is None else th
+        m = make_model("fp64", None, n, nz, 80e-6, workload, pull)
+        job = OracleJob(m.descriptors(), threads=th)
+        t0 = time.perf_counter()
+        job.solve(sample_t)
+        wall = time.perf_counter() - t0
+        _, _, inc = job.scalars()
+        ne = m.getElementsNumber()
+        res.append(dict(value=ne * inc / wall, cores=th, wall_s=wall, cycles=inc))
+        print(f"[bench] cpu baseline {th} thread(s): {res[-1]['value']:.4g} elem-cycles/s, {wall:.1f} s",
+              file=sys.stderr, flush=True)
+    best = max(res, key=lambda r: r["value"])
+    ne = make_model("fp64", None, n, nz, 80e-6, workload, pull).getElementsNumber()
+    return dict(value=best["value"], unit="elem-cycles/s", cores=best["cores"], kind="port",
+                sample=f"first {sample_t * 1e6:g} us of the same model ({best['cycles']} cycles x {ne} elements), "
+                       f"FP64 oracle, element phases OpenMP-parallel",
+                per_threads=[{k: r[k] for k in ("cores", "value", "wall_s")} for r in res], host=info)
 
 
 def copy_calibration(mib):
@@ -276,16 +319,54 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_mib": mib, "bytes_read": n * 4, "bytes_written": n * 4}))
 
 
+METRIC = "elem-cycles/s mixed vs FP64 at <=1e-5 rel disp error; p50/p99 solve latency"
+
+
+def disp_rel_err(a, b):
+    """max_i |a_i - b_i| / max_i |b_i| over node displacement vectors."""
+    return float(np.linalg.norm(a - b, axis=0).max() / np.linalg.norm(b, axis=0).max())
+
+
+def energy_balance(r, speed=227.0):
+    """Closed system (the rigid wall does no work): |KE + IE - KE0| / KE0
+    with KE0 = 1/2 M v0^2; the Chung-Hulbert numerical dissipation plus
+    arithmetic error."""
+    ke0 = 0.5 * r["mass_total"] * speed ** 2
+    return float(abs(r["energies"][3] - ke0) / ke0)
+
+
+def c1_legs(args, ctx, precs):
+    """The 120 000-hex Taylor impact in each precision: the first is the
+    headline (args.steps timed solves), the others args.side_steps."""
+    res = {}
+    for k, p in enumerate(precs):
+        steps, warm = (args.steps, args.warmup) if k == 0 else (args.side_steps, min(args.warmup, 1))
+        print(f"[bench] precision={p} steps={steps} warmup={warm}", file=sys.stderr, flush=True)
+        res[p] = run_precision(p, ctx, steps, warm, args.n, args.nz, args.t_end, args.workload, args.pull)
+        res[p]["steps"] = steps
+        print(f"[bench] {p}: {res[p]['value']:.4g} elem-cycles/s, cycles={res[p]['cycles'][0]}, "
+              f"wall={res[p]['wall']:.3f}s", file=sys.stderr, flush=True)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=3, help="timed solves of the headline precision")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="mixed,fp64")
+    ap.add_argument("--precision", default="mixed,mixed-rr32,fp64",
+                    help="precisions of the C1 legs; the first is the headline value")
+    ap.add_argument("--side-steps", type=int, default=3, help="timed solves of the other precisions")
     ap.add_argument("--n", type=int, default=20)
     ap.add_argument("--nz", type=int, default=100)
     ap.add_argument("--t-end", type=float, default=80e-6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample", type=float, default=1e-6, help="simulated time of the CPU baseline sample")
+    ap.add_argument("--no-batched", action="store_true", help="leave out the batched sweep leg")
+    ap.add_argument("--no-serve", action="store_true", help="leave out the serving leg")
+    ap.add_argument("--batched-jobs", type=int, default=256)
+    ap.add_argument("--batched-cycle-cap", type=int, default=20000)
+    ap.add_argument("--serve-jobs", type=int, default=24)
     ap.add_argument("--workload", default="taylor3d", choices=["taylor3d", "necking", "sweep", "serve"])
     ap.add_argument("--pull", type=float, default=1.0, help="necking: pulled-end speed (m/s)")
     ap.add_argument("--jobs", type=int, default=256, help="sweep / serve workloads: number of jobs")
@@ -294,7 +375,7 @@ def main():
     ap.add_argument("--util", type=float, default=0.7, help="serve: offered load for the arrival rate")
     ap.add_argument("--rate", type=float, default=None, help="serve: arrival rate in jobs/s (overrides --util)")
     ap.add_argument("--step-cycles", type=int, default=32, help="serve: cycles per server step")
-    ap.add_argument("--cycle-budget", type=int, default=0,
+    ap.add_argument("--cycle-budget", type=int, default=200000,
                     help="serve: stop a job still short of t_end after this many increments (0: none)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the plain-copy reference kernel for the traffic counters")
@@ -311,19 +392,14 @@ def main():
     import dnlPython as dnl
     ctx = dnl.Context.default()
     precs = [p.strip() for p in args.precision.split(",") if p.strip()]
+    dtype = lambda p: "f64" if p == "fp64" else "f32+f64"  # noqa: E731
     if args.workload == "serve":
         p = precs[0]
         r = run_serve(p, ctx, args.jobs, args.util, args.rate, args.step_cycles, cycle_budget=args.cycle_budget)
-        print(f"[bench] serve {p}: {r['value']:.4g} elem-cycles/s, p50 {r['p50']:.3f}s, p99 {r['p99']:.3f}s, "
-              f"busy {r['busy']:.2f}, failed {r['failed']}", file=sys.stderr, flush=True)
         line = {"metric": "served elem-cycles/s (continuous batching, Poisson arrivals)", "value": r["value"],
                 "unit": "elem-cycles/s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": r["wall"] * 1e3,
-                "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-                "dtype": "fp32+fp64" if p == "mixed" else "f64", "data": "synthetic",
-                "config": {"workload": f"serving mix of {args.jobs} jobs (axi / 3D x impact / necking, "
-                                       f"{r['elements']} elements in total, 1k-500k each, t_end 20-100 us)",
-                           "precision": p, "arrival_rate_jobs_per_s": r["rate"],
-                           "cycles_per_step": args.step_cycles, "cycle_budget": args.cycle_budget},
+                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype(p),
+                "data": "synthetic", "config": serve_config(r, args.jobs, p, args.step_cycles, args.cycle_budget),
                 "latency_p50_s": r["p50"], "latency_p99_s": r["p99"], "busy_fraction": r["busy"],
                 "failed_jobs": r["failed"]}
         print(json.dumps(line), flush=True)
@@ -332,47 +408,38 @@ def main():
         res = {}
         for p in precs:
             res[p] = run_sweep(p, ctx, args.jobs, args.t_end, max_cycles=args.max_cycles)
-            print(f"[bench] sweep {p}: {res[p]['value']:.4g} elem-cycles/s, wall {res[p]['wall']:.2f}s, "
-                  f"max cycles {res[p]['max_cycles']}", file=sys.stderr, flush=True)
         r = res[precs[0]]
         line = {"metric": "batched elem-cycles/s (parameter sweep)", "value": r["value"], "unit": "elem-cycles/s",
                 "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": r["wall"] * 1e3, "higher_is_better": True,
-                "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64" if precs[0] == "mixed" else "f64",
-                "data": "synthetic",
-                "config": {"workload": f"batched sweep of {r['jobs']} Taylor impacts (2D axi / 3D hex, "
-                                       f"{r['elements']} elements in total), {args.t_end * 1e6:g} us",
-                           "precision": precs[0], "max_cycles": r["max_cycles"], "failed_jobs": r["failed"],
-                           "cycle_cap": args.max_cycles, "jobs_stopped_by_cap": r["capped"]},
-                "element_kernel_ms_per_cycle": r["elem_ms"] / max(r["elem_launches"], 1)}
+                "scaling": "weak", "vs_baseline": None, "dtype": dtype(precs[0]), "data": "synthetic",
+                "config": sweep_config(r, precs[0], args.t_end, args.max_cycles)}
         for p in precs[1:]:
             line[f"value_{p}"] = res[p]["value"]
         if "mixed" in res and "fp64" in res:
             line["speedup_mixed_vs_fp64"] = res["mixed"]["value"] / res["fp64"]["value"]
-            # only jobs that reached t_end in both precisions are comparable
-            errs = [float(np.linalg.norm(a - b, axis=0).max() / np.linalg.norm(b, axis=0).max())
-                    for a, b, da, db in zip(res["mixed"]["disp"], res["fp64"]["disp"], res["mixed"]["done"],
-                                            res["fp64"]["done"]) if da and db]
+            errs = [disp_rel_err(a, b) for a, b, da, db in zip(res["mixed"]["disp"], res["fp64"]["disp"],
+                                                              res["mixed"]["done"], res["fp64"]["done"]) if da and db]
             if errs:
                 line["mixed_disp_err_vs_fp64"] = {"max": max(errs), "median": float(np.median(errs)),
                                                   "jobs_compared": len(errs)}
         print(json.dumps(line), flush=True)
         return
-    res = {}
-    for p in precs:
-        print(f"[bench] precision={p} steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
-        res[p] = run_precision(p, ctx, args.steps, args.warmup, args.n, args.nz, args.t_end, args.workload,
-                               args.pull)
-        r = res[p]
-        print(f"[bench] {p}: {r['value']:.4g} elem-cycles/s, cycles={r['cycles']}, wall={r['wall']:.3f}s",
-              file=sys.stderr, flush=True)
+
+    res = c1_legs(args, ctx, precs)
     head = precs[0]
     r = res[head]
     npe = r["nn"] / r["ne"]
     bytes_el = algo_bytes_per_elem(head, npe)
     avg_ms = r["elem_ms"] / max(r["elem_launches"], 1)
     achieved = bytes_el * r["ne"] / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    if args.workload == "taylor3d":
+        wl = (f"Taylor impact 3D, {args.n}x{args.n}x{args.nz} O-grid hex8 ({r['ne']} elements, {r['nn']} nodes), "
+              f"Johnson-Cook copper, 227 m/s, {args.t_end * 1e6:g} us")
+    else:
+        wl = (f"3D round bar necking, O-grid {args.n} x {args.nz} layers hex8 ({r['ne']} elements, {r['nn']} nodes), "
+              f"Voce + linear law, 2 % taper, pulled at {args.pull:g} m/s, {args.t_end * 1e6:g} us")
     line = {
-        "metric": "elem-cycles/s (mixed precision; FP64 path and error vs FP64 alongside)",
+        "metric": METRIC,
         "value": r["value"],
         "unit": "elem-cycles/s",
         "n_gpus": 1,
@@ -382,46 +449,65 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32+fp64" if head == "mixed" else "f64",
+        "dtype": dtype(head),
         "data": "synthetic",
-        "config": {"workload": (f"Taylor impact 3D, {args.n}x{args.n}x{args.nz} hex8 "
-                                f"({r['ne']} elements, {r['nn']} nodes), Johnson-Cook copper, 227 m/s, "
-                                f"{args.t_end * 1e6:g} us") if args.workload == "taylor3d" else
-                               (f"3D round bar necking, O-grid {args.n} x {args.nz} layers hex8 "
-                                f"({r['ne']} elements, {r['nn']} nodes), Voce + linear law, 2 % taper, "
-                                f"pulled at {args.pull:g} m/s, {args.t_end * 1e6:g} us"),
-                   "precision": head, "cycles_per_step": r["cycles"][0]},
+        "config": {"workload": wl, "precision": head, "cycles_per_step": r["cycles"][0]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.workload, head, args.n, args.nz),
-                     "kernel": "k_elem (hex8)", "kernel_avg_ms": avg_ms,
-                     "algo_bytes_per_elem_cycle": bytes_el},
+                     "kernel": "k_elem_mixed<8,3,128> (hex8)" if head != "fp64" else "k_elem_fp64<8,3,128> (hex8)",
+                     "kernel_avg_ms": avg_ms, "algo_bytes_per_elem_cycle": bytes_el},
     }
     for p in precs[1:]:
-        line[f"value_{p}"] = res[p]["value"]
-    if "mixed" in res and "fp64" in res:
-        line["speedup_mixed_vs_fp64"] = res["mixed"]["value"] / res["fp64"]["value"]
-        d64 = res["fp64"]["disp"]
-        dmx = res["mixed"]["disp"]
-        err = np.linalg.norm(dmx - d64, axis=0).max() / np.linalg.norm(d64, axis=0).max()
-        line["mixed_disp_err_vs_fp64"] = float(err)
-        e64, emx = res["fp64"]["ep"], res["mixed"]["ep"]
-        line["mixed_ep_err_vs_fp64"] = float(np.abs(emx - e64).max() / max(np.abs(e64).max(), 1e-300))
+        line[f"value_{p.replace('-', '_')}"] = res[p]["value"]
+        line[f"steps_{p.replace('-', '_')}"] = res[p]["steps"]
+    if "fp64" in res:
+        d64, e64 = res["fp64"]["disp"], res["fp64"]["ep"]
+        if "mixed" in res:
+            line["speedup_mixed_vs_fp64"] = res["mixed"]["value"] / res["fp64"]["value"]
+        errs = {}
+        for p in precs:
+            if p == "fp64":
+                continue
+            errs[p] = {"disp": disp_rel_err(res[p]["disp"], d64),
+                       "plastic_strain": float(np.abs(res[p]["ep"] - e64).max() / max(np.abs(e64).max(), 1e-300))}
+        line["errors_vs_fp64"] = errs
         line["max_plastic_strain"] = float(np.abs(e64).max())
     if args.workload == "taylor3d":
-        # closed system (rigid wall does no work): KE + IE against the initial
-        # kinetic energy, 1/2 M v0^2; the difference is the Chung-Hulbert
-        # numerical dissipation plus arithmetic error
-        bal = {}
-        for p in precs:
-            ke0 = 0.5 * res[p]["mass_total"] * 227.0 ** 2
-            bal[p] = float(abs(res[p]["energies"][3] - ke0) / ke0)
-        line["energy_balance_err"] = bal
+        line["energy_balance_err"] = {p: energy_balance(res[p]) for p in precs}
+    for p in precs:
+        res[p].pop("disp", None)
+        res[p].pop("ep", None)
+    if args.workload == "taylor3d" and not args.no_batched:
+        b = run_sweep(head, ctx, args.batched_jobs, 80e-6, max_cycles=args.batched_cycle_cap)
+        line["batched"] = {"value": b["value"], "unit": "elem-cycles/s", "wall_s": b["wall"],
+                           "config": sweep_config(b, head, 80e-6, args.batched_cycle_cap)}
+    if args.workload == "taylor3d" and not args.no_serve:
+        sv = run_serve(head, ctx, args.serve_jobs, 0.7, None, 32, cycle_budget=args.cycle_budget)
+        line["serving"] = {"value": sv["value"], "unit": "elem-cycles/s", "latency_p50_s": sv["p50"],
+                           "latency_p99_s": sv["p99"], "busy_fraction": sv["busy"], "wall_s": sv["wall"],
+                           "failed_jobs": sv["failed"],
+                           "config": serve_config(sv, args.serve_jobs, head, 32, args.cycle_budget)}
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (oracle)", file=sys.stderr, flush=True)
-        cb = cpu_baseline(args.n, args.nz, workload=args.workload, pull=args.pull)
-        line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        line["cpu_baseline"] = cpu_baseline(args.n, args.nz, sample_t=args.cpu_sample, workload=args.workload,
+                                            pull=args.pull)
     print(json.dumps(line), flush=True)
+
+
+def sweep_config(r, prec, t_end, cap):
+    return {"workload": f"batched sweep of {r['jobs']} seeded Taylor impacts (2D axisymmetric / 3D hex, "
+                        f"{r['elements']} elements in total, 2k-60k each, Johnson-Cook A, B, n, C, m x U(0.8, 1.2), "
+                        f"150-300 m/s), {t_end * 1e6:g} us",
+            "precision": prec, "max_cycles": r["max_cycles"], "failed_jobs": r["failed"], "cycle_cap": cap,
+            "jobs_stopped_by_cap": r["capped"], "elem_cycles": r["elem_cycles"]}
+
+
+def serve_config(r, n_jobs, prec, step_cycles, budget):
+    return {"workload": f"serving mix of {n_jobs} jobs (axisymmetric / 3D x impact / necking, {r['elements']} "
+                        f"elements in total, 1k-500k each, t_end 20-100 us), Poisson arrivals",
+            "precision": prec, "arrival_rate_jobs_per_s": r["rate"], "cycles_per_step": step_cycles,
+            "cycle_budget": budget, "elem_cycles": r["elem_cycles"]}
 
 
 if __name__ == "__main__":

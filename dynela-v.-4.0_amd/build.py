@@ -8,7 +8,8 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SOURCES = ["csrc/kernels.hip", "csrc/history.hip", "csrc/batch.hip", "csrc/api.hip", "csrc/server.hip"]
+SOURCES = ["csrc/kernels.hip", "csrc/history.hip", "csrc/batch.hip", "csrc/api.hip", "csrc/server.hip",
+           "csrc/energy.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950",

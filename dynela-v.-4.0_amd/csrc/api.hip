@@ -108,7 +108,7 @@ int64_t footprint(int type, int64_t N, int64_t NE, int prec, int track) {
   b += (N + 1) * 4 * 2 + NE * nn * 4 * 2;  // CSRs, connectivity
   b += N * 4;                      // BC node list (upper bound)
   b += NE * nn * dim * 8;          // elemF
-  if (prec == DNLG_PREC_MIXED) {
+  if (is_mixed(prec)) {
     b += NE * nn * dim * 4;        // elemF32
     b += G * (6 * 4 + 3 * 4 + 4 * 8 + 8 + 8 + 4);
   } else {
@@ -467,6 +467,7 @@ int dnlg_ctx_destroy(dnlg_ctx *ctx) {
   hipSetDevice(ctx->device);
   batch_cache_clear(ctx);
   hipStreamSynchronize(ctx->stream);
+  if (ctx->energy_ws) hipFree(ctx->energy_ws);
   hipStreamDestroy(ctx->stream);
   delete ctx;
   return DNLG_OK;
@@ -496,9 +497,10 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
     return fail(ctx, DNLG_E_INVALID, "unknown stress integration method");
   if (prm->stress_method == DNLG_STRESS_DIRECT && prm->precision != DNLG_PREC_FP64)
     return fail(ctx, DNLG_E_INVALID, "the Direct stress integration runs in FP64 precision only");
-  if (prm->dt_freq <= 0 || !(prm->safety > 0) || prm->rho_b < 0.0 || prm->rho_b > 1.0)
+  if (prm->dt_freq <= 0 || !(prm->safety > 0) || prm->rho_b < 0.0 || prm->rho_b > 1.0 || prm->nr_max_iter < 0)
     return fail(ctx, DNLG_E_INVALID, "invalid solver parameters");
-  if (prm->precision != DNLG_PREC_FP64 && prm->precision != DNLG_PREC_MIXED)
+  if (prm->precision != DNLG_PREC_FP64 && prm->precision != DNLG_PREC_MIXED &&
+      prm->precision != DNLG_PREC_MIXED_RR32)
     return fail(ctx, DNLG_E_INVALID, "invalid precision");
   if (mat->law < DNLG_LAW_JOHNSON_COOK || mat->law > DNLG_LAW_ANN2SIG)
     return fail(ctx, DNLG_E_INVALID, "invalid hardening law");
@@ -661,6 +663,7 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
   H.dt_freq = prm->dt_freq;
   H.dt_method = prm->dt_method;
   H.stress_method = prm->stress_method;
+  H.nr_max_it = prm->nr_max_iter > 0 ? prm->nr_max_iter : kItMax;
   H.rho0 = mat->density;
   H.young = mat->young;
   H.nu = mat->poisson;
@@ -769,7 +772,7 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
     H.ltab = pt;
     H.n_ltab = (int32_t)mat->n_table;
   }
-  if (prm->precision == DNLG_PREC_MIXED) {
+  if (is_mixed(prm->precision)) {
     ALLOC(H.elemF32, NE * nn * dim, (const float *)nullptr);
     ALLOC(H.sig32, 6 * G, (const float *)nullptr);
     ALLOC(H.dep32, G, (const float *)nullptr);
@@ -886,7 +889,7 @@ int dnlg_job_read_ip(dnlg_job *job, int field, double *out) {
   HIPCHK(job->ctx, hipStreamSynchronize(job->ctx->stream));
   const int64_t G = job->G;
   const JobDev &H = job->host;
-  const bool mixed = job->prec == DNLG_PREC_MIXED;
+  const bool mixed = is_mixed(job->prec);
   auto d64 = [&](const double *src, int64_t count, double *dst) -> int {
     HIPCHK(job->ctx, hipMemcpy(dst, src, count * sizeof(double), hipMemcpyDeviceToHost));
     return DNLG_OK;
@@ -955,27 +958,23 @@ int dnlg_job_scalars(dnlg_job *job, double *time, double *time_step, int64_t *in
 
 int dnlg_job_energies(dnlg_job *job, double out[4]) {
   if (!job || !out) return DNLG_E_INVALID;
-  const int64_t N = job->N, G = job->G;
-  std::vector<double> v(3 * N), m(N), eint(G), einel(G), dj0(G);
-  int st;
-  if ((st = dnlg_job_read_nodes(job, nullptr, nullptr, v.data()))) return st;
-  if ((st = dnlg_job_read_mass(job, m.data()))) return st;
-  if ((st = dnlg_job_read_ip(job, DNLG_F_INTERNAL_ENERGY, eint.data()))) return st;
-  if ((st = dnlg_job_read_ip(job, DNLG_F_INELASTIC_ENERGY, einel.data()))) return st;
-  HIPCHK(job->ctx, hipMemcpy(dj0.data(), job->host.detJ0, G * sizeof(double), hipMemcpyDeviceToHost));
-  double ke = 0.0;
-  for (int64_t n = 0; n < N; n++) ke += m[n] * (sqd(v[n]) + sqd(v[N + n]) + sqd(v[2 * N + n])) / 2.0;
-  double ie = 0.0, wp = 0.0;
-  const double scale = job->type == DNLG_EL_QUA4N_AX ? k2PI : 1.0;
-  for (int64_t g = 0; g < G; g++) {
-    const double vol = dj0[g] * scale;
-    ie += job->host.rho0 * vol * eint[g];
-    wp += job->host.rho0 * vol * einel[g];
+  dnlg_ctx *ctx = job->ctx;
+  hipSetDevice(ctx->device);
+  const int64_t need = 6 * (int64_t)energy_blocks(job->N, job->G) + 4;
+  if (ctx->energy_ws_doubles < need) {
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->energy_ws) hipFree(ctx->energy_ws);
+    ctx->energy_ws = nullptr;
+    ctx->energy_ws_doubles = 0;
+    HIPCHK(ctx, hipMalloc(&ctx->energy_ws, need * sizeof(double)));
+    ctx->energy_ws_doubles = need;
   }
-  out[0] = ke;
-  out[1] = ie;
-  out[2] = wp;
-  out[3] = ke + ie;
+  double *d_out = ctx->energy_ws + (need - 4);
+  launch_energies(job->dev, job->N, job->G, job->dim, job->type == DNLG_EL_QUA4N_AX, ctx->energy_ws, d_out,
+                  ctx->stream);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipMemcpyAsync(out, d_out, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return DNLG_OK;
 }
 

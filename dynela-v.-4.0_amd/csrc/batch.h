@@ -75,6 +75,9 @@ struct dnlg_ctx {
   std::vector<dnlg::BatchPtr> cache;
   int graph_cycles = 16;  // cycles per captured HIP graph (0 = plain launches)
   int pi_slots = 2;       // power iterations per time-step cycle inside a sequence
+  // device workspace of the energy reduction (grown on demand)
+  double *energy_ws = nullptr;
+  int64_t energy_ws_doubles = 0;
 };
 
 namespace dnlg {

@@ -28,11 +28,23 @@ constexpr int kTypes[kNumTypes] = {DNLG_EL_QUA4N_AX, DNLG_EL_HEX8N_3D, DNLG_EL_Q
 inline int type_dim(int type) { return type == DNLG_EL_HEX8N_3D ? 3 : 2; }
 inline int type_nodes(int type) { return type == DNLG_EL_HEX8N_3D ? 8 : 4; }
 
-// Element kernel variants of a type: FP64, mixed, FP64 with a network law.
-constexpr int kElemVariants = 3;
+// Element kernel variants of a type: FP64, mixed (FP64 radial return),
+// FP64 with a network law, mixed with the FP32 radial return.
+constexpr int kElemVariants = 4;
 inline int elem_variant(const dnlg_job *j) {
   if (j->prec == DNLG_PREC_MIXED) return 1;
+  if (j->prec == DNLG_PREC_MIXED_RR32) return 3;
   return j->host.law == DNLG_LAW_ANN2SIG ? 2 : 0;
+}
+inline bool variant_mixed(int mx) { return mx == 1 || mx == 3; }
+
+// One element launch of variant mx over a task list.
+void launch_variant(int type, int mx, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int derive,
+                    hipStream_t s, const hipEvent_t *ev) {
+  if (variant_mixed(mx))
+    launch_elem_mixed(type, jobs, tasks, nblk, cs, derive, s, ev, mx == 1);
+  else
+    launch_elem_fp64(type, jobs, tasks, nblk, cs, derive, s, ev, mx == 2);
 }
 
 struct Batch {
@@ -137,7 +149,7 @@ int build_batch(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, Batch &B) {
           idx.push_back(i);
           cnt.push_back(jobs[i]->NE);
         }
-      const int threads = mx == 1 ? elem_block_threads_mixed(type) : elem_block_threads(type);
+      const int threads = variant_mixed(mx) ? elem_block_threads_mixed(type) : elem_block_threads(type);
       if ((st = upload_tasks(B, xcd_order(make_tasks(idx, cnt, threads / nn)), B.elem[hx][mx]))) return st;
     }
     std::vector<int> idx;
@@ -168,7 +180,7 @@ int build_batch(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, Batch &B) {
       std::vector<int> idx;
       std::vector<int64_t> cnt;
       for (int i = 0; i < n; i++)
-        if ((jobs[i]->dim == 3) == (d3 == 1) && (jobs[i]->prec == DNLG_PREC_MIXED) == (mx == 1)) {
+        if ((jobs[i]->dim == 3) == (d3 == 1) && is_mixed(jobs[i]->prec) == (mx == 1)) {
           idx.push_back(i);
           cnt.push_back(jobs[i]->N);
         }
@@ -224,10 +236,7 @@ int launch_element(dnlg_ctx *ctx, Batch &B, const hipEvent_t *ev, int cs, int de
       const TaskList &L = B.elem[hx][mx];
       if (L.n <= 0) continue;
       const hipEvent_t *e = ev ? &ev[2 * k] : nullptr;
-      if (mx == 1)
-        launch_elem_mixed(type, B.d_jobs, L.d, L.n, cs, derive, s, e);
-      else
-        launch_elem_fp64(type, B.d_jobs, L.d, L.n, cs, derive, s, e, mx == 2);
+      launch_variant(type, mx, B.d_jobs, L.d, L.n, cs, derive, s, e);
       k++;
     }
   }
@@ -335,9 +344,8 @@ int run_preloop(dnlg_ctx *ctx, Batch &B) {
   }
   for (int hx = 0; hx < kNumTypes; hx++) {
     const int type = kTypes[hx];
-    launch_elem_fp64(type, B.d_jobs, B.elem[hx][0].d, B.elem[hx][0].n, 0, 0, s);
-    launch_elem_mixed(type, B.d_jobs, B.elem[hx][1].d, B.elem[hx][1].n, 0, 0, s);
-    launch_elem_fp64(type, B.d_jobs, B.elem[hx][2].d, B.elem[hx][2].n, 0, 0, s, nullptr, true);
+    for (int mx = 0; mx < kElemVariants; mx++)
+      launch_variant(type, mx, B.d_jobs, B.elem[hx][mx].d, B.elem[hx][mx].n, 0, 0, s, nullptr);
   }
   // the pre-loop's history rows (Explicit.C:182)
   launch_history(ctx, B, 0, 1);

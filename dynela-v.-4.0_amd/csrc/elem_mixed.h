@@ -168,7 +168,106 @@ struct LaneRows {
   __device__ static int at(int le, int lane, int v) { return le * kGroup + lane * R + v; }
 };
 
-template <int NN, int DIM, int BLOCK, bool AXI>
+// Newton on gamma for Element::computeStress's radial return
+// (Element.C:485-601) in FP32: the bisection safeguard of the reference, the
+// law from law_f32.  Stops at the reference tolerance, or 4 ulp of gamma
+// where FP32 cannot resolve it; after a Newton step below 2e-4 gamma the
+// remaining error is of order |f''/f'| dg^2 < 1e-11 (hardening, rate and
+// thermal terms of the laws here), so the confirming iteration is left out.
+// In: the trial stress Strial = sqrt(3/2) |s_trial|, yield = the stored
+// yield stress, g = the previous gamma (the start unless ep0 == 0, as in the
+// reference).  Out: gamma and the yield stress of the last iterate.  The
+// state at the start of the last iteration -- its gamma, the yield stress
+// its temperature was formed with, and the bracket before its update -- goes
+// to `last` (for the FP64 return that resumes there).  Returns false at the
+// iteration limit.
+struct NewtonState {
+  float g, y, gmin, gmax;
+};
+__device__ __forceinline__ bool radial_return_fp32(const LawF &L, int max_it, float Strial, float TwoG32, float heat,
+                                                   float Tc, double ep0, float Snorm0, float &yield, float &g,
+                                                   NewtonState &last) {
+  constexpr float s23 = (float)kSqrt23;
+  const float ep0f = (float)ep0;
+  float gmin = 0.f, gmax = (Strial - yield) * __builtin_amdgcn_rcpf(TwoG32);
+  if (ep0 == 0.0) g = (float)(kSqrt32 * kGammaInitial);
+  float epsp = ep0f + s23 * g;
+  float T = Tc + 0.5f * g * heat * (s23 * yield + Snorm0);
+  int iterate = 0;
+  while (true) {
+    last = NewtonState{g, yield, gmin, gmax};
+    float hard;
+    law_f32(L, epsp, g, T, yield, hard);
+    const float fun = Strial - g * TwoG32 - yield;
+    if (fun < 0.f)
+      gmax = g;
+    else
+      gmin = g;
+    const float dfun = TwoG32 + s23 * hard;
+    float dg = fun * __builtin_amdgcn_rcpf(dfun);
+    g += dg;
+    bool newton = true;
+    if ((gmax - g) * (g - gmin) < 0.f) {
+      dg = 0.5f * (gmax - gmin);
+      g = gmin + dg;
+      newton = false;
+    }
+    if (fabsf(dg) < fmaxf((float)kTolNR, (newton ? 2e-4f : 4.8e-7f) * g)) return true;
+    epsp = ep0f + s23 * g;
+    T = Tc + 0.5f * g * heat * (s23 * yield + Snorm0);
+    if (++iterate > max_it) return false;
+  }
+}
+
+// The same Newton in FP64 with the reference's stopping rule (|dgamma| <
+// 1e-8, at most 250 iterations) and operation order, the yield stress and
+// slope from the device libm (law_yield_and_slope), resuming the FP32
+// iteration at the start of its last iteration: gamma g, the yield stress y
+// that iteration's temperature was formed with, and the bracket [gmin,
+// gmax] before its update.  From there the reference's own iterates are
+// taken again in FP64: its path is reproduced up to the FP32 rounding of
+// the resumed state, which the Newton step contracts, so one or two FP64
+// iterations end where the reference ends.  Resuming anywhere else would not
+// do: the reference's result depends on its path -- its absolute tolerance
+// leaves a small gamma (first yield) 1e-3 to 1e-1 short of the root, and
+// when the root lies above the initial bound (Strial - y_stored) /
+// (sqrt(3/2) 2G) a start above the bound moves gmax up to itself and
+// reaches the root while a start below it is held under the bound.  Out:
+// gamma and the FP64 yield stress of its last iterate.  Returns false at the
+// iteration limit.
+__device__ __forceinline__ bool radial_return_fp64(int law, const double *lp, int max_it, double heatFrac, double TwoG32,
+                                                   double Strial, double Snorm0, double Tcur, double ep0,
+                                                   double dt, double gmin, double gmax, double &y, double &g) {
+  double depsp = kSqrt23 * g / dt;
+  double epsp = ep0 + kSqrt23 * g;
+  double T = Tcur + 0.5 * g * heatFrac * (kSqrt23 * y + Snorm0);
+  int iterate = 0;
+  while (true) {
+    double hard;
+    law_yield_and_slope(law, lp, heatFrac, epsp, depsp, T, dt, y, hard);
+    const double fun = Strial - g * TwoG32 - y;
+    if (fun < 0.0)
+      gmax = g;
+    else
+      gmin = g;
+    const double dfun = TwoG32 + kSqrt23 * hard;
+    double dg = fun / dfun;
+    g += dg;
+    if ((gmax - g) * (g - gmin) < 0.0) {
+      dg = 0.5 * (gmax - gmin);
+      g = gmin + dg;
+    }
+    if (fabs(dg) < kTolNR) return true;
+    depsp = kSqrt23 * g / dt;
+    epsp = ep0 + kSqrt23 * g;
+    T = Tcur + 0.5 * g * heatFrac * (kSqrt23 * y + Snorm0);
+    if (++iterate > max_it) return false;
+  }
+}
+
+// RR64: the radial return in FP64 (DNLG_PREC_MIXED); otherwise in FP32
+// (DNLG_PREC_MIXED_RR32).
+template <int NN, int DIM, int BLOCK, bool AXI, bool RR64>
 __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const Task *tasks, int ctl, int derive) {
   constexpr int NB = DIM;               // lanes per element = 2^DIM = NN
   constexpr float inv2D = 1.f / NN;     // 2^-D
@@ -397,58 +496,32 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
       gbl(C->sy32)[gi] = yieldf;
     }
     if (Strialf > yieldf) {
-      // radial return (Element.C:485-601): Newton on gamma with bisection
-      // safeguard, FP32 arithmetic; ep, T and the energies accumulate in FP64
-      constexpr float s23 = (float)kSqrt23;
-      const float TwoG32f = (float)TwoG32;
-      const float heatf = (float)heatFrac;
-      const float Tc = (float)Tcur;
-      const float ep0f = (float)ep0;
+      // radial return (Element.C:485-601): Newton on gamma in FP32, then
+      // (RR64) polished by the FP64 Newton; ep, T and the energies
+      // accumulate in FP64
       float pf[8];
 #pragma unroll
       for (int k = 0; k < 8; k++) pf[k] = C->lpf[k];
       const LawF L = law_f32_setup(law, pf, C->inv_dTf, (float)dt, (float)heatFrac);
-      float gmin = 0.f, gmax = (Strialf - yieldf) * __builtin_amdgcn_rcpf(TwoG32f);
       float g = gam0;
-      if (ep0 == 0.0) g = (float)(kSqrt32 * kGammaInitial);
-      float epsp = ep0f + s23 * g;
-      float T = Tc + 0.5f * g * heatf * (s23 * yieldf + Snorm0);
-      int iterate = 0;
-      while (true) {
-        float hard;
-        law_f32(L, epsp, g, T, yieldf, hard);
-        const float fun = Strialf - g * TwoG32f - yieldf;
-        if (fun < 0.f)
-          gmax = g;
-        else
-          gmin = g;
-        const float dfun = TwoG32f + s23 * hard;
-        float dg = fun * __builtin_amdgcn_rcpf(dfun);
-        g += dg;
-        bool newton = true;
-        if ((gmax - g) * (g - gmin) < 0.f) {
-          dg = 0.5f * (gmax - gmin);
-          g = gmin + dg;
-          newton = false;
-        }
-        // the reference tolerance, or 4 ulp of gamma where FP32 cannot resolve
-        // it; after a Newton step below 2e-4 gamma the remaining error is of
-        // order |f''/f'| dg^2 < 1e-11 (hardening, rate and thermal terms of the
-        // laws here), so the confirming iteration is left out
-        if (fabsf(dg) < fmaxf((float)kTolNR, (newton ? 2e-4f : 4.8e-7f) * g)) break;
-        epsp = ep0f + s23 * g;
-        T = Tc + 0.5f * g * heatf * (s23 * yieldf + Snorm0);
-        if (++iterate > kItMax) {
-          set_status(J, DNLG_E_NR_DIVERGED, par);
-          break;
-        }
-      }
+      NewtonState last;
+      bool ok = radial_return_fp32(L, C->nr_max_it, Strialf, (float)TwoG32, (float)heatFrac, (float)Tcur, ep0, Snorm0,
+                                   yieldf, g, last);
       gamma = (double)g;
+      if constexpr (RR64) {
+        double y = (double)last.y;
+        gamma = (double)last.g;
+        ok = radial_return_fp64(law, lp, C->nr_max_it, heatFrac, TwoG32, (double)Strialf, (double)Snorm0, Tcur, ep0, dt,
+                                (double)last.gmin, (double)last.gmax, y, gamma);
+        g = (float)gamma;
+        yieldf = (float)y;
+      }
+      if (!ok) set_status(J, DNLG_E_NR_DIVERGED, par);
       const float scale = 1.f - TwoGf * g * __builtin_amdgcn_rcpf(Snorm);
 #pragma unroll
       for (int c = 0; c < 6; c++) s[c] *= scale;
       gbl(C->ep)[gi] = ep0 + kSqrt23 * gamma;
-      gbl(C->dep32)[gi] = s23 * g * __builtin_amdgcn_rcpf((float)dt);
+      gbl(C->dep32)[gi] = (float)(kSqrt23 * gamma / dt);
       gbl(C->gam32)[gi] = g;
       gbl(C->sy32)[gi] = yieldf;
     }

@@ -30,6 +30,9 @@ constexpr int kItMax = 250;
 
 __host__ __device__ __forceinline__ double sqd(double a) { return a * a; }
 
+// Precisions with FP32 element state (DNLG_PREC_MIXED, DNLG_PREC_MIXED_RR32).
+__host__ __device__ __forceinline__ bool is_mixed(int prec) { return prec != DNLG_PREC_FP64; }
+
 struct Boundary {
   int32_t kind;  // DNLG_BC_*
   int32_t ramp;  // DNLG_RAMP_*
@@ -149,6 +152,7 @@ struct JobDev {
   int32_t type, dim, nn, nip;
   int64_t N, NE, G;  // nodes, elements, integration points
   int32_t prec, track, law, dt_freq, dt_method, stress_method;
+  int32_t nr_max_it;  // Newton iterations of the radial return (250 by default)
   // material (Material.h:110-169) and law parameters
   double rho0, young, nu, twoG, K, heatFrac, eta, cp, T0;
   double lp[8];

@@ -98,7 +98,7 @@ __device__ __forceinline__ void load_tensor(CJobDev *C, int src, int64_t g, doub
     const DNLG_GLOBAL double *t = gbl(src == kHEps ? C->eps : C->epsP);
 #pragma unroll
     for (int k = 0; k < 6; k++) s[k] = t[k * G + g];
-  } else if (C->prec == DNLG_PREC_MIXED) {
+  } else if (is_mixed(C->prec)) {
 #pragma unroll
     for (int k = 0; k < 6; k++) s[k] = (double)gbl(C->sig32)[k * G + g];
   } else {
@@ -115,7 +115,7 @@ __device__ __forceinline__ double sym_norm(const double s[6]) {
 // One IP's value of a field (Element::getIntPointValueExtract,
 // Element.C:1195-1221; the mixed path's FP32 arrays widened).
 __device__ double ip_value(CJobDev *C, int src, int comp, int64_t g) {
-  const bool mixed = C->prec == DNLG_PREC_MIXED;
+  const bool mixed = is_mixed(C->prec);
   switch (src) {
     case kHEp: return gbl(C->ep)[g];
     case kHDep: return mixed ? (double)gbl(C->dep32)[g] : gbl(C->dep)[g];

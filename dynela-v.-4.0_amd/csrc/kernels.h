@@ -17,8 +17,10 @@ int elem_block_threads_mixed(int type);
 // ann: the variant for jobs with a network law (DNLG_LAW_ANN2SIG).
 void launch_elem_fp64(int type, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int derive,
                       hipStream_t s, const hipEvent_t *ev = nullptr, bool ann = false);
+// rr64: the radial return in FP64 (DNLG_PREC_MIXED), else in FP32
+// (DNLG_PREC_MIXED_RR32)
 void launch_elem_mixed(int type, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int derive,
-                       hipStream_t s, const hipEvent_t *ev = nullptr);
+                       hipStream_t s, const hipEvent_t *ev, bool rr64);
 // fold != 0: BC nodes also get the next cycle's field0 BC values (the next
 // cycle's element launch then derives its record, derive != 0)
 void launch_correct(int dim, bool f32, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int fold,
@@ -61,5 +63,15 @@ void launch_history(JobDev *const *jobs, const Task *eval, int n_eval, const Tas
 // comp < 0 with a tensor source: all six components, out [6][N]); needs the
 // job's nf_ptr / nf_pairs lists
 void launch_node_field(JobDev *job, int64_t N, int src, int comp, double *out, hipStream_t s);
+
+// Global energies of one job (energy.hip): kinetic energy of the field0
+// speeds, stored internal energy and plastic work, by a fixed-order
+// Neumaier-compensated reduction.  part: 6 doubles per block,
+// energy_blocks(N, G) blocks; out: {KE, IE, W_p, KE + IE} (device).  The job
+// must be between launch sequences (its record in control slot 0).
+constexpr int64_t kEnergyItemsPerBlock = 4096;
+int energy_blocks(int64_t N, int64_t G);
+void launch_energies(JobDev *job, int64_t N, int64_t G, int dim, bool axi, double *part, double *out,
+                     hipStream_t s);
 
 }  // namespace dnlg

@@ -419,7 +419,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
         depsp = kSqrt23 * gamma / dt;
         epsp = ep0 + kSqrt23 * gamma;
         T = Tcur + 0.5 * gamma * heatFrac * (kSqrt23 * yield + Snorm0);
-        if (++iterate > kItMax) {
+        if (++iterate > C->nr_max_it) {
           set_status(J, DNLG_E_NR_DIVERGED, par);
           break;
         }
@@ -1299,15 +1299,24 @@ void launch_elem(K kern, int block, int nblk, hipStream_t s, const hipEvent_t *e
     hipLaunchKernelGGL(kern, dim3(nblk), dim3(block), 0, s, jobs, tasks, cs, derive);
 }
 
-void launch_elem_mixed(int type, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int derive,
-                       hipStream_t s, const hipEvent_t *ev) {
-  if (nblk <= 0) return;
+template <bool RR64>
+void launch_elem_mixed_rr(int type, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int derive,
+                          hipStream_t s, const hipEvent_t *ev) {
   if (type == DNLG_EL_HEX8N_3D)
-    launch_elem(k_elem_mixed<8, 3, 128, false>, 128, nblk, s, ev, jobs, tasks, cs, derive);
+    launch_elem(k_elem_mixed<8, 3, 128, false, RR64>, 128, nblk, s, ev, jobs, tasks, cs, derive);
   else if (type == DNLG_EL_QUA4N_AX)
-    launch_elem(k_elem_mixed<4, 2, 256, true>, 256, nblk, s, ev, jobs, tasks, cs, derive);
+    launch_elem(k_elem_mixed<4, 2, 256, true, RR64>, 256, nblk, s, ev, jobs, tasks, cs, derive);
   else
-    launch_elem(k_elem_mixed<4, 2, 256, false>, 256, nblk, s, ev, jobs, tasks, cs, derive);
+    launch_elem(k_elem_mixed<4, 2, 256, false, RR64>, 256, nblk, s, ev, jobs, tasks, cs, derive);
+}
+
+void launch_elem_mixed(int type, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int derive,
+                       hipStream_t s, const hipEvent_t *ev, bool rr64) {
+  if (nblk <= 0) return;
+  if (rr64)
+    launch_elem_mixed_rr<true>(type, jobs, tasks, nblk, cs, derive, s, ev);
+  else
+    launch_elem_mixed_rr<false>(type, jobs, tasks, nblk, cs, derive, s, ev);
 }
 
 template <bool ANN>

@@ -68,17 +68,19 @@ __device__ __forceinline__ void law_yield_and_slope(int law, const double *p, do
 __device__ __forceinline__ bool law_tabular(const double *t, int n, double epsp, double &yield, double &hard) {
   const DNLG_GLOBAL double *x = gbl(t), *y = gbl(t) + n;
   if (!(epsp >= x[0] && epsp <= x[n - 1])) return false;
-  double x0 = x[0], y0 = y[0];
-  for (int i = 1; i < n; i++) {
-    const double xi = x[i], yi = y[i];
-    if (xi >= epsp) {
-      yield = (xi == epsp) ? yi : ((epsp - x0) / (xi - x0)) * (yi - y0) + y0;
-      hard = (yi - y0) / (xi - x0);
-      return true;
-    }
-    x0 = xi;
-    y0 = yi;
+  // the first point i >= 1 with x_i >= epsp, by bisection over the ascending
+  // strains (the reference scans the points in order; same segment)
+  int lo = 1, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (x[mid] >= epsp)
+      hi = mid;
+    else
+      lo = mid + 1;
   }
+  const double x0 = x[lo - 1], y0 = y[lo - 1], xi = x[lo], yi = y[lo];
+  yield = (xi == epsp) ? yi : ((epsp - x0) / (xi - x0)) * (yi - y0) + y0;
+  hard = (yi - y0) / (xi - x0);
   return true;
 }
 

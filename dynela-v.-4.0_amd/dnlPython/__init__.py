@@ -498,7 +498,9 @@ class Explicit:
     """Explicit solver settings (Solver.h:73-78, Explicit.h:56).  Defaults are
     the reference's: Courant step every 10 increments, safety 0.9,
     rho_b = 0.8182 (Solver.h:30-33, configuration.dnl:9-11, Explicit.h:16).
-    ``setPrecision("mixed")`` selects the FP32-force / FP64-kinematics path."""
+    ``setPrecision("mixed")`` selects FP32 strain / stress / forces with the
+    radial-return Newton solve, kinematics and accumulators in FP64;
+    ``"mixed-rr32"`` also runs the radial return in FP32."""
     Courant, PowerIteration, PowerIterationUnder = 0, 1, 2
 
     def __init__(self, name=""):
@@ -511,6 +513,7 @@ class Explicit:
         self.dissipation = 0.8182
         self.precision = _lib.PREC_FP64
         self.trackTensors = True
+        self.nrMaxIterations = 0
         self.model = None
 
     def setTimes(self, start, end):
@@ -537,13 +540,22 @@ class Explicit:
         self.dissipation = float(rho_b)
 
     def setPrecision(self, precision):
-        p = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED}.get(str(precision).lower(), precision)
-        if p not in (_lib.PREC_FP64, _lib.PREC_MIXED):
+        p = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED,
+             "mixed-rr32": _lib.PREC_MIXED_RR32}.get(str(precision).lower(), precision)
+        if p not in (_lib.PREC_FP64, _lib.PREC_MIXED, _lib.PREC_MIXED_RR32):
             raise DnlgError(_lib.E_INVALID, f"unknown precision {precision!r}")
         self.precision = p
 
     def setTrackTensors(self, flag):
         self.trackTensors = bool(flag)
+
+    def setMaxNewtonIterations(self, n):
+        """Iterations of the radial return's Newton solve before the job
+        stops with NR_DIVERGED (0: the reference's 250, Element.C:560-581).
+        No counterpart in the reference, whose limit is fixed."""
+        if int(n) < 0:
+            raise DnlgError(_lib.E_INVALID, "the Newton iteration limit must be >= 0")
+        self.nrMaxIterations = int(n)
 
     def getTimeStep(self):
         return self.model.getTimeStep() if self.model else 0.0
@@ -556,6 +568,7 @@ class Explicit:
         p.stress_method = _lib.STRESS_NR
         p.precision = self.precision
         p.track_tensors = 1 if self.trackTensors else 0
+        p.nr_max_iter = self.nrMaxIterations
         return p
 
 

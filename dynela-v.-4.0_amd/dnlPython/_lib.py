@@ -32,7 +32,7 @@ BC_RESTRAIN, BC_SPEED = 0, 1
 RAMP_NONE, RAMP_LINEAR, RAMP_SINUS, RAMP_CONSTANT, RAMP_ERF, RAMP_SOFT = range(6)
 DT_COURANT, DT_POWER_ITERATION, DT_POWER_ITERATION_UNDER = 0, 1, 2
 STRESS_NR, STRESS_DIRECT = 0, 1
-PREC_FP64, PREC_MIXED = 0, 1
+PREC_FP64, PREC_MIXED, PREC_MIXED_RR32 = 0, 1, 2
 (F_PLASTIC_STRAIN, F_PLASTIC_STRAIN_RATE, F_GAMMA, F_GAMMA_CUMULATE, F_YIELD_STRESS, F_TEMPERATURE,
  F_DENSITY, F_PRESSURE, F_INTERNAL_ENERGY, F_INELASTIC_ENERGY, F_VON_MISES, F_STRESS, F_STRAIN,
  F_PLASTIC_STRAIN_TENSOR) = range(14)
@@ -71,7 +71,7 @@ class SolverParams(ctypes.Structure):
     _fields_ = [("t_start", ctypes.c_double), ("t_end", ctypes.c_double), ("rho_b", ctypes.c_double),
                 ("safety", ctypes.c_double), ("dt_method", ctypes.c_int32), ("dt_freq", ctypes.c_int32),
                 ("stress_method", ctypes.c_int32), ("precision", ctypes.c_int32),
-                ("track_tensors", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("track_tensors", ctypes.c_int32), ("nr_max_iter", ctypes.c_int32)]
 
 
 class HistoryItem(ctypes.Structure):

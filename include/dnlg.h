@@ -96,9 +96,16 @@ extern "C" {
 
 /* arithmetic of the element phases */
 #define DNLG_PREC_FP64 0  /* everything in FP64                              */
-#define DNLG_PREC_MIXED 1 /* FP32 strain/stress/radial return/forces, FP64
-                             kinematics and accumulated plastic strain,
+#define DNLG_PREC_MIXED 1 /* FP32 geometry, strain, stress and internal
+                             forces; FP64 radial-return Newton solve for the
+                             plastic multiplier (the reference's control flow
+                             and 1e-8 tolerance, Element.C:485-601), nodal
+                             kinematics, accumulated plastic strain,
                              temperature and energies                         */
+#define DNLG_PREC_MIXED_RR32 2 /* DNLG_PREC_MIXED with the radial return in
+                             FP32 as well (Newton to 4 ulp of gamma, hardware
+                             log2 / exp2 in the law): faster, looser plastic
+                             strain; no counterpart in the reference          */
 
 /* integration-point fields for dnlg_job_read_ip (names of Field.h:46-79) */
 #define DNLG_F_PLASTIC_STRAIN 0      /* scalar */
@@ -175,7 +182,9 @@ typedef struct dnlg_solver_params {
   int32_t stress_method; /* DNLG_STRESS_NR */
   int32_t precision;     /* DNLG_PREC_* */
   int32_t track_tensors; /* 1: carry Strain and PlasticStrain tensors */
-  int32_t reserved;
+  int32_t nr_max_iter;   /* iterations of the radial return's Newton solve
+                            before DNLG_E_NR_DIVERGED; 0 = the reference's
+                            250 (Element.C:560-581) */
 } dnlg_solver_params;
 
 /* ---- opaque handles ------------------------------------------------------- */

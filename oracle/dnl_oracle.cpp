@@ -1213,7 +1213,7 @@ void element_pressure(OracleJob *J, int64_t e) {
  * safeguarded Newton-Raphson on gamma, adiabatic heating, energies. */
 bool element_stress(OracleJob *J, int64_t e, double timeStep) {
   const double gammaInitial = 1e-8, tolNR = 1e-8;
-  const int itMax = 250;
+  const int itMax = J->prm.nr_max_iter > 0 ? J->prm.nr_max_iter : 250; /* Element.C:560-581 */
   const double TwoG = J->mat.young / (1.0 + J->mat.poisson);
   const double TwoG32 = kSqrt32 * TwoG;
   const double heatFrac = J->mat.taylor_quinney / (J->mat.density * J->mat.heat_capacity);
@@ -1495,9 +1495,10 @@ void densities(OracleJob *J) {
   for (int64_t e = 0; e < J->NE; e++) element_density(J, e);
 }
 
-bool nodes_finite(const OracleJob *J) {
+/* the coordinates and the speeds of field `f` are finite */
+bool nodes_finite(const OracleJob *J, int f) {
   for (int64_t q = 0; q < 3 * J->N; q++)
-    if (!std::isfinite(J->x[q]) || !std::isfinite(J->fv[J->f0][q])) return false;
+    if (!std::isfinite(J->x[q]) || !std::isfinite(J->fv[f][q])) return false;
   return true;
 }
 
@@ -1809,14 +1810,16 @@ int oracle_job_solve(OracleJob *J, double upTo) {
     if (!element_phases(J, J->dt)) return J->status;
     corrector(J);
     densities(J);
+    /* a non-finite state stops the job in this cycle, before its end of
+     * step (the reference has no such check and would run on) */
+    if (!nodes_finite(J, 1 - J->f0)) {
+      set_err(J, DNLG_E_NONFINITE, "non-finite nodal state");
+      return J->status;
+    }
     /* Explicit::endStep (Explicit.C:539-552): time, history rows, field swap */
     J->t += J->dt;
     write_history(J);
     J->f0 = 1 - J->f0;
-    if (!nodes_finite(J)) {
-      set_err(J, DNLG_E_NONFINITE, "non-finite nodal state");
-      return J->status;
-    }
     if (J->t < upTo) {
       if (!compute_jacobian(J, false)) return J->status;
       if (!compute_time_step(J, false)) return J->status;

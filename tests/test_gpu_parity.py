@@ -74,12 +74,23 @@ def test_necking_small(ctx):
 
 
 # ---- mixed precision: FP32 strain / stress / forces, FP64 kinematics ------
-# Bound stated for the mixed path: nodal displacement <= 1e-5 normwise
-# relative to the FP64 oracle; IP fields are reported (looser, FP32 state).
+# Bounds stated for the mixed paths against the FP64 oracle (normwise
+# relative, max_i |a_i - ref_i| / max_i |ref_i|), both mixed modes:
+#   nodal displacement 1e-5; equivalent plastic strain, temperature and
+#   gammaCumulate 5e-5; energy balance (KE + IE) 5e-6;
+#   stress and yield stress 1e-2: an integration point whose trial stress
+#   sits within FP32 rounding of its yield stress can take the elastic branch
+#   in one run and a tiny plastic step in the other, and then keeps a yield
+#   stress from another cycle's strain rate; such flips dominate these two
+#   fields (measured 2e-3 to 5e-3 on the 80 us Taylor impacts) while the
+#   plastic strain, the temperature, the energies and the displacements stay
+#   1e-7 to 1e-5.  gamma (the last increment) is reported, not bounded.
 MIXED_DISP_TOL = 1e-5
+_TOLS = dict(plasticStrain=5e-5, T=5e-5, gammaCumulate=5e-5, energy=5e-6, Stress=1e-2, yieldStress=1e-2)
+MIXED_TOLS = {"mixed": _TOLS, "mixed-rr32": _TOLS}
 
 
-def compare_mixed(model, o):
+def compare_mixed(model, o, prec):
     import dnlPython as dnl
     _, d_ref, _ = o.nodes()
     err = disp_err(model.getNodalDisplacements(), d_ref)
@@ -89,39 +100,42 @@ def compare_mixed(model, o):
         out[f] = rel_norm_err(model.getIntPointField(getattr(dnl.Field, f)), o.ip(k))
     e_g, e_o = model.getEnergies(), o.energies()
     out["energy"] = float(abs(e_g[3] - e_o[3]) / abs(e_o[3]))
-    print("mixed errors:", {k: f"{v:.2e}" for k, v in out.items()})
+    print(f"{prec} errors:", {k: f"{v:.2e}" for k, v in out.items()})
+    assert out["disp"] <= MIXED_DISP_TOL
+    for k, tol in MIXED_TOLS[prec].items():
+        assert out[k] <= tol, (k, out[k], tol)
     return out
 
 
-def test_mixed_taylor_axi(ctx):
+MIXED_MODES = ["mixed", "mixed-rr32"]
+
+
+@pytest.mark.parametrize("prec", MIXED_MODES)
+def test_mixed_taylor_axi(ctx, prec):
     from dnlPython import samples
-    m = samples.taylor_axi(t_end=80e-6, precision="mixed", track_tensors=False, context=ctx)
+    m = samples.taylor_axi(t_end=80e-6, precision=prec, track_tensors=False, context=ctx)
     o = run_both(m, 80e-6)
-    e = compare_mixed(m, o)
-    assert e["disp"] <= MIXED_DISP_TOL
-    # the mixed path derives gammaCumulate from the FP64 plastic strain: it
-    # follows the plastic strain's error
-    assert e["gammaCumulate"] <= 10 * e["plasticStrain"] + 1e-6
+    compare_mixed(m, o, prec)
 
 
-def test_mixed_taylor_3d(ctx):
+@pytest.mark.parametrize("prec", MIXED_MODES)
+def test_mixed_taylor_3d(ctx, prec):
     from dnlPython import samples
-    m = samples.taylor_3d(n=6, nz=30, t_end=40e-6, precision="mixed", track_tensors=False, context=ctx)
+    m = samples.taylor_3d(n=6, nz=30, t_end=40e-6, precision=prec, track_tensors=False, context=ctx)
     o = run_both(m, 40e-6)
-    e = compare_mixed(m, o)
-    assert e["disp"] <= MIXED_DISP_TOL
+    compare_mixed(m, o, prec)
 
 
-def test_mixed_necking(ctx):
+@pytest.mark.parametrize("prec", MIXED_MODES)
+def test_mixed_necking(ctx, prec):
     from dnlPython import samples
-    m = samples.necking_3d(n=4, nz=12, t_end=40e-6, pull=5.0, precision="mixed", track_tensors=False,
+    m = samples.necking_3d(n=4, nz=12, t_end=40e-6, pull=5.0, precision=prec, track_tensors=False,
                            context=ctx)
     o = run_both(m, 40e-6)
-    e = compare_mixed(m, o)
-    assert e["disp"] <= MIXED_DISP_TOL
+    compare_mixed(m, o, prec)
 
 
-@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("prec", ["fp64", "mixed", "mixed-rr32"])
 def test_taylor_axi_published_results_on_gpu(ctx, prec):
     """The reference's Taylor-Axi sample (Samples/Taylor/Taylor-Axi/Taylor.py,
     4340 steel, 5 x 50, 287 m/s, 80 us) through the engine: published final

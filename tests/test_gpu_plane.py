@@ -26,7 +26,7 @@ def test_plate_necking_mixed(ctx):
     m = samples.necking_plane(nx=8, ny=40, t_end=20e-6, pull=5.0, precision="mixed", track_tensors=False,
                               context=ctx)
     o = run_both(m, 20e-6)
-    e = compare_mixed(m, o)
+    e = compare_mixed(m, o, "mixed")
     assert e["disp"] <= MIXED_DISP_TOL
 
 

@@ -40,7 +40,7 @@ def test_pi_necking_axi(ctx):
 def test_pi_mixed(ctx):
     m = samples.taylor_axi(t_end=40e-6, freq=1, dt_method=PI, precision="mixed", track_tensors=False, context=ctx)
     o = run_both(m, 40e-6)
-    e = compare_mixed(m, o)
+    e = compare_mixed(m, o, "mixed")
     assert e["disp"] <= MIXED_DISP_TOL
 
 
