@@ -480,6 +480,23 @@ int64_t dnlg_footprint_bytes(int elem_type, int64_t n_nodes, int64_t n_elems, in
   return footprint(elem_type, n_nodes, n_elems, precision, track_tensors);
 }
 
+int64_t dnlg_model_footprint_bytes(const dnlg_mesh *mesh, const dnlg_material *mat, const dnlg_bcs *bcs,
+                                   const dnlg_solver_params *prm) {
+  if (!mesh || !mat || !prm) return -1;
+  int64_t fp = dnlg_footprint_bytes(mesh->elem_type, mesh->n_nodes, mesh->n_elems, prm->precision,
+                                    prm->track_tensors);
+  if (fp < 0) return fp;
+  // beyond the mesh-size estimate: the law's table (tabular point pairs or
+  // the packed network) and the constant-BC entry lists
+  if (mat->law == DNLG_LAW_TABULAR && mat->table) fp += 2 * mat->n_table * (int64_t)sizeof(double);
+  if (mat->law == DNLG_LAW_ANN2SIG && mat->table) fp += mat->n_table * (int64_t)sizeof(double);
+  if (bcs) {
+    if (bcs->constant_ptr) fp += bcs->constant_ptr[mesh->n_nodes] * (int64_t)sizeof(int32_t);
+    fp += (int64_t)bcs->n_boundaries * (int64_t)sizeof(Boundary);
+  }
+  return fp;
+}
+
 int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *mat, const dnlg_bcs *bcs,
                     const dnlg_solver_params *prm, dnlg_job **out) {
   if (!ctx || !mesh || !mat || !prm || !out) return fail(ctx, DNLG_E_INVALID, "null argument");
@@ -976,6 +993,130 @@ int dnlg_job_energies(dnlg_job *job, double out[4]) {
   HIPCHK(ctx, hipMemcpyAsync(out, d_out, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return DNLG_OK;
+}
+
+// ---- checkpoint / restart -------------------------------------------------------
+namespace {
+
+// Device <-> host copies of one array, FP64 on the host; `f` the FP32
+// device array of a mixed job (or nullptr), `d` the FP64 one.
+int ip_copy(dnlg_job *job, bool to_host, double *h, double *d, float *f, int64_t count) {
+  dnlg_ctx *ctx = job->ctx;
+  if (d) {
+    if (to_host)
+      HIPCHK(ctx, hipMemcpy(h, d, count * sizeof(double), hipMemcpyDeviceToHost));
+    else
+      HIPCHK(ctx, hipMemcpy(d, h, count * sizeof(double), hipMemcpyHostToDevice));
+    return DNLG_OK;
+  }
+  std::vector<float> tmp(count);
+  if (to_host) {
+    HIPCHK(ctx, hipMemcpy(tmp.data(), f, count * sizeof(float), hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < count; k++) h[k] = tmp[k];
+  } else {
+    for (int64_t k = 0; k < count; k++) tmp[k] = (float)h[k];
+    HIPCHK(ctx, hipMemcpy(f, tmp.data(), count * sizeof(float), hipMemcpyHostToDevice));
+  }
+  return DNLG_OK;
+}
+
+// Moves the state between the host record and the device, either way.
+int state_transfer(dnlg_job *job, dnlg_state *s, bool to_host) {
+  dnlg_ctx *ctx = job->ctx;
+  const JobDev &H = job->host;
+  const int64_t N = job->N, G = job->G;
+  const int dim = job->dim;
+  Control c;
+  HIPCHK(ctx, hipMemcpy(&c, &job->dev->c[0], sizeof(Control), hipMemcpyDeviceToHost));
+  // nodal arrays: [3][N] on the host, [dim][N] on the device
+  double *nd[4] = {s->coords, s->disp_inc, s->speed, s->accel};
+  double *dv[4] = {H.x, H.fu[c.f0], H.fv[c.f0], H.fa[c.f0]};
+  double *dv2[4] = {nullptr, H.fu[1 - c.f0], H.fv[1 - c.f0], H.fa[1 - c.f0]};
+  for (int k = 0; k < 4; k++) {
+    if (to_host) {
+      std::memset(nd[k], 0, 3 * N * sizeof(double));
+      HIPCHK(ctx, hipMemcpy(nd[k], dv[k], dim * N * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+      HIPCHK(ctx, hipMemcpy(dv[k], nd[k], dim * N * sizeof(double), hipMemcpyHostToDevice));
+      if (dv2[k]) HIPCHK(ctx, hipMemcpy(dv2[k], nd[k], dim * N * sizeof(double), hipMemcpyHostToDevice));
+    }
+  }
+  const bool mixed = is_mixed(job->prec);
+  int st;
+  if ((st = ip_copy(job, to_host, s->stress, H.sig, H.sig32, 6 * G))) return st;
+  double *q = s->ip_scalars;
+  if ((st = ip_copy(job, to_host, q + DNLG_STATE_PLASTIC_STRAIN * G, H.ep, nullptr, G)) ||
+      (st = ip_copy(job, to_host, q + DNLG_STATE_PLASTIC_STRAIN_RATE * G, H.dep, H.dep32, G)) ||
+      (st = ip_copy(job, to_host, q + DNLG_STATE_GAMMA * G, H.gam, H.gam32, G)) ||
+      (st = ip_copy(job, to_host, q + DNLG_STATE_YIELD_STRESS * G, H.sy, H.sy32, G)) ||
+      (st = ip_copy(job, to_host, q + DNLG_STATE_TEMPERATURE * G, H.T, nullptr, G)) ||
+      (st = ip_copy(job, to_host, q + DNLG_STATE_INTERNAL_ENERGY * G, H.eint, nullptr, G)) ||
+      (st = ip_copy(job, to_host, q + DNLG_STATE_INELASTIC_ENERGY * G, H.einel, nullptr, G)) ||
+      (st = ip_copy(job, to_host, q + DNLG_STATE_DENSITY * G, H.rho, nullptr, G)))
+    return st;
+  double *gc = q + DNLG_STATE_GAMMA_CUMULATE * G;
+  if (!mixed) {
+    if ((st = ip_copy(job, to_host, gc, H.gamc, nullptr, G))) return st;
+  } else if (to_host) {
+    // the mixed paths keep ep only: gammaCumulate = ep / sqrt(2/3) (Element.C:589-598)
+    const double *ep = q + DNLG_STATE_PLASTIC_STRAIN * G;
+    for (int64_t g = 0; g < G; g++) gc[g] = ep[g] / kSqrt23;
+  }
+  if (job->track && s->strain && s->plastic_strain_tensor) {
+    if ((st = ip_copy(job, to_host, s->strain, H.eps, nullptr, 6 * G)) ||
+        (st = ip_copy(job, to_host, s->plastic_strain_tensor, H.epsP, nullptr, 6 * G)))
+      return st;
+  }
+  if (to_host) {
+    s->time = c.t;
+    s->time_step = c.dt;
+    s->increment = c.inc;
+  } else {
+    Control n{};
+    n.t = s->time;
+    n.dt = s->time_step;
+    n.up_to = s->time;
+    n.inc = s->increment;
+    n.f0 = c.f0;
+    HIPCHK(ctx, hipMemcpy(&job->dev->c[0], &n, sizeof(Control), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(&job->dev->c[1], &n, sizeof(Control), hipMemcpyHostToDevice));
+    job->t_now = n.t;
+    job->up_to = n.t;
+    job->inc_now = n.inc;
+  }
+  return DNLG_OK;
+}
+
+}  // namespace
+
+int dnlg_job_read_state(dnlg_job *job, dnlg_state *state) {
+  if (!job || !state || !state->coords || !state->disp_inc || !state->speed || !state->accel || !state->stress ||
+      !state->ip_scalars)
+    return job ? fail(job->ctx, DNLG_E_INVALID, "dnlg_job_read_state: null array") : DNLG_E_INVALID;
+  hipSetDevice(job->ctx->device);
+  HIPCHK(job->ctx, hipStreamSynchronize(job->ctx->stream));
+  return state_transfer(job, state, true);
+}
+
+int dnlg_job_write_state(dnlg_job *job, const dnlg_state *state) {
+  if (!job || !state || !state->coords || !state->disp_inc || !state->speed || !state->accel || !state->stress ||
+      !state->ip_scalars)
+    return job ? fail(job->ctx, DNLG_E_INVALID, "dnlg_job_write_state: null array") : DNLG_E_INVALID;
+  dnlg_ctx *ctx = job->ctx;
+  if (job->host.dt_method != DNLG_DT_COURANT)
+    return fail(ctx, DNLG_E_INVALID, "dnlg_job_write_state: Courant time step only (the power iteration's "
+                                     "iterate is not part of the state)");
+  if (!(state->time_step > 0) || !(state->time >= 0) || state->increment < 0)
+    return fail(ctx, DNLG_E_INVALID, "dnlg_job_write_state: invalid time, time step or increment");
+  if (job->track && (!state->strain || !state->plastic_strain_tensor))
+    return fail(ctx, DNLG_E_INVALID, "dnlg_job_write_state: a job with track_tensors needs the strain tensors");
+  hipSetDevice(ctx->device);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (dnlg_job_status(job) != DNLG_OK) return fail(ctx, DNLG_E_STATE, "dnlg_job_write_state: the job has failed");
+  int st = job_setup(ctx, job);
+  if (st) return fail(ctx, st, "dnlg_job_write_state: set-up of the reference Jacobian and masses failed");
+  if (dnlg_job_status(job) != DNLG_OK) return dnlg_job_status(job);
+  return state_transfer(job, const_cast<dnlg_state *>(state), false);
 }
 
 // ---- result (VTK) files and node fields ---------------------------------------

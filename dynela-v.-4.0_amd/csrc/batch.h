@@ -92,6 +92,11 @@ void batch_cache_clear(dnlg_ctx *ctx);
 // Copies the rows that the job's history files took since the last drain
 // from the device rings to the host records and files (api.hip).
 int history_drain(dnlg_job *job);
+// The set-up half of the pre-loop for a job that has not been solved: the
+// reference Jacobian (detJ0) and the lumped masses (Model.C:754-775,
+// 537-590), without the forced time step and the element pass; marks the
+// job started.  Blocking.
+int job_setup(dnlg_ctx *ctx, dnlg_job *job);
 // Writes the result file of a job held at the end of a cycle (the written
 // cycle's field0 is the other ping-pong slot) and releases it (api.hip).
 int result_file_release(dnlg_job *job);

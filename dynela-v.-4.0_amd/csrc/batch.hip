@@ -353,6 +353,27 @@ int run_preloop(dnlg_ctx *ctx, Batch &B) {
   return 0;
 }
 
+}  // namespace
+
+int job_setup(dnlg_ctx *ctx, dnlg_job *job) {
+  if (job->started) return 0;
+  Batch P;
+  int st = build_batch(ctx, {job}, P);
+  if (st) return st;
+  hipStream_t s = ctx->stream;
+  for (int hx = 0; hx < kNumTypes; hx++) launch_prep(kTypes[hx], P.d_jobs, P.prep[hx].d, P.prep[hx].n, s);
+  for (int d3 = 0; d3 < 2; d3++)
+    for (int mx = 0; mx < 2; mx++) launch_mass_gather(d3 ? 3 : 2, P.d_jobs, P.node[d3][mx].d, P.node[d3][mx].n, s);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) return DNLG_E_HIP;
+  Sync sy;
+  if (hipMemcpy(&sy, &job->dev->sync, sizeof(Sync), hipMemcpyDeviceToHost) != hipSuccess) return DNLG_E_HIP;
+  job->status = sy.st[0] != 0 ? sy.st[0] : sy.st[1];  // a negative reference Jacobian
+  job->started = true;
+  return 0;
+}
+
+namespace {
+
 Batch *find_or_build(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, int &st) {
   for (auto &b : ctx->cache)
     if (b->jobs == jobs) return b.get();

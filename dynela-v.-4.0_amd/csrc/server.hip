@@ -142,17 +142,8 @@ int dnlg_server_submit(dnlg_server *s, const dnlg_mesh *mesh, const dnlg_materia
   if (!s || !mesh || !mat || !prm || !ticket) return srv_fail(s, DNLG_E_INVALID, "null argument");
   if (mesh->n_nodes <= 0 || mesh->n_elems <= 0 || !mesh->coords || !mesh->conn)
     return srv_fail(s, DNLG_E_INVALID, "empty mesh");
-  int64_t fp = dnlg_footprint_bytes(mesh->elem_type, mesh->n_nodes, mesh->n_elems, prm->precision,
-                                    prm->track_tensors);
+  const int64_t fp = dnlg_model_footprint_bytes(mesh, mat, bcs, prm);
   if (fp < 0) return srv_fail(s, DNLG_E_INVALID, "unsupported element type");
-  // beyond the mesh-size estimate: the law's table (tabular point pairs or
-  // the packed network) and the constant-BC entry lists
-  if (mat->law == DNLG_LAW_TABULAR && mat->table) fp += 2 * mat->n_table * (int64_t)sizeof(double);
-  if (mat->law == DNLG_LAW_ANN2SIG && mat->table) fp += mat->n_table * (int64_t)sizeof(double);
-  if (bcs) {
-    if (bcs->constant_ptr) fp += bcs->constant_ptr[mesh->n_nodes] * (int64_t)sizeof(int32_t);
-    fp += (int64_t)bcs->n_boundaries * (int64_t)sizeof(dnlg::Boundary);
-  }
   if (fp > s->budget) return srv_fail(s, DNLG_E_CAPACITY, "job footprint exceeds the server's memory budget");
   std::unique_ptr<ServerJob> j(new ServerJob());
   j->ticket = s->next_ticket++;

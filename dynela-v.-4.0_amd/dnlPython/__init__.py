@@ -1184,6 +1184,31 @@ class DynELA:
     def getTotalKineticEnergy(self):
         return float(self.getEnergies()[0])
 
+    # ---- checkpoint / restart (no counterpart in the reference, whose
+    # Model::readData / writeData are commented out, Model.C:1031-1100) -------
+    def getState(self):
+        """The solver state as a dict of numpy arrays (dnlg_job_read_state):
+        time, timeStep, increment, coords / disp_inc / speed / accel [3, N]
+        (field0), stress [6, G], ip_scalars [9, G] (rows named by
+        _lib.STATE_IP_NAMES), and strain / plastic_strain_tensor with tracked
+        tensors."""
+        self._require_job()
+        n, g = self.getNodesNumber(), self.getElementsNumber() * NODES_PER_ELEMENT[self._elem_type]
+        arr, rec = _lib.state_alloc(n, g, self._solver.trackTensors)
+        self.context.check(self.context.lib.dnlg_job_read_state(self._job, ctypes.byref(rec)))
+        arr.update(time=rec.time, timeStep=rec.time_step, increment=rec.increment)
+        return arr
+
+    def setState(self, state):
+        """Continue this model from a state of getState (of this model, a
+        copy of it built the same way, or the CPU oracle's): the next solve
+        goes on from state['time'] (dnlg_job_write_state)."""
+        self.initSolve()
+        arr = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in state.items()
+               if isinstance(v, np.ndarray)}
+        rec = _lib.state_record(arr, float(state["time"]), float(state["timeStep"]), int(state["increment"]))
+        self.context.check(self.context.lib.dnlg_job_write_state(self._job, ctypes.byref(rec)))
+
     def getTotalMass(self):
         return float(self.getNodalMasses().sum())
 

@@ -67,6 +67,43 @@ class Bcs(ctypes.Structure):
                 ("initial_idx", _i32p), ("constant_ptr", _i64p), ("constant_idx", _i32p)]
 
 
+class State(ctypes.Structure):
+    """dnlg_state of include/dnlg.h (checkpoint / restart record)."""
+    _fields_ = [("time", ctypes.c_double), ("time_step", ctypes.c_double), ("increment", ctypes.c_int64),
+                ("coords", _dp), ("disp_inc", _dp), ("speed", _dp), ("accel", _dp), ("stress", _dp),
+                ("ip_scalars", _dp), ("strain", _dp), ("plastic_strain_tensor", _dp)]
+
+
+STATE_IP_NAMES = ("plasticStrain", "plasticStrainRate", "gamma", "gammaCumulate", "yieldStress", "T",
+                    "internalEnergy", "inelasticEnergy", "density")
+STATE_NODAL = ("coords", "disp_inc", "speed", "accel")
+STATE_IP_SCALARS = len(STATE_IP_NAMES)
+
+
+def state_alloc(n_nodes, n_ip_total, tensors):
+    """numpy arrays of a state record of these sizes, and the record
+    pointing at them."""
+    arr = {k: np.zeros((3, n_nodes)) for k in STATE_NODAL}
+    arr["stress"] = np.zeros((6, n_ip_total))
+    arr["ip_scalars"] = np.zeros((len(STATE_IP_NAMES), n_ip_total))
+    if tensors:
+        arr["strain"] = np.zeros((6, n_ip_total))
+        arr["plastic_strain_tensor"] = np.zeros((6, n_ip_total))
+    return arr, state_record(arr)
+
+
+def state_record(arr, time=0.0, time_step=0.0, increment=0):
+    """A State record pointing at the (C-contiguous, float64) arrays."""
+    s = State()
+    s.time, s.time_step, s.increment = time, time_step, increment
+    for k in STATE_NODAL + ("stress", "ip_scalars", "strain", "plastic_strain_tensor"):
+        a = arr.get(k)
+        if a is not None:
+            assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+            setattr(s, k, a.ctypes.data_as(_dp))
+    return s
+
+
 class SolverParams(ctypes.Structure):
     _fields_ = [("t_start", ctypes.c_double), ("t_end", ctypes.c_double), ("rho_b", ctypes.c_double),
                 ("safety", ctypes.c_double), ("dt_method", ctypes.c_int32), ("dt_freq", ctypes.c_int32),
@@ -98,9 +135,12 @@ SIGNATURES = {
     "dnlg_job_read_ip": (ctypes.c_int, [_VP, ctypes.c_int, _dp]),
     "dnlg_job_scalars": (ctypes.c_int, [_VP, _dp, _dp, _i64p]),
     "dnlg_job_energies": (ctypes.c_int, [_VP, _dp]),
+    "dnlg_job_read_state": (ctypes.c_int, [_VP, ctypes.POINTER(State)]),
+    "dnlg_job_write_state": (ctypes.c_int, [_VP, ctypes.POINTER(State)]),
     "dnlg_job_read_mass": (ctypes.c_int, [_VP, _dp]),
     "dnlg_footprint_bytes": (ctypes.c_int64, [ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int,
                                               ctypes.c_int]),
+    "dnlg_model_footprint_bytes": (ctypes.c_int64, [_VP, _VP, _VP, _VP]),
     "dnlg_job_status": (ctypes.c_int, [_VP]),
     "dnlg_ctx_set_timing": (ctypes.c_int, [_VP, ctypes.c_int]),
     "dnlg_ctx_kernel_time": (ctypes.c_int, [_VP, ctypes.c_char_p, _dp, _i64p]),

@@ -284,7 +284,55 @@ int dnlg_job_read_mass(dnlg_job *job, double *mass);
 /* device bytes a job of this size and options holds (admission control) */
 int64_t dnlg_footprint_bytes(int elem_type, int64_t n_nodes, int64_t n_elems,
                              int precision, int track_tensors);
+/* dnlg_footprint_bytes plus what the descriptors add: the law's table and
+ * the constant-BC entry lists; the figure the server admits a job by
+ * (-1 for an unsupported element type) */
+int64_t dnlg_model_footprint_bytes(const dnlg_mesh *mesh, const dnlg_material *mat, const dnlg_bcs *bcs,
+                                   const dnlg_solver_params *params);
 int dnlg_job_status(const dnlg_job *job);
+
+/* ---- checkpoint / restart --------------------------------------------------
+ * The solver state of a job as plain host arrays.  The reference keeps no
+ * checkpoint (Model::readData / writeData are commented out,
+ * Model.C:1031-1100); its dormant NodalField::write / read
+ * (NodalField.C:97-130) and IntegrationPoint::write / read
+ * (IntegrationPoint.C:142-171) cover the same fields.  A job whose state is
+ * written continues exactly as the job it was read from would (FP64: the
+ * same bits), so a solve can be stopped, saved and resumed elsewhere -- on
+ * another device, or in the CPU oracle.  All arrays are caller-allocated;
+ * nodal arrays [3 * n_nodes] SoA (2D: z components ignored / zero),
+ * integration-point arrays element-major [n_elems * n_ip] per component. */
+#define DNLG_STATE_PLASTIC_STRAIN 0      /* rows of dnlg_state.ip_scalars */
+#define DNLG_STATE_PLASTIC_STRAIN_RATE 1
+#define DNLG_STATE_GAMMA 2
+#define DNLG_STATE_GAMMA_CUMULATE 3      /* mixed paths: derived from the plastic strain, ignored on write */
+#define DNLG_STATE_YIELD_STRESS 4
+#define DNLG_STATE_TEMPERATURE 5
+#define DNLG_STATE_INTERNAL_ENERGY 6
+#define DNLG_STATE_INELASTIC_ENERGY 7
+#define DNLG_STATE_DENSITY 8
+#define DNLG_STATE_IP_SCALARS 9
+typedef struct dnlg_state {
+  double time;          /* model->currentTime */
+  double time_step;     /* Solver::timeStep (the step the next cycle starts with) */
+  int64_t increment;    /* Solver::currentIncrement */
+  double *coords;       /* current coordinates */
+  double *disp_inc;     /* field0 u: the last cycle's displacement increment (NodalField.h:32) */
+  double *speed;        /* field0 v */
+  double *accel;        /* field0 a */
+  double *stress;       /* [6][n_elems * n_ip] */
+  double *ip_scalars;   /* [DNLG_STATE_IP_SCALARS][n_elems * n_ip] */
+  double *strain;       /* [6][n_elems * n_ip] Strain, track_tensors only (else NULL) */
+  double *plastic_strain_tensor; /* [6][n_elems * n_ip], track_tensors only (else NULL) */
+} dnlg_state;
+/* Copies the state of a job that is between solves. */
+int dnlg_job_read_state(dnlg_job *job, dnlg_state *state);
+/* Sets the state of a job (Courant time step only; not after a failure).
+ * A job that has not been solved yet first takes its reference Jacobian and
+ * lumped masses from its mesh, as the pre-loop does (Model.C:754-775,
+ * 537-590); the pre-loop's forced time step and element pass are skipped:
+ * the state stands for them.  The job then continues from state->time. */
+int dnlg_job_write_state(dnlg_job *job, const dnlg_state *state);
 
 /* ---- history files --------------------------------------------------------
  * HistoryFile (Sources/dnlFEM/HistoryFile.C) attached to a model with

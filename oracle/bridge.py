@@ -65,6 +65,8 @@ def load():
                                                 ctypes.POINTER(ctypes.c_int64)]
         lib.oracle_extrapolation.argtypes = [ctypes.c_int, _dp]
         lib.oracle_job_read_node_field.argtypes = [_vp, ctypes.c_int, _dp]
+        lib.oracle_job_read_state.argtypes = [_vp, _vp]
+        lib.oracle_job_write_state.argtypes = [_vp, _vp]
         _lib = lib
     return _lib
 
@@ -184,6 +186,23 @@ class OracleJob:
         if n.value:
             self.lib.oracle_job_history_rows(self.h, fid, p(out), n.value, ctypes.byref(n))
         return out
+
+    def get_state(self):
+        """The solver state (oracle_job_read_state), as dnlPython's
+        DynELA.getState returns it."""
+        from dnlPython import _lib as dl
+        arr, rec = dl.state_alloc(self.N, self.NE * self.nip, True)
+        self.lib.oracle_job_read_state(self.h, ctypes.addressof(rec))
+        arr.update(time=rec.time, timeStep=rec.time_step, increment=rec.increment)
+        return arr
+
+    def set_state(self, state):
+        """Continue from a state of DynELA.getState or get_state
+        (oracle_job_write_state); returns the status."""
+        from dnlPython import _lib as dl
+        arr = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in state.items() if isinstance(v, np.ndarray)}
+        rec = dl.state_record(arr, float(state["time"]), float(state["timeStep"]), int(state["increment"]))
+        return self.lib.oracle_job_write_state(self.h, ctypes.addressof(rec))
 
     def pi_state(self):
         """(last power-iteration maximum, iterations of the last call)"""

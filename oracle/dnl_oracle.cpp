@@ -1887,6 +1887,73 @@ int oracle_job_read_ip(const OracleJob *J, int field, double *out) {
   return DNLG_E_INVALID;
 }
 
+/* Checkpoint / restart (dnlg_job_read_state / dnlg_job_write_state of the
+ * engine, same record).  Writing into a job that has not been solved first
+ * computes its reference Jacobian and masses from the mesh, as the pre-loop
+ * does (Model.C:754-775, 537-590); the forced step and the element pass of
+ * the pre-loop are what the state stands for.  The displacement
+ * accumulator becomes coords - initial coordinates. */
+int oracle_job_read_state(const OracleJob *J, dnlg_state *s) {
+  const int64_t N = J->N, G = J->NE * J->nip;
+  std::memcpy(s->coords, J->x.data(), 3 * N * sizeof(double));
+  std::memcpy(s->disp_inc, J->fu[J->f0].data(), 3 * N * sizeof(double));
+  std::memcpy(s->speed, J->fv[J->f0].data(), 3 * N * sizeof(double));
+  std::memcpy(s->accel, J->fa[J->f0].data(), 3 * N * sizeof(double));
+  for (int c = 0; c < 6; c++)
+    for (int64_t g = 0; g < G; g++) s->stress[c * G + g] = J->sig[g * 6 + c];
+  const std::vector<double> *rows[DNLG_STATE_IP_SCALARS] = {&J->ep, &J->dep, &J->gam, &J->gamc, &J->sy,
+                                                             &J->T, &J->eint, &J->einel, &J->rho};
+  for (int k = 0; k < DNLG_STATE_IP_SCALARS; k++)
+    std::memcpy(s->ip_scalars + k * G, rows[k]->data(), G * sizeof(double));
+  if (s->strain && !J->eps.empty())
+    for (int c = 0; c < 6; c++)
+      for (int64_t g = 0; g < G; g++) s->strain[c * G + g] = J->eps[g * 6 + c];
+  if (s->plastic_strain_tensor && !J->epsP.empty())
+    for (int c = 0; c < 6; c++)
+      for (int64_t g = 0; g < G; g++) s->plastic_strain_tensor[c * G + g] = J->epsP[g * 6 + c];
+  s->time = J->t;
+  s->time_step = J->dt;
+  s->increment = J->inc;
+  return DNLG_OK;
+}
+
+int oracle_job_write_state(OracleJob *J, const dnlg_state *s) {
+  if (J->status != DNLG_OK) return DNLG_E_STATE;
+  if (J->prm.dt_method != DNLG_DT_COURANT || !(s->time_step > 0)) return DNLG_E_INVALID;
+  if (!J->started) {
+    J->started = true;
+    if (!compute_jacobian(J, true)) return J->status;
+    compute_mass(J);
+  }
+  const int64_t N = J->N, G = J->NE * J->nip;
+  for (int64_t q = 0; q < 3 * N; q++) {
+    const double x0 = J->x[q] - J->disp[q];
+    J->x[q] = s->coords[q];
+    J->disp[q] = s->coords[q] - x0;
+  }
+  for (int k = 0; k < 2; k++) {
+    std::memcpy(J->fu[k].data(), s->disp_inc, 3 * N * sizeof(double));
+    std::memcpy(J->fv[k].data(), s->speed, 3 * N * sizeof(double));
+    std::memcpy(J->fa[k].data(), s->accel, 3 * N * sizeof(double));
+  }
+  for (int c = 0; c < 6; c++)
+    for (int64_t g = 0; g < G; g++) J->sig[g * 6 + c] = s->stress[c * G + g];
+  std::vector<double> *rows[DNLG_STATE_IP_SCALARS] = {&J->ep, &J->dep, &J->gam, &J->gamc, &J->sy,
+                                                       &J->T, &J->eint, &J->einel, &J->rho};
+  for (int k = 0; k < DNLG_STATE_IP_SCALARS; k++)
+    std::memcpy(rows[k]->data(), s->ip_scalars + k * G, G * sizeof(double));
+  if (!J->eps.empty() && s->strain)
+    for (int c = 0; c < 6; c++)
+      for (int64_t g = 0; g < G; g++) J->eps[g * 6 + c] = s->strain[c * G + g];
+  if (!J->epsP.empty() && s->plastic_strain_tensor)
+    for (int c = 0; c < 6; c++)
+      for (int64_t g = 0; g < G; g++) J->epsP[g * 6 + c] = s->plastic_strain_tensor[c * G + g];
+  J->t = s->time;
+  J->dt = s->time_step;
+  J->inc = s->increment;
+  return DNLG_OK;
+}
+
 int oracle_job_scalars(const OracleJob *J, double *time, double *time_step, int64_t *increment) {
   if (time) *time = J->t;
   if (time_step) *time_step = J->dt;

@@ -2,6 +2,8 @@
 footprint and advanced together in short steps finish with the same results
 as the same job solved alone.  Per-job task lists and reductions make the
 results bit-identical, in FP64 and in mixed precision."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -62,8 +64,8 @@ def test_server_results_equal_alone(ctx, prec):
 def _footprint(model):
     from dnlPython import _lib
     a = model.descriptors()
-    return _lib.load().dnlg_footprint_bytes(a.mesh.elem_type, a.mesh.n_nodes, a.mesh.n_elems,
-                                            a.params.precision, a.params.track_tensors)
+    return _lib.load().dnlg_model_footprint_bytes(ctypes.byref(a.mesh), ctypes.byref(a.material),
+                                                  ctypes.byref(a.bcs), ctypes.byref(a.params))
 
 
 def test_server_admission_by_footprint(ctx):
