@@ -444,6 +444,13 @@ int dnlg_ctx_create(int device, dnlg_ctx **out) {
     delete ctx;
     return DNLG_E_HIP;
   }
+  // the device address of the allocation-free entry points' scratch, once
+  // per device, with the device current
+  if (energy_scratch_init(device) != DNLG_OK) {
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return DNLG_E_HIP;
+  }
   {
     std::lock_guard<std::mutex> lk(g_tables_mu);
     if (!g_tables_uploaded[device]) {
@@ -1117,6 +1124,27 @@ int dnlg_job_write_state(dnlg_job *job, const dnlg_state *state) {
   if (st) return fail(ctx, st, "dnlg_job_write_state: set-up of the reference Jacobian and masses failed");
   if (dnlg_job_status(job) != DNLG_OK) return dnlg_job_status(job);
   return state_transfer(job, const_cast<dnlg_state *>(state), false);
+}
+
+int dnlg_job_energies_static(dnlg_job *job, double out[4]) {
+  if (!job || !out) return DNLG_E_INVALID;
+  dnlg_ctx *ctx = job->ctx;
+  const int nb = energy_blocks(job->N, job->G);
+  if (nb > kEnergyScratchBlocks)
+    return fail(ctx, DNLG_E_CAPACITY,
+                "dnlg_job_energies_static: the job needs " + std::to_string(nb) + " reduction blocks, the static "
+                "scratch holds " + std::to_string(kEnergyScratchBlocks) + " (at most " +
+                std::to_string((int64_t)kEnergyScratchBlocks * kEnergyItemsPerBlock) +
+                " nodes or integration points); use dnlg_job_energies");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  double *ws = energy_scratch(ctx->device);
+  if (!ws) return fail(ctx, DNLG_E_STATE, "dnlg_job_energies_static: scratch of this device not set up");
+  double *d_out = ws + 6 * (int64_t)kEnergyScratchBlocks;
+  launch_energies(job->dev, job->N, job->G, job->dim, job->type == DNLG_EL_QUA4N_AX, ws, d_out, ctx->stream);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipMemcpyAsync(out, d_out, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return DNLG_OK;
 }
 
 // ---- result (VTK) files and node fields ---------------------------------------

@@ -13,11 +13,22 @@
 // few ulp of the exactly rounded sum whatever the node count.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "kernels.h"
 
 namespace dnlg {
 
+// Scratch of dnlg_job_energies_static: the partials of at most
+// kEnergyScratchBlocks blocks and the four results.  One copy per GPU, at
+// its own address on each; host code uses the per-device address taken by
+// energy_scratch_init, never this name (on the host it is only a stand-in).
+static __device__ __align__(256) double scratch[6 * kEnergyScratchBlocks + 4];
+
 namespace {
+
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {nullptr};
 
 struct Acc {
   double s, c;  // sum, accumulated rounding error
@@ -112,6 +123,22 @@ __global__ __launch_bounds__(kEnergyThreads) void k_energy_final(const double *p
 }
 
 }  // namespace
+
+int energy_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return DNLG_E_INVALID;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return DNLG_OK;
+  void *ptr = nullptr;
+  if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return DNLG_E_HIP;
+  g_scratch_ptr[device] = (double *)ptr;
+  return DNLG_OK;
+}
+
+double *energy_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
 
 int energy_blocks(int64_t N, int64_t G) {
   const int64_t items = N > G ? N : G;

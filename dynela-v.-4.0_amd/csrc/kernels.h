@@ -71,6 +71,15 @@ void launch_node_field(JobDev *job, int64_t N, int src, int comp, double *out, h
 // must be between launch sequences (its record in control slot 0).
 constexpr int64_t kEnergyItemsPerBlock = 4096;
 int energy_blocks(int64_t N, int64_t G);
+// The file-scope device scratch of the allocation-free variant: partials of
+// at most kEnergyScratchBlocks blocks (jobs of up to 16.7 M nodes and
+// integration points) and the results, 192 KiB per GPU.  energy_scratch_init
+// takes its address on the current device (once per device, at context
+// creation); energy_scratch returns it (nullptr before).
+constexpr int kEnergyScratchBlocks = 4096;
+constexpr int kMaxDevices = 64;
+int energy_scratch_init(int device);
+double *energy_scratch(int device);
 void launch_energies(JobDev *job, int64_t N, int64_t G, int dim, bool axi, double *part, double *out,
                      hipStream_t s);
 

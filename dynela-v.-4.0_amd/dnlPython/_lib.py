@@ -135,6 +135,7 @@ SIGNATURES = {
     "dnlg_job_read_ip": (ctypes.c_int, [_VP, ctypes.c_int, _dp]),
     "dnlg_job_scalars": (ctypes.c_int, [_VP, _dp, _dp, _i64p]),
     "dnlg_job_energies": (ctypes.c_int, [_VP, _dp]),
+    "dnlg_job_energies_static": (ctypes.c_int, [_VP, _dp]),
     "dnlg_job_read_state": (ctypes.c_int, [_VP, ctypes.POINTER(State)]),
     "dnlg_job_write_state": (ctypes.c_int, [_VP, ctypes.POINTER(State)]),
     "dnlg_job_read_mass": (ctypes.c_int, [_VP, _dp]),

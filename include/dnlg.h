@@ -279,6 +279,16 @@ int dnlg_job_scalars(dnlg_job *job, double *time, double *time_step,
  * energy sum_ip rho0 w detJ0 e_int, out[2] dissipated plastic work
  * sum_ip rho0 w detJ0 e_inel, out[3] out[0] + out[1] */
 int dnlg_job_energies(dnlg_job *job, double out[4]);
+/* dnlg_job_energies without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (tight serving
+ * loops, code that forbids hipMalloc).  Same kernels, same bits as
+ * dnlg_job_energies; the reduction's partials go to a fixed scratch array
+ * of 192 KiB per GPU held until the process ends, so the job may have at
+ * most 4096 x 4096 = 16 777 216 nodes and as many integration points
+ * (DNLG_E_CAPACITY beyond).  Every call on one GPU shares that array:
+ * calls that overlap on one GPU (several threads or contexts on the same
+ * device) race and must be serialised by the caller. */
+int dnlg_job_energies_static(dnlg_job *job, double out[4]);
 /* lumped nodal masses, [n_nodes] (Model.C:537-590) */
 int dnlg_job_read_mass(dnlg_job *job, double *mass);
 /* device bytes a job of this size and options holds (admission control) */
