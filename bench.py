@@ -21,9 +21,9 @@ algo_bytes_per_elem); ``traffic`` is the rocprofv3 measurement committed under
 profiles/ for this kernel and workload.
 
 cpu_baseline: the FP64 CPU oracle (the restatement of the reference loop,
-test infrastructure under oracle/) timed on this host on a bounded sample of
-the same model (the first 1 us of the impact), element phases OpenMP
-parallel.
+test infrastructure under oracle/) timed on this host, at 1 thread and on all
+physical cores, over 20 cycles of the same model from 40 us (the plastic
+phase), starting from the FP64 engine's state at that time.
 
 Other workloads: --workload necking (round bar in tension, e.g. --n 40
 --nz 208 for 998 400 hexes), sweep (batched seeded parameter sweep), serve
@@ -99,7 +99,8 @@ def run_precision(prec, ctx, steps, warmup, n, nz, t_end, workload="taylor3d", p
     energies = last.getEnergies()       # kinetic, internal, plastic work, kinetic + internal
     mass = last.getNodalMasses()
     out = dict(wall=wall, cycles=cycles, ne=ne, nn=nn, elem_ms=el_ms, elem_launches=el_n,
-               value=ne * sum(cycles) / wall, disp=disp, ep=ep, energies=energies, mass_total=float(mass.sum()))
+               value=ne * sum(cycles) / wall, disp=disp, ep=ep, energies=energies, mass_total=float(mass.sum()),
+               schedule=last.getSchedule())
     for m in models:
         m.close()
     return out
@@ -129,8 +130,10 @@ def run_sweep(prec, ctx, n_jobs, t_end, seed=20261203, max_cycles=None):
     capped = [(i, models[i].getCurrentTime(), models[i].getTimeStep()) for i, s in enumerate(statuses)
               if s == 0 and models[i].getCurrentTime() < t_end]
     if capped:
-        print(f"[bench] sweep {prec}: jobs stopped by the cycle cap (index, time, dt): {capped}", file=sys.stderr,
-              flush=True)
+        ts = np.array([c[1] for c in capped])
+        print(f"[bench] sweep {prec}: {len(capped)} jobs stopped by the cycle cap {max_cycles} at t = "
+              f"{ts.min() * 1e6:.1f}-{ts.max() * 1e6:.1f} us (time step collapsed to "
+              f"{min(c[2] for c in capped):.2e}-{max(c[2] for c in capped):.2e} s)", file=sys.stderr, flush=True)
     ctx.set_timing(False)
     el_ms, el_n = ctx.kernel_time("element")
     ec = sum(m.getElementsNumber() * m.getIncrement() for m in models)
@@ -279,31 +282,38 @@ def physical_cores(info):
     return max(1, usable // max(tpc, 1))
 
 
-def cpu_baseline(n, nz, sample_t=1e-6, workload="taylor3d", pull=1.0, threads=(1, None)):
+def cpu_baseline(n, nz, ctx, t_window=40e-6, window_cycles=20, workload="taylor3d", pull=1.0, threads=(1, None)):
     """The FP64 CPU oracle (restatement of the reference loop, test
-    infrastructure) on the same model, timed on this host for the first
-    sample_t of the run, once per thread count (None: all physical cores)."""
+    infrastructure) on the same model, timed on this host over a window of
+    the plastic phase: the FP64 engine runs the job to t_window, its state
+    goes to the oracle (checkpoint / restart), and the oracle advances
+    window_cycles cycles from there, once per thread count (None: all
+    physical cores)."""
     from oracle.bridge import OracleJob
     info = host_cpu()
+    m = make_model("fp64", ctx, n, nz, 80e-6, workload, pull)
+    m.solve(t_window)
+    st = m.getState()
+    ne = m.getElementsNumber()
+    t_stop = st["time"] + window_cycles * st["timeStep"] * (1.0 - 1e-12)
     res = []
     for th in threads:
         th = physical_cores(info) if th is None else th
-        m = make_model("fp64", None, n, nz, 80e-6, workload, pull)
         job = OracleJob(m.descriptors(), threads=th)
+        assert job.set_state(st) == 0
         t0 = time.perf_counter()
-        job.solve(sample_t)
+        status = job.solve(t_stop)
         wall = time.perf_counter() - t0
-        _, _, inc = job.scalars()
-        ne = m.getElementsNumber()
-        res.append(dict(value=ne * inc / wall, cores=th, wall_s=wall, cycles=inc))
-        print(f"[bench] cpu baseline {th} thread(s): {res[-1]['value']:.4g} elem-cycles/s, {wall:.1f} s",
-              file=sys.stderr, flush=True)
+        cyc = job.scalars()[2] - st["increment"]
+        res.append(dict(value=ne * cyc / wall, cores=th, wall_s=wall, cycles=cyc, status=status))
+        print(f"[bench] cpu baseline {th} thread(s): {res[-1]['value']:.4g} elem-cycles/s, {cyc} cycles in "
+              f"{wall:.1f} s", file=sys.stderr, flush=True)
+    m.close()
     best = max(res, key=lambda r: r["value"])
-    ne = make_model("fp64", None, n, nz, 80e-6, workload, pull).getElementsNumber()
     return dict(value=best["value"], unit="elem-cycles/s", cores=best["cores"], kind="port",
-                sample=f"first {sample_t * 1e6:g} us of the same model ({best['cycles']} cycles x {ne} elements), "
-                       f"FP64 oracle, element phases OpenMP-parallel",
-                per_threads=[{k: r[k] for k in ("cores", "value", "wall_s")} for r in res], host=info)
+                sample=f"{best['cycles']} cycles x {ne} elements of the same model from t = {t_window * 1e6:g} us "
+                       f"(plastic phase; state from the FP64 engine), FP64 oracle, element phases OpenMP-parallel",
+                per_threads=[{k: r[k] for k in ("cores", "value", "wall_s", "cycles")} for r in res], host=info)
 
 
 def copy_calibration(mib):
@@ -361,7 +371,8 @@ def main():
     ap.add_argument("--nz", type=int, default=100)
     ap.add_argument("--t-end", type=float, default=80e-6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=float, default=1e-6, help="simulated time of the CPU baseline sample")
+    ap.add_argument("--cpu-window", type=float, default=40e-6, help="start of the CPU baseline's window (s)")
+    ap.add_argument("--cpu-cycles", type=int, default=20, help="cycles of the CPU baseline's window")
     ap.add_argument("--no-batched", action="store_true", help="leave out the batched sweep leg")
     ap.add_argument("--no-serve", action="store_true", help="leave out the serving leg")
     ap.add_argument("--batched-jobs", type=int, default=256)
@@ -451,7 +462,8 @@ def main():
         "vs_baseline": None,
         "dtype": dtype(head),
         "data": "synthetic",
-        "config": {"workload": wl, "precision": head, "cycles_per_step": r["cycles"][0]},
+        "config": {"workload": wl, "precision": head, "cycles_per_step": r["cycles"][0],
+                   "schedule": r["schedule"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.workload, head, args.n, args.nz),
@@ -490,8 +502,8 @@ def main():
                            "config": serve_config(sv, args.serve_jobs, head, 32, args.cycle_budget)}
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (oracle)", file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline(args.n, args.nz, sample_t=args.cpu_sample, workload=args.workload,
-                                            pull=args.pull)
+        line["cpu_baseline"] = cpu_baseline(args.n, args.nz, ctx, t_window=min(args.cpu_window, 0.5 * args.t_end),
+                                            window_cycles=args.cpu_cycles, workload=args.workload, pull=args.pull)
     print(json.dumps(line), flush=True)
 
 

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "engine.h"
+#include "sched.h"
 
 struct dnlg_ctx;
 
@@ -26,6 +27,10 @@ struct dnlg_history {
 
 struct dnlg_job {
   dnlg_ctx *ctx = nullptr;
+  // launch schedule (sched.h): set explicitly (dnlg_job_set_schedule) or
+  // resolved from the context's rules when a batch is built
+  dnlg_schedule sched{};
+  bool sched_set = false;
   int type = 0, dim = 0, nn = 0, prec = 0, track = 0;
   int64_t N = 0, NE = 0, G = 0;
   double t_end = 0.0;
@@ -75,12 +80,27 @@ struct dnlg_ctx {
   std::vector<dnlg::BatchPtr> cache;
   int graph_cycles = 16;  // cycles per captured HIP graph (0 = plain launches)
   int pi_slots = 2;       // power iterations per time-step cycle inside a sequence
+  // schedule rules (dnlg_ctx_add_schedule): the first rule matching a job's
+  // element type, precision and size gives its schedule
+  struct SchedRule {
+    int type, prec;
+    int64_t min_elems, max_elems;
+    dnlg_schedule sched;
+  };
+  std::vector<SchedRule> sched_rules;
   // device workspace of the energy reduction (grown on demand)
   double *energy_ws = nullptr;
   int64_t energy_ws_doubles = 0;
 };
 
 namespace dnlg {
+
+// The schedule a job runs with: its own if set, else the first matching
+// rule of the context, else the defaults (sched.h); fields left 0 in a rule
+// take the defaults.
+dnlg_schedule resolve_schedule(const dnlg_ctx *ctx, const dnlg_job *job);
+int job_variant(const dnlg_job *job);
+bool schedule_valid(int type, int precision, int law, const dnlg_schedule &s);
 
 // Advance the jobs together toward up_to[i] (clipped to each job's end
 // time); with max_cycles >= 0, return after about that many cycles (whole

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <array>
 #include <map>
 
 #include "batch.h"
@@ -28,32 +29,40 @@ constexpr int kTypes[kNumTypes] = {DNLG_EL_QUA4N_AX, DNLG_EL_HEX8N_3D, DNLG_EL_Q
 inline int type_dim(int type) { return type == DNLG_EL_HEX8N_3D ? 3 : 2; }
 inline int type_nodes(int type) { return type == DNLG_EL_HEX8N_3D ? 8 : 4; }
 
-// Element kernel variants of a type: FP64, mixed (FP64 radial return),
-// FP64 with a network law, mixed with the FP32 radial return.
-constexpr int kElemVariants = 4;
-inline int elem_variant(const dnlg_job *j) {
-  if (j->prec == DNLG_PREC_MIXED) return 1;
-  if (j->prec == DNLG_PREC_MIXED_RR32) return 3;
-  return j->host.law == DNLG_LAW_ANN2SIG ? 2 : 0;
-}
-inline bool variant_mixed(int mx) { return mx == 1 || mx == 3; }
+inline bool variant_mixed(int mx) { return mx == kVarMixed || mx == kVarMixedRR32; }
 
-// One element launch of variant mx over a task list.
-void launch_variant(int type, int mx, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int derive,
-                    hipStream_t s, const hipEvent_t *ev) {
+// One element launch of variant mx with schedule (block, waves) over a task
+// list.
+void launch_variant(int type, int mx, int block, int waves, JobDev *const *jobs, const Task *tasks, int nblk, int cs,
+                    int derive, hipStream_t s, const hipEvent_t *ev) {
   if (variant_mixed(mx))
-    launch_elem_mixed(type, jobs, tasks, nblk, cs, derive, s, ev, mx == 1);
+    launch_elem_mixed(type, block, waves, jobs, tasks, nblk, cs, derive, s, ev, mx == kVarMixed);
   else
-    launch_elem_fp64(type, jobs, tasks, nblk, cs, derive, s, ev, mx == 2);
+    launch_elem_fp64(type, block, waves, jobs, tasks, nblk, cs, derive, s, ev, mx == kVarFP64Ann);
 }
+
+// Element launches of a batch: one per (element type, variant, schedule).
+struct ElemGroup {
+  int type, variant, block, waves;
+  TaskList tl;
+};
+// Corrector launches: one per (dimension, force precision, slot batch).
+struct CorrGroup {
+  int dim;
+  bool f32;
+  int batch;
+  TaskList tl;
+};
 
 struct Batch {
   std::vector<dnlg_job *> jobs;
   JobDev **d_jobs = nullptr;
-  TaskList elem[kNumTypes][kElemVariants];  // [type][elem_variant]
+  std::vector<ElemGroup> elem;
+  std::vector<CorrGroup> corr;
+  int graph_cycles = kDefaultGraphCycles;  // cycles per captured graph (the jobs' schedules)
   TaskList prep[kNumTypes];     // [type]
   TaskList cour[kNumTypes];     // [type]: Courant lengths and power-iteration products
-  TaskList node[2][2];  // [dim==3?][mixed?]
+  TaskList node[2][2];  // [dim==3?][mixed?]: mass assembly, power iteration
   TaskList step;        // cycle control + field0 BCs, over the BC node lists
   TaskList close;       // end of a launch sequence, one block per job
   // history files: evaluation (item ranges and kinetic-energy node ranges,
@@ -138,20 +147,24 @@ int build_batch(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, Batch &B) {
   B.d_pend = (int *)p;
   B.h_pend.assign(n, 0);
   int st;
+  // schedules, then element groups in (type, variant, block, waves) order
+  for (auto *j : jobs)
+    if (!j->sched_set) j->sched = resolve_schedule(ctx, j);
+  B.graph_cycles = 0;
+  for (auto *j : jobs) B.graph_cycles = std::max(B.graph_cycles, (int)j->sched.graph_cycles);
+  std::map<std::array<int, 4>, std::vector<int>> groups;
+  for (int i = 0; i < n; i++)
+    groups[{jobs[i]->type, job_variant(jobs[i]), jobs[i]->sched.elem_block, jobs[i]->sched.waves}].push_back(i);
+  for (auto &g : groups) {
+    std::vector<int64_t> cnt;
+    for (int i : g.second) cnt.push_back(jobs[i]->NE);
+    ElemGroup eg{g.first[0], g.first[1], g.first[2], g.first[3], TaskList{}};
+    const int nn = type_nodes(eg.type);
+    if ((st = upload_tasks(B, xcd_order(make_tasks(g.second, cnt, eg.block / nn)), eg.tl))) return st;
+    B.elem.push_back(eg);
+  }
   for (int hx = 0; hx < kNumTypes; hx++) {
     const int type = kTypes[hx];
-    const int nn = type_nodes(type);
-    for (int mx = 0; mx < kElemVariants; mx++) {
-      std::vector<int> idx;
-      std::vector<int64_t> cnt;
-      for (int i = 0; i < n; i++)
-        if (jobs[i]->type == type && elem_variant(jobs[i]) == mx) {
-          idx.push_back(i);
-          cnt.push_back(jobs[i]->NE);
-        }
-      const int threads = variant_mixed(mx) ? elem_block_threads_mixed(type) : elem_block_threads(type);
-      if ((st = upload_tasks(B, xcd_order(make_tasks(idx, cnt, threads / nn)), B.elem[hx][mx]))) return st;
-    }
     std::vector<int> idx;
     std::vector<int64_t> cnt;
     for (int i = 0; i < n; i++)
@@ -186,6 +199,17 @@ int build_batch(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, Batch &B) {
         }
       if ((st = upload_tasks(B, make_tasks(idx, cnt, kNodeBlock), B.node[d3][mx]))) return st;
     }
+  {
+    std::map<std::array<int, 3>, std::vector<int>> cg;
+    for (int i = 0; i < n; i++) cg[{jobs[i]->dim, is_mixed(jobs[i]->prec) ? 1 : 0, jobs[i]->sched.corr_batch}].push_back(i);
+    for (auto &g : cg) {
+      std::vector<int64_t> cnt;
+      for (int i : g.second) cnt.push_back(jobs[i]->N);
+      CorrGroup c{g.first[0], g.first[1] == 1, g.first[2], TaskList{}};
+      if ((st = upload_tasks(B, make_tasks(g.second, cnt, kNodeBlock), c.tl))) return st;
+      B.corr.push_back(c);
+    }
+  }
   {
     // ranges of about equal size, hist_blocks(count) of them
     auto ranges = [](std::vector<Task> &t, int job, int64_t count, bool ke) {
@@ -230,15 +254,11 @@ void launch_close(dnlg_ctx *ctx, Batch &B, int cs) {
 int launch_element(dnlg_ctx *ctx, Batch &B, const hipEvent_t *ev, int cs, int derive) {
   hipStream_t s = ctx->stream;
   int k = 0;
-  for (int hx = 0; hx < kNumTypes; hx++) {
-    const int type = kTypes[hx];
-    for (int mx = 0; mx < kElemVariants; mx++) {
-      const TaskList &L = B.elem[hx][mx];
-      if (L.n <= 0) continue;
-      const hipEvent_t *e = ev ? &ev[2 * k] : nullptr;
-      launch_variant(type, mx, B.d_jobs, L.d, L.n, cs, derive, s, e);
-      k++;
-    }
+  for (const ElemGroup &g : B.elem) {
+    if (g.tl.n <= 0) continue;
+    const hipEvent_t *e = ev ? &ev[2 * k] : nullptr;
+    launch_variant(g.type, g.variant, g.block, g.waves, B.d_jobs, g.tl.d, g.tl.n, cs, derive, s, e);
+    k++;
   }
   return k;
 }
@@ -280,9 +300,7 @@ void launch_history(dnlg_ctx *ctx, Batch &B, int cs, int preloop) {
 
 void launch_post(dnlg_ctx *ctx, Batch &B, int cs, bool courant, bool fold = false) {
   hipStream_t s = ctx->stream;
-  for (int d3 = 0; d3 < 2; d3++)
-    for (int mx = 0; mx < 2; mx++)
-      launch_correct(d3 ? 3 : 2, mx == 1, B.d_jobs, B.node[d3][mx].d, B.node[d3][mx].n, cs, fold ? 1 : 0, s);
+  for (const CorrGroup &g : B.corr) launch_correct(g.dim, g.f32, g.batch, B.d_jobs, g.tl.d, g.tl.n, cs, fold ? 1 : 0, s);
   // Explicit::endStep's history rows, before the time-step computation
   launch_history(ctx, B, cs, 0);
   if (courant && B.has_courant)
@@ -342,11 +360,8 @@ int run_preloop(dnlg_ctx *ctx, Batch &B) {
     if (st) return st;
     launch_pi_finish(B.d_jobs, n, 1, s);
   }
-  for (int hx = 0; hx < kNumTypes; hx++) {
-    const int type = kTypes[hx];
-    for (int mx = 0; mx < kElemVariants; mx++)
-      launch_variant(type, mx, B.d_jobs, B.elem[hx][mx].d, B.elem[hx][mx].n, 0, 0, s, nullptr);
-  }
+  for (const ElemGroup &g : B.elem)
+    launch_variant(g.type, g.variant, g.block, g.waves, B.d_jobs, g.tl.d, g.tl.n, 0, 0, s, nullptr);
   // the pre-loop's history rows (Explicit.C:182)
   launch_history(ctx, B, 0, 1);
   if (hipGetLastError() != hipSuccess) return DNLG_E_HIP;
@@ -354,6 +369,42 @@ int run_preloop(dnlg_ctx *ctx, Batch &B) {
 }
 
 }  // namespace
+
+int job_variant(const dnlg_job *j) {
+  if (j->prec == DNLG_PREC_MIXED) return kVarMixed;
+  if (j->prec == DNLG_PREC_MIXED_RR32) return kVarMixedRR32;
+  return j->host.law == DNLG_LAW_ANN2SIG ? kVarFP64Ann : kVarFP64;
+}
+
+static int variant_of(int precision, int law) {
+  if (precision == DNLG_PREC_MIXED) return kVarMixed;
+  if (precision == DNLG_PREC_MIXED_RR32) return kVarMixedRR32;
+  return law == DNLG_LAW_ANN2SIG ? kVarFP64Ann : kVarFP64;
+}
+
+bool schedule_valid(int type, int precision, int law, const dnlg_schedule &s) {
+  const int v = variant_of(precision, law);
+  return elem_sched_valid(type, v, s.elem_block, s.waves) && corr_batch_valid(precision != DNLG_PREC_FP64, s.corr_batch) &&
+         s.graph_cycles >= 2 && s.graph_cycles <= 4096;
+}
+
+dnlg_schedule resolve_schedule(const dnlg_ctx *ctx, const dnlg_job *job) {
+  const int v = job_variant(job);
+  dnlg_schedule s{default_elem_block(job->type), default_waves(v), default_corr_batch(is_mixed(job->prec)),
+                  ctx->graph_cycles > 0 ? ctx->graph_cycles : kDefaultGraphCycles};
+  for (const auto &r : ctx->sched_rules)
+    if (r.type == job->type && r.prec == job->prec && job->NE >= r.min_elems &&
+        (r.max_elems <= 0 || job->NE < r.max_elems)) {
+      dnlg_schedule t = s;
+      if (r.sched.elem_block) t.elem_block = r.sched.elem_block;
+      if (r.sched.waves) t.waves = r.sched.waves;
+      if (r.sched.corr_batch) t.corr_batch = r.sched.corr_batch;
+      if (r.sched.graph_cycles) t.graph_cycles = r.sched.graph_cycles;
+      if (schedule_valid(job->type, job->prec, job->host.law, t)) s = t;
+      break;
+    }
+  return s;
+}
 
 int job_setup(dnlg_ctx *ctx, dnlg_job *job) {
   if (job->started) return 0;
@@ -422,7 +473,7 @@ hipGraphExec_t graph_for(dnlg_ctx *ctx, Batch &B, int kind, int phase, int freq,
   const auto key = std::make_pair(kind * 65536 + (phase < 0 ? 0 : freq), phase);
   auto it = B.graphs.find(key);
   if (it != B.graphs.end()) return it->second;
-  const int len = graph_len(ctx->graph_cycles, phase, freq);
+  const int len = graph_len(B.graph_cycles, phase, freq);
   hipGraphExec_t g = nullptr;
   switch (kind) {
     case kWhole:
@@ -595,7 +646,7 @@ int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vec
     int chunk = (int)std::min(need, 4096.0);
     if (B->hist_cap > 0) {
       // at most one history row per cycle and file between two drains
-      const int per = use_graph ? graph_len(ctx->graph_cycles, phase, freq) : 1;
+      const int per = use_graph ? graph_len(B->graph_cycles, phase, freq) : 1;
       chunk = std::min(chunk, std::max(1, B->hist_cap / per) * per);
     }
     launched += chunk;
@@ -604,7 +655,7 @@ int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vec
       if (timed) {
         g_pre = graph_for(ctx, *B, kPre, -1, 1, st);
         if (!st) g_post = graph_for(ctx, *B, kPost, -1, 1, st);
-        if (!st && graph_len(ctx->graph_cycles, phase, freq) > 1) g_rest = graph_for(ctx, *B, kRest, phase, freq, st);
+        if (!st && graph_len(B->graph_cycles, phase, freq) > 1) g_rest = graph_for(ctx, *B, kRest, phase, freq, st);
       } else {
         g_whole = graph_for(ctx, *B, kWhole, phase, freq, st);
       }
@@ -616,10 +667,10 @@ int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vec
     int cs = 0;  // control slot of the cycle in flight (slot 0 between sequences)
     if (timed) {
       // units of one sampled cycle + the rest of a graph's cycles
-      const int per = use_graph ? graph_len(ctx->graph_cycles, phase, freq) : 1;
+      const int per = use_graph ? graph_len(B->graph_cycles, phase, freq) : 1;
       const int units = std::min((chunk + per - 1) / per, kTimedChunk);
       launched += (int64_t)units * per - chunk;
-      while (ctx->events.size() < (size_t)2 * kElemVariants * kNumTypes * units) {
+      while (ctx->events.size() < (size_t)2 * B->elem.size() * units) {
         hipEvent_t e;
         if (hipEventCreate(&e) != hipSuccess) return DNLG_E_HIP;
         ctx->events.push_back(e);
@@ -645,7 +696,7 @@ int batch_run(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, const std::vec
         }
       }
     } else if (use_graph) {
-      const int len = graph_len(ctx->graph_cycles, phase, freq);
+      const int len = graph_len(B->graph_cycles, phase, freq);
       const int reps = (chunk + len - 1) / len;
       launched += (int64_t)reps * len - chunk;
       for (int r = 0; r < reps; r++)

@@ -1,4 +1,4 @@
-// elem_mixed.h -- mixed-precision element kernel (included by kernels.hip).
+// elem_mixed.h -- mixed-precision element kernel (instantiated by elem_mixed.hip).
 //
 // Same phases as k_elem_fp64 (Element.C:1103-1161, 362-404, 407-622,
 // 1243-1256) with FP32 geometry, strain, stress, radial return and forces,
@@ -29,6 +29,12 @@
 // transform of T over the Gauss-point lanes.  Lane b <-> node / Gauss point
 // b ^ ((b >> 1) & 1) of the reference numbering (ElHex8N3D.C:30-231,
 // ElQua4NAx.C:30-110), whose local signs are the bits of b.
+
+#pragma once
+
+#include "control.h"
+#include "material.h"
+#include "polar_fp64.h"
 
 namespace dnlg {
 
@@ -157,6 +163,48 @@ __device__ __forceinline__ float group_sum(float v) {
 
 __device__ __forceinline__ float sgn(int b, int d) { return ((b >> d) & 1) ? 1.f : -1.f; }
 
+// Jacobian J_ij = dx_i/dxi_j at the Gauss point xi from the Walsh modes mx
+// of the element's coordinates (x(xi) = sum_q mx_q prod_{d in q} xi_d),
+// its determinant and inverse.
+template <int DIM, int NN>
+__device__ __forceinline__ void jacobian_inverse(const float (&mx)[NN][DIM], const float (&xi)[DIM], float &detJ,
+                                                 float (&inv)[DIM][DIM]) {
+  float Jm[DIM][DIM];
+#pragma unroll
+  for (int i = 0; i < DIM; i++) {
+    if constexpr (DIM == 3) {
+      Jm[i][0] = mx[1][i] + mx[3][i] * xi[1] + mx[5][i] * xi[2] + mx[7][i] * (xi[1] * xi[2]);
+      Jm[i][1] = mx[2][i] + mx[3][i] * xi[0] + mx[6][i] * xi[2] + mx[7][i] * (xi[0] * xi[2]);
+      Jm[i][2] = mx[4][i] + mx[5][i] * xi[0] + mx[6][i] * xi[1] + mx[7][i] * (xi[0] * xi[1]);
+    } else {
+      Jm[i][0] = mx[1][i] + mx[3][i] * xi[1];
+      Jm[i][1] = mx[2][i] + mx[3][i] * xi[0];
+    }
+  }
+  if constexpr (DIM == 3) {
+    const float *dd = &Jm[0][0];
+    detJ = dd[1] * dd[5] * dd[6] - dd[2] * dd[4] * dd[6] + dd[2] * dd[3] * dd[7] - dd[0] * dd[5] * dd[7] -
+           dd[1] * dd[3] * dd[8] + dd[0] * dd[4] * dd[8];
+    const float id = __builtin_amdgcn_rcpf(detJ);
+    inv[0][0] = (dd[4] * dd[8] - dd[5] * dd[7]) * id;
+    inv[0][1] = (dd[2] * dd[7] - dd[1] * dd[8]) * id;
+    inv[0][2] = (dd[1] * dd[5] - dd[2] * dd[4]) * id;
+    inv[1][0] = (dd[5] * dd[6] - dd[3] * dd[8]) * id;
+    inv[1][1] = (dd[0] * dd[8] - dd[2] * dd[6]) * id;
+    inv[1][2] = (dd[2] * dd[3] - dd[0] * dd[5]) * id;
+    inv[2][0] = (dd[3] * dd[7] - dd[4] * dd[6]) * id;
+    inv[2][1] = (dd[1] * dd[6] - dd[0] * dd[7]) * id;
+    inv[2][2] = (dd[0] * dd[4] - dd[1] * dd[3]) * id;
+  } else {
+    detJ = Jm[0][0] * Jm[1][1] - Jm[1][0] * Jm[0][1];
+    const float id = __builtin_amdgcn_rcpf(detJ);
+    inv[0][0] = Jm[1][1] * id;
+    inv[0][1] = -Jm[0][1] * id;
+    inv[1][0] = -Jm[1][0] * id;
+    inv[1][1] = Jm[0][0] * id;
+  }
+}
+
 // Per-lane rows of R float4 in LDS, exchanged inside a wave (the NN lanes of
 // an element are in one wave; one wave's LDS accesses execute in order, and
 // wave_barrier keeps the compiler from moving them across the exchange).
@@ -267,8 +315,11 @@ __device__ __forceinline__ bool radial_return_fp64(int law, const double *lp, in
 
 // RR64: the radial return in FP64 (DNLG_PREC_MIXED); otherwise in FP32
 // (DNLG_PREC_MIXED_RR32).
-template <int NN, int DIM, int BLOCK, bool AXI, bool RR64>
-__global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const Task *tasks, int ctl, int derive) {
+// WAVES > 0: that many waves per SIMD requested from the compiler (registers
+// capped, spills allowed); 0: none requested.
+template <int NN, int DIM, int BLOCK, bool AXI, bool RR64, int WAVES>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_elem_mixed(
+    JobDev *const *jobs, const Task *tasks, int ctl, int derive) {
   constexpr int NB = DIM;               // lanes per element = 2^DIM = NN
   constexpr float inv2D = 1.f / NN;     // 2^-D
   const float g = 0.577350269189625764f;  // Gauss abscissa 1/sqrt(3)
@@ -316,13 +367,13 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
   // the idle path would make the hardware wait for these loads right here.
   const int64_t G = C->G;
   const int64_t gi = e * NN + ref;
-  float sig[6], sy0, gam0;
+  float sig0[6], sy0, gam0;  // stress, yield stress, gamma of the last cycle
   double Tcur, ep0;
   int slot;  // incidence slot of this lane's node, where its force goes
   if (valid) {
     slot = gbl(C->e2slot)[gi];
 #pragma unroll
-    for (int c = 0; c < 6; c++) sig[c] = gbl(C->sig32)[c * G + gi];
+    for (int c = 0; c < 6; c++) sig0[c] = gbl(C->sig32)[c * G + gi];
     sy0 = gbl(C->sy32)[gi];
     gam0 = gbl(C->gam32)[gi];
     Tcur = gbl(C->T)[gi];
@@ -380,46 +431,20 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
 #pragma unroll
   for (int d = 0; d < DIM; d++) xi[d] = g * sgn(b, d);
 
-  // ---- Jacobian J_ij = dx_i/dxi_j and G_ij = du_i/dxi_j
-  float Jm[DIM][DIM], Gm[DIM][DIM];
+  // ---- Jacobian, its inverse, and G_ij = du_i/dxi_j
+  float detJ, inv[DIM][DIM];
+  jacobian_inverse<DIM>(mx, xi, detJ, inv);
+  float Gm[DIM][DIM];
 #pragma unroll
   for (int i = 0; i < DIM; i++) {
     if constexpr (DIM == 3) {
-      Jm[i][0] = mx[1][i] + mx[3][i] * xi[1] + mx[5][i] * xi[2] + mx[7][i] * (xi[1] * xi[2]);
-      Jm[i][1] = mx[2][i] + mx[3][i] * xi[0] + mx[6][i] * xi[2] + mx[7][i] * (xi[0] * xi[2]);
-      Jm[i][2] = mx[4][i] + mx[5][i] * xi[0] + mx[6][i] * xi[1] + mx[7][i] * (xi[0] * xi[1]);
       Gm[i][0] = mu[1][i] + mu[3][i] * xi[1] + mu[5][i] * xi[2] + mu[7][i] * (xi[1] * xi[2]);
       Gm[i][1] = mu[2][i] + mu[3][i] * xi[0] + mu[6][i] * xi[2] + mu[7][i] * (xi[0] * xi[2]);
       Gm[i][2] = mu[4][i] + mu[5][i] * xi[0] + mu[6][i] * xi[1] + mu[7][i] * (xi[0] * xi[1]);
     } else {
-      Jm[i][0] = mx[1][i] + mx[3][i] * xi[1];
-      Jm[i][1] = mx[2][i] + mx[3][i] * xi[0];
       Gm[i][0] = mu[1][i] + mu[3][i] * xi[1];
       Gm[i][1] = mu[2][i] + mu[3][i] * xi[0];
     }
-  }
-  float detJ, inv[DIM][DIM];
-  if constexpr (DIM == 3) {
-    const float *dd = &Jm[0][0];
-    detJ = dd[1] * dd[5] * dd[6] - dd[2] * dd[4] * dd[6] + dd[2] * dd[3] * dd[7] - dd[0] * dd[5] * dd[7] -
-           dd[1] * dd[3] * dd[8] + dd[0] * dd[4] * dd[8];
-    const float id = __builtin_amdgcn_rcpf(detJ);
-    inv[0][0] = (dd[4] * dd[8] - dd[5] * dd[7]) * id;
-    inv[0][1] = (dd[2] * dd[7] - dd[1] * dd[8]) * id;
-    inv[0][2] = (dd[1] * dd[5] - dd[2] * dd[4]) * id;
-    inv[1][0] = (dd[5] * dd[6] - dd[3] * dd[8]) * id;
-    inv[1][1] = (dd[0] * dd[8] - dd[2] * dd[6]) * id;
-    inv[1][2] = (dd[2] * dd[3] - dd[0] * dd[5]) * id;
-    inv[2][0] = (dd[3] * dd[7] - dd[4] * dd[6]) * id;
-    inv[2][1] = (dd[1] * dd[6] - dd[0] * dd[7]) * id;
-    inv[2][2] = (dd[0] * dd[4] - dd[1] * dd[3]) * id;
-  } else {
-    detJ = Jm[0][0] * Jm[1][1] - Jm[1][0] * Jm[0][1];
-    const float id = __builtin_amdgcn_rcpf(detJ);
-    inv[0][0] = Jm[1][1] * id;
-    inv[0][1] = -Jm[0][1] * id;
-    inv[1][0] = -Jm[1][0] * id;
-    inv[1][1] = Jm[0][0] * id;
   }
   if (valid && detJ < 0.f) set_status(J, DNLG_E_NEG_JACOBIAN, par);
   // H = du/dx = G J^-1 (Element3D.C:172-195, ElementAxi.C:78-100)
@@ -456,19 +481,18 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
 
   float T9[DIM][DIM];
   float hoop = 0.f;
+  float sig[6];
   if (valid) {
-    const float pressure = (sig[0] + sig[3] + sig[5]) * (1.f / 3.f) + (float)C->K * inc;
+    const float pressure = (sig0[0] + sig0[3] + sig0[5]) * (1.f / 3.f) + (float)C->K * inc;
 
     const double dt = cc.dt;
     const float TwoGf = (float)C->twoG;
     const double TwoG32 = kSqrt32 * C->twoG;
     const double heatFrac = C->heatFrac;
-    float old[6], s[6];
+    float s[6];
+    const float pm = (sig0[0] + sig0[3] + sig0[5]) * (1.f / 3.f);
 #pragma unroll
-    for (int c = 0; c < 6; c++) old[c] = sig[c];
-    const float pm = (old[0] + old[3] + old[5]) * (1.f / 3.f);
-#pragma unroll
-    for (int c = 0; c < 6; c++) s[c] = old[c];
+    for (int c = 0; c < 6; c++) s[c] = sig0[c];
     s[0] -= pm;
     s[3] -= pm;
     s[5] -= pm;
@@ -483,10 +507,32 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
     s[5] += TwoGf * (dE[5] - em);
     const float Snorm =
         __builtin_amdgcn_sqrtf(s[0] * s[0] + s[3] * s[3] + s[5] * s[5] + 2.f * (s[1] * s[1] + s[2] * s[2] + s[4] * s[4]));
-    // trial test and radial return in FP32 (the yield stress is stored in FP32)
+    // Work of the increment, 1/2 dE : (old + new) with new = scale s + p I
+    // (the deviator scaled by the return, plus the pressure), from three
+    // sums formed now; and the final rotation (Element.C:1146-1161) of the
+    // trial deviator: the new stress R (scale s + p I) R^T is
+    // scale R s R^T + p I for the orthogonal R.  dE, the old stress and R
+    // are then not kept through the radial return (its FP64 law evaluation
+    // is the register peak of the kernel).
+    auto wdot = [](const float x[6], const float y[6]) {
+      return x[0] * y[0] + x[3] * y[3] + x[5] * y[5] + 2.f * (x[1] * y[1] + x[2] * y[2] + x[4] * y[4]);
+    };
+    const float work_old = wdot(dE, sig0), work_s = wdot(dE, s), trdE = dE[0] + dE[3] + dE[5];
+    float Rs[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++) Rs[c] = s[c];
+    rotate_sym_f32(R, Rs);
+    // Element::computeDensity (Element.C:1243-1256)
+    if (cc.density_due) {
+      double rr = C->rho0 * (double)gbl(C->detJ0f)[gi] / (double)detJ;
+      if constexpr (AXI) rr /= (double)radius;
+      gbl(C->rho)[gi] = rr;
+    }
+    // trial test and radial return (the yield stress is stored in FP32)
     const float Strialf = (float)kSqrt32 * Snorm;
     float yieldf = sy0;
     double gamma = 0.0;
+    float scale = 1.f;
     const int law = C->law;
     double lp[8];
 #pragma unroll
@@ -497,8 +543,8 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
     }
     if (Strialf > yieldf) {
       // radial return (Element.C:485-601): Newton on gamma in FP32, then
-      // (RR64) polished by the FP64 Newton; ep, T and the energies
-      // accumulate in FP64
+      // (RR64) the FP64 Newton resumed at its last iteration; ep, T and the
+      // energies accumulate in FP64
       float pf[8];
 #pragma unroll
       for (int k = 0; k < 8; k++) pf[k] = C->lpf[k];
@@ -517,43 +563,49 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
         yieldf = (float)y;
       }
       if (!ok) set_status(J, DNLG_E_NR_DIVERGED, par);
-      const float scale = 1.f - TwoGf * g * __builtin_amdgcn_rcpf(Snorm);
-#pragma unroll
-      for (int c = 0; c < 6; c++) s[c] *= scale;
+      scale = 1.f - TwoGf * g * __builtin_amdgcn_rcpf(Snorm);
       gbl(C->ep)[gi] = ep0 + kSqrt23 * gamma;
       gbl(C->dep32)[gi] = (float)(kSqrt23 * gamma / dt);
       gbl(C->gam32)[gi] = g;
       gbl(C->sy32)[gi] = yieldf;
     }
-#pragma unroll
-    for (int c = 0; c < 6; c++) sig[c] = s[c];
-    sig[0] += pressure;
-    sig[3] += pressure;
-    sig[5] += pressure;
-    const float power = 0.5f * (dE[0] * (old[0] + sig[0]) + dE[3] * (old[3] + sig[3]) + dE[5] * (old[5] + sig[5]) +
-                                2.f * (dE[1] * (old[1] + sig[1]) + dE[2] * (old[2] + sig[2]) +
-                                       dE[4] * (old[4] + sig[4])));
+    const float power = 0.5f * (work_old + scale * work_s + pressure * trdE);
     gbl(C->eint)[gi] += (double)power * C->inv_rho0;
     if (gamma != 0.0) {
-      const float sn =
-          __builtin_amdgcn_sqrtf(s[0] * s[0] + s[3] * s[3] + s[5] * s[5] + 2.f * (s[1] * s[1] + s[2] * s[2] + s[4] * s[4]));
+      const float sn = scale * Snorm;  // |scale s|
       const double w = 0.5 * gamma * ((double)sn + (double)Snorm0);
       gbl(C->einel)[gi] += w * C->inv_rho0;
       gbl(C->T)[gi] = Tcur + heatFrac * w;
     }
-    // Element::computeFinalRotation (Element.C:1146-1161)
-    rotate_sym_f32(R, sig);
+#pragma unroll
+    for (int c = 0; c < 6; c++) sig[c] = scale * Rs[c];
+    sig[0] += pressure;
+    sig[3] += pressure;
+    sig[5] += pressure;
 #pragma unroll
     for (int c = 0; c < 6; c++) gbl(C->sig32)[c * G + gi] = sig[c];
-    // Element::computeDensity (Element.C:1243-1256)
-    if (cc.density_due) {
-      double rr = C->rho0 * (double)gbl(C->detJ0f)[gi] / (double)detJ;
-      if constexpr (AXI) rr /= (double)radius;
-      gbl(C->rho)[gi] = rr;
-    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 6; c++) sig[c] = 0.f;
   }
-  // ---- internal forces (Element.C:362-404): T = w J^-1 sigma per Gauss point
-  float w = detJ;
+  // ---- internal forces (Element.C:362-404): T = w J^-1 sigma per Gauss
+  // point, with J^-1 formed again from the coordinate modes in LDS (it is
+  // not kept through the radial return); the empty asm keeps the compiler
+  // from reusing the first reads of the modes
+  asm volatile("" ::: "memory");
+  float detJf, invf[DIM][DIM];
+  {
+    float mxr[NN][DIM];
+#pragma unroll
+    for (int q = 0; q < NN; q++) {
+      const float4 t = s_mode[ModeRows::at(le, q, 0)];
+      mxr[q][0] = t.x;
+      mxr[q][1] = t.y;
+      if constexpr (DIM == 3) mxr[q][2] = t.z;
+    }
+    jacobian_inverse<DIM>(mxr, xi, detJf, invf);
+  }
+  float w = detJf;
   if constexpr (AXI) w *= (float)k2PI * radius;
 #pragma unroll
   for (int k = 0; k < DIM; k++)
@@ -563,7 +615,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_mixed(JobDev *const *jobs, const
 #pragma unroll
       for (int j = 0; j < DIM; j++) {
         const int si = (i == j) ? (i == 0 ? 0 : (i == 1 ? 3 : 5)) : (i + j == 1 ? 1 : (i + j == 2 ? 2 : 4));
-        t += inv[k][j] * sig[si];
+        t += invf[k][j] * sig[si];
       }
       T9[k][i] = valid ? w * t : 0.f;
     }

@@ -5,25 +5,27 @@
 
 namespace dnlg {
 
+// Shape tables of every unit that holds a copy (kernels.hip, elem_fp64.hip).
 void upload_tables(const double *hexN, const double *hexD, const double *quaN, const double *quaD);
-int elem_block_threads(int type);
-int elem_block_threads_mixed(int type);
+void upload_tables_fp64(const double *hexN, const double *hexD, const double *quaN, const double *quaD);
 
 // cs: the control slot that holds the cycle's record (derive != 0: the
 // previous cycle's, from which the element kernel derives the new record
 // and stores it in slot 1 - cs).  With ev != nullptr the launch records the
 // kernel's start/stop in ev[0..1] (hipExtLaunchKernel), so its duration is
-// measured on the launch stream.
+// measured on the launch stream.  block, waves: the schedule (sched.h).
 // ann: the variant for jobs with a network law (DNLG_LAW_ANN2SIG).
-void launch_elem_fp64(int type, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int derive,
-                      hipStream_t s, const hipEvent_t *ev = nullptr, bool ann = false);
+void launch_elem_fp64(int type, int block, int waves, JobDev *const *jobs, const Task *tasks, int nblk, int cs,
+                      int derive, hipStream_t s, const hipEvent_t *ev = nullptr, bool ann = false);
 // rr64: the radial return in FP64 (DNLG_PREC_MIXED), else in FP32
 // (DNLG_PREC_MIXED_RR32)
-void launch_elem_mixed(int type, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int derive,
-                       hipStream_t s, const hipEvent_t *ev, bool rr64);
+void launch_elem_mixed(int type, int block, int waves, JobDev *const *jobs, const Task *tasks, int nblk, int cs,
+                       int derive, hipStream_t s, const hipEvent_t *ev, bool rr64);
 // fold != 0: BC nodes also get the next cycle's field0 BC values (the next
 // cycle's element launch then derives its record, derive != 0)
-void launch_correct(int dim, bool f32, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int fold,
+// batch: the incidence slots loaded together (2, 4 or 8 for FP32 forces; 2
+// or 4 for FP64)
+void launch_correct(int dim, bool f32, int batch, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int fold,
                     hipStream_t s);
 void launch_courant(int dim, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int force, hipStream_t s);
 // close the cycle in slot cs (end of step, Courant time step) and, with

@@ -23,506 +23,14 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include "control.h"
 #include "kernels.h"
 #include "material.h"
 #include "polar_fp64.h"
-#include "control.h"
+#include "shape.h"
 
 namespace dnlg {
 
-// Element shape tables at the Gauss points: the closed forms tabulated in
-// ElQua4NAx.C:63-110 and ElHex8N3D.C:93-231, generated on the host
-// (api.hip, make_tables) and uploaded once.
-__constant__ double c_hexN[8][8];
-__constant__ double c_hexD[8][8][3];
-__constant__ double c_quaN[4][4];
-__constant__ double c_quaD[4][4][2];
-__constant__ float c_hexNf[8][8];
-__constant__ float c_hexDf[8][8][3];
-__constant__ float c_quaNf[4][4];
-__constant__ float c_quaDf[4][4][2];
-
-template <int NN>
-__device__ __forceinline__ double tabN(int p, int a) {
-  if constexpr (NN == 8) return c_hexN[p][a];
-  else return c_quaN[p][a];
-}
-template <int NN>
-__device__ __forceinline__ double tabD(int p, int a, int j) {
-  if constexpr (NN == 8) return c_hexD[p][a][j];
-  else return c_quaD[p][a][j];
-}
-
-__device__ __forceinline__ int sym_idx(int i, int j) { return (i == j) ? (i == 0 ? 0 : (i == 1 ? 3 : 5)) : (i + j == 1 ? 1 : (i + j == 2 ? 2 : 4)); }
-
-// Sticky job status of the cycle with increment parity `par`.
-__device__ __forceinline__ void set_status(JobDev *J, int st, int par) { atomicCAS(&J->sync.st[par], 0, st); }
-
-// ---------------------------------------------------------------------------
-// Cycle control.  Between two cycles a job's control record goes through
-//   close  Explicit::endStep (Explicit.C:539-552): t += dt, field1 becomes
-//          field0; then Solver::computeTimeStep (Solver.C:196-236, Courant
-//          branch: omega_max = 2 / dt_crit) when the time step was due;
-//   open   Explicit::updateTimes (Explicit.C:414-427): last-step clip,
-//          increment, and the engine's per-cycle flags.
-// k_cycle_step does it where a launch sequence starts, after time-step
-// cycles, and (close only) where a sequence ends; between the other cycles of
-// a sequence the next cycle's element kernel derives the new record itself
-// (cycle_control) and the corrector has already written the new cycle's
-// field0 BC values.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void store_control(JobDev *J, int slot, const Control &c) {
-  DNLG_GLOBAL Control &d = *gbl(&J->c[slot]);
-  d.t = c.t;
-  d.dt = c.dt;
-  d.up_to = c.up_to;
-  d.inc = c.inc;
-  d.f0 = c.f0;
-  d.running = c.running;
-  d.courant_due = c.courant_due;
-  d.density_due = c.density_due;
-  d.preloop = c.preloop;
-  d.pending = c.pending;
-}
-
-// Close the cycle that ran last (if any) and, with `open`, open the next.
-// status: the job's status after that cycle; freq > 0: the maximum
-// frequency of the time-step update (Courant: 2 / min L/c; power iteration:
-// sqrt(fmax)), dt = safety Omega_s / freq.  Returns whether the field0 BC
-// values of the new cycle differ from what the corrector left in field0
-// (first cycle of an advance, a ramped speed BC, or a speed BC under a
-// changed time step).
-__device__ __forceinline__ bool close_open(CJobDev *C, Control &c, int status, double freq, bool open) {
-  const double dt_last = c.dt;
-  const bool resumed = !c.pending;
-  if (c.pending && status == 0) {
-    c.t += c.dt;
-    c.f0 = 1 - c.f0;
-    if (freq > 0.0) c.dt = C->safety * C->omegaS / freq;
-  }
-  c.pending = 0;
-  bool bc_work = false;
-  if (open) {
-    c.preloop = 0;
-    // a job whose power iteration stalled waits for the host-driven
-    // iteration (k_pi_finish clears the flag), a held one for its result
-    // file
-    c.running = (status == 0 && c.t < c.up_to && !C->pi.stall && !C->res.hold) ? 1 : 0;
-    if (c.running) {
-      if (c.t + c.dt > c.up_to) c.dt = c.up_to - c.t;
-      c.inc++;
-      c.courant_due = (c.inc % C->dt_freq == 0) ? 1 : 0;
-      // densities are consumed by the Courant step, read back at the end
-      // and, when a history item reads them, after every cycle
-      c.density_due = (C->density_always || c.courant_due || !(c.t + c.dt < c.up_to)) ? 1 : 0;
-      c.pending = 1;
-      bc_work = resumed || (C->bc_flags & 2) || ((C->bc_flags & 1) && c.dt != dt_last);
-    }
-  } else {
-    c.running = 0;
-  }
-  return bc_work;
-}
-
-// Whether the Courant kernel computed partial minima in the cycle that ran
-// last (Explicit.C:243-251: every dt_freq increments, while t_{n+1} < t_up_to).
-__device__ __forceinline__ bool courant_ran(const Control &c, int status) {
-  return c.pending && status == 0 && c.courant_due && c.t + c.dt < c.up_to;
-}
-
-// The record of the cycle that a launch works on.  derive == 0: slot cs
-// holds it.  derive != 0 (the first launch of a cycle that no k_cycle_step
-// opened, after a cycle without a time-step update): slot cs holds the
-// previous cycle; every block derives the new record from it and the job's
-// block 0 stores it in slot 1 - cs for the cycle's later launches.  The
-// previous cycle's failures are in its own parity word, which no launch of
-// the new cycle writes; a job that did not run in it stays stopped (it
-// finished or failed), so reading both words is then race-free too.
-// The record is uniform over the block; it is returned in scalar registers
-// (the FP64 arithmetic of the derivation runs on the vector unit).
-__device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ __forceinline__ int64_t uniform_l(int64_t v) {
-  const unsigned lo = (unsigned)uniform_i((int)(v & 0xffffffffLL));
-  const unsigned hi = (unsigned)uniform_i((int)(v >> 32));
-  return (int64_t)(((uint64_t)hi << 32) | lo);
-}
-__device__ __forceinline__ double uniform_d(double v) { return __longlong_as_double(uniform_l(__double_as_longlong(v))); }
-
-__device__ __forceinline__ Control cycle_control(JobDev *J, CJobDev *C, int cs, int derive, int blk) {
-  Control c = load_control(C, cs);
-  if (derive) {
-    const int st0 = C->sync.st[0], st1 = C->sync.st[1];
-    const int status = c.pending ? ((c.inc & 1) ? st1 : st0) : (st0 != 0 ? st0 : st1);
-    close_open(C, c, status, 0.0, true);
-    if (blk == 0 && threadIdx.x == 0) store_control(J, 1 - cs, c);
-    c.t = uniform_d(c.t);
-    c.dt = uniform_d(c.dt);
-    c.inc = uniform_l(c.inc);
-    c.f0 = uniform_i(c.f0);
-    c.running = uniform_i(c.running);
-    c.courant_due = uniform_i(c.courant_due);
-    c.density_due = uniform_i(c.density_due);
-    c.preloop = uniform_i(c.preloop);
-    c.pending = uniform_i(c.pending);
-  }
-  return c;
-}
-
-
-// The hardening law's yield stress (and slope) at (epsp, depsp, T) for the
-// FP64 radial returns: a tabular law through its table (out of it: `ok`
-// cleared), the closed-form laws through material.h; ANN: the kernel
-// variant for network laws.
-template <bool ANN>
-__device__ __forceinline__ double law_yield_of(CJobDev *C, int law, const double *lp, double epsp, double depsp,
-                                               double T, double dt, bool &ok) {
-  if (ANN) {
-    double y, h;
-    law_ann(C->ltab, epsp, depsp, T, dt, 0.0, false, y, h);
-    return y;
-  }
-  if (law == DNLG_LAW_TABULAR) {
-    double y = 0.0, h;
-    ok = law_tabular(C->ltab, C->n_ltab, epsp, y, h) && ok;
-    return y;
-  }
-  return law_yield(law, lp, epsp, depsp, T);
-}
-template <bool ANN>
-__device__ __forceinline__ void law_eval(CJobDev *C, int law, const double *lp, double eta_rcp, double epsp,
-                                         double depsp, double T, double dt, double &yield, double &hard, bool &ok) {
-  if (ANN) {
-    law_ann(C->ltab, epsp, depsp, T, dt, eta_rcp, true, yield, hard);
-    return;
-  }
-  if (law == DNLG_LAW_TABULAR) {
-    yield = hard = 0.0;
-    ok = law_tabular(C->ltab, C->n_ltab, epsp, yield, hard) && ok;
-    return;
-  }
-  law_yield_and_slope(law, lp, eta_rcp, epsp, depsp, T, dt, yield, hard);
-}
-
-// ---------------------------------------------------------------------------
-// Element kernel, FP64.  One lane per integration point (NIP lanes per
-// element, NIP == NN for both element types); the element's nodes are staged
-// in LDS by its own lanes, per-IP inverse Jacobians, stresses and weights go back to
-// LDS so that lane a can sum node a's force over the IPs in the reference's
-// order (IP, then direction j).
-// ---------------------------------------------------------------------------
-// Three waves per SIMD (168 VGPRs, a short spill); the LDS of a 16-hex block
-// (24 KB) allows three.  Measured 11 % faster than two waves without spill.
-// ANN: the variant for jobs whose law is DNLG_LAW_ANN2SIG (its network
-// evaluation keeps per-lane arrays that the other laws' variants do without).
-template <int NN, int DIM, int BLOCK, bool AXI, bool ANN>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void k_elem_fp64(
-    JobDev *const *jobs, const Task *tasks, int cs, int derive) {
-  constexpr int NIP = NN;
-  constexpr int EPB = BLOCK / NIP;
-  __shared__ double s_x[EPB][NN][DIM];
-  __shared__ double s_u[EPB][NN][DIM];
-  __shared__ double s_inv[EPB][NIP][DIM][DIM];  // J^-1 per IP, for the force phase
-  __shared__ double s_sw[EPB][NIP][8];
-  __shared__ double s_tr[EPB][NIP];
-
-  const Task tk = load_task(tasks);
-  JobDev *J = tk.jd;
-  CJobDev *C = (CJobDev *)J;
-  const Control cc = cycle_control(J, C, cs, derive, tk.blk);
-  if (!cc.running) return;
-  const int par = (int)(cc.inc & 1);
-  const int le = threadIdx.x / NIP;
-  const int p = threadIdx.x % NIP;
-  const int64_t e = tk.first + le;
-  const bool valid = e < tk.end;
-  const int64_t N = C->N;
-  const int f0 = cc.f0;
-
-  // stage the element's nodes: lane p brings node p
-  if (valid) {
-    const int nd = gbl(C->conn)[e * NN + p];
-    const DNLG_GLOBAL double *u0 = gbl(C->fu[f0]);
-#pragma unroll
-    for (int d = 0; d < DIM; d++) {
-      s_x[le][p][d] = gbl(C->x)[d * N + nd];
-      s_u[le][p][d] = u0[d * N + nd];
-    }
-  }
-  __syncthreads();
-
-  const int64_t G = C->G;
-  const int64_t g = e * NIP + p;
-  double dN[NN][DIM];
-  double detJ = 1.0, radius = 1.0;
-  double dE[6] = {0, 0, 0, 0, 0, 0}, R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-  if (valid) {
-    // Jacobian, inverse, B-matrix (Element3D.C:64-115, ElementPlane.C:46-102)
-    double Jm[DIM][DIM];
-#pragma unroll
-    for (int i = 0; i < DIM; i++)
-#pragma unroll
-      for (int j = 0; j < DIM; j++) Jm[i][j] = 0.0;
-#pragma unroll
-    for (int a = 0; a < NN; a++)
-#pragma unroll
-      for (int i = 0; i < DIM; i++)
-#pragma unroll
-        for (int j = 0; j < DIM; j++) Jm[i][j] += tabD<NN>(p, a, j) * s_x[le][a][i];
-    double inv[DIM][DIM];
-    if constexpr (DIM == 3) {
-      const double *dd = &Jm[0][0];
-      detJ = dd[1] * dd[5] * dd[6] - dd[2] * dd[4] * dd[6] + dd[2] * dd[3] * dd[7] - dd[0] * dd[5] * dd[7] -
-             dd[1] * dd[3] * dd[8] + dd[0] * dd[4] * dd[8];
-      const double id = 1.0 / detJ;
-      inv[0][0] = (dd[4] * dd[8] - dd[5] * dd[7]) * id;
-      inv[0][1] = (dd[2] * dd[7] - dd[1] * dd[8]) * id;
-      inv[0][2] = (dd[1] * dd[5] - dd[2] * dd[4]) * id;
-      inv[1][0] = (dd[5] * dd[6] - dd[3] * dd[8]) * id;
-      inv[1][1] = (dd[0] * dd[8] - dd[2] * dd[6]) * id;
-      inv[1][2] = (dd[2] * dd[3] - dd[0] * dd[5]) * id;
-      inv[2][0] = (dd[3] * dd[7] - dd[4] * dd[6]) * id;
-      inv[2][1] = (dd[1] * dd[6] - dd[0] * dd[7]) * id;
-      inv[2][2] = (dd[0] * dd[4] - dd[1] * dd[3]) * id;
-    } else {
-      detJ = Jm[0][0] * Jm[1][1] - Jm[1][0] * Jm[0][1];
-      const double id = 1.0 / detJ;
-      inv[0][0] = Jm[1][1] * id;
-      inv[0][1] = -Jm[0][1] * id;
-      inv[1][0] = -Jm[1][0] * id;
-      inv[1][1] = Jm[0][0] * id;
-    }
-    if (detJ < 0.0) set_status(J, DNLG_E_NEG_JACOBIAN, par);
-#pragma unroll
-    for (int a = 0; a < NN; a++)
-#pragma unroll
-      for (int j = 0; j < DIM; j++) {
-        double s = 0.0;
-#pragma unroll
-        for (int k = 0; k < DIM; k++) s += tabD<NN>(p, a, k) * inv[k][j];
-        dN[a][j] = s;
-      }
-#pragma unroll
-    for (int k = 0; k < DIM; k++)
-#pragma unroll
-      for (int j = 0; j < DIM; j++) s_inv[le][p][k][j] = inv[k][j];
-    // deformation gradient from the field0 increment (Element3D.C:172-195,
-    // ElementAxi.C:78-100) and polar / log strain (Tensor2.C:1442-1457)
-    double F[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-#pragma unroll
-    for (int a = 0; a < NN; a++)
-#pragma unroll
-      for (int i = 0; i < DIM; i++)
-#pragma unroll
-        for (int j = 0; j < DIM; j++) F[3 * i + j] += dN[a][j] * s_u[le][a][i];
-    if constexpr (AXI) {
-      double r = 0.0, vr = 0.0;
-#pragma unroll
-      for (int a = 0; a < NN; a++) {
-        r += tabN<NN>(p, a) * s_x[le][a][0];
-        vr += tabN<NN>(p, a) * s_u[le][a][0];
-      }
-      radius = r;
-      F[8] += vr / radius;
-    }
-    polar_lnU_fp64(F, dE, R);
-    s_tr[le][p] = dE[0] + dE[3] + dE[5];
-  }
-  __syncthreads();
-
-  double sig[6], wdj = 0.0;
-  if (valid) {
-    // Element::computePressure (Element.C:1125-1143)
-    double inc = 0.0;
-#pragma unroll
-    for (int q = 0; q < NIP; q++) inc += s_tr[le][q];
-    inc /= NIP;
-#pragma unroll
-    for (int c = 0; c < 6; c++) sig[c] = gbl(C->sig)[c * G + g];
-    const double pressure = (sig[0] + sig[3] + sig[5]) / 3.0 + C->K * inc;
-
-    // Element::computeStress (Element.C:407-622)
-    const double dt = cc.dt;
-    const double TwoG = C->twoG;
-    const double TwoG32 = kSqrt32 * TwoG;
-    const double heatFrac = C->heatFrac;
-    double old[6], s[6];
-#pragma unroll
-    for (int c = 0; c < 6; c++) old[c] = sig[c];
-    const double pm = (old[0] + old[3] + old[5]) / 3.0;
-#pragma unroll
-    for (int c = 0; c < 6; c++) s[c] = old[c];
-    s[0] -= pm;
-    s[3] -= pm;
-    s[5] -= pm;
-    const double Snorm0 = sqrt(sqd(s[0]) + sqd(s[3]) + sqd(s[5]) + 2.0 * (sqd(s[1]) + sqd(s[2]) + sqd(s[4])));
-    const double em = (dE[0] + dE[3] + dE[5]) / 3.0;
-    double de[6];
-#pragma unroll
-    for (int c = 0; c < 6; c++) de[c] = dE[c];
-    de[0] -= em;
-    de[3] -= em;
-    de[5] -= em;
-#pragma unroll
-    for (int c = 0; c < 6; c++) s[c] += TwoG * de[c];
-    const double Snorm = sqrt(sqd(s[0]) + sqd(s[3]) + sqd(s[5]) + 2.0 * (sqd(s[1]) + sqd(s[2]) + sqd(s[4])));
-    const double Strial = kSqrt32 * Snorm;
-    const double Tcur = gbl(C->T)[g];
-    const double ep0 = gbl(C->ep)[g];
-    double yield = gbl(C->sy)[g];
-    double gamma = 0.0;
-    const int law = C->law;
-    double lp[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) lp[k] = C->lp[k];
-    bool in_table = true;
-    if (yield == 0.0) {
-      yield = law_yield_of<ANN>(C, law, lp, kGammaInitial, kGammaInitial / dt, Tcur, dt, in_table);
-      gbl(C->sy)[g] = yield;
-    }
-    const bool plastic = Strial > yield;
-    if (plastic && C->stress_method == DNLG_STRESS_DIRECT) {
-      // Element::computeStressDirect (Element.C:625-768): one linearised
-      // return from the stored yield stress and the slope at the stored
-      // state (plastic strain, or gammaInitial before the first yield)
-      double y0, hard;
-      law_eval<ANN>(C, law, lp, heatFrac, ep0 > 0.0 ? ep0 : kGammaInitial, gbl(C->dep)[g], Tcur, dt, y0, hard,
-                    in_table);
-      gamma = (Snorm - kSqrt23 * yield) / (TwoG * (1.0 + hard / (1.5 * TwoG)));
-      const double T = Tcur + 0.5 * gamma * heatFrac * (kSqrt23 * yield + Snorm0);
-      yield = law_yield_of<ANN>(C, law, lp, ep0 + kSqrt23 * gamma, kSqrt23 * gamma / dt, T, dt, in_table);
-    } else if (plastic) {
-      double gmin = 0.0, gmax = (Strial - yield) / TwoG32;
-      gamma = gbl(C->gam)[g];
-      if (ep0 == 0.0) gamma = kSqrt32 * kGammaInitial;
-      double depsp = kSqrt23 * gamma / dt;
-      double epsp = ep0 + kSqrt23 * gamma;
-      double T = Tcur + 0.5 * gamma * heatFrac * (kSqrt23 * yield + Snorm0);
-      const double eta_rcp = heatFrac;
-      int iterate = 0;
-      while (true) {
-        double hard;
-        law_eval<ANN>(C, law, lp, eta_rcp, epsp, depsp, T, dt, yield, hard, in_table);
-        if (!in_table) break;
-        const double fun = Strial - gamma * TwoG32 - yield;
-        if (fun < 0.0)
-          gmax = gamma;
-        else
-          gmin = gamma;
-        const double dfun = TwoG32 + kSqrt23 * hard;
-        double dgamma = fun / dfun;
-        gamma += dgamma;
-        if ((gmax - gamma) * (gamma - gmin) < 0.0) {
-          dgamma = 0.5 * (gmax - gmin);
-          gamma = gmin + dgamma;
-        }
-        if (fabs(dgamma) < kTolNR) break;
-        depsp = kSqrt23 * gamma / dt;
-        epsp = ep0 + kSqrt23 * gamma;
-        T = Tcur + 0.5 * gamma * heatFrac * (kSqrt23 * yield + Snorm0);
-        if (++iterate > C->nr_max_it) {
-          set_status(J, DNLG_E_NR_DIVERGED, par);
-          break;
-        }
-      }
-    }
-    if (!in_table) set_status(J, DNLG_E_TABLE_RANGE, par);
-    if (plastic) {
-      if (C->track) {
-#pragma unroll
-        for (int c = 0; c < 6; c++) gbl(C->epsP)[c * G + g] += gamma * s[c] / Snorm;
-      }
-      const double scale = 1.0 - TwoG * gamma / Snorm;
-#pragma unroll
-      for (int c = 0; c < 6; c++) s[c] *= scale;
-      gbl(C->ep)[g] = ep0 + kSqrt23 * gamma;
-      gbl(C->dep)[g] = kSqrt23 * gamma / dt;
-      gbl(C->gam)[g] = gamma;
-      gbl(C->gamc)[g] += gamma;
-      gbl(C->sy)[g] = yield;
-    }
-#pragma unroll
-    for (int c = 0; c < 6; c++) sig[c] = s[c];
-    sig[0] += pressure;
-    sig[3] += pressure;
-    sig[5] += pressure;
-    // specific internal energy, plastic work, adiabatic heating
-    const double power = 0.5 * (dE[0] * (old[0] + sig[0]) + dE[3] * (old[3] + sig[3]) + dE[5] * (old[5] + sig[5]) +
-                                2.0 * (dE[1] * (old[1] + sig[1]) + dE[2] * (old[2] + sig[2]) +
-                                       dE[4] * (old[4] + sig[4])));
-    gbl(C->eint)[g] += power / C->rho0;
-    if (gamma != 0.0) {
-      const double sn = sqrt(sqd(s[0]) + sqd(s[3]) + sqd(s[5]) + 2.0 * (sqd(s[1]) + sqd(s[2]) + sqd(s[4])));
-      const double w = 0.5 * gamma * (sn + Snorm0);
-      gbl(C->einel)[g] += w / C->rho0;
-      gbl(C->T)[g] = Tcur + heatFrac * w;
-    }
-    // Element::computeFinalRotation (Element.C:1146-1161)
-    rotate_sym(R, sig);
-#pragma unroll
-    for (int c = 0; c < 6; c++) gbl(C->sig)[c * G + g] = sig[c];
-    if (C->track) {
-      double t6[6];
-#pragma unroll
-      for (int c = 0; c < 6; c++) t6[c] = gbl(C->eps)[c * G + g] + dE[c];
-      rotate_sym(R, t6);
-#pragma unroll
-      for (int c = 0; c < 6; c++) gbl(C->eps)[c * G + g] = t6[c];
-#pragma unroll
-      for (int c = 0; c < 6; c++) t6[c] = gbl(C->epsP)[c * G + g];
-      rotate_sym(R, t6);
-#pragma unroll
-      for (int c = 0; c < 6; c++) gbl(C->epsP)[c * G + g] = t6[c];
-    }
-    // Element::computeDensity (Element.C:1243-1256), on x_n
-    if (cc.density_due) {
-      double rr = C->rho0 * gbl(C->detJ0)[g] / detJ;
-      if constexpr (AXI) rr /= radius;
-      gbl(C->rho)[g] = rr;
-    }
-    wdj = detJ;  // Gauss weights are 1 (ElQua4NAx.C:63-110, ElHex8N3D.C:93-231)
-    if constexpr (AXI) wdj *= k2PI * radius;
-#pragma unroll
-    for (int c = 0; c < 6; c++) s_sw[le][p][c] = sig[c];
-    s_sw[le][p][6] = wdj;
-    s_sw[le][p][7] = radius;
-  }
-  __syncthreads();
-
-  // Element::computeInternalForces (Element.C:362-404): lane a sums node a
-  if (valid) {
-    const int a = p;
-    double f[DIM];
-#pragma unroll
-    for (int i = 0; i < DIM; i++) f[i] = 0.0;
-#pragma unroll
-    for (int q = 0; q < NIP; q++) {
-      const double w = s_sw[le][q][6];
-      // node a's shape derivatives at IP q, formed as in the strain phase
-      double dNq[DIM];
-#pragma unroll
-      for (int j = 0; j < DIM; j++) {
-        double d = 0.0;
-#pragma unroll
-        for (int k = 0; k < DIM; k++) d += tabD<NN>(q, a, k) * s_inv[le][q][k][j];
-        dNq[j] = d;
-      }
-#pragma unroll
-      for (int i = 0; i < DIM; i++)
-#pragma unroll
-        for (int j = 0; j < DIM; j++) f[i] -= dNq[j] * s_sw[le][q][sym_idx(j, i)] * w;
-      if constexpr (AXI) f[0] -= tabN<NN>(q, a) * s_sw[le][q][5] / s_sw[le][q][7] * w;
-    }
-    DNLG_GLOBAL double *out = gbl(C->elemF) + (int64_t)gbl(C->e2slot)[e * NN + a] * DIM;
-#pragma unroll
-    for (int i = 0; i < DIM; i++) out[i] = f[i];
-  }
-}
-
-}  // namespace dnlg
-
-#include "elem_mixed.h"
-
-namespace dnlg {
 
 // ---------------------------------------------------------------------------
 // Nodal kernels
@@ -590,7 +98,9 @@ __device__ __forceinline__ void add_slots(const DNLG_GLOBAL FT *ef, int64_t k, d
     for (int d = 0; d < DIM; d++) f[d] += (double)v[i][d];
 }
 
-template <int DIM, typename FT>
+// MAXB: the incidence slots loaded together (a schedule parameter; the
+// sums run in slot order whatever the batching)
+template <int DIM, typename FT, int MAXB>
 __global__ __launch_bounds__(kNodeBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_correct(
     JobDev *const *jobs, const Task *tasks, int cs, int fold) {
   const Task tk = load_task(tasks);
@@ -619,18 +129,18 @@ __global__ __launch_bounds__(kNodeBlock) __attribute__((amdgpu_waves_per_eu(4)))
 #pragma unroll
   for (int d = 0; d < DIM; d++) f[d] = 0.0;
   int64_t k = k0;
-  if constexpr (sizeof(FT) == 4) {
-    for (; k + 8 <= k1; k += 8) add_slots<8, DIM>(ef, k, f);
+  for (; k + MAXB <= k1; k += MAXB) add_slots<MAXB, DIM>(ef, k, f);
+  if constexpr (MAXB > 4) {
     if (k + 4 <= k1) {
       add_slots<4, DIM>(ef, k, f);
       k += 4;
     }
-  } else {  // FP64 slots: batches of 4 keep the kernel within its registers
-    for (; k + 4 <= k1; k += 4) add_slots<4, DIM>(ef, k, f);
   }
-  if (k + 2 <= k1) {
-    add_slots<2, DIM>(ef, k, f);
-    k += 2;
+  if constexpr (MAXB > 2) {
+    if (k + 2 <= k1) {
+      add_slots<2, DIM>(ef, k, f);
+      k += 2;
+    }
   }
   if (k < k1) add_slots<1, DIM>(ef, k, f);
   const double alphaM = C->alphaM, gam = C->gammaCH, beta = C->beta;
@@ -1270,89 +780,40 @@ __global__ __launch_bounds__(kNodeBlock) void k_mass_gather(JobDev *const *jobs,
 // Host-side launchers
 // ---------------------------------------------------------------------------
 void upload_tables(const double *hexN, const double *hexD, const double *quaN, const double *quaD) {
-  hipMemcpyToSymbol(HIP_SYMBOL(c_hexN), hexN, sizeof(double) * 64);
-  hipMemcpyToSymbol(HIP_SYMBOL(c_hexD), hexD, sizeof(double) * 192);
-  hipMemcpyToSymbol(HIP_SYMBOL(c_quaN), quaN, sizeof(double) * 16);
-  hipMemcpyToSymbol(HIP_SYMBOL(c_quaD), quaD, sizeof(double) * 32);
-  float f[192];
-  for (int k = 0; k < 64; k++) f[k] = (float)hexN[k];
-  hipMemcpyToSymbol(HIP_SYMBOL(c_hexNf), f, sizeof(float) * 64);
-  for (int k = 0; k < 192; k++) f[k] = (float)hexD[k];
-  hipMemcpyToSymbol(HIP_SYMBOL(c_hexDf), f, sizeof(float) * 192);
-  for (int k = 0; k < 16; k++) f[k] = (float)quaN[k];
-  hipMemcpyToSymbol(HIP_SYMBOL(c_quaNf), f, sizeof(float) * 16);
-  for (int k = 0; k < 32; k++) f[k] = (float)quaD[k];
-  hipMemcpyToSymbol(HIP_SYMBOL(c_quaDf), f, sizeof(float) * 32);
+  upload_shape_tables(hexN, hexD, quaN, quaD);
+  upload_tables_fp64(hexN, hexD, quaN, quaD);
 }
 
-int elem_block_threads(int type) { return type == DNLG_EL_HEX8N_3D ? 128 : 256; }
-int elem_block_threads_mixed(int type) { return type == DNLG_EL_HEX8N_3D ? 128 : 256; }
-
-// One launch of an element kernel template: with events (hipExtLaunchKernelGGL
-// records start / stop on the stream) or without.
-template <typename K>
-void launch_elem(K kern, int block, int nblk, hipStream_t s, const hipEvent_t *ev, JobDev *const *jobs,
-                 const Task *tasks, int cs, int derive) {
-  if (ev)
-    hipExtLaunchKernelGGL(kern, dim3(nblk), dim3(block), 0, s, ev[0], ev[1], 0, jobs, tasks, cs, derive);
-  else
-    hipLaunchKernelGGL(kern, dim3(nblk), dim3(block), 0, s, jobs, tasks, cs, derive);
+template <int DIM, typename FT, int MAXB>
+void launch_correct_b(JobDev *const *jobs, const Task *tasks, int nblk, int cs, int fold, hipStream_t s) {
+  hipLaunchKernelGGL((k_correct<DIM, FT, MAXB>), dim3(nblk), dim3(kNodeBlock), 0, s, jobs, tasks, cs, fold);
 }
 
-template <bool RR64>
-void launch_elem_mixed_rr(int type, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int derive,
-                          hipStream_t s, const hipEvent_t *ev) {
-  if (type == DNLG_EL_HEX8N_3D)
-    launch_elem(k_elem_mixed<8, 3, 128, false, RR64>, 128, nblk, s, ev, jobs, tasks, cs, derive);
-  else if (type == DNLG_EL_QUA4N_AX)
-    launch_elem(k_elem_mixed<4, 2, 256, true, RR64>, 256, nblk, s, ev, jobs, tasks, cs, derive);
-  else
-    launch_elem(k_elem_mixed<4, 2, 256, false, RR64>, 256, nblk, s, ev, jobs, tasks, cs, derive);
+template <int DIM>
+void launch_correct_d(bool f32, int batch, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int fold,
+                      hipStream_t s) {
+  if (f32) {
+    if (batch == 2)
+      launch_correct_b<DIM, float, 2>(jobs, tasks, nblk, cs, fold, s);
+    else if (batch == 4)
+      launch_correct_b<DIM, float, 4>(jobs, tasks, nblk, cs, fold, s);
+    else
+      launch_correct_b<DIM, float, 8>(jobs, tasks, nblk, cs, fold, s);
+  } else {
+    if (batch == 2)
+      launch_correct_b<DIM, double, 2>(jobs, tasks, nblk, cs, fold, s);
+    else
+      launch_correct_b<DIM, double, 4>(jobs, tasks, nblk, cs, fold, s);
+  }
 }
 
-void launch_elem_mixed(int type, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int derive,
-                       hipStream_t s, const hipEvent_t *ev, bool rr64) {
-  if (nblk <= 0) return;
-  if (rr64)
-    launch_elem_mixed_rr<true>(type, jobs, tasks, nblk, cs, derive, s, ev);
-  else
-    launch_elem_mixed_rr<false>(type, jobs, tasks, nblk, cs, derive, s, ev);
-}
-
-template <bool ANN>
-void launch_elem_fp64_law(int type, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int derive,
-                          hipStream_t s, const hipEvent_t *ev) {
-  if (type == DNLG_EL_HEX8N_3D)
-    launch_elem(k_elem_fp64<8, 3, 128, false, ANN>, 128, nblk, s, ev, jobs, tasks, cs, derive);
-  else if (type == DNLG_EL_QUA4N_AX)
-    launch_elem(k_elem_fp64<4, 2, 256, true, ANN>, 256, nblk, s, ev, jobs, tasks, cs, derive);
-  else
-    launch_elem(k_elem_fp64<4, 2, 256, false, ANN>, 256, nblk, s, ev, jobs, tasks, cs, derive);
-}
-
-void launch_elem_fp64(int type, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int derive,
-                      hipStream_t s, const hipEvent_t *ev, bool ann) {
-  if (nblk <= 0) return;
-  if (ann)
-    launch_elem_fp64_law<true>(type, jobs, tasks, nblk, cs, derive, s, ev);
-  else
-    launch_elem_fp64_law<false>(type, jobs, tasks, nblk, cs, derive, s, ev);
-}
-
-void launch_correct(int dim, bool f32, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int fold,
+void launch_correct(int dim, bool f32, int batch, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int fold,
                     hipStream_t s) {
   if (nblk <= 0) return;
-  if (dim == 3) {
-    if (f32)
-      hipLaunchKernelGGL((k_correct<3, float>), dim3(nblk), dim3(kNodeBlock), 0, s, jobs, tasks, cs, fold);
-    else
-      hipLaunchKernelGGL((k_correct<3, double>), dim3(nblk), dim3(kNodeBlock), 0, s, jobs, tasks, cs, fold);
-  } else {
-    if (f32)
-      hipLaunchKernelGGL((k_correct<2, float>), dim3(nblk), dim3(kNodeBlock), 0, s, jobs, tasks, cs, fold);
-    else
-      hipLaunchKernelGGL((k_correct<2, double>), dim3(nblk), dim3(kNodeBlock), 0, s, jobs, tasks, cs, fold);
-  }
+  if (dim == 3)
+    launch_correct_d<3>(f32, batch, jobs, tasks, nblk, cs, fold, s);
+  else
+    launch_correct_d<2>(f32, batch, jobs, tasks, nblk, cs, fold, s);
 }
 
 void launch_courant(int dim, JobDev *const *jobs, const Task *tasks, int nblk, int cs, int force, hipStream_t s) {

@@ -168,23 +168,39 @@ __device__ __forceinline__ void eig_secular(const double d[3], const double z[3]
     }
     for (int it = 0; it < 500; it++) {
       double F = 1.0, dF = 0.0, err = 1.0;
-      {
-        const double rr = 1.0 / r0, zr = z[0] * rr;
-        F += zr;
-        err += fabs(zr);
-        dF += zr * rr;
+      // the three reciprocals from one division of their product (a few
+      // ulp each; the stopping test allows 8 eps of slack), or three
+      // divisions when the product leaves the normal range (a pole hit
+      // exactly, or one close enough to underflow the product)
+      double q0, q1, q2;
+      const double r01 = r0 * r1, prod = r01 * r2;
+      if (fabs(prod) > 1e-280 && fabs(prod) < 1e280) {
+        const double ip = 1.0 / prod;
+        q0 = (r1 * r2) * ip;
+        q1 = (r0 * r2) * ip;
+        q2 = r01 * ip;
+      } else {
+        q0 = 1.0 / r0;
+        q1 = 1.0 / r1;
+        q2 = 1.0 / r2;
       }
       {
-        const double rr = 1.0 / r1, zr = z[1] * rr;
+        const double zr = z[0] * q0;
         F += zr;
         err += fabs(zr);
-        dF += zr * rr;
+        dF += zr * q0;
       }
       {
-        const double rr = 1.0 / r2, zr = z[2] * rr;
+        const double zr = z[1] * q1;
         F += zr;
         err += fabs(zr);
-        dF += zr * rr;
+        dF += zr * q1;
+      }
+      {
+        const double zr = z[2] * q2;
+        F += zr;
+        err += fabs(zr);
+        dF += zr * q2;
       }
       if (fabs(F) <= kEps * (8.0 * err + fabs(x * dF))) break;
       if (F < 0.0)

@@ -1,0 +1,39 @@
+// shape.h -- Gauss-point shape tables of the element kernels, one copy per
+// translation unit that includes it (each device module holds its own
+// constant memory); upload_shape_tables fills this unit's copy.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "engine.h"
+
+namespace dnlg {
+
+// Element shape tables at the Gauss points: the closed forms tabulated in
+// ElQua4NAx.C:63-110 and ElHex8N3D.C:93-231, generated on the host
+// (api.hip, make_tables) and uploaded once per device.
+static __constant__ double c_hexN[8][8];
+static __constant__ double c_hexD[8][8][3];
+static __constant__ double c_quaN[4][4];
+static __constant__ double c_quaD[4][4][2];
+
+template <int NN>
+__device__ __forceinline__ double tabN(int p, int a) {
+  if constexpr (NN == 8) return c_hexN[p][a];
+  else return c_quaN[p][a];
+}
+template <int NN>
+__device__ __forceinline__ double tabD(int p, int a, int j) {
+  if constexpr (NN == 8) return c_hexD[p][a][j];
+  else return c_quaD[p][a][j];
+}
+
+static inline void upload_shape_tables(const double *hexN, const double *hexD, const double *quaN,
+                                       const double *quaD) {
+  hipMemcpyToSymbol(HIP_SYMBOL(c_hexN), hexN, sizeof(double) * 64);
+  hipMemcpyToSymbol(HIP_SYMBOL(c_hexD), hexD, sizeof(double) * 192);
+  hipMemcpyToSymbol(HIP_SYMBOL(c_quaN), quaN, sizeof(double) * 16);
+  hipMemcpyToSymbol(HIP_SYMBOL(c_quaD), quaD, sizeof(double) * 32);
+}
+
+}  // namespace dnlg

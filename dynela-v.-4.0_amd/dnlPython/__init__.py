@@ -17,6 +17,7 @@ Errors never exit the interpreter: engine statuses become exceptions
 ``PowerIterationError``, ``DnlgError``).
 """
 import ctypes
+import os
 import time
 import warnings
 import weakref
@@ -583,6 +584,10 @@ class Context:
         _lib.check(self.lib.dnlg_ctx_create(int(device), ctypes.byref(self.handle)))
         self.device = device
         self._models = weakref.WeakSet()  # models whose engine job lives in this context
+        # the measured schedule table next to the library (dnlPython.autotune)
+        if os.environ.get("DNLG_SCHEDULES", "1") != "0":
+            from . import autotune
+            autotune.load(self)
 
     @classmethod
     def default(cls, device=0):
@@ -604,6 +609,15 @@ class Context:
         n = ctypes.c_int64()
         self.check(self.lib.dnlg_ctx_kernel_time(self.handle, name.encode(), ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+    def add_schedule(self, elem_type, precision, min_elems=0, max_elems=0, **fields):
+        """A rule of the launch-schedule table (dnlg_ctx_add_schedule)."""
+        s = _lib.Schedule(**{k: int(v) for k, v in fields.items()})
+        self.check(self.lib.dnlg_ctx_add_schedule(self.handle, int(elem_type), int(precision), int(min_elems),
+                                                  int(max_elems), ctypes.byref(s)))
+
+    def clear_schedules(self):
+        self.check(self.lib.dnlg_ctx_clear_schedules(self.handle))
 
     def close(self):
         """Destroys the context; the engine jobs of its models are released
@@ -1183,6 +1197,33 @@ class DynELA:
 
     def getTotalKineticEnergy(self):
         return float(self.getEnergies()[0])
+
+    # ---- launch schedule (no counterpart in the reference, whose element
+    # loops take fixed OpenMP chunks, Parallel.C:78-113) ---------------------
+    def getSchedule(self):
+        """The launch schedule the job runs with (dict: elem_block, waves,
+        corr_batch, graph_cycles)."""
+        self.initSolve()
+        s = _lib.Schedule()
+        self.context.check(self.context.lib.dnlg_job_get_schedule(self._job, ctypes.byref(s)))
+        return s.as_dict()
+
+    def setSchedule(self, **fields):
+        """Set this job's schedule (fields not given keep the defaults)."""
+        self.initSolve()
+        s = _lib.Schedule(**{k: int(v) for k, v in fields.items()})
+        self.context.check(self.context.lib.dnlg_job_set_schedule(self._job, ctypes.byref(s)))
+
+    def autotune(self, cycles_per_trial=200, rounds=3):
+        """Measure the candidate schedules on this job by advancing it
+        (dnlg_job_autotune); returns (schedule dict, ms per cycle of the best
+        schedule, ms per cycle of the starting one) and leaves the best
+        schedule set."""
+        self.initSolve()
+        s, ms = _lib.Schedule(), (ctypes.c_double * 2)()
+        self.context.check(self.context.lib.dnlg_job_autotune(self._job, int(cycles_per_trial), int(rounds),
+                                                              ctypes.byref(s), ms))
+        return s.as_dict(), ms[0], ms[1]
 
     # ---- checkpoint / restart (no counterpart in the reference, whose
     # Model::readData / writeData are commented out, Model.C:1031-1100) -------

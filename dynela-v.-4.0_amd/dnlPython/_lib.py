@@ -104,6 +104,15 @@ def state_record(arr, time=0.0, time_step=0.0, increment=0):
     return s
 
 
+class Schedule(ctypes.Structure):
+    """dnlg_schedule of include/dnlg.h (launch schedule of a job)."""
+    _fields_ = [("elem_block", ctypes.c_int32), ("waves", ctypes.c_int32), ("corr_batch", ctypes.c_int32),
+                ("graph_cycles", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
 class SolverParams(ctypes.Structure):
     _fields_ = [("t_start", ctypes.c_double), ("t_end", ctypes.c_double), ("rho_b", ctypes.c_double),
                 ("safety", ctypes.c_double), ("dt_method", ctypes.c_int32), ("dt_freq", ctypes.c_int32),
@@ -137,6 +146,13 @@ SIGNATURES = {
     "dnlg_job_energies": (ctypes.c_int, [_VP, _dp]),
     "dnlg_job_energies_static": (ctypes.c_int, [_VP, _dp]),
     "dnlg_job_read_state": (ctypes.c_int, [_VP, ctypes.POINTER(State)]),
+    "dnlg_schedule_valid": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Schedule)]),
+    "dnlg_ctx_add_schedule": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
+                                             ctypes.POINTER(Schedule)]),
+    "dnlg_ctx_clear_schedules": (ctypes.c_int, [_VP]),
+    "dnlg_job_set_schedule": (ctypes.c_int, [_VP, ctypes.POINTER(Schedule)]),
+    "dnlg_job_get_schedule": (ctypes.c_int, [_VP, ctypes.POINTER(Schedule)]),
+    "dnlg_job_autotune": (ctypes.c_int, [_VP, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(Schedule), _dp]),
     "dnlg_job_write_state": (ctypes.c_int, [_VP, ctypes.POINTER(State)]),
     "dnlg_job_read_mass": (ctypes.c_int, [_VP, _dp]),
     "dnlg_footprint_bytes": (ctypes.c_int64, [ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int,

@@ -441,6 +441,48 @@ int dnlg_job_write_result_file(dnlg_job *job, int32_t force);
  * function does not know. */
 int dnlg_job_read_node_field(dnlg_job *job, int32_t field, double *out);
 
+/* ---- launch schedules and the autotuner -------------------------------------
+ * How the element kernels and the corrector are laid out on the GPU; never
+ * what they compute (results are the same bits under every schedule).  The
+ * reference splits its element loops into fixed OpenMP chunks
+ * (Parallel.C:78-113); here a schedule per element type, precision and job
+ * size comes from a table that the autotuner measured on the device. */
+typedef struct dnlg_schedule {
+  int32_t elem_block;   /* threads per element-kernel block: hex 64 / 128 / 256,
+                           axisymmetric quad 128 / 256 / 512; 0 = default */
+  int32_t waves;        /* waves per SIMD asked of the compiler: 0 = the
+                           variant's default, or its alternative (mixed 4,
+                           mixed-rr32 5, FP64 2; FP64's default is 3) */
+  int32_t corr_batch;   /* incidence slots the corrector loads together:
+                           2, 4, 8 (FP32 forces) or 2, 4 (FP64); 0 = default */
+  int32_t graph_cycles; /* cycles per captured HIP graph; 0 = default */
+} dnlg_schedule;
+/* Whether the kernels are built for this schedule of a job of this element
+ * type, precision and hardening law (1 yes, 0 no); the plane-strain quad and
+ * the network law run their default schedule only.  Needs no device. */
+int dnlg_schedule_valid(int elem_type, int precision, int law, const dnlg_schedule *sched);
+/* A rule of the context's table: jobs of this element type and precision
+ * with min_elems <= n_elems < max_elems (max_elems <= 0: no bound) run with
+ * this schedule; the first matching rule wins; DNLG_E_INVALID for a schedule
+ * the kernels are not built for. */
+int dnlg_ctx_add_schedule(dnlg_ctx *ctx, int elem_type, int precision, int64_t min_elems, int64_t max_elems,
+                          const dnlg_schedule *sched);
+int dnlg_ctx_clear_schedules(dnlg_ctx *ctx);
+/* The schedule of one job (overrides the rules), and the one it runs with. */
+int dnlg_job_set_schedule(dnlg_job *job, const dnlg_schedule *sched);
+int dnlg_job_get_schedule(dnlg_job *job, dnlg_schedule *sched);
+/* Autotuner: measures the candidate schedules on this job by advancing it
+ * (it must be between solves and at least rounds * (number of candidates +
+ * 3) * cycles_per_trial cycles short of its end time): each trial runs
+ * cycles_per_trial cycles after one untimed graph's worth, trials go
+ * round-robin over the candidates `rounds` times, and a candidate's time is
+ * its median.  Coordinate-wise: element block and waves, then the corrector
+ * batch, then the graph length.  The best schedule is left set on the job
+ * and returned; ms_per_cycle[0] its time per cycle (ms), ms_per_cycle[1]
+ * the starting schedule's (as measured among the candidates). */
+int dnlg_job_autotune(dnlg_job *job, int64_t cycles_per_trial, int rounds, dnlg_schedule *best,
+                      double ms_per_cycle[2]);
+
 /* ---- measurement hooks ------------------------------------------------------ */
 /* Device time (ms) and launch count of the element kernels, accumulated with
  * HIP events recorded around them on the engine's own stream since the last
