@@ -240,15 +240,28 @@ def run_serve(prec, ctx, n_jobs, util, rate, cycles_per_step, seed=20261203, cyc
     return res
 
 
-def measured_traffic(workload, prec, n, nz):
+def element_kernel(prec, schedule, dim=3, nn=8):
+    """Name of the element-kernel instantiation a schedule launches (as the
+    profiler prints it): <nodes, dim, block, axisymmetric, FP64 return,
+    waves> for the mixed kernel, <nodes, dim, block, axisymmetric, network
+    law, waves> for the FP64 one."""
+    b, w = schedule["elem_block"], schedule["waves"]
+    if prec == "fp64":
+        return f"k_elem_fp64<{nn}, {dim}, {b}, false, false, {w}>"
+    return f"k_elem_mixed<{nn}, {dim}, {b}, false, {'true' if prec == 'mixed' else 'false'}, {w}>"
+
+
+def measured_traffic(workload, kernel, n, nz):
     """HBM bytes per element-kernel launch measured offline with rocprofv3
     (FETCH_SIZE and WRITE_SIZE passes, scaled by a plain-copy calibration),
     when a summary exists for exactly this kernel and workload."""
-    path = os.path.join(ROOT, "profiles", "k_elem_mixed_hex8_traffic.json")
-    if workload != "taylor3d" or prec != "mixed" or (n, nz) != (20, 100) or not os.path.exists(path):
+    path = os.path.join(ROOT, "profiles", "element_kernels_hbm_traffic.json")
+    if workload != "taylor3d" or (n, nz) != (20, 100) or not os.path.exists(path):
         return None
     with open(path) as f:
-        return float(json.load(f)["traffic_bytes_per_launch"])
+        table = json.load(f)
+    k = table["kernels"].get(kernel)
+    return float(k["traffic_bytes_per_launch"]) if k else None
 
 
 def host_cpu():
@@ -442,6 +455,7 @@ def main():
     npe = r["nn"] / r["ne"]
     bytes_el = algo_bytes_per_elem(head, npe)
     avg_ms = r["elem_ms"] / max(r["elem_launches"], 1)
+    kernel = element_kernel(head, r["schedule"])
     achieved = bytes_el * r["ne"] / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     if args.workload == "taylor3d":
         wl = (f"Taylor impact 3D, {args.n}x{args.n}x{args.nz} O-grid hex8 ({r['ne']} elements, {r['nn']} nodes), "
@@ -466,8 +480,8 @@ def main():
                    "schedule": r["schedule"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(args.workload, head, args.n, args.nz),
-                     "kernel": "k_elem_mixed<8,3,128> (hex8)" if head != "fp64" else "k_elem_fp64<8,3,128> (hex8)",
+                     "traffic": measured_traffic(args.workload, kernel, args.n, args.nz),
+                     "kernel": kernel,
                      "kernel_avg_ms": avg_ms, "algo_bytes_per_elem_cycle": bytes_el},
     }
     for p in precs[1:]:

@@ -224,11 +224,13 @@ struct LaneRows {
 // thermal terms of the laws here), so the confirming iteration is left out.
 // In: the trial stress Strial = sqrt(3/2) |s_trial|, yield = the stored
 // yield stress, g = the previous gamma (the start unless ep0 == 0, as in the
-// reference).  Out: gamma and the yield stress of the last iterate.  The
-// state at the start of the last iteration -- its gamma, the yield stress
-// its temperature was formed with, and the bracket before its update -- goes
-// to `last` (for the FP64 return that resumes there).  Returns false at the
-// iteration limit.
+// reference).  Out: gamma and the yield stress of the last iterate, and in
+// `last` the state an FP64 return resumes at -- a gamma, the yield stress
+// its temperature is formed with and the bracket: the start of the last
+// iteration, or, when that iteration ended on a Newton step of at least the
+// reference's tolerance 1e-8, the start of the next one (the reference takes
+// that iterate too, and a Newton step from it lands on the root to
+// rounding).  Returns false at the iteration limit.
 struct NewtonState {
   float g, y, gmin, gmax;
 };
@@ -260,7 +262,10 @@ __device__ __forceinline__ bool radial_return_fp32(const LawF &L, int max_it, fl
       g = gmin + dg;
       newton = false;
     }
-    if (fabsf(dg) < fmaxf((float)kTolNR, (newton ? 2e-4f : 4.8e-7f) * g)) return true;
+    if (fabsf(dg) < fmaxf((float)kTolNR, (newton ? 2e-4f : 4.8e-7f) * g)) {
+      if (newton && fabsf(dg) >= (float)kTolNR) last = NewtonState{g, yield, gmin, gmax};
+      return true;
+    }
     epsp = ep0f + s23 * g;
     T = Tc + 0.5f * g * heat * (s23 * yield + Snorm0);
     if (++iterate > max_it) return false;
@@ -270,13 +275,14 @@ __device__ __forceinline__ bool radial_return_fp32(const LawF &L, int max_it, fl
 // The same Newton in FP64 with the reference's stopping rule (|dgamma| <
 // 1e-8, at most 250 iterations) and operation order, the yield stress and
 // slope from the device libm (law_yield_and_slope), resuming the FP32
-// iteration at the start of its last iteration: gamma g, the yield stress y
-// that iteration's temperature was formed with, and the bracket [gmin,
-// gmax] before its update.  From there the reference's own iterates are
-// taken again in FP64: its path is reproduced up to the FP32 rounding of
-// the resumed state, which the Newton step contracts, so one or two FP64
-// iterations end where the reference ends.  Resuming anywhere else would not
-// do: the reference's result depends on its path -- its absolute tolerance
+// iteration where radial_return_fp32 leaves it: gamma g, the yield stress y
+// that iteration's temperature is formed with, and the bracket [gmin,
+// gmax].  From there the reference's own iterates are taken again in FP64:
+// its path is reproduced up to the FP32 rounding of the resumed state,
+// which the Newton step contracts, so one FP64 iteration ends where the
+// reference ends (more after a bisection, or where the two precisions fall
+// on different sides of the tolerance).  Resuming further back would only cost iterations;
+// resuming anywhere else would not do: the reference's result depends on its path -- its absolute tolerance
 // leaves a small gamma (first yield) 1e-3 to 1e-1 short of the root, and
 // when the root lies above the initial bound (Strial - y_stored) /
 // (sqrt(3/2) 2G) a start above the bound moves gmax up to itself and

@@ -91,6 +91,18 @@ __device__ __forceinline__ void eig_2x2(double a, double b, double c, double &rt
   }
 }
 
+// Roots w of 1 + sum z_j / (d_j - w) = 0 (Eigen3x3.C, the secular step of
+// dsyevd3) and P[j][i] = d_j - w_i, each root refined by the reference's
+// safeguarded Newton / bisection iteration and stopping test from a closed-
+// form starting value.  One deliberate difference: the starting values are
+// the roots of the characteristic cubic written in d - m, m the mean of d,
+// where the reference writes it in d.  For F^T F near the identity (every
+// strain increment) the reference's coefficients cancel to a few digits,
+// its starting values miss by about 1 % of the eigenvalue gaps, and the
+// refinement takes about 9 iterations per root, many of them bisections;
+// from the shifted cubic's values it takes 2, and lands inside the same
+// acceptance band of the stopping test (same eigen pairs to rounding; the
+// CPU restatement of the reference keeps the reference's start).
 template <int I0, int I1, int I2>
 __device__ __forceinline__ void eig_secular(const double d[3], const double z[3], double w[3], double P[3][3]) {
   double a[4];
@@ -105,10 +117,12 @@ __device__ __forceinline__ void eig_secular(const double d[3], const double z[3]
     a[2] = d[I1];
     a[3] = d[I2];
   }
-  const double t0 = d[1] * d[2], t1 = d[0] * d[2], t2 = d[0] * d[1];
-  const double cg = t0 * d[0] + (z[0] * t0 + z[1] * t1 + z[2] * t2);
-  const double cb = (z[0] * (d[1] + d[2]) + z[1] * (d[0] + d[2]) + z[2] * (d[0] + d[1])) + (t0 + t1 + t2);
-  const double ca = (z[0] + z[1] + z[2]) + (d[0] + d[1] + d[2]);
+  const double msh = (d[0] + d[1] + d[2]) * (1.0 / 3.0);
+  const double e0 = d[0] - msh, e1 = d[1] - msh, e2 = d[2] - msh;
+  const double t0 = e1 * e2, t1 = e0 * e2, t2 = e0 * e1;
+  const double cg = t0 * e0 + (z[0] * t0 + z[1] * t1 + z[2] * t2);
+  const double cb = (z[0] * (e1 + e2) + z[1] * (e0 + e2) + z[2] * (e0 + e1)) + (t0 + t1 + t2);
+  const double ca = (z[0] + z[1] + z[2]) + (e0 + e1 + e2);
   const double p = sqd(ca) - 3.0 * cb;
   const double q = ca * (p - (3.0 / 2.0) * cb) + (27.0 / 2.0) * cg;
   const double sqrt_p = sqrt(fabs(p));
@@ -133,6 +147,8 @@ __device__ __forceinline__ void eig_secular(const double d[3], const double z[3]
     x0[1] += c;
     x0[2] += s;
   }
+#pragma unroll
+  for (int i = 0; i < 3; i++) x0[i] += msh;
 #pragma unroll
   for (int i = 0; i < 3; i++) {
     double xl = a[i], xh = a[i + 1];

@@ -28,11 +28,21 @@ def test_algorithmic_bytes_per_element_cycle():
     assert round(bench.algo_bytes_per_elem("mixed", npe), 1) == 1298.1
 
 
-def test_measured_traffic_only_for_its_workload():
-    t = bench.measured_traffic("taylor3d", "mixed", 20, 100)
-    with open(os.path.join(ROOT, "profiles", "k_elem_mixed_hex8_traffic.json")) as f:
+def test_element_kernel_names_the_launched_instantiation():
+    sched = {"elem_block": 64, "waves": 0, "corr_batch": 4, "graph_cycles": 80}
+    assert bench.element_kernel("mixed", sched) == "k_elem_mixed<8, 3, 64, false, true, 0>"
+    assert bench.element_kernel("mixed-rr32", sched) == "k_elem_mixed<8, 3, 64, false, false, 0>"
+    assert bench.element_kernel("fp64", dict(sched, waves=3)) == "k_elem_fp64<8, 3, 64, false, false, 3>"
+
+
+def test_measured_traffic_only_for_its_kernel_and_workload():
+    with open(os.path.join(ROOT, "profiles", "element_kernels_hbm_traffic.json")) as f:
         d = json.load(f)
-    expect = (d["fetch_size_kib_per_launch"] * d["fetch_scale"] + d["write_size_kib_per_launch"] * d["write_scale"]) * 1024
-    assert t == pytest.approx(expect, rel=1e-3)
-    assert bench.measured_traffic("taylor3d", "fp64", 20, 100) is None
-    assert bench.measured_traffic("necking", "mixed", 40, 208) is None
+    assert d["kernels"]
+    for name, k in d["kernels"].items():
+        expect = (k["fetch_size_kib_per_launch"] * d["fetch_scale"]
+                  + k["write_size_kib_per_launch"] * d["write_scale"]) * 1024
+        assert bench.measured_traffic("taylor3d", name, 20, 100) == pytest.approx(expect, rel=1e-3)
+        assert bench.measured_traffic("necking", name, 40, 208) is None
+        assert bench.measured_traffic("taylor3d", name, 10, 50) is None
+    assert bench.measured_traffic("taylor3d", "k_elem_mixed<8, 3, 32, false, true, 0>", 20, 100) is None

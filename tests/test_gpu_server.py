@@ -68,13 +68,18 @@ def _footprint(model):
                                                   ctypes.byref(a.bcs), ctypes.byref(a.params))
 
 
-def test_server_admission_by_footprint(ctx):
+def test_server_admission_by_footprint():
     """With a budget just above the largest job's footprint, the running jobs
     never hold more than the budget (the largest job runs alone, smaller
     ones may share), all finish and match the alone results; a job larger
-    than the whole budget is refused at submission."""
+    than the whole budget is refused at submission.  Its own context, with
+    no schedule table: the default 16-cycle graphs make a 32-cycle step
+    stop inside the largest job's 79 cycles, so the test sees it running
+    (a tuned 80-cycle graph would run it to its end within one step)."""
     import dnlPython as dnl
     from dnlPython import samples
+    ctx = dnl.Context()
+    ctx.clear_schedules()
     specs = _specs()[:3]
     alone = [f(ctx, "fp64") for f in specs]
     for m in alone:
