@@ -274,15 +274,17 @@ __device__ __forceinline__ bool radial_return_fp32(const LawF &L, int max_it, fl
 
 // The same Newton in FP64 with the reference's stopping rule (|dgamma| <
 // 1e-8, at most 250 iterations) and operation order, the yield stress and
-// slope from the device libm (law_yield_and_slope), resuming the FP32
-// iteration where radial_return_fp32 leaves it: gamma g, the yield stress y
-// that iteration's temperature is formed with, and the bracket [gmin,
-// gmax].  From there the reference's own iterates are taken again in FP64:
-// its path is reproduced up to the FP32 rounding of the resumed state,
-// which the Newton step contracts, so one FP64 iteration ends where the
-// reference ends (more after a bisection, or where the two precisions fall
-// on different sides of the tolerance).  Resuming further back would only cost iterations;
-// resuming anywhere else would not do: the reference's result depends on its path -- its absolute tolerance
+// slope from law_yield_and_slope in FP64 (its logarithms and reciprocals
+// within 2 ulp, ShortOps), resuming the FP32 iteration where
+// radial_return_fp32 leaves it: gamma g, the yield stress y that
+// iteration's temperature is formed with, and the bracket [gmin, gmax].
+// From there the reference's own iterates are taken again in FP64: its path
+// is reproduced up to the FP32 rounding of the resumed state, which the
+// Newton step contracts, so one FP64 iteration ends where the reference
+// ends (more after a bisection, or where the two precisions fall on
+// different sides of the tolerance).  Resuming further back would only
+// cost iterations; resuming anywhere else would not do: the reference's
+// result depends on its path -- its absolute tolerance
 // leaves a small gamma (first yield) 1e-3 to 1e-1 short of the root, and
 // when the root lies above the initial bound (Strial - y_stored) /
 // (sqrt(3/2) 2G) a start above the bound moves gmax up to itself and
@@ -298,14 +300,14 @@ __device__ __forceinline__ bool radial_return_fp64(int law, const double *lp, in
   int iterate = 0;
   while (true) {
     double hard;
-    law_yield_and_slope(law, lp, heatFrac, epsp, depsp, T, dt, y, hard);
+    law_yield_and_slope<ShortOps>(law, lp, heatFrac, epsp, depsp, T, dt, y, hard);
     const double fun = Strial - g * TwoG32 - y;
     if (fun < 0.0)
       gmax = g;
     else
       gmin = g;
     const double dfun = TwoG32 + kSqrt23 * hard;
-    double dg = fun / dfun;
+    double dg = fun * rcp_f64(dfun);
     g += dg;
     if ((gmax - g) * (g - gmin) < 0.0) {
       dg = 0.5 * (gmax - gmin);

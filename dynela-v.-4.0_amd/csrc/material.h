@@ -19,40 +19,101 @@ __device__ __forceinline__ double law_yield(int law, const double *p, double eps
   return p[0] + p[1] * (1.0 - exp(-p[2] * epsp)) + p[3] * epsp;
 }
 
+// Logarithms and divisions of the FP64 law evaluation: the device libm and
+// IEEE division (the FP64 path, which tracks the reference to rounding) ...
+struct LibmOps {
+  static constexpr bool kLibm = true;
+  __device__ static double lg(double x) { return log(x); }
+  __device__ static double div(double a, double b) { return a / b; }
+};
+
+// ... or a shorter logarithm and a reciprocal from the hardware estimate
+// (the FP64 Newton of the mixed path).  log: x = 2^k m with m in
+// [1/sqrt(2), sqrt(2)), log m = 2 atanh(s), s = (m - 1) / (m + 1), |s| <=
+// 0.1716, by its series to s^21 (truncation below 2.3e-17 relative), k ln 2
+// in two parts; within 2 ulp of the exact logarithm on [1e-9, 20] and near 1
+// (30 VALU instructions against the libm's 98).  Positive normal x only:
+// the law's arguments are.  Reciprocal: the hardware estimate refined by two
+// Newton steps.
+__device__ __forceinline__ double rcp_f64(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = __builtin_fma(-d, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-d, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+__device__ __forceinline__ double log_f64_short(double x) {
+  double m = __builtin_amdgcn_frexp_mant(x);  // [0.5, 1)
+  int k = __builtin_amdgcn_frexp_exp(x);
+  const bool low = m < 0.70710678118654752;
+  m = low ? 2.0 * m : m;
+  k = low ? k - 1 : k;
+  const double num = m - 1.0, den = m + 1.0;
+  const double r = rcp_f64(den);
+  double s = num * r;
+  s = __builtin_fma(r, __builtin_fma(-den, s, num), s);
+  const double z = s * s;
+  double q = 2.0 / 21.0;
+  q = __builtin_fma(q, z, 2.0 / 19.0);
+  q = __builtin_fma(q, z, 2.0 / 17.0);
+  q = __builtin_fma(q, z, 2.0 / 15.0);
+  q = __builtin_fma(q, z, 2.0 / 13.0);
+  q = __builtin_fma(q, z, 2.0 / 11.0);
+  q = __builtin_fma(q, z, 2.0 / 9.0);
+  q = __builtin_fma(q, z, 2.0 / 7.0);
+  q = __builtin_fma(q, z, 2.0 / 5.0);
+  q = __builtin_fma(q, z, 2.0 / 3.0);
+  const double kd = (double)k;
+  const double t = __builtin_fma(s * z, q, kd * 1.90821492927058770002e-10);  // + k ln2 (low part)
+  return __builtin_fma(kd, 6.93147180369123816490e-01, 2.0 * s + t);          // + k ln2 (high part)
+}
+struct ShortOps {
+  static constexpr bool kLibm = false;
+  __device__ static double lg(double x) { return log_f64_short(x); }
+  __device__ static double div(double a, double b) { return a * rcp_f64(b); }
+};
+
 // Yield stress and its consistent derivative in one pass: shares the power
 // and logarithm terms of JohnsonCookLaw::getYieldStress and
 // getDerivateYieldStress (JohnsonCookLaw.C:298-339).  eta_rcp = eta/(rho*Cp).
+template <class Ops = LibmOps>
 __device__ __forceinline__ void law_yield_and_slope(int law, const double *p, double eta_rcp, double epsp,
                                                     double depsp, double T, double dtime, double &yield,
                                                     double &hard) {
   if (law == DNLG_LAW_JOHNSON_COOK) {
     const double A = p[0], B = p[1], C = p[2], n = p[3], m = p[4], d0 = p[5], Tm = p[6], T0 = p[7];
-    const double lg = log(epsp);
+    const double lg = Ops::lg(epsp);
     const double pn = exp(n * lg);  // epsp^n
     const double hard0 = A + B * pn;
     const bool rate = depsp > d0;
-    const double visc0 = rate ? 1.0 + C * log(depsp / d0) : 1.0;
+    const double visc0 = rate ? 1.0 + C * Ops::lg(Ops::div(depsp, d0)) : 1.0;
     double tp1 = 0.0, temp0 = 1.0;
     const bool warm = T > T0;
     if (warm) {
       if (T < Tm) {
-        tp1 = exp(m * log((T - T0) / (Tm - T0)));  // ((T - T0) / (Tm - T0))^m, 0 < base < 1
+        tp1 = exp(m * Ops::lg(Ops::div(T - T0, Tm - T0)));  // ((T - T0) / (Tm - T0))^m, 0 < base < 1
         temp0 = 1.0 - tp1;
       } else {
         temp0 = 0.0;
       }
     }
     yield = hard0 * visc0 * temp0;
-    double h = n * B * (pn / epsp) * visc0 * temp0;
-    if (rate) h += hard0 * C * temp0 / (depsp * dtime);
-    if (warm && T < Tm) h -= m * eta_rcp * hard0 * hard0 * visc0 * visc0 * temp0 * tp1 / (T - T0);
+    double h = n * B * Ops::div(pn, epsp) * visc0 * temp0;
+    if (rate) h += Ops::div(hard0 * C * temp0, depsp * dtime);
+    if (warm && T < Tm) h -= Ops::div(m * eta_rcp * hard0 * hard0 * visc0 * visc0 * temp0 * tp1, T - T0);
     hard = h;
     return;
   }
   if (law == DNLG_LAW_POWER) {
-    const double pn = pow(epsp, p[2]);
-    yield = p[0] + p[1] * pn;
-    hard = p[2] * p[1] * pow(epsp, p[2] - 1.0);
+    if constexpr (Ops::kLibm) {
+      const double pn = pow(epsp, p[2]);
+      yield = p[0] + p[1] * pn;
+      hard = p[2] * p[1] * pow(epsp, p[2] - 1.0);
+    } else {
+      const double pn = exp(p[2] * Ops::lg(epsp));
+      yield = p[0] + p[1] * pn;
+      hard = p[2] * p[1] * Ops::div(pn, epsp);
+    }
     return;
   }
   const double ex = exp(-p[2] * epsp);
