@@ -402,6 +402,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
   constexpr int MR = (2 * DIM + 3) / 4;  // float4 per lane row
   typedef LaneRows<NN, MR> ModeRows;
   __shared__ float4 s_mode[(BLOCK / NN) * ModeRows::kGroup];
+  __shared__ float s_keep[9][BLOCK];  // values held across the radial return
   {
     float row[4 * MR];
 #pragma unroll
@@ -491,7 +492,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
   float hoop = 0.f;
   float sig[6];
   if (valid) {
-    const float pressure = (sig0[0] + sig0[3] + sig0[5]) * (1.f / 3.f) + (float)C->K * inc;
+    float pressure = (sig0[0] + sig0[3] + sig0[5]) * (1.f / 3.f) + (float)C->K * inc;
 
     const double dt = cc.dt;
     const float TwoGf = (float)C->twoG;
@@ -525,7 +526,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
     auto wdot = [](const float x[6], const float y[6]) {
       return x[0] * y[0] + x[3] * y[3] + x[5] * y[5] + 2.f * (x[1] * y[1] + x[2] * y[2] + x[4] * y[4]);
     };
-    const float work_old = wdot(dE, sig0), work_s = wdot(dE, s), trdE = dE[0] + dE[3] + dE[5];
+    // power = 1/2 (work_a + scale work_s) once the return has fixed scale
+    float work_a = wdot(dE, sig0) + pressure * (dE[0] + dE[3] + dE[5]), work_s = wdot(dE, s);
     float Rs[6];
 #pragma unroll
     for (int c = 0; c < 6; c++) Rs[c] = s[c];
@@ -551,8 +553,17 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
     }
     if (Strialf > yieldf) {
       // radial return (Element.C:485-601): Newton on gamma in FP32, then
-      // (RR64) the FP64 Newton resumed at its last iteration; ep, T and the
-      // energies accumulate in FP64
+      // (RR64) the FP64 Newton resumed where it leaves off; ep, T and the
+      // energies accumulate in FP64.  What the stress update needs after
+      // the return waits in LDS meanwhile (the return's FP64 law evaluation
+      // is the kernel's register peak); the empty asm keeps the compiler
+      // from holding the values in registers across it.
+      s_keep[0][threadIdx.x] = work_a;
+      s_keep[1][threadIdx.x] = work_s;
+      s_keep[2][threadIdx.x] = pressure;
+#pragma unroll
+      for (int c = 0; c < 6; c++) s_keep[3 + c][threadIdx.x] = Rs[c];
+      asm volatile("" ::: "memory");
       float pf[8];
 #pragma unroll
       for (int k = 0; k < 8; k++) pf[k] = C->lpf[k];
@@ -571,13 +582,19 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
         yieldf = (float)y;
       }
       if (!ok) set_status(J, DNLG_E_NR_DIVERGED, par);
+      asm volatile("" ::: "memory");
+      work_a = s_keep[0][threadIdx.x];
+      work_s = s_keep[1][threadIdx.x];
+      pressure = s_keep[2][threadIdx.x];
+#pragma unroll
+      for (int c = 0; c < 6; c++) Rs[c] = s_keep[3 + c][threadIdx.x];
       scale = 1.f - TwoGf * g * __builtin_amdgcn_rcpf(Snorm);
       gbl(C->ep)[gi] = ep0 + kSqrt23 * gamma;
       gbl(C->dep32)[gi] = (float)(kSqrt23 * gamma / dt);
       gbl(C->gam32)[gi] = g;
       gbl(C->sy32)[gi] = yieldf;
     }
-    const float power = 0.5f * (work_old + scale * work_s + pressure * trdE);
+    const float power = 0.5f * (work_a + scale * work_s);
     gbl(C->eint)[gi] += (double)power * C->inv_rho0;
     if (gamma != 0.0) {
       const float sn = scale * Snorm;  // |scale s|

@@ -27,10 +27,10 @@ constexpr int kHexBlocks[kSchedBlocks] = {64, 128, 256};
 constexpr int kQuadBlocks[kSchedBlocks] = {128, 256, 512};
 // Waves per SIMD requested from the compiler (amdgpu_waves_per_eu): each
 // variant has its default (0: none requested, the registers the kernel
-// needs set the occupancy) and one alternative that trades register spills
-// for occupancy.
+// needs set the occupancy -- 4 for both mixed variants) and one alternative
+// that trades register spills for occupancy.
 constexpr int kFP64Waves = 3, kFP64WavesAlt = 2;
-constexpr int kMixedWavesAlt = 4, kRR32WavesAlt = 5;
+constexpr int kMixedWavesAlt = 5, kRR32WavesAlt = 5;
 // Incidence slots the corrector loads together (FP32 forces: 2, 4, 8; FP64
 // forces: 2, 4).
 // Cycles per captured graph (rounded up to a multiple of the time-step

@@ -12,7 +12,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 BLOCKS = {"hex": (64, 128, 256), "axi": (128, 256, 512)}
-ALT = {"fp64": 2, "mixed": 4, "mixed-rr32": 5}
+ALT = {"fp64": 2, "mixed": 5, "mixed-rr32": 5}
 
 
 def _model(ctx, family, prec, t_end=6e-6):

@@ -22,7 +22,7 @@ def valid(t, prec, law=JC, **kw):
 def test_candidates_are_built():
     for t, blocks in ((HEX, (64, 128, 256)), (AXI, (128, 256, 512))):
         for b in blocks:
-            for prec, alt in ((_lib.PREC_MIXED, 4), (_lib.PREC_MIXED_RR32, 5), (_lib.PREC_FP64, 2)):
+            for prec, alt in ((_lib.PREC_MIXED, 5), (_lib.PREC_MIXED_RR32, 5), (_lib.PREC_FP64, 2)):
                 assert valid(t, prec, elem_block=b)
                 assert valid(t, prec, elem_block=b, waves=alt)
     assert valid(HEX, _lib.PREC_FP64, waves=3) and valid(HEX, _lib.PREC_MIXED, waves=0)
@@ -34,7 +34,7 @@ def test_candidates_are_built():
 def test_other_schedules_are_refused():
     assert not valid(HEX, _lib.PREC_MIXED, elem_block=512)      # a quad block size
     assert not valid(AXI, _lib.PREC_MIXED, elem_block=64)
-    assert not valid(HEX, _lib.PREC_MIXED, waves=5)             # the FP32-return alternative
+    assert not valid(HEX, _lib.PREC_MIXED, waves=4)             # not built: the natural occupancy
     assert not valid(HEX, _lib.PREC_MIXED, corr_batch=3)
     assert not valid(HEX, _lib.PREC_MIXED, graph_cycles=1)
     # the plane-strain quad and the network law run their defaults only
