@@ -24,17 +24,20 @@ __device__ __forceinline__ double law_yield(int law, const double *p, double eps
 struct LibmOps {
   static constexpr bool kLibm = true;
   __device__ static double lg(double x) { return log(x); }
+  __device__ static double ex(double x) { return exp(x); }
   __device__ static double div(double a, double b) { return a / b; }
 };
 
-// ... or a shorter logarithm and a reciprocal from the hardware estimate
-// (the FP64 Newton of the mixed path).  log: x = 2^k m with m in
-// [1/sqrt(2), sqrt(2)), log m = 2 atanh(s), s = (m - 1) / (m + 1), |s| <=
-// 0.1716, by its series to s^21 (truncation below 2.3e-17 relative), k ln 2
-// in two parts; within 2 ulp of the exact logarithm on [1e-9, 20] and near 1
-// (30 VALU instructions against the libm's 98).  Positive normal x only:
-// the law's arguments are.  Reciprocal: the hardware estimate refined by two
-// Newton steps.
+// ... or a shorter logarithm and exponential and a reciprocal from the
+// hardware estimate (the FP64 Newton of the mixed path).  log: x = 2^k m
+// with m in [1/sqrt(2), sqrt(2)), log m = 2 atanh(s), s = (m - 1) / (m +
+// 1), |s| <= 0.1716, by its series to s^21 (truncation below 2.3e-17
+// relative), k ln 2 in two parts; within 2 ulp of the exact logarithm on
+// [1e-9, 20] and near 1 (30 VALU instructions against the libm's 98).
+// Positive normal x only: the law's arguments are.  exp: x = k ln 2 + r,
+// |r| <= ln 2 / 2, e^r by its Taylor series to r^13, times 2^k; within 1
+// ulp on [-30, 5] (22 instructions against 36).  Reciprocal: the hardware
+// estimate refined by two Newton steps.
 __device__ __forceinline__ double rcp_f64(double d) {
   double r = __builtin_amdgcn_rcp(d);
   double e = __builtin_fma(-d, r, 1.0);
@@ -67,9 +70,30 @@ __device__ __forceinline__ double log_f64_short(double x) {
   const double t = __builtin_fma(s * z, q, kd * 1.90821492927058770002e-10);  // + k ln2 (low part)
   return __builtin_fma(kd, 6.93147180369123816490e-01, 2.0 * s + t);          // + k ln2 (high part)
 }
+__device__ __forceinline__ double exp_f64_short(double x) {
+  const double k = __builtin_rint(x * 1.4426950408889634074);  // x / ln 2
+  double r = __builtin_fma(-k, 6.93147180369123816490e-01, x);
+  r = __builtin_fma(-k, 1.90821492927058770002e-10, r);
+  double q = 1.0 / 6227020800.0;  // 1/13!
+  q = __builtin_fma(q, r, 1.0 / 479001600.0);
+  q = __builtin_fma(q, r, 1.0 / 39916800.0);
+  q = __builtin_fma(q, r, 1.0 / 3628800.0);
+  q = __builtin_fma(q, r, 1.0 / 362880.0);
+  q = __builtin_fma(q, r, 1.0 / 40320.0);
+  q = __builtin_fma(q, r, 1.0 / 5040.0);
+  q = __builtin_fma(q, r, 1.0 / 720.0);
+  q = __builtin_fma(q, r, 1.0 / 120.0);
+  q = __builtin_fma(q, r, 1.0 / 24.0);
+  q = __builtin_fma(q, r, 1.0 / 6.0);
+  q = __builtin_fma(q, r, 0.5);
+  q = __builtin_fma(q, r, 1.0);
+  q = __builtin_fma(q, r, 1.0);
+  return __builtin_amdgcn_ldexp(q, (int)k);
+}
 struct ShortOps {
   static constexpr bool kLibm = false;
   __device__ static double lg(double x) { return log_f64_short(x); }
+  __device__ static double ex(double x) { return exp_f64_short(x); }
   __device__ static double div(double a, double b) { return a * rcp_f64(b); }
 };
 
@@ -83,7 +107,7 @@ __device__ __forceinline__ void law_yield_and_slope(int law, const double *p, do
   if (law == DNLG_LAW_JOHNSON_COOK) {
     const double A = p[0], B = p[1], C = p[2], n = p[3], m = p[4], d0 = p[5], Tm = p[6], T0 = p[7];
     const double lg = Ops::lg(epsp);
-    const double pn = exp(n * lg);  // epsp^n
+    const double pn = Ops::ex(n * lg);  // epsp^n
     const double hard0 = A + B * pn;
     const bool rate = depsp > d0;
     const double visc0 = rate ? 1.0 + C * Ops::lg(Ops::div(depsp, d0)) : 1.0;
@@ -91,7 +115,7 @@ __device__ __forceinline__ void law_yield_and_slope(int law, const double *p, do
     const bool warm = T > T0;
     if (warm) {
       if (T < Tm) {
-        tp1 = exp(m * Ops::lg(Ops::div(T - T0, Tm - T0)));  // ((T - T0) / (Tm - T0))^m, 0 < base < 1
+        tp1 = Ops::ex(m * Ops::lg(Ops::div(T - T0, Tm - T0)));  // ((T - T0) / (Tm - T0))^m, 0 < base < 1
         temp0 = 1.0 - tp1;
       } else {
         temp0 = 0.0;
@@ -110,13 +134,13 @@ __device__ __forceinline__ void law_yield_and_slope(int law, const double *p, do
       yield = p[0] + p[1] * pn;
       hard = p[2] * p[1] * pow(epsp, p[2] - 1.0);
     } else {
-      const double pn = exp(p[2] * Ops::lg(epsp));
+      const double pn = Ops::ex(p[2] * Ops::lg(epsp));
       yield = p[0] + p[1] * pn;
       hard = p[2] * p[1] * Ops::div(pn, epsp);
     }
     return;
   }
-  const double ex = exp(-p[2] * epsp);
+  const double ex = Ops::ex(-p[2] * epsp);
   yield = p[0] + p[1] * (1.0 - ex) + p[3] * epsp;
   hard = p[1] * p[2] * ex + p[3];
 }
