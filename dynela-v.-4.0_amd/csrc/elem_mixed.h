@@ -274,7 +274,7 @@ __device__ __forceinline__ bool radial_return_fp32(const LawF &L, int max_it, fl
 
 // The same Newton in FP64 with the reference's stopping rule (|dgamma| <
 // 1e-8, at most 250 iterations) and operation order, the yield stress and
-// slope from law_yield_and_slope in FP64 (its logarithms and reciprocals
+// slope from law_yield_and_slope in FP64 (logarithms, exponentials, reciprocals
 // within 2 ulp, ShortOps), resuming the FP32 iteration where
 // radial_return_fp32 leaves it: gamma g, the yield stress y that
 // iteration's temperature is formed with, and the bracket [gmin, gmax].
