@@ -98,18 +98,55 @@ int dalloc(dnlg_job *job, T **p, int64_t count, const T *init = nullptr) {
   return DNLG_OK;
 }
 
-int64_t footprint(int type, int64_t N, int64_t NE, int prec, int track) {
+// Incidence slots of a mesh (the layout JobDev describes): slot_w = the
+// largest incidence count, at most nn (the count at an interior node of a
+// structured mesh); node n's k-th incident (element, local node) in element
+// order goes to slot n slot_w + k for k < slot_w, else to the overflow slot
+// N slot_w + ovf[n] + k - slot_w.  Fills e2slot (and ovf, left empty when
+// no node has more than slot_w incidences) when given; returns the number
+// of slots.
+int64_t incidence_slots(const int32_t *conn, int64_t N, int64_t NE, int nn, int32_t *slot_w,
+                        std::vector<int32_t> *ovf, std::vector<int32_t> *e2slot) {
+  std::vector<int32_t> cnt(N, 0);
+  for (int64_t k = 0; k < NE * nn; k++) cnt[conn[k]]++;
+  int32_t mx = 1;
+  for (int64_t n = 0; n < N; n++) mx = std::max(mx, cnt[n]);
+  const int32_t W = std::min(mx, (int32_t)nn);
+  std::vector<int32_t> optr(N + 1, 0);
+  for (int64_t n = 0; n < N; n++) optr[n + 1] = optr[n] + std::max(cnt[n] - W, 0);
+  const int64_t total = N * W + optr[N];
+  if (slot_w) *slot_w = W;
+  if (e2slot) {
+    e2slot->assign(NE * nn, 0);
+    std::vector<int32_t> fill(N, 0);
+    for (int64_t e = 0; e < NE; e++)
+      for (int a = 0; a < nn; a++) {
+        const int32_t n = conn[e * nn + a], k = fill[n]++;
+        (*e2slot)[e * nn + a] = k < W ? (int32_t)(n * W + k) : (int32_t)(N * W + optr[n] + k - W);
+      }
+  }
+  if (ovf) {
+    ovf->clear();
+    if (optr[N] > 0) *ovf = std::move(optr);
+  }
+  return total;
+}
+
+// Device bytes of a job.  n_slots < 0: the incidence slots are not known
+// (no connectivity at hand) and are bounded by NE nn + N nn.
+int64_t footprint(int type, int64_t N, int64_t NE, int prec, int track, int64_t n_slots = -1) {
   const int dim = type == DNLG_EL_HEX8N_3D ? 3 : 2;
   const int nn = type == DNLG_EL_HEX8N_3D ? 8 : 4;
   const int64_t G = NE * nn;
+  if (n_slots < 0) n_slots = NE * nn + N * nn;
   int64_t b = 0;
   b += N * dim * 8 * 8;            // x, x0, fu/fv/fa x2
   b += N * 8;                      // mass
-  b += (N + 1) * 4 * 2 + NE * nn * 4 * 2;  // CSRs, connectivity
+  b += (N + 1) * 4 * 2 + NE * nn * 4 * 2;  // overflow ranges, BC ranges, connectivity, slots
   b += N * 4;                      // BC node list (upper bound)
-  b += NE * nn * dim * 8;          // elemF
+  b += n_slots * dim * 8;          // elemF
   if (is_mixed(prec)) {
-    b += NE * nn * dim * 4;        // elemF32
+    b += n_slots * dim * 4;        // elemF32
     b += G * (6 * 4 + 3 * 4 + 4 * 8 + 8 + 8 + 4);
   } else {
     b += G * (6 * 8 + 8 * 8 + 8 + 8);
@@ -490,9 +527,17 @@ int64_t dnlg_footprint_bytes(int elem_type, int64_t n_nodes, int64_t n_elems, in
 int64_t dnlg_model_footprint_bytes(const dnlg_mesh *mesh, const dnlg_material *mat, const dnlg_bcs *bcs,
                                    const dnlg_solver_params *prm) {
   if (!mesh || !mat || !prm) return -1;
-  int64_t fp = dnlg_footprint_bytes(mesh->elem_type, mesh->n_nodes, mesh->n_elems, prm->precision,
-                                    prm->track_tensors);
-  if (fp < 0) return fp;
+  const int type = mesh->elem_type;
+  if (type != DNLG_EL_HEX8N_3D && type != DNLG_EL_QUA4N_AX && type != DNLG_EL_QUA4N_2D) return -1;
+  const int nn = type == DNLG_EL_HEX8N_3D ? 8 : 4;
+  int64_t n_slots = -1;
+  if (mesh->conn && mesh->n_nodes > 0 && mesh->n_elems > 0) {
+    bool in_range = true;
+    for (int64_t k = 0; k < mesh->n_elems * nn && in_range; k++)
+      in_range = mesh->conn[k] >= 0 && mesh->conn[k] < mesh->n_nodes;
+    if (in_range) n_slots = incidence_slots(mesh->conn, mesh->n_nodes, mesh->n_elems, nn, nullptr, nullptr, nullptr);
+  }
+  int64_t fp = footprint(type, mesh->n_nodes, mesh->n_elems, prm->precision, prm->track_tensors, n_slots);
   // beyond the mesh-size estimate: the law's table (tabular point pairs or
   // the packed network) and the constant-BC entry lists
   if (mat->law == DNLG_LAW_TABULAR && mat->table) fp += 2 * mat->n_table * (int64_t)sizeof(double);
@@ -513,7 +558,9 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
     return fail(ctx, DNLG_E_INVALID, "unsupported element type");
   if (mesh->n_nodes <= 0 || mesh->n_elems <= 0 || !mesh->coords || !mesh->conn)
     return fail(ctx, DNLG_E_INVALID, "empty mesh");
-  if (mesh->n_nodes >= (int64_t)1 << 31 || mesh->n_elems * 8 >= (int64_t)1 << 31)
+  // node, integration-point and incidence-slot indices are 32-bit (slots:
+  // at most n_elems nn + n_nodes nn)
+  if ((mesh->n_nodes + mesh->n_elems) * 8 >= (int64_t)1 << 31)
     return fail(ctx, DNLG_E_CAPACITY, "mesh exceeds 32-bit indexing");
   if (prm->dt_method != DNLG_DT_COURANT && prm->dt_method != DNLG_DT_POWER_ITERATION)
     return fail(ctx, DNLG_E_INVALID, "time-step method not implemented (Courant and PowerIteration are)");
@@ -582,20 +629,13 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
     return st;
   };
 
-  // connectivity check; node -> (element, local node) incidence ranges in
-  // element order, and the slot of each (element, local node) in them
+  // connectivity check; the incidence slot of each (element, local node)
   std::vector<int32_t> conn(mesh->conn, mesh->conn + NE * nn);
-  std::vector<int32_t> n2e_ptr(N + 1, 0), e2slot(NE * nn);
-  for (int64_t k = 0; k < NE * nn; k++) {
+  for (int64_t k = 0; k < NE * nn; k++)
     if (conn[k] < 0 || conn[k] >= N) return bail(fail(ctx, DNLG_E_INVALID, "connectivity index out of range"));
-    n2e_ptr[conn[k] + 1]++;
-  }
-  for (int64_t n = 0; n < N; n++) n2e_ptr[n + 1] += n2e_ptr[n];
-  {
-    std::vector<int32_t> fill(n2e_ptr.begin(), n2e_ptr.end() - 1);
-    for (int64_t e = 0; e < NE; e++)
-      for (int a = 0; a < nn; a++) e2slot[e * nn + a] = fill[conn[e * nn + a]]++;
-  }
+  std::vector<int32_t> e2slot, ovf_ptr;
+  int32_t slot_w = 0;
+  const int64_t n_slots = incidence_slots(conn.data(), N, NE, nn, &slot_w, &ovf_ptr, &e2slot);
   // coordinates (ElementAxi::checkLevel2, ElementAxi.C:103-126; the plane
   // element checks nothing, Element2D.C)
   std::vector<double> x(dim * N);
@@ -746,13 +786,17 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
   H.bcn = p32;
   H.n_bcn = job->n_bcn;
   H.bc_flags = bc_flags;
-  ALLOC(p32, N + 1, n2e_ptr.data());
-  H.n2e_ptr = p32;
+  H.slot_w = slot_w;
+  H.ovf_ptr = nullptr;
+  if (!ovf_ptr.empty()) {
+    ALLOC(p32, N + 1, ovf_ptr.data());
+    H.ovf_ptr = p32;
+  }
   ALLOC(p32, NE * nn, e2slot.data());
   H.e2slot = p32;
   ALLOC(p32, NE * nn, conn.data());
   H.conn = p32;
-  ALLOC(H.elemF, NE * nn * dim, (const double *)nullptr);
+  ALLOC(H.elemF, n_slots * dim, (const double *)nullptr);
   job->n_cmin = (int32_t)((NE + 255) / 256);
   H.n_cmin = job->n_cmin;
   ALLOC(H.cmin, (int64_t)job->n_cmin, (const double *)nullptr);
@@ -797,7 +841,7 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
     H.n_ltab = (int32_t)mat->n_table;
   }
   if (is_mixed(prm->precision)) {
-    ALLOC(H.elemF32, NE * nn * dim, (const float *)nullptr);
+    ALLOC(H.elemF32, n_slots * dim, (const float *)nullptr);
     ALLOC(H.sig32, 6 * G, (const float *)nullptr);
     ALLOC(H.dep32, G, (const float *)nullptr);
     ALLOC(H.sy32, G, (const float *)nullptr);

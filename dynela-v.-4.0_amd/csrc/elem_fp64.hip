@@ -72,6 +72,20 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) 
   const Task tk = load_task(tasks);
   JobDev *J = tk.jd;
   CJobDev *C = (CJobDev *)J;
+  // The job-record fields of the control derivation and of the node loads
+  // in one batch of scalar loads ahead of the first branch (the empty asm
+  // takes them as operands, so none is sunk behind a branch to cost a round
+  // trip of its own; cycle_control reads the same constant-space words).
+  const int64_t N = C->N;
+  const DNLG_GLOBAL int32_t *conn = gbl(C->conn);
+  const DNLG_GLOBAL double *xg = gbl(C->x), *fu0 = gbl(C->fu[0]), *fu1 = gbl(C->fu[1]);
+  {
+    const Control c0 = load_control(C, cs);
+    asm volatile("" ::"s"(tk.first), "s"(tk.end), "s"(tk.blk), "s"(c0.t), "s"(c0.dt), "s"(c0.up_to), "s"(c0.inc),
+                 "s"(c0.f0), "s"(c0.running), "s"(c0.pending), "s"(C->sync.st[0]), "s"(C->sync.st[1]),
+                 "s"(C->pi.stall), "s"(C->res.hold), "s"(C->dt_freq), "s"(C->density_always), "s"(N), "s"(conn),
+                 "s"(xg), "s"(fu0), "s"(fu1));
+  }
   const Control cc = cycle_control(J, C, cs, derive, tk.blk);
   if (!cc.running) return;
   const int par = (int)(cc.inc & 1);
@@ -79,16 +93,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) 
   const int p = threadIdx.x % NIP;
   const int64_t e = tk.first + le;
   const bool valid = e < tk.end;
-  const int64_t N = C->N;
   const int f0 = cc.f0;
 
   // stage the element's nodes: lane p brings node p
   if (valid) {
-    const int nd = gbl(C->conn)[e * NN + p];
-    const DNLG_GLOBAL double *u0 = gbl(C->fu[f0]);
+    const int nd = conn[e * NN + p];
+    const DNLG_GLOBAL double *u0 = f0 ? fu1 : fu0;
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
-      s_x[le][p][d] = gbl(C->x)[d * N + nd];
+      s_x[le][p][d] = xg[d * N + nd];
       s_u[le][p][d] = u0[d * N + nd];
     }
   }

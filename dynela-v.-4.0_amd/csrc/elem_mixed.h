@@ -335,6 +335,23 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
   const Task tk = load_task(tasks);
   JobDev *J = tk.jd;
   CJobDev *C = (CJobDev *)J;
+  // Every job-record field of the control derivation and of the loads below
+  // in one batch of scalar loads ahead of the first branch (the empty asm
+  // takes them as operands, so none is sunk behind a branch to cost a round
+  // trip of its own; cycle_control reads the same constant-space words).
+  const int64_t N = C->N, G = C->G;
+  const DNLG_GLOBAL int32_t *conn = gbl(C->conn), *e2slot = gbl(C->e2slot);
+  const DNLG_GLOBAL double *xg = gbl(C->x), *fu0 = gbl(C->fu[0]), *fu1 = gbl(C->fu[1]);
+  const DNLG_GLOBAL float *sig32 = gbl(C->sig32), *sy32 = gbl(C->sy32), *gam32 = gbl(C->gam32);
+  const DNLG_GLOBAL double *Tg = gbl(C->T), *epg = gbl(C->ep);
+  {
+    const Control c0 = load_control(C, ctl);
+    asm volatile("" ::"s"(tk.first), "s"(tk.end), "s"(tk.blk), "s"(c0.t), "s"(c0.dt), "s"(c0.up_to), "s"(c0.inc),
+                 "s"(c0.f0), "s"(c0.running), "s"(c0.pending), "s"(C->sync.st[0]), "s"(C->sync.st[1]),
+                 "s"(C->pi.stall), "s"(C->res.hold), "s"(C->dt_freq), "s"(C->density_always), "s"(N), "s"(G),
+                 "s"(conn), "s"(e2slot), "s"(xg), "s"(fu0), "s"(fu1), "s"(sig32), "s"(sy32), "s"(gam32), "s"(Tg),
+                 "s"(epg));
+  }
   const Control cc = cycle_control(J, C, ctl, derive, tk.blk);
   if (!cc.running) return;
   const int par = (int)(cc.inc & 1);
@@ -346,46 +363,43 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
   const bool valid = e < tk.end;
 
   const int ref = b ^ ((b >> 1) & 1);       // node / IP index in the reference order
-  const int64_t N = C->N;
   const int f0 = cc.f0;
 
-  // ---- node b of the element: coordinates relative to node 0, increment
+  // ---- loads: the element's connectivity, then this lane's IP state (it
+  // does not depend on the connectivity, so its latency overlaps the node
+  // loads and the geometry), then node b.  Idle lanes of a partial block
+  // read the block's first element (in range) and discard the values, so the
+  // loads need no branch, which would make the hardware wait for them there.
+  const int64_t ec = valid ? e : tk.first;
+  const int64_t gi = e * NN + ref;
+  const int64_t gl = ec * NN + ref;
+  const int nd = conn[gl];
+  const int n0 = conn[ec * NN];
+  const int slot = e2slot[gl];  // incidence slot of this lane's node, where its force goes
+  float sig0[6];                // stress, yield stress, gamma of the last cycle
+#pragma unroll
+  for (int c = 0; c < 6; c++) sig0[c] = sig32[c * G + gl];
+  const float sy0 = sy32[gl], gam0 = gam32[gl];
+  const double Tcur = Tg[gl], ep0 = epg[gl];
+  // node b of the element: coordinates relative to node 0, increment
+  const DNLG_GLOBAL double *u0 = f0 ? fu1 : fu0;
   float xl[DIM], ul[DIM];
   double x0r = 0.0;
-  if (valid) {
-    const int nd = gbl(C->conn)[e * NN + ref];
-    const int n0 = gbl(C->conn)[e * NN];
-    const DNLG_GLOBAL double *u0 = gbl(C->fu[f0]);
+  {
+    double xb[DIM], xa[DIM], ub[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
-      xl[d] = (float)(gbl(C->x)[d * N + nd] - gbl(C->x)[d * N + n0]);
-      ul[d] = (float)u0[d * N + nd];
+      xb[d] = xg[d * N + nd];
+      xa[d] = xg[d * N + n0];
+      ub[d] = u0[d * N + nd];
     }
-    if constexpr (AXI) x0r = gbl(C->x)[n0];
-  } else {
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
-      xl[d] = sgn(b, d);  // dummy unit element keeps the idle lanes finite
-      ul[d] = 0.f;
+      // idle lanes: a dummy unit element keeps them finite
+      xl[d] = valid ? (float)(xb[d] - xa[d]) : sgn(b, d);
+      ul[d] = valid ? (float)ub[d] : 0.f;
     }
-  }
-  // ---- integration-point state, issued after the node loads (vector loads
-  // retire in order) so that its latency overlaps the geometry below.  Idle
-  // lanes leave it unset: every use is under `valid`, and a zero written on
-  // the idle path would make the hardware wait for these loads right here.
-  const int64_t G = C->G;
-  const int64_t gi = e * NN + ref;
-  float sig0[6], sy0, gam0;  // stress, yield stress, gamma of the last cycle
-  double Tcur, ep0;
-  int slot;  // incidence slot of this lane's node, where its force goes
-  if (valid) {
-    slot = gbl(C->e2slot)[gi];
-#pragma unroll
-    for (int c = 0; c < 6; c++) sig0[c] = gbl(C->sig32)[c * G + gi];
-    sy0 = gbl(C->sy32)[gi];
-    gam0 = gbl(C->gam32)[gi];
-    Tcur = gbl(C->T)[gi];
-    ep0 = gbl(C->ep)[gi];
+    if constexpr (AXI) x0r = valid ? xa[0] : 0.0;
   }
 
   // ---- Walsh modes of coordinates and increments, scaled by 2^-D

@@ -173,12 +173,17 @@ struct JobDev {
   const int32_t *bcn;     // nodes that carry constant BCs, ascending
   int32_t n_bcn;
   int32_t bc_flags;       // 1: a constant speed BC, 2: a ramped one
-  const int32_t *n2e_ptr;  // [N+1] incidence range of each node, element order
+  // Incidence slots, node-major in element order: node n's first slot_w
+  // incidences are slots n slot_w .. n slot_w + slot_w - 1 (slots past its
+  // incidence count stay zero), further ones (a node with more than slot_w
+  // incident elements) slots N slot_w + ovf_ptr[n] .. N slot_w + ovf_ptr[n+1]
+  // - 1; ovf_ptr is null when no node has more.
+  int32_t slot_w;
+  const int32_t *ovf_ptr;  // [N+1] or null
   const int32_t *e2slot;   // [NE*nn] incidence slot of (element, local node)
   // elements
   const int32_t *conn;  // [NE][nn]
-  double *elemF;        // [slot][dim] element internal forces, node-major:
-                        // node n's incident forces are slots n2e_ptr[n]..n2e_ptr[n+1]
+  double *elemF;        // [slot][dim] element internal forces (per incidence slot)
   float *elemF32;       // mixed-precision variant of elemF
   double *cmin;         // Courant partial minima, one per Courant block
   int32_t n_cmin;

@@ -76,7 +76,7 @@ __device__ __forceinline__ void apply_bcs(CJobDev *C, int64_t n, double time, do
 // Explicit::computePredictions + Explicit::explicitSolve (Explicit.C:438-536)
 // in one pass per node: the predicted field1 values are formed from field0
 // in registers (they are read nowhere else), the element forces of the node
-// are read from its contiguous incidence slots in element order (same
+// are read from its incidence slots in element order (same
 // summation order as the reference's scatter-add of Model.C:864-899), a = f/M,
 // Chung-Hulbert correction, BCs, coordinates (the node displacement of
 // Explicit.C:530-531 is x - x0, formed at read-back).
@@ -98,8 +98,45 @@ __device__ __forceinline__ void add_slots(const DNLG_GLOBAL FT *ef, int64_t k, d
     for (int d = 0; d < DIM; d++) f[d] += (double)v[i][d];
 }
 
+// Adds node n's incident forces to f in element order: its slot_w fixed
+// slots (those past its incidence count hold zeros, which change no sum),
+// then its overflow slots, if any.  The fixed slots' addresses need no
+// index load, so their loads issue with the node's own values.  MAXB: the
+// slots loaded together.
+template <int MAXB, int DIM, typename FT>
+__device__ __forceinline__ void add_node_slots(const DNLG_GLOBAL FT *ef, int W, const DNLG_GLOBAL int32_t *ovf,
+                                               int64_t N, int64_t n, double f[DIM]) {
+  const int64_t k0 = n * W;
+  int k = 0;
+  for (; k + MAXB <= W; k += MAXB) add_slots<MAXB, DIM>(ef, k0 + k, f);
+  if constexpr (MAXB > 4) {
+    if (k + 4 <= W) {
+      add_slots<4, DIM>(ef, k0 + k, f);
+      k += 4;
+    }
+  }
+  if constexpr (MAXB > 2) {
+    if (k + 2 <= W) {
+      add_slots<2, DIM>(ef, k0 + k, f);
+      k += 2;
+    }
+  }
+  if (k < W) add_slots<1, DIM>(ef, k0 + k, f);
+  if (ovf) {
+    const int64_t o1 = N * W + ovf[n + 1];
+    for (int64_t o = N * W + ovf[n]; o < o1; o++) add_slots<1, DIM>(ef, o, f);
+  }
+}
+
 // MAXB: the incidence slots loaded together (a schedule parameter; the
 // sums run in slot order whatever the batching)
+// Latency: with one node per lane the launch has about two waves per SIMD,
+// so a wave's chain of dependent loads is the kernel's duration.  The task,
+// the control record and every job-record field the node update needs are
+// read as one batch of scalar loads ahead of the first branch (the empty asm
+// takes them all as operands, so none of them is sunk behind a branch, where
+// each would cost a round trip of its own), and the node's own values --
+// coordinates included -- are loaded before the incidence slots.
 template <int DIM, typename FT, int MAXB>
 __global__ __launch_bounds__(kNodeBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_correct(
     JobDev *const *jobs, const Task *tasks, int cs, int fold) {
@@ -108,42 +145,42 @@ __global__ __launch_bounds__(kNodeBlock) __attribute__((amdgpu_waves_per_eu(4)))
   CJobDev *C = (CJobDev *)J;
   const Control c = load_control(C, cs);
   const int st0 = C->sync.st[0], st1 = C->sync.st[1];
+  const int64_t N = C->N;
+  const int W = C->slot_w;
+  const DNLG_GLOBAL int32_t *ovf = gbl(C->ovf_ptr);
+  const DNLG_GLOBAL int32_t *bc_ptr = gbl(C->bc_ptr);
+  const DNLG_GLOBAL double *mass = gbl(C->mass);
+  DNLG_GLOBAL double *x = gbl(C->x);
+  DNLG_GLOBAL double *fv0 = gbl(C->fv[0]), *fv1 = gbl(C->fv[1]);
+  DNLG_GLOBAL double *fa0 = gbl(C->fa[0]), *fa1 = gbl(C->fa[1]);
+  const DNLG_GLOBAL FT *ef;
+  if constexpr (sizeof(FT) == 8) ef = gbl(C->elemF); else ef = gbl(C->elemF32);
+  const double alphaM = C->alphaM, gam = C->gammaCH, beta = C->beta;
+  asm volatile("" ::"s"(tk.first), "s"(tk.end), "s"(c.dt), "s"(c.inc), "s"(c.f0), "s"(c.running), "s"(st0),
+               "s"(st1), "s"(N), "s"(W), "s"(ovf), "s"(bc_ptr), "s"(mass), "s"(x), "s"(fv0), "s"(fv1), "s"(fa0),
+               "s"(fa1), "s"(ef), "s"(alphaM), "s"(gam), "s"(beta));
   if (!c.running || ((c.inc & 1) ? st1 : st0) != 0) return;
   const int64_t n = tk.first + threadIdx.x;
   if (n >= tk.end) return;
-  const int64_t N = C->N;
   const int a = c.f0, b = 1 - a;
   const double h = c.dt;
-  const DNLG_GLOBAL FT *ef;
-  if constexpr (sizeof(FT) == 8) ef = gbl(C->elemF); else ef = gbl(C->elemF32);
-  const int k0 = gbl(C->n2e_ptr)[n], k1 = gbl(C->n2e_ptr)[n + 1];
-  // the node's own values first, then the incidence slots in batches
-  const double m = gbl(C->mass)[n];
-  double va[DIM], aa[DIM];
+  DNLG_GLOBAL double *fva = a ? fv1 : fv0, *faa = a ? fa1 : fa0;
+  DNLG_GLOBAL double *fvb = a ? fv0 : fv1, *fab = a ? fa0 : fa1;
+  // the node's own values, then the incidence slots in batches (the fixed
+  // slots' loads issue with these)
+  const bool has_bc = bc_ptr[n + 1] > bc_ptr[n];
+  const double m = mass[n];
+  double va[DIM], aa[DIM], xo[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; d++) {
-    va[d] = gbl(C->fv[a])[d * N + n];
-    aa[d] = gbl(C->fa[a])[d * N + n];
+    va[d] = fva[d * N + n];
+    aa[d] = faa[d * N + n];
+    xo[d] = x[d * N + n];
   }
   double f[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; d++) f[d] = 0.0;
-  int64_t k = k0;
-  for (; k + MAXB <= k1; k += MAXB) add_slots<MAXB, DIM>(ef, k, f);
-  if constexpr (MAXB > 4) {
-    if (k + 4 <= k1) {
-      add_slots<4, DIM>(ef, k, f);
-      k += 4;
-    }
-  }
-  if constexpr (MAXB > 2) {
-    if (k + 2 <= k1) {
-      add_slots<2, DIM>(ef, k, f);
-      k += 2;
-    }
-  }
-  if (k < k1) add_slots<1, DIM>(ef, k, f);
-  const double alphaM = C->alphaM, gam = C->gammaCH, beta = C->beta;
+  add_node_slots<MAXB, DIM>(ef, W, ovf, N, n, f);
   double ua[DIM], ub[DIM], vb[DIM], ab[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; d++) {
@@ -157,23 +194,23 @@ __global__ __launch_bounds__(kNodeBlock) __attribute__((amdgpu_waves_per_eu(4)))
     vb[d] += gam * h * ab[d];
     ub[d] += beta * (h * h) * ab[d];
   }
-  const bool has_bc = gbl(C->bc_ptr)[n + 1] > gbl(C->bc_ptr)[n];
   if (has_bc) {
+    DNLG_GLOBAL double *fua = gbl(C->fu[a]);
 #pragma unroll
-    for (int d = 0; d < DIM; d++) ua[d] = gbl(C->fu[a])[d * N + n];
+    for (int d = 0; d < DIM; d++) ua[d] = fua[d * N + n];
     apply_bcs<DIM>(C, n, c.t, h, ua, va, aa, ub, vb, ab);
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
-      gbl(C->fu[a])[d * N + n] = ua[d];
-      gbl(C->fv[a])[d * N + n] = va[d];
-      gbl(C->fa[a])[d * N + n] = aa[d];
+      fua[d * N + n] = ua[d];
+      fva[d * N + n] = va[d];
+      faa[d * N + n] = aa[d];
     }
   }
   bool finite = true;
 #pragma unroll
   for (int d = 0; d < DIM; d++) {
-    const double xn = gbl(C->x)[d * N + n] + ub[d];
-    gbl(C->x)[d * N + n] = xn;
+    const double xn = xo[d] + ub[d];
+    x[d * N + n] = xn;
     finite = finite && isfinite(xn) && isfinite(vb[d]);
   }
   if (has_bc && fold) {
@@ -185,11 +222,12 @@ __global__ __launch_bounds__(kNodeBlock) __attribute__((amdgpu_waves_per_eu(4)))
       apply_bcs<DIM>(C, n, nx.t, nx.dt, ub, vb, ab, u2, v2, a2);
     }
   }
+  DNLG_GLOBAL double *fub = gbl(C->fu[b]);
 #pragma unroll
   for (int d = 0; d < DIM; d++) {
-    gbl(C->fu[b])[d * N + n] = ub[d];
-    gbl(C->fv[b])[d * N + n] = vb[d];
-    gbl(C->fa[b])[d * N + n] = ab[d];
+    fub[d * N + n] = ub[d];
+    fvb[d * N + n] = vb[d];
+    fab[d * N + n] = ab[d];
   }
   if (!finite) set_status(J, DNLG_E_NONFINITE, (int)(c.inc & 1));
 }
@@ -576,15 +614,7 @@ __global__ __launch_bounds__(kNodeBlock) void k_pi_node(JobDev *const *jobs, con
     double acc[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; d++) acc[d] = src[d * N + n] / scale;
-    const DNLG_GLOBAL double *ef = gbl(C->elemF);
-    const int64_t k1 = gbl(C->n2e_ptr)[n + 1];
-    int64_t k = gbl(C->n2e_ptr)[n];
-    for (; k + 4 <= k1; k += 4) add_slots<4, DIM>(ef, k, acc);
-    if (k + 2 <= k1) {
-      add_slots<2, DIM>(ef, k, acc);
-      k += 2;
-    }
-    if (k < k1) add_slots<1, DIM>(ef, k, acc);
+    add_node_slots<4, DIM>(gbl(C->elemF), C->slot_w, gbl(C->ovf_ptr), N, n, acc);
     const double m = gbl(C->mass)[n];
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
@@ -772,7 +802,10 @@ __global__ __launch_bounds__(kNodeBlock) void k_mass_gather(JobDev *const *jobs,
   const int64_t n = tk.first + threadIdx.x;
   if (n >= tk.end) return;
   double m = 0.0;
-  for (int64_t k = J->n2e_ptr[n]; k < J->n2e_ptr[n + 1]; k++) m += J->elemF[k * DIM];
+  const int W = J->slot_w;
+  for (int k = 0; k < W; k++) m += J->elemF[(n * W + k) * DIM];
+  if (J->ovf_ptr)
+    for (int64_t o = J->N * W + J->ovf_ptr[n]; o < J->N * W + J->ovf_ptr[n + 1]; o++) m += J->elemF[o * DIM];
   J->mass[n] = m;
 }
 
