@@ -370,17 +370,18 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
   // loads and the geometry), then node b.  Idle lanes of a partial block
   // read the block's first element (in range) and discard the values, so the
   // loads need no branch, which would make the hardware wait for them there.
-  const int64_t ec = valid ? e : tk.first;
-  const int64_t gi = e * NN + ref;
-  const int64_t gl = ec * NN + ref;
-  const int nd = conn[gl];
-  const int n0 = conn[ec * NN];
-  const int slot = e2slot[gl];  // incidence slot of this lane's node, where its force goes
-  float sig0[6];                // stress, yield stress, gamma of the last cycle
+  // Per-IP arrays are addressed from the block's first IP (uniform) with a
+  // lane-local 32-bit byte offset: ip = this lane's IP within the block.
+  const int64_t g0 = tk.first * NN;
+  const uint32_t ip = (uint32_t)((valid ? le * NN : 0) + ref), e0 = (uint32_t)(valid ? le * NN : 0);
+  const int nd = at(conn + g0, 4 * ip);
+  const int n0 = at(conn + g0, 4 * e0);
+  const int slot = at(e2slot + g0, 4 * ip);  // incidence slot of this lane's node, where its force goes
+  float sig0[6];                             // stress, yield stress, gamma of the last cycle
 #pragma unroll
-  for (int c = 0; c < 6; c++) sig0[c] = sig32[c * G + gl];
-  const float sy0 = sy32[gl], gam0 = gam32[gl];
-  const double Tcur = Tg[gl], ep0 = epg[gl];
+  for (int c = 0; c < 6; c++) sig0[c] = at(sig32 + c * G + g0, 4 * ip);
+  const float sy0 = at(sy32 + g0, 4 * ip), gam0 = at(gam32 + g0, 4 * ip);
+  const double Tcur = at(Tg + g0, 8 * ip), ep0 = at(epg + g0, 8 * ip);
   // node b of the element: coordinates relative to node 0, increment
   const DNLG_GLOBAL double *u0 = f0 ? fu1 : fu0;
   float xl[DIM], ul[DIM];
@@ -389,9 +390,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
     double xb[DIM], xa[DIM], ub[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
-      xb[d] = xg[d * N + nd];
-      xa[d] = xg[d * N + n0];
-      ub[d] = u0[d * N + nd];
+      xb[d] = at(xg + d * N, 8 * (uint32_t)nd);
+      xa[d] = at(xg + d * N, 8 * (uint32_t)n0);
+      ub[d] = at(u0 + d * N, 8 * (uint32_t)nd);
     }
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
@@ -548,9 +549,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
     rotate_sym_f32(R, Rs);
     // Element::computeDensity (Element.C:1243-1256)
     if (cc.density_due) {
-      double rr = C->rho0 * (double)gbl(C->detJ0f)[gi] / (double)detJ;
+      double rr = C->rho0 * (double)at(gbl(C->detJ0f) + g0, 4 * ip) / (double)detJ;
       if constexpr (AXI) rr /= (double)radius;
-      gbl(C->rho)[gi] = rr;
+      at(gbl(C->rho) + g0, 8 * ip) = rr;
     }
     // trial test and radial return (the yield stress is stored in FP32)
     const float Strialf = (float)kSqrt32 * Snorm;
@@ -563,7 +564,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
     for (int k = 0; k < 8; k++) lp[k] = C->lp[k];
     if (yieldf == 0.f) {
       yieldf = (float)law_yield(law, lp, kGammaInitial, kGammaInitial / dt, Tcur);
-      gbl(C->sy32)[gi] = yieldf;
+      at(gbl(C->sy32) + g0, 4 * ip) = yieldf;
     }
     if (Strialf > yieldf) {
       // radial return (Element.C:485-601): Newton on gamma in FP32, then
@@ -603,18 +604,18 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
 #pragma unroll
       for (int c = 0; c < 6; c++) Rs[c] = s_keep[3 + c][threadIdx.x];
       scale = 1.f - TwoGf * g * __builtin_amdgcn_rcpf(Snorm);
-      gbl(C->ep)[gi] = ep0 + kSqrt23 * gamma;
-      gbl(C->dep32)[gi] = (float)(kSqrt23 * gamma / dt);
-      gbl(C->gam32)[gi] = g;
-      gbl(C->sy32)[gi] = yieldf;
+      at(gbl(C->ep) + g0, 8 * ip) = ep0 + kSqrt23 * gamma;
+      at(gbl(C->dep32) + g0, 4 * ip) = (float)(kSqrt23 * gamma / dt);
+      at(gbl(C->gam32) + g0, 4 * ip) = g;
+      at(gbl(C->sy32) + g0, 4 * ip) = yieldf;
     }
     const float power = 0.5f * (work_a + scale * work_s);
-    gbl(C->eint)[gi] += (double)power * C->inv_rho0;
+    at(gbl(C->eint) + g0, 8 * ip) += (double)power * C->inv_rho0;
     if (gamma != 0.0) {
       const float sn = scale * Snorm;  // |scale s|
       const double w = 0.5 * gamma * ((double)sn + (double)Snorm0);
-      gbl(C->einel)[gi] += w * C->inv_rho0;
-      gbl(C->T)[gi] = Tcur + heatFrac * w;
+      at(gbl(C->einel) + g0, 8 * ip) += w * C->inv_rho0;
+      at(gbl(C->T) + g0, 8 * ip) = Tcur + heatFrac * w;
     }
 #pragma unroll
     for (int c = 0; c < 6; c++) sig[c] = scale * Rs[c];
@@ -622,7 +623,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
     sig[3] += pressure;
     sig[5] += pressure;
 #pragma unroll
-    for (int c = 0; c < 6; c++) gbl(C->sig32)[c * G + gi] = sig[c];
+    for (int c = 0; c < 6; c++) at(gbl(C->sig32) + c * G + g0, 4 * ip) = sig[c];
   } else {
 #pragma unroll
     for (int c = 0; c < 6; c++) sig[c] = 0.f;

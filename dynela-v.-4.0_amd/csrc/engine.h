@@ -229,6 +229,20 @@ __device__ __forceinline__ DNLG_GLOBAL T *gbl(T *p) {
   return (DNLG_GLOBAL T *)p;
 }
 
+// Reference at a uniform base plus a 32-bit byte offset: the global
+// instructions' scalar-base + 32-bit vector-offset form, so that a lane
+// forms no 64-bit address of its own.  Kernels give the base the block's
+// first item, which keeps the offsets of per-item arrays lane-local.
+// The base passes through readfirstlane (it is uniform), which keeps the
+// compiler from folding uniform and per-lane terms into one 64-bit
+// per-lane address.
+template <typename T>
+__device__ __forceinline__ DNLG_GLOBAL T &at(const DNLG_GLOBAL T *base, uint32_t byte_off) {
+  const uint64_t b = (uint64_t)base;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b), hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
+  return *(DNLG_GLOBAL T *)((DNLG_GLOBAL char *)(((uint64_t)hi << 32) | lo) + byte_off);
+}
+
 // Threads per block of the node kernels (corrector, mass assembly): one node
 // per thread, contiguous node ranges per block.
 constexpr int kNodeBlock = 256;
