@@ -413,11 +413,29 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
     Mx[d] = inv2D * wht<NB>(xl[d], cs);
     Mu[d] = inv2D * wht<NB>(ul[d], cs);
   }
-  // modes of the group to every lane of the element: mx[q][d], mu[q][d]
-  constexpr int MR = (2 * DIM + 3) / 4;  // float4 per lane row
+  // LDS, one region per wave (the waves of a block are not synchronised):
+  // the inverse Jacobian and det J of each lane, kept from the geometry to
+  // the forces, then one area that the three phases take in turn -- the
+  // element modes (geometry), the values held across the radial return,
+  // the transformed force terms (forces).  A wave's LDS accesses execute in
+  // order, so a phase's writes follow the previous phase's reads.
+  constexpr int MR = (2 * DIM + 3) / 4;  // float4 per lane row of modes
+  constexpr int FR = DIM == 3 ? 3 : 2;   // float4 per lane row of force terms
   typedef LaneRows<NN, MR> ModeRows;
-  __shared__ float4 s_mode[(BLOCK / NN) * ModeRows::kGroup];
-  __shared__ float s_keep[9][BLOCK];  // values held across the radial return
+  typedef LaneRows<NN, FR> ForceRows;
+  constexpr int EPW = 64 / NN;           // elements per wave
+  constexpr int NJ = DIM * DIM + 1;      // J^-1 entries and det J
+  constexpr int kModeF4 = EPW * ModeRows::kGroup, kKeepF4 = 9 * 16, kWtF4 = EPW * ForceRows::kGroup;
+  constexpr int kPhaseF4 = kModeF4 > kKeepF4 ? (kModeF4 > kWtF4 ? kModeF4 : kWtF4) : (kKeepF4 > kWtF4 ? kKeepF4 : kWtF4);
+  constexpr int kWaveF4 = NJ * 16 + kPhaseF4;
+  __shared__ float4 s_lds[(BLOCK / 64) * kWaveF4];
+  float4 *const s_wave = s_lds + (threadIdx.x >> 6) * kWaveF4;
+  float *const s_jinv = (float *)s_wave;               // [NJ][64]
+  float4 *const s_mode = s_wave + NJ * 16;             // geometry
+  float *const s_keep = (float *)(s_wave + NJ * 16);   // radial return: [9][64]
+  float4 *const s_wt = s_wave + NJ * 16;               // forces
+  const int lw = lane / NN;                            // element within the wave
+  // modes of the group to every lane of the element: mx[q][d], mu[q][d]
   {
     float row[4 * MR];
 #pragma unroll
@@ -429,7 +447,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
     }
 #pragma unroll
     for (int v = 0; v < MR; v++)
-      s_mode[ModeRows::at(le, b, v)] = make_float4(row[4 * v], row[4 * v + 1], row[4 * v + 2], row[4 * v + 3]);
+      s_mode[ModeRows::at(lw, b, v)] = make_float4(row[4 * v], row[4 * v + 1], row[4 * v + 2], row[4 * v + 3]);
   }
   __builtin_amdgcn_wave_barrier();
   float mx[NN][DIM], mu[NN][DIM];
@@ -438,7 +456,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
     float row[4 * MR];
 #pragma unroll
     for (int v = 0; v < MR; v++) {
-      const float4 t = s_mode[ModeRows::at(le, q, v)];
+      const float4 t = s_mode[ModeRows::at(lw, q, v)];
       row[4 * v] = t.x;
       row[4 * v + 1] = t.y;
       row[4 * v + 2] = t.z;
@@ -458,6 +476,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
   // ---- Jacobian, its inverse, and G_ij = du_i/dxi_j
   float detJ, inv[DIM][DIM];
   jacobian_inverse<DIM>(mx, xi, detJ, inv);
+#pragma unroll
+  for (int k = 0; k < DIM; k++)
+#pragma unroll
+    for (int j = 0; j < DIM; j++) s_jinv[(k * DIM + j) * 64 + lane] = inv[k][j];
+  s_jinv[DIM * DIM * 64 + lane] = detJ;
   float Gm[DIM][DIM];
 #pragma unroll
   for (int i = 0; i < DIM; i++) {
@@ -573,11 +596,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
       // the return waits in LDS meanwhile (the return's FP64 law evaluation
       // is the kernel's register peak); the empty asm keeps the compiler
       // from holding the values in registers across it.
-      s_keep[0][threadIdx.x] = work_a;
-      s_keep[1][threadIdx.x] = work_s;
-      s_keep[2][threadIdx.x] = pressure;
+      s_keep[0 * 64 + lane] = work_a;
+      s_keep[1 * 64 + lane] = work_s;
+      s_keep[2 * 64 + lane] = pressure;
 #pragma unroll
-      for (int c = 0; c < 6; c++) s_keep[3 + c][threadIdx.x] = Rs[c];
+      for (int c = 0; c < 6; c++) s_keep[(3 + c) * 64 + lane] = Rs[c];
       asm volatile("" ::: "memory");
       float pf[8];
 #pragma unroll
@@ -598,11 +621,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
       }
       if (!ok) set_status(J, DNLG_E_NR_DIVERGED, par);
       asm volatile("" ::: "memory");
-      work_a = s_keep[0][threadIdx.x];
-      work_s = s_keep[1][threadIdx.x];
-      pressure = s_keep[2][threadIdx.x];
+      work_a = s_keep[0 * 64 + lane];
+      work_s = s_keep[1 * 64 + lane];
+      pressure = s_keep[2 * 64 + lane];
 #pragma unroll
-      for (int c = 0; c < 6; c++) Rs[c] = s_keep[3 + c][threadIdx.x];
+      for (int c = 0; c < 6; c++) Rs[c] = s_keep[(3 + c) * 64 + lane];
       scale = 1.f - TwoGf * g * __builtin_amdgcn_rcpf(Snorm);
       at(gbl(C->ep) + g0, 8 * ip) = ep0 + kSqrt23 * gamma;
       at(gbl(C->dep32) + g0, 4 * ip) = (float)(kSqrt23 * gamma / dt);
@@ -629,23 +652,16 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
     for (int c = 0; c < 6; c++) sig[c] = 0.f;
   }
   // ---- internal forces (Element.C:362-404): T = w J^-1 sigma per Gauss
-  // point, with J^-1 formed again from the coordinate modes in LDS (it is
-  // not kept through the radial return); the empty asm keeps the compiler
-  // from reusing the first reads of the modes
+  // point, J^-1 and det J read back from LDS (they are not kept in
+  // registers through the radial return); the empty asm keeps the compiler
+  // from holding them in registers instead
   asm volatile("" ::: "memory");
-  float detJf, invf[DIM][DIM];
-  {
-    float mxr[NN][DIM];
+  float invf[DIM][DIM];
 #pragma unroll
-    for (int q = 0; q < NN; q++) {
-      const float4 t = s_mode[ModeRows::at(le, q, 0)];
-      mxr[q][0] = t.x;
-      mxr[q][1] = t.y;
-      if constexpr (DIM == 3) mxr[q][2] = t.z;
-    }
-    jacobian_inverse<DIM>(mxr, xi, detJf, invf);
-  }
-  float w = detJf;
+  for (int k = 0; k < DIM; k++)
+#pragma unroll
+    for (int j = 0; j < DIM; j++) invf[k][j] = s_jinv[(k * DIM + j) * 64 + lane];
+  float w = s_jinv[DIM * DIM * 64 + lane];
   if constexpr (AXI) w *= (float)k2PI * radius;
 #pragma unroll
   for (int k = 0; k < DIM; k++)
@@ -667,15 +683,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
   for (int k = 0; k < DIM; k++)
 #pragma unroll
     for (int i = 0; i < DIM; i++) WT[k][i] = wht<NB>(T9[k][i], cs);
-  constexpr int FR = DIM == 3 ? 3 : 2;  // float4 per lane row
-  typedef LaneRows<NN, FR> ForceRows;
-  __shared__ float4 s_wt[(BLOCK / NN) * ForceRows::kGroup];
   if constexpr (DIM == 3) {
 #pragma unroll
-    for (int k = 0; k < 3; k++) s_wt[ForceRows::at(le, b, k)] = make_float4(WT[k][0], WT[k][1], WT[k][2], 0.f);
+    for (int k = 0; k < 3; k++) s_wt[ForceRows::at(lw, b, k)] = make_float4(WT[k][0], WT[k][1], WT[k][2], 0.f);
   } else {
-    s_wt[ForceRows::at(le, b, 0)] = make_float4(WT[0][0], WT[0][1], WT[1][0], WT[1][1]);
-    if constexpr (AXI) s_wt[ForceRows::at(le, b, 1)] = make_float4(wht<NB>(hoop, cs), 0.f, 0.f, 0.f);
+    s_wt[ForceRows::at(lw, b, 0)] = make_float4(WT[0][0], WT[0][1], WT[1][0], WT[1][1]);
+    if constexpr (AXI) s_wt[ForceRows::at(lw, b, 1)] = make_float4(wht<NB>(hoop, cs), 0.f, 0.f, 0.f);
   }
   __builtin_amdgcn_wave_barrier();
   float f[DIM];
@@ -687,18 +700,18 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
     if constexpr (DIM == 3) {
       const int l = (k + 1) % 3, m = (k + 2) % 3;
       const float cl = g * sgn(b, l), cm = g * sgn(b, m), clm = cl * cm;
-      const float4 a0 = s_wt[ForceRows::at(le, 0, k)];
-      const float4 al = s_wt[ForceRows::at(le, 1 << l, k)];
-      const float4 am = s_wt[ForceRows::at(le, 1 << m, k)];
-      const float4 alm = s_wt[ForceRows::at(le, (1 << l) + (1 << m), k)];
+      const float4 a0 = s_wt[ForceRows::at(lw, 0, k)];
+      const float4 al = s_wt[ForceRows::at(lw, 1 << l, k)];
+      const float4 am = s_wt[ForceRows::at(lw, 1 << m, k)];
+      const float4 alm = s_wt[ForceRows::at(lw, (1 << l) + (1 << m), k)];
       f[0] -= sk * (a0.x + cl * al.x + cm * am.x + clm * alm.x);
       f[1] -= sk * (a0.y + cl * al.y + cm * am.y + clm * alm.y);
       f[2] -= sk * (a0.z + cl * al.z + cm * am.z + clm * alm.z);
     } else {
       const int l = 1 - k;
       const float cl = g * sgn(b, l);
-      const float4 a0 = s_wt[ForceRows::at(le, 0, 0)];
-      const float4 al = s_wt[ForceRows::at(le, 1 << l, 0)];
+      const float4 a0 = s_wt[ForceRows::at(lw, 0, 0)];
+      const float4 al = s_wt[ForceRows::at(lw, 1 << l, 0)];
       if (k == 0) {
         f[0] -= sk * (a0.x + cl * al.x);
         f[1] -= sk * (a0.y + cl * al.y);
@@ -710,8 +723,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
   }
   if constexpr (AXI) {
     // hoop term: f_r -= sum_p N_a(p) w_p sigma_tt / r_p
-    const float h0 = s_wt[ForceRows::at(le, 0, 1)].x, h1 = s_wt[ForceRows::at(le, 1, 1)].x;
-    const float h2 = s_wt[ForceRows::at(le, 2, 1)].x, h3 = s_wt[ForceRows::at(le, 3, 1)].x;
+    const float h0 = s_wt[ForceRows::at(lw, 0, 1)].x, h1 = s_wt[ForceRows::at(lw, 1, 1)].x;
+    const float h2 = s_wt[ForceRows::at(lw, 2, 1)].x, h3 = s_wt[ForceRows::at(lw, 3, 1)].x;
     const float c0 = g * sgn(b, 0), c1 = g * sgn(b, 1);
     f[0] -= h0 + c0 * h1 + c1 * h2 + c0 * c1 * h3;
   }
