@@ -566,8 +566,8 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
     return fail(ctx, DNLG_E_INVALID, "time-step method not implemented (Courant and PowerIteration are)");
   if (prm->stress_method != DNLG_STRESS_NR && prm->stress_method != DNLG_STRESS_DIRECT)
     return fail(ctx, DNLG_E_INVALID, "unknown stress integration method");
-  if (prm->stress_method == DNLG_STRESS_DIRECT && prm->precision != DNLG_PREC_FP64)
-    return fail(ctx, DNLG_E_INVALID, "the Direct stress integration runs in FP64 precision only");
+  if (prm->stress_method == DNLG_STRESS_DIRECT && prm->precision == DNLG_PREC_MIXED_RR32)
+    return fail(ctx, DNLG_E_INVALID, "the Direct stress integration has no FP32 form (mixed-rr32)");
   if (prm->dt_freq <= 0 || !(prm->safety > 0) || prm->rho_b < 0.0 || prm->rho_b > 1.0 || prm->nr_max_iter < 0)
     return fail(ctx, DNLG_E_INVALID, "invalid solver parameters");
   if (prm->precision != DNLG_PREC_FP64 && prm->precision != DNLG_PREC_MIXED &&
@@ -591,8 +591,8 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
       if (!(tail[5 + k] != tail[1 + k]))
         return fail(ctx, DNLG_E_INVALID, "network law scaling ranges must not be empty");
     if (!(tail[0] > 0)) return fail(ctx, DNLG_E_INVALID, "network law logBase must be > 0");
-    if (prm->precision != DNLG_PREC_FP64)
-      return fail(ctx, DNLG_E_INVALID, "the network law runs in FP64 precision only");
+    if (prm->precision == DNLG_PREC_MIXED_RR32)
+      return fail(ctx, DNLG_E_INVALID, "the network law has no FP32 radial return (mixed-rr32)");
   }
   if (mat->law == DNLG_LAW_TABULAR) {
     // DiscreteFunction with at least one segment, strains strictly ascending
@@ -602,8 +602,8 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
       if (!std::isfinite(mat->table[2 * k]) || !std::isfinite(mat->table[2 * k + 1]) ||
           (k > 0 && !(mat->table[2 * k] > mat->table[2 * k - 2])))
         return fail(ctx, DNLG_E_INVALID, "tabular law points must be finite with ascending strains");
-    if (prm->precision != DNLG_PREC_FP64)
-      return fail(ctx, DNLG_E_INVALID, "the tabular law runs in FP64 precision only");
+    if (prm->precision == DNLG_PREC_MIXED_RR32)
+      return fail(ctx, DNLG_E_INVALID, "the tabular law has no FP32 radial return (mixed-rr32)");
   }
   if (!(mat->density > 0) || !(mat->young > 0) || !(mat->heat_capacity > 0))
     return fail(ctx, DNLG_E_INVALID, "material needs density, Young modulus and heat capacity > 0");

@@ -99,7 +99,7 @@ extern "C" {
 int dnlg_schedule_valid(int elem_type, int precision, int law, const dnlg_schedule *sched) {
   if (!sched) return 0;
   if (elem_type != DNLG_EL_HEX8N_3D && elem_type != DNLG_EL_QUA4N_AX && elem_type != DNLG_EL_QUA4N_2D) return 0;
-  const int v = precision == DNLG_PREC_MIXED ? kVarMixed
+  const int v = precision == DNLG_PREC_MIXED ? (law == DNLG_LAW_TABULAR || law == DNLG_LAW_ANN2SIG ? kVarMixedX : kVarMixed)
                 : precision == DNLG_PREC_MIXED_RR32 ? kVarMixedRR32
                 : (law == DNLG_LAW_ANN2SIG ? kVarFP64Ann : kVarFP64);
   dnlg_schedule s = *sched;
@@ -134,7 +134,7 @@ int dnlg_ctx_clear_schedules(dnlg_ctx *ctx) {
 int dnlg_job_set_schedule(dnlg_job *job, const dnlg_schedule *sched) {
   if (!job || !sched) return DNLG_E_INVALID;
   const dnlg_schedule s = normalised(job, *sched);
-  if (!schedule_valid(job->type, job->prec, job->host.law, s)) {
+  if (!schedule_valid(job->type, job->prec, job->host.law, s, job->host.stress_method)) {
     job->ctx->err = "dnlg_job_set_schedule: the kernels are not built for this schedule";
     return DNLG_E_INVALID;
   }
@@ -171,7 +171,7 @@ int dnlg_job_autotune(dnlg_job *job, int64_t cycles_per_trial, int rounds, dnlg_
         dnlg_schedule s = cur;
         s.elem_block = bl[b];
         s.waves = w;
-        if (schedule_valid(job->type, job->prec, job->host.law, s)) cand.push_back(s);
+        if (schedule_valid(job->type, job->prec, job->host.law, s, job->host.stress_method)) cand.push_back(s);
       }
     if (cand.size() > 1 && (st = pick(job, cand, cycles_per_trial, rounds, &cur, &ms, &start, &start_ms))) return st;
   }
@@ -181,7 +181,7 @@ int dnlg_job_autotune(dnlg_job *job, int64_t cycles_per_trial, int rounds, dnlg_
     for (int b : {2, 4, 8}) {
       dnlg_schedule s = cur;
       s.corr_batch = b;
-      if (schedule_valid(job->type, job->prec, job->host.law, s)) cand.push_back(s);
+      if (schedule_valid(job->type, job->prec, job->host.law, s, job->host.stress_method)) cand.push_back(s);
     }
     if ((st = pick(job, cand, cycles_per_trial, rounds, &cur, &ms, start_ms < 0 ? &start : nullptr,
                    start_ms < 0 ? &start_ms : nullptr)))

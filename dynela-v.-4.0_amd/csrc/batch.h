@@ -100,7 +100,7 @@ namespace dnlg {
 // take the defaults.
 dnlg_schedule resolve_schedule(const dnlg_ctx *ctx, const dnlg_job *job);
 int job_variant(const dnlg_job *job);
-bool schedule_valid(int type, int precision, int law, const dnlg_schedule &s);
+bool schedule_valid(int type, int precision, int law, const dnlg_schedule &s, int stress_method = DNLG_STRESS_NR);
 
 // Advance the jobs together toward up_to[i] (clipped to each job's end
 // time); with max_cycles >= 0, return after about that many cycles (whole

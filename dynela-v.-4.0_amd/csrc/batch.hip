@@ -29,14 +29,14 @@ constexpr int kTypes[kNumTypes] = {DNLG_EL_QUA4N_AX, DNLG_EL_HEX8N_3D, DNLG_EL_Q
 inline int type_dim(int type) { return type == DNLG_EL_HEX8N_3D ? 3 : 2; }
 inline int type_nodes(int type) { return type == DNLG_EL_HEX8N_3D ? 8 : 4; }
 
-inline bool variant_mixed(int mx) { return mx == kVarMixed || mx == kVarMixedRR32; }
+inline bool variant_mixed(int mx) { return mx == kVarMixed || mx == kVarMixedRR32 || mx == kVarMixedX; }
 
 // One element launch of variant mx with schedule (block, waves) over a task
 // list.
 void launch_variant(int type, int mx, int block, int waves, JobDev *const *jobs, const Task *tasks, int nblk, int cs,
                     int derive, hipStream_t s, const hipEvent_t *ev) {
   if (variant_mixed(mx))
-    launch_elem_mixed(type, block, waves, jobs, tasks, nblk, cs, derive, s, ev, mx == kVarMixed);
+    launch_elem_mixed(type, block, waves, jobs, tasks, nblk, cs, derive, s, ev, mx != kVarMixedRR32, mx == kVarMixedX);
   else
     launch_elem_fp64(type, block, waves, jobs, tasks, nblk, cs, derive, s, ev, mx == kVarFP64Ann);
 }
@@ -370,20 +370,18 @@ int run_preloop(dnlg_ctx *ctx, Batch &B) {
 
 }  // namespace
 
-int job_variant(const dnlg_job *j) {
-  if (j->prec == DNLG_PREC_MIXED) return kVarMixed;
-  if (j->prec == DNLG_PREC_MIXED_RR32) return kVarMixedRR32;
-  return j->host.law == DNLG_LAW_ANN2SIG ? kVarFP64Ann : kVarFP64;
-}
-
-static int variant_of(int precision, int law) {
-  if (precision == DNLG_PREC_MIXED) return kVarMixed;
+static int variant_of(int precision, int law, int stress_method) {
+  if (precision == DNLG_PREC_MIXED)
+    return (law == DNLG_LAW_TABULAR || law == DNLG_LAW_ANN2SIG || stress_method == DNLG_STRESS_DIRECT) ? kVarMixedX
+                                                                                                    : kVarMixed;
   if (precision == DNLG_PREC_MIXED_RR32) return kVarMixedRR32;
   return law == DNLG_LAW_ANN2SIG ? kVarFP64Ann : kVarFP64;
 }
 
-bool schedule_valid(int type, int precision, int law, const dnlg_schedule &s) {
-  const int v = variant_of(precision, law);
+int job_variant(const dnlg_job *j) { return variant_of(j->prec, j->host.law, j->host.stress_method); }
+
+bool schedule_valid(int type, int precision, int law, const dnlg_schedule &s, int stress_method) {
+  const int v = variant_of(precision, law, stress_method);
   return elem_sched_valid(type, v, s.elem_block, s.waves) && corr_batch_valid(precision != DNLG_PREC_FP64, s.corr_batch) &&
          s.graph_cycles >= 2 && s.graph_cycles <= 4096;
 }
@@ -400,7 +398,7 @@ dnlg_schedule resolve_schedule(const dnlg_ctx *ctx, const dnlg_job *job) {
       if (r.sched.waves) t.waves = r.sched.waves;
       if (r.sched.corr_batch) t.corr_batch = r.sched.corr_batch;
       if (r.sched.graph_cycles) t.graph_cycles = r.sched.graph_cycles;
-      if (schedule_valid(job->type, job->prec, job->host.law, t)) s = t;
+      if (schedule_valid(job->type, job->prec, job->host.law, t, job->host.stress_method)) s = t;
       break;
     }
   return s;

@@ -13,40 +13,6 @@
 
 namespace dnlg {
 
-// The hardening law's yield stress (and slope) at (epsp, depsp, T) for the
-// FP64 radial returns: a tabular law through its table (out of it: `ok`
-// cleared), the closed-form laws through material.h; ANN: the kernel
-// variant for network laws.
-template <bool ANN>
-__device__ __forceinline__ double law_yield_of(CJobDev *C, int law, const double *lp, double epsp, double depsp,
-                                               double T, double dt, bool &ok) {
-  if (ANN) {
-    double y, h;
-    law_ann(C->ltab, epsp, depsp, T, dt, 0.0, false, y, h);
-    return y;
-  }
-  if (law == DNLG_LAW_TABULAR) {
-    double y = 0.0, h;
-    ok = law_tabular(C->ltab, C->n_ltab, epsp, y, h) && ok;
-    return y;
-  }
-  return law_yield(law, lp, epsp, depsp, T);
-}
-template <bool ANN>
-__device__ __forceinline__ void law_eval(CJobDev *C, int law, const double *lp, double eta_rcp, double epsp,
-                                         double depsp, double T, double dt, double &yield, double &hard, bool &ok) {
-  if (ANN) {
-    law_ann(C->ltab, epsp, depsp, T, dt, eta_rcp, true, yield, hard);
-    return;
-  }
-  if (law == DNLG_LAW_TABULAR) {
-    yield = hard = 0.0;
-    ok = law_tabular(C->ltab, C->n_ltab, epsp, yield, hard) && ok;
-    return;
-  }
-  law_yield_and_slope(law, lp, eta_rcp, epsp, depsp, T, dt, yield, hard);
-}
-
 // ---------------------------------------------------------------------------
 // Element kernel, FP64.  One lane per integration point (NIP lanes per
 // element, NIP == NN for both element types); the element's nodes are staged

@@ -291,7 +291,11 @@ __device__ __forceinline__ bool radial_return_fp32(const LawF &L, int max_it, fl
 // reaches the root while a start below it is held under the bound.  Out:
 // gamma and the FP64 yield stress of its last iterate.  Returns false at the
 // iteration limit.
-__device__ __forceinline__ bool radial_return_fp64(int law, const double *lp, int max_it, double heatFrac, double TwoG32,
+// law_at(epsp, depsp, T, yield, slope) evaluates the law; it returns false
+// where the law has no value (outside a table), which ends the iteration
+// (the caller reports it).
+template <class Law>
+__device__ __forceinline__ bool radial_return_fp64(const Law &law_at, int max_it, double heatFrac, double TwoG32,
                                                    double Strial, double Snorm0, double Tcur, double ep0,
                                                    double dt, double gmin, double gmax, double &y, double &g) {
   double depsp = kSqrt23 * g / dt;
@@ -300,7 +304,7 @@ __device__ __forceinline__ bool radial_return_fp64(int law, const double *lp, in
   int iterate = 0;
   while (true) {
     double hard;
-    law_yield_and_slope<ShortOps>(law, lp, heatFrac, epsp, depsp, T, dt, y, hard);
+    if (!law_at(epsp, depsp, T, y, hard)) return true;
     const double fun = Strial - g * TwoG32 - y;
     if (fun < 0.0)
       gmax = g;
@@ -325,7 +329,11 @@ __device__ __forceinline__ bool radial_return_fp64(int law, const double *lp, in
 // (DNLG_PREC_MIXED_RR32).
 // WAVES > 0: that many waves per SIMD requested from the compiler (registers
 // capped, spills allowed); 0: none requested.
-template <int NN, int DIM, int BLOCK, bool AXI, bool RR64, int WAVES>
+// XL (with RR64): the variant for the laws that have no FP32 form -- the
+// tabular and network laws -- and for the Direct integration: the FP64
+// return of the FP64 path (Element.C:485-601, 625-768) from the
+// reference's own starting state, on the FP32 trial stress.
+template <int NN, int DIM, int BLOCK, bool AXI, bool RR64, int WAVES, bool XL = false>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void k_elem_mixed(
     JobDev *const *jobs, const Task *tasks, int ctl, int derive) {
   constexpr int NB = DIM;               // lanes per element = 2^DIM = NN
@@ -586,7 +594,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
 #pragma unroll
     for (int k = 0; k < 8; k++) lp[k] = C->lp[k];
     if (yieldf == 0.f) {
-      yieldf = (float)law_yield(law, lp, kGammaInitial, kGammaInitial / dt, Tcur);
+      if constexpr (XL) {
+        bool ok0 = true;
+        yieldf = (float)(law == DNLG_LAW_ANN2SIG
+                             ? law_yield_of<true>(C, law, lp, kGammaInitial, kGammaInitial / dt, Tcur, dt, ok0)
+                             : law_yield_of<false>(C, law, lp, kGammaInitial, kGammaInitial / dt, Tcur, dt, ok0));
+        if (!ok0) set_status(J, DNLG_E_TABLE_RANGE, par);
+      } else {
+        yieldf = (float)law_yield(law, lp, kGammaInitial, kGammaInitial / dt, Tcur);
+      }
       at(gbl(C->sy32) + g0, 4 * ip) = yieldf;
     }
     if (Strialf > yieldf) {
@@ -602,22 +618,61 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
 #pragma unroll
       for (int c = 0; c < 6; c++) s_keep[(3 + c) * 64 + lane] = Rs[c];
       asm volatile("" ::: "memory");
-      float pf[8];
-#pragma unroll
-      for (int k = 0; k < 8; k++) pf[k] = C->lpf[k];
-      const LawF L = law_f32_setup(law, pf, C->inv_dTf, (float)dt, (float)heatFrac);
-      float g = gam0;
-      NewtonState last;
-      bool ok = radial_return_fp32(L, C->nr_max_it, Strialf, (float)TwoG32, (float)heatFrac, (float)Tcur, ep0, Snorm0,
-                                   yieldf, g, last);
-      gamma = (double)g;
-      if constexpr (RR64) {
-        double y = (double)last.y;
-        gamma = (double)last.g;
-        ok = radial_return_fp64(law, lp, C->nr_max_it, heatFrac, TwoG32, (double)Strialf, (double)Snorm0, Tcur, ep0, dt,
-                                (double)last.gmin, (double)last.gmax, y, gamma);
+      float g;
+      bool ok = true;
+      if constexpr (XL) {
+        bool in_table = true;
+        auto law_at = [&](double epsp, double depsp, double T, double &y, double &h) {
+          if (law == DNLG_LAW_ANN2SIG)
+            law_eval<true>(C, law, lp, heatFrac, epsp, depsp, T, dt, y, h, in_table);
+          else
+            law_eval<false>(C, law, lp, heatFrac, epsp, depsp, T, dt, y, h, in_table);
+          return in_table;
+        };
+        double y = (double)yieldf;  // the stored yield stress
+        if (C->stress_method == DNLG_STRESS_DIRECT) {
+          // Element::computeStressDirect (Element.C:625-768): one linearised
+          // return from the stored yield stress and the slope at the stored
+          // state (plastic strain, or gammaInitial before the first yield)
+          double y0, hard;
+          const double dep0 = (double)at(gbl(C->dep32) + g0, 4 * ip);
+          law_at(ep0 > 0.0 ? ep0 : kGammaInitial, dep0, Tcur, y0, hard);
+          const double TwoG = C->twoG;
+          gamma = ((double)Snorm - kSqrt23 * y) / (TwoG * (1.0 + hard / (1.5 * TwoG)));
+          const double T = Tcur + 0.5 * gamma * heatFrac * (kSqrt23 * y + (double)Snorm0);
+          const double epsp = ep0 + kSqrt23 * gamma, depsp = kSqrt23 * gamma / dt;
+          y = law == DNLG_LAW_ANN2SIG ? law_yield_of<true>(C, law, lp, epsp, depsp, T, dt, in_table)
+                                      : law_yield_of<false>(C, law, lp, epsp, depsp, T, dt, in_table);
+        } else {
+          gamma = ep0 == 0.0 ? kSqrt32 * kGammaInitial : (double)gam0;
+          ok = radial_return_fp64(law_at, C->nr_max_it, heatFrac, TwoG32, (double)Strialf, (double)Snorm0, Tcur, ep0,
+                                  dt, 0.0, ((double)Strialf - y) / TwoG32, y, gamma);
+        }
+        if (!in_table) set_status(J, DNLG_E_TABLE_RANGE, par);
         g = (float)gamma;
         yieldf = (float)y;
+      } else {
+        float pf[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) pf[k] = C->lpf[k];
+        const LawF L = law_f32_setup(law, pf, C->inv_dTf, (float)dt, (float)heatFrac);
+        g = gam0;
+        NewtonState last;
+        ok = radial_return_fp32(L, C->nr_max_it, Strialf, (float)TwoG32, (float)heatFrac, (float)Tcur, ep0, Snorm0,
+                                yieldf, g, last);
+        gamma = (double)g;
+        if constexpr (RR64) {
+          double y = (double)last.y;
+          gamma = (double)last.g;
+          auto law_at = [&](double epsp, double depsp, double T, double &yl, double &h) {
+            law_yield_and_slope<ShortOps>(law, lp, heatFrac, epsp, depsp, T, dt, yl, h);
+            return true;
+          };
+          ok = radial_return_fp64(law_at, C->nr_max_it, heatFrac, TwoG32, (double)Strialf, (double)Snorm0, Tcur, ep0,
+                                  dt, (double)last.gmin, (double)last.gmax, y, gamma);
+          g = (float)gamma;
+          yieldf = (float)y;
+        }
       }
       if (!ok) set_status(J, DNLG_E_NR_DIVERGED, par);
       asm volatile("" ::: "memory");

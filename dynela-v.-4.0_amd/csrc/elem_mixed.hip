@@ -47,8 +47,18 @@ void launch_mixed_rr(int type, int block, int waves, JobDev *const *jobs, const 
 }
 
 void launch_elem_mixed(int type, int block, int waves, JobDev *const *jobs, const Task *tasks, int nblk, int cs,
-                       int derive, hipStream_t s, const hipEvent_t *ev, bool rr64) {
+                       int derive, hipStream_t s, const hipEvent_t *ev, bool rr64, bool xl) {
   if (nblk <= 0) return;
+  if (xl) {
+    // the table laws and the Direct integration: default schedules only
+    if (type == DNLG_EL_HEX8N_3D)
+      launch_elem(k_elem_mixed<8, 3, 128, false, true, 0, true>, 128, nblk, s, ev, jobs, tasks, cs, derive);
+    else if (type == DNLG_EL_QUA4N_AX)
+      launch_elem(k_elem_mixed<4, 2, 256, true, true, 0, true>, 256, nblk, s, ev, jobs, tasks, cs, derive);
+    else
+      launch_elem(k_elem_mixed<4, 2, 256, false, true, 0, true>, 256, nblk, s, ev, jobs, tasks, cs, derive);
+    return;
+  }
   if (rr64)
     launch_mixed_rr<true>(type, block, waves, jobs, tasks, nblk, cs, derive, s, ev);
   else

@@ -20,7 +20,7 @@ void launch_elem_fp64(int type, int block, int waves, JobDev *const *jobs, const
 // rr64: the radial return in FP64 (DNLG_PREC_MIXED), else in FP32
 // (DNLG_PREC_MIXED_RR32)
 void launch_elem_mixed(int type, int block, int waves, JobDev *const *jobs, const Task *tasks, int nblk, int cs,
-                       int derive, hipStream_t s, const hipEvent_t *ev, bool rr64);
+                       int derive, hipStream_t s, const hipEvent_t *ev, bool rr64, bool xl);
 // fold != 0: BC nodes also get the next cycle's field0 BC values (the next
 // cycle's element launch then derives its record, derive != 0)
 // batch: the incidence slots loaded together (2, 4 or 8 for FP32 forces; 2

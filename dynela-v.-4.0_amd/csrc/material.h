@@ -223,6 +223,41 @@ __device__ __noinline__ void law_ann(const double *t, double epsp, double depsp,
   hard = r3 / r0 * yd0 + (r3 / r1 * yd1 / depsp) / dtime + eta_rcp * yield * (r3 / r2 * yd2);
 }
 
+// The hardening law's yield stress (and slope) at (epsp, depsp, T) for the
+// FP64 radial returns (the FP64 element kernel, and the mixed kernel's
+// variant for the table laws and the Direct integration): a tabular law
+// through its table (out of it: `ok` cleared), the closed-form laws
+// through the functions above; ANN: the network law.
+template <bool ANN>
+__device__ __forceinline__ double law_yield_of(CJobDev *C, int law, const double *lp, double epsp, double depsp,
+                                               double T, double dt, bool &ok) {
+  if (ANN) {
+    double y, h;
+    law_ann(C->ltab, epsp, depsp, T, dt, 0.0, false, y, h);
+    return y;
+  }
+  if (law == DNLG_LAW_TABULAR) {
+    double y = 0.0, h;
+    ok = law_tabular(C->ltab, C->n_ltab, epsp, y, h) && ok;
+    return y;
+  }
+  return law_yield(law, lp, epsp, depsp, T);
+}
+template <bool ANN>
+__device__ __forceinline__ void law_eval(CJobDev *C, int law, const double *lp, double eta_rcp, double epsp,
+                                         double depsp, double T, double dt, double &yield, double &hard, bool &ok) {
+  if (ANN) {
+    law_ann(C->ltab, epsp, depsp, T, dt, eta_rcp, true, yield, hard);
+    return;
+  }
+  if (law == DNLG_LAW_TABULAR) {
+    yield = hard = 0.0;
+    ok = law_tabular(C->ltab, C->n_ltab, epsp, yield, hard) && ok;
+    return;
+  }
+  law_yield_and_slope(law, lp, eta_rcp, epsp, depsp, T, dt, yield, hard);
+}
+
 // FP32 law evaluation of the mixed path's radial return.  Same yield stress
 // and slope as law_yield_and_slope, written in terms of gamma (depsp =
 // sqrt(2/3) gamma / dt, so depsp * dt = sqrt(2/3) gamma), with powers and

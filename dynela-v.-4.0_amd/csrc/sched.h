@@ -17,8 +17,10 @@
 namespace dnlg {
 
 // Element-kernel variants (precision paths) of a job.
-enum ElemVariant { kVarFP64 = 0, kVarMixed = 1, kVarFP64Ann = 2, kVarMixedRR32 = 3 };
-constexpr int kElemVariants = 4;
+// kVarMixedX: the mixed kernel for the tabular and network laws and the
+// Direct integration (an FP64 return throughout).
+enum ElemVariant { kVarFP64 = 0, kVarMixed = 1, kVarFP64Ann = 2, kVarMixedRR32 = 3, kVarMixedX = 4 };
+constexpr int kElemVariants = 5;
 
 // Threads per element-kernel block: 8 / 16 / 32 hexes, 32 / 64 / 128 quads
 // (one lane per node and Gauss point).
@@ -45,10 +47,10 @@ inline int default_corr_batch(bool f32) { return f32 ? 8 : 4; }
 constexpr int kDefaultGraphCycles = 16;
 
 // Whether (block, waves) is a schedule the element kernels of this type and
-// variant are built for.  The plane-strain quad and the network-law variant
-// take their defaults only.
+// variant are built for.  The plane-strain quad, the network-law variant and
+// the mixed table-law variant take their defaults only.
 inline bool elem_sched_valid(int type, int variant, int block, int waves) {
-  if (type == DNLG_EL_QUA4N_2D || variant == kVarFP64Ann)
+  if (type == DNLG_EL_QUA4N_2D || variant == kVarFP64Ann || variant == kVarMixedX)
     return block == default_elem_block(type) && waves == default_waves(variant);
   const int *bl = type == DNLG_EL_HEX8N_3D ? kHexBlocks : kQuadBlocks;
   bool ok = false;

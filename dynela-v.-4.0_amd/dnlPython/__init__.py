@@ -256,7 +256,8 @@ class DiscreteFunction:
 
 class TabularLaw:
     """TabularLaw (TabularLaw.C): yield stress as a DiscreteFunction of
-    the equivalent plastic strain (FP64 precision)."""
+    the equivalent plastic strain (FP64 and mixed precision; the mixed path
+    evaluates it in its FP64 radial return; not "mixed-rr32")."""
     law = _lib.LAW_TABULAR
 
     def __init__(self, name=""):
@@ -279,7 +280,8 @@ class ANN2SigLaw:
     w2 (h1, h2), b2 (h2, 1), w3 (h2, 1), b3 (1, 1), logBase,
     minEntries (4) and maxEntries (4) (ANN2SigLaw.C:137-158: the weight
     matrices are transposed on reading, the biases are their first
-    column).  FP64 precision only."""
+    column).  FP64 and mixed precision (the mixed path evaluates it in its
+    FP64 radial return); not "mixed-rr32"."""
     law = _lib.LAW_ANN2SIG
     _NAMES = ("w1", "b1", "w2", "b2", "w3", "b3", "logBase", "minEntries", "maxEntries")
 
@@ -980,7 +982,8 @@ class DynELA:
 
     def setStressComputation(self, method):
         """DynELA::setStressComputation (DynELA.C:1406-1430): "NewtonRaphson"
-        (default) or "Direct" (one linearised return, FP64 precision)."""
+        (default) or "Direct" (one linearised return, in FP64 on the FP64 and
+        mixed paths; not "mixed-rr32")."""
         if method not in ("NewtonRaphson", "Direct"):
             raise DnlgError(_lib.E_INVALID, f"unknown stress computation method {method!r}")
         self._invalidate()

@@ -60,10 +60,12 @@ extern "C" {
 #define DNLG_LAW_VOCE_LINEAR 2  /* sy = A + Q(1-exp(-b ep)) + H ep;     p = A,Q,b,H          */
 #define DNLG_LAW_TABULAR 3      /* TabularLaw.C: sy(ep) piecewise linear through the
                                    material's table (DiscreteFunction::getValue /
-                                   getSlope, DiscreteFunction.C:188-308); FP64 only */
+                                   getSlope, DiscreteFunction.C:188-308); FP64 and
+                                   mixed (an FP64 radial return), not mixed-rr32 */
 #define DNLG_LAW_ANN2SIG 4      /* ANN2SigLaw.C:173-245: sy(ep, dep, T) from a network of
                                    3 inputs, two sigmoid hidden layers of h1 and h2
-                                   neurons and one linear output; FP64 only.  The
+                                   neurons and one linear output; FP64 and mixed
+                                   (an FP64 radial return), not mixed-rr32.  The
                                    material's table holds the network, packed as
                                    h1, h2, w1[h1][3], b1[h1], w2[h2][h1], b2[h2],
                                    w3[h2], b3, logBase, minEntries[4], maxEntries[4]
@@ -92,7 +94,8 @@ extern "C" {
 
 /* stress integration (Model.h StressIntNR / StressIntDirect, Model.C:840-851) */
 #define DNLG_STRESS_NR 0     /* Newton-Raphson radial return, Element.C:407-622 */
-#define DNLG_STRESS_DIRECT 1 /* one linearised return, Element.C:625-768 (FP64 only) */
+#define DNLG_STRESS_DIRECT 1 /* one linearised return, Element.C:625-768 (FP64 and
+                                mixed, in FP64; not mixed-rr32) */
 
 /* arithmetic of the element phases */
 #define DNLG_PREC_FP64 0  /* everything in FP64                              */

@@ -104,13 +104,29 @@ def test_direct_stress_integration(ctx, kind):
     compare(m, o)
 
 
-def test_direct_stress_integration_needs_fp64(ctx):
+def test_direct_stress_integration_has_no_fp32_form(ctx):
+    """mixed-rr32 (the radial return in FP32) refuses the Direct integration."""
     import dnlPython as dnl
     from dnlPython import samples
-    m = samples.taylor_axi(nr=3, nz=8, precision="mixed", context=ctx)
+    m = samples.taylor_axi(nr=3, nz=8, precision="mixed-rr32", context=ctx)
     m.setStressComputation("Direct")
     with pytest.raises(dnl.DnlgError):
         m.solve()
+
+
+@pytest.mark.parametrize("kind", ["axi", "3d"])
+def test_direct_stress_integration_mixed(ctx, kind):
+    """Direct on the mixed path (FP32 strain / stress, the linearised return
+    in FP64) against the FP64 oracle, at the mixed tolerances."""
+    from dnlPython import samples
+    from test_gpu_parity import compare_mixed
+    if kind == "axi":
+        m = samples.taylor_axi(nr=5, nz=25, t_end=10e-6, precision="mixed", track_tensors=False, context=ctx)
+    else:
+        m = samples.taylor_3d(n=4, nz=12, t_end=10e-6, precision="mixed", track_tensors=False, context=ctx)
+    m.setStressComputation("Direct")
+    o = run_both(m, 10e-6)
+    compare_mixed(m, o, "mixed")
 
 
 def _tabular(points):
@@ -136,22 +152,45 @@ def test_tabular_law(ctx):
     compare(m, o)
 
 
-def test_tabular_law_errors(ctx):
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_tabular_law_errors(ctx, prec):
     """Out of the table: the job stops with DNLG_E_TABLE_RANGE (the reference
-    exits, DiscreteFunction.C:195-196); mixed precision is refused."""
+    exits, DiscreteFunction.C:195-196), in both precisions that run the
+    law; mixed-rr32 (an FP32 return) is refused."""
     import dnlPython as dnl
     from dnlPython import _lib, samples
-    m = samples.taylor_axi(nr=4, nz=20, t_end=8e-6, context=ctx)
+    m = samples.taylor_axi(nr=4, nz=20, t_end=8e-6, precision=prec, context=ctx)
     m._material.setHardeningLaw(_tabular([(0.0, 120e6), (0.01, 125e6)]))
     m._invalidate()
     with pytest.raises(dnl.DnlgError) as e:
         m.solve()
     assert e.value.status == _lib.E_TABLE_RANGE
-    m = samples.taylor_axi(nr=4, nz=20, precision="mixed", context=ctx)
+    m = samples.taylor_axi(nr=4, nz=20, precision="mixed-rr32", context=ctx)
     m._material.setHardeningLaw(_tabular([(0.0, 120e6), (1.0, 125e6)]))
     m._invalidate()
-    with pytest.raises(dnl.DnlgError):
+    with pytest.raises(dnl.DnlgError) as e:
         m.solve()
+    assert e.value.status == _lib.E_INVALID
+
+
+@pytest.mark.parametrize("kind", ["axi", "3d"])
+def test_tabular_law_mixed(ctx, kind):
+    """The tabular law on the mixed path (its FP64 return from the
+    reference's starting state) against the FP64 oracle, at the mixed
+    tolerances: a dense table of the Johnson-Cook strain hardening."""
+    import dnlPython as dnl
+    from dnlPython import samples
+    from test_gpu_parity import compare_mixed
+    pts = [(e, 90e6 + 292e6 * e ** 0.31) for e in np.concatenate([[0.0], np.geomspace(1e-8, 4.0, 400)])]
+    if kind == "axi":
+        m = samples.taylor_axi(nr=5, nz=25, t_end=10e-6, precision="mixed", track_tensors=False, context=ctx)
+    else:
+        m = samples.taylor_3d(n=4, nz=12, t_end=10e-6, precision="mixed", track_tensors=False, context=ctx)
+    m._material.setHardeningLaw(_tabular(pts))
+    m._invalidate()
+    o = run_both(m, 10e-6)
+    compare_mixed(m, o, "mixed")
+    assert np.max(m.getIntPointField(dnl.Field.plasticStrain)) > 0.05
 
 
 @pytest.mark.parametrize("kind", ["axi", "3d"])
@@ -170,6 +209,32 @@ def test_ann_law(ctx, tmp_path, kind):
     o = run_both(m, 10e-6)
     compare(m, o)
     assert np.max(m.getIntPointField(dnl.Field.plasticStrain)) > 0.05
+
+
+@pytest.mark.parametrize("kind", ["axi", "3d"])
+def test_ann_law_mixed(ctx, tmp_path, kind):
+    """The network law on the mixed path (its FP64 return) against the FP64
+    oracle at the mixed tolerances, alone and in a batch with a
+    Johnson-Cook mixed job that stays bit-identical to its solo run."""
+    import dnlPython as dnl
+    from dnlPython import samples
+    from helpers import ann_law, ann_network
+    from test_gpu_parity import compare_mixed
+    from oracle.bridge import OracleJob
+    mk = samples.taylor_axi if kind == "axi" else samples.taylor_3d
+    kw = dict(nr=5, nz=25) if kind == "axi" else dict(n=4, nz=12)
+    m = mk(t_end=10e-6, precision="mixed", track_tensors=False, context=ctx, **kw)
+    m._material.setHardeningLaw(ann_law(tmp_path / "net.npz", ann_network(seed=5)))
+    m._invalidate()
+    jc = mk(t_end=10e-6, precision="mixed", track_tensors=False, context=ctx, **kw)
+    solo = mk(t_end=10e-6, precision="mixed", track_tensors=False, context=ctx, **kw)
+    o = OracleJob(m.descriptors())
+    assert o.solve(10e-6) == 0
+    dnl.batch_solve([m, jc])
+    solo.solve()
+    compare_mixed(m, o, "mixed")
+    assert np.max(m.getIntPointField(dnl.Field.plasticStrain)) > 0.05
+    np.testing.assert_array_equal(jc.getNodalDisplacements(), solo.getNodalDisplacements())
 
 
 def test_ann_law_in_a_batch_with_another_law(ctx, tmp_path):
@@ -195,8 +260,9 @@ def test_ann_law_in_a_batch_with_another_law(ctx, tmp_path):
 
 
 def test_ann_law_errors(ctx, tmp_path):
-    """Job creation refuses mixed precision for a network law, and packed
-    networks whose sizes, table length or scaling ranges are invalid."""
+    """Job creation refuses mixed-rr32 (an FP32 return) for a network law,
+    and packed networks whose sizes, table length or scaling ranges are
+    invalid."""
     import dnlPython as dnl
     from dnlPython import _lib, samples
     from helpers import ann_law, ann_network
@@ -212,7 +278,7 @@ def test_ann_law_errors(ctx, tmp_path):
             m.solve()
         assert e.value.status == _lib.E_INVALID
 
-    attempt("mixed")
+    attempt("mixed-rr32")
 
     def too_wide(t):
         t[0] = 65
