@@ -562,8 +562,8 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
   // at most n_elems nn + n_nodes nn)
   if ((mesh->n_nodes + mesh->n_elems) * 8 >= (int64_t)1 << 31)
     return fail(ctx, DNLG_E_CAPACITY, "mesh exceeds 32-bit indexing");
-  if (prm->dt_method != DNLG_DT_COURANT && prm->dt_method != DNLG_DT_POWER_ITERATION)
-    return fail(ctx, DNLG_E_INVALID, "time-step method not implemented (Courant and PowerIteration are)");
+  if (prm->dt_method != DNLG_DT_COURANT && !is_pi(prm->dt_method))
+    return fail(ctx, DNLG_E_INVALID, "unknown time-step method");
   if (prm->stress_method != DNLG_STRESS_NR && prm->stress_method != DNLG_STRESS_DIRECT)
     return fail(ctx, DNLG_E_INVALID, "unknown stress integration method");
   if (prm->stress_method == DNLG_STRESS_DIRECT && prm->precision == DNLG_PREC_MIXED_RR32)
@@ -802,7 +802,7 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
   ALLOC(H.cmin, (int64_t)job->n_cmin, (const double *)nullptr);
   H.pi = PiState{0.0, 1.0, 0, 0, 0, 0, 0, 0};
   H.n_pibmax = 0;
-  if (prm->dt_method == DNLG_DT_POWER_ITERATION) {
+  if (is_pi(prm->dt_method)) {
     // starting vector of Model.C:966-970 in the reference's DOF order
     // (node * dim + d), stored [dim][N]
     const int64_t ndof = (int64_t)dim * N;

@@ -139,7 +139,7 @@ int build_batch(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, Batch &B) {
   B.d_snap = (double *)p;
   B.h_snap.assign(kSnap * n, 0.0);
   for (auto *j : jobs) {
-    B.has_pi = B.has_pi || j->host.dt_method == DNLG_DT_POWER_ITERATION;
+    B.has_pi = B.has_pi || is_pi(j->host.dt_method);
     B.has_courant = B.has_courant || j->host.dt_method == DNLG_DT_COURANT;
   }
   if (hipMalloc(&p, n * sizeof(int)) != hipSuccess) return DNLG_E_OOM;

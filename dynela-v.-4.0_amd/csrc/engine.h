@@ -30,6 +30,12 @@ constexpr int kItMax = 250;
 
 __host__ __device__ __forceinline__ double sqd(double a) { return a * a; }
 
+// Time-step methods that run the power iteration (fully or under-integrated
+// element stiffness).
+__host__ __device__ __forceinline__ bool is_pi(int dt_method) {
+  return dt_method == DNLG_DT_POWER_ITERATION || dt_method == DNLG_DT_POWER_ITERATION_UNDER;
+}
+
 // Precisions with FP32 element state (DNLG_PREC_MIXED, DNLG_PREC_MIXED_RR32).
 __host__ __device__ __forceinline__ bool is_mixed(int prec) { return prec != DNLG_PREC_FP64; }
 
@@ -187,7 +193,7 @@ struct JobDev {
   float *elemF32;       // mixed-precision variant of elemF
   double *cmin;         // Courant partial minima, one per Courant block
   int32_t n_cmin;
-  // power iteration (DNLG_DT_POWER_ITERATION only): iterates [dim][N] x 2,
+  // power iteration (DNLG_DT_POWER_ITERATION[_UNDER] only): iterates [dim][N] x 2,
   // per-node-block partial maxima
   double *pi_ev[2];
   double *pi_bmax;

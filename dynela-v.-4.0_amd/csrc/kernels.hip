@@ -397,7 +397,7 @@ __global__ __launch_bounds__(256) void k_cycle_step(JobDev *const *jobs, const T
   double freq = 0.0;
   bool stall = false;
   if (courant_ran(c, status)) {
-    if (C->dt_method == DNLG_DT_POWER_ITERATION) {
+    if (is_pi(C->dt_method)) {
       // the iteration slots of the launch sequence did not converge: close
       // the cycle and stop the job until the host-driven iteration finishes
       if (C->pi.done) freq = sqrt(C->pi.fmax);
@@ -430,7 +430,7 @@ __global__ __launch_bounds__(256) void k_cycle_step(JobDev *const *jobs, const T
 // its pi.call is set and pi.done is not.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ bool pi_active(CJobDev *C) {
-  return C->dt_method == DNLG_DT_POWER_ITERATION && C->pi.call && !C->pi.done;
+  return is_pi(C->dt_method) && C->pi.call && !C->pi.done;
 }
 
 // Starts a time-step computation: force (the pre-loop's forced step,
@@ -441,7 +441,7 @@ __global__ void k_pi_begin(JobDev *const *jobs, int n_jobs, int cs, int force) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n_jobs) return;
   JobDev *J = jobs[j];
-  if (J->dt_method != DNLG_DT_POWER_ITERATION || J->pi.stall) return;
+  if (!is_pi(J->dt_method) || J->pi.stall) return;
   const Control &c = J->c[cs];
   const int status = J->sync.st[0] != 0 ? J->sync.st[0] : J->sync.st[1];
   const bool need = force ? status == 0 : courant_ran(c, status);
@@ -480,7 +480,17 @@ __global__ __launch_bounds__(256) void k_pi_elem(JobDev *const *jobs, const Task
   const double nu = C->nu;
   const double fct = C->young / ((1 + nu) * (1 - 2 * nu));
   const double c11 = (1 - nu) * fct, c12 = nu * fct, c44 = (1 - 2 * nu) * fct;
-  for (int p = 0; p < NN; p++) {
+  // PowerIterationUnder (computeElasticStiffnessMatrix(true)): one point at
+  // the element centre, weight NN (8 / 4), shape functions 1 / NN and
+  // derivatives +-1 / NN by the nodes' local signs (ElHex8N3D.C:233-252,
+  // ElQua4NAx.C:112-124, ElQua4N2D.C:115-127), on the current coordinates
+  // (Element3D::computeUnderJacobian, ElementPlane::computeUnderJacobian)
+  const bool under = C->dt_method == DNLG_DT_POWER_ITERATION_UNDER;
+  const int npts = under ? 1 : NN;
+  auto dshape = [&](int p, int a, int k) { return under ? node_sign<NN>(a, k) / NN : tabD<NN>(p, a, k); };
+  auto shape = [&](int p, int a) { return under ? 1.0 / NN : tabN<NN>(p, a); };
+  const double wgt = under ? (double)NN : 1.0;  // full integration: Gauss weights 1
+  for (int p = 0; p < npts; p++) {
     double Jm[DIM][DIM];
 #pragma unroll
     for (int i = 0; i < DIM; i++)
@@ -491,7 +501,7 @@ __global__ __launch_bounds__(256) void k_pi_elem(JobDev *const *jobs, const Task
 #pragma unroll
       for (int i = 0; i < DIM; i++)
 #pragma unroll
-        for (int j = 0; j < DIM; j++) Jm[i][j] += tabD<NN>(p, a, j) * X[a][i];
+        for (int j = 0; j < DIM; j++) Jm[i][j] += dshape(p, a, j) * X[a][i];
     double det, inv[DIM][DIM];
     if constexpr (DIM == 3) {
       const double *dd = &Jm[0][0];
@@ -522,7 +532,7 @@ __global__ __launch_bounds__(256) void k_pi_elem(JobDev *const *jobs, const Task
       for (int j = 0; j < DIM; j++) {
         double s = 0.0;
 #pragma unroll
-        for (int k = 0; k < DIM; k++) s += tabD<NN>(p, a, k) * inv[k][j];
+        for (int k = 0; k < DIM; k++) s += dshape(p, a, k) * inv[k][j];
         dN[a][j] = s;
       }
     if constexpr (DIM == 3) {
@@ -539,7 +549,7 @@ __global__ __launch_bounds__(256) void k_pi_elem(JobDev *const *jobs, const Task
       const double sxx = c11 * exx + c12 * (eyy + ezz), syy = c11 * eyy + c12 * (exx + ezz);
       const double szz = c11 * ezz + c12 * (exx + eyy);
       const double sxy = c44 * gxy, syz = c44 * gyz, sxz = c44 * gxz;
-      const double w = det;  // Gauss weights are 1
+      const double w = wgt * det;
 #pragma unroll
       for (int a = 0; a < NN; a++) {
         f[a][0] += (dN[a][0] * sxx + dN[a][1] * sxy + dN[a][2] * sxz) * w;
@@ -556,7 +566,7 @@ __global__ __launch_bounds__(256) void k_pi_elem(JobDev *const *jobs, const Task
         gxy += dN[a][1] * v[a][0] + dN[a][0] * v[a][1];
       }
       const double sxx = c11 * exx + c12 * eyy, syy = c11 * eyy + c12 * exx, sxy = c44 * gxy;
-      const double w = det;
+      const double w = wgt * det;
 #pragma unroll
       for (int a = 0; a < NN; a++) {
         f[a][0] += (dN[a][0] * sxx + dN[a][1] * sxy) * w;
@@ -565,21 +575,21 @@ __global__ __launch_bounds__(256) void k_pi_elem(JobDev *const *jobs, const Task
     } else {
       double r = 0.0;
 #pragma unroll
-      for (int a = 0; a < NN; a++) r += tabN<NN>(p, a) * X[a][0];
+      for (int a = 0; a < NN; a++) r += shape(p, a) * X[a][0];
       double err = 0, ezz = 0, grz = 0, ett = 0;
 #pragma unroll
       for (int a = 0; a < NN; a++) {
         err += dN[a][0] * v[a][0];
         ezz += dN[a][1] * v[a][1];
         grz += dN[a][1] * v[a][0] + dN[a][0] * v[a][1];
-        ett += tabN<NN>(p, a) / r * v[a][0];
+        ett += shape(p, a) / r * v[a][0];
       }
       const double srr = c11 * err + c12 * (ezz + ett), szz = c11 * ezz + c12 * (err + ett);
       const double stt = c11 * ett + c12 * (err + ezz), srz = c44 * grz;
-      const double w = det * k2PI * r;
+      const double w = wgt * det * k2PI * r;
 #pragma unroll
       for (int a = 0; a < NN; a++) {
-        f[a][0] += (dN[a][0] * srr + dN[a][1] * srz + tabN<NN>(p, a) / r * stt) * w;
+        f[a][0] += (dN[a][0] * srr + dN[a][1] * srz + shape(p, a) / r * stt) * w;
         f[a][1] += (dN[a][1] * szz + dN[a][0] * srz) * w;
       }
     }
@@ -676,7 +686,7 @@ __global__ void k_pi_finish(JobDev *const *jobs, int n_jobs, int preloop) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n_jobs) return;
   JobDev *J = jobs[j];
-  if (J->dt_method != DNLG_DT_POWER_ITERATION || !J->pi.call) return;
+  if (!is_pi(J->dt_method) || !J->pi.call) return;
   if (!preloop && !J->pi.stall) return;
   if (J->pi.done && J->sync.st[0] == 0 && J->sync.st[1] == 0) {
     const double freq = sqrt(J->pi.fmax);
@@ -691,7 +701,7 @@ __global__ void k_pi_pending(JobDev *const *jobs, int n_jobs, int *out) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n_jobs) return;
   const JobDev *J = jobs[j];
-  out[j] = (J->dt_method == DNLG_DT_POWER_ITERATION && J->pi.call && !J->pi.done) ? 1 : 0;
+  out[j] = (is_pi(J->dt_method) && J->pi.call && !J->pi.done) ? 1 : 0;
 }
 
 // Host-loop snapshot: t, dt, up_to, status, increment, power-iteration stall

@@ -28,6 +28,16 @@ __device__ __forceinline__ double tabD(int p, int a, int j) {
   else return c_quaD[p][a][j];
 }
 
+// Local coordinate sign (+-1) of node a along direction k in the reference's
+// numbering (ElHex8N3D.C:30-92, ElQua4NAx.C:30-62): hex x + for nodes 1, 2,
+// 5, 6, y + for 2, 3, 6, 7, z + for 4-7; quad x + for 1, 2, y + for 2, 3.
+template <int NN>
+__host__ __device__ __forceinline__ double node_sign(int a, int k) {
+  if (k == 0) return (a == 1 || a == 2 || a == 5 || a == 6) ? 1.0 : -1.0;
+  if (k == 1) return (a == 2 || a == 3 || a == 6 || a == 7) ? 1.0 : -1.0;
+  return a >= 4 ? 1.0 : -1.0;
+}
+
 static inline void upload_shape_tables(const double *hexN, const double *hexD, const double *quaN,
                                        const double *quaD) {
   hipMemcpyToSymbol(HIP_SYMBOL(c_hexN), hexN, sizeof(double) * 64);

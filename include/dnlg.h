@@ -86,8 +86,9 @@ extern "C" {
 #define DNLG_RAMP_ERF 4
 #define DNLG_RAMP_SOFT 5
 
-/* time-step methods (Solver.h:28-33); Courant and PowerIteration are
- * implemented, PowerIterationUnder is not (DNLG_E_INVALID) */
+/* time-step methods (Solver.h:28-33, Solver.C:196-236): Courant, and the
+ * power iteration on the fully integrated or (PowerIterationUnder) the
+ * one-point under-integrated element stiffness (Model.C:942-1009) */
 #define DNLG_DT_COURANT 0
 #define DNLG_DT_POWER_ITERATION 1
 #define DNLG_DT_POWER_ITERATION_UNDER 2

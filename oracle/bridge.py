@@ -57,7 +57,8 @@ def load():
         lib.oracle_chung_hulbert.argtypes = [ctypes.c_double, _dp]
         lib.oracle_ramp_value.restype = ctypes.c_double
         lib.oracle_ramp_value.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double]
-        lib.oracle_element_stiffness.argtypes = [ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double, _dp]
+        lib.oracle_element_stiffness.argtypes = [ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                                 _dp]
         lib.oracle_job_pi_state.argtypes = [_vp, _dp, ctypes.POINTER(ctypes.c_int)]
         lib.oracle_job_add_history.argtypes = [_vp, ctypes.c_double, ctypes.c_double, ctypes.POINTER(HistoryItem),
                                                ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
@@ -218,14 +219,15 @@ class OracleJob:
             pass
 
 
-def element_stiffness(elem_type, xyz, young, poisson):
+def element_stiffness(elem_type, xyz, young, poisson, under=False):
     """Elastic stiffness matrix (nn dim x nn dim) of one element with node
-    coordinates xyz[a] = (x, y, z)."""
+    coordinates xyz[a] = (x, y, z); under: the one-point under-integrated
+    matrix (PowerIterationUnder)."""
     lib = load()
     xyz = np.ascontiguousarray(xyz, dtype=np.float64)
     nd = xyz.shape[0] * (3 if elem_type == 4 else 2)  # DNLG_EL_HEX8N_3D = 4
     K = np.zeros((nd, nd))
-    st = lib.oracle_element_stiffness(elem_type, p(xyz), young, poisson, K.ctypes.data_as(_dp))
+    st = lib.oracle_element_stiffness(elem_type, p(xyz), young, poisson, 1 if under else 0, K.ctypes.data_as(_dp))
     if st != 0:
         raise RuntimeError(f"oracle_element_stiffness failed: {st}")
     return K

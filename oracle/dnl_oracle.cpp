@@ -1019,95 +1019,180 @@ void hooke(const OracleJob *J, double C[6][6]) {
     for (int j = 0; j < 6; j++) C[i][j] *= f;
 }
 
-/* Element3D::computeElasticStiffnessMatrix (Element3D.C:303-399) and
- * ElementPlane::computeElasticStiffnessMatrix (ElementPlane.C:182-292),
- * axisymmetric and plane-strain branches: K_e = sum_ip B^T C B w detJ (x 2 pi r), built as C B
- * then B^T (C B) per IP in the reference's term order, on the Jacobian of the
- * last compute_jacobian.  K is (nn dim) x (nn dim), row-major. */
-void element_stiffness(const OracleJob *J, int64_t e, double *K) {
+/* One integration point's term of Element3D::computeElasticStiffnessMatrix
+ * (Element3D.C:303-399) and ElementPlane::computeElasticStiffnessMatrix
+ * (ElementPlane.C:182-292), axisymmetric and plane-strain branches: C B,
+ * then K += B^T (C B) WxdJ in the reference's term order.  dn: the shape
+ * derivatives [node][dir] at the point, WxdJ = weight detJ, Np: the shape
+ * functions at the point, R: its radius (axisymmetric). */
+void add_point_stiffness(const OracleJob *J, const double C[6][6], const double *dn, double WxdJ, const double *Np,
+                         double R, double *K) {
   const int nn = J->nn, dim = J->dim, nd = nn * dim;
-  double C[6][6];
-  hooke(J, C);
-  for (int k = 0; k < nd * nd; k++) K[k] = 0.0;
   std::vector<double> CB((size_t)6 * nd);
   auto cb = [&](int r, int c) -> double & { return CB[(size_t)r * nd + c]; };
   auto kk = [&](int r, int c) -> double & { return K[(size_t)r * nd + c]; };
-  for (int p = 0; p < J->nip; p++) {
-    const int64_t g = e * J->nip + p;
-    const double *dn = &J->dN[g * nn * dim];
-    double WxdJ = J->tab.w[p] * J->detJ[g];
-    if (dim == 3) {
-      for (int i = 0; i < nn; i++) {
-        const int I = 3 * i;
-        const double dNx = dn[i * 3], dNy = dn[i * 3 + 1], dNz = dn[i * 3 + 2];
-        for (int r = 0; r < 6; r++) {
-          cb(r, I) = (C[r][0] * dNx + C[r][3] * dNy + C[r][5] * dNz);
-          cb(r, I + 1) = (C[r][1] * dNy + C[r][3] * dNx + C[r][4] * dNz);
-          cb(r, I + 2) = (C[r][2] * dNz + C[r][4] * dNy + C[r][5] * dNx);
+  if (dim == 3) {
+    for (int i = 0; i < nn; i++) {
+      const int I = 3 * i;
+      const double dNx = dn[i * 3], dNy = dn[i * 3 + 1], dNz = dn[i * 3 + 2];
+      for (int r = 0; r < 6; r++) {
+        cb(r, I) = (C[r][0] * dNx + C[r][3] * dNy + C[r][5] * dNz);
+        cb(r, I + 1) = (C[r][1] * dNy + C[r][3] * dNx + C[r][4] * dNz);
+        cb(r, I + 2) = (C[r][2] * dNz + C[r][4] * dNy + C[r][5] * dNx);
+      }
+    }
+    for (int i = 0; i < nn; i++) {
+      const double dNx = dn[i * 3], dNy = dn[i * 3 + 1], dNz = dn[i * 3 + 2];
+      for (int j = 0; j < nn; j++) {
+        const int I = 3 * i, Jc = 3 * j;
+        for (int b = 0; b < 3; b++) {
+          kk(I, Jc + b) += (dNx * cb(0, Jc + b) + dNy * cb(3, Jc + b) + dNz * cb(5, Jc + b)) * WxdJ;
+          kk(I + 1, Jc + b) += (dNy * cb(1, Jc + b) + dNx * cb(3, Jc + b) + dNz * cb(4, Jc + b)) * WxdJ;
+          kk(I + 2, Jc + b) += (dNz * cb(2, Jc + b) + dNy * cb(4, Jc + b) + dNx * cb(5, Jc + b)) * WxdJ;
         }
       }
-      for (int i = 0; i < nn; i++) {
-        const double dNx = dn[i * 3], dNy = dn[i * 3 + 1], dNz = dn[i * 3 + 2];
-        for (int j = 0; j < nn; j++) {
-          const int I = 3 * i, Jc = 3 * j;
-          for (int b = 0; b < 3; b++) {
-            kk(I, Jc + b) += (dNx * cb(0, Jc + b) + dNy * cb(3, Jc + b) + dNz * cb(5, Jc + b)) * WxdJ;
-            kk(I + 1, Jc + b) += (dNy * cb(1, Jc + b) + dNx * cb(3, Jc + b) + dNz * cb(4, Jc + b)) * WxdJ;
-            kk(I + 2, Jc + b) += (dNz * cb(2, Jc + b) + dNy * cb(4, Jc + b) + dNx * cb(5, Jc + b)) * WxdJ;
-          }
-        }
+    }
+  } else {
+    const bool axi = J->tab.axi;
+    if (axi) WxdJ *= k2PI * R;
+    for (int i = 0; i < nn; i++) {
+      const int I = 2 * i;
+      const double dNx = dn[i * 2], dNy = dn[i * 2 + 1];
+      const double NoverR = Np[i] / R;
+      for (int r = 0; r < 3; r++) {
+        cb(r, I) = (C[r][0] * dNx + C[r][2] * dNy);
+        cb(r, I + 1) = (C[r][1] * dNy + C[r][2] * dNx);
       }
-    } else {
-      const bool axi = J->tab.axi;
-      const double R = axi ? J->radius[g] : 1.0;
-      if (axi) WxdJ *= k2PI * R;
-      for (int i = 0; i < nn; i++) {
-        const int I = 2 * i;
-        const double dNx = dn[i * 2], dNy = dn[i * 2 + 1];
-        const double NoverR = J->tab.N[p][i] / R;
-        for (int r = 0; r < 3; r++) {
-          cb(r, I) = (C[r][0] * dNx + C[r][2] * dNy);
-          cb(r, I + 1) = (C[r][1] * dNy + C[r][2] * dNx);
+      if (axi) {
+        cb(3, I) = (C[3][0] * dNx + C[3][2] * dNy);
+        cb(3, I + 1) = (C[3][1] * dNy + C[3][2] * dNx);
+        for (int r = 0; r < 4; r++) cb(r, I) += C[r][3] * NoverR;
+      }
+    }
+    for (int i = 0; i < nn; i++) {
+      const double dNx = dn[i * 2], dNy = dn[i * 2 + 1];
+      const double NoverR = Np[i] / R;
+      for (int j = 0; j < nn; j++) {
+        const int I = 2 * i, Jc = 2 * j;
+        for (int b = 0; b < 2; b++) {
+          kk(I, Jc + b) += (dNx * cb(0, Jc + b) + dNy * cb(2, Jc + b)) * WxdJ;
+          kk(I + 1, Jc + b) += (dNy * cb(1, Jc + b) + dNx * cb(2, Jc + b)) * WxdJ;
         }
         if (axi) {
-          cb(3, I) = (C[3][0] * dNx + C[3][2] * dNy);
-          cb(3, I + 1) = (C[3][1] * dNy + C[3][2] * dNx);
-          for (int r = 0; r < 4; r++) cb(r, I) += C[r][3] * NoverR;
-        }
-      }
-      for (int i = 0; i < nn; i++) {
-        const double dNx = dn[i * 2], dNy = dn[i * 2 + 1];
-        const double NoverR = J->tab.N[p][i] / R;
-        for (int j = 0; j < nn; j++) {
-          const int I = 2 * i, Jc = 2 * j;
-          for (int b = 0; b < 2; b++) {
-            kk(I, Jc + b) += (dNx * cb(0, Jc + b) + dNy * cb(2, Jc + b)) * WxdJ;
-            kk(I + 1, Jc + b) += (dNy * cb(1, Jc + b) + dNx * cb(2, Jc + b)) * WxdJ;
-          }
-          if (axi) {
-            kk(I, Jc) += cb(3, Jc) * NoverR * WxdJ;
-            kk(I, Jc + 1) += cb(3, Jc + 1) * NoverR * WxdJ;
-          }
+          kk(I, Jc) += cb(3, Jc) * NoverR * WxdJ;
+          kk(I, Jc + 1) += cb(3, Jc + 1) * NoverR * WxdJ;
         }
       }
     }
   }
 }
 
-/* Model::computePowerIterationTimeStep (Model.C:942-1009), full
- * integration: element stiffness matrices, then iterations of
+/* computeElasticStiffnessMatrix(false): K_e = sum_ip B^T C B w detJ
+ * (x 2 pi r) on the Jacobian of the last compute_jacobian.  K is
+ * (nn dim) x (nn dim), row-major. */
+void element_stiffness(const OracleJob *J, int64_t e, double *K) {
+  const int nn = J->nn, dim = J->dim, nd = nn * dim;
+  double C[6][6];
+  hooke(J, C);
+  for (int k = 0; k < nd * nd; k++) K[k] = 0.0;
+  for (int p = 0; p < J->nip; p++) {
+    const int64_t g = e * J->nip + p;
+    const bool axi = dim == 2 && J->tab.axi;
+    add_point_stiffness(J, C, &J->dN[g * nn * dim], J->tab.w[p] * J->detJ[g], J->tab.N[p], axi ? J->radius[g] : 1.0,
+                        K);
+  }
+}
+
+/* computeElasticStiffnessMatrix(true), PowerIterationUnder: one point at the
+ * element centre (the under-integration point of ElHex8N3D.C:233-252,
+ * ElQua4NAx.C:112-124, ElQua4N2D.C:115-127: weight 8 / 4, shape functions
+ * 1/8 / 1/4, derivatives +-1/8 / +-1/4 by the nodes' local signs), its
+ * Jacobian, inverse and radius formed on the current coordinates
+ * (Element3D::computeUnderJacobian, Element3D.C:118-167;
+ * ElementPlane::computeUnderJacobian, ElementPlane.C:104-160).  false for a
+ * negative determinant (the reference exits, Model.C:777-798). */
+bool element_stiffness_under(const OracleJob *J, int64_t e, double *K) {
+  const int nn = J->nn, dim = J->dim, nd = nn * dim;
+  /* local coordinate signs of the reference's node numbering */
+  static const double sgn8[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                                    {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+  static const double sgn4[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+  double dNdr[8][3], Np[8];
+  for (int a = 0; a < nn; a++) {
+    Np[a] = 1.0 / nn;
+    for (int k = 0; k < dim; k++) dNdr[a][k] = (nn == 8 ? sgn8[a][k] : sgn4[a][k]) / nn;
+  }
+  const int32_t *cn = &J->conn[e * nn];
+  double Jm[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  for (int a = 0; a < nn; a++)
+    for (int i = 0; i < dim; i++)
+      for (int j = 0; j < dim; j++) Jm[i][j] += dNdr[a][j] * J->x[i * J->N + cn[a]];
+  double det, inv[3][3];
+  if (dim == 3) {
+    const double *d = &Jm[0][0];
+    det = d[1] * d[5] * d[6] - d[2] * d[4] * d[6] + d[2] * d[3] * d[7] - d[0] * d[5] * d[7] - d[1] * d[3] * d[8] +
+          d[0] * d[4] * d[8];
+    if (det < 0.0) return false;
+    double *o = &inv[0][0];
+    o[0] = (d[4] * d[8] - d[5] * d[7]) / det;
+    o[1] = (d[2] * d[7] - d[1] * d[8]) / det;
+    o[2] = (d[1] * d[5] - d[2] * d[4]) / det;
+    o[3] = (d[5] * d[6] - d[3] * d[8]) / det;
+    o[4] = (d[0] * d[8] - d[2] * d[6]) / det;
+    o[5] = (d[2] * d[3] - d[0] * d[5]) / det;
+    o[6] = (d[3] * d[7] - d[4] * d[6]) / det;
+    o[7] = (d[1] * d[6] - d[0] * d[7]) / det;
+    o[8] = (d[0] * d[4] - d[1] * d[3]) / det;
+  } else {
+    det = Jm[0][0] * Jm[1][1] - Jm[1][0] * Jm[0][1];
+    if (det < 0.0) return false;
+    inv[0][0] = Jm[1][1] / det;
+    inv[0][1] = -Jm[0][1] / det;
+    inv[1][0] = -Jm[1][0] / det;
+    inv[1][1] = Jm[0][0] / det;
+  }
+  double dn[8 * 3];
+  for (int a = 0; a < nn; a++)
+    for (int j = 0; j < dim; j++) {
+      double s = 0.0;
+      for (int k = 0; k < dim; k++) s += dNdr[a][k] * inv[k][j];
+      dn[a * dim + j] = s;
+    }
+  double R = 1.0;
+  if (dim == 2 && J->tab.axi) {
+    R = 0.0;
+    for (int a = 0; a < nn; a++) R += Np[a] * J->x[cn[a]];
+  }
+  double C[6][6];
+  hooke(J, C);
+  for (int k = 0; k < nd * nd; k++) K[k] = 0.0;
+  add_point_stiffness(J, C, dn, (double)nn * det, Np, R, K);
+  return true;
+}
+
+/* Model::computePowerIterationTimeStep (Model.C:942-1009): element
+ * stiffness matrices (full integration, or for PowerIterationUnder the
+ * one-point under-integration), then iterations of
  * v <- M^-1 (v + sum_e K_e v_e) (Vector::gatherFrom accumulates into the
  * iterate, Vector.C:893-945; MatrixDiag::divideBy, MatrixDiag.C:655-670),
  * fmax = max |v|, v <- v / fmax, until |fmax - fmax_prev| / fmax < 1e-4
  * (Model.h:46-48).  The iterate and fmax_prev persist between calls; the
  * first call starts from v_0 = 1, v_i = v_{i-1} - 2 / (n - 1).  Returns the
- * maximum frequency sqrt(fmax), or a negative value after 1000 iterations. */
+ * maximum frequency sqrt(fmax), a negative value after 1000 iterations, or
+ * 0 for a negative under-integration Jacobian. */
 double power_iteration(OracleJob *J) {
   const int nd = J->nn * J->dim;
   const int64_t ndof = (int64_t)J->dim * J->N;
   std::vector<double> Ke((size_t)J->NE * nd * nd);
+  if (J->prm.dt_method == DNLG_DT_POWER_ITERATION_UNDER) {
+    bool ok = true;
+#pragma omp parallel for schedule(static) num_threads(J->threads) reduction(&& : ok)
+    for (int64_t e = 0; e < J->NE; e++) ok = element_stiffness_under(J, e, &Ke[(size_t)e * nd * nd]) && ok;
+    if (!ok) return 0.0;
+  } else {
 #pragma omp parallel for schedule(static) num_threads(J->threads)
-  for (int64_t e = 0; e < J->NE; e++) element_stiffness(J, e, &Ke[(size_t)e * nd * nd]);
+    for (int64_t e = 0; e < J->NE; e++) element_stiffness(J, e, &Ke[(size_t)e * nd * nd]);
+  }
   if (J->pi_fmax == 0.0 || (int64_t)J->pi_ev.size() != ndof) {
     J->pi_ev.assign(ndof, 0.0);
     J->pi_ev[0] = 1.0;
@@ -1156,8 +1241,12 @@ double power_iteration(OracleJob *J) {
 bool compute_time_step(OracleJob *J, bool force) {
   if (force || (J->inc % J->prm.dt_freq == 0)) {
     double fmax;
-    if (J->prm.dt_method == DNLG_DT_POWER_ITERATION) {
+    if (J->prm.dt_method == DNLG_DT_POWER_ITERATION || J->prm.dt_method == DNLG_DT_POWER_ITERATION_UNDER) {
       fmax = power_iteration(J);
+      if (fmax == 0.0) {
+        set_err(J, DNLG_E_NEG_JACOBIAN, "negative detJ at an under-integration point");
+        return false;
+      }
       if (!(fmax > 0.0)) {
         set_err(J, DNLG_E_PI_DIVERGED, "power iteration not converged in 1000 iterations");
         return false;
@@ -1668,7 +1757,9 @@ int oracle_job_create(const dnlg_mesh *mesh, const dnlg_material *mat, const dnl
       mesh->elem_type != DNLG_EL_QUA4N_2D)
     return DNLG_E_INVALID;
   if (mesh->n_nodes <= 0 || mesh->n_elems <= 0 || !mesh->coords || !mesh->conn) return DNLG_E_INVALID;
-  if (prm->dt_method != DNLG_DT_COURANT && prm->dt_method != DNLG_DT_POWER_ITERATION) return DNLG_E_INVALID;
+  if (prm->dt_method != DNLG_DT_COURANT && prm->dt_method != DNLG_DT_POWER_ITERATION &&
+      prm->dt_method != DNLG_DT_POWER_ITERATION_UNDER)
+    return DNLG_E_INVALID;
   if (prm->stress_method != DNLG_STRESS_NR && prm->stress_method != DNLG_STRESS_DIRECT) return DNLG_E_INVALID;
   if (prm->dt_freq <= 0) return DNLG_E_INVALID;
   OracleJob *J = new OracleJob();
@@ -2149,8 +2240,10 @@ void oracle_chung_hulbert(double rho_b, double out[4]) {
 double oracle_ramp_value(int type, double mn, double mx, double x) { return ramp_value(type, mn, mx, x); }
 
 /* Elastic stiffness matrix of one element (node coordinates xyz[a * 3 + c]),
- * row-major (nn dim) x (nn dim), on its Jacobian; 0, or DNLG_E_NEG_JACOBIAN. */
-int oracle_element_stiffness(int type, const double *xyz, double young, double poisson, double *K) {
+ * row-major (nn dim) x (nn dim), on its Jacobian (under != 0: the one-point
+ * under-integrated matrix of PowerIterationUnder); 0, or
+ * DNLG_E_NEG_JACOBIAN. */
+int oracle_element_stiffness(int type, const double *xyz, double young, double poisson, int under, double *K) {
   OracleJob J;
   J.tab = make_tables(type);
   J.dim = J.tab.dim;
@@ -2169,6 +2262,7 @@ int oracle_element_stiffness(int type, const double *xyz, double young, double p
   J.detJ0.assign(J.nip, 0.0);
   J.radius.assign(J.nip, 0.0);
   J.dN.assign((size_t)J.nip * J.nn * J.dim, 0.0);
+  if (under) return element_stiffness_under(&J, 0, K) ? DNLG_OK : DNLG_E_NEG_JACOBIAN;
   if (!element_jacobian(&J, 0, false)) return DNLG_E_NEG_JACOBIAN;
   element_stiffness(&J, 0, K);
   return DNLG_OK;

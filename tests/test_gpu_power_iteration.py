@@ -99,3 +99,35 @@ def test_taylor_axi_sample_with_its_power_iteration_step_on_gpu(ctx, prec):
     print(f"{prec}: L_f={L_f:.3f} mm D_f={D_f:.3f} mm")
     assert L_f == pytest.approx(26.52, rel=1e-3)
     assert D_f == pytest.approx(11.15, rel=1e-3)
+
+
+PIU = dnl.Explicit.PowerIterationUnder
+
+
+@pytest.mark.parametrize("kind", ["axi", "3d", "plane"])
+def test_pi_under(ctx, kind):
+    """PowerIterationUnder (Solver.C:212-216: the power iteration on the
+    one-point under-integrated element stiffness) against the oracle, the
+    step recomputed every increment, FP64."""
+    if kind == "axi":
+        m = samples.taylor_axi(t_end=10e-6, freq=1, dt_method=PIU, context=ctx)
+    elif kind == "3d":
+        m = samples.taylor_3d(n=4, nz=12, t_end=10e-6, freq=1, dt_method=PIU, context=ctx)
+    else:
+        m = samples.necking_plane(nx=8, ny=24, t_end=10e-6, pull=4.0, freq=1, dt_method=PIU, context=ctx)
+    o = run_both(m, 10e-6)
+    compare(m, o)
+
+
+def test_pi_under_mixed_and_larger_step(ctx):
+    """The under-integrated stiffness leaves out the hourglass modes, so its
+    highest frequency is lower and the step larger than the fully
+    integrated one's on the same mesh; the mixed path against the oracle."""
+    m = samples.taylor_axi(t_end=20e-6, freq=1, dt_method=PIU, precision="mixed", track_tensors=False,
+                           context=ctx)
+    full = samples.taylor_axi(t_end=20e-6, freq=1, dt_method=PI, precision="mixed", track_tensors=False,
+                              context=ctx)
+    o = run_both(m, 20e-6)
+    compare_mixed(m, o, "mixed")
+    full.solve()
+    assert m.getScalars()[2] < full.getScalars()[2]

@@ -10,7 +10,7 @@ import dnlPython as dnl
 from dnlPython import samples
 from oracle.bridge import F_PLASTIC_STRAIN, F_TEMPERATURE, OracleJob, element_stiffness
 
-HEX, QUAD_AXI = 4, 3
+HEX, QUAD_AXI, QUAD_PLANE = 4, 3, 1
 E, NU = 200000.0, 0.3
 FCT = E / ((1 + NU) * (1 - 2 * NU))
 
@@ -89,15 +89,16 @@ def test_axi_stiffness_closed_forms():
     np.testing.assert_allclose(f[:, 1], fz, rtol=1e-12)
 
 
-def _assembled_operator(a, mass):
+def _assembled_operator(a, mass, under=False):
     """M^-1 (I + K) in the reference's DOF order (node * dim + d), from the
-    oracle's element matrices at the initial coordinates."""
+    oracle's element matrices at the initial coordinates (under: the
+    one-point under-integrated ones)."""
     elem_type = a.elem_type
     dim = 3 if elem_type == HEX else 2
     n = a.coords.shape[1]
     K = np.zeros((dim * n, dim * n))
     for conn in a.conn:
-        Ke = element_stiffness(elem_type, a.coords[:, conn].T, a.material.young, a.material.poisson)
+        Ke = element_stiffness(elem_type, a.coords[:, conn].T, a.material.young, a.material.poisson, under=under)
         dofs = (np.asarray(conn)[:, None] * dim + np.arange(dim)[None, :]).ravel()
         K[np.ix_(dofs, dofs)] += Ke
     m = np.repeat(mass, dim)
@@ -169,3 +170,156 @@ def test_taylor_axi_sample_with_its_power_iteration_step():
     assert D_f == pytest.approx(11.15, rel=1e-3)
     assert ep == pytest.approx(1.78, rel=1e-2)
     assert T == pytest.approx(582.34, rel=2e-3)
+
+
+# ---- PowerIterationUnder: the one-point under-integrated stiffness ---------
+# (Model.C:942-1009 with underIntegration = true; Element3D.C:118-167,
+# ElementPlane.C:104-160, the centre point of ElHex8N3D.C:233-252 and
+# ElQua4NAx.C:112-124.)  Pinned against the strain at the element centre
+# formed by central differences of the element's multilinear displacement
+# interpolant (exact for a multilinear function), independently of the
+# oracle's shape tables: u^T K u = V_w eps_c^T C eps_c, with V_w the
+# centre rule's weighted volume (8 det J for a box = its volume; 2 pi r_c
+# times the area for an axisymmetric rectangle).
+
+def _hooke(dim_kind):
+    """Material.C:98-152 with the reference's (1 - 2 nu) shear diagonal:
+    rows (xx, yy, zz, xy, yz, xz) in 3D, (rr, zz, rz, tt) axisymmetric."""
+    if dim_kind == "3d":
+        C = np.zeros((6, 6))
+        C[:3, :3] = NU
+        np.fill_diagonal(C[:3, :3], 1 - NU)
+        C[3, 3] = C[4, 4] = C[5, 5] = 1 - 2 * NU
+    else:
+        C = np.array([[1 - NU, NU, 0, NU], [NU, 1 - NU, 0, NU], [0, 0, 1 - 2 * NU, 0], [NU, NU, 0, 1 - NU]])
+    return FCT * C
+
+
+def _interp(lo, hi, nodal, pt):
+    """Multilinear interpolation over the box [lo, hi] of nodal values given
+    at the box corners in the reference's node numbering."""
+    dim = len(lo)
+    signs = ([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1], [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]]
+             if dim == 3 else [[-1, -1], [1, -1], [1, 1], [-1, 1]])
+    t = (2 * (np.asarray(pt) - lo) / (np.asarray(hi) - lo)) - 1
+    w = [np.prod([(1 + s[d] * t[d]) / 2 for d in range(dim)]) for s in signs]
+    return np.tensordot(w, nodal, axes=1)
+
+
+def _centre_strain(lo, hi, u, axi):
+    lo, hi = np.asarray(lo, float), np.asarray(hi, float)
+    c = 0.5 * (lo + hi)
+    dim = len(lo)
+    grad = np.zeros((dim, dim))  # grad[i, j] = du_i / dx_j
+    for j in range(dim):
+        h = 1e-3 * (hi[j] - lo[j])
+        e = np.zeros(dim)
+        e[j] = h
+        grad[:, j] = (_interp(lo, hi, u, c + e) - _interp(lo, hi, u, c - e)) / (2 * h)
+    if dim == 3:
+        return np.array([grad[0, 0], grad[1, 1], grad[2, 2], grad[0, 1] + grad[1, 0], grad[1, 2] + grad[2, 1],
+                         grad[0, 2] + grad[2, 0]])
+    uc = _interp(lo, hi, u, c)
+    return np.array([grad[0, 0], grad[1, 1], grad[0, 1] + grad[1, 0], uc[0] / c[0] if axi else 0.0])
+
+
+def test_under_stiffness_hex_energy_rank_and_hourglass():
+    """Box hex: u^T K u = V eps_c^T C eps_c for random u; K symmetric, PSD
+    of rank 6 (one point, six strain components: six rigid modes and
+    twelve hourglass modes in its null space), the hourglass vectors
+    (products of two or three local signs, per direction) among them while
+    the fully integrated matrix resists them; on linear fields the two
+    matrices agree (the centre rule is exact for constant strain on a
+    parallelepiped)."""
+    rng = np.random.default_rng(3)
+    lo, hi = np.array([0.3, -0.2, 0.1]), np.array([2.3, 2.8, 5.1])
+    X = box(*(hi - lo)) + lo
+    K = element_stiffness(HEX, X, E, NU, under=True)
+    Kf = element_stiffness(HEX, X, E, NU)
+    V = np.prod(hi - lo)
+    C = _hooke("3d")
+    for _ in range(5):
+        u = rng.standard_normal((8, 3))
+        eps = _centre_strain(lo, hi, u, False)
+        assert u.ravel() @ K @ u.ravel() == pytest.approx(V * eps @ C @ eps, rel=1e-8)
+    assert np.abs(K - K.T).max() <= 1e-12 * np.abs(K).max()
+    ev = np.linalg.eigvalsh(0.5 * (K + K.T))
+    assert ev.min() > -1e-9 * ev.max()
+    assert np.sum(ev > 1e-9 * ev.max()) == 6
+    s = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1], [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]])
+    for h in (s[:, 0] * s[:, 1], s[:, 1] * s[:, 2], s[:, 0] * s[:, 2], s[:, 0] * s[:, 1] * s[:, 2]):
+        for d in range(3):
+            u = np.zeros((8, 3))
+            u[:, d] = h
+            assert np.abs(K @ u.ravel()).max() <= 1e-10 * np.abs(K).max()
+            assert np.abs(Kf @ u.ravel()).max() > 1e-3 * np.abs(Kf).max()
+    # a distorted parallelepiped (affine map of the box) and a linear field
+    A = np.array([[1.0, 0.2, -0.1], [0.1, 0.9, 0.3], [-0.2, 0.1, 1.1]])
+    Xp = X @ A.T
+    G = rng.standard_normal((3, 3))
+    u = (Xp @ G.T).ravel()
+    Ku = element_stiffness(HEX, Xp, E, NU, under=True) @ u
+    np.testing.assert_allclose(Ku, element_stiffness(HEX, Xp, E, NU) @ u, rtol=0, atol=1e-9 * np.abs(Ku).max())
+
+
+@pytest.mark.parametrize("axi", [True, False])
+def test_under_stiffness_quad_energy_and_rank(axi):
+    """Axisymmetric ring / plane-strain rectangle: u^T K u = V_w eps_c^T C
+    eps_c (axisymmetric: V_w = 2 pi r_c x area, the hoop strain u_r / r at
+    the centre; plane strain: the area, unit thickness); rank 4 / 3; an
+    axial translation (and for plane strain a rigid motion) is a null
+    mode."""
+    rng = np.random.default_rng(5)
+    lo, hi = np.array([1.5, 0.4]), np.array([2.5, 1.9])
+    X = np.array([[lo[0], lo[1], 0], [hi[0], lo[1], 0], [hi[0], hi[1], 0], [lo[0], hi[1], 0]])
+    K = element_stiffness(QUAD_AXI if axi else QUAD_PLANE, X, E, NU, under=True)
+    area = np.prod(hi - lo)
+    Vw = 2 * np.pi * 0.5 * (lo[0] + hi[0]) * area if axi else area
+    C = _hooke("axi")
+    if not axi:
+        C = C[:3, :3]
+    for _ in range(5):
+        u = rng.standard_normal((4, 2))
+        eps = _centre_strain(lo, hi, u, axi)
+        if not axi:
+            eps = eps[:3]
+        assert u.ravel() @ K @ u.ravel() == pytest.approx(Vw * eps @ C @ eps, rel=1e-8)
+    ev = np.linalg.eigvalsh(0.5 * (K + K.T))
+    assert ev.min() > -1e-9 * ev.max()
+    assert np.sum(ev > 1e-9 * ev.max()) == (4 if axi else 3)
+    u = np.zeros(8)
+    u[1::2] = 1.0
+    assert np.abs(K @ u).max() <= 1e-10 * np.abs(K).max()
+    if not axi:
+        u = np.cross([0, 0, 1.0], X)[:, :2].ravel()
+        assert np.abs(K @ u).max() <= 1e-10 * np.abs(K).max() * 3
+
+
+@pytest.mark.parametrize("kind", ["axi", "3d"])
+def test_power_iteration_under_converges_to_dominant_eigenvalue(kind):
+    """PowerIterationUnder's pre-loop step: the maximum approaches the
+    dominant eigenvalue of M^-1 (I + K_under) from below, below the fully
+    integrated operator's (the hourglass stiffness is left out), and the
+    step is safety * Omega_s / sqrt(fmax).  The under-integrated operator's
+    leading eigenvalues lie closer together, so the reference's test (the
+    maximum changed by less than 1e-4 in one iteration) stops the cold start
+    further below the eigenvalue (2.7 % on the axisymmetric mesh); warm
+    starts over the next increments close the gap."""
+    mk = (lambda **k: samples.taylor_axi(nr=3, nz=8, **k)) if kind == "axi" else \
+        (lambda **k: samples.taylor_3d(n=2, nz=4, **k))
+    m = mk(dt_method=dnl.Explicit.PowerIterationUnder)
+    a = m.descriptors()
+    o = OracleJob(a)
+    assert o.solve(0.0) == 0
+    fmax, iters = o.pi_state()
+    lam = np.abs(np.linalg.eigvals(_assembled_operator(a, o.mass(), under=True))).max()
+    lam_full = np.abs(np.linalg.eigvals(_assembled_operator(a, o.mass()))).max()
+    print(f"{kind}: fmax={fmax:.6e} lambda_under={lam:.6e} lambda_full={lam_full:.6e} iterations={iters}")
+    assert iters > 1
+    assert fmax <= lam * (1 + 1e-12)
+    assert fmax == pytest.approx(lam, rel=5e-2)
+    assert lam < lam_full
+    _, dt, _ = o.scalars()
+    rb = 0.8182
+    omega_s = np.sqrt((12 * (1 + rb) ** 3 * (2 - rb)) / (10 + 15 * rb - rb * rb + rb ** 3 - rb ** 4))
+    assert dt == pytest.approx(0.9 * omega_s / np.sqrt(fmax), rel=1e-14)
