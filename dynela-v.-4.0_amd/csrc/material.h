@@ -336,7 +336,7 @@ __device__ __forceinline__ void law_f32(const LawF &L, float epsp, float gamma, 
 
 // RampFunction::getValue (RampFunction.C:87-131) with the coefficients of
 // RampFunction::setFunction (RampFunction.C:40-84).
-__device__ __forceinline__ double ramp_value(int type, double mn, double mx, double x) {
+__device__ __noinline__ double ramp_value(int type, double mn, double mx, double x) {
   if (type == DNLG_RAMP_NONE) return 0.0;
   if ((x < mn) || (x > mx)) return 0.0;
   const double mm = mx - mn;
