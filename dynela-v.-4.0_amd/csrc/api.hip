@@ -135,9 +135,9 @@ int64_t incidence_slots(const int32_t *conn, int64_t N, int64_t NE, int nn, int3
 // Device bytes of a job.  n_slots < 0: the incidence slots are not known
 // (no connectivity at hand) and are bounded by NE nn + N nn.
 int64_t footprint(int type, int64_t N, int64_t NE, int prec, int track, int64_t n_slots = -1) {
-  const int dim = type == DNLG_EL_HEX8N_3D ? 3 : 2;
-  const int nn = type == DNLG_EL_HEX8N_3D ? 8 : 4;
-  const int64_t G = NE * nn;
+  const int dim = type_dim(type);
+  const int nn = type_nodes(type);
+  const int64_t G = NE * type_ips(type);
   if (n_slots < 0) n_slots = NE * nn + N * nn;
   int64_t b = 0;
   b += N * dim * 8 * 8;            // x, x0, fu/fv/fa x2
@@ -357,7 +357,8 @@ int write_vtk(dnlg_job *job, double time, int f0) {
     }
     // vtk cell types of ElQua4NAx (9, quad) and ElHex8N3D (12, hexahedron)
     txt += "\nCELL_TYPES " + std::to_string(NE) + "\n";
-    const char *ct = job->type == DNLG_EL_HEX8N_3D ? "12\n" : "9\n";
+    // VTK cell types (ElementData's VTK type: hexahedron 12, quad 9, tetra 10)
+    const char *ct = job->type == DNLG_EL_HEX8N_3D ? "12\n" : (job->type == DNLG_EL_TET4N_3D ? "10\n" : "9\n");
     for (int64_t e = 0; e < NE; e++) txt += ct;
     std::fwrite(txt.data(), 1, txt.size(), fp);
   }
@@ -520,7 +521,7 @@ int dnlg_ctx_destroy(dnlg_ctx *ctx) {
 const char *dnlg_last_error(const dnlg_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 int64_t dnlg_footprint_bytes(int elem_type, int64_t n_nodes, int64_t n_elems, int precision, int track_tensors) {
-  if (elem_type != DNLG_EL_HEX8N_3D && elem_type != DNLG_EL_QUA4N_AX && elem_type != DNLG_EL_QUA4N_2D) return -1;
+  if (!type_known(elem_type)) return -1;
   return footprint(elem_type, n_nodes, n_elems, precision, track_tensors);
 }
 
@@ -528,8 +529,8 @@ int64_t dnlg_model_footprint_bytes(const dnlg_mesh *mesh, const dnlg_material *m
                                    const dnlg_solver_params *prm) {
   if (!mesh || !mat || !prm) return -1;
   const int type = mesh->elem_type;
-  if (type != DNLG_EL_HEX8N_3D && type != DNLG_EL_QUA4N_AX && type != DNLG_EL_QUA4N_2D) return -1;
-  const int nn = type == DNLG_EL_HEX8N_3D ? 8 : 4;
+  if (!type_known(type)) return -1;
+  const int nn = type_nodes(type);
   int64_t n_slots = -1;
   if (mesh->conn && mesh->n_nodes > 0 && mesh->n_elems > 0) {
     bool in_range = true;
@@ -553,9 +554,16 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
                     const dnlg_solver_params *prm, dnlg_job **out) {
   if (!ctx || !mesh || !mat || !prm || !out) return fail(ctx, DNLG_E_INVALID, "null argument");
   *out = nullptr;
-  if (mesh->elem_type != DNLG_EL_QUA4N_AX && mesh->elem_type != DNLG_EL_HEX8N_3D &&
-      mesh->elem_type != DNLG_EL_QUA4N_2D)
-    return fail(ctx, DNLG_E_INVALID, "unsupported element type");
+  if (!type_known(mesh->elem_type)) return fail(ctx, DNLG_E_INVALID, "unsupported element type");
+  if (mesh->elem_type == DNLG_EL_TET4N_3D) {
+    // ElTet4N3D::getCharacteristicLength returns 0 (ElTet4N3D.h:45): the
+    // reference's Courant step of a tetrahedral model is 0
+    if (prm->dt_method == DNLG_DT_COURANT)
+      return fail(ctx, DNLG_E_INVALID,
+                  "ElTet4N3D has no characteristic length (0 in the reference): use a power-iteration time step");
+    if (prm->precision != DNLG_PREC_FP64)
+      return fail(ctx, DNLG_E_INVALID, "ElTet4N3D runs in FP64 precision only");
+  }
   if (mesh->n_nodes <= 0 || mesh->n_elems <= 0 || !mesh->coords || !mesh->conn)
     return fail(ctx, DNLG_E_INVALID, "empty mesh");
   // node, integration-point and incidence-slot indices are 32-bit (slots:
@@ -612,9 +620,9 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
   dnlg_job *job = new dnlg_job();
   job->ctx = ctx;
   const int type = mesh->elem_type;
-  const int dim = type == DNLG_EL_HEX8N_3D ? 3 : 2;
-  const int nn = type == DNLG_EL_HEX8N_3D ? 8 : 4;
-  const int64_t N = mesh->n_nodes, NE = mesh->n_elems, G = NE * nn;
+  const int dim = type_dim(type);
+  const int nn = type_nodes(type);
+  const int64_t N = mesh->n_nodes, NE = mesh->n_elems, G = NE * type_ips(type);
   job->type = type;
   job->dim = dim;
   job->nn = nn;
@@ -717,7 +725,7 @@ int dnlg_job_create(dnlg_ctx *ctx, const dnlg_mesh *mesh, const dnlg_material *m
   H.type = type;
   H.dim = dim;
   H.nn = nn;
-  H.nip = nn;
+  H.nip = type_ips(type);
   H.N = N;
   H.NE = NE;
   H.G = G;
@@ -908,7 +916,7 @@ int dnlg_job_sizes(const dnlg_job *job, int64_t *n_nodes, int64_t *n_elems, int3
   if (!job) return DNLG_E_INVALID;
   if (n_nodes) *n_nodes = job->N;
   if (n_elems) *n_elems = job->NE;
-  if (n_ip) *n_ip = job->nn;
+  if (n_ip) *n_ip = type_ips(job->type);
   if (dim) *dim = job->dim;
   return DNLG_OK;
 }
@@ -1038,7 +1046,7 @@ int dnlg_job_energies(dnlg_job *job, double out[4]) {
     ctx->energy_ws_doubles = need;
   }
   double *d_out = ctx->energy_ws + (need - 4);
-  launch_energies(job->dev, job->N, job->G, job->dim, job->type == DNLG_EL_QUA4N_AX, ctx->energy_ws, d_out,
+  launch_energies(job->dev, job->N, job->G, job->dim, type_vol_scale(job->type), ctx->energy_ws, d_out,
                   ctx->stream);
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipMemcpyAsync(out, d_out, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -1184,7 +1192,7 @@ int dnlg_job_energies_static(dnlg_job *job, double out[4]) {
   double *ws = energy_scratch(ctx->device);
   if (!ws) return fail(ctx, DNLG_E_STATE, "dnlg_job_energies_static: scratch of this device not set up");
   double *d_out = ws + 6 * (int64_t)kEnergyScratchBlocks;
-  launch_energies(job->dev, job->N, job->G, job->dim, job->type == DNLG_EL_QUA4N_AX, ws, d_out, ctx->stream);
+  launch_energies(job->dev, job->N, job->G, job->dim, type_vol_scale(job->type), ws, d_out, ctx->stream);
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipMemcpyAsync(out, d_out, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -1252,7 +1260,7 @@ int dnlg_job_add_history(dnlg_job *job, const char *path, double start_time, dou
     return fail(ctx, DNLG_E_INVALID, "invalid history save times");
   if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, DNLG_E_HIP, "hipSetDevice");
   const int fid = (int)job->hist.size();
-  const int nip = job->nn;
+  const int nip = type_ips(job->type);
   std::vector<HistItem> add;
   std::vector<int64_t> ip_nodes;  // nodes of node-from-IP items, per item (-1: none)
   bool need_ke = false, need_rho = false;

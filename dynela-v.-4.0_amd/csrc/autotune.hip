@@ -98,7 +98,7 @@ extern "C" {
 
 int dnlg_schedule_valid(int elem_type, int precision, int law, const dnlg_schedule *sched) {
   if (!sched) return 0;
-  if (elem_type != DNLG_EL_HEX8N_3D && elem_type != DNLG_EL_QUA4N_AX && elem_type != DNLG_EL_QUA4N_2D) return 0;
+  if (!type_known(elem_type)) return 0;
   const int v = precision == DNLG_PREC_MIXED ? (law == DNLG_LAW_TABULAR || law == DNLG_LAW_ANN2SIG ? kVarMixedX : kVarMixed)
                 : precision == DNLG_PREC_MIXED_RR32 ? kVarMixedRR32
                 : (law == DNLG_LAW_ANN2SIG ? kVarFP64Ann : kVarFP64);

@@ -24,10 +24,8 @@ struct TaskList {
 };
 
 // Element types, indexed as the per-type task lists below.
-constexpr int kNumTypes = 3;
-constexpr int kTypes[kNumTypes] = {DNLG_EL_QUA4N_AX, DNLG_EL_HEX8N_3D, DNLG_EL_QUA4N_2D};
-inline int type_dim(int type) { return type == DNLG_EL_HEX8N_3D ? 3 : 2; }
-inline int type_nodes(int type) { return type == DNLG_EL_HEX8N_3D ? 8 : 4; }
+constexpr int kNumTypes = 4;
+constexpr int kTypes[kNumTypes] = {DNLG_EL_QUA4N_AX, DNLG_EL_HEX8N_3D, DNLG_EL_QUA4N_2D, DNLG_EL_TET4N_3D};
 
 inline bool variant_mixed(int mx) { return mx == kVarMixed || mx == kVarMixedRR32 || mx == kVarMixedX; }
 

@@ -15,7 +15,7 @@ namespace dnlg {
 
 // ---------------------------------------------------------------------------
 // Element kernel, FP64.  One lane per integration point (NIP lanes per
-// element, NIP == NN for both element types); the element's nodes are staged
+// element, NIP == NN for the hex and quads); the element's nodes are staged
 // in LDS by its own lanes, per-IP inverse Jacobians, stresses and weights go back to
 // LDS so that lane a can sum node a's force over the IPs in the reference's
 // order (IP, then direction j).
@@ -24,11 +24,15 @@ namespace dnlg {
 // (24 KB) allows three.  Measured 11 % faster than two waves without spill.
 // ANN: the variant for jobs whose law is DNLG_LAW_ANN2SIG (its network
 // evaluation keeps per-lane arrays that the other laws' variants do without).
-template <int NN, int DIM, int BLOCK, bool AXI, bool ANN, int WAVES>
+// TET (NN = 4, DIM = 3): the 4-node tetrahedron of ElTet4N3D.C, one
+// integration point (ElTet4N3D.C:62-71: the centroid, weight 1/6, linear
+// shape functions): four lanes per element stage its nodes and sum its
+// nodal forces; lane 0 computes the integration point.
+template <int NN, int DIM, int BLOCK, bool AXI, bool ANN, int WAVES, bool TET = false>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_elem_fp64(
     JobDev *const *jobs, const Task *tasks, int cs, int derive) {
-  constexpr int NIP = NN;
-  constexpr int EPB = BLOCK / NIP;
+  constexpr int NIP = TET ? 1 : NN;  // integration points per element
+  constexpr int EPB = BLOCK / NN;    // elements per block (NN lanes each)
   __shared__ double s_x[EPB][NN][DIM];
   __shared__ double s_u[EPB][NN][DIM];
   __shared__ double s_inv[EPB][NIP][DIM][DIM];  // J^-1 per IP, for the force phase
@@ -55,10 +59,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) 
   const Control cc = cycle_control(J, C, cs, derive, tk.blk);
   if (!cc.running) return;
   const int par = (int)(cc.inc & 1);
-  const int le = threadIdx.x / NIP;
-  const int p = threadIdx.x % NIP;
+  const int le = threadIdx.x / NN;
+  const int p = threadIdx.x % NN;
   const int64_t e = tk.first + le;
   const bool valid = e < tk.end;
+  const bool ipl = valid && p < NIP;  // this lane computes integration point p
   const int f0 = cc.f0;
 
   // stage the element's nodes: lane p brings node p
@@ -78,7 +83,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) 
   double dN[NN][DIM];
   double detJ = 1.0, radius = 1.0;
   double dE[6] = {0, 0, 0, 0, 0, 0}, R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-  if (valid) {
+  if (ipl) {
     // Jacobian, inverse, B-matrix (Element3D.C:64-115, ElementPlane.C:46-102)
     double Jm[DIM][DIM];
 #pragma unroll
@@ -90,7 +95,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) 
 #pragma unroll
       for (int i = 0; i < DIM; i++)
 #pragma unroll
-        for (int j = 0; j < DIM; j++) Jm[i][j] += tabD<NN>(p, a, j) * s_x[le][a][i];
+        for (int j = 0; j < DIM; j++) Jm[i][j] += shapeD<NN, TET>(p, a, j) * s_x[le][a][i];
     double inv[DIM][DIM];
     if constexpr (DIM == 3) {
       const double *dd = &Jm[0][0];
@@ -121,7 +126,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) 
       for (int j = 0; j < DIM; j++) {
         double s = 0.0;
 #pragma unroll
-        for (int k = 0; k < DIM; k++) s += tabD<NN>(p, a, k) * inv[k][j];
+        for (int k = 0; k < DIM; k++) s += shapeD<NN, TET>(p, a, k) * inv[k][j];
         dN[a][j] = s;
       }
 #pragma unroll
@@ -153,7 +158,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) 
   __syncthreads();
 
   double sig[6], wdj = 0.0;
-  if (valid) {
+  if (ipl) {
     // Element::computePressure (Element.C:1125-1143)
     double inc = 0.0;
 #pragma unroll
@@ -302,7 +307,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) 
       if constexpr (AXI) rr /= radius;
       gbl(C->rho)[g] = rr;
     }
-    wdj = detJ;  // Gauss weights are 1 (ElQua4NAx.C:63-110, ElHex8N3D.C:93-231)
+    // Gauss weights: 1 (ElQua4NAx.C:63-110, ElHex8N3D.C:93-231), 1/6 for the
+    // tetrahedron's point
+    wdj = TET ? ip_weight<TET>() * detJ : detJ;
     if constexpr (AXI) wdj *= k2PI * radius;
 #pragma unroll
     for (int c = 0; c < 6; c++) s_sw[le][p][c] = sig[c];
@@ -326,7 +333,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) 
       for (int j = 0; j < DIM; j++) {
         double d = 0.0;
 #pragma unroll
-        for (int k = 0; k < DIM; k++) d += tabD<NN>(q, a, k) * s_inv[le][q][k][j];
+        for (int k = 0; k < DIM; k++) d += shapeD<NN, TET>(q, a, k) * s_inv[le][q][k][j];
         dNq[j] = d;
       }
 #pragma unroll
@@ -367,6 +374,13 @@ void launch_fp64_type(int block, int waves, JobDev *const *jobs, const Task *tas
 void launch_elem_fp64(int type, int block, int waves, JobDev *const *jobs, const Task *tasks, int nblk, int cs,
                       int derive, hipStream_t s, const hipEvent_t *ev, bool ann) {
   if (nblk <= 0) return;
+  if (type == DNLG_EL_TET4N_3D) {  // default schedule only: 64 tetrahedra per block
+    if (ann)
+      launch_elem(k_elem_fp64<4, 3, 256, false, true, kFP64Waves, true>, 256, nblk, s, ev, jobs, tasks, cs, derive);
+    else
+      launch_elem(k_elem_fp64<4, 3, 256, false, false, kFP64Waves, true>, 256, nblk, s, ev, jobs, tasks, cs, derive);
+    return;
+  }
   if (ann) {  // the network-law variant: default schedule only
     if (type == DNLG_EL_HEX8N_3D)
       launch_elem(k_elem_fp64<8, 3, 128, false, true, kFP64Waves>, 128, nblk, s, ev, jobs, tasks, cs, derive);

@@ -69,8 +69,8 @@ __device__ __forceinline__ void block_merge(Acc (&s)[kQ][kEnergyThreads], Acc (&
 
 // Block b sums nodes [b pn, (b+1) pn) and integration points
 // [b pg, (b+1) pg), thread t every kEnergyThreads-th item from its first;
-// writes (sum, error) x 3 to part[6 b ..].  vol_scale: 2 pi (axisymmetric)
-// or 1.
+// writes (sum, error) x 3 to part[6 b ..].  vol_scale: the Gauss weight
+// (1, the tetrahedron's 1/6) times 2 pi for the axisymmetric quad.
 template <int DIM>
 __global__ __launch_bounds__(kEnergyThreads) void k_energy_partials(const JobDev *J, int64_t pn, int64_t pg,
                                                                    double vol_scale, double *part) {
@@ -146,11 +146,10 @@ int energy_blocks(int64_t N, int64_t G) {
   return (int)(nb < 1 ? 1 : nb);
 }
 
-void launch_energies(JobDev *job, int64_t N, int64_t G, int dim, bool axi, double *part, double *out,
+void launch_energies(JobDev *job, int64_t N, int64_t G, int dim, double scale, double *part, double *out,
                      hipStream_t s) {
   const int nb = energy_blocks(N, G);
   const int64_t pn = (N + nb - 1) / nb, pg = (G + nb - 1) / nb;
-  const double scale = axi ? k2PI : 1.0;
   if (dim == 3)
     hipLaunchKernelGGL(k_energy_partials<3>, dim3(nb), dim3(kEnergyThreads), 0, s, job, pn, pg, scale, part);
   else

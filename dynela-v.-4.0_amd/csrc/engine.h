@@ -30,6 +30,25 @@ constexpr int kItMax = 250;
 
 __host__ __device__ __forceinline__ double sqd(double a) { return a * a; }
 
+// Element types: spatial dimensions, nodes and integration points.
+__host__ __device__ __forceinline__ bool type_known(int type) {
+  return type == DNLG_EL_HEX8N_3D || type == DNLG_EL_QUA4N_AX || type == DNLG_EL_QUA4N_2D || type == DNLG_EL_TET4N_3D;
+}
+__host__ __device__ __forceinline__ int type_dim(int type) {
+  return (type == DNLG_EL_HEX8N_3D || type == DNLG_EL_TET4N_3D) ? 3 : 2;
+}
+__host__ __device__ __forceinline__ int type_nodes(int type) { return type == DNLG_EL_HEX8N_3D ? 8 : 4; }
+__host__ __device__ __forceinline__ int type_ips(int type) {
+  return type == DNLG_EL_HEX8N_3D ? 8 : (type == DNLG_EL_TET4N_3D ? 1 : 4);
+}
+
+// Factor from an integration point's reference det J to its volume: the
+// Gauss weight (1, the tetrahedron's 1/6) times 2 pi for the axisymmetric
+// quad, whose detJ0 carries the radius.
+__host__ __device__ __forceinline__ double type_vol_scale(int type) {
+  return type == DNLG_EL_QUA4N_AX ? k2PI : (type == DNLG_EL_TET4N_3D ? 1.0 / 6 : 1.0);
+}
+
 // Time-step methods that run the power iteration (fully or under-integrated
 // element stiffness).
 __host__ __device__ __forceinline__ bool is_pi(int dt_method) {

@@ -65,7 +65,11 @@ void upload_extrapolation() {
 
 namespace {
 
+// nip == 1: the tetrahedron (ElTet4N3D.C:33-60), whose nodes' vectors hold
+// four entries while it has one integration point: the first entry,
+// (1 + 3 sqrt5)/4 at node 0 and (1 - sqrt5)/4 at the others, is the one used.
 __device__ __forceinline__ double extrap(int nip, int node, int p) {
+  if (nip == 1) return node == 0 ? 1.9270509831248424 : -0.30901699437494745;
   return nip == 8 ? c_hexE[node][p] : c_quaE[node][p];
 }
 

@@ -82,7 +82,7 @@ constexpr int kEnergyScratchBlocks = 4096;
 constexpr int kMaxDevices = 64;
 int energy_scratch_init(int device);
 double *energy_scratch(int device);
-void launch_energies(JobDev *job, int64_t N, int64_t G, int dim, bool axi, double *part, double *out,
+void launch_energies(JobDev *job, int64_t N, int64_t G, int dim, double vol_scale, double *part, double *out,
                      hipStream_t s);
 
 }  // namespace dnlg

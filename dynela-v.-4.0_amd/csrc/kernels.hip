@@ -455,7 +455,7 @@ __global__ void k_pi_begin(JobDev *const *jobs, int n_jobs, int cs, int force) {
 // hoop row), and B^T C B v w detJ (x 2 pi r), on the Jacobian of the current
 // coordinates (Explicit.C:245-249 computes it before the time step).  Equal
 // to the reference's assembled K_e times v up to rounding.
-template <int NN, int DIM, bool AXI>
+template <int NN, int DIM, bool AXI, bool TET = false>
 __global__ __launch_bounds__(256) void k_pi_elem(JobDev *const *jobs, const Task *tasks) {
   const Task tk = load_task(tasks);
   JobDev *J = tk.jd;
@@ -485,11 +485,13 @@ __global__ __launch_bounds__(256) void k_pi_elem(JobDev *const *jobs, const Task
   // derivatives +-1 / NN by the nodes' local signs (ElHex8N3D.C:233-252,
   // ElQua4NAx.C:112-124, ElQua4N2D.C:115-127), on the current coordinates
   // (Element3D::computeUnderJacobian, ElementPlane::computeUnderJacobian)
-  const bool under = C->dt_method == DNLG_DT_POWER_ITERATION_UNDER;
-  const int npts = under ? 1 : NN;
-  auto dshape = [&](int p, int a, int k) { return under ? node_sign<NN>(a, k) / NN : tabD<NN>(p, a, k); };
-  auto shape = [&](int p, int a) { return under ? 1.0 / NN : tabN<NN>(p, a); };
-  const double wgt = under ? (double)NN : 1.0;  // full integration: Gauss weights 1
+  // The tetrahedron's one point is also its under-integration point
+  // (ElTet4N3D.C:73-82), with weight 1/6.
+  const bool under = !TET && C->dt_method == DNLG_DT_POWER_ITERATION_UNDER;
+  const int npts = (TET || under) ? 1 : NN;
+  auto dshape = [&](int p, int a, int k) { return under ? node_sign<NN>(a, k) / NN : shapeD<NN, TET>(p, a, k); };
+  auto shape = [&](int p, int a) { return under ? 1.0 / NN : shapeN<NN, TET>(p, a); };
+  const double wgt = TET ? ip_weight<TET>() : (under ? (double)NN : 1.0);  // Gauss weights 1
   for (int p = 0; p < npts; p++) {
     double Jm[DIM][DIM];
 #pragma unroll
@@ -745,7 +747,7 @@ __global__ void k_force_dt(JobDev *const *jobs, int n_jobs) {
 // ElementPlane.C:92-100) and the element lumped masses
 // (Element::computeMassMatrix, Element.C:304-339) into elemF's first slot.
 // ---------------------------------------------------------------------------
-template <int NN, int DIM, bool AXI>
+template <int NN, int DIM, bool AXI, bool TET = false>
 __global__ __launch_bounds__(256) void k_prep(JobDev *const *jobs, const Task *tasks) {
   const Task tk = load_task(tasks);
   JobDev *J = tk.jd;
@@ -763,7 +765,8 @@ __global__ __launch_bounds__(256) void k_prep(JobDev *const *jobs, const Task *t
 #pragma unroll
   for (int a = 0; a < NN; a++) me[a] = 0.0;
   bool neg = false;
-  for (int p = 0; p < NN; p++) {
+  constexpr int NIP = TET ? 1 : NN;
+  for (int p = 0; p < NIP; p++) {
     double Jm[DIM][DIM];
 #pragma unroll
     for (int i = 0; i < DIM; i++)
@@ -774,7 +777,7 @@ __global__ __launch_bounds__(256) void k_prep(JobDev *const *jobs, const Task *t
 #pragma unroll
       for (int i = 0; i < DIM; i++)
 #pragma unroll
-        for (int jj = 0; jj < DIM; jj++) Jm[i][jj] += tabD<NN>(p, a, jj) * X[a][i];
+        for (int jj = 0; jj < DIM; jj++) Jm[i][jj] += shapeD<NN, TET>(p, a, jj) * X[a][i];
     double det;
     if constexpr (DIM == 3) {
       const double *dd = &Jm[0][0];
@@ -784,7 +787,7 @@ __global__ __launch_bounds__(256) void k_prep(JobDev *const *jobs, const Task *t
       det = Jm[0][0] * Jm[1][1] - Jm[1][0] * Jm[0][1];
     }
     neg = neg || (det < 0.0);
-    double wdj = det;
+    double wdj = TET ? ip_weight<TET>() * det : det;
     double det0 = det;
     if constexpr (AXI) {
       double r = 0.0;
@@ -793,10 +796,10 @@ __global__ __launch_bounds__(256) void k_prep(JobDev *const *jobs, const Task *t
       det0 = det * r;
       wdj *= k2PI * r;
     }
-    J->detJ0[e * NN + p] = det0;
-    if (J->detJ0f) J->detJ0f[e * NN + p] = (float)det0;
+    J->detJ0[e * NIP + p] = det0;
+    if (J->detJ0f) J->detJ0f[e * NIP + p] = (float)det0;
 #pragma unroll
-    for (int a = 0; a < NN; a++) me[a] += J->rho0 * tabN<NN>(p, a) * wdj;
+    for (int a = 0; a < NN; a++) me[a] += J->rho0 * shapeN<NN, TET>(p, a) * wdj;
   }
   if (neg) set_status(J, DNLG_E_NEG_JACOBIAN, 0);
 #pragma unroll
@@ -880,6 +883,8 @@ void launch_pi_elem(int type, JobDev *const *jobs, const Task *tasks, int nblk, 
     hipLaunchKernelGGL((k_pi_elem<8, 3, false>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
   else if (type == DNLG_EL_QUA4N_AX)
     hipLaunchKernelGGL((k_pi_elem<4, 2, true>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
+  else if (type == DNLG_EL_TET4N_3D)
+    hipLaunchKernelGGL((k_pi_elem<4, 3, false, true>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
   else
     hipLaunchKernelGGL((k_pi_elem<4, 2, false>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
 }
@@ -915,6 +920,8 @@ void launch_prep(int type, JobDev *const *jobs, const Task *tasks, int nblk, hip
     hipLaunchKernelGGL((k_prep<8, 3, false>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
   else if (type == DNLG_EL_QUA4N_AX)
     hipLaunchKernelGGL((k_prep<4, 2, true>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
+  else if (type == DNLG_EL_TET4N_3D)
+    hipLaunchKernelGGL((k_prep<4, 3, false, true>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
   else
     hipLaunchKernelGGL((k_prep<4, 2, false>), dim3(nblk), dim3(256), 0, s, jobs, tasks);
 }

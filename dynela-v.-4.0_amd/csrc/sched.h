@@ -47,10 +47,11 @@ inline int default_corr_batch(bool f32) { return f32 ? 8 : 4; }
 constexpr int kDefaultGraphCycles = 16;
 
 // Whether (block, waves) is a schedule the element kernels of this type and
-// variant are built for.  The plane-strain quad, the network-law variant and
-// the mixed table-law variant take their defaults only.
+// variant are built for.  The plane-strain quad, the tetrahedron, the
+// network-law variant and the mixed table-law variant take their defaults
+// only.
 inline bool elem_sched_valid(int type, int variant, int block, int waves) {
-  if (type == DNLG_EL_QUA4N_2D || variant == kVarFP64Ann || variant == kVarMixedX)
+  if (type == DNLG_EL_QUA4N_2D || type == DNLG_EL_TET4N_3D || variant == kVarFP64Ann || variant == kVarMixedX)
     return block == default_elem_block(type) && waves == default_waves(variant);
   const int *bl = type == DNLG_EL_HEX8N_3D ? kHexBlocks : kQuadBlocks;
   bool ok = false;

@@ -149,8 +149,8 @@ int dnlg_server_submit(dnlg_server *s, const dnlg_mesh *mesh, const dnlg_materia
   j->ticket = s->next_ticket++;
   j->footprint = fp;
   j->mesh = *mesh;
-  const int dim = mesh->elem_type == DNLG_EL_HEX8N_3D ? 3 : 2;
-  const int nn = mesh->elem_type == DNLG_EL_HEX8N_3D ? 8 : 4;
+  const int dim = type_dim(mesh->elem_type);
+  const int nn = type_nodes(mesh->elem_type);
   j->coords.assign(mesh->coords, mesh->coords + (size_t)dim * mesh->n_nodes);
   j->conn.assign(mesh->conn, mesh->conn + (size_t)nn * mesh->n_elems);
   j->mat = *mat;

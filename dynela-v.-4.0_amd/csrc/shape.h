@@ -28,6 +28,27 @@ __device__ __forceinline__ double tabD(int p, int a, int j) {
   else return c_quaD[p][a][j];
 }
 
+// Shape functions, their local derivatives and the weight of integration
+// point p: the Gauss tables of the hex and quads (weights 1), or with TET
+// the 4-node tetrahedron's single point (ElTet4N3D.C:62-71: the centroid,
+// weight 1/6, N = 1/4, derivatives -1 at node 0 and the unit vectors at
+// nodes 1-3).
+template <int NN, bool TET>
+__device__ __forceinline__ double shapeN(int p, int a) {
+  if constexpr (TET) return 0.25;
+  else return tabN<NN>(p, a);
+}
+template <int NN, bool TET>
+__device__ __forceinline__ double shapeD(int p, int a, int k) {
+  if constexpr (TET) return a == 0 ? -1.0 : (a - 1 == k ? 1.0 : 0.0);
+  else return tabD<NN>(p, a, k);
+}
+template <bool TET>
+__device__ __forceinline__ double ip_weight() {
+  if constexpr (TET) return 1.0 / 6;
+  else return 1.0;
+}
+
 // Local coordinate sign (+-1) of node a along direction k in the reference's
 // numbering (ElHex8N3D.C:30-92, ElQua4NAx.C:30-62): hex x + for nodes 1, 2,
 // 5, 6, y + for 2, 3, 6, 7, z + for 4-7; quad x + for 1, 2, y + for 2, 3.

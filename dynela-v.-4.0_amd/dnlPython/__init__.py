@@ -43,11 +43,13 @@ class Element:
     ElTri3N2D = 2
     ElQua4NAx = _lib.EL_QUA4N_AX
     ElHex8N3D = _lib.EL_HEX8N_3D
-    ElTet4N3D = 5
+    ElTet4N3D = _lib.EL_TET4N_3D
     ElTet10N3D = 6
 
 
-NODES_PER_ELEMENT = {Element.ElQua4N2D: 4, Element.ElQua4NAx: 4, Element.ElHex8N3D: 8}
+NODES_PER_ELEMENT = {Element.ElQua4N2D: 4, Element.ElQua4NAx: 4, Element.ElHex8N3D: 8, Element.ElTet4N3D: 4}
+# integration points per element (ElTet4N3D.C:62-71: one point)
+IPS_PER_ELEMENT = {Element.ElQua4N2D: 4, Element.ElQua4NAx: 4, Element.ElHex8N3D: 8, Element.ElTet4N3D: 1}
 
 
 def _field_names():
@@ -676,7 +678,7 @@ class _ElementView:
         return NODES_PER_ELEMENT[self._m._elem_type]
 
     def getNumberOfIntegrationPoints(self):
-        return NODES_PER_ELEMENT[self._m._elem_type]
+        return IPS_PER_ELEMENT[self._m._elem_type]
 
     def getIntPointValue(self, field, ip):
         """A scalar integration-point field at IP `ip` of this element."""
@@ -1169,7 +1171,7 @@ class DynELA:
             raise DnlgError(_lib.E_INVALID, f"no integration-point field {field!r}")
         f = _IP_FIELDS[key]
         ne = self.getElementsNumber()
-        nip = NODES_PER_ELEMENT[self._elem_type]
+        nip = IPS_PER_ELEMENT[self._elem_type]
         ncomp = 6 if f in _lib.TENSOR_FIELDS else 1
         out = np.zeros(ncomp * ne * nip)
         self.context.check(self.context.lib.dnlg_job_read_ip(self._job, f, _lib.as_dp(out)))
@@ -1237,7 +1239,7 @@ class DynELA:
         _lib.STATE_IP_NAMES), and strain / plastic_strain_tensor with tracked
         tensors."""
         self._require_job()
-        n, g = self.getNodesNumber(), self.getElementsNumber() * NODES_PER_ELEMENT[self._elem_type]
+        n, g = self.getNodesNumber(), self.getElementsNumber() * IPS_PER_ELEMENT[self._elem_type]
         arr, rec = _lib.state_alloc(n, g, self._solver.trackTensors)
         self.context.check(self.context.lib.dnlg_job_read_state(self._job, ctypes.byref(rec)))
         arr.update(time=rec.time, timeStep=rec.time_step, increment=rec.increment)

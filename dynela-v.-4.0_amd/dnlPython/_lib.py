@@ -26,6 +26,7 @@ STATUS_NAMES = {
 EL_QUA4N_2D = 1
 EL_QUA4N_AX = 3
 EL_HEX8N_3D = 4
+EL_TET4N_3D = 5
 LAW_JOHNSON_COOK, LAW_POWER, LAW_VOCE_LINEAR, LAW_TABULAR, LAW_ANN2SIG = 0, 1, 2, 3, 4
 ANN_MAX_HIDDEN = 64
 BC_RESTRAIN, BC_SPEED = 0, 1

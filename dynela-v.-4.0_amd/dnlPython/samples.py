@@ -14,6 +14,8 @@ Workloads
                Voce + linear hardening, one end fixed, the other pulled
   necking_axi  the same bar with 4-node axisymmetric quads
   necking_plane the same half-plane as a plane-strain plate (ElQua4N2D)
+  taylor_tet   3D impact of a square copper bar of 4-node tetrahedra
+               (ElTet4N3D, six per cell), power-iteration time step
   sweep_params seeded parameter sweep around the copper impact
   serve_params seeded serving mix (axi / 3D x impact / necking)
 """
@@ -190,6 +192,66 @@ def taylor_3d(n=20, nz=100, radius=3.2e-3, length=32.4e-3, speed=227.0, t_end=80
     bottom = BoundaryRestrain("BC_bottom")
     bottom.setValue(0, 1, 0)
     model.attachConstantBC(bottom, _set(range(1, nl + 1), "NS_Bottom"))
+    v = BoundarySpeed("BC_speed")
+    v.setValue(0, -speed, 0)
+    model.attachInitialBC(v, _set(nums, "NS_All"))
+    model.add(_solver(t_end, precision, track_tensors, freq, dt_method))
+    return model
+
+
+def _tet_box(nx, ny, nz, lx, ly, lz):
+    """Box [0, lx] x [0, ly] x [0, lz] of nx ny nz cells, each split into six
+    tetrahedra around its diagonal from corner (0, 0, 0) to (1, 1, 1); the
+    same split in every cell makes shared faces match.  Node order of each
+    tetrahedron gives a positive volume (ElTet4N3D's node 0 at the origin of
+    its local axes, nodes 1-3 along them).  Returns node numbers, [N, 3]
+    coordinates and [NE, 4] connectivity in node numbers (from 1)."""
+    xs, ys, zs = np.linspace(0, lx, nx + 1), np.linspace(0, ly, ny + 1), np.linspace(0, lz, nz + 1)
+    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+    xyz = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+
+    def nid(i, j, k):
+        return (k * (ny + 1) + j) * (nx + 1) + i
+
+    # the cell's corners in the hexahedron numbering of ElHex8N3D
+    corners = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+    tets = [(0, 1, 2, 6), (0, 2, 3, 6), (0, 3, 7, 6), (0, 7, 4, 6), (0, 4, 5, 6), (0, 5, 1, 6)]
+    conn = []
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                c = [nid(i + a, j + b, k + d) for a, b, d in corners]
+                for t in tets:
+                    q = [c[v] for v in t]
+                    v0 = xyz[q[0]]
+                    if np.linalg.det(np.stack([xyz[q[1]] - v0, xyz[q[2]] - v0, xyz[q[3]] - v0])) < 0:
+                        q[1], q[2] = q[2], q[1]
+                    conn.append(q)
+    return np.arange(1, len(xyz) + 1), xyz, np.asarray(conn) + 1
+
+
+def taylor_tet(n=4, nz=16, side=6.4e-3, length=32.4e-3, speed=227.0, t_end=20e-6, material=None,
+               precision="fp64", track_tensors=True, context=None, freq=10, dt_method=Explicit.PowerIteration):
+    """Impact of a square copper bar (side x side x length, axis along y)
+    meshed with 4-node tetrahedra (ElTet4N3D, six per cell of an n x n x nz
+    grid); bottom face restrained along the axis, initial speed -speed on
+    every node.  The reference's tetrahedron has no characteristic length
+    (0), so the time step comes from the power iteration."""
+    p = dict(COPPER)
+    if material:
+        p.update(material)
+    model = DynELA("Taylor-Tet", context=context)
+    nums, xyz, conn = _tet_box(n, nz, n, side, length, side)
+    model.createNodes(nums, xyz)
+    model.setDefaultElement(Element.ElTet4N3D)
+    ne = conn.shape[0]
+    model.createElements(np.arange(1, ne + 1), conn)
+    allES = ElementSet("ES_All")
+    model.add(allES, 1, ne)
+    model.add(_jc_material(p), allES)
+    bottom = BoundaryRestrain("BC_bottom")
+    bottom.setValue(0, 1, 0)
+    model.attachConstantBC(bottom, _set(nums[xyz[:, 1] == 0.0], "NS_Bottom"))
     v = BoundarySpeed("BC_speed")
     v.setValue(0, -speed, 0)
     model.attachInitialBC(v, _set(nums, "NS_All"))

@@ -53,6 +53,10 @@ extern "C" {
 #define DNLG_EL_QUA4N_2D 1 /* 4-node plane-strain quad, 2x2 Gauss points (ElQua4N2D.C) */
 #define DNLG_EL_QUA4N_AX 3 /* 4-node axisymmetric quad, 2x2 Gauss points */
 #define DNLG_EL_HEX8N_3D 4 /* 8-node hexahedron, 2x2x2 Gauss points     */
+#define DNLG_EL_TET4N_3D 5 /* 4-node tetrahedron, one integration point (ElTet4N3D.C);
+                              FP64 precision, power-iteration time steps only: the
+                              reference's characteristic length of this element is 0
+                              (ElTet4N3D.h:45), so its Courant step is 0 */
 
 /* hardening laws */
 #define DNLG_LAW_JOHNSON_COOK 0 /* JohnsonCookLaw.C:146-170,298-339; p = A,B,C,n,m,depsp0,Tm,T0 */

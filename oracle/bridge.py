@@ -85,7 +85,8 @@ class HistoryItem(ctypes.Structure):
 def extrapolation(elem_type):
     """The oracle's IP-to-node extrapolation table, [node][ip]."""
     nn = 8 if elem_type == 4 else 4
-    E = np.zeros((nn, nn))
+    nip = 1 if elem_type == 5 else nn  # DNLG_EL_TET4N_3D = 5: one integration point
+    E = np.zeros((nn, nip))
     load().oracle_extrapolation(elem_type, E.ctypes.data_as(_dp))
     return E
 
@@ -225,7 +226,7 @@ def element_stiffness(elem_type, xyz, young, poisson, under=False):
     matrix (PowerIterationUnder)."""
     lib = load()
     xyz = np.ascontiguousarray(xyz, dtype=np.float64)
-    nd = xyz.shape[0] * (3 if elem_type == 4 else 2)  # DNLG_EL_HEX8N_3D = 4
+    nd = xyz.shape[0] * (3 if elem_type in (4, 5) else 2)  # DNLG_EL_HEX8N_3D = 4, DNLG_EL_TET4N_3D = 5
     K = np.zeros((nd, nd))
     st = lib.oracle_element_stiffness(elem_type, p(xyz), young, poisson, 1 if under else 0, K.ctypes.data_as(_dp))
     if st != 0:

@@ -721,6 +721,18 @@ Tables make_tables(int type) {
         T.dNdr[p][a][2] = 0.0;
       }
     }
+  } else if (type == DNLG_EL_TET4N_3D) {
+    /* ElTet4N3D.C:25-80: one point at the centroid, weight 1/6, linear
+     * shape functions 1 - xi - eta - zeta, xi, eta, zeta */
+    T.dim = 3;
+    T.nn = 4;
+    T.nip = 1;
+    T.w[0] = 1.0 / 6;
+    const double d[4][3] = {{-1, -1, -1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int a = 0; a < 4; a++) {
+      T.N[0][a] = 1.0 / 4;
+      for (int c = 0; c < 3; c++) T.dNdr[0][a][c] = d[a][c];
+    }
   } else {
     T.dim = 3;
     T.nn = 8;
@@ -921,6 +933,7 @@ void compute_mass(OracleJob *J) {
  * (ElQua4NAx.C:312-321); ElHex8N3D::getCharacteristicLength / getVolume
  * (ElHex8N3D.C:614-684). */
 double char_length(const OracleJob *J, int64_t e) {
+  if (J->tab.type == DNLG_EL_TET4N_3D) return 0.0; /* ElTet4N3D.h:45 */
   const int32_t *cn = &J->conn[e * J->nn];
   auto X = [&](int a, int c) { return J->x[c * J->N + cn[a]]; };
   if (J->dim == 2) {
@@ -1184,7 +1197,7 @@ double power_iteration(OracleJob *J) {
   const int nd = J->nn * J->dim;
   const int64_t ndof = (int64_t)J->dim * J->N;
   std::vector<double> Ke((size_t)J->NE * nd * nd);
-  if (J->prm.dt_method == DNLG_DT_POWER_ITERATION_UNDER) {
+  if (J->prm.dt_method == DNLG_DT_POWER_ITERATION_UNDER && J->nip > 1) {
     bool ok = true;
 #pragma omp parallel for schedule(static) num_threads(J->threads) reduction(&& : ok)
     for (int64_t e = 0; e < J->NE; e++) ok = element_stiffness_under(J, e, &Ke[(size_t)e * nd * nd]) && ok;
@@ -1599,6 +1612,15 @@ bool nodes_finite(const OracleJob *J, int f) {
  * 1 -> -1/2, 0 -> 1 - sqrt3/2). */
 void extrapolation(int type, double E[8][8]) {
   const double r3 = std::sqrt(3.0);
+  if (type == DNLG_EL_TET4N_3D) {
+    /* ElTet4N3D.C:33-60: each node's vector has four entries (those of a
+     * four-point rule) while the element has one integration point, so the
+     * first entry is the one used: (1 + 3 sqrt5)/4 at node 1, (1 - sqrt5)/4
+     * at the others */
+    const double r5 = std::sqrt(5.0);
+    for (int a = 0; a < 4; a++) E[a][0] = a == 0 ? (1.0 + 3.0 * r5) / 4 : (1.0 - r5) / 4;
+    return;
+  }
   if (type == DNLG_EL_HEX8N_3D) {
     const int ns[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
                           {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
@@ -1754,9 +1776,12 @@ int oracle_job_create(const dnlg_mesh *mesh, const dnlg_material *mat, const dnl
                       const dnlg_solver_params *prm, OracleJob **out) {
   if (!mesh || !mat || !prm || !out) return DNLG_E_INVALID;
   if (mesh->elem_type != DNLG_EL_QUA4N_AX && mesh->elem_type != DNLG_EL_HEX8N_3D &&
-      mesh->elem_type != DNLG_EL_QUA4N_2D)
+      mesh->elem_type != DNLG_EL_QUA4N_2D && mesh->elem_type != DNLG_EL_TET4N_3D)
     return DNLG_E_INVALID;
   if (mesh->n_nodes <= 0 || mesh->n_elems <= 0 || !mesh->coords || !mesh->conn) return DNLG_E_INVALID;
+  /* ElTet4N3D::getCharacteristicLength returns 0 (ElTet4N3D.h:45): its
+   * Courant step is 0 */
+  if (mesh->elem_type == DNLG_EL_TET4N_3D && prm->dt_method == DNLG_DT_COURANT) return DNLG_E_INVALID;
   if (prm->dt_method != DNLG_DT_COURANT && prm->dt_method != DNLG_DT_POWER_ITERATION &&
       prm->dt_method != DNLG_DT_POWER_ITERATION_UNDER)
     return DNLG_E_INVALID;
@@ -2134,9 +2159,9 @@ int oracle_job_read_node_field(OracleJob *J, int field, double *out) {
 int oracle_extrapolation(int type, double *E) {
   double e[8][8];
   extrapolation(type, e);
-  const int nn = type == DNLG_EL_HEX8N_3D ? 8 : 4;
-  for (int a = 0; a < nn; a++)
-    for (int p = 0; p < nn; p++) E[a * nn + p] = e[a][p];
+  const Tables T = make_tables(type);
+  for (int a = 0; a < T.nn; a++)
+    for (int p = 0; p < T.nip; p++) E[a * T.nip + p] = e[a][p];
   return DNLG_OK;
 }
 
@@ -2262,7 +2287,7 @@ int oracle_element_stiffness(int type, const double *xyz, double young, double p
   J.detJ0.assign(J.nip, 0.0);
   J.radius.assign(J.nip, 0.0);
   J.dN.assign((size_t)J.nip * J.nn * J.dim, 0.0);
-  if (under) return element_stiffness_under(&J, 0, K) ? DNLG_OK : DNLG_E_NEG_JACOBIAN;
+  if (under && J.nip > 1) return element_stiffness_under(&J, 0, K) ? DNLG_OK : DNLG_E_NEG_JACOBIAN;
   if (!element_jacobian(&J, 0, false)) return DNLG_E_NEG_JACOBIAN;
   element_stiffness(&J, 0, K);
   return DNLG_OK;
