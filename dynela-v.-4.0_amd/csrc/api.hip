@@ -1103,6 +1103,13 @@ int state_transfer(dnlg_job *job, dnlg_state *s, bool to_host) {
   const bool mixed = is_mixed(job->prec);
   int st;
   if ((st = ip_copy(job, to_host, s->stress, H.sig, H.sig32, 6 * G))) return st;
+  if (!to_host && mixed && dim == 2) {
+    // the mixed kernel of the plane and axisymmetric quads carries no xz / yz
+    // stress (their kinematics produce none): it neither reads nor writes
+    // those rows, which stay 0
+    HIPCHK(ctx, hipMemset(H.sig32 + 2 * G, 0, G * sizeof(float)));
+    HIPCHK(ctx, hipMemset(H.sig32 + 4 * G, 0, G * sizeof(float)));
+  }
   double *q = s->ip_scalars;
   if ((st = ip_copy(job, to_host, q + DNLG_STATE_PLASTIC_STRAIN * G, H.ep, nullptr, G)) ||
       (st = ip_copy(job, to_host, q + DNLG_STATE_PLASTIC_STRAIN_RATE * G, H.dep, H.dep32, G)) ||

@@ -38,19 +38,58 @@
 
 namespace dnlg {
 
+// Symmetric tensors here are [xx, xy, xz, yy, yz, zz].  B2: the 2+1 block
+// structure of the plane and axisymmetric elements (xz = yz = 0, and in the
+// velocity gradient H02 = H12 = H20 = H21 = 0).  The B2 forms keep the terms
+// of the general forms that remain, in the same order and grouping, so their
+// nonzero results carry the same bits; the dropped terms are exact zeros.
+
+// C = A B for commuting symmetric A, B (powers of one matrix): symmetric
+template <bool B2>
 __device__ __forceinline__ void sym_mul(const float A[6], const float B[6], float C[6]) {
-  // C = A B for commuting symmetric A, B (powers of one matrix): symmetric
-  C[0] = A[0] * B[0] + A[1] * B[1] + A[2] * B[2];
-  C[1] = A[0] * B[1] + A[1] * B[3] + A[2] * B[4];
-  C[2] = A[0] * B[2] + A[1] * B[4] + A[2] * B[5];
-  C[3] = A[1] * B[1] + A[3] * B[3] + A[4] * B[4];
-  C[4] = A[1] * B[2] + A[3] * B[4] + A[4] * B[5];
-  C[5] = A[2] * B[2] + A[4] * B[4] + A[5] * B[5];
+  if constexpr (B2) {
+    C[0] = A[0] * B[0] + A[1] * B[1];
+    C[1] = A[0] * B[1] + A[1] * B[3];
+    C[2] = 0.f;
+    C[3] = A[1] * B[1] + A[3] * B[3];
+    C[4] = 0.f;
+    C[5] = A[5] * B[5];
+  } else {
+    C[0] = A[0] * B[0] + A[1] * B[1] + A[2] * B[2];
+    C[1] = A[0] * B[1] + A[1] * B[3] + A[2] * B[4];
+    C[2] = A[0] * B[2] + A[1] * B[4] + A[2] * B[5];
+    C[3] = A[1] * B[1] + A[3] * B[3] + A[4] * B[4];
+    C[4] = A[1] * B[2] + A[3] * B[4] + A[4] * B[5];
+    C[5] = A[2] * B[2] + A[4] * B[4] + A[5] * B[5];
+  }
+}
+
+// x : y for symmetric x, y
+template <bool B2>
+__device__ __forceinline__ float sym_dot(const float x[6], const float y[6]) {
+  if constexpr (B2)
+    return x[0] * y[0] + x[3] * y[3] + x[5] * y[5] + 2.f * (x[1] * y[1]);
+  else
+    return x[0] * y[0] + x[3] * y[3] + x[5] * y[5] + 2.f * (x[1] * y[1] + x[2] * y[2] + x[4] * y[4]);
 }
 
 // R S R^T for symmetric S as (R S) R^T: 45 FMAs (SymTensor2::dotRxRT,
-// SymTensor2.C:512-525, which the FP64 path follows term by term).
+// SymTensor2.C:512-525, which the FP64 path follows term by term); B2: the
+// block form (R02 = R12 = R20 = R21 = 0), 14 FMAs.
+template <bool B2>
 __device__ __forceinline__ void rotate_sym_f32(const float r[9], float S[6]) {
+  if constexpr (B2) {
+    const float m00 = r[0] * S[0] + r[1] * S[1], m01 = r[0] * S[1] + r[1] * S[3];
+    const float m10 = r[3] * S[0] + r[4] * S[1], m11 = r[3] * S[1] + r[4] * S[3];
+    const float m22 = r[8] * S[5];
+    S[0] = m00 * r[0] + m01 * r[1];
+    S[1] = m00 * r[3] + m01 * r[4];
+    S[2] = 0.f;
+    S[3] = m10 * r[3] + m11 * r[4];
+    S[4] = 0.f;
+    S[5] = m22 * r[8];
+    return;
+  }
   const float s3[3][3] = {{S[0], S[1], S[2]}, {S[1], S[3], S[4]}, {S[2], S[4], S[5]}};
   float m[3][3];
 #pragma unroll
@@ -78,51 +117,80 @@ __device__ __forceinline__ void polar_lnU_large(const float *H, float *dE, float
 }
 
 // H (row-major 3x3, FP32) -> dE = ln U (sym), R = F U^-1 (row-major).
+// B2: H has the 2+1 block structure; Z8 (plane strain): also H22 = 0, so
+// ln U and R keep the zz entries 0 and 1.
+template <bool B2, bool Z8>
 __device__ __forceinline__ void polar_lnU_mixed(const float H[9], float dE[6], float R[9], int order) {
   float X[6];
-  X[0] = 2.f * H[0] + (H[0] * H[0] + H[3] * H[3] + H[6] * H[6]);
-  X[1] = H[1] + H[3] + (H[0] * H[1] + H[3] * H[4] + H[6] * H[7]);
-  X[2] = H[2] + H[6] + (H[0] * H[2] + H[3] * H[5] + H[6] * H[8]);
-  X[3] = 2.f * H[4] + (H[1] * H[1] + H[4] * H[4] + H[7] * H[7]);
-  X[4] = H[5] + H[7] + (H[1] * H[2] + H[4] * H[5] + H[7] * H[8]);
-  X[5] = 2.f * H[8] + (H[2] * H[2] + H[5] * H[5] + H[8] * H[8]);
-  const float nrm2 = X[0] * X[0] + X[3] * X[3] + X[5] * X[5] + 2.f * (X[1] * X[1] + X[2] * X[2] + X[4] * X[4]);
+  if constexpr (B2) {
+    X[0] = 2.f * H[0] + (H[0] * H[0] + H[3] * H[3]);
+    X[1] = H[1] + H[3] + (H[0] * H[1] + H[3] * H[4]);
+    X[2] = 0.f;
+    X[3] = 2.f * H[4] + (H[1] * H[1] + H[4] * H[4]);
+    X[4] = 0.f;
+    X[5] = Z8 ? 0.f : 2.f * H[8] + (H[8] * H[8]);
+  } else {
+    X[0] = 2.f * H[0] + (H[0] * H[0] + H[3] * H[3] + H[6] * H[6]);
+    X[1] = H[1] + H[3] + (H[0] * H[1] + H[3] * H[4] + H[6] * H[7]);
+    X[2] = H[2] + H[6] + (H[0] * H[2] + H[3] * H[5] + H[6] * H[8]);
+    X[3] = 2.f * H[4] + (H[1] * H[1] + H[4] * H[4] + H[7] * H[7]);
+    X[4] = H[5] + H[7] + (H[1] * H[2] + H[4] * H[5] + H[7] * H[8]);
+    X[5] = 2.f * H[8] + (H[2] * H[2] + H[5] * H[5] + H[8] * H[8]);
+  }
+  const float nrm2 = sym_dot<B2>(X, X);
   if (nrm2 > 0.05f * 0.05f) {
     polar_lnU_large(H, dE, R);
-    return;
-  }
-  // powers to X^order (order 3, 4 or 6, uniform over the wave)
-  float X2[6], X3[6], X4[6], X5[6], X6[6];
-  sym_mul(X, X, X2);
-  sym_mul(X2, X, X3);
+  } else {
+    // powers to X^order (order 3, 4 or 6, uniform over the wave)
+    float X2[6], X3[6], X4[6], X5[6], X6[6];
+    sym_mul<B2>(X, X, X2);
+    sym_mul<B2>(X2, X, X3);
 #pragma unroll
-  for (int c = 0; c < 6; c++) X4[c] = X5[c] = X6[c] = 0.f;
-  if (order >= 4) sym_mul(X2, X2, X4);
-  if (order >= 6) {
-    sym_mul(X4, X, X5);
-    sym_mul(X3, X3, X6);
-  }
-  float Ui[6];
+    for (int c = 0; c < 6; c++) X4[c] = X5[c] = X6[c] = 0.f;
+    if (order >= 4) sym_mul<B2>(X2, X2, X4);
+    if (order >= 6) {
+      sym_mul<B2>(X4, X, X5);
+      sym_mul<B2>(X3, X3, X6);
+    }
+    float Ui[6];
 #pragma unroll
-  for (int c = 0; c < 6; c++) {
-    dE[c] = 0.5f * (X[c] - X2[c] * (1.f / 2.f) + X3[c] * (1.f / 3.f) - X4[c] * (1.f / 4.f) +
-                    X5[c] * (1.f / 5.f) - X6[c] * (1.f / 6.f));
-    Ui[c] = -X[c] * (1.f / 2.f) + X2[c] * (3.f / 8.f) - X3[c] * (5.f / 16.f) + X4[c] * (35.f / 128.f) -
-            X5[c] * (63.f / 256.f) + X6[c] * (231.f / 1024.f);
-  }
-  Ui[0] += 1.f;
-  Ui[3] += 1.f;
-  Ui[5] += 1.f;
-  // R = (I + H) Ui
+    for (int c = 0; c < 6; c++) {
+      dE[c] = 0.5f * (X[c] - X2[c] * (1.f / 2.f) + X3[c] * (1.f / 3.f) - X4[c] * (1.f / 4.f) +
+                      X5[c] * (1.f / 5.f) - X6[c] * (1.f / 6.f));
+      Ui[c] = -X[c] * (1.f / 2.f) + X2[c] * (3.f / 8.f) - X3[c] * (5.f / 16.f) + X4[c] * (35.f / 128.f) -
+              X5[c] * (63.f / 256.f) + X6[c] * (231.f / 1024.f);
+    }
+    Ui[0] += 1.f;
+    Ui[3] += 1.f;
+    Ui[5] += 1.f;
+    // R = (I + H) Ui
+    if constexpr (B2) {
+      R[0] = Ui[0] + (H[0] * Ui[0] + H[1] * Ui[1]);
+      R[1] = Ui[1] + (H[0] * Ui[1] + H[1] * Ui[3]);
+      R[3] = Ui[1] + (H[3] * Ui[0] + H[4] * Ui[1]);
+      R[4] = Ui[3] + (H[3] * Ui[1] + H[4] * Ui[3]);
+      const float t = H[8] * Ui[5];  // rounded on its own, as in the general form
+      R[8] = Ui[5] + t;
+    } else {
 #pragma unroll
-  for (int r = 0; r < 3; r++) {
-    const float a0 = H[3 * r], a1 = H[3 * r + 1], a2 = H[3 * r + 2];
-    const float u0 = (r == 0) ? Ui[0] : (r == 1 ? Ui[1] : Ui[2]);
-    const float u1 = (r == 0) ? Ui[1] : (r == 1 ? Ui[3] : Ui[4]);
-    const float u2 = (r == 0) ? Ui[2] : (r == 1 ? Ui[4] : Ui[5]);
-    R[3 * r + 0] = u0 + (a0 * Ui[0] + a1 * Ui[1] + a2 * Ui[2]);
-    R[3 * r + 1] = u1 + (a0 * Ui[1] + a1 * Ui[3] + a2 * Ui[4]);
-    R[3 * r + 2] = u2 + (a0 * Ui[2] + a1 * Ui[4] + a2 * Ui[5]);
+      for (int r = 0; r < 3; r++) {
+        const float a0 = H[3 * r], a1 = H[3 * r + 1], a2 = H[3 * r + 2];
+        const float u0 = (r == 0) ? Ui[0] : (r == 1 ? Ui[1] : Ui[2]);
+        const float u1 = (r == 0) ? Ui[1] : (r == 1 ? Ui[3] : Ui[4]);
+        const float u2 = (r == 0) ? Ui[2] : (r == 1 ? Ui[4] : Ui[5]);
+        R[3 * r + 0] = u0 + (a0 * Ui[0] + a1 * Ui[1] + a2 * Ui[2]);
+        R[3 * r + 1] = u1 + (a0 * Ui[1] + a1 * Ui[3] + a2 * Ui[4]);
+        R[3 * r + 2] = u2 + (a0 * Ui[2] + a1 * Ui[4] + a2 * Ui[5]);
+      }
+    }
+  }
+  if constexpr (B2) {
+    dE[2] = dE[4] = 0.f;
+    R[2] = R[5] = R[6] = R[7] = 0.f;
+    if constexpr (Z8) {
+      dE[5] = 0.f;
+      R[8] = 1.f;
+    }
   }
 }
 
@@ -338,6 +406,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
     JobDev *const *jobs, const Task *tasks, int ctl, int derive) {
   constexpr int NB = DIM;               // lanes per element = 2^DIM = NN
   constexpr float inv2D = 1.f / NN;     // 2^-D
+  // plane and axisymmetric quads: 2+1 block structure, no xz / yz stress
+  // (those entries of the stored stress stay 0); plane strain: no hoop strain
+  constexpr bool B2 = DIM == 2, Z8 = B2 && !AXI;
   const float g = 0.577350269189625764f;  // Gauss abscissa 1/sqrt(3)
 
   const Task tk = load_task(tasks);
@@ -387,7 +458,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
   const int slot = at(e2slot + g0, 4 * ip);  // incidence slot of this lane's node, where its force goes
   float sig0[6];                             // stress, yield stress, gamma of the last cycle
 #pragma unroll
-  for (int c = 0; c < 6; c++) sig0[c] = at(sig32 + c * G + g0, 4 * ip);
+  for (int c = 0; c < 6; c++) sig0[c] = (B2 && (c == 2 || c == 4)) ? 0.f : at(sig32 + c * G + g0, 4 * ip);
   const float sy0 = at(sy32 + g0, 4 * ip), gam0 = at(gam32 + g0, 4 * ip);
   const double Tcur = at(Tg + g0, 8 * ip), ep0 = at(epg + g0, 8 * ip);
   // node b of the element: coordinates relative to node 0, increment
@@ -524,12 +595,17 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
   float Xn2;
   {
     const float a0 = 2.f * H[0], a3 = 2.f * H[4], a5 = 2.f * H[8];
-    const float a1 = H[1] + H[3], a2 = H[2] + H[6], a4 = H[5] + H[7];
-    Xn2 = a0 * a0 + a3 * a3 + a5 * a5 + 2.f * (a1 * a1 + a2 * a2 + a4 * a4);
+    const float a1 = H[1] + H[3];
+    if constexpr (B2) {
+      Xn2 = a0 * a0 + a3 * a3 + a5 * a5 + 2.f * (a1 * a1);
+    } else {
+      const float a2 = H[2] + H[6], a4 = H[5] + H[7];
+      Xn2 = a0 * a0 + a3 * a3 + a5 * a5 + 2.f * (a1 * a1 + a2 * a2 + a4 * a4);
+    }
   }
   const int order = __any(Xn2 > 1e-4f) ? 6 : (__any(Xn2 > 1e-6f) ? 4 : 3);
   float dE[6], R[9];
-  polar_lnU_mixed(H, dE, R, order);
+  polar_lnU_mixed<B2, Z8>(H, dE, R, order);
 
   // ---- pressure: element mean of tr(dE) (Element.C:1125-1143)
   const float inc = group_sum<NB>(dE[0] + dE[3] + dE[5]) * inv2D;
@@ -551,17 +627,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
     s[0] -= pm;
     s[3] -= pm;
     s[5] -= pm;
-    const float Snorm0 =
-        __builtin_amdgcn_sqrtf(s[0] * s[0] + s[3] * s[3] + s[5] * s[5] + 2.f * (s[1] * s[1] + s[2] * s[2] + s[4] * s[4]));
+    const float Snorm0 = __builtin_amdgcn_sqrtf(sym_dot<B2>(s, s));
     const float em = (dE[0] + dE[3] + dE[5]) * (1.f / 3.f);
     s[0] += TwoGf * (dE[0] - em);
     s[1] += TwoGf * dE[1];
-    s[2] += TwoGf * dE[2];
+    if constexpr (!B2) s[2] += TwoGf * dE[2];
     s[3] += TwoGf * (dE[3] - em);
-    s[4] += TwoGf * dE[4];
+    if constexpr (!B2) s[4] += TwoGf * dE[4];
     s[5] += TwoGf * (dE[5] - em);
-    const float Snorm =
-        __builtin_amdgcn_sqrtf(s[0] * s[0] + s[3] * s[3] + s[5] * s[5] + 2.f * (s[1] * s[1] + s[2] * s[2] + s[4] * s[4]));
+    const float Snorm = __builtin_amdgcn_sqrtf(sym_dot<B2>(s, s));
     // Work of the increment, 1/2 dE : (old + new) with new = scale s + p I
     // (the deviator scaled by the return, plus the pressure), from three
     // sums formed now; and the final rotation (Element.C:1146-1161) of the
@@ -569,15 +643,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
     // scale R s R^T + p I for the orthogonal R.  dE, the old stress and R
     // are then not kept through the radial return (its FP64 law evaluation
     // is the register peak of the kernel).
-    auto wdot = [](const float x[6], const float y[6]) {
-      return x[0] * y[0] + x[3] * y[3] + x[5] * y[5] + 2.f * (x[1] * y[1] + x[2] * y[2] + x[4] * y[4]);
-    };
     // power = 1/2 (work_a + scale work_s) once the return has fixed scale
-    float work_a = wdot(dE, sig0) + pressure * (dE[0] + dE[3] + dE[5]), work_s = wdot(dE, s);
+    float work_a = sym_dot<B2>(dE, sig0) + pressure * (dE[0] + dE[3] + dE[5]), work_s = sym_dot<B2>(dE, s);
     float Rs[6];
 #pragma unroll
     for (int c = 0; c < 6; c++) Rs[c] = s[c];
-    rotate_sym_f32(R, Rs);
+    rotate_sym_f32<B2>(R, Rs);
     // Element::computeDensity (Element.C:1243-1256)
     if (cc.density_due) {
       double rr = C->rho0 * (double)at(gbl(C->detJ0f) + g0, 4 * ip) / (double)detJ;
@@ -616,7 +687,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
       s_keep[1 * 64 + lane] = work_s;
       s_keep[2 * 64 + lane] = pressure;
 #pragma unroll
-      for (int c = 0; c < 6; c++) s_keep[(3 + c) * 64 + lane] = Rs[c];
+      for (int c = 0; c < 6; c++)
+        if (!B2 || (c != 2 && c != 4)) s_keep[(3 + c) * 64 + lane] = Rs[c];
       asm volatile("" ::: "memory");
       float g;
       bool ok = true;
@@ -680,7 +752,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
       work_s = s_keep[1 * 64 + lane];
       pressure = s_keep[2 * 64 + lane];
 #pragma unroll
-      for (int c = 0; c < 6; c++) Rs[c] = s_keep[(3 + c) * 64 + lane];
+      for (int c = 0; c < 6; c++)
+        if (!B2 || (c != 2 && c != 4)) Rs[c] = s_keep[(3 + c) * 64 + lane];
       scale = 1.f - TwoGf * g * __builtin_amdgcn_rcpf(Snorm);
       at(gbl(C->ep) + g0, 8 * ip) = ep0 + kSqrt23 * gamma;
       at(gbl(C->dep32) + g0, 4 * ip) = (float)(kSqrt23 * gamma / dt);
@@ -701,7 +774,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
     sig[3] += pressure;
     sig[5] += pressure;
 #pragma unroll
-    for (int c = 0; c < 6; c++) at(gbl(C->sig32) + c * G + g0, 4 * ip) = sig[c];
+    for (int c = 0; c < 6; c++)
+      if (!B2 || (c != 2 && c != 4)) at(gbl(C->sig32) + c * G + g0, 4 * ip) = sig[c];
   } else {
 #pragma unroll
     for (int c = 0; c < 6; c++) sig[c] = 0.f;
