@@ -45,6 +45,17 @@ __device__ __forceinline__ double rcp_f64(double d) {
   e = __builtin_fma(-d, r, 1.0);
   return __builtin_fma(r, e, r);
 }
+// a b + c for a constant c: one multiply-add that reads c from a scalar
+// register pair.  Written as a plain fma, the compiler takes the
+// two-address form, whose addend is also its destination, and first copies
+// the constant into vector registers (two extra instructions per step of a
+// polynomial); inside a loop it also keeps every constant of the loop in
+// vector registers.
+__device__ __forceinline__ double fmak(double a, double b, double c) {
+  double r;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(c));
+  return r;
+}
 __device__ __forceinline__ double log_f64_short(double x) {
   double m = __builtin_amdgcn_frexp_mant(x);  // [0.5, 1)
   int k = __builtin_amdgcn_frexp_exp(x);
@@ -56,16 +67,15 @@ __device__ __forceinline__ double log_f64_short(double x) {
   double s = num * r;
   s = __builtin_fma(r, __builtin_fma(-den, s, num), s);
   const double z = s * s;
-  double q = 2.0 / 21.0;
-  q = __builtin_fma(q, z, 2.0 / 19.0);
-  q = __builtin_fma(q, z, 2.0 / 17.0);
-  q = __builtin_fma(q, z, 2.0 / 15.0);
-  q = __builtin_fma(q, z, 2.0 / 13.0);
-  q = __builtin_fma(q, z, 2.0 / 11.0);
-  q = __builtin_fma(q, z, 2.0 / 9.0);
-  q = __builtin_fma(q, z, 2.0 / 7.0);
-  q = __builtin_fma(q, z, 2.0 / 5.0);
-  q = __builtin_fma(q, z, 2.0 / 3.0);
+  double q = fmak(2.0 / 21.0, z, 2.0 / 19.0);
+  q = fmak(q, z, 2.0 / 17.0);
+  q = fmak(q, z, 2.0 / 15.0);
+  q = fmak(q, z, 2.0 / 13.0);
+  q = fmak(q, z, 2.0 / 11.0);
+  q = fmak(q, z, 2.0 / 9.0);
+  q = fmak(q, z, 2.0 / 7.0);
+  q = fmak(q, z, 2.0 / 5.0);
+  q = fmak(q, z, 2.0 / 3.0);
   const double kd = (double)k;
   const double t = __builtin_fma(s * z, q, kd * 1.90821492927058770002e-10);  // + k ln2 (low part)
   return __builtin_fma(kd, 6.93147180369123816490e-01, 2.0 * s + t);          // + k ln2 (high part)
@@ -74,17 +84,16 @@ __device__ __forceinline__ double exp_f64_short(double x) {
   const double k = __builtin_rint(x * 1.4426950408889634074);  // x / ln 2
   double r = __builtin_fma(-k, 6.93147180369123816490e-01, x);
   r = __builtin_fma(-k, 1.90821492927058770002e-10, r);
-  double q = 1.0 / 6227020800.0;  // 1/13!
-  q = __builtin_fma(q, r, 1.0 / 479001600.0);
-  q = __builtin_fma(q, r, 1.0 / 39916800.0);
-  q = __builtin_fma(q, r, 1.0 / 3628800.0);
-  q = __builtin_fma(q, r, 1.0 / 362880.0);
-  q = __builtin_fma(q, r, 1.0 / 40320.0);
-  q = __builtin_fma(q, r, 1.0 / 5040.0);
-  q = __builtin_fma(q, r, 1.0 / 720.0);
-  q = __builtin_fma(q, r, 1.0 / 120.0);
-  q = __builtin_fma(q, r, 1.0 / 24.0);
-  q = __builtin_fma(q, r, 1.0 / 6.0);
+  double q = fmak(1.0 / 6227020800.0, r, 1.0 / 479001600.0);  // 1/13!, 1/12!
+  q = fmak(q, r, 1.0 / 39916800.0);
+  q = fmak(q, r, 1.0 / 3628800.0);
+  q = fmak(q, r, 1.0 / 362880.0);
+  q = fmak(q, r, 1.0 / 40320.0);
+  q = fmak(q, r, 1.0 / 5040.0);
+  q = fmak(q, r, 1.0 / 720.0);
+  q = fmak(q, r, 1.0 / 120.0);
+  q = fmak(q, r, 1.0 / 24.0);
+  q = fmak(q, r, 1.0 / 6.0);
   q = __builtin_fma(q, r, 0.5);
   q = __builtin_fma(q, r, 1.0);
   q = __builtin_fma(q, r, 1.0);
