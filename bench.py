@@ -106,14 +106,17 @@ def run_precision(prec, ctx, steps, warmup, n, nz, t_end, workload="taylor3d", p
     return out
 
 
-def run_sweep(prec, ctx, n_jobs, t_end, seed=20261203, max_cycles=None):
+def run_sweep(prec, ctx, n_jobs, t_end, seed=20261203, max_cycles=None, perturb=0.0):
     """Batched parameter sweep: n_jobs seeded Taylor impacts (2D axisymmetric
     or 3D, 2k-60k elements log-uniform, Johnson-Cook A, B, n, C, m x U(0.8,
     1.2), 150-300 m/s) advanced together by one batched solve (one launch per
-    phase and cycle for all jobs)."""
+    phase and cycle for all jobs).  perturb: every impact speed times
+    (1 + perturb), for the sensitivity of the solutions to their input."""
     import dnlPython as dnl
     from dnlPython import samples
     jobs = samples.sweep_params(n_jobs, seed=seed)
+    for j in jobs:
+        j["speed"] *= 1.0 + perturb
     models = [samples.build_sweep_job(j, t_end=t_end, precision=prec, context=ctx) for j in jobs]
     for m in models:
         m.initSolve()
@@ -141,7 +144,7 @@ def run_sweep(prec, ctx, n_jobs, t_end, seed=20261203, max_cycles=None):
     res = dict(wall=wall, value=ec / wall, elem_cycles=ec, jobs=n_jobs, failed=len(failed), capped=len(capped),
                elements=sum(m.getElementsNumber() for m in models), max_cycles=max(cyc),
                elem_ms=el_ms, elem_launches=el_n,
-               done=[m.getCurrentTime() >= t_end for m in models],
+               done=[m.getCurrentTime() >= t_end for m in models], kinds=[j["kind"] for j in jobs],
                disp=[m.getNodalDisplacements() for m in models])
     for m in models:
         m.close()
@@ -396,6 +399,8 @@ def main():
     ap.add_argument("--jobs", type=int, default=256, help="sweep / serve workloads: number of jobs")
     ap.add_argument("--max-cycles", type=int, default=None,
                     help="sweep: stop the batch after about this many cycles (jobs whose time step collapsed)")
+    ap.add_argument("--no-sensitivity", dest="sensitivity", action="store_false",
+                    help="sweep: leave out the FP64 rerun with perturbed impact speeds")
     ap.add_argument("--util", type=float, default=0.7, help="serve: offered load for the arrival rate")
     ap.add_argument("--rate", type=float, default=None, help="serve: arrival rate in jobs/s (overrides --util)")
     ap.add_argument("--step-cycles", type=int, default=32, help="serve: cycles per server step")
@@ -441,11 +446,34 @@ def main():
             line[f"value_{p}"] = res[p]["value"]
         if "mixed" in res and "fp64" in res:
             line["speedup_mixed_vs_fp64"] = res["mixed"]["value"] / res["fp64"]["value"]
-            errs = [disp_rel_err(a, b) for a, b, da, db in zip(res["mixed"]["disp"], res["fp64"]["disp"],
-                                                              res["mixed"]["done"], res["fp64"]["done"]) if da and db]
+            mx, fp = res["mixed"], res["fp64"]
+            both = [i for i in range(len(fp["done"])) if mx["done"][i] and fp["done"][i]]
+            errs = [disp_rel_err(mx["disp"][i], fp["disp"][i]) for i in both]
             if errs:
                 line["mixed_disp_err_vs_fp64"] = {"max": max(errs), "median": float(np.median(errs)),
                                                   "jobs_compared": len(errs)}
+                for kind in ("axi", "3d"):
+                    ek = [e for e, i in zip(errs, both) if fp["kinds"][i] == kind]
+                    if ek:
+                        line["mixed_disp_err_vs_fp64"][f"max_{kind}"] = max(ek)
+            if errs and args.sensitivity:
+                # the same FP64 solve with every impact speed perturbed by
+                # the FP32 unit roundoff: how far the jobs' own solutions
+                # move under an input change of the size mixed precision
+                # makes at every step
+                pr = run_sweep("fp64", ctx, args.jobs, args.t_end, max_cycles=args.max_cycles, perturb=2.0 ** -24)
+                sens = [disp_rel_err(pr["disp"][i], fp["disp"][i]) if pr["done"][i] else float("nan") for i in both]
+                ratio = [e / s for e, s in zip(errs, sens) if s == s and s > 0]
+                line["fp64_sensitivity"] = {
+                    "perturbation": "impact speed x (1 + 2^-24)", "max": float(np.nanmax(sens)),
+                    "median": float(np.nanmedian(sens)),
+                    "mixed_err_over_sensitivity_max": max(ratio) if ratio else None,
+                    "mixed_err_over_sensitivity_median": float(np.median(ratio)) if ratio else None}
+                for kind in ("axi", "3d"):
+                    sk = [s for s, i in zip(sens, both) if fp["kinds"][i] == kind and s == s]
+                    if sk:
+                        line["fp64_sensitivity"][f"max_{kind}"] = max(sk)
+                        line["fp64_sensitivity"][f"median_{kind}"] = float(np.median(sk))
         print(json.dumps(line), flush=True)
         return
 
