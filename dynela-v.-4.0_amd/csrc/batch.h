@@ -99,6 +99,7 @@ namespace dnlg {
 // rule of the context, else the defaults (sched.h); fields left 0 in a rule
 // take the defaults.
 dnlg_schedule resolve_schedule(const dnlg_ctx *ctx, const dnlg_job *job);
+dnlg_schedule resolve_schedule(const dnlg_ctx *ctx, const dnlg_job *job, int64_t ne);
 int job_variant(const dnlg_job *job);
 bool schedule_valid(int type, int precision, int law, const dnlg_schedule &s, int stress_method = DNLG_STRESS_NR);
 

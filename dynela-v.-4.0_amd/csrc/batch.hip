@@ -145,9 +145,18 @@ int build_batch(dnlg_ctx *ctx, const std::vector<dnlg_job *> &jobs, Batch &B) {
   B.d_pend = (int *)p;
   B.h_pend.assign(n, 0);
   int st;
-  // schedules, then element groups in (type, variant, block, waves) order
-  for (auto *j : jobs)
-    if (!j->sched_set) j->sched = resolve_schedule(ctx, j);
+  // schedules, then element groups in (type, variant, block, waves) order.
+  // Jobs without a schedule of their own take the rule for the element
+  // count of all such jobs of their type and precision together: they are
+  // launched together, so that count is the launch's size, and one schedule
+  // per kind keeps them in one element launch and one corrector launch.
+  {
+    std::map<std::array<int, 3>, int64_t> total;
+    for (auto *j : jobs)
+      if (!j->sched_set) total[{j->type, j->prec, job_variant(j)}] += j->NE;
+    for (auto *j : jobs)
+      if (!j->sched_set) j->sched = resolve_schedule(ctx, j, total[{j->type, j->prec, job_variant(j)}]);
+  }
   B.graph_cycles = 0;
   for (auto *j : jobs) B.graph_cycles = std::max(B.graph_cycles, (int)j->sched.graph_cycles);
   std::map<std::array<int, 4>, std::vector<int>> groups;
@@ -384,13 +393,17 @@ bool schedule_valid(int type, int precision, int law, const dnlg_schedule &s, in
          s.graph_cycles >= 2 && s.graph_cycles <= 4096;
 }
 
-dnlg_schedule resolve_schedule(const dnlg_ctx *ctx, const dnlg_job *job) {
+dnlg_schedule resolve_schedule(const dnlg_ctx *ctx, const dnlg_job *job) { return resolve_schedule(ctx, job, job->NE); }
+
+// The schedule rule for a launch of `ne` elements of job's type and
+// precision (a job alone: its own count; in a batch: the count of the
+// launch it shares with the other jobs of its kind).
+dnlg_schedule resolve_schedule(const dnlg_ctx *ctx, const dnlg_job *job, int64_t ne) {
   const int v = job_variant(job);
   dnlg_schedule s{default_elem_block(job->type), default_waves(v), default_corr_batch(is_mixed(job->prec)),
                   ctx->graph_cycles > 0 ? ctx->graph_cycles : kDefaultGraphCycles};
   for (const auto &r : ctx->sched_rules)
-    if (r.type == job->type && r.prec == job->prec && job->NE >= r.min_elems &&
-        (r.max_elems <= 0 || job->NE < r.max_elems)) {
+    if (r.type == job->type && r.prec == job->prec && ne >= r.min_elems && (r.max_elems <= 0 || ne < r.max_elems)) {
       dnlg_schedule t = s;
       if (r.sched.elem_block) t.elem_block = r.sched.elem_block;
       if (r.sched.waves) t.waves = r.sched.waves;
