@@ -478,13 +478,27 @@ int dnlg_ctx_create(int device, dnlg_ctx **out) {
   // DNLG_PI_SLOTS=<n>: power iterations per time-step cycle inside a launch
   // sequence (0..16); a job that needs more is finished by the host
   if (const char *g = std::getenv("DNLG_PI_SLOTS")) ctx->pi_slots = std::min(16, std::max(0, std::atoi(g)));
+  // DNLG_BRANCHES=0: a batch's cycle kernels all on one stream
+  if (const char *g = std::getenv("DNLG_BRANCHES")) ctx->branches = std::atoi(g) != 0;
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return DNLG_E_HIP;
+  }
+  if (hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming) != hipSuccess) {
+    if (ctx->fork_ev) hipEventDestroy(ctx->fork_ev);
+    if (ctx->side) hipStreamDestroy(ctx->side);
+    hipStreamDestroy(ctx->stream);
     delete ctx;
     return DNLG_E_HIP;
   }
   // the device address of the allocation-free entry points' scratch, once
   // per device, with the device current
   if (energy_scratch_init(device) != DNLG_OK) {
+    hipEventDestroy(ctx->join_ev);
+    hipEventDestroy(ctx->fork_ev);
+    hipStreamDestroy(ctx->side);
     hipStreamDestroy(ctx->stream);
     delete ctx;
     return DNLG_E_HIP;
@@ -497,6 +511,10 @@ int dnlg_ctx_create(int device, dnlg_ctx **out) {
       upload_tables(hexN, hexD, quaN, quaD);
       upload_extrapolation();
       if (hipDeviceSynchronize() != hipSuccess) {
+        hipEventDestroy(ctx->join_ev);
+        hipEventDestroy(ctx->fork_ev);
+        hipStreamDestroy(ctx->side);
+        hipStreamDestroy(ctx->stream);
         delete ctx;
         return DNLG_E_HIP;
       }
@@ -513,6 +531,9 @@ int dnlg_ctx_destroy(dnlg_ctx *ctx) {
   batch_cache_clear(ctx);
   hipStreamSynchronize(ctx->stream);
   if (ctx->energy_ws) hipFree(ctx->energy_ws);
+  hipEventDestroy(ctx->join_ev);
+  hipEventDestroy(ctx->fork_ev);
+  hipStreamDestroy(ctx->side);
   hipStreamDestroy(ctx->stream);
   delete ctx;
   return DNLG_OK;

@@ -73,6 +73,12 @@ using BatchPtr = std::unique_ptr<Batch, BatchDeleter>;
 struct dnlg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // a second stream and two events: a batch with jobs of both dimensions
+  // runs the 3D jobs' and the 2D jobs' cycle kernels as two branches
+  // (branches = false: all on `stream`)
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  bool branches = true;
   std::string err;
   bool timing = false;
   std::map<std::string, std::pair<double, int64_t>> ktime;

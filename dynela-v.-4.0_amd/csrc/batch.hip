@@ -330,20 +330,67 @@ void launch_post(dnlg_ctx *ctx, Batch &B, int cs, bool courant, bool fold = fals
 // next cycle to the corrector (field0 BCs) and the next element launch
 // (control record), so k_cycle_step runs where the sequence starts and after
 // time-step cycles only.  Every cycle moves the record to the other slot.
+// One cycle's element, corrector and Courant kernels with the jobs of each
+// dimension in a branch of their own: the 3D branch on the engine's stream,
+// the 2D branch on the side stream, forked after the cycle opens and joined
+// before the end-of-step launches.  The branches touch disjoint jobs, so
+// either order gives the same results; running them together lets one
+// branch's kernels fill the GPU while the other's launches drain (the
+// element kernels are bound by their arithmetic, the correctors by HBM).
+// Returns false (nothing launched) when the batch holds one dimension only.
+bool launch_branches(dnlg_ctx *ctx, Batch &B, int el_cs, int derive, int cs, bool courant, bool fold) {
+  bool dims[2] = {false, false};  // [dim == 3]
+  for (const ElemGroup &g : B.elem)
+    if (g.tl.n > 0) dims[type_dim(g.type) == 3] = true;
+  if (!ctx->branches || !dims[0] || !dims[1]) return false;
+  hipStream_t main = ctx->stream, side = ctx->side;
+  hipEventRecord(ctx->fork_ev, main);
+  hipStreamWaitEvent(side, ctx->fork_ev, 0);
+  for (int d3 = 1; d3 >= 0; d3--) {
+    hipStream_t s = d3 ? main : side;
+    for (const ElemGroup &g : B.elem)
+      if (g.tl.n > 0 && (type_dim(g.type) == 3) == (d3 == 1))
+        launch_variant(g.type, g.variant, g.block, g.waves, B.d_jobs, g.tl.d, g.tl.n, el_cs, derive, s, nullptr);
+    for (const CorrGroup &g : B.corr)
+      if ((g.dim == 3) == (d3 == 1)) launch_correct(g.dim, g.f32, g.batch, B.d_jobs, g.tl.d, g.tl.n, cs, fold ? 1 : 0, s);
+    if (courant && B.has_courant)
+      for (int hx = 0; hx < kNumTypes; hx++)
+        if ((type_dim(kTypes[hx]) == 3) == (d3 == 1))
+          launch_courant(type_dim(kTypes[hx]), B.d_jobs, B.cour[hx].d, B.cour[hx].n, cs, 0, s);
+  }
+  hipEventRecord(ctx->join_ev, side);
+  hipStreamWaitEvent(main, ctx->join_ev, 0);
+  // Explicit::endStep's history rows (the Courant kernels above only write
+  // their partial minima, which the history reads nowhere)
+  launch_history(ctx, B, cs, 0);
+  if (courant && B.has_pi) {
+    launch_pi_begin(B.d_jobs, (int)B.jobs.size(), cs, 0, main);
+    for (int k = 0; k < ctx->pi_slots; k++) launch_pi_iteration(ctx, B, cs);
+  }
+  return true;
+}
+
 int launch_cycles(dnlg_ctx *ctx, Batch &B, int j0, int j1, int phase, int freq, int cs) {
   auto due = [&](int j) { return phase < 0 || (phase + j + 1) % freq == 0; };
   bool folded = false;
   for (int j = j0; j < j1; j++) {
+    // the element launch reads the record in el_cs (derive: and writes the
+    // new one into the other slot); cs: the cycle's record from then on
+    int el_cs, derive;
     if (folded) {
-      launch_element(ctx, B, nullptr, cs, 1);
-      cs = 1 - cs;
+      el_cs = cs;
+      derive = 1;
     } else {
       launch_pre(ctx, B, cs);
-      cs = 1 - cs;
-      launch_element(ctx, B, nullptr, cs, 0);
+      el_cs = 1 - cs;
+      derive = 0;
     }
+    cs = 1 - cs;
     folded = phase >= 0 && j + 1 < j1 && !due(j);
-    launch_post(ctx, B, cs, due(j), folded);
+    if (!launch_branches(ctx, B, el_cs, derive, cs, due(j), folded)) {
+      launch_element(ctx, B, nullptr, el_cs, derive);
+      launch_post(ctx, B, cs, due(j), folded);
+    }
   }
   return cs;
 }
