@@ -399,6 +399,8 @@ def main():
     ap.add_argument("--jobs", type=int, default=256, help="sweep / serve workloads: number of jobs")
     ap.add_argument("--max-cycles", type=int, default=None,
                     help="sweep: stop the batch after about this many cycles (jobs whose time step collapsed)")
+    ap.add_argument("--no-batched-fp64", action="store_true",
+                    help="leave out the FP64 run of the batched sweep (speedup and errors of the batched leg)")
     ap.add_argument("--no-sensitivity", dest="sensitivity", action="store_false",
                     help="sweep: leave out the FP64 rerun with perturbed impact speeds")
     ap.add_argument("--util", type=float, default=0.7, help="serve: offered load for the arrival rate")
@@ -446,34 +448,7 @@ def main():
             line[f"value_{p}"] = res[p]["value"]
         if "mixed" in res and "fp64" in res:
             line["speedup_mixed_vs_fp64"] = res["mixed"]["value"] / res["fp64"]["value"]
-            mx, fp = res["mixed"], res["fp64"]
-            both = [i for i in range(len(fp["done"])) if mx["done"][i] and fp["done"][i]]
-            errs = [disp_rel_err(mx["disp"][i], fp["disp"][i]) for i in both]
-            if errs:
-                line["mixed_disp_err_vs_fp64"] = {"max": max(errs), "median": float(np.median(errs)),
-                                                  "jobs_compared": len(errs)}
-                for kind in ("axi", "3d"):
-                    ek = [e for e, i in zip(errs, both) if fp["kinds"][i] == kind]
-                    if ek:
-                        line["mixed_disp_err_vs_fp64"][f"max_{kind}"] = max(ek)
-            if errs and args.sensitivity:
-                # the same FP64 solve with every impact speed perturbed by
-                # the FP32 unit roundoff: how far the jobs' own solutions
-                # move under an input change of the size mixed precision
-                # makes at every step
-                pr = run_sweep("fp64", ctx, args.jobs, args.t_end, max_cycles=args.max_cycles, perturb=2.0 ** -24)
-                sens = [disp_rel_err(pr["disp"][i], fp["disp"][i]) if pr["done"][i] else float("nan") for i in both]
-                ratio = [e / s for e, s in zip(errs, sens) if s == s and s > 0]
-                line["fp64_sensitivity"] = {
-                    "perturbation": "impact speed x (1 + 2^-24)", "max": float(np.nanmax(sens)),
-                    "median": float(np.nanmedian(sens)),
-                    "mixed_err_over_sensitivity_max": max(ratio) if ratio else None,
-                    "mixed_err_over_sensitivity_median": float(np.median(ratio)) if ratio else None}
-                for kind in ("axi", "3d"):
-                    sk = [s for s, i in zip(sens, both) if fp["kinds"][i] == kind and s == s]
-                    if sk:
-                        line["fp64_sensitivity"][f"max_{kind}"] = max(sk)
-                        line["fp64_sensitivity"][f"median_{kind}"] = float(np.median(sk))
+            line.update(sweep_errors(res["mixed"], res["fp64"], args, ctx))
         print(json.dumps(line), flush=True)
         return
 
@@ -536,6 +511,13 @@ def main():
         b = run_sweep(head, ctx, args.batched_jobs, 80e-6, max_cycles=args.batched_cycle_cap)
         line["batched"] = {"value": b["value"], "unit": "elem-cycles/s", "wall_s": b["wall"],
                            "config": sweep_config(b, head, 80e-6, args.batched_cycle_cap)}
+        if head != "fp64" and not args.no_batched_fp64:
+            # the same batch in FP64: the batched speedup and errors
+            bf = run_sweep("fp64", ctx, args.batched_jobs, 80e-6, max_cycles=args.batched_cycle_cap)
+            line["batched"]["value_fp64"] = bf["value"]
+            line["batched"]["wall_s_fp64"] = bf["wall"]
+            line["batched"][f"speedup_{head}_vs_fp64"] = b["value"] / bf["value"]
+            line["batched"].update(sweep_errors(b, bf, args, ctx, jobs=args.batched_jobs, cap=args.batched_cycle_cap))
     if args.workload == "taylor3d" and not args.no_serve:
         sv = run_serve(head, ctx, args.serve_jobs, 0.7, None, 32, cycle_budget=args.cycle_budget)
         line["serving"] = {"value": sv["value"], "unit": "elem-cycles/s", "latency_p50_s": sv["p50"],
@@ -547,6 +529,42 @@ def main():
         line["cpu_baseline"] = cpu_baseline(args.n, args.nz, ctx, t_window=min(args.cpu_window, 0.5 * args.t_end),
                                             window_cycles=args.cpu_cycles, workload=args.workload, pull=args.pull)
     print(json.dumps(line), flush=True)
+
+
+def sweep_errors(mx, fp, args, ctx, jobs=None, cap=None):
+    """Mixed vs FP64 displacement errors of the sweep's jobs that both runs
+    finished (overall and per kind) and, with args.sensitivity, the FP64
+    solutions' own sensitivity: the same FP64 sweep with every impact speed
+    times (1 + 2^-24), the FP32 unit roundoff -- how far the jobs' answers
+    move under an input change of the size that mixed precision makes at
+    every step."""
+    out = {}
+    both = [i for i in range(len(fp["done"])) if mx["done"][i] and fp["done"][i]]
+    errs = [disp_rel_err(mx["disp"][i], fp["disp"][i]) for i in both]
+    if not errs:
+        return out
+    out["mixed_disp_err_vs_fp64"] = {"max": max(errs), "median": float(np.median(errs)), "jobs_compared": len(errs)}
+    for kind in ("axi", "3d"):
+        ek = [e for e, i in zip(errs, both) if fp["kinds"][i] == kind]
+        if ek:
+            out["mixed_disp_err_vs_fp64"][f"max_{kind}"] = max(ek)
+            out["mixed_disp_err_vs_fp64"][f"jobs_{kind}"] = len(ek)
+    if not args.sensitivity:
+        return out
+    pr = run_sweep("fp64", ctx, jobs or args.jobs, 80e-6 if jobs else args.t_end,
+                   max_cycles=cap if jobs else args.max_cycles, perturb=2.0 ** -24)
+    sens = [disp_rel_err(pr["disp"][i], fp["disp"][i]) if pr["done"][i] else float("nan") for i in both]
+    ratio = [e / s for e, s in zip(errs, sens) if s == s and s > 0]
+    out["fp64_sensitivity"] = {
+        "perturbation": "impact speed x (1 + 2^-24)", "max": float(np.nanmax(sens)), "median": float(np.nanmedian(sens)),
+        "mixed_err_over_sensitivity_max": max(ratio) if ratio else None,
+        "mixed_err_over_sensitivity_median": float(np.median(ratio)) if ratio else None}
+    for kind in ("axi", "3d"):
+        sk = [s for s, i in zip(sens, both) if fp["kinds"][i] == kind and s == s]
+        if sk:
+            out["fp64_sensitivity"][f"max_{kind}"] = max(sk)
+            out["fp64_sensitivity"][f"median_{kind}"] = float(np.median(sk))
+    return out
 
 
 def sweep_config(r, prec, t_end, cap):
