@@ -14,6 +14,12 @@ and length, seeded subsets of the batched sweep and of the serving stream.
     continued from the job's own state: FP64 window displacement <= 1e-9
     and IP fields <= 1e-7, mixed window displacement <= 1e-5 and plastic
     strain <= 5e-5.
+  * Sweep jobs run to 80 us, mixed vs FP64 against the FP64 solution's own
+    sensitivity: the same FP64 job with the impact speed times (1 +- 2^-24),
+    one FP32 rounding.  Hex jobs: mixed within 1e-5.  Axisymmetric jobs of
+    the sweep are ill-conditioned -- a one-ulp change of the speed moves
+    the FP64 answer by more than 2e-6 -- and their mixed difference stays
+    within 10x that movement.
   * Serving stream: 8 seeded jobs of the serving mix (axisymmetric / 3D x
     impact / necking, 1k-20k elements, end times 5-15 us) submitted at
     Poisson-drawn server steps while the server advances the others: each
@@ -122,6 +128,31 @@ def test_sweep_subset_batch_equals_solo_and_oracle(ctx, prec):
         _same(s, m)
         s.close()
         m.close()
+
+
+@pytest.mark.parametrize("idx", [5, 8, 16, 31])
+def test_sweep_job_full_run_mixed_vs_fp64_sensitivity(ctx, idx):
+    from dnlPython import samples
+    job = samples.sweep_params(64, seed=20261203)[idx]
+
+    def run(prec, scale=1.0):
+        j = dict(job, speed=job["speed"] * scale)
+        m = samples.build_sweep_job(j, t_end=80e-6, precision=prec, context=ctx)
+        m.solve()
+        d = m.getNodalDisplacements().copy()
+        m.close()
+        return d
+
+    ref = run("fp64")
+    sens = max(disp_err(run("fp64", 1.0 + s * 2.0 ** -24), ref) for s in (1, -1))
+    err = disp_err(run("mixed"), ref)
+    print(f"job {idx} {job['kind']} {job['shape']}: FP64 one-ulp sensitivity {sens:.2e}, mixed vs FP64 {err:.2e}",
+          flush=True)
+    if job["kind"] == "3d":
+        assert err <= 1e-5
+    else:
+        assert sens > 2e-6
+        assert err <= 10.0 * sens
 
 
 def test_serving_stream_jobs_equal_solo(ctx):
