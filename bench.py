@@ -247,10 +247,10 @@ def element_kernel(prec, schedule, dim=3, nn=8):
     """Name of the element-kernel instantiation a schedule launches (as the
     profiler prints it): <nodes, dim, block, axisymmetric, FP64 return,
     waves, table-law variant> for the mixed kernel, <nodes, dim, block,
-    axisymmetric, network law, waves> for the FP64 one."""
+    axisymmetric, network law, waves, tetrahedron> for the FP64 one."""
     b, w = schedule["elem_block"], schedule["waves"]
     if prec == "fp64":
-        return f"k_elem_fp64<{nn}, {dim}, {b}, false, false, {w}>"
+        return f"k_elem_fp64<{nn}, {dim}, {b}, false, false, {w}, false>"
     return f"k_elem_mixed<{nn}, {dim}, {b}, false, {'true' if prec == 'mixed' else 'false'}, {w}, false>"
 
 

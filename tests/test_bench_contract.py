@@ -32,7 +32,7 @@ def test_element_kernel_names_the_launched_instantiation():
     sched = {"elem_block": 64, "waves": 0, "corr_batch": 4, "graph_cycles": 80}
     assert bench.element_kernel("mixed", sched) == "k_elem_mixed<8, 3, 64, false, true, 0, false>"
     assert bench.element_kernel("mixed-rr32", sched) == "k_elem_mixed<8, 3, 64, false, false, 0, false>"
-    assert bench.element_kernel("fp64", dict(sched, waves=3)) == "k_elem_fp64<8, 3, 64, false, false, 3>"
+    assert bench.element_kernel("fp64", dict(sched, waves=3)) == "k_elem_fp64<8, 3, 64, false, false, 3, false>"
 
 
 def test_measured_traffic_only_for_its_kernel_and_workload():
