@@ -2,7 +2,9 @@
 developed plastic state at the full sizes of the 3D workloads.
 
   * restart: a job that takes another job's state at t1 continues exactly
-    as that job continues (FP64 and mixed: the same bits);
+    as that job continues (FP64 and mixed: the same bits); a mixed quad job
+    clears the xz / yz stress rows of a state it takes (its kernel carries
+    none);
   * the 120 000-hex Taylor impact (80 us in total) run by the engine to
     40 us, deep in its plastic phase, then continued by both the engine and
     the CPU oracle from the engine's state for about 45 cycles: the FP64
@@ -40,6 +42,34 @@ def test_restart_continues_bit_for_bit(ctx, prec):
     for f in (dnl.Field.Stress, dnl.Field.plasticStrain, dnl.Field.T, dnl.Field.gammaCumulate):
         assert np.array_equal(a.getIntPointField(f), b.getIntPointField(f)), f
     assert np.array_equal(a.getEnergies(), b.getEnergies())
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("prec", ["mixed", "mixed-rr32"])
+def test_restart_quad_mixed_out_of_plane_stress(ctx, prec):
+    """The mixed kernel of the plane and axisymmetric quads carries no xz /
+    yz stress: a state handed to such a job has those rows cleared, and the
+    job continues with the same bits as the original continuing."""
+    import dnlPython as dnl
+    from dnlPython import samples
+
+    def make():
+        return samples.taylor_axi(8, 40, t_end=16e-6, precision=prec, track_tensors=False, context=ctx)
+    a = make()
+    a.solve(7e-6)
+    st = a.getState()
+    assert not np.any(st["stress"][[2, 4]])
+    st["stress"][[2, 4]] = 1.0e3
+    b = make()
+    b.setState(st)
+    assert not np.any(b.getState()["stress"][[2, 4]])
+    a.solve()
+    b.solve()
+    assert a.getScalars() == b.getScalars()
+    assert np.array_equal(a.getNodalCoordinates(), b.getNodalCoordinates())
+    for f in (dnl.Field.Stress, dnl.Field.plasticStrain, dnl.Field.T):
+        assert np.array_equal(a.getIntPointField(f), b.getIntPointField(f)), f
     a.close()
     b.close()
 
