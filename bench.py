@@ -554,6 +554,8 @@ def sweep_errors(mx, fp, args, ctx, jobs=None, cap=None):
     pr = run_sweep("fp64", ctx, jobs or args.jobs, 80e-6 if jobs else args.t_end,
                    max_cycles=cap if jobs else args.max_cycles, perturb=2.0 ** -24)
     sens = [disp_rel_err(pr["disp"][i], fp["disp"][i]) if pr["done"][i] else float("nan") for i in both]
+    if not any(s == s for s in sens):
+        return out
     ratio = [e / s for e, s in zip(errs, sens) if s == s and s > 0]
     out["fp64_sensitivity"] = {
         "perturbation": "impact speed x (1 + 2^-24)", "max": float(np.nanmax(sens)), "median": float(np.nanmedian(sens)),
