@@ -141,24 +141,39 @@ __device__ __forceinline__ void polar_lnU_mixed(const float H[9], float dE[6], f
   if (nrm2 > 0.05f * 0.05f) {
     polar_lnU_large(H, dE, R);
   } else {
-    // powers to X^order (order 3, 4 or 6, uniform over the wave)
-    float X2[6], X3[6], X4[6], X5[6], X6[6];
+    // powers to X^order (order 3, 4 or 6, uniform over the wave, so the
+    // branches below do not diverge); each order's sums are the leading
+    // terms of the order-6 sums in the same order, whose remaining terms
+    // are exact zeros at the lower orders
+    float X2[6], X3[6], Ui[6];
     sym_mul<B2>(X, X, X2);
     sym_mul<B2>(X2, X, X3);
-#pragma unroll
-    for (int c = 0; c < 6; c++) X4[c] = X5[c] = X6[c] = 0.f;
-    if (order >= 4) sym_mul<B2>(X2, X2, X4);
     if (order >= 6) {
+      float X4[6], X5[6], X6[6];
+      sym_mul<B2>(X2, X2, X4);
       sym_mul<B2>(X4, X, X5);
       sym_mul<B2>(X3, X3, X6);
-    }
-    float Ui[6];
 #pragma unroll
-    for (int c = 0; c < 6; c++) {
-      dE[c] = 0.5f * (X[c] - X2[c] * (1.f / 2.f) + X3[c] * (1.f / 3.f) - X4[c] * (1.f / 4.f) +
-                      X5[c] * (1.f / 5.f) - X6[c] * (1.f / 6.f));
-      Ui[c] = -X[c] * (1.f / 2.f) + X2[c] * (3.f / 8.f) - X3[c] * (5.f / 16.f) + X4[c] * (35.f / 128.f) -
-              X5[c] * (63.f / 256.f) + X6[c] * (231.f / 1024.f);
+      for (int c = 0; c < 6; c++) {
+        dE[c] = 0.5f * (X[c] - X2[c] * (1.f / 2.f) + X3[c] * (1.f / 3.f) - X4[c] * (1.f / 4.f) +
+                        X5[c] * (1.f / 5.f) - X6[c] * (1.f / 6.f));
+        Ui[c] = -X[c] * (1.f / 2.f) + X2[c] * (3.f / 8.f) - X3[c] * (5.f / 16.f) + X4[c] * (35.f / 128.f) -
+                X5[c] * (63.f / 256.f) + X6[c] * (231.f / 1024.f);
+      }
+    } else if (order >= 4) {
+      float X4[6];
+      sym_mul<B2>(X2, X2, X4);
+#pragma unroll
+      for (int c = 0; c < 6; c++) {
+        dE[c] = 0.5f * (X[c] - X2[c] * (1.f / 2.f) + X3[c] * (1.f / 3.f) - X4[c] * (1.f / 4.f));
+        Ui[c] = -X[c] * (1.f / 2.f) + X2[c] * (3.f / 8.f) - X3[c] * (5.f / 16.f) + X4[c] * (35.f / 128.f);
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 6; c++) {
+        dE[c] = 0.5f * (X[c] - X2[c] * (1.f / 2.f) + X3[c] * (1.f / 3.f));
+        Ui[c] = -X[c] * (1.f / 2.f) + X2[c] * (3.f / 8.f) - X3[c] * (5.f / 16.f);
+      }
     }
     Ui[0] += 1.f;
     Ui[3] += 1.f;
