@@ -215,17 +215,18 @@ __device__ __forceinline__ void polar_lnU_mixed(const float H[9], float dE[6], f
 // Stage d pairs the lanes differing in bit d: the lane with the bit clear
 // keeps v_lo + v_hi, the one with the bit set v_hi - v_lo, i.e. one FMA
 // v + cs[d] * partner with cs[d] = -1 where bit d is set, +1 otherwise.
-// Value of lane (this lane ^ 2^d), d < 3, by DPP moves inside the 16-lane row
-// (no LDS traffic): d = 0, 1 are quad permutations [1,0,3,2] / [2,3,0,1];
-// d = 2 takes lane + 4 in banks 0 and 2 (row_shl:4) and lane - 4 in banks 1
-// and 3 (row_shr:4) of each row.
+// Value of lane (this lane ^ 2^d), d < 3: d = 0, 1 by DPP quad permutations
+// [1,0,3,2] / [2,3,0,1] (one VALU instruction each); d = 2 by a swizzle in
+// bit-mask mode (and 0x1F, xor 4), which runs in the LDS pipeline without
+// touching LDS memory -- in DPP it took a copy and two row shifts under bank
+// masks (row_shl:4 for banks 0 and 2, row_shr:4 for 1 and 3), three VALU
+// instructions in a kernel bound by its VALU issue.
 template <int D>
 __device__ __forceinline__ float lane_xor(float v) {
   const int x = __float_as_int(v);
   if constexpr (D == 0) return __int_as_float(__builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, true));
   if constexpr (D == 1) return __int_as_float(__builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, true));
-  const int up = __builtin_amdgcn_update_dpp(x, x, 0x104, 0xF, 0x5, false);
-  return __int_as_float(__builtin_amdgcn_update_dpp(up, x, 0x114, 0xF, 0xA, false));
+  return __int_as_float(__builtin_amdgcn_ds_swizzle(x, (4 << 10) | 0x1F));
 }
 
 template <int NB>
