@@ -31,6 +31,38 @@ def test_batch_equals_alone(ctx, prec):
         np.testing.assert_array_equal(a.getIntPointField(dnl.Field.Stress), b.getIntPointField(dnl.Field.Stress))
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_batch_branches_on_one_stream_give_the_same_bits(prec):
+    """The 2D and 3D jobs of a batch run as two branches of each cycle on
+    two streams; on one stream (DNLG_BRANCHES=0) the results are the same
+    bits, with and without captured graphs."""
+    import os
+    import dnlPython as dnl
+
+    def run(branches, graph_cycles):
+        env = {"DNLG_BRANCHES": branches, "DNLG_GRAPH_CYCLES": graph_cycles}
+        old = {k: os.environ.get(k) for k in env}
+        os.environ.update(env)
+        try:
+            c = dnl.Context(0)
+        finally:
+            for k, v in old.items():
+                if v is None:
+                    del os.environ[k]
+                else:
+                    os.environ[k] = v
+        ms = _models(c, prec)
+        dnl.batch_solve(ms)
+        return [(m.getScalars(), m.getNodalDisplacements(), m.getIntPointField(dnl.Field.Stress)) for m in ms]
+
+    ref = run("0", "16")
+    for branches, graph_cycles in (("1", "16"), ("1", "0")):
+        for (sa, da, fa), (sb, db, fb) in zip(ref, run(branches, graph_cycles)):
+            assert sa == sb
+            np.testing.assert_array_equal(da, db)
+            np.testing.assert_array_equal(fa, fb)
+
+
 def test_batch_in_segments(ctx):
     """Advancing a batch in several solve calls equals one call."""
     import dnlPython as dnl
