@@ -375,14 +375,15 @@ __device__ __forceinline__ bool radial_return_fp32(const LawF &L, int max_it, fl
 // reaches the root while a start below it is held under the bound.  Out:
 // gamma and the FP64 yield stress of its last iterate.  Returns false at the
 // iteration limit.
+// rdt = 1 / dt (the rate is sqrt(2/3) gamma rdt).
 // law_at(epsp, depsp, T, yield, slope) evaluates the law; it returns false
 // where the law has no value (outside a table), which ends the iteration
 // (the caller reports it).
 template <class Law>
 __device__ __forceinline__ bool radial_return_fp64(const Law &law_at, int max_it, double heatFrac, double TwoG32,
                                                    double Strial, double Snorm0, double Tcur, double ep0,
-                                                   double dt, double gmin, double gmax, double &y, double &g) {
-  double depsp = kSqrt23 * g / dt;
+                                                   double rdt, double gmin, double gmax, double &y, double &g) {
+  double depsp = kSqrt23 * g * rdt;
   double epsp = ep0 + kSqrt23 * g;
   double T = Tcur + 0.5 * g * heatFrac * (kSqrt23 * y + Snorm0);
   int iterate = 0;
@@ -402,7 +403,7 @@ __device__ __forceinline__ bool radial_return_fp64(const Law &law_at, int max_it
       g = gmin + dg;
     }
     if (fabs(dg) < kTolNR) return true;
-    depsp = kSqrt23 * g / dt;
+    depsp = kSqrt23 * g * rdt;
     epsp = ep0 + kSqrt23 * g;
     T = Tcur + 0.5 * g * heatFrac * (kSqrt23 * y + Snorm0);
     if (++iterate > max_it) return false;
@@ -708,6 +709,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
       asm volatile("" ::: "memory");
       float g;
       bool ok = true;
+      const double rdt = 1.0 / dt;  // one division for the return's rates
       if constexpr (XL) {
         bool in_table = true;
         auto law_at = [&](double epsp, double depsp, double T, double &y, double &h) {
@@ -734,7 +736,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
         } else {
           gamma = ep0 == 0.0 ? kSqrt32 * kGammaInitial : (double)gam0;
           ok = radial_return_fp64(law_at, C->nr_max_it, heatFrac, TwoG32, (double)Strialf, (double)Snorm0, Tcur, ep0,
-                                  dt, 0.0, ((double)Strialf - y) / TwoG32, y, gamma);
+                                  rdt, 0.0, ((double)Strialf - y) / TwoG32, y, gamma);
         }
         if (!in_table) set_status(J, DNLG_E_TABLE_RANGE, par);
         g = (float)gamma;
@@ -757,7 +759,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
             return true;
           };
           ok = radial_return_fp64(law_at, C->nr_max_it, heatFrac, TwoG32, (double)Strialf, (double)Snorm0, Tcur, ep0,
-                                  dt, (double)last.gmin, (double)last.gmax, y, gamma);
+                                  rdt, (double)last.gmin, (double)last.gmax, y, gamma);
           g = (float)gamma;
           yieldf = (float)y;
         }
@@ -772,7 +774,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
         if (!B2 || (c != 2 && c != 4)) Rs[c] = s_keep[(3 + c) * 64 + lane];
       scale = 1.f - TwoGf * g * __builtin_amdgcn_rcpf(Snorm);
       at(gbl(C->ep) + g0, 8 * ip) = ep0 + kSqrt23 * gamma;
-      at(gbl(C->dep32) + g0, 4 * ip) = (float)(kSqrt23 * gamma / dt);
+      at(gbl(C->dep32) + g0, 4 * ip) = (float)(kSqrt23 * gamma * rdt);
       at(gbl(C->gam32) + g0, 4 * ip) = g;
       at(gbl(C->sy32) + g0, 4 * ip) = yieldf;
     }
