@@ -428,6 +428,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
   constexpr bool B2 = DIM == 2, Z8 = B2 && !AXI;
   const float g = 0.577350269189625764f;  // Gauss abscissa 1/sqrt(3)
 
+  // raised issue priority until this wave's loads are out (below), so that
+  // a wave that starts while others compute gets its loads in flight first
+  __builtin_amdgcn_s_setprio(2);
   const Task tk = load_task(tasks);
   JobDev *J = tk.jd;
   CJobDev *C = (CJobDev *)J;
@@ -499,6 +502,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
     if constexpr (AXI) x0r = valid ? xa[0] : 0.0;
   }
 
+  __builtin_amdgcn_s_setprio(0);
   // ---- Walsh modes of coordinates and increments, scaled by 2^-D
   float cs[3];
 #pragma unroll
