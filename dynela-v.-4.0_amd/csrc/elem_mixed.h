@@ -428,8 +428,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
   constexpr bool B2 = DIM == 2, Z8 = B2 && !AXI;
   const float g = 0.577350269189625764f;  // Gauss abscissa 1/sqrt(3)
 
-  // raised issue priority until this wave's loads are out (below), so that
-  // a wave that starts while others compute gets its loads in flight first
+  // raised issue priority until this wave's element modes are in its
+  // registers (below): a wave that starts while others compute gets its
+  // loads in flight and its butterflies and mode exchange done first
   __builtin_amdgcn_s_setprio(2);
   const Task tk = load_task(tasks);
   JobDev *J = tk.jd;
@@ -502,7 +503,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
     if constexpr (AXI) x0r = valid ? xa[0] : 0.0;
   }
 
-  __builtin_amdgcn_s_setprio(0);
   // ---- Walsh modes of coordinates and increments, scaled by 2^-D
   float cs[3];
 #pragma unroll
@@ -568,6 +568,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
       mu[q][d] = row[DIM + d];
     }
   }
+  __builtin_amdgcn_s_setprio(0);
   // Gauss point of this lane
   float xi[DIM];
 #pragma unroll
